@@ -1,0 +1,749 @@
+"""CPU oracle: a NumPy float64 restatement of JAX-Fluids' convective-flux hot path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this module;
+only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may use it, and only as the checker.
+
+Every function follows the reference expression by expression (same operands,
+same association order) and cites the reference file:line it restates.  All
+citations are relative to the JAX-Fluids v0.2.1 source tree
+(``src/jaxfluids/...``).
+
+Layout (same as the reference): a field buffer has shape
+``(nvar, Nx+2nh, Ny+2nh, Nz+2nh)`` where an inactive axis has extent 1 and no
+halo cells (``initialization/helper_functions.py:49``).  A face array along
+``axis`` has extent ``N_axis+1`` along that axis and the interior extent along
+the other active axes (``solvers/space_solver.py:142-146``).
+
+Pinning: every public function here is checked in ``tests/test_oracle_*.py``
+against closed forms / invariants (see the header of each test module).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Callable, Dict, Optional, Sequence, Tuple
+
+import numpy as np
+
+# config/precision.py:39,53 -- spatial stencil epsilon (both precisions)
+SPATIAL_STENCIL_EPS = 1e-30
+# config/precision.py:36,50 -- general epsilon = machine eps of the dtype
+EPS64 = float(np.finfo(np.float64).eps)
+EPS32 = float(np.finfo(np.float32).eps)
+
+# equation_information.py:110
+VELOCITY_MINOR_AXES = ((2, 3), (3, 1), (1, 2))
+# solvers/riemann_solvers/eigendecomposition.py:69-73
+VEL_ID_EIGS = ((2, 3), (3, 2), (2, 3))
+
+FACE_LOCATIONS = ("east", "west", "north", "south", "top", "bottom")
+FACE_AXIS = {"east": 0, "west": 0, "north": 1, "south": 1, "top": 2, "bottom": 2}
+
+
+# --------------------------------------------------------------------------
+# Domain / slices
+# --------------------------------------------------------------------------
+@dataclass
+class Domain:
+    """Uniform grid (domain/domain_information.py:68-300, homogenous.py:7-21)."""
+
+    cells: Tuple[int, int, int]
+    ranges: Tuple[Tuple[float, float], ...] = ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0))
+    nh: int = 3
+
+    def __post_init__(self):
+        self.cells = tuple(int(c) for c in self.cells)
+        self.active = tuple(i for i in range(3) if self.cells[i] > 1)
+        # homogenous.py:13  cell size = (b - a) / n
+        self.dx = tuple((float(r[1]) - float(r[0])) / n for r, n in zip(self.ranges, self.cells))
+        # domain_information.py:290  1/dx
+        self.one_dx = tuple(1.0 / d for d in self.dx)
+        # domain_information.py:698-701 smallest cell size over active axes
+        self.dx_min = min(self.dx[i] for i in self.active)
+
+    @property
+    def shape(self):
+        # initialization/helper_functions.py:49
+        return tuple(n + 2 * self.nh if n > 1 else 1 for n in self.cells)
+
+    def dom(self, i):
+        """Interior slice along dim i (spatial_stencil.py:24-26)."""
+        return slice(self.nh, -self.nh) if i in self.active else slice(None)
+
+    @property
+    def interior(self):
+        return (Ellipsis, self.dom(0), self.dom(1), self.dom(2))
+
+    def cell_centers(self):
+        """homogenous.py:13-14 (linspace of centers)."""
+        out = []
+        for i in range(3):
+            a, b = float(self.ranges[i][0]), float(self.ranges[i][1])
+            d = self.dx[i]
+            out.append(np.linspace(a + d / 2, b - d / 2, self.cells[i]))
+        return out
+
+    def mesh_grid(self):
+        c = self.cells_centers_active()
+        return np.meshgrid(*c, indexing="ij")
+
+    def cells_centers_active(self):
+        cc = self.cell_centers()
+        return [cc[i] for i in range(3)]
+
+
+def shift_slice(dom: Domain, axis: int, k: int, at_cell_center: bool = False):
+    """Slice of the buffer shifted by k along axis (spatial_stencil.py:84-100).
+
+    nlo = n + k ; nhi = -n + k + 1 - at_cell_center (None if 0); other axes take
+    the domain slice.  For faces the result has N_axis+1 entries along axis.
+    """
+    n = dom.nh
+    nlo = n + k
+    nhi = -n + k + 1 - int(at_cell_center)
+    assert nlo >= 0
+    if nhi == 0:
+        nhi = None
+    else:
+        assert nhi < 0
+    s = [dom.dom(0), dom.dom(1), dom.dom(2)]
+    s[axis] = slice(nlo, nhi)
+    return (Ellipsis,) + tuple(s)
+
+
+# --------------------------------------------------------------------------
+# WENO5 (stencils/reconstruction/shock_capturing/weno5_base.py, weno/*.py)
+# --------------------------------------------------------------------------
+_DR = (1 / 10, 6 / 10, 3 / 10)                  # weno5_base.py:24
+_CR = ((1 / 3, -7 / 6, 11 / 6),                 # weno5_base.py:25-27
+       (-1 / 6, 5 / 6, 1 / 3),
+       (1 / 3, 5 / 6, -1 / 6))
+# weno5_base.py:30 array_slices([range(-3,2,1), range(2,-3,-1)])
+WENO5_SHIFTS = ((-3, -2, -1, 0, 1), (2, 1, 0, -1, -2))
+# weno5_base.py:31 stencil_slices([range(0,5,1), range(5,0,-1)]) (window indices)
+WENO5_WINDOW_IDX = ((0, 1, 2, 3, 4), (5, 4, 3, 2, 1))
+
+
+def weno5_smoothness(u_imm, u_im, u_i, u_ip, u_ipp):
+    """weno5_base.py:34-41 (Jiang-Shu beta_0..beta_2)."""
+    beta_0 = 13.0 / 12.0 * np.square(u_imm - 2 * u_im + u_i) + 1.0 / 4.0 * np.square(u_imm - 4 * u_im + 3 * u_i)
+    beta_1 = 13.0 / 12.0 * np.square(u_im - 2 * u_i + u_ip) + 1.0 / 4.0 * np.square(u_im - u_ip)
+    beta_2 = 13.0 / 12.0 * np.square(u_i - 2 * u_ip + u_ipp) + 1.0 / 4.0 * np.square(3 * u_i - 4 * u_ip + u_ipp)
+    return beta_0, beta_1, beta_2
+
+
+def weno5_polynomials(u_imm, u_im, u_i, u_ip, u_ipp):
+    """weno5_base.py:44-50 (three 3-point candidate values)."""
+    p_0 = _CR[0][0] * u_imm + _CR[0][1] * u_im + _CR[0][2] * u_i
+    p_1 = _CR[1][0] * u_im + _CR[1][1] * u_i + _CR[1][2] * u_ip
+    p_2 = _CR[2][0] * u_i + _CR[2][1] * u_ip + _CR[2][2] * u_ipp
+    return p_0, p_1, p_2
+
+
+def weno5_js_point(u_imm, u_im, u_i, u_ip, u_ipp, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno5_js.py:39-58."""
+    beta_0, beta_1, beta_2 = weno5_smoothness(u_imm, u_im, u_i, u_ip, u_ipp)
+    one_beta_0_sq = 1.0 / (beta_0 * beta_0 + eps)
+    one_beta_1_sq = 1.0 / (beta_1 * beta_1 + eps)
+    one_beta_2_sq = 1.0 / (beta_2 * beta_2 + eps)
+    alpha_0 = _DR[0] * one_beta_0_sq
+    alpha_1 = _DR[1] * one_beta_1_sq
+    alpha_2 = _DR[2] * one_beta_2_sq
+    one_alpha = 1.0 / (alpha_0 + alpha_1 + alpha_2)
+    omega_0 = alpha_0 * one_alpha
+    omega_1 = alpha_1 * one_alpha
+    omega_2 = alpha_2 * one_alpha
+    p_0, p_1, p_2 = weno5_polynomials(u_imm, u_im, u_i, u_ip, u_ipp)
+    return omega_0 * p_0 + omega_1 * p_1 + omega_2 * p_2
+
+
+def weno5_z_point(u_imm, u_im, u_i, u_ip, u_ipp, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno5_z.py:38-59 (Borges et al. 2008)."""
+    beta_0, beta_1, beta_2 = weno5_smoothness(u_imm, u_im, u_i, u_ip, u_ipp)
+    tau_5 = np.abs(beta_0 - beta_2)
+    alpha_z_0 = _DR[0] * (1.0 + tau_5 / (beta_0 + eps))
+    alpha_z_1 = _DR[1] * (1.0 + tau_5 / (beta_1 + eps))
+    alpha_z_2 = _DR[2] * (1.0 + tau_5 / (beta_2 + eps))
+    one_alpha_z = 1.0 / (alpha_z_0 + alpha_z_1 + alpha_z_2)
+    omega_z_0 = alpha_z_0 * one_alpha_z
+    omega_z_1 = alpha_z_1 * one_alpha_z
+    omega_z_2 = alpha_z_2 * one_alpha_z
+    p_0, p_1, p_2 = weno5_polynomials(u_imm, u_im, u_i, u_ip, u_ipp)
+    return omega_z_0 * p_0 + omega_z_1 * p_1 + omega_z_2 * p_2
+
+
+WENO5_POINT = {"WENO5-JS": weno5_js_point, "WENO5-Z": weno5_z_point}
+
+
+def weno5_reconstruct_xi(buffer, dom: Domain, axis: int, j: int, stencil: str = "WENO5-JS",
+                         eps=SPATIAL_STENCIL_EPS):
+    """SpatialReconstruction.reconstruct_xi for WENO5 on a cell buffer.
+
+    weno/weno5_js.py:23-37 / weno5_z.py:24-36: the five operands are the buffer
+    shifted by WENO5_SHIFTS[j] along axis (spatial_stencil.py:45-112).
+    """
+    ops = [buffer[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[j]]
+    return WENO5_POINT[stencil](*ops, eps=eps)
+
+
+def weno5_reconstruct_window(window, j: int, stencil: str, eps=SPATIAL_STENCIL_EPS):
+    """reconstruct_xi after set_slices_stencil (spatial_reconstruction.py:133-153).
+
+    ``window`` has the 6-cell stencil window on axis 1: shape (nvar, 6, *face).
+    """
+    ops = [window[:, i] for i in WENO5_WINDOW_IDX[j]]
+    return WENO5_POINT[stencil](*ops, eps=eps)
+
+
+# --------------------------------------------------------------------------
+# Ideal gas + equation manager
+# --------------------------------------------------------------------------
+def speed_of_sound(p, rho, gamma):
+    """materials/single_materials/ideal_gas.py:69-71."""
+    return np.sqrt(gamma * p / rho)
+
+
+def cons_from_prim(W, gamma):
+    """equation_manager.py:93-101 with ideal_gas.py:85-88 (e = p/(rho(gamma-1)))."""
+    rho = W[0]
+    vel = W[1:4]
+    e = W[4] / (rho * (gamma - 1.0))
+    mom = rho * vel
+    # math/sum_consistent.py:22-24 (plain a+b+c)
+    E = rho * (0.5 * (np.square(vel[0]) + np.square(vel[1]) + np.square(vel[2])) + e)
+    return np.stack([rho, mom[0], mom[1], mom[2], E], axis=0)
+
+
+def prim_from_cons(U, gamma):
+    """equation_manager.py:164-171 with ideal_gas.py:73-75 (p=(gamma-1) e rho)."""
+    rho = U[0]
+    one_rho = 1.0 / rho
+    vel = U[1:4] * one_rho
+    e = U[4] * one_rho - 0.5 * (np.square(vel[0]) + np.square(vel[1]) + np.square(vel[2]))
+    p = (gamma - 1.0) * e * rho
+    return np.stack([rho, vel[0], vel[1], vel[2], p], axis=0)
+
+
+def physical_flux_xi(W, U, axis):
+    """equation_manager.py:237-252 (single phase)."""
+    rho_ui = U[axis + 1]
+    f = [rho_ui, U[axis + 1] * W[1], U[axis + 1] * W[2], U[axis + 1] * W[3], None]
+    f[axis + 1] = f[axis + 1] + W[4]
+    f[4] = W[axis + 1] * (U[4] + W[4])
+    return np.stack(f, axis=0)
+
+
+# --------------------------------------------------------------------------
+# Eigendecomposition / characteristic projection
+# (solvers/riemann_solvers/eigendecomposition.py)
+# --------------------------------------------------------------------------
+def total_enthalpy(p, rho, vel, gamma):
+    """ideal_gas.py:90-110."""
+    E = p / (gamma - 1) + 0.5 * rho * (np.square(vel[0]) + np.square(vel[1]) + np.square(vel[2]))
+    return (E + p) / rho
+
+
+def frozen_state(W_L, W_R, gamma, frozen: str = "ARITHMETIC"):
+    """compute_frozen_state, single phase (eigendecomposition.py:139-281).
+
+    Returns primes_ave, enthalpy_ave, grueneisen_ave, c_ave, cc_ave, velocity_square.
+    """
+    if frozen == "ARITHMETIC":
+        primes_ave = 0.5 * (W_L + W_R)                                         # :148
+        grueneisen_ave = (gamma - 1) * np.ones_like(primes_ave[0])             # :216-218
+        enthalpy_ave = total_enthalpy(primes_ave[4], primes_ave[0], primes_ave[1:4], gamma)  # :219-222
+        c_ave = speed_of_sound(primes_ave[4], primes_ave[0], gamma)             # :223-225
+        cc_ave = c_ave * c_ave                                                  # :230
+        velocity_square = np.square(primes_ave[1]) + np.square(primes_ave[2]) + np.square(primes_ave[3])  # :231
+    elif frozen == "ROE":
+        sL, sR = np.sqrt(W_L[0]), np.sqrt(W_R[0])
+        primes_ave = (np.sqrt(W_L[0]) * W_L + np.sqrt(W_R[0]) * W_R) / (np.sqrt(W_L[0]) + np.sqrt(W_R[0]))  # :293
+        primes_ave = primes_ave.copy()
+        primes_ave[0] = np.sqrt(W_L[0] * W_R[0])                                # :237
+        rho_div = 1.0 / (sL + sR)                                               # :240
+        H_L = total_enthalpy(W_L[4], W_L[0], W_L[1:4], gamma)
+        H_R = total_enthalpy(W_R[4], W_R[0], W_R[1:4], gamma)
+        enthalpy_ave = (sL * H_L + sR * H_R) * rho_div                          # :250
+        psi_L, psi_R = W_L[4] / W_L[0], W_R[4] / W_R[0]                         # ideal_gas.py:61-63
+        psi_ave = (sL * psi_L + sR * psi_R) * rho_div                           # :258
+        g = gamma - 1
+        grueneisen_ave = (sL * g + sR * g) * rho_div                            # :260-262
+        dvel = W_R[1:4] - W_L[1:4]
+        sq_dvel = np.square(dvel[0]) + np.square(dvel[1]) + np.square(dvel[2])  # :265
+        p_over_rho_ave = (sL * W_L[4] / W_L[0] + sR * W_R[4] / W_R[0]) * rho_div \
+            + 0.5 * primes_ave[0] * rho_div * rho_div * sq_dvel                 # :267-268
+        velocity_square = np.square(primes_ave[1]) + np.square(primes_ave[2]) + np.square(primes_ave[3])  # :271
+        cc_ave = psi_ave + grueneisen_ave * p_over_rho_ave                      # :274
+        c_ave = np.sqrt(cc_ave)                                                 # :275
+    else:
+        raise NotImplementedError(frozen)
+    return primes_ave, enthalpy_ave, grueneisen_ave, c_ave, cc_ave, velocity_square
+
+
+def char_from_prim(xin, primes_ave, c_ave, cc_ave, axis):
+    """get_characteristics_from_primitives, single phase (eigendecomposition.py:421-433).
+
+    ``xin`` is (5, ...); the frozen state broadcasts against it.
+    """
+    vel = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    e0, e1 = VEL_ID_EIGS[axis]
+    xout = [None] * 5
+    xout[0] = -0.5 / c_ave * xin[vel] + 0.5 / (cc_ave * primes_ave[0]) * xin[4]
+    xout[1] = xin[0] - 1.0 / cc_ave * xin[4]
+    xout[e0] = xin[m0]
+    xout[e1] = xin[m1]
+    xout[4] = 0.5 / c_ave * xin[vel] + 0.5 / (cc_ave * primes_ave[0]) * xin[4]
+    return np.stack(xout, axis=0)
+
+
+def prim_from_char(xin, primes_ave, c_ave, cc_ave, axis):
+    """get_primitives_from_characteristics, single phase (eigendecomposition.py:512-523)."""
+    vel = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    e0, e1 = VEL_ID_EIGS[axis]
+    xout = [None] * 5
+    xout[0] = primes_ave[0] * (xin[0] + xin[4]) + xin[1]
+    xout[vel] = c_ave * (-xin[0] + xin[4])
+    xout[m0] = xin[e0]
+    xout[m1] = xin[e1]
+    xout[4] = cc_ave * primes_ave[0] * (xin[0] + xin[4])
+    return np.stack(xout, axis=0)
+
+
+def eigen_conservatives(primes_ave, enthalpy_ave, grueneisen_ave, c_ave, cc_ave, velocity_square, axis):
+    """eigendecomposition_conservatives, single phase (eigendecomposition.py:595-666).
+
+    Returns (R, L), each of shape (5, 5, ...) indexed [row][col].
+    """
+    one_cc_ave = 1.0 / cc_ave
+    one_rho_ave = 1.0 / primes_ave[0]
+    z = np.zeros_like(primes_ave[0])
+    o = np.ones_like(primes_ave[0])
+    R = [[z for _ in range(5)] for _ in range(5)]
+    L = [[z for _ in range(5)] for _ in range(5)]
+    v = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    P = primes_ave
+    g = grueneisen_ave
+    H = enthalpy_ave
+    # RIGHT (:608-631)
+    R[0][0] = o
+    R[v][0] = P[v] - c_ave
+    R[m0][0] = P[m0]
+    R[m1][0] = P[m1]
+    R[4][0] = H - P[v] * c_ave
+    R[0][v] = g
+    R[1][v] = g * P[1]
+    R[2][v] = g * P[2]
+    R[3][v] = g * P[3]
+    R[4][v] = g * H - cc_ave
+    R[m0][m0] = -P[0]
+    R[4][m0] = -P[0] * P[m0]
+    R[m1][m1] = P[0]
+    R[4][m1] = P[0] * P[m1]
+    R[0][4] = o
+    R[v][4] = P[v] + c_ave
+    R[m0][4] = P[m0]
+    R[m1][4] = P[m1]
+    R[4][4] = H + P[v] * c_ave
+    # LEFT (:635-658)
+    L[0][0] = 0.5 * one_cc_ave * (g * velocity_square - g * H + (P[v] + c_ave) * c_ave)
+    L[0][v] = 0.5 * one_cc_ave * (-P[v] * g - c_ave)
+    L[0][m0] = 0.5 * one_cc_ave * (-P[m0] * g)
+    L[0][m1] = 0.5 * one_cc_ave * (-P[m1] * g)
+    L[0][4] = 0.5 * one_cc_ave * g
+    L[v][0] = one_cc_ave * (H - velocity_square)
+    L[v][1] = P[1] * one_cc_ave
+    L[v][2] = P[2] * one_cc_ave
+    L[v][3] = P[3] * one_cc_ave
+    L[v][4] = -one_cc_ave
+    L[m0][0] = P[m0] * one_rho_ave
+    L[m0][m0] = -one_rho_ave
+    L[m1][0] = -P[m1] * one_rho_ave
+    L[m1][m1] = one_rho_ave
+    L[4][0] = 0.5 * one_cc_ave * (g * velocity_square - g * H - (P[v] - c_ave) * c_ave)
+    L[4][v] = 0.5 * one_cc_ave * (-P[v] * g + c_ave)
+    L[4][m0] = 0.5 * one_cc_ave * (-P[m0] * g)
+    L[4][m1] = 0.5 * one_cc_ave * (-P[m1] * g)
+    L[4][4] = 0.5 * one_cc_ave * g
+    return np.array(R), np.array(L)
+
+
+def transform(M, x):
+    """einsum 'ij...,j...->i...' (eigendecomposition.py:717-742), summed in j order."""
+    out = []
+    for i in range(5):
+        acc = M[i][0] * x[0]
+        for jj in range(1, 5):
+            acc = acc + M[i][jj] * x[jj]
+        out.append(acc)
+    return np.stack(out, axis=0)
+
+
+def stencil_window(buffer, dom: Domain, axis: int):
+    """get_stencil_window (eigendecomposition.py:75-118): cells f-3..f+2 of face f.
+
+    Returned with the window on axis 1: (nvar, 6, *face_shape).
+    """
+    return np.stack([buffer[shift_slice(dom, axis, k)] for k in range(-3, 3)], axis=1)
+
+
+def face_LR_cells(buffer, dom: Domain, axis: int):
+    """slice_LR (eigendecomposition.py:98-102): cells left/right of each face."""
+    return buffer[shift_slice(dom, axis, -1)], buffer[shift_slice(dom, axis, 0)]
+
+
+# --------------------------------------------------------------------------
+# Signal speeds and Riemann solvers (solvers/riemann_solvers/)
+# --------------------------------------------------------------------------
+def signal_speed_arithmetic(u_L, u_R, a_L, a_R, **kw):
+    """signal_speeds.py:10-29."""
+    u_mean = 0.5 * (u_L + u_R)
+    a_mean = 0.5 * (a_L + a_R)
+    return np.minimum(u_mean - a_mean, u_L - a_L), np.maximum(u_mean + a_mean, u_R + a_R)
+
+
+def signal_speed_rusanov(u_L, u_R, a_L, a_R, **kw):
+    """signal_speeds.py:31-49."""
+    S_plus = np.maximum(np.abs(u_L) + a_L, np.abs(u_R) + a_R)
+    return -S_plus, S_plus
+
+
+def signal_speed_davis(u_L, u_R, a_L, a_R, **kw):
+    """signal_speeds.py:51-69 (Toro 10.48)."""
+    return np.minimum(u_L - a_L, u_R - a_R), np.maximum(u_L + a_L, u_R + a_R)
+
+
+def signal_speed_einfeldt(u_L, u_R, a_L, a_R, rho_L, rho_R, **kw):
+    """signal_speeds.py:109-133 (Toro 10.52-10.54)."""
+    sqrt_rho_L = np.sqrt(rho_L)
+    sqrt_rho_R = np.sqrt(rho_R)
+    one_dens = 1.0 / (sqrt_rho_L + sqrt_rho_R)
+    eta2 = 0.5 * sqrt_rho_L * sqrt_rho_R * one_dens * one_dens
+    u_bar = (sqrt_rho_L * u_L + sqrt_rho_R * u_R) * one_dens
+    d_bar = np.sqrt((sqrt_rho_L * a_L * a_L + sqrt_rho_R * a_R * a_R) * one_dens + eta2 * np.square(u_R - u_L))
+    return np.minimum(u_bar - d_bar, u_L - a_L), np.maximum(u_bar + d_bar, u_R + a_R)
+
+
+def estimate_pressure(u_L, u_R, a_L, a_R, rho_L, rho_R, p_L, p_R):
+    """signal_speeds.py:201-231 (Toro 9.28)."""
+    rho_bar = 0.5 * (rho_L + rho_R)
+    a_bar = 0.5 * (a_L + a_R)
+    p_pvrs = 0.5 * (p_L + p_R) - 0.5 * (u_R - u_L) * rho_bar * a_bar
+    return np.maximum(0.0, p_pvrs)
+
+
+def signal_speed_toro(u_L, u_R, a_L, a_R, rho_L, rho_R, p_L, p_R, gamma, **kw):
+    """signal_speeds.py:135-157 (Toro 10.59-10.60)."""
+    p_star = estimate_pressure(u_L, u_R, a_L, a_R, rho_L, rho_R, p_L, p_R)
+    gamma_ = (gamma + 1) * 0.5 / gamma
+    with np.errstate(invalid="ignore"):
+        q_L = 1.0 * (p_star <= p_L) + np.sqrt(1 + gamma_ * (p_star / p_L - 1)) * (p_star > p_L)
+        q_R = 1.0 * (p_star <= p_R) + np.sqrt(1 + gamma_ * (p_star / p_R - 1)) * (p_star > p_R)
+    return u_L - a_L * q_L, u_R + a_R * q_R
+
+
+SIGNAL_SPEEDS = {
+    "ARITHMETIC": signal_speed_arithmetic,
+    "RUSANOV": signal_speed_rusanov,
+    "DAVIS": signal_speed_davis,
+    "EINFELDT": signal_speed_einfeldt,
+    "TORO": signal_speed_toro,
+}
+
+
+def compute_sstar(u_L, u_R, p_L, p_R, rho_L, rho_R, S_L, S_R):
+    """signal_speeds.py:159-199 (Toro 10.70)."""
+    rho_delta_u_L = rho_L * (S_L - u_L)
+    rho_delta_u_R = rho_R * (S_R - u_R)
+    return ((p_R - p_L) + (u_L * rho_delta_u_L - u_R * rho_delta_u_R)) / (rho_delta_u_L - rho_delta_u_R)
+
+
+def _hllc_flux_star(W, U, S_K, S_star, axis, side):
+    """HLLC._compute_flux_star_K_single_phase (HLLC.py:41-78)."""
+    v = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    pre = (S_K - W[v]) / (S_K - S_star) * W[0]                                    # :52
+    u_star = [pre, pre, pre, pre,
+              pre * (U[4] / U[0] + (S_star - W[v]) * (S_star + W[4] / W[0] / (S_K - W[v])))]  # :54-60
+    u_star[v] = u_star[v] * S_star                                                # :61
+    u_star[m0] = u_star[m0] * W[m0]                                               # :62
+    u_star[m1] = u_star[m1] * W[m1]                                               # :63
+    u_star = np.stack(u_star)
+    F = physical_flux_xi(W, U, axis)                                              # :66
+    S = np.minimum(S_K, 0.0) if side == "L" else np.maximum(S_K, 0.0)            # :69-72
+    return F + S * (u_star - U)                                                   # :76
+
+
+def hllc(W_L, W_R, U_L, U_R, axis, gamma, signal_speed="EINFELDT"):
+    """HLLC._solve_riemann_problem_xi_single_phase (HLLC.py:80-126)."""
+    v = axis + 1
+    c_L = speed_of_sound(W_L[4], W_L[0], gamma)
+    c_R = speed_of_sound(W_R[4], W_R[0], gamma)
+    S_L, S_R = SIGNAL_SPEEDS[signal_speed](W_L[v], W_R[v], c_L, c_R, rho_L=W_L[0], rho_R=W_R[0],
+                                            p_L=W_L[4], p_R=W_R[4], gamma=gamma)
+    S_star = compute_sstar(W_L[v], W_R[v], W_L[4], W_R[4], W_L[0], W_R[0], S_L, S_R)
+    Fs_L = _hllc_flux_star(W_L, U_L, S_L, S_star, axis, "L")
+    Fs_R = _hllc_flux_star(W_R, U_R, S_R, S_star, axis, "R")
+    sgn = np.sign(S_star)
+    return 0.5 * (1 + sgn) * Fs_L + 0.5 * (1 - sgn) * Fs_R                        # :123-124
+
+
+def rusanov(W_L, W_R, U_L, U_R, axis, gamma, signal_speed=None):
+    """Rusanov._solve_riemann_problem_xi_single_phase (Rusanov.py:25-47)."""
+    v = axis + 1
+    c_L = speed_of_sound(W_L[4], W_L[0], gamma)
+    c_R = speed_of_sound(W_R[4], W_R[0], gamma)
+    alpha = np.maximum(np.abs(W_L[v]) + c_L, np.abs(W_R[v]) + c_R)
+    F_L = physical_flux_xi(W_L, U_L, axis)
+    F_R = physical_flux_xi(W_R, U_R, axis)
+    return 0.5 * (F_L + F_R) - 0.5 * alpha * (U_R - U_L)
+
+
+RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov}
+
+
+# --------------------------------------------------------------------------
+# Setup + high-order Godunov flux
+# --------------------------------------------------------------------------
+@dataclass
+class Setup:
+    """Subset of NumericalSetup / CaseSetup on the hot path."""
+
+    gamma: float = 1.4
+    stencil: str = "WENO5-JS"                 # reconstruction_stencil
+    variable: str = "PRIMITIVE"               # reconstruction_variable
+    riemann: str = "HLLC"                     # riemann_solver
+    signal_speed: str = "EINFELDT"            # signal_speed
+    frozen_state: str = "ARITHMETIC"          # frozen_state
+    cfl: float = 0.5
+    fixed_dt: Optional[float] = None
+    bcs: Dict[str, str] = field(default_factory=lambda: {f: "PERIODIC" for f in FACE_LOCATIONS})
+    eps_stencil: float = SPATIAL_STENCIL_EPS
+
+
+def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
+    """HighOrderGodunov.reconstruct_xi (high_order_godunov.py:233-419).
+
+    Returns U_L, U_R, W_L, W_R on the faces along axis.
+    """
+    g = s.gamma
+    if s.variable == "PRIMITIVE":                                             # :267-280
+        W_L = weno5_reconstruct_xi(W, dom, axis, 0, s.stencil, s.eps_stencil)
+        W_R = weno5_reconstruct_xi(W, dom, axis, 1, s.stencil, s.eps_stencil)
+        U_L, U_R = cons_from_prim(W_L, g), cons_from_prim(W_R, g)
+    elif s.variable == "CONSERVATIVE":                                        # :282-296
+        U_L = weno5_reconstruct_xi(U, dom, axis, 0, s.stencil, s.eps_stencil)
+        U_R = weno5_reconstruct_xi(U, dom, axis, 1, s.stencil, s.eps_stencil)
+        W_L, W_R = prim_from_cons(U_L, g), prim_from_cons(U_R, g)
+    elif s.variable == "CHAR-PRIMITIVE":                                      # :298-316,401-402
+        win = stencil_window(W, dom, axis)
+        cl, cr = face_LR_cells(W, dom, axis)
+        P, _, _, c, cc, _ = frozen_state(cl, cr, g, s.frozen_state)
+        # expand_dims(axis=axis-3) of the frozen state == broadcast over the window axis
+        char = char_from_prim(win, P[:, None], c[None], cc[None], axis)
+        cL = weno5_reconstruct_window(char, 0, s.stencil, s.eps_stencil)
+        cR = weno5_reconstruct_window(char, 1, s.stencil, s.eps_stencil)
+        W_L = prim_from_char(cL, P, c, cc, axis)
+        W_R = prim_from_char(cR, P, c, cc, axis)
+        U_L, U_R = cons_from_prim(W_L, g), cons_from_prim(W_R, g)
+    elif s.variable == "CHAR-CONSERVATIVE":                                   # :404-417
+        win = stencil_window(U, dom, axis)
+        cl, cr = face_LR_cells(W, dom, axis)
+        fs = frozen_state(cl, cr, g, s.frozen_state)
+        R, L = eigen_conservatives(*fs, axis=axis)
+        char = transform(L[:, :, None], win)
+        cL = weno5_reconstruct_window(char, 0, s.stencil, s.eps_stencil)
+        cR = weno5_reconstruct_window(char, 1, s.stencil, s.eps_stencil)
+        U_L, U_R = transform(R, cL), transform(R, cR)
+        W_L, W_R = prim_from_cons(U_L, g), prim_from_cons(U_R, g)
+    else:
+        raise NotImplementedError(s.variable)
+    return U_L, U_R, W_L, W_R
+
+
+def compute_flux_xi(W, U, dom: Domain, axis: int, s: Setup):
+    """HighOrderGodunov.compute_flux_xi (high_order_godunov.py:117-231), single phase,
+    no limiter: reconstruct then solve the Riemann problem."""
+    U_L, U_R, W_L, W_R = reconstruct_xi(W, U, dom, axis, s)
+    return RIEMANN[s.riemann](W_L, W_R, U_L, U_R, axis, s.gamma, s.signal_speed)
+
+
+def flux_divergence(F, one_dx, axis):
+    """space_solver.py:489,597-599: zeros + (1/dx) * (F[:-1] - F[1:]) along axis."""
+    lo = [slice(None)] * 3
+    hi = [slice(None)] * 3
+    lo[axis] = slice(None, -1)
+    hi[axis] = slice(1, None)
+    lo = (Ellipsis,) + tuple(lo)
+    hi = (Ellipsis,) + tuple(hi)
+    return 0.0 + one_dx * (F[lo] - F[hi])
+
+
+def compute_rhs(W, U, dom: Domain, s: Setup):
+    """SpaceSolver.compute_rhs (space_solver.py:266,289-314): 0.0 + rhs_x + rhs_y + rhs_z."""
+    rhs = 0.0
+    for axis in dom.active:
+        F = compute_flux_xi(W, U, dom, axis, s)
+        rhs = rhs + flux_divergence(F, dom.one_dx[axis], axis)
+    return rhs
+
+
+# --------------------------------------------------------------------------
+# Halos (halos/outer/material.py, boundary_condition.py, halo_slices.py)
+# --------------------------------------------------------------------------
+def _face_fill_slice(dom: Domain, loc: str):
+    """halo_slices.py:66-73."""
+    nh = dom.nh
+    a = FACE_AXIS[loc]
+    s = [dom.dom(0), dom.dom(1), dom.dom(2)]
+    s[a] = slice(-nh, None) if loc in ("east", "north", "top") else slice(None, nh)
+    return (Ellipsis,) + tuple(s)
+
+
+def _face_retrieve_slice(dom: Domain, loc: str, bc: str):
+    """boundary_condition.py:564-595."""
+    nh = dom.nh
+    a = FACE_AXIS[loc]
+    hi = loc in ("east", "north", "top")
+    s = [dom.dom(0), dom.dom(1), dom.dom(2)]
+    if bc == "PERIODIC":
+        s[a] = slice(nh, 2 * nh) if hi else slice(-2 * nh, -nh)
+    elif bc == "SYMMETRY":
+        s[a] = slice(-nh - 1, -2 * nh - 1, -1) if hi else slice(2 * nh - 1, nh - 1, -1)
+    elif bc == "ZEROGRADIENT":
+        s[a] = slice(-nh - 1, -nh) if hi else slice(nh, nh + 1)
+    else:
+        raise NotImplementedError(bc)
+    return (Ellipsis,) + tuple(s)
+
+
+def halo_update(W, U, dom: Domain, s: Setup):
+    """BoundaryConditionMaterial.face_halo_update for PERIODIC / ZEROGRADIENT /
+    SYMMETRY (outer/material.py:94-287, miscellaneous :868-894).  U halo is
+    recomputed from the W halo (:248-250).  Face halos only; edge and vertex
+    halos are not filled on the inviscid path (halo_manager.py:125-129)."""
+    W = W.copy()
+    U = None if U is None else U.copy()
+    for loc in FACE_LOCATIONS:
+        if FACE_AXIS[loc] not in dom.active:
+            continue
+        bc = s.bcs[loc]
+        halos = W[_face_retrieve_slice(dom, loc, bc)]
+        if bc == "SYMMETRY":
+            sign = np.ones((5, 1, 1, 1))
+            sign[FACE_AXIS[loc] + 1] = -1.0                  # boundary_condition.py:721-731
+            halos = halos * sign
+        fill = _face_fill_slice(dom, loc)
+        # mask = 1: buffer*(1-1) + halos*1 == halos for finite values (:278-282)
+        W[fill] = halos
+        if U is not None:
+            U[fill] = cons_from_prim(halos, s.gamma)
+    return W, U
+
+
+# --------------------------------------------------------------------------
+# Time integration (time_integration/RK3.py, time_integrator.py, time_step_size.py)
+# --------------------------------------------------------------------------
+RK3_TIMESTEP_MULTIPLIER = (1.0, 0.25, 2.0 / 3.0)            # RK3.py:27
+RK3_INCREMENT_FACTOR = (1.0, 0.5, 1.0)                      # RK3.py:28
+RK3_CONS_MULTIPLIER = ((0.25, 0.75), (2.0 / 3.0, 1.0 / 3.0))  # RK3.py:29
+
+
+def rk3_stage_update(U, U_n, rhs, dt, stage, dom: Domain):
+    """perform_stage_integration (time_integrator.py:108-227) for RK3:
+    stage>0: U = c0*U + c1*U_n on the whole buffer (RK3.py:49-50);
+    then U[interior] += (dt*m_stage) * rhs (RK3.py:60-61, time_integrator.py:57)."""
+    if stage > 0:
+        c0, c1 = RK3_CONS_MULTIPLIER[stage - 1]
+        U = c0 * U + c1 * U_n
+    else:
+        U = U.copy()
+    timestep = dt * RK3_TIMESTEP_MULTIPLIER[stage]
+    U[dom.interior] = U[dom.interior] + timestep * rhs
+    return U
+
+
+def compute_dt(W, dom: Domain, s: Setup, eps=EPS64):
+    """compute_time_step_size, convective part (time_step_size.py:40-43,103-109,154-155)."""
+    if s.fixed_dt:
+        return s.fixed_dt
+    Wi = W[dom.interior]
+    c = speed_of_sound(Wi[4], Wi[0], s.gamma)
+    abs_velocity = 0.0
+    for i in dom.active:
+        abs_velocity = abs_velocity + (np.abs(Wi[i + 1]) + c)
+    dt = dom.dx_min / (np.max(abs_velocity) + eps)
+    return dt * s.cfl
+
+
+def rk3_step(W, U, dt, dom: Domain, s: Setup, rhs_fn: Callable = None):
+    """One TVD-RK3 step in the order of SimulationManager.do_runge_kutta_stages
+    (simulation_manager.py:770-1047): rhs -> stage integration -> W = prim(U) on the
+    whole buffer -> face halo update with U halo = cons(W halo)."""
+    rhs_fn = rhs_fn or (lambda W_, U_: compute_rhs(W_, U_, dom, s))
+    U_n = U
+    for stage in range(3):
+        rhs = rhs_fn(W, U)
+        U = rk3_stage_update(U, U_n, rhs, dt, stage, dom)
+        with np.errstate(all="ignore"):
+            W = prim_from_cons(U, s.gamma)
+        W, U = halo_update(W, U, dom, s)
+    return W, U
+
+
+def create_buffers(prim_interior, dom: Domain, s: Setup, eps=EPS64):
+    """from_primitive_initial_condition (material_fields_initializer.py:590-690):
+    eps-filled buffer (helper_functions.py:49-60), interior set, U = cons(W) on the
+    whole buffer, face halo update at t=0."""
+    W = np.ones((5,) + dom.shape) * eps
+    W[dom.interior] = prim_interior
+    U = cons_from_prim(W, s.gamma)
+    return halo_update(W, U, dom, s)
+
+
+def simulate(W, U, dom: Domain, s: Setup, end_time=None, end_step=None):
+    """SimulationManager.advance loop (simulation_manager.py:325-328): first dt from the
+    initial state (time_control_initializer.py:55-66); step with the current dt; new dt
+    from the new state (:612-628); stop at the first t >= end_time (no clipping)."""
+    t, step = 0.0, 0
+    end_time = np.inf if end_time is None else end_time
+    end_step = np.iinfo(np.int64).max if end_step is None else end_step
+    dt = compute_dt(W, dom, s)
+    while t < end_time and step < end_step:
+        W, U = rk3_step(W, U, dt, dom, s)
+        t += dt
+        step += 1
+        dt = compute_dt(W, dom, s)
+    return W, U, t, step
+
+
+# --------------------------------------------------------------------------
+# Initial conditions of the workloads
+# --------------------------------------------------------------------------
+def sod_initial(n=1000):
+    """1-D Sod on [0,1]: (rho,u,p) = (1,0,1) for x<=0.5 else (0.125,0,0.1)
+    (examples/examples_1D/02_sod_shock_tube/sod.json)."""
+    dom = Domain((n, 1, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), nh=5)
+    x = dom.cell_centers()[0].reshape(-1, 1, 1)
+    rho = 1.0 * (x <= 0.5) + 0.125 * (x > 0.5)
+    p = 1.0 * (x <= 0.5) + 0.1 * (x > 0.5)
+    z = np.zeros_like(x)
+    return dom, np.stack([rho, z, z, z, p])
+
+
+def tgv_initial(n=256, nh=5, gamma=1.4, mach=0.1):
+    """Inviscid Taylor-Green vortex on [0,2pi)^3: rho=1, u=sin x cos y cos z,
+    v=-cos x sin y cos z, w=0, p = p0 + (1/16)(cos2x+cos2y)(cos2z+2), p0 = 1/(gamma Ma^2)
+    (examples/examples_3D/01_tgv/tgv.json)."""
+    L = 2.0 * np.pi
+    dom = Domain((n, n, n), ((0.0, L), (0.0, L), (0.0, L)), nh=nh)
+    xs = dom.cell_centers()
+    X, Y, Z = np.meshgrid(xs[0], xs[1], xs[2], indexing="ij")
+    rho = np.ones_like(X)
+    u = np.sin(X) * np.cos(Y) * np.cos(Z)
+    v = -np.cos(X) * np.sin(Y) * np.cos(Z)
+    w = np.zeros_like(X)
+    p = 1.0 / gamma / mach ** 2 + 1.0 / 16.0 * ((np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2))
+    return dom, np.stack([rho, u, v, w, p])

@@ -1,0 +1,409 @@
+"""Pins for the CPU oracle (oracle/jxf_oracle.py).
+
+The reference ships no tests, golden vectors or fixtures for this path, and it
+cannot be imported here (no jax).  Each oracle function is therefore pinned by
+a check that mathematics fixes and that a plausible transcription mistake
+(dropped term, wrong sign or index, transposed operand) would fail:
+
+* WENO5-JS / -Z: exactness on quadratics (all candidate polynomials), constant
+  data, left/right mirror identity, WENO-Z linear weights when beta0 == beta2
+  (closed-form 5-point linear scheme), 5th-order convergence on smooth data.
+* characteristic projection: L A L^-1 = diag(-c,0,0,0,c) for the primitive
+  Euler Jacobian (eigenvectors from mathematics), round trip.
+* conservative eigenvectors: R L = I and L A R = diag(u-c,u,u,u,u+c) with A the
+  finite-difference Jacobian of the physical flux.
+* prim/cons and physical flux: closed forms.
+* HLLC / Rusanov: consistency, supersonic upwinding, exact resolution of
+  stationary / moving contacts and shear waves, mirror symmetry.
+* RHS: free-stream preservation, discrete conservation, axis-permutation
+  equivariance.
+* RK3: stability polynomial on a linear problem.
+* halos: index identities; CFL: closed form.
+* whole path: Sod shock tube against the exact Riemann solution (Toro ch. 4).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import jxf_oracle as o
+from tests.exact_riemann import sample as exact_sample
+
+RNG = np.random.default_rng(1234)
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+# ----------------------------------------------------------------------------- WENO
+def _dom1d(n, nh=3, a=0.0, b=1.0):
+    return o.Domain((n, 1, 1), ((a, b), (0, 1), (0, 1)), nh=nh)
+
+
+def _cell_avg_buffer(dom, antiderivative):
+    """Exact cell averages (interior and halos) of f given its antiderivative."""
+    h = dom.dx[0]
+    a = dom.ranges[0][0]
+    idx = np.arange(dom.shape[0]) - dom.nh
+    lo = a + idx * h
+    hi = lo + h
+    avg = (antiderivative(hi) - antiderivative(lo)) / h
+    return avg.reshape(1, -1, 1, 1)
+
+
+@pytest.mark.parametrize("stencil", ["WENO5-JS", "WENO5-Z"])
+@pytest.mark.parametrize("j", [0, 1])
+def test_weno5_quadratic_exact(stencil, j):
+    dom = _dom1d(24)
+    c0, c1, c2 = 0.7, -1.3, 2.1
+    F = lambda x: c0 * x + c1 * x ** 2 / 2 + c2 * x ** 3 / 3
+    buf = _cell_avg_buffer(dom, F)
+    faces = np.arange(dom.cells[0] + 1) * dom.dx[0]
+    exact = c0 + c1 * faces + c2 * faces ** 2
+    got = o.weno5_reconstruct_xi(buf, dom, 0, j, stencil)[0, :, 0, 0]
+    assert rel(got, exact) < 1e-13
+
+
+@pytest.mark.parametrize("stencil", ["WENO5-JS", "WENO5-Z"])
+def test_weno5_constant_and_mirror(stencil):
+    dom = _dom1d(16)
+    buf = np.full((1,) + dom.shape, 3.25)
+    for j in (0, 1):
+        assert np.all(o.weno5_reconstruct_xi(buf, dom, 0, j, stencil) == 3.25)
+    buf = RNG.standard_normal((2,) + dom.shape)
+    L = o.weno5_reconstruct_xi(buf, dom, 0, 0, stencil)
+    Rrev = o.weno5_reconstruct_xi(buf[:, ::-1].copy(), dom, 0, 1, stencil)
+    # face f of the reversed buffer is face N-f of the original
+    assert rel(Rrev[:, ::-1], L) < 1e-14
+
+
+def test_weno5_z_linear_weights_closed_form():
+    # beta0 == beta2 when the 5 values are symmetric about u_i -> tau5 = 0 ->
+    # alpha_k = d_k -> the 5-point linear upwind scheme of Jiang & Shu (1996).
+    a, b, c = RNG.standard_normal((3, 1000))
+    got = o.weno5_z_point(a, b, c, b, a)
+    lin = (2 * a - 13 * b + 47 * c + 27 * b - 3 * a) / 60.0
+    assert rel(got, lin) < 1e-13
+
+
+@pytest.mark.parametrize("stencil,fun", [("WENO5-JS", "exp"), ("WENO5-Z", "exp"), ("WENO5-Z", "sin")])
+def test_weno5_fifth_order(stencil, fun):
+    if fun == "exp":
+        F, f = np.exp, np.exp
+    else:
+        F = lambda x: -np.cos(2 * np.pi * x) / (2 * np.pi)
+        f = lambda x: np.sin(2 * np.pi * x)
+    errs = []
+    for n in (40, 80, 160):
+        dom = _dom1d(n)
+        buf = _cell_avg_buffer(dom, F)
+        faces = np.arange(n + 1) * dom.dx[0]
+        got = o.weno5_reconstruct_xi(buf, dom, 0, 0, stencil)[0, :, 0, 0]
+        errs.append(np.max(np.abs(got - f(faces))))
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(orders > 4.5), orders
+
+
+# ----------------------------------------------------------------------------- EOS / flux
+def _random_prims(shape, rng=RNG):
+    rho = rng.uniform(0.2, 2.0, shape)
+    vel = rng.uniform(-1.0, 1.0, (3,) + shape)
+    p = rng.uniform(0.2, 3.0, shape)
+    return np.stack([rho, vel[0], vel[1], vel[2], p])
+
+
+@pytest.mark.parametrize("gamma", [1.4, 5 / 3])
+def test_prim_cons_closed_form_and_roundtrip(gamma):
+    W = _random_prims((50,))
+    U = o.cons_from_prim(W, gamma)
+    E = W[4] / (gamma - 1) + 0.5 * W[0] * (W[1] ** 2 + W[2] ** 2 + W[3] ** 2)
+    assert rel(U[0], W[0]) == 0
+    assert rel(U[1:4], W[0] * W[1:4]) < 1e-15
+    assert rel(U[4], E) < 1e-14
+    assert rel(o.prim_from_cons(U, gamma), W) < 1e-13
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_physical_flux_closed_form(axis):
+    g = 1.4
+    W = _random_prims((20,))
+    U = o.cons_from_prim(W, g)
+    un = W[axis + 1]
+    expect = np.stack([W[0] * un,
+                       W[0] * un * W[1] + (axis == 0) * W[4],
+                       W[0] * un * W[2] + (axis == 1) * W[4],
+                       W[0] * un * W[3] + (axis == 2) * W[4],
+                       un * (U[4] + W[4])])
+    assert rel(o.physical_flux_xi(W, U, axis), expect) < 1e-14
+
+
+# ----------------------------------------------------------------------------- eigen
+def _prim_jacobian(rho, un, c, axis):
+    """Primitive-variable Jacobian of the Euler equations along axis (any textbook,
+    e.g. Toro eq. 3.80 rotated): d_t w + A d_n w = 0, w = (rho,u,v,w,p)."""
+    A = np.zeros((5, 5))
+    v = axis + 1
+    for i in range(5):
+        A[i, i] = un
+    A[0, v] = rho
+    A[v, 4] = 1.0 / rho
+    A[4, v] = rho * c * c
+    return A
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("frozen", ["ARITHMETIC", "ROE"])
+def test_char_projection_diagonalises_jacobian(axis, frozen):
+    g = 1.4
+    WL, WR = _random_prims((1,)), _random_prims((1,))
+    P, _, _, c, cc, _ = o.frozen_state(WL, WR, g, frozen)
+    Lm = np.stack([o.char_from_prim(np.eye(5)[:, k].reshape(5, 1), P, c, cc, axis)[:, 0] for k in range(5)], axis=1)
+    Rm = np.stack([o.prim_from_char(np.eye(5)[:, k].reshape(5, 1), P, c, cc, axis)[:, 0] for k in range(5)], axis=1)
+    assert np.max(np.abs(Lm @ Rm - np.eye(5))) < 1e-13
+    A = _prim_jacobian(P[0, 0], 0.0, c[0], axis)
+    D = Lm @ A @ Rm
+    expect = np.diag([-c[0], 0, 0, 0, c[0]])
+    assert np.max(np.abs(D - expect)) < 1e-12 * c[0]
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_eigen_conservatives(axis):
+    g = 1.4
+    WL, WR = _random_prims((1,)), _random_prims((1,))
+    fs = o.frozen_state(WL, WR, g, "ARITHMETIC")
+    R, L = o.eigen_conservatives(*fs, axis=axis)
+    R, L = R[:, :, 0], L[:, :, 0]
+    assert np.max(np.abs(R @ L - np.eye(5))) < 1e-12
+    # conservative flux Jacobian at the frozen (arithmetic-mean) state, central FD
+    Wf = fs[0][:, 0]
+    U0 = o.cons_from_prim(Wf.reshape(5, 1), g)[:, 0]
+    A = np.zeros((5, 5))
+    for k in range(5):
+        h = 1e-6 * max(1.0, abs(U0[k]))
+        Up, Um = U0.copy(), U0.copy()
+        Up[k] += h
+        Um[k] -= h
+        Fp = o.physical_flux_xi(o.prim_from_cons(Up.reshape(5, 1), g), Up.reshape(5, 1), axis)[:, 0]
+        Fm = o.physical_flux_xi(o.prim_from_cons(Um.reshape(5, 1), g), Um.reshape(5, 1), axis)[:, 0]
+        A[:, k] = (Fp - Fm) / (2 * h)
+    D = L @ A @ R
+    u, c = Wf[axis + 1], fs[3][0]
+    expect = np.diag([u - c, u, u, u, u + c])
+    assert np.max(np.abs(D - expect)) < 1e-6
+
+
+# ----------------------------------------------------------------------------- Riemann
+SPEEDS = ["ARITHMETIC", "RUSANOV", "DAVIS", "EINFELDT", "TORO"]
+
+
+@pytest.mark.parametrize("speed", SPEEDS)
+def test_signal_speeds_equal_states(speed):
+    u, a = RNG.uniform(-1, 1, 10), RNG.uniform(0.5, 2, 10)
+    rho, p = RNG.uniform(0.5, 2, 10), RNG.uniform(0.5, 2, 10)
+    a = np.sqrt(1.4 * p / rho)
+    SL, SR = o.SIGNAL_SPEEDS[speed](u, u, a, a, rho_L=rho, rho_R=rho, p_L=p, p_R=p, gamma=1.4)
+    if speed == "RUSANOV":
+        assert rel(SR, np.abs(u) + a) < 1e-15 and rel(SL, -(np.abs(u) + a)) < 1e-15
+    else:
+        assert rel(SL, u - a) < 1e-14 and rel(SR, u + a) < 1e-14
+
+
+@pytest.mark.parametrize("solver,speed", [("HLLC", s) for s in SPEEDS] + [("RUSANOV", None)])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_riemann_consistency(solver, speed, axis):
+    g = 1.4
+    W = _random_prims((30,))
+    U = o.cons_from_prim(W, g)
+    F = o.RIEMANN[solver](W, W, U, U, axis, g, speed)
+    assert rel(F, o.physical_flux_xi(W, U, axis)) < 1e-13
+
+
+@pytest.mark.parametrize("speed", [s for s in SPEEDS if s != "RUSANOV"])  # Rusanov speeds: S_L < 0 always
+def test_hllc_supersonic_upwinding(speed):
+    g = 1.4
+    WL, WR = _random_prims((20,)), _random_prims((20,))
+    for sgn in (+1, -1):
+        wl, wr = WL.copy(), WR.copy()
+        wl[1] = sgn * 6.0
+        wr[1] = sgn * 6.5  # |u| > c_max (c <= sqrt(1.4*3/0.2) ~ 4.6)
+        ul, ur = o.cons_from_prim(wl, g), o.cons_from_prim(wr, g)
+        F = o.hllc(wl, wr, ul, ur, 0, g, speed)
+        up = o.physical_flux_xi(wl, ul, 0) if sgn > 0 else o.physical_flux_xi(wr, ur, 0)
+        assert rel(F, up) < 1e-13
+
+
+@pytest.mark.parametrize("speed", SPEEDS)
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_hllc_resolves_contact_and_shear(speed, axis):
+    """A contact/shear discontinuity (equal p and normal velocity) is an exact solution
+    the HLLC solver reproduces: the flux is the upwind physical flux (Toro ch. 10.4)."""
+    g = 1.4
+    n = 20
+    WL, WR = _random_prims((n,)), _random_prims((n,))
+    v = axis + 1
+    WR[4] = WL[4]
+    for un in (0.0, 0.3, -0.4):
+        WL[v] = un
+        WR[v] = un
+        UL, UR = o.cons_from_prim(WL, g), o.cons_from_prim(WR, g)
+        F = o.hllc(WL, WR, UL, UR, axis, g, speed)
+        if un > 0:
+            expect = o.physical_flux_xi(WL, UL, axis)
+        elif un < 0:
+            expect = o.physical_flux_xi(WR, UR, axis)
+        else:  # stationary contact: only the pressure flux
+            expect = np.zeros_like(UL)
+            expect[v] = WL[4]
+        assert np.max(np.abs(F - expect)) < 1e-13 * np.max(np.abs(UL))
+
+
+@pytest.mark.parametrize("solver,speed", [("HLLC", s) for s in SPEEDS] + [("RUSANOV", None)])
+def test_riemann_mirror_symmetry(solver, speed):
+    """x -> -x: swap L/R and negate the normal velocity; mass, tangential-momentum and
+    energy fluxes change sign, the normal-momentum flux does not."""
+    g = 1.4
+    WL, WR = _random_prims((40,)), _random_prims((40,))
+    UL, UR = o.cons_from_prim(WL, g), o.cons_from_prim(WR, g)
+    F = o.RIEMANN[solver](WL, WR, UL, UR, 0, g, speed)
+    ML, MR = WR.copy(), WL.copy()
+    ML[1] *= -1
+    MR[1] *= -1
+    G = o.RIEMANN[solver](ML, MR, o.cons_from_prim(ML, g), o.cons_from_prim(MR, g), 0, g, speed)
+    sign = np.array([-1, 1, -1, -1, -1]).reshape(5, 1)
+    assert np.max(np.abs(G - sign * F)) < 1e-12 * np.max(np.abs(F))
+
+
+def test_rusanov_closed_form():
+    g = 1.4
+    WL, WR = _random_prims((15,)), _random_prims((15,))
+    UL, UR = o.cons_from_prim(WL, g), o.cons_from_prim(WR, g)
+    a = np.maximum(np.abs(WL[2]) + np.sqrt(g * WL[4] / WL[0]), np.abs(WR[2]) + np.sqrt(g * WR[4] / WR[0]))
+    expect = 0.5 * (o.physical_flux_xi(WL, UL, 1) + o.physical_flux_xi(WR, UR, 1)) - 0.5 * a * (UR - UL)
+    assert rel(o.rusanov(WL, WR, UL, UR, 1, g), expect) < 1e-14
+
+
+# ----------------------------------------------------------------------------- RHS
+SETUPS = [
+    o.Setup(stencil=st, variable=var, riemann=rs, signal_speed=sp, frozen_state=fz)
+    for st, var, rs, sp, fz in [
+        ("WENO5-JS", "PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
+        ("WENO5-Z", "CHAR-PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
+        ("WENO5-Z", "CONSERVATIVE", "RUSANOV", None, "ARITHMETIC"),
+        ("WENO5-JS", "CHAR-CONSERVATIVE", "HLLC", "TORO", "ARITHMETIC"),
+        ("WENO5-Z", "CHAR-PRIMITIVE", "HLLC", "DAVIS", "ROE"),
+    ]
+]
+
+
+def _periodic_3d(n=(10, 8, 6), nh=3, seed=7):
+    rng = np.random.default_rng(seed)
+    dom = o.Domain(n, ((0, 1.0), (0, 0.8), (0, 0.6)), nh=nh)
+    xs = dom.cell_centers()
+    X, Y, Z = np.meshgrid(*xs, indexing="ij")
+    ph = rng.uniform(0, 2 * np.pi, 8)
+    k = 2 * np.pi
+    rho = 1.0 + 0.3 * np.sin(k * X + ph[0]) * np.cos(k * Y / 0.8 + ph[1])
+    u = 0.5 * np.sin(k * Y / 0.8 + ph[2]) + 0.2 * np.cos(k * Z / 0.6 + ph[3])
+    v = 0.4 * np.cos(k * X + ph[4])
+    w = 0.3 * np.sin(k * Z / 0.6 + ph[5]) * np.sin(k * X + ph[6])
+    p = 1.0 + 0.2 * np.cos(k * X + k * Y / 0.8 + ph[7])
+    return dom, np.stack([rho, u, v, w, p])
+
+
+@pytest.mark.parametrize("s", SETUPS, ids=lambda s: f"{s.stencil}-{s.variable}-{s.riemann}-{s.signal_speed}-{s.frozen_state}")
+def test_rhs_free_stream_and_conservation(s):
+    dom, W0 = _periodic_3d()
+    W0u = np.broadcast_to(np.array([1.1, 0.3, -0.2, 0.5, 0.9]).reshape(5, 1, 1, 1), W0.shape).copy()
+    W, U = o.create_buffers(W0u, dom, s)
+    rhs = o.compute_rhs(W, U, dom, s)
+    assert np.max(np.abs(rhs)) < 1e-12
+    W, U = o.create_buffers(W0, dom, s)
+    rhs = o.compute_rhs(W, U, dom, s)
+    tot = rhs.sum(axis=(1, 2, 3))
+    scale = np.max(np.abs(rhs), axis=(1, 2, 3)) * rhs[0].size
+    assert np.all(np.abs(tot) < 1e-13 * scale)
+
+
+@pytest.mark.parametrize("s", SETUPS[:2] + SETUPS[3:4], ids=["js-prim", "z-charprim", "js-charcons"])
+def test_rhs_axis_permutation_equivariance(s):
+    """Swapping the x and y axes of the state (and the u, v components) swaps the RHS."""
+    dom, W0 = _periodic_3d(n=(8, 8, 6))
+    dom = o.Domain((8, 8, 6), ((0, 1.0), (0, 1.0), (0, 0.6)), nh=3)
+    W, U = o.create_buffers(W0, dom, s)
+    rhs = o.compute_rhs(W, U, dom, s)
+    W0t = W0.transpose(0, 2, 1, 3)[[0, 2, 1, 3, 4]].copy()
+    Wt, Ut = o.create_buffers(W0t, dom, s)
+    rhs_t = o.compute_rhs(Wt, Ut, dom, s)
+    back = rhs_t.transpose(0, 2, 1, 3)[[0, 2, 1, 3, 4]]
+    assert rel(back, rhs) < 1e-11
+
+
+# ----------------------------------------------------------------------------- RK3, halos, CFL
+def test_rk3_stability_polynomial():
+    dom = _dom1d(8)
+    s = o.Setup(bcs={"east": "PERIODIC", "west": "PERIODIC"})
+    W0 = _random_prims((8, 1, 1))
+    W, U = o.create_buffers(W0, dom, s)
+    lam, dt = -0.7, 0.3
+    rhs_fn = lambda W_, U_: lam * U_[dom.interior]
+    W1, U1 = o.rk3_step(W, U, dt, dom, s, rhs_fn=rhs_fn)
+    z = lam * dt
+    assert rel(U1[dom.interior], (1 + z + z * z / 2 + z ** 3 / 6) * U[dom.interior]) < 1e-14
+
+
+@pytest.mark.parametrize("bc", ["PERIODIC", "ZEROGRADIENT", "SYMMETRY"])
+def test_halo_identities(bc):
+    nh, n = 3, (6, 5, 4)
+    dom = o.Domain(n, nh=nh)
+    s = o.Setup(bcs={f: bc for f in o.FACE_LOCATIONS})
+    W0 = _random_prims(n)
+    W, U = o.create_buffers(W0, dom, s)
+    I = W[dom.interior]
+    for axis in range(3):
+        N = n[axis]
+        for m in range(nh):
+            lo = [dom.dom(0), dom.dom(1), dom.dom(2)]
+            hi = list(lo)
+            lo[axis] = m                 # ghost m on the low side  (position m - nh)
+            hi[axis] = N + nh + m        # ghost m on the high side (position N + m)
+            glo, ghi = W[(Ellipsis,) + tuple(lo)], W[(Ellipsis,) + tuple(hi)]
+            take = lambda i: np.take(I, i, axis=axis + 1)
+            if bc == "PERIODIC":
+                elo, ehi = take(N - nh + m), take(m)
+            elif bc == "ZEROGRADIENT":
+                elo, ehi = take(0), take(N - 1)
+            else:
+                elo, ehi = take(nh - 1 - m).copy(), take(N - 1 - m).copy()
+                elo[axis + 1] *= -1
+                ehi[axis + 1] *= -1
+            assert np.array_equal(glo, elo) and np.array_equal(ghi, ehi)
+            assert rel(U[(Ellipsis,) + tuple(hi)], o.cons_from_prim(ghi, s.gamma)) == 0
+
+
+def test_cfl_closed_form():
+    dom = o.Domain((6, 5, 4), ((0, 1.2), (0, 1.0), (0, 0.8)), nh=3)
+    s = o.Setup(cfl=0.4)
+    W0 = _random_prims((6, 5, 4))
+    W, U = o.create_buffers(W0, dom, s)
+    c = np.sqrt(1.4 * W0[4] / W0[0])
+    smax = np.max(np.abs(W0[1]) + np.abs(W0[2]) + np.abs(W0[3]) + 3 * c)
+    dt = o.compute_dt(W, dom, s)
+    assert rel(np.array(dt), np.array(0.4 * 0.2 / (smax + o.EPS64))) < 1e-15
+
+
+# ----------------------------------------------------------------------------- whole path
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+def test_sod_against_exact_solution(variable):
+    dom, W0 = o.sod_initial(400)
+    s = o.Setup(stencil="WENO5-JS", variable=variable, riemann="HLLC", signal_speed="EINFELDT",
+                bcs={"east": "ZEROGRADIENT", "west": "ZEROGRADIENT"})
+    W, U = o.create_buffers(W0, dom, s)
+    W, U, t, steps = o.simulate(W, U, dom, s, end_time=0.2)
+    x = dom.cell_centers()[0]
+    ex = exact_sample((x - 0.5) / t, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4)
+    Wi = W[:, dom.nh:-dom.nh, 0, 0]
+    # L1 errors at 400 cells are ~1.4e-3 (rho), ~3e-3 (u), ~1e-3 (p); a sign or
+    # wave-speed mistake anywhere on the path moves them by orders of magnitude.
+    l1 = [np.mean(np.abs(Wi[i] - ex[k])) for i, k in ((0, 0), (1, 1), (4, 2))]
+    assert l1[0] < 2.5e-3 and l1[1] < 5e-3 and l1[2] < 2e-3, l1
