@@ -1,0 +1,162 @@
+/*
+ * jaxfluids_amd.h -- C ABI of the MI355X (gfx950) convective-flux engine.
+ *
+ * Plain C: pointers, sizes and two POD structs; no C++ or torch types cross the
+ * boundary.  All array pointers are DEVICE pointers.  Field buffers use the
+ * JAX-Fluids layout (nvar, Nx+2nh, Ny+2nh, Nz+2nh), C-contiguous, z fastest; an
+ * inactive axis has extent 1 and no halos (initialization/helper_functions.py:49).
+ * Face arrays along `axis` are (nvar, N_axis+1, interior extents of the other
+ * axes) (solvers/space_solver.py:142-146).  `stream` is a hipStream_t (NULL =
+ * default stream).  Every call is asynchronous on `stream`; none allocates
+ * device memory; scratch comes from the caller.
+ *
+ * Status codes: 0 ok, JFA_EINVAL bad argument/shape, JFA_EUNSUPPORTED option not
+ * built, JFA_ELAUNCH a HIP launch failed.  No C++ exception crosses the ABI.
+ *
+ * Citations are relative to the JAX-Fluids v0.2.1 tree (src/jaxfluids/...).
+ */
+#ifndef JAXFLUIDS_AMD_H
+#define JAXFLUIDS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JFA_OK 0
+#define JFA_EINVAL 1
+#define JFA_EUNSUPPORTED 2
+#define JFA_ELAUNCH 3
+
+/* reconstruction_variable (solvers/__init__.py:9-12) */
+#define JFA_PRIMITIVE 0
+#define JFA_CONSERVATIVE 1
+#define JFA_CHAR_PRIMITIVE 2
+#define JFA_CHAR_CONSERVATIVE 3
+/* reconstruction_stencil (stencils/reconstruction/shock_capturing/weno/__init__.py:26-41) */
+#define JFA_WENO5_JS 0
+#define JFA_WENO5_Z 1
+/* riemann_solver (solvers/riemann_solvers/__init__.py:16-25) */
+#define JFA_HLLC 0
+#define JFA_RUSANOV 1
+/* signal_speed (solvers/riemann_solvers/__init__.py:27-34; DAVIS2 is not callable from HLLC there) */
+#define JFA_SS_ARITHMETIC 0
+#define JFA_SS_RUSANOV 1
+#define JFA_SS_DAVIS 2
+#define JFA_SS_EINFELDT 3
+#define JFA_SS_TORO 4
+/* frozen_state (solvers/__init__.py:5-7) */
+#define JFA_FROZEN_ARITHMETIC 0
+#define JFA_FROZEN_ROE 1
+/* boundary types handled on the path (halos/outer/material.py:142-144) */
+#define JFA_BC_PERIODIC 0
+#define JFA_BC_ZEROGRADIENT 1
+#define JFA_BC_SYMMETRY 2
+/* storage / compute precision */
+#define JFA_F64 0
+#define JFA_F32 1
+
+/* Uniform grid of one domain (domain/domain_information.py:68-300). */
+typedef struct jfa_grid {
+    int32_t n[3];   /* interior cells per axis; 1 marks an inactive axis */
+    int32_t nh;     /* halo cells on each side of an active axis (>= 3 for WENO5) */
+    double dx[3];   /* cell size per axis, (b - a) / n (domain/mesh_creation/homogenous.py:13) */
+} jfa_grid;
+
+/* Numerical scheme of the high-order Godunov flux (data_types/numerical_setup/conservatives.py:35-42). */
+typedef struct jfa_scheme {
+    int32_t recon_var;     /* JFA_PRIMITIVE .. JFA_CHAR_CONSERVATIVE */
+    int32_t stencil;       /* JFA_WENO5_JS / JFA_WENO5_Z */
+    int32_t riemann;       /* JFA_HLLC / JFA_RUSANOV */
+    int32_t signal_speed;  /* JFA_SS_* (HLLC only) */
+    int32_t frozen_state;  /* JFA_FROZEN_* (characteristic variables only) */
+    int32_t dtype;         /* JFA_F64 (float64 buffers) or JFA_F32 (float32 buffers) */
+    double gamma;          /* ideal-gas specific heat ratio (materials/single_materials/ideal_gas.py:29) */
+    double eps_stencil;    /* WENO epsilon, 1e-30 (config/precision.py:39,53) */
+} jfa_scheme;
+
+/* Library version string; never NULL. */
+const char* jfa_version(void);
+
+/* Last HIP error string seen by this thread (for diagnostics after JFA_ELAUNCH). */
+const char* jfa_last_error(void);
+
+/* Cell-face WENO5 reconstruction of every variable of `buffer` along `axis`:
+ * j = 0 left state, j = 1 right state of each face.
+ * Replaces WENO5JS.reconstruct_xi (stencils/reconstruction/shock_capturing/weno/weno5_js.py:73-115)
+ * and WENO5Z.reconstruct_xi (.../weno/weno5_z.py:138-173) with the slice tables of
+ * WENO5Base (.../weno5_base.py:30) and SpatialStencil.array_slices (stencils/spatial_stencil.py:45-112).
+ * buffer: (nvar, buffer shape) of `dtype`; out: (nvar, face shape). */
+int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* grid, int32_t axis,
+                             int32_t j, int32_t stencil, double eps, int32_t dtype, void* out,
+                             void* stream);
+
+/* Riemann flux from reconstructed face states, all (5, nfaces) contiguous.
+ * Replaces RiemannSolver.solve_riemann_problem_xi (solvers/riemann_solvers/riemann_solver.py:52-102)
+ * for HLLC (HLLC.py:80-126) and Rusanov (Rusanov.py:25-47), single phase. */
+int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void* U_R, int64_t nfaces,
+                   int32_t axis, const jfa_scheme* scheme, void* flux, void* stream);
+
+/* Numerical flux along `axis` on every face: reconstruction (4 variable modes), prim<->cons
+ * conversion and Riemann solve.  flux: (5, face shape).
+ * Replaces HighOrderGodunov.compute_flux_xi (solvers/convective_fluxes/high_order_godunov.py:117-231)
+ * with reconstruct_xi (:233-419). */
+int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme,
+                        int32_t axis, void* flux, void* stream);
+
+/* Convective right-hand side 0.0 + rhs_x + rhs_y + rhs_z, rhs_xi = (1/dx)(F[:-1]-F[1:]).
+ * rhs: (5, Nx, Ny, Nz).  workspace: >= jfa_rhs_workspace_bytes(grid, dtype) bytes (face fluxes).
+ * Replaces SpaceSolver.compute_rhs (solvers/space_solver.py:151-453) and compute_rhs_xi (:456-674),
+ * convective flux only. */
+int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme,
+                    void* rhs, void* workspace, size_t workspace_bytes, void* stream);
+size_t jfa_rhs_workspace_bytes(const jfa_grid* grid, int32_t dtype);
+
+/* One TVD-RK3 stage: for stage > 0, U = c0*U + c1*U_n (RK3.py:31-51); then
+ * U[interior] += (dt*m_stage)*rhs (RK3.py:53-62, time_integrator.py:34-58); then
+ * W[interior] = prim(U) (equation_manager.py:108-176, simulation_manager.py:943-944).
+ * dt is read from device memory `dt_dev` when non-NULL, else `dt_host` is used.
+ * Replaces TimeIntegrator.perform_stage_integration (time_integration/time_integrator.py:108-227). */
+int jfa_rk3_stage(void* U, const void* U_n, const void* rhs, void* W, const jfa_grid* grid,
+                  const jfa_scheme* scheme, int32_t stage, const double* dt_dev, double dt_host,
+                  void* stream);
+
+/* Face-halo fill of W and U = cons(W) in the halos for PERIODIC / ZEROGRADIENT / SYMMETRY;
+ * bc[6] in the order east, west, north, south, top, bottom (JFA_BC_*).
+ * Replaces HaloManager.perform_halo_update_material (halos/halo_manager.py:146-234) ->
+ * BoundaryConditionMaterial.face_halo_update (halos/outer/material.py:94-287, miscellaneous :868-894). */
+int jfa_halo_update(void* W, void* U, const jfa_grid* grid, const int32_t bc[6], double gamma,
+                    int32_t dtype, void* stream);
+
+/* CFL time step dt = CFL * dx_min / (max_cells sum_active(|u_i| + c) + eps), written to dt_dev[0]
+ * (a device double).  workspace: >= 16 bytes of device memory.
+ * Replaces compute_time_step_size, convective part (time_integration/time_step_size.py:15-157). */
+int jfa_compute_dt(const void* W, const jfa_grid* grid, double gamma, double cfl, double eps,
+                   int32_t dtype, double* dt_dev, void* workspace, void* stream);
+
+/* Whole-buffer conversions (every cell, halos included).
+ * Replace EquationManager.get_conservatives_from_primitives (equation_manager.py:44-106) and
+ * get_primitives_from_conservatives (:108-176), single phase. */
+int jfa_cons_from_prim(const void* W, void* U, int64_t ncells, double gamma, int32_t dtype, void* stream);
+int jfa_prim_from_cons(const void* U, void* W, int64_t ncells, double gamma, int32_t dtype, void* stream);
+
+/* One fused RK3 stage of the whole hot path, for the HLLC + WENO5 path on a single domain:
+ * face fluxes of all active axes, flux divergence, stage blend and update, prim(U), and the
+ * face-halo fill of W and U for the given boundary types -- the work of jfa_compute_rhs +
+ * jfa_rk3_stage + jfa_halo_update in one launch per stage.  U_out/W_out must not alias
+ * U_in/W_in (ping-pong buffers).  Faithful to SimulationManager.do_runge_kutta_stages
+ * (simulation_manager.py:770-1047) for one stage.
+ * workspace: >= jfa_stage_workspace_bytes(grid, scheme) bytes. */
+int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out,
+                        const jfa_grid* grid, const jfa_scheme* scheme, const int32_t bc[6],
+                        int32_t stage, const double* dt_dev, double dt_host, void* workspace,
+                        size_t workspace_bytes, void* stream);
+size_t jfa_stage_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JAXFLUIDS_AMD_H */

@@ -1,0 +1,161 @@
+"""ctypes binding of libjaxfluids_amd.so (the C ABI declared in include/jaxfluids_amd.h).
+
+The product path has no CPU fallback: if the library is missing or no HIP device
+is present, calls raise.  Tensors are passed as raw device pointers together with
+torch's current HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjaxfluids_amd.so")
+
+# constants (mirror include/jaxfluids_amd.h)
+JFA_OK, JFA_EINVAL, JFA_EUNSUPPORTED, JFA_ELAUNCH = 0, 1, 2, 3
+RECON_VAR = {"PRIMITIVE": 0, "CONSERVATIVE": 1, "CHAR-PRIMITIVE": 2, "CHAR-CONSERVATIVE": 3}
+STENCIL = {"WENO5-JS": 0, "WENO5-Z": 1}
+RIEMANN = {"HLLC": 0, "RUSANOV": 1}
+SIGNAL_SPEED = {"ARITHMETIC": 0, "RUSANOV": 1, "DAVIS": 2, "EINFELDT": 3, "TORO": 4}
+FROZEN = {"ARITHMETIC": 0, "ROE": 1}
+BC = {"PERIODIC": 0, "ZEROGRADIENT": 1, "SYMMETRY": 2}
+F64, F32 = 0, 1
+FACE_LOCATIONS = ("east", "west", "north", "south", "top", "bottom")
+
+
+class JfaGrid(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("nh", ctypes.c_int32), ("dx", ctypes.c_double * 3)]
+
+
+class JfaScheme(ctypes.Structure):
+    _fields_ = [
+        ("recon_var", ctypes.c_int32),
+        ("stencil", ctypes.c_int32),
+        ("riemann", ctypes.c_int32),
+        ("signal_speed", ctypes.c_int32),
+        ("frozen_state", ctypes.c_int32),
+        ("dtype", ctypes.c_int32),
+        ("gamma", ctypes.c_double),
+        ("eps_stencil", ctypes.c_double),
+    ]
+
+
+class JFAError(RuntimeError):
+    pass
+
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+_PG = ctypes.POINTER(JfaGrid)
+_PS = ctypes.POINTER(JfaScheme)
+_PI32 = ctypes.POINTER(ctypes.c_int32)
+_PD = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); every symbol declared in include/jaxfluids_amd.h
+SIGNATURES = {
+    "jfa_version": (ctypes.c_char_p, []),
+    "jfa_last_error": (ctypes.c_char_p, []),
+    "jfa_weno5_reconstruct_xi": (_I32, [_P, _I32, _PG, _I32, _I32, _I32, _D, _I32, _P, _P]),
+    "jfa_riemann_xi": (_I32, [_P, _P, _P, _P, _I64, _I32, _PS, _P, _P]),
+    "jfa_compute_flux_xi": (_I32, [_P, _P, _PG, _PS, _I32, _P, _P]),
+    "jfa_compute_rhs": (_I32, [_P, _P, _PG, _PS, _P, _P, _SZ, _P]),
+    "jfa_rhs_workspace_bytes": (_SZ, [_PG, _I32]),
+    "jfa_rk3_stage": (_I32, [_P, _P, _P, _P, _PG, _PS, _I32, _P, _D, _P]),
+    "jfa_halo_update": (_I32, [_P, _P, _PG, _PI32, _D, _I32, _P]),
+    "jfa_compute_dt": (_I32, [_P, _PG, _D, _D, _D, _I32, _P, _P, _P]),
+    "jfa_cons_from_prim": (_I32, [_P, _P, _I64, _D, _I32, _P]),
+    "jfa_prim_from_cons": (_I32, [_P, _P, _I64, _D, _I32, _P]),
+    "jfa_rk3_stage_fused": (_I32, [_P, _P, _P, _P, _P, _PG, _PS, _PI32, _I32, _P, _D, _P, _SZ, _P]),
+    "jfa_stage_workspace_bytes": (_SZ, [_PG, _PS]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise JFAError(
+            f"{LIB_PATH} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(jaxfluids_amd has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != JFA_OK:
+        msg = {JFA_EINVAL: "invalid argument", JFA_EUNSUPPORTED: "unsupported option",
+               JFA_ELAUNCH: "HIP launch failure"}.get(rc, f"status {rc}")
+        detail = load().jfa_last_error().decode() if rc == JFA_ELAUNCH else ""
+        raise JFAError(f"{what}: {msg} {detail}".strip())
+
+
+def require_device(t: torch.Tensor, name: str = "tensor") -> None:
+    if not t.is_cuda:
+        raise JFAError(f"{name} must be a HIP device tensor (jaxfluids_amd has no CPU path)")
+    if not t.is_contiguous():
+        raise JFAError(f"{name} must be contiguous")
+    if t.dtype not in (torch.float64, torch.float32):
+        raise JFAError(f"{name} must be float64 or float32")
+
+
+def dtype_code(t: torch.Tensor) -> int:
+    return F64 if t.dtype == torch.float64 else F32
+
+
+def stream_ptr() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t: Optional[torch.Tensor]) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def make_grid(cells: Sequence[int], nh: int, dx: Sequence[float]) -> JfaGrid:
+    g = JfaGrid()
+    for i in range(3):
+        g.n[i] = int(cells[i])
+        g.dx[i] = float(dx[i])
+    g.nh = int(nh)
+    return g
+
+
+def make_scheme(recon_var: str, stencil: str, riemann: str, signal_speed: Optional[str], frozen_state: str,
+                dtype: int, gamma: float, eps_stencil: float) -> JfaScheme:
+    s = JfaScheme()
+    try:
+        s.recon_var = RECON_VAR[recon_var]
+        s.stencil = STENCIL[stencil]
+        s.riemann = RIEMANN[riemann]
+        s.signal_speed = SIGNAL_SPEED[signal_speed or "EINFELDT"]
+        s.frozen_state = FROZEN[frozen_state or "ARITHMETIC"]
+    except KeyError as e:
+        raise NotImplementedError(f"option {e} is not built into jaxfluids_amd") from None
+    s.dtype = int(dtype)
+    s.gamma = float(gamma)
+    s.eps_stencil = float(eps_stencil)
+    return s
+
+
+def bc_array(bcs) -> ctypes.Array:
+    arr = (ctypes.c_int32 * 6)()
+    for i, loc in enumerate(FACE_LOCATIONS):
+        arr[i] = BC.get(bcs.get(loc, "PERIODIC"), -1) if isinstance(bcs, dict) else BC[bcs[i]]
+        if arr[i] < 0:
+            raise NotImplementedError(f"boundary type {bcs.get(loc)} at {loc} is not on the hot path")
+    return arr

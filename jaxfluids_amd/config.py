@@ -1,0 +1,52 @@
+"""Precision configuration (config/precision.py:19-231).
+
+A process-wide singleton like the reference's ``precision``: double precision by
+default; the spatial-stencil epsilon is 1e-30 in both precisions and the general
+epsilon is the machine epsilon of the compute dtype.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+class PrecisionConfig:
+    def __init__(self) -> None:
+        self.enable_double_precision()
+        self.is_consistent_summation = False
+
+    def enable_double_precision(self) -> None:
+        # precision.py:44-56
+        self.is_double = True
+        self._eps = float(np.finfo(np.float64).eps)
+        self._spatial_stencil_eps = 1e-30
+
+    def enable_single_precision(self) -> None:
+        # precision.py:30-42
+        self.is_double = False
+        self._eps = float(np.finfo(np.float32).eps)
+        self._spatial_stencil_eps = 1e-30
+
+    def get_eps(self) -> float:
+        return self._eps
+
+    def get_spatial_stencil_eps(self) -> float:
+        return self._spatial_stencil_eps
+
+    def set_spatial_stencil_eps(self, eps: float) -> None:
+        assert isinstance(eps, float) and eps > 0.0, "spatial stencil eps must be a positive float"
+        self._spatial_stencil_eps = eps
+
+    def set_eps(self, eps: float) -> None:
+        assert isinstance(eps, float) and eps > 0.0, "eps must be a positive float"
+        self._eps = eps
+
+    def enable_consistent_summation(self) -> None:
+        raise NotImplementedError("consistent summation is not built on this path")
+
+    @property
+    def torch_dtype(self):
+        return torch.float64 if self.is_double else torch.float32
+
+
+precision = PrecisionConfig()

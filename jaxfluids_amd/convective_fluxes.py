@@ -1,0 +1,96 @@
+"""High-order Godunov convective flux on the HIP path.
+
+Mirrors solvers/convective_fluxes/convective_flux_solver.py (ABC, :17-67) and
+high_order_godunov.py (HighOrderGodunov, :29-419).  ``compute_flux_xi`` runs one
+HIP kernel per call (jfa_compute_flux_xi): per face, the 6-cell window is loaded,
+reconstructed in the configured variables (PRIMITIVE, CONSERVATIVE,
+CHAR-PRIMITIVE, CHAR-CONSERVATIVE) with WENO5-JS/-Z, converted, and the Riemann
+problem is solved -- the reference's three steps without intermediate arrays.
+"""
+from __future__ import annotations
+
+from typing import Dict, Tuple
+
+import torch
+
+from . import _native
+from .config import precision
+from .riemann_solvers import HLLC, Rusanov, SignalSpeed
+from .stencils import WENO5JS, WENO5Z
+
+
+class ConvectiveFluxSolver:
+    """convective_flux_solver.py:17-67."""
+
+    def __init__(self, convective_fluxes_setup, material_manager, domain_information, equation_manager, **kw):
+        self.convective_fluxes_setup = convective_fluxes_setup
+        self.material_manager = material_manager
+        self.domain_information = domain_information
+        self.equation_manager = equation_manager
+        self.equation_information = equation_manager.equation_information
+
+    def compute_flux_xi(self, primitives, conservatives, axis, **kwargs):
+        raise NotImplementedError
+
+
+class HighOrderGodunov(ConvectiveFluxSolver):
+    """Reconstruct -> convert -> Riemann solve, per face (high_order_godunov.py:29-231)."""
+
+    def __init__(self, convective_fluxes_setup, positivity_setup=None, diffuse_interface_setup=None,
+                 material_manager=None, domain_information=None, equation_manager=None,
+                 diffuse_interface_handler=None, positivity_handler=None, **kwargs) -> None:
+        super().__init__(convective_fluxes_setup, material_manager, domain_information, equation_manager)
+        godunov = convective_fluxes_setup.godunov
+        self.reconstruction_variable = godunov.reconstruction_variable
+        if positivity_setup is not None and getattr(positivity_setup, "is_interpolation_limiter", False):
+            raise NotImplementedError("positivity interpolation limiter is not on this path")
+        self.riemann_solver = godunov.riemann_solver(
+            material_manager=material_manager, equation_manager=equation_manager,
+            signal_speed=godunov.signal_speed, catum_setup=godunov.catum_setup)
+        self.reconstruction_stencil = godunov.reconstruction_stencil(
+            nh=domain_information.nh_conservatives, inactive_axes=domain_information.inactive_axes)
+        if self.reconstruction_variable in ("CHAR-PRIMITIVE", "CHAR-CONSERVATIVE"):
+            self.reconstruction_stencil.set_slices_stencil()
+        self.frozen_state = godunov.frozen_state
+        stencil_name = {WENO5JS: "WENO5-JS", WENO5Z: "WENO5-Z"}.get(godunov.reconstruction_stencil)
+        riemann_name = {HLLC: "HLLC", Rusanov: "RUSANOV"}.get(godunov.riemann_solver)
+        if stencil_name is None or riemann_name is None:
+            raise NotImplementedError("only WENO5-JS/-Z with HLLC or Rusanov are built on this path")
+        ss = godunov.signal_speed
+        ss_name = ss.name if isinstance(ss, SignalSpeed) else (ss if isinstance(ss, str) else "EINFELDT")
+        self._names = (self.reconstruction_variable, stencil_name, riemann_name, ss_name, self.frozen_state)
+        self._schemes = {}
+
+    def scheme(self, dtype_code: int) -> _native.JfaScheme:
+        if dtype_code not in self._schemes:
+            self._schemes[dtype_code] = _native.make_scheme(
+                *self._names, dtype_code, self.material_manager.get_gamma(), precision.get_spatial_stencil_eps())
+        return self._schemes[dtype_code]
+
+    def compute_flux_xi(self, primitives: torch.Tensor, conservatives: torch.Tensor, axis: int,
+                        curvature=None, volume_fraction=None, apertures=None, ml_setup=None
+                        ) -> Tuple[torch.Tensor, None, None, None, None]:
+        """Numerical flux on every face along axis (high_order_godunov.py:117-231).
+
+        Returns (fluxes_xi, u_hat, alpha_hat, count_interpolation_limiter,
+        count_interpolation_limiter_thinc) with the single-phase Nones.
+        """
+        _native.require_device(primitives, "primitives")
+        _native.require_device(conservatives, "conservatives")
+        dom = self.domain_information
+        if axis not in dom.active_axes_indices:
+            raise ValueError(f"axis {axis} is inactive")
+        if tuple(primitives.shape) != (5,) + dom.buffer_shape or conservatives.shape != primitives.shape:
+            raise ValueError(f"buffers must have shape {(5,) + dom.buffer_shape}")
+        face = list(dom.global_number_of_cells)
+        face[axis] += 1
+        flux = torch.empty((5, *face), dtype=primitives.dtype, device=primitives.device)
+        rc = _native.load().jfa_compute_flux_xi(
+            _native.ptr(primitives), _native.ptr(conservatives), dom.grid,
+            self.scheme(_native.dtype_code(primitives)), int(axis), _native.ptr(flux), _native.stream_ptr())
+        _native.check(rc, "jfa_compute_flux_xi")
+        return flux, None, None, None, None
+
+
+# solvers/convective_fluxes/__init__.py:7-14 (the solver built on this path)
+DICT_CONVECTIVE_SOLVER: Dict[str, type] = {"GODUNOV": HighOrderGodunov}

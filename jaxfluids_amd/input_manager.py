@@ -1,0 +1,210 @@
+"""Case / numerical setup readers for the convective-flux path.
+
+Accepts the reference's two setup dicts (or paths to their JSON files) with the
+same keys (input/input_manager.py:24-120, input/numerical_setup/
+read_conservatives.py:20-203, input/case_setup/*) and resolves the registry
+strings to the classes built here.  Initial-condition strings of the form
+"lambda x, ...: expr" are accepted through a whitelisting parser instead of
+Python eval of arbitrary text (the reference evals them,
+input/setup_reader.py:171).
+"""
+from __future__ import annotations
+
+import ast
+import json
+import math
+from typing import Any, Callable, Dict, Union
+
+import numpy as np
+
+from .config import precision
+from .convective_fluxes import DICT_CONVECTIVE_SOLVER
+from .data_types import (ActivePhysicsSetup, CaseSetup, ConservativesSetup, ConvectiveFluxesSetup, DomainSetup,
+                         GeneralSetup, HighOrderGodunovSetup, MaterialSetup, NumericalSetup, PositivitySetup,
+                         PrecisionSetup, TimeIntegrationSetup)
+from .riemann_solvers import DICT_RIEMANN_SOLVER, DICT_SIGNAL_SPEEDS
+from .stencils import DICT_SPATIAL_RECONSTRUCTION
+from .time_integration import DICT_TIME_INTEGRATION
+
+TUPLE_RECONSTRUCTION_VARIABLES = ("PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE", "CHAR-CONSERVATIVE")
+TUPLE_FROZEN_STATE = ("ARITHMETIC", "ROE")
+FACE_LOCATIONS = ("east", "west", "north", "south", "top", "bottom")
+PRIMES = ("rho", "u", "v", "w", "p")
+
+# ------------------------------------------------------------------ safe lambda parsing
+_ALLOWED_FUNCS = {
+    "sin": np.sin, "cos": np.cos, "tan": np.tan, "exp": np.exp, "log": np.log, "sqrt": np.sqrt,
+    "tanh": np.tanh, "sinh": np.sinh, "cosh": np.cosh, "arctan": np.arctan, "arctan2": np.arctan2,
+    "abs": np.abs, "where": np.where, "minimum": np.minimum, "maximum": np.maximum, "heaviside": np.heaviside,
+    "ones_like": np.ones_like, "zeros_like": np.zeros_like, "power": np.power, "sign": np.sign,
+}
+_ALLOWED_CONSTS = {"pi": math.pi, "e": math.e}
+_MODULE_NAMES = ("jnp", "np", "numpy")
+_ALLOWED_NODES = (
+    ast.Expression, ast.Lambda, ast.arguments, ast.arg, ast.BinOp, ast.UnaryOp, ast.Compare, ast.BoolOp,
+    ast.Call, ast.Name, ast.Load, ast.Constant, ast.Attribute, ast.IfExp, ast.Tuple,
+    ast.Add, ast.Sub, ast.Mult, ast.Div, ast.Pow, ast.Mod, ast.FloorDiv, ast.USub, ast.UAdd,
+    ast.Lt, ast.LtE, ast.Gt, ast.GtE, ast.Eq, ast.NotEq, ast.And, ast.Or, ast.BitAnd, ast.BitOr, ast.Not,
+)
+
+
+class _Namespace:
+    def __init__(self):
+        for k, v in {**_ALLOWED_FUNCS, **_ALLOWED_CONSTS}.items():
+            setattr(self, k, v)
+
+
+def parse_lambda(text: str) -> Callable:
+    """Compile "lambda x, y, z: <expr>" after checking every AST node against a whitelist
+    of arithmetic, comparisons and numpy math functions."""
+    tree = ast.parse(text.strip(), mode="eval")
+    if not isinstance(tree.body, ast.Lambda):
+        raise ValueError(f"expected a lambda expression, got {text!r}")
+    params = {a.arg for a in tree.body.args.args}
+    for node in ast.walk(tree):
+        if not isinstance(node, _ALLOWED_NODES):
+            raise ValueError(f"disallowed syntax {type(node).__name__} in {text!r}")
+        if isinstance(node, ast.Attribute):
+            if not (isinstance(node.value, ast.Name) and node.value.id in _MODULE_NAMES
+                    and (node.attr in _ALLOWED_FUNCS or node.attr in _ALLOWED_CONSTS)):
+                raise ValueError(f"disallowed attribute access in {text!r}")
+        if isinstance(node, ast.Name) and isinstance(node.ctx, ast.Load):
+            if node.id not in params and node.id not in _MODULE_NAMES and node.id not in _ALLOWED_FUNCS \
+                    and node.id not in _ALLOWED_CONSTS:
+                raise ValueError(f"unknown name {node.id!r} in {text!r}")
+        if isinstance(node, ast.Constant) and not isinstance(node.value, (int, float, bool)):
+            raise ValueError(f"only numeric constants are allowed in {text!r}")
+    ns = _Namespace()
+    env = {"__builtins__": {}, **{m: ns for m in _MODULE_NAMES}, **_ALLOWED_FUNCS, **_ALLOWED_CONSTS}
+    return eval(compile(tree, "<initial_condition>", "eval"), env)  # noqa: S307 - AST whitelisted above
+
+
+def _as_callable(value: Any) -> Callable:
+    if callable(value):
+        return value
+    if isinstance(value, (int, float)):
+        c = float(value)
+        return lambda *xs: c * np.ones_like(xs[0], dtype=float)
+    if isinstance(value, str):
+        return parse_lambda(value)
+    raise ValueError(f"cannot interpret initial condition {value!r}")
+
+
+def _load(setup: Union[str, Dict]) -> Dict:
+    if isinstance(setup, str):
+        with open(setup) as f:
+            return json.load(f)
+    return dict(setup)
+
+
+# ------------------------------------------------------------------ readers
+def read_numerical_setup(d: Dict) -> NumericalSetup:
+    """read_conservatives.py:20-203 (convective-flux keys)."""
+    assert "conservatives" in d, "numerical_setup: key 'conservatives' is required"
+    c = d["conservatives"]
+    assert "halo_cells" in c, "numerical_setup/conservatives: key 'halo_cells' is required"
+    halo_cells = int(c["halo_cells"])
+    ti = c.get("time_integration")
+    assert ti is not None, "numerical_setup/conservatives: key 'time_integration' is required"
+    integrator = ti.get("integrator")
+    assert integrator in DICT_TIME_INTEGRATION, f"time_integration/integrator must be one of {tuple(DICT_TIME_INTEGRATION)}"
+    cfl = float(ti.get("CFL", 0.5))
+    assert cfl > 0.0, "CFL must be > 0"
+    fixed = ti.get("fixed_timestep", False)
+    if fixed:
+        fixed = float(fixed)
+        assert fixed > 0.0, "fixed_timestep must be > 0"
+    cf = c.get("convective_fluxes", {})
+    solver = cf.get("convective_solver", "GODUNOV")
+    if solver not in DICT_CONVECTIVE_SOLVER:
+        raise NotImplementedError(f"convective_solver {solver!r} is not on this path (built: GODUNOV)")
+    g = cf.get("godunov", {})
+    rs = g.get("riemann_solver", "HLLC")
+    if rs not in DICT_RIEMANN_SOLVER:
+        raise NotImplementedError(f"riemann_solver {rs!r} is not built (built: {tuple(DICT_RIEMANN_SOLVER)})")
+    ss = g.get("signal_speed", "EINFELDT")
+    if ss not in DICT_SIGNAL_SPEEDS:
+        raise NotImplementedError(f"signal_speed {ss!r} is not built (built: {tuple(DICT_SIGNAL_SPEEDS)})")
+    st = g.get("reconstruction_stencil", "WENO5-Z")
+    if st not in DICT_SPATIAL_RECONSTRUCTION:
+        raise NotImplementedError(f"reconstruction_stencil {st!r} is not built (built: {tuple(DICT_SPATIAL_RECONSTRUCTION)})")
+    rv = g.get("reconstruction_variable", "PRIMITIVE")
+    assert rv in TUPLE_RECONSTRUCTION_VARIABLES, f"reconstruction_variable must be one of {TUPLE_RECONSTRUCTION_VARIABLES}"
+    fz = g.get("frozen_state", "ARITHMETIC")
+    assert fz in TUPLE_FROZEN_STATE, f"frozen_state must be one of {TUPLE_FROZEN_STATE}"
+    stencil_cls = DICT_SPATIAL_RECONSTRUCTION[st]
+    assert halo_cells >= stencil_cls.required_halos, \
+        f"halo_cells={halo_cells} < {stencil_cls.required_halos} required by {st}"
+    godunov = HighOrderGodunovSetup(DICT_RIEMANN_SOLVER[rs], DICT_SIGNAL_SPEEDS[ss], stencil_cls, None, rv, fz, None)
+    conv = ConvectiveFluxesSetup(DICT_CONVECTIVE_SOLVER[solver], godunov)
+    pos = c.get("positivity", {})
+    if pos.get("is_interpolation_limiter") or pos.get("flux_limiter"):
+        raise NotImplementedError("positivity limiters are not on this path")
+    cons = ConservativesSetup(halo_cells, TimeIntegrationSetup(DICT_TIME_INTEGRATION[integrator], cfl, fixed), conv)
+    ap = d.get("active_physics", {})
+    active = ActivePhysicsSetup(
+        is_convective_flux=bool(ap.get("is_convective_flux", True)),
+        is_viscous_flux=bool(ap.get("is_viscous_flux", False)),
+        is_heat_flux=bool(ap.get("is_heat_flux", False)),
+        is_volume_force=bool(ap.get("is_volume_force", False)))
+    if active.is_viscous_flux or active.is_heat_flux or active.is_volume_force or not active.is_convective_flux:
+        raise NotImplementedError("only the convective flux is built on this path")
+    pr = d.get("precision", {})
+    prec = PrecisionSetup(bool(pr.get("is_double_precision_compute", True)),
+                          bool(pr.get("is_double_precision_output", True)))
+    return NumericalSetup(cons, active, prec)
+
+
+def read_case_setup(d: Dict) -> CaseSetup:
+    gen = d.get("general", {})
+    general = GeneralSetup(
+        case_name=gen.get("case_name", "case"),
+        end_time=float(gen.get("end_time", float("inf"))),
+        end_step=int(gen.get("end_step", 2 ** 62)),
+        save_path=gen.get("save_path", "./results"),
+        save_dt=gen.get("save_dt"))
+    dom = d["domain"]
+    cells = tuple(int(dom[a]["cells"]) for a in ("x", "y", "z"))
+    ranges = tuple(tuple(float(v) for v in dom[a]["range"]) for a in ("x", "y", "z"))
+    dec = dom.get("decomposition", {})
+    if any(int(dec.get(k, 1)) != 1 for k in ("split_x", "split_y", "split_z")):
+        raise NotImplementedError("domain decomposition across devices is not part of the one-GPU build")
+    bcs = {}
+    for loc in FACE_LOCATIONS:
+        entry = d["boundary_conditions"].get(loc, {"type": "INACTIVE"})
+        bcs[loc] = entry["type"] if isinstance(entry, dict) else entry
+    ic = d["initial_condition"]
+    ic_calls = {k: _as_callable(ic[k]) for k in PRIMES}
+    mp = d.get("material_properties", {}).get("equation_of_state", {})
+    model = mp.get("model", "IdealGas")
+    if model != "IdealGas":
+        raise NotImplementedError(f"equation of state {model!r} is not on this path")
+    material = MaterialSetup(model, float(mp.get("specific_heat_ratio", 1.4)),
+                             float(mp.get("specific_gas_constant", 1.0)))
+    return CaseSetup(general, DomainSetup(cells, ranges), bcs, ic_calls, material)
+
+
+class InputManager:
+    """input/input_manager.py:24-120: reads both setups and builds the physics objects."""
+
+    def __init__(self, case_setup: Union[str, Dict], numerical_setup: Union[str, Dict]) -> None:
+        from .domain import DomainInformation
+        from .equation_manager import EquationInformation, EquationManager, IdealGas, MaterialManager
+
+        self.case_setup_dict = _load(case_setup)
+        self.numerical_setup_dict = _load(numerical_setup)
+        self.numerical_setup = read_numerical_setup(self.numerical_setup_dict)
+        self.case_setup = read_case_setup(self.case_setup_dict)
+        if self.numerical_setup.precision.is_double_precision_compute:
+            precision.enable_double_precision()
+        else:
+            precision.enable_single_precision()
+        cs = self.case_setup
+        self.domain_information = DomainInformation(cs.domain.cells, cs.domain.ranges,
+                                                    self.numerical_setup.conservatives.halo_cells)
+        for loc in self.domain_information.active_face_locations:
+            assert cs.boundary_conditions[loc] != "INACTIVE", f"boundary {loc} of an active axis cannot be INACTIVE"
+        self.material_manager = MaterialManager(IdealGas(cs.material.specific_heat_ratio,
+                                                         cs.material.specific_gas_constant))
+        self.equation_information = EquationInformation(self.domain_information.active_axes_indices)
+        self.equation_manager = EquationManager(self.material_manager, self.equation_information)

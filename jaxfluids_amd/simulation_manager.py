@@ -1,0 +1,182 @@
+"""Step loop of the convective-flux path on one GPU.
+
+Mirrors SimulationManager (simulation_manager.py:49-1412): ``simulate`` /
+``advance`` (:186-425) drive ``do_integration_step`` (:1178-1203) whose
+``_do_integration_step`` (:536-668) runs ``do_runge_kutta_stages`` (:670-1077)
+and then the CFL time step (:612-628).
+
+Two stage implementations share the same arithmetic:
+* ``fused=True`` (default): one ``jfa_rk3_stage_fused`` call per stage -- face
+  fluxes of all axes, divergence, RK3 blend/update, prim(U) and face halos --
+  on three ping-pong buffer pairs, the state ending back in the caller's buffers;
+* ``fused=False``: the reference's call order through the plugin objects
+  (SpaceSolver.compute_rhs -> RungeKutta3.perform_stage_integration ->
+  HaloManager.perform_halo_update_material).
+The time step stays on the device (float64 scalar); ``advance`` reads it back
+once per step to test ``t < end_time`` as the reference does (:325-328).
+Optionally one whole step is captured in a HIP graph and replayed.
+"""
+from __future__ import annotations
+
+import time
+from typing import Optional
+
+import torch
+
+from . import _native
+from .config import precision
+from .data_types import (EulerIntegrationBuffers, IntegrationBuffers, JaxFluidsBuffers, MaterialFieldBuffers,
+                         TimeControlVariables)
+from .halos import HaloManager
+from .space_solver import SpaceSolver
+from .time_integration import compute_time_step_size
+
+
+class SimulationManager:
+    def __init__(self, input_manager, fused: bool = True, use_graph: bool = False) -> None:
+        im = input_manager
+        self.input_manager = im
+        self.numerical_setup = im.numerical_setup
+        self.case_setup = im.case_setup
+        self.domain_information = im.domain_information
+        self.material_manager = im.material_manager
+        self.equation_manager = im.equation_manager
+        self.equation_information = im.equation_information
+        self.halo_manager = HaloManager(im.case_setup.boundary_conditions, im.domain_information,
+                                        im.equation_manager, im.material_manager)
+        cons = self.numerical_setup.conservatives
+        dom = self.domain_information
+        self.time_integrator = cons.time_integration.integrator(nh=dom.nh_conservatives,
+                                                                inactive_axes=dom.inactive_axes)
+        cf = cons.convective_fluxes
+        self.convective_flux_solver = cf.convective_solver(
+            convective_fluxes_setup=cf, positivity_setup=cons.positivity, diffuse_interface_setup=None,
+            material_manager=self.material_manager, domain_information=dom, equation_manager=self.equation_manager,
+            diffuse_interface_handler=None, positivity_handler=None)
+        self.space_solver = SpaceSolver(dom, self.material_manager, self.equation_manager,
+                                        self.convective_flux_solver, self.numerical_setup)
+        self.fused = fused
+        self.use_graph = use_graph
+        self._bufs = None
+        self._graph = None
+        self._graph_key = None
+        self.wall_clock_times = []
+
+    # ------------------------------------------------------------------ buffers
+    def _ensure_buffers(self, W: torch.Tensor):
+        key = (W.dtype, W.device, tuple(W.shape))
+        if self._bufs is not None and self._bufs["key"] == key:
+            return self._bufs
+        lib = _native.load()
+        scheme = self.convective_flux_solver.scheme(_native.dtype_code(W))
+        ws_bytes = lib.jfa_stage_workspace_bytes(self.domain_information.grid, scheme)
+        self._bufs = {
+            "key": key,
+            "B": (torch.empty_like(W), torch.empty_like(W)),
+            "C": (torch.empty_like(W), torch.empty_like(W)),
+            "ws": torch.empty(ws_bytes, dtype=torch.uint8, device=W.device),
+            "dt_ws": torch.empty(16, dtype=torch.uint8, device=W.device),
+        }
+        self._graph = None
+        return self._bufs
+
+    # ------------------------------------------------------------------ stages
+    def _fused_stage(self, W_in, U_in, U_n, W_out, U_out, stage: int, dt: torch.Tensor) -> None:
+        b = self._bufs
+        scheme = self.convective_flux_solver.scheme(_native.dtype_code(W_in))
+        rc = _native.load().jfa_rk3_stage_fused(
+            _native.ptr(W_in), _native.ptr(U_in), _native.ptr(U_n), _native.ptr(W_out), _native.ptr(U_out),
+            self.domain_information.grid, scheme, self.halo_manager.bc_array, int(stage), _native.ptr(dt), 0.0,
+            _native.ptr(b["ws"]), b["ws"].numel(), _native.stream_ptr())
+        _native.check(rc, "jfa_rk3_stage_fused")
+
+    def do_runge_kutta_stages(self, primitives: torch.Tensor, conservatives: torch.Tensor, dt: torch.Tensor):
+        """Three RK3 stages with a device time step.  The fused path updates the given
+        buffers in place; the reference-order path returns new tensors."""
+        if self.fused:
+            b = self._ensure_buffers(primitives)
+            (Wb, Ub), (Wc, Uc) = b["B"], b["C"]
+            self._fused_stage(primitives, conservatives, None, Wb, Ub, 0, dt)
+            self._fused_stage(Wb, Ub, conservatives, Wc, Uc, 1, dt)
+            # U_out may alias U_n: each cell reads U_n[c] before writing U_out[c], and no cell
+            # reads U_n in the halos.
+            self._fused_stage(Wc, Uc, conservatives, primitives, conservatives, 2, dt)
+            return primitives, conservatives
+        W, U = primitives, conservatives
+        init = IntegrationBuffers(EulerIntegrationBuffers(U))
+        integ = init
+        for stage in range(self.time_integrator.no_stages):
+            rhs, _, _ = self.space_solver.compute_rhs(U, W)
+            W_new = torch.empty_like(W)
+            W_new.copy_(W)
+            integ = self.time_integrator.perform_stage_integration(
+                integ, rhs, init, dt, stage, self.equation_information, self.domain_information,
+                self.material_manager.get_gamma(), primitives_out=W_new)
+            U = integ.euler_buffers.conservatives
+            W, U = self.halo_manager.perform_halo_update_material(W_new, 0.0, False, False, U, inplace=True)
+            integ = IntegrationBuffers(EulerIntegrationBuffers(U))
+        return W, U
+
+    def compute_time_step(self, primitives: torch.Tensor, out: torch.Tensor = None) -> torch.Tensor:
+        b = self._ensure_buffers(primitives) if self.fused else None
+        return compute_time_step_size(primitives, None, None, None, None, self.domain_information,
+                                      self.equation_information, self.material_manager, None,
+                                      self.numerical_setup, out=out,
+                                      workspace=None if b is None else b["dt_ws"])
+
+    def _step_kernels(self, W, U, dt):
+        W, U = self.do_runge_kutta_stages(W, U, dt)
+        self.compute_time_step(W, out=dt)
+        return W, U
+
+    def _capture(self, W, U, dt):
+        self._ensure_buffers(W)
+        key = (W.data_ptr(), U.data_ptr(), dt.data_ptr())
+        if self._graph is not None and self._graph_key == key:
+            return self._graph
+        # warm up on a side stream (required before capture), then capture one step
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            self._step_kernels(W, U, dt)
+        torch.cuda.current_stream().wait_stream(s)
+        self._graph, self._graph_key = g, key
+        return g
+
+    # ------------------------------------------------------------------ steps
+    def do_integration_step(self, jxf_buffers: JaxFluidsBuffers) -> JaxFluidsBuffers:
+        """One step with the current dt, then the next dt (simulation_manager.py:536-668)."""
+        mf, tcv = jxf_buffers.material_fields, jxf_buffers.time_control_variables
+        W, U = mf.primitives, mf.conservatives
+        dt = tcv.physical_timestep_size
+        if not isinstance(dt, torch.Tensor):
+            dt = torch.full((1,), float(dt), dtype=torch.float64, device=W.device)
+        dt_used = float(dt.item())
+        if self.use_graph and self.fused:
+            # the replayed step reads and writes dt in place
+            self._capture(W, U, dt).replay()
+        else:
+            W, U = self.do_runge_kutta_stages(W, U, dt)
+            if tcv.fixed_timestep_size:
+                dt.fill_(float(tcv.fixed_timestep_size))
+            else:
+                self.compute_time_step(W, out=dt)
+        tcv = tcv._replace(physical_simulation_time=tcv.physical_simulation_time + dt_used,
+                           simulation_step=tcv.simulation_step + 1, physical_timestep_size=dt)
+        return JaxFluidsBuffers(MaterialFieldBuffers(U, W, mf.temperature), tcv)
+
+    def advance(self, jxf_buffers: JaxFluidsBuffers) -> JaxFluidsBuffers:
+        """Host loop: while t < end_time and step < end_step (simulation_manager.py:325-328)."""
+        tcv = jxf_buffers.time_control_variables
+        self.wall_clock_times = []
+        while tcv.physical_simulation_time < tcv.end_time and tcv.simulation_step < tcv.end_step:
+            t0 = time.perf_counter()
+            jxf_buffers = self.do_integration_step(jxf_buffers)
+            tcv = jxf_buffers.time_control_variables
+            self.wall_clock_times.append(time.perf_counter() - t0)
+        torch.cuda.synchronize()
+        return jxf_buffers
+
+    def simulate(self, jxf_buffers: JaxFluidsBuffers) -> JaxFluidsBuffers:
+        return self.advance(jxf_buffers)
