@@ -1,0 +1,200 @@
+"""Benchmark: inviscid Taylor-Green vortex 256^3, FP64, WENO5-Z (characteristic/primitive
+variables) + HLLC (Einfeldt speeds) + TVD-RK3, CFL 0.5, periodic, halo_cells 5 -- the
+reference's own TGV numerical setup (examples/examples_3D/01_tgv/numerical_setup.json)
+on the 256^3 grid.
+
+One "step" = one RK3 step (3 stages + the CFL time step) over the whole grid, with the
+state resident in HBM.  Prints ONE JSON line.
+
+  python bench.py [--gpus 1] [--steps 100] [--warmup 10]
+
+value = Mcell-updates/s per RK3 stage = N_cells * 3 * steps / wall_s / 1e6.
+roofline: algorithmic bytes of the stage kernel (160 B/cell for stage 1, 200 B/cell for
+stages 2-3: read W,U(,U_n), write W,U; FP64) divided by its average launch duration,
+measured with HIP events on the launch stream over the timed region; peak 8 TB/s.
+cpu_baseline: the C/OpenMP restatement (oracle/c) of the same step on the host cores,
+timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-updates/s per RK3 stage at 256^3 FP64; %HBM peak; adjoint/fwd ratio"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell and stage (W and U resident, 5 fields each)
+STAGE_BYTES_PER_CELL = {8: (160.0, 200.0, 200.0), 4: (80.0, 100.0, 100.0)}
+
+
+def tgv_setups(n: int, nh: int, dtype: str, variable: str):
+    L = 2 * np.pi
+    case = {
+        "general": {"case_name": f"tgv{n}", "end_time": 1e9},
+        "domain": {a: {"cells": n, "range": [0.0, L]} for a in ("x", "y", "z")},
+        "boundary_conditions": {f: {"type": "PERIODIC"} for f in ("east", "west", "north", "south", "top", "bottom")},
+        "initial_condition": {"rho": 1.0, "u": 0.0, "v": 0.0, "w": 0.0, "p": 1.0},  # replaced below
+        "material_properties": {"equation_of_state": {"model": "IdealGas", "specific_heat_ratio": 1.4,
+                                                      "specific_gas_constant": 1.0}},
+    }
+    num = {
+        "conservatives": {
+            "halo_cells": nh,
+            "time_integration": {"integrator": "RK3", "CFL": 0.5},
+            "convective_fluxes": {"convective_solver": "GODUNOV", "godunov": {
+                "riemann_solver": "HLLC", "signal_speed": "EINFELDT", "reconstruction_stencil": "WENO5-Z",
+                "reconstruction_variable": variable}},
+        },
+        "active_physics": {"is_convective_flux": True},
+        "precision": {"is_double_precision_compute": dtype == "f64"},
+    }
+    return case, num
+
+
+def tgv_primitives(n: int, gamma=1.4, mach=0.1):
+    """Inviscid TGV on [0,2pi)^3: rho=1, u=sin x cos y cos z, v=-cos x sin y cos z, w=0,
+    p = 1/(gamma Ma^2) + (1/16)(cos2x+cos2y)(cos2z+2)."""
+    L = 2 * np.pi
+    d = L / n
+    x = np.linspace(d / 2, L - d / 2, n)
+    sx, cx = np.sin(x), np.cos(x)
+    c2 = np.cos(2 * x)
+    W = np.empty((5, n, n, n))
+    W[0] = 1.0
+    W[1] = sx[:, None, None] * cx[None, :, None] * cx[None, None, :]
+    W[2] = -cx[:, None, None] * sx[None, :, None] * cx[None, None, :]
+    W[3] = 0.0
+    W[4] = 1.0 / gamma / mach ** 2 + 1.0 / 16.0 * ((c2[:, None, None] + c2[None, :, None]) * (c2[None, None, :] + 2))
+    return W
+
+
+def cpu_baseline(n: int, nh: int, variable: str, seconds: float):
+    """Time the C/OpenMP restatement on a bounded number of steps of the same workload."""
+    from oracle import c_port
+    from oracle import jxf_oracle as o
+
+    dom = o.Domain((n, n, n), ((0, 2 * np.pi),) * 3, nh=nh)
+    s = o.Setup(gamma=1.4, stencil="WENO5-Z", variable=variable, riemann="HLLC", signal_speed="EINFELDT")
+    prim = tgv_primitives(n)
+    W = np.full((5,) + dom.shape, o.EPS64)
+    W[dom.interior] = prim
+    del prim
+    U = o.cons_from_prim(W, s.gamma)
+    st = c_port.CPUStepper(dom, s)
+    dt = st.dt(W, o.EPS64)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        st.step(W, U, dt)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 50:
+            break
+    return {"value": round(n ** 3 * 3 * steps / el / 1e6, 3), "unit": "Mcell-updates/s",
+            "cores": int(c_port.max_threads()), "kind": "port",
+            "sample": f"{steps} RK3 step(s) of the same TGV {n}^3 workload, C/OpenMP restatement "
+                      f"(oracle/c/jxf_cpu.c, gcc -O3 -march=x86-64-v3), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--nh", type=int, default=5)
+    ap.add_argument("--dtype", choices=("f64", "f32"), default="f64")
+    ap.add_argument("--variable", default="CHAR-PRIMITIVE")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU domain decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from jaxfluids_amd import InitializationManager, InputManager, SimulationManager
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    case, num = tgv_setups(args.n, args.nh, args.dtype, args.variable)
+    im = InputManager(case, num)
+    bufs = InitializationManager(im).initialization(tgv_primitives(args.n))
+    sim = SimulationManager(im)
+    W, U = bufs.material_fields.primitives, bufs.material_fields.conservatives
+    dt = bufs.time_control_variables.physical_timestep_size
+    sim._ensure_buffers(W)
+    (Wb, Ub), (Wc, Uc) = sim._bufs["B"], sim._bufs["C"]
+    stream = torch.cuda.current_stream()
+
+    def step(events=None):
+        seq = ((W, U, None, Wb, Ub), (Wb, Ub, U, Wc, Uc), (Wc, Uc, U, W, U))
+        for k, (wi, ui, un, wo, uo) in enumerate(seq):
+            if events is not None:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            sim._fused_stage(wi, ui, un, wo, uo, k, dt)
+            if events is not None:
+                e1.record(stream)
+                events.append((k, e0, e1))
+        sim.compute_time_step(W, out=dt)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(events)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    ncell = args.n ** 3
+    esize = 8 if args.dtype == "f64" else 4
+    byt = STAGE_BYTES_PER_CELL[esize]
+    tot_ms = sum(e0.elapsed_time(e1) for _, e0, e1 in events)
+    tot_bytes = sum(byt[k] * ncell for k, _, _ in events)
+    achieved = tot_bytes / (tot_ms * 1e-3) / 1e9
+    value = ncell * 3 * args.steps / wall / 1e6
+    W_h = W[(slice(None),) + im.domain_information.domain_slices_conservatives]
+    finite = bool(torch.isfinite(W_h).all().item())
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {"workload": f"inviscid Taylor-Green vortex {args.n}^3 periodic, WENO5-Z {args.variable} + "
+                               f"HLLC-Einfeldt + RK3, CFL 0.5, halo_cells {args.nh}",
+                   "cells": ncell, "halo_cells": args.nh, "storage": args.dtype, "finite": finite},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "RK3 stage (jfa_rk3_stage_fused)", "avg_stage_ms": round(tot_ms / len(events), 4),
+                     "alg_bytes_per_cell_stage": round(sum(byt) / 3, 2)},
+    }
+    if not args.no_cpu_baseline:
+        del sim, bufs, W, U, Wb, Ub, Wc, Uc
+        torch.cuda.empty_cache()
+        out["cpu_baseline"] = cpu_baseline(args.n, args.nh, args.variable if args.variable != "CHAR-CONSERVATIVE"
+                                           else "CHAR-PRIMITIVE", args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

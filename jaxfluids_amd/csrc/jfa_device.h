@@ -1,11 +1,16 @@
 // jfa_device.h -- device-side arithmetic of the convective-flux path (gfx950).
 //
 // Each function restates the JAX-Fluids expression it cites (src/jaxfluids/...),
-// evaluated per face in registers.  Two algebraic rearrangements keep the values
-// equal up to rounding but cut FP64 divisions (the path is FP64-VALU bound):
-//  * WENO weights are normalised with a single division: omega_k = a_k / sum(a)
-//    with a_k multiplied through by the product of the other two denominators;
-//  * 1/rho, 1/c are formed once and reused.
+// evaluated per face in registers.  The FP64 path is VALU bound, so the algebra
+// is rearranged to cut divisions and square roots while keeping every value equal
+// to the reference's up to rounding:
+//  * reciprocals and inverse square roots use the hardware estimates
+//    (v_rcp_f64 / v_rsq_f64) refined by two Newton-Raphson steps (relative error
+//    ~1 ulp for the normal, positive operands of this path);
+//  * WENO weights are normalised with one reciprocal: omega_k = a_k / sum(a), with
+//    a_k multiplied through by the product of the other two denominators;
+//  * E = rho|u|^2/2 + p/(gamma-1) (no division), the characteristic projection
+//    uses c^2 rho = gamma p and a single rsq, the HLLC star state one reciprocal.
 // Velocity components are addressed through compile-time axis tables so the
 // three sweep directions compile to straight-line register code.
 #pragma once
@@ -27,10 +32,43 @@ struct Ax {
 
 struct Params {
     double gamma;
+    double gm1;        // gamma - 1
+    double inv_gm1;    // 1 / (gamma - 1)
     double eps;        // WENO epsilon (config/precision.py:39,53)
     int signal_speed;  // JFA_SS_*
     int frozen;        // JFA_FROZEN_*
 };
+
+// ---------------------------------------------------------------- refined hardware estimates
+// Newton-Raphson for 1/x: r <- r + r(1 - x r), twice from the v_rcp_f64 estimate.
+__device__ __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+// a / b with a final residual correction q <- q + r(a - b q).
+__device__ __forceinline__ double fdiv(double a, double b) {
+    const double r = rcp(b);
+    const double q = a * r;
+    return fma(r, fma(-b, q, a), q);
+}
+// Newton-Raphson for 1/sqrt(x): y <- y + y(1/2 - (x/2) y^2), twice from v_rsq_f64.
+__device__ __forceinline__ double rsq(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double t = fma(-h * y, y, 0.5);
+    y = fma(y, t, y);
+    t = fma(-h * y, y, 0.5);
+    return fma(y, t, y);
+}
+// sqrt(x) = x rsq(x) with one residual correction g <- g + (y/2)(x - g^2).
+__device__ __forceinline__ double fsqrt(double x) {
+    const double y = rsq(x);
+    const double g = x * y;
+    return fma(0.5 * y, fma(-g, g, x), g);
+}
 
 // ---------------------------------------------------------------- WENO5
 // weno5_base.py:34-50 smoothness / candidate polynomials; weno5_js.py:39-58; weno5_z.py:152-166.
@@ -65,39 +103,41 @@ __device__ __forceinline__ double weno5(double a, double b, double c, double d, 
         const double q0 = beta0 + eps;
         const double q1 = beta1 + eps;
         const double q2 = beta2 + eps;
-        a0 = 0.1 * (q0 + tau5) * (q1 * q2);
-        a1 = 0.6 * (q1 + tau5) * (q0 * q2);
-        a2 = 0.3 * (q2 + tau5) * (q0 * q1);
+        a0 = (0.1 * (q0 + tau5)) * (q1 * q2);
+        a1 = (0.6 * (q1 + tau5)) * (q0 * q2);
+        a2 = (0.3 * (q2 + tau5)) * (q0 * q1);
     }
-    return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+    return (a0 * p0 + a1 * p1 + a2 * p2) * rcp(a0 + a1 + a2);
 }
 
 // ---------------------------------------------------------------- ideal gas
 // materials/single_materials/ideal_gas.py:69-88, equation_manager.py:93-101,164-171
+// c = sqrt(gamma p / rho) = gamma p / sqrt(gamma p rho)
 __device__ __forceinline__ double sound_speed(double p, double rho, double gamma) {
-    return sqrt(gamma * p / rho);
+    const double gp = gamma * p;
+    return gp * rsq(gp * rho);
 }
 
-__device__ __forceinline__ void cons_from_prim(const double W[5], double gamma, double U[5]) {
+// E = rho (|u|^2/2 + p/(rho (gamma-1))) = rho |u|^2 / 2 + p / (gamma - 1)
+__device__ __forceinline__ void cons_from_prim(const double W[5], const Params& prm, double U[5]) {
     const double rho = W[0];
-    const double e = W[4] / (rho * (gamma - 1.0));
     U[0] = rho;
     U[1] = rho * W[1];
     U[2] = rho * W[2];
     U[3] = rho * W[3];
-    U[4] = rho * (0.5 * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]) + e);
+    U[4] = (0.5 * rho) * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]) + W[4] * prm.inv_gm1;
 }
 
-__device__ __forceinline__ void prim_from_cons(const double U[5], double gamma, double W[5]) {
+__device__ __forceinline__ void prim_from_cons(const double U[5], const Params& prm, double W[5]) {
     const double rho = U[0];
-    const double one_rho = 1.0 / rho;
+    const double one_rho = rcp(rho);
     const double u = U[1] * one_rho, v = U[2] * one_rho, w = U[3] * one_rho;
     const double e = U[4] * one_rho - 0.5 * (u * u + v * v + w * w);
     W[0] = rho;
     W[1] = u;
     W[2] = v;
     W[3] = w;
-    W[4] = (gamma - 1.0) * e * rho;
+    W[4] = prm.gm1 * e * rho;
 }
 
 // equation_manager.py:237-252
@@ -114,50 +154,57 @@ __device__ __forceinline__ void physical_flux(const double W[5], const double U[
 }
 
 // ---------------------------------------------------------------- frozen state
-// eigendecomposition.py:139-281 (single phase).  For CHAR-PRIMITIVE only rho, c, c^2 are used.
+// eigendecomposition.py:139-281 (single phase).
 struct Frozen {
-    double P[5];  // primes_ave
-    double H, g, c, cc, q2;  // enthalpy, grueneisen, c, c^2, velocity_square
+    double P[5];            // primes_ave
+    double H, g, c, cc, q2; // enthalpy, grueneisen, c, c^2, velocity_square
+    double inv_c, rcc;      // 1/c and c^2 * rho_ave
 };
 
-__device__ __forceinline__ double total_enthalpy(const double W[5], double gamma) {
-    // ideal_gas.py:90-110
-    const double E = W[4] / (gamma - 1.0) + 0.5 * W[0] * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
-    return (E + W[4]) / W[0];
+__device__ __forceinline__ double total_enthalpy(const double W[5], const Params& prm) {
+    // ideal_gas.py:90-110: (p/(gamma-1) + rho|u|^2/2 + p) / rho
+    const double E = W[4] * prm.inv_gm1 + 0.5 * W[0] * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+    return fdiv(E + W[4], W[0]);
 }
 
 template <bool FULL>
 __device__ __forceinline__ void frozen_state(const double WL[5], const double WR[5], const Params& prm,
                                              Frozen& f) {
-    const double gamma = prm.gamma;
     if (prm.frozen == JFA_FROZEN_ARITHMETIC) {
 #pragma unroll
         for (int i = 0; i < 5; ++i) f.P[i] = 0.5 * (WL[i] + WR[i]);  // :148
-        f.c = sound_speed(f.P[4], f.P[0], gamma);                      // :223-225
-        f.cc = f.c * f.c;                                              // :230
+        // c = sqrt(gamma p / rho) (:223-225); c^2 rho = gamma p; 1/c = rho / sqrt(gamma p rho)
+        const double gp = prm.gamma * f.P[4];
+        const double r = rsq(gp * f.P[0]);
+        f.c = gp * r;
+        f.inv_c = f.P[0] * r;
+        f.cc = f.c * f.c;       // :230
+        f.rcc = gp;
         if (FULL) {
-            f.g = gamma - 1.0;                                         // :216-218
-            f.H = total_enthalpy(f.P, gamma);                          // :219-222
+            f.g = prm.gm1;                                             // :216-218
+            f.H = total_enthalpy(f.P, prm);                            // :219-222
             f.q2 = f.P[1] * f.P[1] + f.P[2] * f.P[2] + f.P[3] * f.P[3];  // :231
         }
     } else {  // ROE :233-275
-        const double sL = sqrt(WL[0]), sR = sqrt(WR[0]);
-        const double rho_div = 1.0 / (sL + sR);
+        const double sL = fsqrt(WL[0]), sR = fsqrt(WR[0]);
+        const double rho_div = rcp(sL + sR);
 #pragma unroll
         for (int i = 1; i < 5; ++i) f.P[i] = (sL * WL[i] + sR * WR[i]) * rho_div;  // :293
-        f.P[0] = sqrt(WL[0] * WR[0]);                                              // :237
-        const double psi_ave = (sL * (WL[4] / WL[0]) + sR * (WR[4] / WR[0])) * rho_div;  // :258
-        const double g = gamma - 1.0;
+        f.P[0] = fsqrt(WL[0] * WR[0]);                                             // :237
+        const double pr_L = fdiv(WL[4], WL[0]), pr_R = fdiv(WR[4], WR[0]);         // psi = p/rho
+        const double psi_ave = (sL * pr_L + sR * pr_R) * rho_div;                  // :258
+        const double g = prm.gm1;
         f.g = (sL * g + sR * g) * rho_div;                                         // :260-262
         const double du = WR[1] - WL[1], dv = WR[2] - WL[2], dw = WR[3] - WL[3];
         const double sq_dvel = du * du + dv * dv + dw * dw;                        // :265
-        const double p_over_rho = (sL * WL[4] / WL[0] + sR * WR[4] / WR[0]) * rho_div +
-                                  0.5 * f.P[0] * rho_div * rho_div * sq_dvel;      // :267-268
+        const double p_over_rho = psi_ave + 0.5 * f.P[0] * rho_div * rho_div * sq_dvel;  // :267-268
         f.cc = psi_ave + f.g * p_over_rho;                                         // :274
-        f.c = sqrt(f.cc);                                                          // :275
+        f.inv_c = rsq(f.cc);
+        f.c = f.cc * f.inv_c;                                                      // :275
+        f.rcc = f.cc * f.P[0];
         if (FULL) {
-            f.H = (sL * total_enthalpy(WL, gamma) + sR * total_enthalpy(WR, gamma)) * rho_div;  // :250
-            f.q2 = f.P[1] * f.P[1] + f.P[2] * f.P[2] + f.P[3] * f.P[3];                        // :271
+            f.H = (sL * total_enthalpy(WL, prm) + sR * total_enthalpy(WR, prm)) * rho_div;  // :250
+            f.q2 = f.P[1] * f.P[1] + f.P[2] * f.P[2] + f.P[3] * f.P[3];                    // :271
         }
     }
 }
@@ -187,18 +234,19 @@ __device__ __forceinline__ void signal_speeds(int kind, double uL, double uR, do
             const double p_pvrs = 0.5 * (pL + pR) - 0.5 * (uR - uL) * rho_bar * a_bar;
             const double ps = fmax(0.0, p_pvrs);
             const double g_ = (gamma + 1.0) * 0.5 / gamma;
-            const double qL = ps <= pL ? 1.0 : sqrt(1.0 + g_ * (ps / pL - 1.0));
-            const double qR = ps <= pR ? 1.0 : sqrt(1.0 + g_ * (ps / pR - 1.0));
+            const double qL = ps <= pL ? 1.0 : fsqrt(1.0 + g_ * (fdiv(ps, pL) - 1.0));
+            const double qR = ps <= pR ? 1.0 : fsqrt(1.0 + g_ * (fdiv(ps, pR) - 1.0));
             SL = uL - aL * qL;
             SR = uR + aR * qR;
         } break;
         default: {  // EINFELDT :125-132
-            const double sL = sqrt(rL), sR = sqrt(rR);
-            const double one_dens = 1.0 / (sL + sR);
+            const double yL = rsq(rL), yR = rsq(rR);
+            const double sL = rL * yL, sR = rR * yR;  // sqrt(rho)
+            const double one_dens = rcp(sL + sR);
             const double eta2 = 0.5 * sL * sR * one_dens * one_dens;
             const double u_bar = (sL * uL + sR * uR) * one_dens;
             const double du = uR - uL;
-            const double d_bar = sqrt((sL * aL * aL + sR * aR * aR) * one_dens + eta2 * (du * du));
+            const double d_bar = fsqrt((sL * aL * aL + sR * aR * aR) * one_dens + eta2 * (du * du));
             SL = fmin(u_bar - d_bar, uL - aL);
             SR = fmax(u_bar + d_bar, uR + aR);
         } break;
@@ -206,24 +254,29 @@ __device__ __forceinline__ void signal_speeds(int kind, double uL, double uR, do
 }
 
 // ---------------------------------------------------------------- Riemann solvers
-// HLLC._compute_flux_star_K_single_phase (HLLC.py:41-78) for one side K with wave speed S
-// (already clipped by min(.,0) for L or max(.,0) for R).
+// HLLC._compute_flux_star_K_single_phase (HLLC.py:41-78) for one side K with wave speed SK,
+// its clipped value S_clip (min(.,0) for L, max(.,0) for R) and contact speed Ss.
+// With d = SK - u and inv = 1/(SK - Ss):
+//   pre = rho d inv                                  (:52)
+//   U*_4 = pre (E/rho + (Ss - u)(Ss + p/(rho d)))   (:59)  = inv (d E + (Ss - u)(rho d Ss + p))
 template <int AXIS>
 __device__ __forceinline__ void hllc_star_flux(const double W[5], const double U[5], double SK, double S_clip,
                                                double Ss, double F[5]) {
     constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
     const double uK = W[v];
-    const double SmU = SK - uK;
-    const double pre = SmU / (SK - Ss) * W[0];                                            // :52
+    const double d = SK - uK;
+    const double inv = rcp(SK - Ss);
+    const double rd = W[0] * d;
+    const double pre = rd * inv;
     double us[5];
     us[0] = pre;
-    us[v] = pre * Ss;                                                                     // :61
-    us[m0] = pre * W[m0];                                                                 // :62
-    us[m1] = pre * W[m1];                                                                 // :63
-    us[4] = pre * (U[4] / U[0] + (Ss - uK) * (Ss + W[4] / W[0] / SmU));                   // :59
-    physical_flux<AXIS>(W, U, F);                                                         // :66
+    us[v] = pre * Ss;                                                   // :61
+    us[m0] = pre * W[m0];                                               // :62
+    us[m1] = pre * W[m1];                                               // :63
+    us[4] = inv * (d * U[4] + (Ss - uK) * (rd * Ss + W[4]));
+    physical_flux<AXIS>(W, U, F);                                       // :66
 #pragma unroll
-    for (int i = 0; i < 5; ++i) F[i] = F[i] + S_clip * (us[i] - U[i]);                   // :76
+    for (int i = 0; i < 5; ++i) F[i] = F[i] + S_clip * (us[i] - U[i]);  // :76
 }
 
 // HLLC._solve_riemann_problem_xi_single_phase (HLLC.py:80-126).  The blend
@@ -240,7 +293,7 @@ __device__ __forceinline__ void hllc(const double WL[5], const double WR[5], con
     // compute_sstar (signal_speeds.py:194-197)
     const double rdL = WL[0] * (SL - WL[v]);
     const double rdR = WR[0] * (SR - WR[v]);
-    const double Ss = ((WR[4] - WL[4]) + (WL[v] * rdL - WR[v] * rdR)) / (rdL - rdR);
+    const double Ss = fdiv((WR[4] - WL[4]) + (WL[v] * rdL - WR[v] * rdR), rdL - rdR);
     const bool left = Ss >= 0.0;
     double Wk[5], Uk[5];
 #pragma unroll
@@ -305,29 +358,31 @@ __device__ __forceinline__ void face_states(double win[6][5], const double wl[5]
             QR[i] = weno5<STENCIL>(win[5][i], win[4][i], win[3][i], win[2][i], win[1][i], eps);
         }
         if constexpr (VAR == JFA_PRIMITIVE) {
-            cons_from_prim(WL, prm.gamma, UL);  // :279-280
-            cons_from_prim(WR, prm.gamma, UR);
+            cons_from_prim(WL, prm, UL);  // :279-280
+            cons_from_prim(WR, prm, UR);
         } else {
-            prim_from_cons(UL, prm.gamma, WL);  // :295-296
-            prim_from_cons(UR, prm.gamma, WR);
+            prim_from_cons(UL, prm, WL);  // :295-296
+            prim_from_cons(UR, prm, WR);
         }
     } else if constexpr (VAR == JFA_CHAR_PRIMITIVE) {
         // frozen state from the cells left/right of the face (eigendecomposition.py:139-140)
         Frozen f;
         frozen_state<false>(win[2], win[3], prm, f);
-        // get_characteristics_from_primitives (eigendecomposition.py:425-431)
-        const double k1 = 0.5 / f.c;
-        const double k2 = 0.5 / (f.cc * f.P[0]);
-        const double k3 = 1.0 / f.cc;
+        // get_characteristics_from_primitives (eigendecomposition.py:425-431):
+        // k1 = 0.5/c, k2 = 0.5/(c^2 rho), k3 = 1/c^2
+        const double k1 = 0.5 * f.inv_c;
+        const double k3 = f.inv_c * f.inv_c;
+        const double k2 = 0.5 * rcp(f.rcc);
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             const double r = win[k][0], un = win[k][v], p = win[k][4];
             const double t0 = win[k][m0], t1 = win[k][m1];
-            win[k][0] = -k1 * un + k2 * p;
+            const double a = k2 * p, b = k1 * un;
+            win[k][0] = a - b;
             win[k][1] = r - k3 * p;
             win[k][2] = t0;
             win[k][3] = t1;
-            win[k][4] = k1 * un + k2 * p;
+            win[k][4] = a + b;
         }
         double cLv[5], cRv[5];
 #pragma unroll
@@ -336,25 +391,25 @@ __device__ __forceinline__ void face_states(double win[6][5], const double wl[5]
             cRv[i] = weno5<STENCIL>(win[5][i], win[4][i], win[3][i], win[2][i], win[1][i], eps);
         }
         // get_primitives_from_characteristics (eigendecomposition.py:517-521)
-        const double rc2 = f.cc * f.P[0];
-        WL[0] = f.P[0] * (cLv[0] + cLv[4]) + cLv[1];
-        WL[v] = f.c * (-cLv[0] + cLv[4]);
+        const double sL = cLv[0] + cLv[4], sR = cRv[0] + cRv[4];
+        WL[0] = f.P[0] * sL + cLv[1];
+        WL[v] = f.c * (cLv[4] - cLv[0]);
         WL[m0] = cLv[2];
         WL[m1] = cLv[3];
-        WL[4] = rc2 * (cLv[0] + cLv[4]);
-        WR[0] = f.P[0] * (cRv[0] + cRv[4]) + cRv[1];
-        WR[v] = f.c * (-cRv[0] + cRv[4]);
+        WL[4] = f.rcc * sL;
+        WR[0] = f.P[0] * sR + cRv[1];
+        WR[v] = f.c * (cRv[4] - cRv[0]);
         WR[m0] = cRv[2];
         WR[m1] = cRv[3];
-        WR[4] = rc2 * (cRv[0] + cRv[4]);
-        cons_from_prim(WL, prm.gamma, UL);  // :401-402
-        cons_from_prim(WR, prm.gamma, UR);
+        WR[4] = f.rcc * sR;
+        cons_from_prim(WL, prm, UL);  // :401-402
+        cons_from_prim(WR, prm, UR);
     } else {  // CHAR-CONSERVATIVE (:404-417)
         Frozen f;
         frozen_state<true>(wl, wr, prm, f);
         // eigendecomposition_conservatives (eigendecomposition.py:595-658)
-        const double one_cc = 1.0 / f.cc;
-        const double one_rho = 1.0 / f.P[0];
+        const double one_cc = f.inv_c * f.inv_c;
+        const double one_rho = rcp(f.P[0]);
         const double* P = f.P;
         const double g = f.g, H = f.H, c = f.c, q2 = f.q2;
         double L[5][5], R[5][5];
@@ -435,8 +490,8 @@ __device__ __forceinline__ void face_states(double win[6][5], const double wl[5]
             UL[a] = accL;
             UR[a] = accR;
         }
-        prim_from_cons(UL, prm.gamma, WL);  // :416-417
-        prim_from_cons(UR, prm.gamma, WR);
+        prim_from_cons(UL, prm, WL);  // :416-417
+        prim_from_cons(UR, prm, WR);
     }
 }
 

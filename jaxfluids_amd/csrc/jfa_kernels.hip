@@ -213,7 +213,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void rk3_stage_kernel(T* __restrict__ U, const T* __restrict__ Un,
                                                            const T* __restrict__ rhs, T* __restrict__ W, Geo G,
                                                            int stage, const double* __restrict__ dt_dev,
-                                                           double dt_host, double gamma) {
+                                                           double dt_host, Params prm) {
     const int64_t plane = (int64_t)G.n[1] * G.n[2];
     int c0i, c1i, c2i;
     if (!flat_index((int64_t)G.n[0] * plane, G.n[1], G.n[2], c0i, c1i, c2i)) return;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void rk3_stage_kernel(T* __restrict__ U, co
         x = x + ts * ld(rhs, i * ncell + ro);                    // time_integrator.py:57
         u[i] = x;
     }
-    prim_from_cons(u, gamma, w);
+    prim_from_cons(u, prm, w);
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         U[i * G.fs + b] = static_cast<T>(u[i]);
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void rk3_stage_kernel(T* __restrict__ U, co
 // (boundary_condition.py:564-595), U halo = cons(W halo) (outer/material.py:248-250).
 template <class T>
 __global__ __launch_bounds__(kBlock) void halo_kernel(T* __restrict__ W, T* __restrict__ U, Geo G, int axis,
-                                                      int high, int bc, double gamma) {
+                                                      int high, int bc, Params prm) {
     // halo slab: nh along axis, interior along the others
     int ext[3];
     for (int i = 0; i < 3; ++i) ext[i] = i == axis ? G.nh : G.n[i];
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void halo_kernel(T* __restrict__ W, T* __re
 #pragma unroll
     for (int i = 0; i < 5; ++i) w[i] = ld(W, i * G.fs + bs);
     if (bc == JFA_BC_SYMMETRY) w[axis + 1] = -w[axis + 1];  // boundary_condition.py:721-731
-    cons_from_prim(w, gamma, u);
+    cons_from_prim(w, prm, u);
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         W[i * G.fs + bd] = static_cast<T>(w[i]);
@@ -303,25 +303,40 @@ __device__ __forceinline__ double wave_sum(const double w[5], const int act[3], 
     return s;
 }
 
+// Block-wide max (wavefront shuffles, then LDS across the block's waves) and one
+// atomic per block.  Every thread of the block must call it.
 __device__ __forceinline__ void block_max_to(unsigned long long* slot, double v) {
-    // wavefront max with DPP/shuffles, then one atomic per wave
+    __shared__ double part[kBlock / 64];
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) part[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = part[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, part[i]);
+        atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
+constexpr int kCflBlocks = 1024;
+
+// grid-stride over the interior cells: at most kCflBlocks atomics per reduction
 template <class T>
 __global__ __launch_bounds__(kBlock) void cfl_max_kernel(const T* __restrict__ W, Geo G, double gamma,
                                                          unsigned long long* slot) {
-    const int64_t plane = (int64_t)G.n[1] * G.n[2];
+    const uint32_t total = (uint32_t)G.n[0] * G.n[1] * G.n[2];
     double v = 0.0;
-    int c0i, c1i, c2i;
-    if (flat_index((int64_t)G.n[0] * plane, G.n[1], G.n[2], c0i, c1i, c2i)) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const uint32_t q = t / (uint32_t)G.n[2];
+        const int c2i = (int)(t - q * (uint32_t)G.n[2]);
+        const int c0i = (int)(q / (uint32_t)G.n[1]);
+        const int c1i = (int)(q - (uint32_t)c0i * (uint32_t)G.n[1]);
         const int64_t b =
             (int64_t)(c0i + G.off[0]) * G.s[0] + (int64_t)(c1i + G.off[1]) * G.s[1] + (c2i + G.off[2]);
         double w[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) w[i] = ld(W, i * G.fs + b);
-        v = wave_sum(w, G.active, gamma);
+        v = fmax(v, wave_sum(w, G.active, gamma));
     }
     block_max_to(slot, v);
 }
@@ -337,16 +352,16 @@ __global__ void dt_finalize_kernel(const unsigned long long* slot, double dx_min
 // ------------------------------------------------------------------ whole-buffer conversions
 template <class T, bool TO_CONS>
 __global__ __launch_bounds__(kBlock) void convert_kernel(const T* __restrict__ in, T* __restrict__ out, int64_t n,
-                                                         double gamma) {
+                                                         Params prm) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     double a[5], b[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) a[i] = ld(in, i * n + t);
     if (TO_CONS)
-        cons_from_prim(a, gamma, b);
+        cons_from_prim(a, prm, b);
     else
-        prim_from_cons(a, gamma, b);
+        prim_from_cons(a, prm, b);
 #pragma unroll
     for (int i = 0; i < 5; ++i) out[i * n + t] = static_cast<T>(b[i]);
 }
@@ -372,7 +387,7 @@ __device__ __forceinline__ void store_cell(T* __restrict__ W, T* __restrict__ U,
 template <class T>
 __device__ __forceinline__ void halo_scatter(T* __restrict__ W, T* __restrict__ U, const Geo& G, const BC6& bc,
                                              const int c[3], int64_t b, const double w[5], const double u[5],
-                                             double gamma) {
+                                             const Params& prm) {
     const int nh = G.nh;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -395,7 +410,7 @@ __device__ __forceinline__ void halo_scatter(T* __restrict__ W, T* __restrict__ 
 #pragma unroll
                 for (int i = 0; i < 5; ++i) ws[i] = w[i];
                 ws[a + 1] = -ws[a + 1];
-                cons_from_prim(ws, gamma, us);
+                cons_from_prim(ws, prm, us);
                 store_cell(W, U, G.fs, bc0 + (int64_t)(nh - 1 - ci) * sa, ws, us);
             }
         }
@@ -410,7 +425,7 @@ __device__ __forceinline__ void halo_scatter(T* __restrict__ W, T* __restrict__ 
 #pragma unroll
                 for (int i = 0; i < 5; ++i) ws[i] = w[i];
                 ws[a + 1] = -ws[a + 1];
-                cons_from_prim(ws, gamma, us);
+                cons_from_prim(ws, prm, us);
                 store_cell(W, U, G.fs, bc0 + (int64_t)(N + nh + (N - 1 - ci)) * sa, ws, us);
             }
         }
@@ -423,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void stage_epilogue_kernel(const T* __restr
                                                                 const T* __restrict__ Un, T* __restrict__ Wout,
                                                                 T* __restrict__ Uout, Geo G, BC6 bc, int stage,
                                                                 const double* __restrict__ dt_dev, double dt_host,
-                                                                double gamma, unsigned long long* cfl_slot) {
+                                                                Params prm, unsigned long long* cfl_slot) {
     const int64_t plane = (int64_t)G.n[1] * G.n[2];
     double vmax = 0.0;
     int c[3];
@@ -458,13 +473,13 @@ __global__ __launch_bounds__(kBlock) void stage_epilogue_kernel(const T* __restr
             if (stage > 0) x = c0 * x + c1 * ld(Un, i * G.fs + b);
             u[i] = x + ts * rhs[i];
         }
-        prim_from_cons(u, gamma, w);
+        prim_from_cons(u, prm, w);
         store_cell(Wout, Uout, G.fs, b, w, u);
         // halo cells: W copied, U recomputed as cons(W) (outer/material.py:248-250)
         double uh[5];
-        cons_from_prim(w, gamma, uh);
-        halo_scatter(Wout, Uout, G, bc, c, b, w, uh, gamma);
-        if (cfl_slot) vmax = wave_sum(w, G.active, gamma);
+        cons_from_prim(w, prm, uh);
+        halo_scatter(Wout, Uout, G, bc, c, b, w, uh, prm);
+        if (cfl_slot) vmax = wave_sum(w, G.active, prm.gamma);
     }
     if (cfl_slot) block_max_to(cfl_slot, vmax);
 }
@@ -523,9 +538,22 @@ static int check_scheme(const jfa_scheme* s) {
 static Params make_params(const jfa_scheme* s) {
     Params p;
     p.gamma = s->gamma;
+    p.gm1 = s->gamma - 1.0;
+    p.inv_gm1 = 1.0 / (s->gamma - 1.0);
     p.eps = s->eps_stencil;
     p.signal_speed = s->signal_speed;
     p.frozen = s->frozen_state;
+    return p;
+}
+
+static Params gas_params(double gamma) {
+    Params p;
+    p.gamma = gamma;
+    p.gm1 = gamma - 1.0;
+    p.inv_gm1 = 1.0 / (gamma - 1.0);
+    p.eps = 1e-30;
+    p.signal_speed = JFA_SS_EINFELDT;
+    p.frozen = JFA_FROZEN_ARITHMETIC;
     return p;
 }
 
@@ -666,10 +694,10 @@ int jfa_rk3_stage(void* U, const void* U_n, const void* rhs, void* W, const jfa_
     hipStream_t st = (hipStream_t)stream;
     if (scheme->dtype == JFA_F32)
         hipLaunchKernelGGL((rk3_stage_kernel<float>), g, dim3(kBlock), 0, st, (float*)U, (const float*)U_n,
-                           (const float*)rhs, (float*)W, G, stage, dt_dev, dt_host, scheme->gamma);
+                           (const float*)rhs, (float*)W, G, stage, dt_dev, dt_host, make_params(scheme));
     else
         hipLaunchKernelGGL((rk3_stage_kernel<double>), g, dim3(kBlock), 0, st, (double*)U, (const double*)U_n,
-                           (const double*)rhs, (double*)W, G, stage, dt_dev, dt_host, scheme->gamma);
+                           (const double*)rhs, (double*)W, G, stage, dt_dev, dt_host, make_params(scheme));
     return check_launch();
 }
 
@@ -695,10 +723,10 @@ int jfa_halo_update(void* W, void* U, const jfa_grid* grid, const int32_t bc[6],
         dim3 g((unsigned)((total + kBlock - 1) / kBlock));
         if (dtype == JFA_F32)
             hipLaunchKernelGGL((halo_kernel<float>), g, dim3(kBlock), 0, st, (float*)W, (float*)U, G, axis, high, b,
-                               gamma);
+                               gas_params(gamma));
         else
             hipLaunchKernelGGL((halo_kernel<double>), g, dim3(kBlock), 0, st, (double*)W, (double*)U, G, axis, high,
-                               b, gamma);
+                               b, gas_params(gamma));
         rc = check_launch();
         if (rc) return rc;
     }
@@ -715,8 +743,9 @@ int jfa_compute_dt(const void* W, const jfa_grid* grid, double gamma, double cfl
     hipStream_t st = (hipStream_t)stream;
     unsigned long long* slot = (unsigned long long*)workspace;
     hipLaunchKernelGGL(zero_slot_kernel, dim3(1), dim3(1), 0, st, slot);
-    const int64_t plane = (int64_t)G.n[1] * G.n[2];
-    dim3 g = flat_grid((int64_t)G.n[0] * plane);
+    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    const int64_t nblk = (ncell + kBlock - 1) / kBlock;
+    dim3 g((unsigned)(nblk < kCflBlocks ? nblk : kCflBlocks));
     if (dtype == JFA_F32)
         hipLaunchKernelGGL((cfl_max_kernel<float>), g, dim3(kBlock), 0, st, (const float*)W, G, gamma, slot);
     else
@@ -731,9 +760,9 @@ int jfa_cons_from_prim(const void* W, void* U, int64_t ncells, double gamma, int
     dim3 g((unsigned)((ncells + kBlock - 1) / kBlock));
     hipStream_t st = (hipStream_t)stream;
     if (dtype == JFA_F32)
-        hipLaunchKernelGGL((convert_kernel<float, true>), g, dim3(kBlock), 0, st, (const float*)W, (float*)U, ncells, gamma);
+        hipLaunchKernelGGL((convert_kernel<float, true>), g, dim3(kBlock), 0, st, (const float*)W, (float*)U, ncells, gas_params(gamma));
     else if (dtype == JFA_F64)
-        hipLaunchKernelGGL((convert_kernel<double, true>), g, dim3(kBlock), 0, st, (const double*)W, (double*)U, ncells, gamma);
+        hipLaunchKernelGGL((convert_kernel<double, true>), g, dim3(kBlock), 0, st, (const double*)W, (double*)U, ncells, gas_params(gamma));
     else
         return JFA_EUNSUPPORTED;
     return check_launch();
@@ -745,9 +774,9 @@ int jfa_prim_from_cons(const void* U, void* W, int64_t ncells, double gamma, int
     dim3 g((unsigned)((ncells + kBlock - 1) / kBlock));
     hipStream_t st = (hipStream_t)stream;
     if (dtype == JFA_F32)
-        hipLaunchKernelGGL((convert_kernel<float, false>), g, dim3(kBlock), 0, st, (const float*)U, (float*)W, ncells, gamma);
+        hipLaunchKernelGGL((convert_kernel<float, false>), g, dim3(kBlock), 0, st, (const float*)U, (float*)W, ncells, gas_params(gamma));
     else if (dtype == JFA_F64)
-        hipLaunchKernelGGL((convert_kernel<double, false>), g, dim3(kBlock), 0, st, (const double*)U, (double*)W, ncells, gamma);
+        hipLaunchKernelGGL((convert_kernel<double, false>), g, dim3(kBlock), 0, st, (const double*)U, (double*)W, ncells, gas_params(gamma));
     else
         return JFA_EUNSUPPORTED;
     return check_launch();
@@ -799,12 +828,12 @@ int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, voi
     if (scheme->dtype == JFA_F32)
         hipLaunchKernelGGL((stage_epilogue_kernel<float>), g, dim3(kBlock), 0, st, (const float*)Fp[0],
                            (const float*)Fp[1], (const float*)Fp[2], (const float*)U_in, (const float*)U_n,
-                           (float*)W_out, (float*)U_out, G, b6, stage, dt_dev, dt_host, scheme->gamma,
+                           (float*)W_out, (float*)U_out, G, b6, stage, dt_dev, dt_host, make_params(scheme),
                            (unsigned long long*)nullptr);
     else
         hipLaunchKernelGGL((stage_epilogue_kernel<double>), g, dim3(kBlock), 0, st, (const double*)Fp[0],
                            (const double*)Fp[1], (const double*)Fp[2], (const double*)U_in, (const double*)U_n,
-                           (double*)W_out, (double*)U_out, G, b6, stage, dt_dev, dt_host, scheme->gamma,
+                           (double*)W_out, (double*)U_out, G, b6, stage, dt_dev, dt_host, make_params(scheme),
                            (unsigned long long*)nullptr);
     return check_launch();
 }
