@@ -107,7 +107,7 @@ int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, cons
                         int32_t axis, void* flux, void* stream);
 
 /* Convective right-hand side 0.0 + rhs_x + rhs_y + rhs_z, rhs_xi = (1/dx)(F[:-1]-F[1:]).
- * rhs: (5, Nx, Ny, Nz).  workspace: >= jfa_rhs_workspace_bytes(grid, dtype) bytes (face fluxes).
+ * rhs: (5, Nx, Ny, Nz).  workspace: >= jfa_rhs_workspace_bytes(grid, dtype) bytes (FP64 partial RHS).
  * Replaces SpaceSolver.compute_rhs (solvers/space_solver.py:151-453) and compute_rhs_xi (:456-674),
  * convective flux only. */
 int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme,
@@ -142,17 +142,23 @@ int jfa_compute_dt(const void* W, const jfa_grid* grid, double gamma, double cfl
 int jfa_cons_from_prim(const void* W, void* U, int64_t ncells, double gamma, int32_t dtype, void* stream);
 int jfa_prim_from_cons(const void* U, void* W, int64_t ncells, double gamma, int32_t dtype, void* stream);
 
-/* One fused RK3 stage of the whole hot path, for the HLLC + WENO5 path on a single domain:
- * face fluxes of all active axes, flux divergence, stage blend and update, prim(U), and the
- * face-halo fill of W and U for the given boundary types -- the work of jfa_compute_rhs +
- * jfa_rk3_stage + jfa_halo_update in one launch per stage.  U_out/W_out must not alias
- * U_in/W_in (ping-pong buffers).  Faithful to SimulationManager.do_runge_kutta_stages
- * (simulation_manager.py:770-1047) for one stage.
+/* One fused RK3 stage of the whole hot path: face fluxes of every active axis, flux divergence
+ * (accumulated in the reference's axis order), stage blend and update, W = prim(U), and the
+ * face-halo fill of W and U for the boundary types bc[6] -- the work of jfa_compute_rhs +
+ * jfa_rk3_stage + jfa_halo_update without face-flux arrays: one marching sweep per active
+ * non-innermost axis and one innermost-axis sweep that finishes the stage.
+ * If dt_next_dev is non-NULL, the CFL time step of the new state (as jfa_compute_dt, with
+ * `cfl` and `eps`) is written to dt_next_dev[0]; it may alias dt_dev.
+ * W_out/U_out must not alias W_in/U_in; U_out may alias U_n.
+ * Replaces one pass of SimulationManager.do_runge_kutta_stages (simulation_manager.py:770-1047):
+ * compute_rhs (space_solver.py:151-453) -> perform_stage_integration (time_integrator.py:108-227)
+ * -> get_primitives_from_conservatives (:943-944) -> perform_halo_update_material (:961-970),
+ * and at the last stage compute_time_step_size (time_step_size.py:15-157).
  * workspace: >= jfa_stage_workspace_bytes(grid, scheme) bytes. */
 int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out,
                         const jfa_grid* grid, const jfa_scheme* scheme, const int32_t bc[6],
-                        int32_t stage, const double* dt_dev, double dt_host, void* workspace,
-                        size_t workspace_bytes, void* stream);
+                        int32_t stage, const double* dt_dev, double dt_host, double cfl, double eps,
+                        double* dt_next_dev, void* workspace, size_t workspace_bytes, void* stream);
 size_t jfa_stage_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme);
 
 #ifdef __cplusplus

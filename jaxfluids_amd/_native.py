@@ -72,7 +72,7 @@ SIGNATURES = {
     "jfa_compute_dt": (_I32, [_P, _PG, _D, _D, _D, _I32, _P, _P, _P]),
     "jfa_cons_from_prim": (_I32, [_P, _P, _I64, _D, _I32, _P]),
     "jfa_prim_from_cons": (_I32, [_P, _P, _I64, _D, _I32, _P]),
-    "jfa_rk3_stage_fused": (_I32, [_P, _P, _P, _P, _P, _PG, _PS, _PI32, _I32, _P, _D, _P, _SZ, _P]),
+    "jfa_rk3_stage_fused": (_I32, [_P, _P, _P, _P, _P, _PG, _PS, _PI32, _I32, _P, _D, _D, _D, _P, _P, _SZ, _P]),
     "jfa_stage_workspace_bytes": (_SZ, [_PG, _PS]),
 }
 

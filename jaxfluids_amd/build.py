@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjaxfluids_amd.so")
-SOURCES = ["jfa_kernels.hip", "jfa_fused.hip"]
+SOURCES = ["jfa_kernels.hip", "jfa_fused_f64.hip", "jfa_fused_f32.hip"]
 ARCH = os.environ.get("JFA_OFFLOAD_ARCH", "gfx950")
 
 
@@ -39,20 +39,26 @@ def flags() -> list:
 
 
 def build(verbose: bool = True) -> str:
-    """Compile every source into one shared library; returns its path."""
+    """Compile every source (in parallel) into one shared library; returns its path."""
+    from concurrent.futures import ThreadPoolExecutor
+
     cc = hipcc()
-    objs = []
     tmp = LIB + ".tmp"
+    jobs = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        cmd = [cc, *flags(), "-c", path, "-o", obj]
+        jobs.append(([cc, *flags(), "-c", path, "-o", obj], obj))
+
+    def run(job):
+        cmd, _ = job
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        list(ex.map(run, jobs))
+    objs = [obj for _, obj in jobs]
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -1,0 +1,250 @@
+// jfa_common.h -- geometry, launch helpers and shared device epilogues of jaxfluids_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "jfa_device.h"
+
+namespace jfa {
+
+// ------------------------------------------------------------------ geometry
+struct Geo {
+    int n[3];        // interior cells (1 = inactive)
+    int nh;
+    int off[3];      // nh on active axes, 0 on inactive ones
+    int B[3];        // buffer extents
+    int64_t s[3];    // buffer strides (elements)
+    int64_t fs;      // field stride
+    double one_dx[3];
+    double dx_min;
+    int active[3];
+};
+
+static inline int make_geo(const jfa_grid* g, Geo& G) {
+    if (!g) return JFA_EINVAL;
+    if (g->nh < 3) return JFA_EINVAL;
+    int nact = 0;
+    G.dx_min = 1e300;
+    for (int i = 0; i < 3; ++i) {
+        if (g->n[i] < 1) return JFA_EINVAL;
+        G.n[i] = g->n[i];
+        G.active[i] = g->n[i] > 1;
+        nact += G.active[i];
+        G.off[i] = G.active[i] ? g->nh : 0;
+        G.B[i] = G.active[i] ? g->n[i] + 2 * g->nh : 1;
+        if (G.active[i]) {
+            if (!(g->dx[i] > 0.0)) return JFA_EINVAL;
+            G.one_dx[i] = 1.0 / g->dx[i];  // domain_information.py:290
+            if (g->dx[i] < G.dx_min) G.dx_min = g->dx[i];
+        } else {
+            G.one_dx[i] = 0.0;
+        }
+    }
+    if (nact == 0) return JFA_EINVAL;
+    G.nh = g->nh;
+    G.s[2] = 1;
+    G.s[1] = G.B[2];
+    G.s[0] = (int64_t)G.B[1] * G.B[2];
+    G.fs = (int64_t)G.B[0] * G.B[1] * G.B[2];
+    return JFA_OK;
+}
+
+extern thread_local char g_last_error[256];
+
+static inline int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        snprintf(g_last_error, sizeof(g_last_error), "%s", hipGetErrorString(e));
+        return JFA_ELAUNCH;
+    }
+    return JFA_OK;
+}
+
+template <class T>
+__device__ __forceinline__ double ld(const T* __restrict__ p, int64_t i) {
+    return static_cast<double>(p[i]);
+}
+
+// Face extents along axis a: N_a+1 on a, interior extents elsewhere.
+struct FaceIdx {
+    int fe[3];
+    int64_t nface;
+};
+static inline FaceIdx face_extents(const Geo& G, int axis) {
+    FaceIdx f;
+    for (int i = 0; i < 3; ++i) f.fe[i] = G.n[i] + (i == axis ? 1 : 0);
+    f.nface = (int64_t)f.fe[0] * f.fe[1] * f.fe[2];
+    return f;
+}
+
+constexpr int kBlock = 256;
+
+// Flat thread index -> (i0, i1, i2) over extents (e0, e1, e2), i2 fastest (32-bit math;
+// every launch here has fewer than 2^31 items).
+__device__ __forceinline__ bool flat_index(int64_t total, int e1, int e2, int& i0, int& i1, int& i2) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int64_t)t >= total) return false;
+    const uint32_t q = t / (uint32_t)e2;
+    i2 = (int)(t - q * (uint32_t)e2);
+    i0 = (int)(q / (uint32_t)e1);
+    i1 = (int)(q - (uint32_t)i0 * (uint32_t)e1);
+    return true;
+}
+
+static inline dim3 flat_grid(int64_t total) { return dim3((unsigned)((total + kBlock - 1) / kBlock)); }
+
+// RK3.py:27-29 tables
+__device__ __forceinline__ void rk3_coeffs(int stage, double& c0, double& c1, double& m) {
+    m = stage == 0 ? 1.0 : (stage == 1 ? 0.25 : 2.0 / 3.0);
+    c0 = stage == 1 ? 0.25 : 2.0 / 3.0;
+    c1 = stage == 1 ? 0.75 : 1.0 / 3.0;
+}
+
+// ------------------------------------------------------------------ CFL reduction
+// max over interior cells of sum_active(|u_i| + c) (time_step_size.py:103-109).
+// Non-negative doubles order like their bit patterns, so a 64-bit atomicMax on the bits
+// is an exact, order-independent max (a NaN propagates as a huge value).
+__device__ __forceinline__ double wave_sum(const double w[5], const int act[3], double gamma) {
+    const double c = sound_speed(w[4], w[0], gamma);
+    double s = 0.0;
+    if (act[0]) s = s + (fabs(w[1]) + c);
+    if (act[1]) s = s + (fabs(w[2]) + c);
+    if (act[2]) s = s + (fabs(w[3]) + c);
+    return s;
+}
+
+// Block-wide max (wavefront shuffles, then LDS across the block's waves) and one
+// atomic per block.  Every thread of the block must call it.
+__device__ __forceinline__ void block_max_to(unsigned long long* slot, double v) {
+    __shared__ double part[kBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) part[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = part[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, part[i]);
+        atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+constexpr int kCflBlocks = 1024;
+
+// Divergence of the three face fluxes + RK3 blend/update + prim + face-halo writes of the
+// result (+ optional CFL max).  Interior cell c writes W_out/U_out at c and, if c lies in a
+// boundary layer, at the halo positions that the face-halo fill copies it to.
+struct BC6 {
+    int t[6];
+};
+
+template <class T>
+__device__ __forceinline__ void store_cell(T* __restrict__ W, T* __restrict__ U, int64_t fs, int64_t b,
+                                           const double w[5], const double u[5]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        W[i * fs + b] = static_cast<T>(w[i]);
+        U[i * fs + b] = static_cast<T>(u[i]);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void halo_scatter(T* __restrict__ W, T* __restrict__ U, const Geo& G, const BC6& bc,
+                                             const int c[3], int64_t b, const double w[5], const double u[5],
+                                             const Params& prm) {
+    const int nh = G.nh;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!G.active[a]) continue;
+        const int N = G.n[a];
+        const int ci = c[a];
+        const int64_t sa = G.s[a];
+        const int64_t bc0 = b - (int64_t)(ci + nh) * sa;  // same cell with axis-a coordinate 0
+        // low side (west/south/bottom): bc index 2a+1
+        const int tl = bc.t[2 * a + 1], th = bc.t[2 * a];
+        // targets of this cell in the low halo
+        if (tl == JFA_BC_PERIODIC) {
+            if (ci >= N - nh) store_cell(W, U, G.fs, bc0 + (int64_t)(ci - N + nh) * sa, w, u);
+        } else if (tl == JFA_BC_ZEROGRADIENT) {
+            if (ci == 0)
+                for (int m = 0; m < nh; ++m) store_cell(W, U, G.fs, bc0 + (int64_t)m * sa, w, u);
+        } else {  // SYMMETRY
+            if (ci < nh) {
+                double ws[5], us[5];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) ws[i] = w[i];
+                ws[a + 1] = -ws[a + 1];
+                cons_from_prim(ws, prm, us);
+                store_cell(W, U, G.fs, bc0 + (int64_t)(nh - 1 - ci) * sa, ws, us);
+            }
+        }
+        if (th == JFA_BC_PERIODIC) {
+            if (ci < nh) store_cell(W, U, G.fs, bc0 + (int64_t)(N + nh + ci) * sa, w, u);
+        } else if (th == JFA_BC_ZEROGRADIENT) {
+            if (ci == N - 1)
+                for (int m = 0; m < nh; ++m) store_cell(W, U, G.fs, bc0 + (int64_t)(N + nh + m) * sa, w, u);
+        } else {
+            if (ci >= N - nh) {
+                double ws[5], us[5];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) ws[i] = w[i];
+                ws[a + 1] = -ws[a + 1];
+                cons_from_prim(ws, prm, us);
+                store_cell(W, U, G.fs, bc0 + (int64_t)(N + nh + (N - 1 - ci)) * sa, ws, us);
+            }
+        }
+    }
+}
+
+static inline int check_scheme(const jfa_scheme* s) {
+    if (!s) return JFA_EINVAL;
+    if (s->recon_var < 0 || s->recon_var > 3) return JFA_EUNSUPPORTED;
+    if (s->stencil != JFA_WENO5_JS && s->stencil != JFA_WENO5_Z) return JFA_EUNSUPPORTED;
+    if (s->riemann != JFA_HLLC && s->riemann != JFA_RUSANOV) return JFA_EUNSUPPORTED;
+    if (s->signal_speed < 0 || s->signal_speed > 4) return JFA_EUNSUPPORTED;
+    if (s->frozen_state != JFA_FROZEN_ARITHMETIC && s->frozen_state != JFA_FROZEN_ROE) return JFA_EUNSUPPORTED;
+    if (s->dtype != JFA_F64 && s->dtype != JFA_F32) return JFA_EUNSUPPORTED;
+    if (!(s->gamma > 1.0)) return JFA_EINVAL;
+    return JFA_OK;
+}
+
+static inline Params make_params(const jfa_scheme* s) {
+    Params p;
+    p.gamma = s->gamma;
+    p.gm1 = s->gamma - 1.0;
+    p.inv_gm1 = 1.0 / (s->gamma - 1.0);
+    p.eps = s->eps_stencil;
+    p.signal_speed = s->signal_speed;
+    p.frozen = s->frozen_state;
+    return p;
+}
+
+static inline Params gas_params(double gamma) {
+    Params p;
+    p.gamma = gamma;
+    p.gm1 = gamma - 1.0;
+    p.inv_gm1 = 1.0 / (gamma - 1.0);
+    p.eps = 1e-30;
+    p.signal_speed = JFA_SS_EINFELDT;
+    p.frozen = JFA_FROZEN_ARITHMETIC;
+    return p;
+}
+
+static inline size_t dsize(int dtype) { return dtype == JFA_F32 ? 4 : 8; }
+
+// Fused sweeps (jfa_fused.h), instantiated per storage type in jfa_fused_f64.hip / _f32.hip.
+int fused_stage_f64(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out, const Geo& G,
+                    const jfa_scheme* s, const BC6& bc, int stage, const double* dt_dev, double dt_host, double cfl,
+                    double eps, double* dt_next, double* R, double* part, hipStream_t st);
+int fused_stage_f32(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out, const Geo& G,
+                    const jfa_scheme* s, const BC6& bc, int stage, const double* dt_dev, double dt_host, double cfl,
+                    double eps, double* dt_next, double* R, double* part, hipStream_t st);
+int fused_rhs_f64(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double* R, void* rhs,
+                  hipStream_t st);
+int fused_rhs_f32(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double* R, void* rhs,
+                  hipStream_t st);
+// number of per-block CFL partials of the inner sweep
+int64_t fused_partials(const Geo& G);
+
+}  // namespace jfa

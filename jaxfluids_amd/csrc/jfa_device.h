@@ -342,8 +342,9 @@ __device__ __forceinline__ void riemann(const double WL[5], const double WR[5], 
 // i.e. the 6-cell stencil window of eigendecomposition.py:75-95.  For VAR PRIMITIVE and
 // CHAR-PRIMITIVE `win` holds primitives; for CONSERVATIVE and CHAR-CONSERVATIVE it holds
 // conservatives and `wl`, `wr` hold the primitives of the two cells adjacent to the face.
+// `win` is read only: the marching sweeps keep it in registers across faces.
 template <int AXIS, int VAR, int STENCIL>
-__device__ __forceinline__ void face_states(double win[6][5], const double wl[5], const double wr[5],
+__device__ __forceinline__ void face_states(const double (&win)[6][5], const double wl[5], const double wr[5],
                                             const Params& prm, double WL[5], double WR[5], double UL[5],
                                             double UR[5]) {
     constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
@@ -373,23 +374,25 @@ __device__ __forceinline__ void face_states(double win[6][5], const double wl[5]
         const double k1 = 0.5 * f.inv_c;
         const double k3 = f.inv_c * f.inv_c;
         const double k2 = 0.5 * rcp(f.rcc);
+        // one characteristic field at a time (x0 = k2 p - k1 u_n, x1 = rho - k3 p,
+        // x2, x3 = tangential velocities, x4 = k2 p + k1 u_n), each reconstructed L and R
+        double cLv[5], cRv[5], t[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const double r = win[k][0], un = win[k][v], p = win[k][4];
-            const double t0 = win[k][m0], t1 = win[k][m1];
-            const double a = k2 * p, b = k1 * un;
-            win[k][0] = a - b;
-            win[k][1] = r - k3 * p;
-            win[k][2] = t0;
-            win[k][3] = t1;
-            win[k][4] = a + b;
-        }
-        double cLv[5], cRv[5];
+        for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] - k1 * win[k][v];
+        cLv[0] = weno5<STENCIL>(t[0], t[1], t[2], t[3], t[4], eps);
+        cRv[0] = weno5<STENCIL>(t[5], t[4], t[3], t[2], t[1], eps);
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            cLv[i] = weno5<STENCIL>(win[0][i], win[1][i], win[2][i], win[3][i], win[4][i], eps);
-            cRv[i] = weno5<STENCIL>(win[5][i], win[4][i], win[3][i], win[2][i], win[1][i], eps);
-        }
+        for (int k = 0; k < 6; ++k) t[k] = win[k][0] - k3 * win[k][4];
+        cLv[1] = weno5<STENCIL>(t[0], t[1], t[2], t[3], t[4], eps);
+        cRv[1] = weno5<STENCIL>(t[5], t[4], t[3], t[2], t[1], eps);
+        cLv[2] = weno5<STENCIL>(win[0][m0], win[1][m0], win[2][m0], win[3][m0], win[4][m0], eps);
+        cRv[2] = weno5<STENCIL>(win[5][m0], win[4][m0], win[3][m0], win[2][m0], win[1][m0], eps);
+        cLv[3] = weno5<STENCIL>(win[0][m1], win[1][m1], win[2][m1], win[3][m1], win[4][m1], eps);
+        cRv[3] = weno5<STENCIL>(win[5][m1], win[4][m1], win[3][m1], win[2][m1], win[1][m1], eps);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] + k1 * win[k][v];
+        cLv[4] = weno5<STENCIL>(t[0], t[1], t[2], t[3], t[4], eps);
+        cRv[4] = weno5<STENCIL>(t[5], t[4], t[3], t[2], t[1], eps);
         // get_primitives_from_characteristics (eigendecomposition.py:517-521)
         const double sL = cLv[0] + cLv[4], sR = cRv[0] + cRv[4];
         WL[0] = f.P[0] * sL + cLv[1];
@@ -459,24 +462,22 @@ __device__ __forceinline__ void face_states(double win[6][5], const double wl[5]
         L[4][m1] = 0.5 * one_cc * (-P[m1] * g);
         L[4][4] = 0.5 * one_cc * g;
         // transformtochar: einsum over j (eigendecomposition.py:729-730)
+        double ch[6][5];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            double x[5];
 #pragma unroll
             for (int a = 0; a < 5; ++a) {
                 double acc = L[a][0] * win[k][0];
 #pragma unroll
                 for (int b = 1; b < 5; ++b) acc = acc + L[a][b] * win[k][b];
-                x[a] = acc;
+                ch[k][a] = acc;
             }
-#pragma unroll
-            for (int a = 0; a < 5; ++a) win[k][a] = x[a];
         }
         double cLv[5], cRv[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            cLv[i] = weno5<STENCIL>(win[0][i], win[1][i], win[2][i], win[3][i], win[4][i], eps);
-            cRv[i] = weno5<STENCIL>(win[5][i], win[4][i], win[3][i], win[2][i], win[1][i], eps);
+            cLv[i] = weno5<STENCIL>(ch[0][i], ch[1][i], ch[2][i], ch[3][i], ch[4][i], eps);
+            cRv[i] = weno5<STENCIL>(ch[5][i], ch[4][i], ch[3][i], ch[2][i], ch[1][i], eps);
         }
         // transformtophysical (eigendecomposition.py:742)
 #pragma unroll

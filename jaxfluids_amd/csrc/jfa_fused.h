@@ -1,0 +1,355 @@
+// jfa_fused.h -- the fused RK3 stage: face fluxes of every active axis, flux divergence,
+// stage blend/update, prim(U), face-halo fill and the CFL partial max, in one pass over
+// the state per axis without materialising face-flux arrays.
+//
+// Let IN be the innermost active axis (unit stride in the buffer) and A the other active
+// axes.  For every A (ascending) a MARCH kernel walks its columns along A with a 6-cell
+// register window (one new cell loaded per face, lanes along IN so every load is a
+// coalesced row), carries the previous face flux in registers and accumulates
+// (1/dx)(F_lo - F_hi) into a scratch RHS R.  The last pass is the INNER kernel: each
+// wavefront evaluates 64 consecutive faces along IN (63 cells plus the shared upper
+// face), takes the upper-face flux of its cell from the neighbouring lane by a wave
+// shuffle, adds the IN divergence to R in the reference's axis order (x, y, z) and
+// finishes the stage for its cell: U = c0 U + c1 U_n + dt m rhs (RK3.py:27-62),
+// W = prim(U), stores both, scatters the cell into the halo positions the face-halo
+// fill copies it to (U halo = cons(W halo), outer/material.py:248-250), and reduces the
+// CFL wave speed of the new state into one partial per block.
+//
+// Included by the per-storage-type translation units jfa_fused_f64.hip / _f32.hip.
+#pragma once
+
+#include "jfa_common.h"
+
+namespace jfa {
+
+constexpr int kInnerBlock = 256;
+constexpr int kMarchBlockMax = 256;
+
+// Face flux from a 6-cell register window (cells k-3 relative to the cell right of the face).
+// CHAR-CONSERVATIVE also needs the primitives of the two cells adjacent to the face.
+template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN>
+__device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* __restrict__ W, int64_t b_right,
+                                            int64_t sa, int64_t fs, const Params& prm, double F[5]) {
+    double wl[5], wr[5];
+    if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            wl[i] = ld(W, i * fs + b_right - sa);
+            wr[i] = ld(W, i * fs + b_right);
+        }
+    }
+    double WL[5], WR[5], UL[5], UR[5];
+    face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
+    riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, F);
+}
+
+// ------------------------------------------------------------------ MARCH kernel
+// grid: x over IN (lanes), y over the third axis O, z over chunks of L cells along A.
+template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN>
+__global__ __launch_bounds__(kMarchBlockMax) void march_kernel(const T* __restrict__ W, const T* __restrict__ U,
+                                                               Geo G, Params prm, int L, int accumulate,
+                                                               double* __restrict__ R) {
+    constexpr int O = 3 - A - IN;
+    const int iin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iin >= G.n[IN]) return;  // no barriers in this kernel
+    const int io = blockIdx.y;
+    const int a0 = blockIdx.z * L;
+    const int a1 = min(a0 + L, G.n[A]);
+    const int64_t sa = G.s[A];
+    const int64_t fs = G.fs;
+    // buffer index of the column's cell with interior index 0 along A
+    const int64_t bcol = (int64_t)(io + G.off[O]) * G.s[O] + (int64_t)(iin + G.off[IN]) * G.s[IN] +
+                         (int64_t)G.off[A] * sa;
+    constexpr bool cons_win = (VAR == JFA_CONSERVATIVE || VAR == JFA_CHAR_CONSERVATIVE);
+    const T* __restrict__ src = cons_win ? U : W;
+    double win[6][5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bcol + (int64_t)(a0 - 3 + k) * sa);
+    double Fp[5];
+    window_flux<T, A, VAR, STENCIL, RIEMANN>(win, W, bcol + (int64_t)a0 * sa, sa, fs, prm, Fp);
+    // R layout (5, N0, N1, N2)
+    int c[3];
+    c[A] = a0;
+    c[O] = io;
+    c[IN] = iin;
+    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    int64_t r = ((int64_t)c[0] * G.n[1] + c[1]) * G.n[2] + c[2];
+    const int64_t rs = A == 0 ? (int64_t)G.n[1] * G.n[2] : (A == 1 ? (int64_t)G.n[2] : 1);
+    const double odx = G.one_dx[A];
+    // next cell to enter the window, loaded one step ahead
+    double nxt[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) nxt[i] = ld(src, i * fs + bcol + (int64_t)(a0 + 3) * sa);
+    for (int a = a0; a < a1; ++a, r += rs) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+#pragma unroll
+            for (int i = 0; i < 5; ++i) win[k][i] = win[k + 1][i];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) win[5][i] = nxt[i];
+        if (a + 1 < a1) {
+#pragma unroll
+            for (int i = 0; i < 5; ++i) nxt[i] = ld(src, i * fs + bcol + (int64_t)(a + 4) * sa);
+        }
+        double Fn[5];
+        window_flux<T, A, VAR, STENCIL, RIEMANN>(win, W, bcol + (int64_t)(a + 1) * sa, sa, fs, prm, Fn);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const double d = odx * (Fp[i] - Fn[i]);  // space_solver.py:597-599
+            double* p = R + i * ncell + r;
+            *p = accumulate ? *p + d : 0.0 + d;
+            Fp[i] = Fn[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ INNER kernel
+template <class T>
+struct InnerArgs {
+    const double* R;        // RHS of the previous sweeps (valid when accumulate)
+    int accumulate;
+    int mode;               // 0: write rhs to rhs_out; 1: finish the RK3 stage
+    T* rhs_out;             // (5, N0, N1, N2) when mode 0
+    const T* Uin;
+    const T* Un;
+    T* Wout;
+    T* Uout;
+    BC6 bc;
+    int stage;
+    const double* dt_dev;
+    double dt_host;
+    double* cfl_part;       // per-block partial max of the new state's wave speed, or null
+};
+
+// Block max written to part[blockIdx.x] (no atomics); every thread must call it.
+__device__ __forceinline__ void block_max_part(double* part, double v) {
+    __shared__ double sm[kInnerBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sm[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sm[i]);
+        part[blockIdx.x] = m;
+    }
+}
+
+template <class T, int IN, int VAR, int STENCIL, int RIEMANN>
+__global__ __launch_bounds__(kInnerBlock) void inner_kernel(const T* __restrict__ W, const T* __restrict__ U,
+                                                            Geo G, Params prm, InnerArgs<T> args) {
+    constexpr int P = IN == 0 ? 1 : 0;
+    constexpr int Q = IN == 2 ? 1 : 2;
+    const uint32_t NIN = (uint32_t)G.n[IN];
+    const uint32_t NF = NIN + 1;
+    const uint32_t nrows = (uint32_t)G.n[P] * (uint32_t)G.n[Q];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t g = (gtid >> 6) * 63u + lane;  // faces [63 w, 63 w + 63] per wavefront
+    const uint32_t row = g / NF;
+    const uint32_t f = g - row * NF;
+    const bool vface = row < nrows;
+    const int64_t fs = G.fs;
+    double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int cP = 0, cQ = 0;
+    int64_t bf = 0;  // buffer index of the cell right of face f (s[IN] == 1)
+    if (vface) {
+        cP = (int)(row / (uint32_t)G.n[Q]);
+        cQ = (int)(row - (uint32_t)cP * (uint32_t)G.n[Q]);
+        bf = (int64_t)(cP + G.off[P]) * G.s[P] + (int64_t)(cQ + G.off[Q]) * G.s[Q] + (int64_t)(f + G.off[IN]);
+        constexpr bool cons_win = (VAR == JFA_CONSERVATIVE || VAR == JFA_CHAR_CONSERVATIVE);
+        const T* __restrict__ src = cons_win ? U : W;
+        double win[6][5];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bf + (k - 3));
+        window_flux<T, IN, VAR, STENCIL, RIEMANN>(win, W, bf, 1, fs, prm, F);
+    }
+    double Fu[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) Fu[i] = __shfl_down(F[i], 1, 64);
+    const bool vcell = vface && lane < 63u && f < NIN;
+    double vmax = 0.0;
+    if (vcell) {
+        int c[3];
+        c[IN] = (int)f;
+        c[P] = cP;
+        c[Q] = cQ;
+        const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+        const int64_t rc = ((int64_t)c[0] * G.n[1] + c[1]) * G.n[2] + c[2];
+        const double odx = G.one_dx[IN];
+        double rhs[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const double d = odx * (F[i] - Fu[i]);
+            rhs[i] = args.accumulate ? args.R[i * ncell + rc] + d : 0.0 + d;  // space_solver.py:314
+        }
+        if (args.mode == 0) {
+#pragma unroll
+            for (int i = 0; i < 5; ++i) args.rhs_out[i * ncell + rc] = static_cast<T>(rhs[i]);
+        } else {
+            double c0, c1, m;
+            rk3_coeffs(args.stage, c0, c1, m);
+            const double ts = (args.dt_dev ? args.dt_dev[0] : args.dt_host) * m;  // RK3.py:60
+            double u[5], w[5];
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                double x = ld(args.Uin, i * fs + bf);
+                if (args.stage > 0) x = c0 * x + c1 * ld(args.Un, i * fs + bf);  // RK3.py:49-50
+                u[i] = x + ts * rhs[i];                                           // time_integrator.py:57
+            }
+            prim_from_cons(u, prm, w);
+            store_cell(args.Wout, args.Uout, fs, bf, w, u);
+            double uh[5];
+            cons_from_prim(w, prm, uh);
+            halo_scatter(args.Wout, args.Uout, G, args.bc, c, bf, w, uh, prm);
+            if (args.cfl_part) vmax = wave_sum(w, G.active, prm.gamma);
+        }
+    }
+    if (args.cfl_part) block_max_part(args.cfl_part, vmax);
+}
+
+// dt = CFL dx_min / (max + eps) from the per-block partials (time_step_size.py:109,154-155)
+template <int UNUSED = 0>  // template: one definition shared by both storage-type translation units
+__global__ __launch_bounds__(1024) void dt_from_parts_kernel(const double* __restrict__ part, int n, double dx_min,
+                                                             double cfl, double eps, double* __restrict__ dt) {
+    __shared__ double sm[16];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, part[i]);
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = sm[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx = fmax(mx, sm[i]);
+        dt[0] = dx_min / (mx + eps) * cfl;
+    }
+}
+
+// ------------------------------------------------------------------ host side
+static inline int innermost_axis(const Geo& G) { return G.active[2] ? 2 : (G.active[1] ? 1 : 0); }
+
+static inline int64_t inner_blocks(const Geo& G) {
+    const int IN = innermost_axis(G);
+    const int P = IN == 0 ? 1 : 0, Q = IN == 2 ? 1 : 2;
+    const int64_t nfaces = (int64_t)G.n[P] * G.n[Q] * (G.n[IN] + 1);
+    const int64_t nwaves = (nfaces - 2) / 63 + 1;
+    return (nwaves + (kInnerBlock / 64) - 1) / (kInnerBlock / 64);
+}
+
+static inline int march_chunk(const Geo& G, int A, int IN) {
+    const int O = 3 - A - IN;
+    const int64_t waves_per_layer = ((int64_t)G.n[IN] + 63) / 64 * G.n[O];
+    int64_t L = ((int64_t)G.n[A] * waves_per_layer + 8191) / 8192;
+    if (L < 4) L = 4;
+    if (L > 64) L = 64;
+    if (L > G.n[A]) L = G.n[A];
+    return (int)L;
+}
+
+template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN>
+static void launch_march(const void* W, const void* U, const Geo& G, const Params& prm, int accumulate, double* R,
+                         hipStream_t st) {
+    constexpr int O = 3 - A - IN;
+    const int L = march_chunk(G, A, IN);
+    int bx = ((G.n[IN] + 63) / 64) * 64;
+    if (bx > kMarchBlockMax) bx = kMarchBlockMax;
+    dim3 grid((unsigned)((G.n[IN] + bx - 1) / bx), (unsigned)G.n[O], (unsigned)((G.n[A] + L - 1) / L));
+    hipLaunchKernelGGL((march_kernel<T, A, IN, VAR, STENCIL, RIEMANN>), grid, dim3(bx), 0, st,
+                       static_cast<const T*>(W), static_cast<const T*>(U), G, prm, L, accumulate, R);
+}
+
+template <class T, int IN, int VAR, int STENCIL, int RIEMANN>
+static void launch_inner(const void* W, const void* U, const Geo& G, const Params& prm, const InnerArgs<T>& a,
+                         hipStream_t st) {
+    dim3 grid((unsigned)inner_blocks(G));
+    hipLaunchKernelGGL((inner_kernel<T, IN, VAR, STENCIL, RIEMANN>), grid, dim3(kInnerBlock), 0, st,
+                       static_cast<const T*>(W), static_cast<const T*>(U), G, prm, a);
+}
+
+// All sweeps of one stage (or of one RHS evaluation) for fixed scheme templates.
+template <class T, int VAR, int STENCIL, int RIEMANN>
+static void sweeps(const void* W, const void* U, const Geo& G, const Params& prm, double* R, InnerArgs<T> ia,
+                   hipStream_t st) {
+    const int IN = innermost_axis(G);
+    int acc = 0;
+    for (int A = 0; A < IN; ++A) {
+        if (!G.active[A]) continue;
+        if (A == 0 && IN == 1) launch_march<T, 0, 1, VAR, STENCIL, RIEMANN>(W, U, G, prm, acc, R, st);
+        else if (A == 0) launch_march<T, 0, 2, VAR, STENCIL, RIEMANN>(W, U, G, prm, acc, R, st);
+        else launch_march<T, 1, 2, VAR, STENCIL, RIEMANN>(W, U, G, prm, acc, R, st);
+        acc = 1;
+    }
+    ia.R = R;
+    ia.accumulate = acc;
+    if (IN == 0) launch_inner<T, 0, VAR, STENCIL, RIEMANN>(W, U, G, prm, ia, st);
+    else if (IN == 1) launch_inner<T, 1, VAR, STENCIL, RIEMANN>(W, U, G, prm, ia, st);
+    else launch_inner<T, 2, VAR, STENCIL, RIEMANN>(W, U, G, prm, ia, st);
+}
+
+template <class T, int VAR, int STENCIL>
+static void sweeps_r(int riemann, const void* W, const void* U, const Geo& G, const Params& prm, double* R,
+                     const InnerArgs<T>& ia, hipStream_t st) {
+    if (riemann == JFA_HLLC) sweeps<T, VAR, STENCIL, JFA_HLLC>(W, U, G, prm, R, ia, st);
+    else sweeps<T, VAR, STENCIL, JFA_RUSANOV>(W, U, G, prm, R, ia, st);
+}
+template <class T, int VAR>
+static void sweeps_s(int stencil, int riemann, const void* W, const void* U, const Geo& G, const Params& prm,
+                     double* R, const InnerArgs<T>& ia, hipStream_t st) {
+    if (stencil == JFA_WENO5_JS) sweeps_r<T, VAR, JFA_WENO5_JS>(riemann, W, U, G, prm, R, ia, st);
+    else sweeps_r<T, VAR, JFA_WENO5_Z>(riemann, W, U, G, prm, R, ia, st);
+}
+
+template <class T>
+static int run_sweeps(const jfa_scheme* s, const void* W, const void* U, const Geo& G, double* R,
+                      const InnerArgs<T>& ia, hipStream_t st) {
+    const Params prm = make_params(s);
+    switch (s->recon_var) {
+        case JFA_PRIMITIVE: sweeps_s<T, JFA_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
+        case JFA_CONSERVATIVE: sweeps_s<T, JFA_CONSERVATIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
+        case JFA_CHAR_PRIMITIVE: sweeps_s<T, JFA_CHAR_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
+        default: sweeps_s<T, JFA_CHAR_CONSERVATIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
+    }
+    return check_launch();
+}
+
+template <class T>
+int fused_stage_impl(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out, const Geo& G,
+                     const jfa_scheme* s, const BC6& bc, int stage, const double* dt_dev, double dt_host, double cfl,
+                     double eps, double* dt_next, double* R, double* part, hipStream_t st) {
+    InnerArgs<T> ia;
+    ia.R = nullptr;
+    ia.accumulate = 0;
+    ia.mode = 1;
+    ia.rhs_out = nullptr;
+    ia.Uin = static_cast<const T*>(U_in);
+    ia.Un = static_cast<const T*>(U_n);
+    ia.Wout = static_cast<T*>(W_out);
+    ia.Uout = static_cast<T*>(U_out);
+    ia.bc = bc;
+    ia.stage = stage;
+    ia.dt_dev = dt_dev;
+    ia.dt_host = dt_host;
+    ia.cfl_part = dt_next ? part : nullptr;
+    int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, st);
+    if (rc) return rc;
+    if (dt_next) {
+        hipLaunchKernelGGL(dt_from_parts_kernel<0>, dim3(1), dim3(1024), 0, st, part, (int)inner_blocks(G), G.dx_min,
+                           cfl, eps, dt_next);
+        rc = check_launch();
+    }
+    return rc;
+}
+
+template <class T>
+int fused_rhs_impl(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double* R, void* rhs,
+                   hipStream_t st) {
+    InnerArgs<T> ia = {};
+    ia.mode = 0;
+    ia.rhs_out = static_cast<T*>(rhs);
+    return run_sweeps<T>(s, W, U, G, R, ia, st);
+}
+
+}  // namespace jfa

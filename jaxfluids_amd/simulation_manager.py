@@ -81,18 +81,23 @@ class SimulationManager:
         return self._bufs
 
     # ------------------------------------------------------------------ stages
-    def _fused_stage(self, W_in, U_in, U_n, W_out, U_out, stage: int, dt: torch.Tensor) -> None:
+    def _fused_stage(self, W_in, U_in, U_n, W_out, U_out, stage: int, dt: torch.Tensor,
+                     dt_next: Optional[torch.Tensor] = None) -> None:
         b = self._bufs
         scheme = self.convective_flux_solver.scheme(_native.dtype_code(W_in))
+        eps = precision.get_eps() if W_in.dtype == torch.float64 else float(torch.finfo(torch.float32).eps)
+        cfl = float(self.numerical_setup.conservatives.time_integration.CFL)
         rc = _native.load().jfa_rk3_stage_fused(
             _native.ptr(W_in), _native.ptr(U_in), _native.ptr(U_n), _native.ptr(W_out), _native.ptr(U_out),
             self.domain_information.grid, scheme, self.halo_manager.bc_array, int(stage), _native.ptr(dt), 0.0,
-            _native.ptr(b["ws"]), b["ws"].numel(), _native.stream_ptr())
+            cfl, eps, _native.ptr(dt_next), _native.ptr(b["ws"]), b["ws"].numel(), _native.stream_ptr())
         _native.check(rc, "jfa_rk3_stage_fused")
 
-    def do_runge_kutta_stages(self, primitives: torch.Tensor, conservatives: torch.Tensor, dt: torch.Tensor):
+    def do_runge_kutta_stages(self, primitives: torch.Tensor, conservatives: torch.Tensor, dt: torch.Tensor,
+                              dt_next: Optional[torch.Tensor] = None):
         """Three RK3 stages with a device time step.  The fused path updates the given
-        buffers in place; the reference-order path returns new tensors."""
+        buffers in place and, if ``dt_next`` is given, writes the next CFL time step into it
+        (it may be ``dt`` itself); the reference-order path returns new tensors."""
         if self.fused:
             b = self._ensure_buffers(primitives)
             (Wb, Ub), (Wc, Uc) = b["B"], b["C"]
@@ -100,7 +105,7 @@ class SimulationManager:
             self._fused_stage(Wb, Ub, conservatives, Wc, Uc, 1, dt)
             # U_out may alias U_n: each cell reads U_n[c] before writing U_out[c], and no cell
             # reads U_n in the halos.
-            self._fused_stage(Wc, Uc, conservatives, primitives, conservatives, 2, dt)
+            self._fused_stage(Wc, Uc, conservatives, primitives, conservatives, 2, dt, dt_next)
             return primitives, conservatives
         W, U = primitives, conservatives
         init = IntegrationBuffers(EulerIntegrationBuffers(U))
@@ -125,8 +130,14 @@ class SimulationManager:
                                       workspace=None if b is None else b["dt_ws"])
 
     def _step_kernels(self, W, U, dt):
+        fixed = self.numerical_setup.conservatives.time_integration.fixed_timestep
+        if self.fused and not fixed:
+            return self.do_runge_kutta_stages(W, U, dt, dt_next=dt)
         W, U = self.do_runge_kutta_stages(W, U, dt)
-        self.compute_time_step(W, out=dt)
+        if fixed:
+            dt.fill_(float(fixed))
+        else:
+            self.compute_time_step(W, out=dt)
         return W, U
 
     def _capture(self, W, U, dt):
@@ -157,11 +168,7 @@ class SimulationManager:
             # the replayed step reads and writes dt in place
             self._capture(W, U, dt).replay()
         else:
-            W, U = self.do_runge_kutta_stages(W, U, dt)
-            if tcv.fixed_timestep_size:
-                dt.fill_(float(tcv.fixed_timestep_size))
-            else:
-                self.compute_time_step(W, out=dt)
+            W, U = self._step_kernels(W, U, dt)
         tcv = tcv._replace(physical_simulation_time=tcv.physical_simulation_time + dt_used,
                            simulation_step=tcv.simulation_step + 1, physical_timestep_size=dt)
         return JaxFluidsBuffers(MaterialFieldBuffers(U, W, mf.temperature), tcv)
