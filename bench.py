@@ -113,6 +113,9 @@ def main():
     ap.add_argument("--variable", default="CHAR-PRIMITIVE")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device-to-device copy of GIB GiB (5 launches): the reference "
+                         "kernel whose FETCH_SIZE/WRITE_SIZE scale the traffic counters of the stage kernels")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -125,6 +128,15 @@ def main():
     from jaxfluids_amd import InitializationManager, InputManager, SimulationManager
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.calibrate_copy > 0:
+        n = int(args.calibrate_copy * (1 << 30)) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 8, "launches": 5}), flush=True)
+        return
     case, num = tgv_setups(args.n, args.nh, args.dtype, args.variable)
     im = InputManager(case, num)
     bufs = InitializationManager(im).initialization(tgv_primitives(args.n))

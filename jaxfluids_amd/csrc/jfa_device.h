@@ -35,79 +35,127 @@ struct Params {
     double gm1;        // gamma - 1
     double inv_gm1;    // 1 / (gamma - 1)
     double eps;        // WENO epsilon (config/precision.py:39,53)
+    double eps_z;      // (12/13) eps: epsilon of the scaled WENO-Z smoothness indicators
+    double eps_js;     // (12/13)^2 eps: epsilon of the scaled WENO-JS indicators (squared)
     int signal_speed;  // JFA_SS_*
     int frozen;        // JFA_FROZEN_*
 };
 
 // ---------------------------------------------------------------- refined hardware estimates
-// Newton-Raphson for 1/x: r <- r + r(1 - x r), twice from the v_rcp_f64 estimate.
-__device__ __forceinline__ double rcp(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
+// v_rcp_f64 / v_rsq_f64 return estimates with relative error below 2^-24 (measured on the
+// device over 1e-30..1e30); each Newton-Raphson step squares the error.  One step (~2^-48)
+// is used inside the face flux, two steps (full precision) where a value is stored in the
+// state; a / b and sqrt end with a residual correction, which restores full precision.
+__device__ __forceinline__ double rcp1(double x) {  // 1/x, one Newton step
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rcp(double x) {  // 1/x, two Newton steps
+    const double r = rcp1(x);
+    return fma(r, fma(-x, r, 1.0), r);
 }
 // a / b with a final residual correction q <- q + r(a - b q).
 __device__ __forceinline__ double fdiv(double a, double b) {
-    const double r = rcp(b);
+    const double r = rcp1(b);
     const double q = a * r;
     return fma(r, fma(-b, q, a), q);
 }
-// Newton-Raphson for 1/sqrt(x): y <- y + y(1/2 - (x/2) y^2), twice from v_rsq_f64.
-__device__ __forceinline__ double rsq(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-    double t = fma(-h * y, y, 0.5);
-    y = fma(y, t, y);
-    t = fma(-h * y, y, 0.5);
-    return fma(y, t, y);
+// 1/sqrt(x), one Newton step y <- y + y(1/2 - (x/2) y^2)
+__device__ __forceinline__ double rsq1(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    return fma(y, fma(-(0.5 * x) * y, y, 0.5), y);
 }
 // sqrt(x) = x rsq(x) with one residual correction g <- g + (y/2)(x - g^2).
 __device__ __forceinline__ double fsqrt(double x) {
-    const double y = rsq(x);
+    const double y = rsq1(x);
     const double g = x * y;
     return fma(0.5 * y, fma(-g, g, x), g);
 }
 
 // ---------------------------------------------------------------- WENO5
 // weno5_base.py:34-50 smoothness / candidate polynomials; weno5_js.py:39-58; weno5_z.py:152-166.
-// Arguments are (u_imm, u_im, u_i, u_ip, u_ipp) of one side.
+//
+// beta_k = 13/12 D_k^2 + 1/4 S_k^2 with D_k the second difference and S_k the one-sided
+// slope of candidate k.  Scaled by 12/13: beta'_k = D_k^2 + (k S_k)^2, k = sqrt(3/13); the
+// weights are unchanged if the epsilon is scaled alike (WENO-Z: tau5/(beta_k + eps) ->
+// (12/13)-scaled; WENO-JS: 1/(beta_k^2 + eps) -> (12/13)^2-scaled).  The linear weights
+// d = (1/10, 6/10, 3/10) are folded into the candidate coefficients.
+constexpr double kS = 0.4803844614152614;    // sqrt(3/13)
+constexpr double kS3 = 1.4411533842457842;   // 3 sqrt(3/13)
+constexpr double kS4 = -1.9215378456610457;  // -4 sqrt(3/13)
+
+// Nonlinear combination of the candidates: b0..b2 scaled indicators, g0..g2 = d_k p_k.
 template <int STENCIL>
-__device__ __forceinline__ double weno5(double a, double b, double c, double d, double e, double eps) {
-    const double t0 = a - 2.0 * b + c;
-    const double t1 = a - 4.0 * b + 3.0 * c;
-    const double t2 = b - 2.0 * c + d;
-    const double t3 = b - d;
-    const double t4 = c - 2.0 * d + e;
-    const double t5 = 3.0 * c - 4.0 * d + e;
-    const double beta0 = (13.0 / 12.0) * (t0 * t0) + 0.25 * (t1 * t1);
-    const double beta1 = (13.0 / 12.0) * (t2 * t2) + 0.25 * (t3 * t3);
-    const double beta2 = (13.0 / 12.0) * (t4 * t4) + 0.25 * (t5 * t5);
-    // candidate values (weno5_base.py:47-49)
-    const double p0 = (1.0 / 3.0) * a + (-7.0 / 6.0) * b + (11.0 / 6.0) * c;
-    const double p1 = (-1.0 / 6.0) * b + (5.0 / 6.0) * c + (1.0 / 3.0) * d;
-    const double p2 = (1.0 / 3.0) * c + (5.0 / 6.0) * d + (-1.0 / 6.0) * e;
-    double a0, a1, a2;
+__device__ __forceinline__ double weno5_combine(double b0, double b1, double b2, double g0, double g1, double g2,
+                                                const double eps_z, const double eps_js) {
+    double e0, e1, e2;
     if constexpr (STENCIL == JFA_WENO5_JS) {
-        // alpha_k = d_k / (beta_k^2 + eps), multiplied by prod_k (beta_k^2 + eps)
-        const double q0 = beta0 * beta0 + eps;
-        const double q1 = beta1 * beta1 + eps;
-        const double q2 = beta2 * beta2 + eps;
-        a0 = 0.1 * (q1 * q2);
-        a1 = 0.6 * (q0 * q2);
-        a2 = 0.3 * (q0 * q1);
+        // alpha_k = d_k / (beta_k^2 + eps): e_k = prod_{j != k} (beta_j^2 + eps)
+        const double q0 = fma(b0, b0, eps_js), q1 = fma(b1, b1, eps_js), q2 = fma(b2, b2, eps_js);
+        e0 = q1 * q2;
+        e1 = q0 * q2;
+        e2 = q0 * q1;
     } else {
-        // alpha_k = d_k (1 + tau5/(beta_k + eps)) = d_k (beta_k + eps + tau5)/(beta_k + eps)
-        const double tau5 = fabs(beta0 - beta2);
-        const double q0 = beta0 + eps;
-        const double q1 = beta1 + eps;
-        const double q2 = beta2 + eps;
-        a0 = (0.1 * (q0 + tau5)) * (q1 * q2);
-        a1 = (0.6 * (q1 + tau5)) * (q0 * q2);
-        a2 = (0.3 * (q2 + tau5)) * (q0 * q1);
+        // alpha_k = d_k (1 + tau5/(beta_k + eps)): e_k = (beta_k + eps + tau5) prod_{j != k} (beta_j + eps)
+        const double tau5 = fabs(b0 - b2);
+        const double q0 = b0 + eps_z, q1 = b1 + eps_z, q2 = b2 + eps_z;
+        e0 = (q0 + tau5) * (q1 * q2);
+        e1 = (q1 + tau5) * (q0 * q2);
+        e2 = (q2 + tau5) * (q0 * q1);
     }
-    return (a0 * p0 + a1 * p1 + a2 * p2) * rcp(a0 + a1 + a2);
+    const double num = fma(e2, g2, fma(e1, g1, e0 * g0));
+    const double den = fma(0.3, e2, fma(0.6, e1, 0.1 * e0));
+    return num * rcp1(den);
+}
+
+// One side: (u_imm, u_im, u_i, u_ip, u_ipp).
+template <int STENCIL>
+__device__ __forceinline__ double weno5(double a, double b, double c, double d, double e, const double eps_z,
+                                        const double eps_js) {
+    const double t0 = a - 2.0 * b + c, t2 = b - 2.0 * c + d, t4 = c - 2.0 * d + e;
+    const double s0 = fma(kS3, c, fma(kS4, b, kS * a));
+    const double s1 = kS * (b - d);
+    const double s2 = fma(kS3, c, fma(kS4, d, kS * e));
+    const double b0 = fma(s0, s0, t0 * t0), b1 = fma(s1, s1, t2 * t2), b2 = fma(s2, s2, t4 * t4);
+    const double g0 = fma(11.0 / 60.0, c, fma(-7.0 / 60.0, b, (1.0 / 30.0) * a));
+    const double g1 = fma(0.2, d, fma(0.5, c, -0.1 * b));
+    const double g2 = fma(-0.05, e, fma(0.25, d, 0.1 * c));
+    return weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, eps_z, eps_js);
+}
+
+// Left (j = 0: cells -3..1) and right (j = 1: cells 2..-2) states of one face from its
+// 6-cell window v[0..5] = cells -3..2 (weno5_base.py:30-31).  The second differences of
+// cells -1 and 0 and their squares are shared by the two sides.
+template <int STENCIL>
+__device__ __forceinline__ void weno5_pair(const double v[6], const double eps_z, const double eps_js, double& L,
+                                           double& R) {
+    const double Dm2 = v[0] - 2.0 * v[1] + v[2];
+    const double Dm1 = v[1] - 2.0 * v[2] + v[3];
+    const double D0 = v[2] - 2.0 * v[3] + v[4];
+    const double D1 = v[3] - 2.0 * v[4] + v[5];
+    const double Qm1 = Dm1 * Dm1, Q0 = D0 * D0;
+    // left: a..e = v0..v4
+    {
+        const double s0 = fma(kS3, v[2], fma(kS4, v[1], kS * v[0]));
+        const double s1 = kS * (v[1] - v[3]);
+        const double s2 = fma(kS3, v[2], fma(kS4, v[3], kS * v[4]));
+        const double b0 = fma(s0, s0, Dm2 * Dm2), b1 = fma(s1, s1, Qm1), b2 = fma(s2, s2, Q0);
+        const double g0 = fma(11.0 / 60.0, v[2], fma(-7.0 / 60.0, v[1], (1.0 / 30.0) * v[0]));
+        const double g1 = fma(0.2, v[3], fma(0.5, v[2], -0.1 * v[1]));
+        const double g2 = fma(-0.05, v[4], fma(0.25, v[3], 0.1 * v[2]));
+        L = weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, eps_z, eps_js);
+    }
+    // right: a..e = v5..v1
+    {
+        const double s0 = fma(kS3, v[3], fma(kS4, v[4], kS * v[5]));
+        const double s1 = kS * (v[4] - v[2]);
+        const double s2 = fma(kS3, v[3], fma(kS4, v[2], kS * v[1]));
+        const double b0 = fma(s0, s0, D1 * D1), b1 = fma(s1, s1, Q0), b2 = fma(s2, s2, Qm1);
+        const double g0 = fma(11.0 / 60.0, v[3], fma(-7.0 / 60.0, v[4], (1.0 / 30.0) * v[5]));
+        const double g1 = fma(0.2, v[2], fma(0.5, v[3], -0.1 * v[4]));
+        const double g2 = fma(-0.05, v[1], fma(0.25, v[2], 0.1 * v[3]));
+        R = weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, eps_z, eps_js);
+    }
 }
 
 // ---------------------------------------------------------------- ideal gas
@@ -115,7 +163,7 @@ __device__ __forceinline__ double weno5(double a, double b, double c, double d, 
 // c = sqrt(gamma p / rho) = gamma p / sqrt(gamma p rho)
 __device__ __forceinline__ double sound_speed(double p, double rho, double gamma) {
     const double gp = gamma * p;
-    return gp * rsq(gp * rho);
+    return gp * rsq1(gp * rho);
 }
 
 // E = rho (|u|^2/2 + p/(rho (gamma-1))) = rho |u|^2 / 2 + p / (gamma - 1)
@@ -175,7 +223,7 @@ __device__ __forceinline__ void frozen_state(const double WL[5], const double WR
         for (int i = 0; i < 5; ++i) f.P[i] = 0.5 * (WL[i] + WR[i]);  // :148
         // c = sqrt(gamma p / rho) (:223-225); c^2 rho = gamma p; 1/c = rho / sqrt(gamma p rho)
         const double gp = prm.gamma * f.P[4];
-        const double r = rsq(gp * f.P[0]);
+        const double r = rsq1(gp * f.P[0]);
         f.c = gp * r;
         f.inv_c = f.P[0] * r;
         f.cc = f.c * f.c;       // :230
@@ -187,7 +235,7 @@ __device__ __forceinline__ void frozen_state(const double WL[5], const double WR
         }
     } else {  // ROE :233-275
         const double sL = fsqrt(WL[0]), sR = fsqrt(WR[0]);
-        const double rho_div = rcp(sL + sR);
+        const double rho_div = rcp1(sL + sR);
 #pragma unroll
         for (int i = 1; i < 5; ++i) f.P[i] = (sL * WL[i] + sR * WR[i]) * rho_div;  // :293
         f.P[0] = fsqrt(WL[0] * WR[0]);                                             // :237
@@ -199,7 +247,7 @@ __device__ __forceinline__ void frozen_state(const double WL[5], const double WR
         const double sq_dvel = du * du + dv * dv + dw * dw;                        // :265
         const double p_over_rho = psi_ave + 0.5 * f.P[0] * rho_div * rho_div * sq_dvel;  // :267-268
         f.cc = psi_ave + f.g * p_over_rho;                                         // :274
-        f.inv_c = rsq(f.cc);
+        f.inv_c = rsq1(f.cc);
         f.c = f.cc * f.inv_c;                                                      // :275
         f.rcc = f.cc * f.P[0];
         if (FULL) {
@@ -240,9 +288,9 @@ __device__ __forceinline__ void signal_speeds(int kind, double uL, double uR, do
             SR = uR + aR * qR;
         } break;
         default: {  // EINFELDT :125-132
-            const double yL = rsq(rL), yR = rsq(rR);
+            const double yL = rsq1(rL), yR = rsq1(rR);
             const double sL = rL * yL, sR = rR * yR;  // sqrt(rho)
-            const double one_dens = rcp(sL + sR);
+            const double one_dens = rcp1(sL + sR);
             const double eta2 = 0.5 * sL * sR * one_dens * one_dens;
             const double u_bar = (sL * uL + sR * uR) * one_dens;
             const double du = uR - uL;
@@ -265,7 +313,7 @@ __device__ __forceinline__ void hllc_star_flux(const double W[5], const double U
     constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
     const double uK = W[v];
     const double d = SK - uK;
-    const double inv = rcp(SK - Ss);
+    const double inv = rcp1(SK - Ss);
     const double rd = W[0] * d;
     const double pre = rd * inv;
     double us[5];
@@ -348,15 +396,15 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
                                             const Params& prm, double WL[5], double WR[5], double UL[5],
                                             double UR[5]) {
     constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
-    const double eps = prm.eps;
+    const double ez = prm.eps_z, ej = prm.eps_js;
     if constexpr (VAR == JFA_PRIMITIVE || VAR == JFA_CONSERVATIVE) {
         double* QL = VAR == JFA_PRIMITIVE ? WL : UL;
         double* QR = VAR == JFA_PRIMITIVE ? WR : UR;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             // WENO5Base slices j=0: cells -3..1, j=1: cells 2..-2 (weno5_base.py:30)
-            QL[i] = weno5<STENCIL>(win[0][i], win[1][i], win[2][i], win[3][i], win[4][i], eps);
-            QR[i] = weno5<STENCIL>(win[5][i], win[4][i], win[3][i], win[2][i], win[1][i], eps);
+            const double c6[6] = {win[0][i], win[1][i], win[2][i], win[3][i], win[4][i], win[5][i]};
+            weno5_pair<STENCIL>(c6, ez, ej, QL[i], QR[i]);
         }
         if constexpr (VAR == JFA_PRIMITIVE) {
             cons_from_prim(WL, prm, UL);  // :279-280
@@ -373,26 +421,25 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         // k1 = 0.5/c, k2 = 0.5/(c^2 rho), k3 = 1/c^2
         const double k1 = 0.5 * f.inv_c;
         const double k3 = f.inv_c * f.inv_c;
-        const double k2 = 0.5 * rcp(f.rcc);
+        const double k2 = 0.5 * rcp1(f.rcc);
         // one characteristic field at a time (x0 = k2 p - k1 u_n, x1 = rho - k3 p,
         // x2, x3 = tangential velocities, x4 = k2 p + k1 u_n), each reconstructed L and R
         double cLv[5], cRv[5], t[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] - k1 * win[k][v];
-        cLv[0] = weno5<STENCIL>(t[0], t[1], t[2], t[3], t[4], eps);
-        cRv[0] = weno5<STENCIL>(t[5], t[4], t[3], t[2], t[1], eps);
+        weno5_pair<STENCIL>(t, ez, ej, cLv[0], cRv[0]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = win[k][0] - k3 * win[k][4];
-        cLv[1] = weno5<STENCIL>(t[0], t[1], t[2], t[3], t[4], eps);
-        cRv[1] = weno5<STENCIL>(t[5], t[4], t[3], t[2], t[1], eps);
-        cLv[2] = weno5<STENCIL>(win[0][m0], win[1][m0], win[2][m0], win[3][m0], win[4][m0], eps);
-        cRv[2] = weno5<STENCIL>(win[5][m0], win[4][m0], win[3][m0], win[2][m0], win[1][m0], eps);
-        cLv[3] = weno5<STENCIL>(win[0][m1], win[1][m1], win[2][m1], win[3][m1], win[4][m1], eps);
-        cRv[3] = weno5<STENCIL>(win[5][m1], win[4][m1], win[3][m1], win[2][m1], win[1][m1], eps);
+        weno5_pair<STENCIL>(t, ez, ej, cLv[1], cRv[1]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) t[k] = win[k][m0];
+        weno5_pair<STENCIL>(t, ez, ej, cLv[2], cRv[2]);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) t[k] = win[k][m1];
+        weno5_pair<STENCIL>(t, ez, ej, cLv[3], cRv[3]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] + k1 * win[k][v];
-        cLv[4] = weno5<STENCIL>(t[0], t[1], t[2], t[3], t[4], eps);
-        cRv[4] = weno5<STENCIL>(t[5], t[4], t[3], t[2], t[1], eps);
+        weno5_pair<STENCIL>(t, ez, ej, cLv[4], cRv[4]);
         // get_primitives_from_characteristics (eigendecomposition.py:517-521)
         const double sL = cLv[0] + cLv[4], sR = cRv[0] + cRv[4];
         WL[0] = f.P[0] * sL + cLv[1];
@@ -412,7 +459,7 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         frozen_state<true>(wl, wr, prm, f);
         // eigendecomposition_conservatives (eigendecomposition.py:595-658)
         const double one_cc = f.inv_c * f.inv_c;
-        const double one_rho = rcp(f.P[0]);
+        const double one_rho = rcp1(f.P[0]);
         const double* P = f.P;
         const double g = f.g, H = f.H, c = f.c, q2 = f.q2;
         double L[5][5], R[5][5];
@@ -476,8 +523,8 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         double cLv[5], cRv[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            cLv[i] = weno5<STENCIL>(ch[0][i], ch[1][i], ch[2][i], ch[3][i], ch[4][i], eps);
-            cRv[i] = weno5<STENCIL>(ch[5][i], ch[4][i], ch[3][i], ch[2][i], ch[1][i], eps);
+            const double c6[6] = {ch[0][i], ch[1][i], ch[2][i], ch[3][i], ch[4][i], ch[5][i]};
+            weno5_pair<STENCIL>(c6, ez, ej, cLv[i], cRv[i]);
         }
         // transformtophysical (eigendecomposition.py:742)
 #pragma unroll

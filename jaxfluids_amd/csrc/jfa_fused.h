@@ -239,23 +239,54 @@ static inline int64_t inner_blocks(const Geo& G) {
     return (nwaves + (kInnerBlock / 64) - 1) / (kInnerBlock / 64);
 }
 
-static inline int march_chunk(const Geo& G, int A, int IN) {
+// Chunk length L along A: every wave marches L cells (L+1 faces).  The waves of one launch
+// run in ceil(waves / slots) rounds, slots = CUs x resident waves per CU of this kernel, so
+// L is chosen to minimise rounds x (L + 1) -- short chunks cost an extra face each, a
+// part-filled last round idles the chip.
+static inline int march_chunk(const Geo& G, int A, int IN, int64_t slots) {
     const int O = 3 - A - IN;
     const int64_t waves_per_layer = ((int64_t)G.n[IN] + 63) / 64 * G.n[O];
-    int64_t L = ((int64_t)G.n[A] * waves_per_layer + 8191) / 8192;
-    if (L < 4) L = 4;
-    if (L > 64) L = 64;
-    if (L > G.n[A]) L = G.n[A];
-    return (int)L;
+    int best_L = 4;
+    double best = 1e300;
+    for (int L = 4; L <= 64; ++L) {
+        const int64_t chunks = ((int64_t)G.n[A] + L - 1) / L;
+        const int64_t waves = chunks * waves_per_layer;
+        const int64_t rounds = (waves + slots - 1) / slots;
+        const double cost = (double)rounds * (L + 1);
+        if (cost < best * 0.999) {
+            best = cost;
+            best_L = L;
+        }
+        if (L >= G.n[A]) break;
+    }
+    return best_L;
+}
+
+// resident waves of a kernel on the whole device (cached per kernel instance)
+static inline int64_t device_wave_slots(const void* kernel, int block) {
+    static int num_cu = 0;
+    if (num_cu == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || num_cu <= 0)
+            num_cu = 256;
+    }
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, block, 0) != hipSuccess || blocks <= 0)
+        blocks = 1;
+    return (int64_t)num_cu * blocks * ((block + 63) / 64);
 }
 
 template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN>
 static void launch_march(const void* W, const void* U, const Geo& G, const Params& prm, int accumulate, double* R,
                          hipStream_t st) {
     constexpr int O = 3 - A - IN;
-    const int L = march_chunk(G, A, IN);
     int bx = ((G.n[IN] + 63) / 64) * 64;
     if (bx > kMarchBlockMax) bx = kMarchBlockMax;
+    static int64_t slots[kMarchBlockMax / 64 + 1] = {0};
+    int64_t& sl = slots[bx / 64];
+    if (sl == 0) sl = device_wave_slots((const void*)march_kernel<T, A, IN, VAR, STENCIL, RIEMANN>, bx);
+    const int L = march_chunk(G, A, IN, sl);
     dim3 grid((unsigned)((G.n[IN] + bx - 1) / bx), (unsigned)G.n[O], (unsigned)((G.n[A] + L - 1) / L));
     hipLaunchKernelGGL((march_kernel<T, A, IN, VAR, STENCIL, RIEMANN>), grid, dim3(bx), 0, st,
                        static_cast<const T*>(W), static_cast<const T*>(U), G, prm, L, accumulate, R);

@@ -29,10 +29,11 @@ __global__ __launch_bounds__(kBlock) void weno5_reconstruct_kernel(const T* __re
     for (int v = 0; v < nvar; ++v) {
         const T* b = buf + v * G.fs + base;
         double r;
+        const double ez = (12.0 / 13.0) * eps, ej = (144.0 / 169.0) * eps;
         if (j == 0)  // cells -3..1 (weno5_base.py:30)
-            r = weno5<STENCIL>(ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), eps);
+            r = weno5<STENCIL>(ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), ez, ej);
         else  // cells 2..-2
-            r = weno5<STENCIL>(ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), eps);
+            r = weno5<STENCIL>(ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), ez, ej);
         out[(int64_t)v * F.nface + o] = static_cast<T>(r);
     }
 }
