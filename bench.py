@@ -75,6 +75,22 @@ def tgv_primitives(n: int, gamma=1.4, mach=0.1):
     return W
 
 
+def measured_stage_traffic(n: int, dtype: str, variable: str, nh: int):
+    """HBM bytes per RK3 stage from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
+    this workload (profiles/tgv256_f64_stage_kernels.md), or None for other workloads."""
+    if (n, dtype, variable, nh) != (256, "f64", "CHAR-PRIMITIVE", 5):
+        return None
+    path = os.path.join(ROOT, "profiles", "tgv256_f64_stage_kernels.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("stage_hbm_bytes:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(n: int, nh: int, variable: str, seconds: float):
     """Time the C/OpenMP restatement on a bounded number of steps of the same workload."""
     from oracle import c_port
@@ -196,8 +212,10 @@ def main():
                                f"HLLC-Einfeldt + RK3, CFL 0.5, halo_cells {args.nh}",
                    "cells": ncell, "halo_cells": args.nh, "storage": args.dtype, "finite": finite},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "RK3 stage (jfa_rk3_stage_fused)", "avg_stage_ms": round(tot_ms / len(events), 4),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_stage_traffic(args.n, args.dtype, args.variable, args.nh),
+                     "kernel": "RK3 stage (jfa_rk3_stage_fused: x/y marching sweeps + z sweep with update)",
+                     "avg_stage_ms": round(tot_ms / len(events), 4),
                      "alg_bytes_per_cell_stage": round(sum(byt) / 3, 2)},
     }
     if not args.no_cpu_baseline:
