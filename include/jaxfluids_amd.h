@@ -161,6 +161,39 @@ int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, voi
                         double* dt_next_dev, void* workspace, size_t workspace_bytes, void* stream);
 size_t jfa_stage_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme);
 
+/* ---- Ensembles of independent 2-D domains (z inactive), advanced by one launch per RK3
+ * stage for all cases together.  Each case is a (5, Nx+2nh, Ny+2nh, 1) block inside pooled
+ * buffers (W, U and their stage copies share one layout); cases may differ in size,
+ * boundary types and time step.  This batching has no counterpart in the reference, which
+ * advances one domain per SimulationManager (simulation_manager.py:670-1077); every case
+ * follows that function's stage arithmetic exactly. */
+typedef struct jfa_case2d {
+    int32_t n[2];      /* interior cells along x and y */
+    int32_t nh;        /* halo cells (>= 3) */
+    int32_t bc[4];     /* east, west, north, south (JFA_BC_*) */
+    int32_t reserved;
+    double dx[2];      /* cell sizes along x and y */
+    int64_t offset;    /* element offset of the case's block in the pooled buffers */
+} jfa_case2d;
+
+/* Workspace (device) for `ncases` cases tiled in x-chunks of `chunk` planes (1..64); 0 if the
+ * table is invalid (nh < 3, fewer than 2 cells on an axis, a periodic or symmetric axis shorter
+ * than nh, a case block of 4 GiB or more). */
+size_t jfa_batch2d_workspace_bytes(const jfa_case2d* cases, int32_t ncases, int32_t chunk);
+
+/* Writes the case table and the (case, x-chunk, 64-cell y-tile) work items into the workspace;
+ * synchronises `stream` before returning (host tables are temporaries). */
+int jfa_batch2d_plan(const jfa_case2d* cases, int32_t ncases, int32_t chunk, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* One RK3 stage of every case in one launch (the arithmetic of jfa_rk3_stage_fused per case):
+ * dt_dev holds one time step per case; if dt_next_dev is non-NULL the next CFL time step of every
+ * case is written there (may alias dt_dev).  The plan must have been made for the same `chunk`. */
+int jfa_batch2d_stage(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out,
+                      const jfa_case2d* cases, int32_t ncases, int32_t chunk, const jfa_scheme* scheme,
+                      int32_t stage, const double* dt_dev, double cfl, double eps, double* dt_next_dev,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

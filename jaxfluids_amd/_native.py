@@ -44,6 +44,11 @@ class JfaScheme(ctypes.Structure):
     ]
 
 
+class JfaCase2D(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 2), ("nh", ctypes.c_int32), ("bc", ctypes.c_int32 * 4),
+                ("reserved", ctypes.c_int32), ("dx", ctypes.c_double * 2), ("offset", ctypes.c_int64)]
+
+
 class JFAError(RuntimeError):
     pass
 
@@ -74,6 +79,9 @@ SIGNATURES = {
     "jfa_prim_from_cons": (_I32, [_P, _P, _I64, _D, _I32, _P]),
     "jfa_rk3_stage_fused": (_I32, [_P, _P, _P, _P, _P, _PG, _PS, _PI32, _I32, _P, _D, _D, _D, _P, _P, _SZ, _P]),
     "jfa_stage_workspace_bytes": (_SZ, [_PG, _PS]),
+    "jfa_batch2d_workspace_bytes": (_SZ, [_P, _I32, _I32]),
+    "jfa_batch2d_plan": (_I32, [_P, _I32, _I32, _P, _SZ, _P]),
+    "jfa_batch2d_stage": (_I32, [_P, _P, _P, _P, _P, _P, _I32, _I32, _PS, _I32, _P, _D, _D, _P, _P, _SZ, _P]),
 }
 
 _lib: Optional[ctypes.CDLL] = None

@@ -62,8 +62,20 @@ static inline int check_launch() {
     return JFA_OK;
 }
 
+// Hooks for a host build of these sources (the test suite's CPU emulation of the kernels):
+// JFA_ACCESS(p, bytes) sees every state load and store, JFA_OPAQUE(x) hides a value from the
+// optimiser.  In the library JFA_ACCESS is empty and JFA_OPAQUE is an empty asm statement
+// that keeps per-tap address arithmetic from being hoisted out of march loops.
+#ifndef JFA_ACCESS
+#define JFA_ACCESS(p, bytes)
+#endif
+#ifndef JFA_OPAQUE
+#define JFA_OPAQUE(x) asm volatile("" : "+v"(x))
+#endif
+
 template <class T>
 __device__ __forceinline__ double ld(const T* __restrict__ p, int64_t i) {
+    JFA_ACCESS(p + i, sizeof(T));
     return static_cast<double>(p[i]);
 }
 
@@ -144,6 +156,8 @@ __device__ __forceinline__ void store_cell(T* __restrict__ W, T* __restrict__ U,
                                            const double w[5], const double u[5]) {
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
+        JFA_ACCESS(W + i * fs + b, sizeof(T));
+        JFA_ACCESS(U + i * fs + b, sizeof(T));
         W[i * fs + b] = static_cast<T>(w[i]);
         U[i * fs + b] = static_cast<T>(u[i]);
     }
