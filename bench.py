@@ -14,6 +14,8 @@ stages 2-3: read W,U(,U_n), write W,U; FP64) divided by its average launch durat
 measured with HIP events on the launch stream over the timed region; peak 8 TB/s.
 cpu_baseline: the C/OpenMP restatement (oracle/c) of the same step on the host cores,
 timed on a bounded sample of the same workload.
+ensemble: the batched-ensemble workload (64 uneven 2-D Riemann problems, one launch per RK3
+stage for all cases) against plain per-case launches of the same stages.
 """
 from __future__ import annotations
 
@@ -118,6 +120,154 @@ def cpu_baseline(n: int, nh: int, variable: str, seconds: float):
                       f"(oracle/c/jxf_cpu.c, gcc -O3 -march=x86-64-v3), {el:.1f} s"}
 
 
+def bench_ensemble(steps: int, warmup: int, ncases: int = 64, seed: int = 0, nh: int = 5):
+    """Batched ensemble of independent 2-D four-quadrant Riemann problems (sizes drawn from
+    {128,192,256,384,512}^2, seeded states), WENO5-JS CHAR-PRIMITIVE + HLLC-Einfeldt + RK3,
+    CFL 0.5, zero-gradient boundaries, each case with its own CFL dt.  Times the batched path
+    (one launch per RK3 stage for all cases, eager and replayed from a HIP graph) against the
+    same cases advanced by plain per-case launches of the fused single-domain stage."""
+    import ctypes
+
+    import torch
+
+    from jaxfluids_amd import _native
+    from jaxfluids_amd.ensemble import Ensemble2D
+    from jaxfluids_amd.workloads import four_quadrant_cases, quadrant_primitives
+
+    qc = four_quadrant_cases(ncases, seed=seed)
+    zg = {f: "ZEROGRADIENT" for f in ("east", "west", "north", "south")}
+    scheme = dict(stencil="WENO5-JS", variable="CHAR-PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT")
+    ens = Ensemble2D([{"cells": (c.nx, c.ny), "ranges": ((0.0, 1.0), (0.0, 1.0)), "bcs": zg} for c in qc],
+                     nh=nh, cfl=0.5, chunk=16, **scheme)
+    ens.initialize([quadrant_primitives(c) for c in qc])
+    chunk = ens.autotune(chunks=(8, 16, 32, 64), reps=2)
+    ncell = ens.cells
+    W0, U0, dt0 = ens.A[0].clone(), ens.A[1].clone(), ens.dt.clone()
+
+    def restore():
+        ens.A[0].copy_(W0)
+        ens.A[1].copy_(U0)
+        ens.dt.copy_(dt0)
+
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(n)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # batched, eager, with per-stage kernel times from HIP events on the launch stream
+    stream = torch.cuda.current_stream()
+    ev = []
+
+    def eager_step_events(n):
+        for _ in range(n):
+            for k, (src, dst) in enumerate(((ens.A, ens.B), (ens.B, ens.C), (ens.C, ens.A))):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                ens._stage(src, dst, k, k == 2)
+                e1.record(stream)
+                ev.append((k, e0, e1))
+
+    restore()
+    ens.step(warmup)
+    restore()
+    t_eager = timed(eager_step_events, steps)
+    W_batched = ens.A[0].clone()
+    kern_ms = sum(e0.elapsed_time(e1) for _, e0, e1 in ev)
+    byt = STAGE_BYTES_PER_CELL[8]
+    kern_bytes = sum(byt[k] * ncell for k, _, _ in ev)
+    # batched, graph replay
+    restore()
+    ens.step(1, use_graph=True)  # capture + one replay
+    restore()
+    t_graph = timed(lambda n: ens.step(n, use_graph=True), steps)
+
+    # per-case: each case in its own buffers, three fused stages per step
+    lib = _native.load()
+    per = []
+    for i in range(ncases):
+        W, U = ens.view(W0, i).clone(), ens.view(U0, i).clone()  # the initial state
+        grid, bcs = ens.grids[i], ens.bcs[i]
+        nb = max(int(lib.jfa_stage_workspace_bytes(grid, ens.scheme)), 16)
+        per.append(dict(W=W, U=U, B=(torch.empty_like(W), torch.empty_like(U)), C=(torch.empty_like(W), torch.empty_like(U)),
+                        ws=torch.empty(nb, dtype=torch.uint8, device="cuda"), grid=grid, bcs=bcs,
+                        dt=torch.empty(1, dtype=torch.float64, device="cuda")))
+    W0s = [(q["W"].clone(), q["U"].clone()) for q in per]
+
+    def per_restore():
+        for i, q in enumerate(per):
+            q["W"].copy_(W0s[i][0])
+            q["U"].copy_(W0s[i][1])
+            q["dt"].copy_(dt0[i:i + 1])
+
+    def per_step(n):
+        st = _native.stream_ptr()
+        P = _native.ptr
+        for _ in range(n):
+            for q in per:
+                W, U, (Wb, Ub), (Wc, Uc), dt = q["W"], q["U"], q["B"], q["C"], q["dt"]
+                for k, (wi, ui, wo, uo) in enumerate(((W, U, Wb, Ub), (Wb, Ub, Wc, Uc), (Wc, Uc, W, U))):
+                    rc = lib.jfa_rk3_stage_fused(P(wi), P(ui), P(U if k else None), P(wo), P(uo), q["grid"],
+                                                 ens.scheme, q["bcs"], k, P(dt), 0.0, ens.cfl, ens.eps,
+                                                 P(dt) if k == 2 else P(None), P(q["ws"]), q["ws"].numel(), st)
+                    if rc:
+                        _native.check(rc, "jfa_rk3_stage_fused")
+
+    def case_diffs(Wpool):
+        """relative max difference per case between the batched state and the per-case state"""
+        out = []
+        for i, q in enumerate(per):
+            We = ens.view(Wpool, i)
+            if not (torch.isfinite(We).all() and torch.isfinite(q["W"]).all()):
+                out.append(float("nan"))
+                continue
+            out.append((q["W"] - We).abs().max().item() / We.abs().max().item())
+        return out
+
+    # agreement of the two paths over a short horizon from the same state
+    restore()
+    ens.step(10)
+    per_restore()
+    per_step(10)
+    d10 = [d for d in case_diffs(ens.A[0]) if d == d]
+    per_restore()
+    per_step(warmup)
+    per_restore()
+    t_per = timed(per_step, steps)
+    d200 = case_diffs(W_batched)
+    # per-case, graph replay (the same launches captured once)
+    per_restore()
+    g = torch.cuda.CUDAGraph()
+    s2 = torch.cuda.Stream()
+    s2.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(g, stream=s2):
+        per_step(1)
+    torch.cuda.current_stream().wait_stream(s2)
+    per_restore()
+    t_per_graph = timed(lambda n: [g.replay() for _ in range(n)], steps)
+    finite = int(sum(bool(torch.isfinite(ens.view(W_batched, i)).all().item()) for i in range(ncases)))
+
+    rate = lambda t: ncell * 3 * steps / t / 1e6
+    achieved = kern_bytes / (kern_ms * 1e-3) / 1e9
+    return {
+        "workload": f"{ncases} independent 2-D four-quadrant Riemann problems, sizes from "
+                    "{128,192,256,384,512}^2 (seed 0), WENO5-JS CHAR-PRIMITIVE + HLLC-Einfeldt + RK3, CFL 0.5, "
+                    f"zero-gradient, halo_cells {nh}, FP64, per-case CFL dt",
+        "cells": ncell, "steps": steps, "chunk": chunk, "unit": "Mcell-updates/s",
+        "batched_graph": round(rate(t_graph), 2), "batched_eager": round(rate(t_eager), 2),
+        "per_case_eager": round(rate(t_per), 2), "per_case_graph": round(rate(t_per_graph), 2),
+        "speedup_vs_per_case_eager": round(t_per / t_graph, 3),
+        "speedup_vs_per_case_graph": round(t_per_graph / t_graph, 3),
+        "batched_vs_per_case_max_rel_diff_10_steps": max(d10) if d10 else None,
+        "cases_within_1e-10_after_all_steps": int(sum(1 for d in d200 if d == d and d <= 1e-10)),
+        "finite_cases": finite,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "batch2d_stage_kernel", "avg_stage_ms": round(kern_ms / len(ev), 4)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +279,10 @@ def main():
     ap.add_argument("--variable", default="CHAR-PRIMITIVE")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("tgv", "ensemble", "all"), default="all",
+                    help="tgv: the headline line only; ensemble: only the batched-ensemble measurement; "
+                         "all: headline line with the ensemble measurement attached")
+    ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device-to-device copy of GIB GiB (5 launches): the reference "
                          "kernel whose FETCH_SIZE/WRITE_SIZE scale the traffic counters of the stage kernels")
@@ -152,6 +306,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": n * 8, "launches": 5}), flush=True)
+        return
+    if args.workload == "ensemble":
+        print(json.dumps(bench_ensemble(args.ensemble_steps, 5)), flush=True)
         return
     case, num = tgv_setups(args.n, args.nh, args.dtype, args.variable)
     im = InputManager(case, num)
@@ -218,9 +375,12 @@ def main():
                      "avg_stage_ms": round(tot_ms / len(events), 4),
                      "alg_bytes_per_cell_stage": round(sum(byt) / 3, 2)},
     }
-    if not args.no_cpu_baseline:
-        del sim, bufs, W, U, Wb, Ub, Wc, Uc
+    del sim, bufs, W, U, Wb, Ub, Wc, Uc, W_h
+    torch.cuda.empty_cache()
+    if args.workload == "all":
+        out["ensemble"] = bench_ensemble(args.ensemble_steps, 5)
         torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.n, args.nh, args.variable if args.variable != "CHAR-CONSERVATIVE"
                                            else "CHAR-PRIMITIVE", args.cpu_seconds)
     print(json.dumps(out), flush=True)

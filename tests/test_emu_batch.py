@@ -125,9 +125,9 @@ class HostEnsemble:
         self.stage(2, 0, 2, True)
 
 
-def oracle_case(c, scheme, prim):
+def oracle_case(c, scheme, prim, nh=3):
     cells, ranges, bcs = CASES[c]
-    dom = o.Domain((cells[0], cells[1], 1), (ranges[0], ranges[1], (0.0, 1.0)), nh=3)
+    dom = o.Domain((cells[0], cells[1], 1), (ranges[0], ranges[1], (0.0, 1.0)), nh=nh)
     st, var, rs, ss, fz = scheme
     s = o.Setup(gamma=1.4, stencil=st, variable=var, riemann=rs, signal_speed=ss, frozen_state=fz,
                 bcs={**bcs, "top": "PERIODIC", "bottom": "PERIODIC"})
@@ -153,13 +153,14 @@ SCHEMES = [
 ]
 
 
-@pytest.mark.parametrize("scheme", SCHEMES, ids=lambda s: "-".join(s))
-def test_emulated_batch_step_matches_oracle(emu, scheme):
+@pytest.mark.parametrize("scheme,nh", [(s, 3) for s in SCHEMES] + [(SCHEMES[1], 5), (SCHEMES[0], 4)],
+                         ids=lambda v: "-".join(v) if isinstance(v, tuple) else f"nh{v}")
+def test_emulated_batch_step_matches_oracle(emu, scheme, nh):
     """One RK3 step of 5 uneven cases (ragged x chunks and y tiles, every boundary type):
     every load/store in bounds, state and next dt per case equal to the oracle's."""
     prims = [case_state(c[0], c[1], seed=700 + i) for i, c in enumerate(CASES)]
-    ens = HostEnsemble(emu, scheme, chunk=16)
-    refs = [oracle_case(i, scheme, p) for i, p in enumerate(prims)]
+    ens = HostEnsemble(emu, scheme, nh=nh, chunk=16)
+    refs = [oracle_case(i, scheme, p, nh) for i, p in enumerate(prims)]
     for i, (dom, s, W, U) in enumerate(refs):
         ens.view(0, i)[...] = W
         ens.view(1, i)[...] = U
