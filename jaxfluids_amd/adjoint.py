@@ -1,0 +1,258 @@
+"""Reverse mode of fixed-dt RK3 rollouts on the GPU.
+
+The reference differentiates a simulation with ``jax.grad`` through
+``SimulationManager.do_runge_kutta_stages`` (simulation_manager.py:770-1047; feed-forward
+training in simulation_manager.py:1180-1294) and relies on XLA to store or rematerialise the
+forward.  Here the derivative of one RK3 stage is a hand-written HIP VJP
+(``jfa_rk3_stage_vjp``), one step's VJP recomputes that step's stage states from the state at
+its start, and a rollout of many steps is reversed with binomial checkpointing: with ``s``
+snapshot slots the schedule that minimises recomputation is found by dynamic programming over
+T(t, s) = min_m [m + T(t-m, s-1) + T(m, s)] (Griewank's binomial checkpointing), and the
+reverse sweep follows it with an explicit stack.
+
+``RK3Rollout`` wraps a rollout as a ``torch.autograd.Function``: its input is the interior
+primitive state (5, Nx, Ny, Nz) at t = 0, its output the interior primitives after ``nsteps``
+steps at fixed ``dt``; any torch loss of the output back-propagates into the initial state.
+"""
+from __future__ import annotations
+
+import ctypes
+import functools
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _native
+from .config import precision
+
+
+# ------------------------------------------------------------------ binomial checkpointing
+@functools.lru_cache(maxsize=16)
+def binomial_table(nsteps: int, snapshots: int) -> Tuple[np.ndarray, np.ndarray]:
+    """T[t, s]: fewest forward steps to reverse t steps from a stored start state with s free
+    snapshot slots; M[t, s]: the first advance of an optimal schedule."""
+    S = max(0, min(snapshots, nsteps))
+    T = np.zeros((nsteps + 1, S + 1), dtype=np.int64)
+    M = np.ones((nsteps + 1, S + 1), dtype=np.int64)
+    t = np.arange(nsteps + 1, dtype=np.int64)
+    T[:, 0] = t * (t - 1) // 2
+    T[0, :] = 0
+    for s in range(1, S + 1):
+        T[1:, s] = t[1:] - 1  # with a slot for every step: advance one step at a time
+        for n in range(s + 2, nsteps + 1):
+            m = np.arange(1, n)
+            c = m + T[n - m, s - 1] + T[m, s]
+            j = int(np.argmin(c))
+            T[n, s] = c[j]
+            M[n, s] = m[j]
+    return T, M
+
+
+def reverse_rollout(snaps: dict, nsteps: int, snapshots: int, lam, advance, step_vjp, store):
+    """Reverse sweep of a rollout.  snaps[p] holds the state at step p (snaps[0] at least);
+    advance(state, n) returns a working state n steps later, step_vjp(state, lam) the cotangent
+    one step earlier, store(state) a copy to keep.  At most `snapshots` states beyond snaps[0]
+    are alive at once, and the forward steps taken are T[nsteps, snapshots] of binomial_table
+    (fewer where snaps already holds first-level snapshots)."""
+    T, M = binomial_table(nsteps, snapshots)
+    stack = [("seg", 0, nsteps, min(snapshots, nsteps))]
+    while stack:
+        item = stack.pop()
+        if item[0] == "free":
+            snaps.pop(item[1], None)
+            continue
+        _, st, en, s = item
+        if en - st == 1:
+            lam = step_vjp(snaps[st], lam)
+            continue
+        if s == 0:
+            for k in range(en - 1, st - 1, -1):
+                lam = step_vjp(snaps[st] if k == st else advance(snaps[st], k - st), lam)
+            continue
+        m = int(M[en - st, s])
+        if st + m not in snaps:
+            snaps[st + m] = store(advance(snaps[st], m))
+        stack.append(("seg", st, st + m, s))
+        stack.append(("free", st + m))
+        stack.append(("seg", st + m, en, s - 1))
+    return lam
+
+
+def first_level_snapshots(nsteps: int, snapshots: int) -> List[int]:
+    """Positions the forward sweep stores so the reverse sweep starts without re-advancing."""
+    T, M = binomial_table(nsteps, snapshots)
+    out, p, s = [], 0, min(snapshots, nsteps)
+    while nsteps - p > 1 and s > 0:
+        p += int(M[nsteps - p, s])
+        out.append(p)
+        s -= 1
+    return out
+
+
+# ------------------------------------------------------------------ torch pointwise VJPs
+def prim_vjp(W: torch.Tensor, gW: torch.Tensor, gamma: float) -> torch.Tensor:
+    """(dW/dU)^T gW for W = prim(U) (equation_manager.py:164-171)."""
+    rho, u, v, w = W[0], W[1], W[2], W[3]
+    g0, g1, g2, g3, g4 = gW[0], gW[1], gW[2], gW[3], gW[4]
+    ir = 1.0 / rho
+    gm1 = gamma - 1.0
+    q2 = u * u + v * v + w * w
+    return torch.stack([g0 - (u * g1 + v * g2 + w * g3) * ir + gm1 * 0.5 * q2 * g4,
+                        g1 * ir - gm1 * u * g4, g2 * ir - gm1 * v * g4, g3 * ir - gm1 * w * g4, gm1 * g4])
+
+
+def cons_vjp(W: torch.Tensor, gU: torch.Tensor, gamma: float) -> torch.Tensor:
+    """(dU/dW)^T gU for U = cons(W) (equation_manager.py:93-101)."""
+    rho, u, v, w = W[0], W[1], W[2], W[3]
+    q2 = u * u + v * v + w * w
+    return torch.stack([gU[0] + gU[1] * u + gU[2] * v + gU[3] * w + gU[4] * 0.5 * q2,
+                        (gU[1] + gU[4] * u) * rho, (gU[2] + gU[4] * v) * rho, (gU[3] + gU[4] * w) * rho,
+                        gU[4] / (gamma - 1.0)])
+
+
+# ------------------------------------------------------------------ stepper
+class FixedDtStepper:
+    """Fixed-dt RK3 steps through the fused HIP stage and their VJPs, FP64, primitive
+    reconstruction.  ``cells``/``ranges``/``bcs`` as in the case setup; the scheme strings are
+    the reference's registry names."""
+
+    def __init__(self, cells, ranges, bcs: Dict[str, str], dt: float, *, nh: int = 3, stencil: str = "WENO5-JS",
+                 riemann: str = "HLLC", signal_speed: str = "EINFELDT", gamma: float = 1.4, device=None) -> None:
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
+        self.cells = tuple(int(c) for c in cells)
+        self.nh = int(nh)
+        self.gamma = float(gamma)
+        dx = [(r[1] - r[0]) / n for r, n in zip(ranges, self.cells)]
+        self.grid = _native.make_grid(self.cells, self.nh, dx)
+        self.scheme = _native.make_scheme("PRIMITIVE", stencil, riemann, signal_speed if riemann == "HLLC" else None,
+                                          "ARITHMETIC", _native.F64, gamma, precision.get_spatial_stencil_eps())
+        full = {f: bcs.get(f, "PERIODIC") for f in _native.FACE_LOCATIONS}
+        self.bc = _native.bc_array(full)
+        self.shape = (5,) + tuple(n + 2 * self.nh if n > 1 else 1 for n in self.cells)
+        self.interior = (slice(None),) + tuple(slice(self.nh, -self.nh) if n > 1 else slice(None) for n in self.cells)
+        lib = _native.load()
+        e = lambda: torch.empty(self.shape, dtype=torch.float64, device=self.device)
+        self._B = (e(), e())
+        self._C = (e(), e())
+        self._ws = torch.empty(max(int(lib.jfa_stage_workspace_bytes(self.grid, self.scheme)), 16), dtype=torch.uint8,
+                               device=self.device)
+        nb = int(lib.jfa_stage_vjp_workspace_bytes(self.grid))
+        if nb == 0:
+            raise _native.JFAError("invalid grid for the adjoint")
+        self._aws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        self._lam = [e(), e(), e()]
+        self.dt_value = float(dt)
+        self.dt = torch.full((1,), float(dt), dtype=torch.float64, device=self.device)
+        self.forward_steps = 0  # forward steps executed (rollouts and recomputation)
+
+    # -- state
+    def fill(self, W_int: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+        """Buffers from interior primitives: eps fill, interior set, U = cons(W), face halos
+        (material_fields_initializer.py:590-690)."""
+        lib = _native.load()
+        W = torch.full(self.shape, precision.get_eps(), dtype=torch.float64, device=self.device)
+        W[self.interior] = W_int.to(device=self.device, dtype=torch.float64)
+        U = torch.empty_like(W)
+        _native.check(lib.jfa_cons_from_prim(_native.ptr(W), _native.ptr(U), W[0].numel(), self.gamma, _native.F64,
+                                             _native.stream_ptr()), "jfa_cons_from_prim")
+        _native.check(lib.jfa_halo_update(_native.ptr(W), _native.ptr(U), self.grid, self.bc, self.gamma, _native.F64,
+                                          _native.stream_ptr()), "jfa_halo_update")
+        return W, U
+
+    def _stage(self, wi, ui, un, wo, uo, k) -> None:
+        P = _native.ptr
+        rc = _native.load().jfa_rk3_stage_fused(P(wi), P(ui), P(un), P(wo), P(uo), self.grid, self.scheme, self.bc, k,
+                                                P(self.dt), 0.0, 1.0, 0.0, P(None), P(self._ws), self._ws.numel(),
+                                                _native.stream_ptr())
+        _native.check(rc, "jfa_rk3_stage_fused")
+
+    def step(self, W: torch.Tensor, U: torch.Tensor, n: int = 1) -> None:
+        """n RK3 steps in place (simulation_manager.py:770-1047 at fixed dt)."""
+        (Wb, Ub), (Wc, Uc) = self._B, self._C
+        for _ in range(n):
+            self._stage(W, U, None, Wb, Ub, 0)
+            self._stage(Wb, Ub, U, Wc, Uc, 1)
+            self._stage(Wc, Uc, U, W, U, 2)  # U_out aliases U_n (cell-local read-before-write)
+        self.forward_steps += n
+
+    def _vjp(self, W_in, lam_out, lam_in, lam_n, k) -> None:
+        P = _native.ptr
+        rc = _native.load().jfa_rk3_stage_vjp(P(W_in), P(lam_out), P(lam_in), P(lam_n), self.grid, self.scheme,
+                                              self.bc, k, self.dt_value, P(self._aws), self._aws.numel(),
+                                              _native.stream_ptr())
+        _native.check(rc, "jfa_rk3_stage_vjp")
+
+    def step_vjp(self, W: torch.Tensor, U: torch.Tensor, lam: torch.Tensor) -> torch.Tensor:
+        """dL/dU_n from lam = dL/dU_{n+1} (interior of buffer-layout tensors); (W, U) is the state at
+        the start of the step and is not modified.  The step's stage states are recomputed."""
+        (Wb, Ub), (Wc, Uc) = self._B, self._C
+        self._stage(W, U, None, Wb, Ub, 0)
+        self._stage(Wb, Ub, U, Wc, Uc, 1)
+        l2, l1, l0 = self._lam
+        for t in (l2, l1, l0):
+            t.zero_()  # fill kernels (graph-safe), halos stay zero
+        self._vjp(Wc, lam, l2, l0, 2)
+        self._vjp(Wb, l2, l1, l0, 1)
+        self._vjp(W, l1, l0, None, 0)
+        return l0.clone()
+
+    # -- rollouts
+    def reverse(self, snaps: Dict[int, Tuple[torch.Tensor, torch.Tensor]], nsteps: int, snapshots: int,
+                lam: torch.Tensor) -> torch.Tensor:
+        """Reverse nsteps steps from the stored state snaps[0] (plus any first-level snapshots) with
+        at most `snapshots` extra stored states, following the binomial schedule."""
+        work = (torch.empty(self.shape, dtype=torch.float64, device=self.device),
+                torch.empty(self.shape, dtype=torch.float64, device=self.device))
+
+        def advance(src, n):
+            work[0].copy_(src[0])
+            work[1].copy_(src[1])
+            self.step(work[0], work[1], n)
+            return work
+
+        return reverse_rollout(snaps, nsteps, snapshots, lam, advance,
+                               lambda state, l: self.step_vjp(state[0], state[1], l),
+                               lambda state: (state[0].clone(), state[1].clone()))
+
+
+class RK3Rollout(torch.autograd.Function):
+    """W_T = rollout(W_0): nsteps fixed-dt RK3 steps from interior primitives W_0.
+
+    Forward stores snaps[0] and the first-level binomial snapshots; backward reverses the
+    rollout with ``stepper.reverse`` and maps dL/dU_0 to dL/dW_0."""
+
+    @staticmethod
+    def forward(ctx, W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: int) -> torch.Tensor:
+        W, U = stepper.fill(W0.detach())
+        snaps = {0: (W.clone(), U.clone())}
+        chain = set(first_level_snapshots(nsteps, snapshots))
+        for n in range(1, nsteps + 1):
+            stepper.step(W, U)
+            if n in chain:
+                snaps[n] = (W.clone(), U.clone())
+        ctx.stepper, ctx.nsteps, ctx.snapshots, ctx.snaps = stepper, nsteps, snapshots, snaps
+        ctx.W0 = W0.detach()
+        WT = W[stepper.interior].clone()
+        ctx.WT = WT
+        return WT
+
+    @staticmethod
+    def backward(ctx, gWT: torch.Tensor):
+        st: FixedDtStepper = ctx.stepper
+        lam = torch.zeros(st.shape, dtype=torch.float64, device=st.device)
+        lam[st.interior] = prim_vjp(ctx.WT, gWT.to(torch.float64), st.gamma)
+        lam = st.reverse(ctx.snaps, ctx.nsteps, ctx.snapshots, lam)
+        ctx.snaps = None
+        gW0 = cons_vjp(ctx.W0.to(torch.float64), lam[st.interior], st.gamma)
+        return gW0.to(ctx.W0.dtype), None, None, None
+
+
+def rollout(W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: Optional[int] = None) -> torch.Tensor:
+    """Differentiable fixed-dt rollout; snapshots defaults to storing every step's state."""
+    return RK3Rollout.apply(W0, stepper, int(nsteps), int(nsteps if snapshots is None else snapshots))
+
+
+def kinetic_energy(W: torch.Tensor, cell_volume: float) -> torch.Tensor:
+    """Domain-integrated kinetic energy sum(0.5 rho |u|^2) dV of interior primitives."""
+    return 0.5 * cell_volume * torch.sum(W[0] * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]))

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjaxfluids_amd.so")
-SOURCES = ["jfa_kernels.hip", "jfa_fused_f64.hip", "jfa_fused_f32.hip", "jfa_batch_f64.hip", "jfa_batch_f32.hip"]
+SOURCES = ["jfa_kernels.hip", "jfa_fused_f64.hip", "jfa_fused_f32.hip", "jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip"]
 ARCH = os.environ.get("JFA_OFFLOAD_ARCH", "gfx950")
 
 

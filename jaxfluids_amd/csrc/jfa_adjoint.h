@@ -1,0 +1,635 @@
+// jfa_adjoint.h -- reverse mode (vector-Jacobian products) of the RK3 stage at fixed dt for
+// primitive-variable WENO5 reconstruction with the HLLC or Rusanov Riemann solver.
+//
+// The forward stage is   U_out = c0 U_in + c1 U_n + ts R(U_in),   ts = m dt,
+// with R the flux divergence of prim(U_in) after the face-halo fill (RK3.py:27-62,
+// space_solver.py:266-314).  Given lam = dL/dU_out on the interior, one backward stage adds
+//   lam_in += c0 lam + (dW/dU)^T  halo^T  sum_faces (dF/dW_window)^T mu_f,
+//   lam_n  += c1 lam,           mu_f = ts (lam_{f} - lam_{f-1}) / dx   (face f between cells f-1, f)
+// where halo^T sends the cotangent of every face-halo cell back to the interior cell the halo
+// fill copied it from (with the symmetric-wall sign flip).
+//
+// Kernels mirror the forward sweeps, transposed:
+// * adj_march_kernel (every active axis but the innermost): lanes along the innermost axis,
+//   each thread walks the faces of a column and accumulates window cotangents in a 6-cell
+//   register window; each chunk owns a range of cells and evaluates every face that touches
+//   them (5 extra faces per chunk), so all writes are owned -- no atomics.
+// * adj_inner_kernel (innermost axis): one face per lane; a cell gathers the six window
+//   cotangents of the faces f = c-2..c+3 from lanes l..l-5 by wave shuffles; a wave owns 59
+//   cells (lanes 5..63).
+// * adj_update_kernel: halo^T, (dW/dU)^T and the stage blend per interior cell.
+// Face states are recomputed with the forward's own face_states, so branch choices (the sign
+// of S*, the min/max of the wave-speed estimates) are those of the forward evaluation.
+#pragma once
+
+#include "jfa_common.h"
+
+namespace jfa {
+
+// ------------------------------------------------------------------ WENO5 VJP
+// Adds vbar * d v / d(a..e) to g for v = WENO5(a, b, c, d, e), the reference's formulas
+// (weno5_base.py:34-50, weno5_js.py:39-58, weno5_z.py:38-59).
+template <int STENCIL>
+__device__ __forceinline__ void weno5_vjp(double a, double b, double c, double d, double e, double eps, double vbar,
+                                          double& ga, double& gb, double& gc, double& gd, double& ge) {
+    const double D0 = a - 2.0 * b + c, E0 = a - 4.0 * b + 3.0 * c;
+    const double D1 = b - 2.0 * c + d, E1 = b - d;
+    const double D2 = c - 2.0 * d + e, E2 = 3.0 * c - 4.0 * d + e;
+    const double b0 = 13.0 / 12.0 * D0 * D0 + 0.25 * E0 * E0;
+    const double b1 = 13.0 / 12.0 * D1 * D1 + 0.25 * E1 * E1;
+    const double b2 = 13.0 / 12.0 * D2 * D2 + 0.25 * E2 * E2;
+    const double p0 = (1.0 / 3.0) * a - (7.0 / 6.0) * b + (11.0 / 6.0) * c;
+    const double p1 = (-1.0 / 6.0) * b + (5.0 / 6.0) * c + (1.0 / 3.0) * d;
+    const double p2 = (1.0 / 3.0) * c + (5.0 / 6.0) * d - (1.0 / 6.0) * e;
+    constexpr double d0 = 0.1, d1 = 0.6, d2 = 0.3;
+    double al0, al1, al2, da0, da1, da2;  // alpha_k and d alpha_k / d beta_k
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, tau = 0.0;
+    if constexpr (STENCIL == JFA_WENO5_JS) {
+        const double q0 = 1.0 / (b0 * b0 + eps), q1 = 1.0 / (b1 * b1 + eps), q2 = 1.0 / (b2 * b2 + eps);
+        al0 = d0 * q0;
+        al1 = d1 * q1;
+        al2 = d2 * q2;
+        da0 = -2.0 * b0 * al0 * q0;
+        da1 = -2.0 * b1 * al1 * q1;
+        da2 = -2.0 * b2 * al2 * q2;
+    } else {
+        tau = fabs(b0 - b2);
+        r0 = 1.0 / (b0 + eps);
+        r1 = 1.0 / (b1 + eps);
+        r2 = 1.0 / (b2 + eps);
+        al0 = d0 * (1.0 + tau * r0);
+        al1 = d1 * (1.0 + tau * r1);
+        al2 = d2 * (1.0 + tau * r2);
+        da0 = -d0 * tau * r0 * r0;
+        da1 = -d1 * tau * r1 * r1;
+        da2 = -d2 * tau * r2 * r2;
+    }
+    const double iS = 1.0 / (al0 + al1 + al2);
+    const double w0 = al0 * iS, w1 = al1 * iS, w2 = al2 * iS;
+    const double v = w0 * p0 + w1 * p1 + w2 * p2;
+    // omega_k = alpha_k / S:  abar_k = vbar (p_k - v) / S
+    const double ab0 = vbar * (p0 - v) * iS, ab1 = vbar * (p1 - v) * iS, ab2 = vbar * (p2 - v) * iS;
+    double bb0 = ab0 * da0, bb1 = ab1 * da1, bb2 = ab2 * da2;
+    if constexpr (STENCIL == JFA_WENO5_Z) {
+        const double tb = ab0 * d0 * r0 + ab1 * d1 * r1 + ab2 * d2 * r2;
+        const double s = b0 > b2 ? 1.0 : (b0 < b2 ? -1.0 : 0.0);
+        bb0 += tb * s;
+        bb2 -= tb * s;
+    }
+    const double pb0 = vbar * w0, pb1 = vbar * w1, pb2 = vbar * w2;
+    ga += (1.0 / 3.0) * pb0;
+    gb += (-7.0 / 6.0) * pb0 + (-1.0 / 6.0) * pb1;
+    gc += (11.0 / 6.0) * pb0 + (5.0 / 6.0) * pb1 + (1.0 / 3.0) * pb2;
+    gd += (1.0 / 3.0) * pb1 + (5.0 / 6.0) * pb2;
+    ge += (-1.0 / 6.0) * pb2;
+    const double t0 = 13.0 / 6.0 * D0 * bb0, u0 = 0.5 * E0 * bb0;
+    ga += t0 + u0;
+    gb += -2.0 * t0 - 4.0 * u0;
+    gc += t0 + 3.0 * u0;
+    const double t1 = 13.0 / 6.0 * D1 * bb1, u1 = 0.5 * E1 * bb1;
+    gb += t1 + u1;
+    gc += -2.0 * t1;
+    gd += t1 - u1;
+    const double t2 = 13.0 / 6.0 * D2 * bb2, u2 = 0.5 * E2 * bb2;
+    gc += t2 + 3.0 * u2;
+    gd += -2.0 * t2 - 4.0 * u2;
+    ge += t2 + u2;
+}
+
+// ------------------------------------------------------------------ pointwise VJPs
+// U = cons(W) (equation_manager.py:93-101): adds (dU/dW)^T gU to gW.
+__device__ __forceinline__ void cons_vjp(const double W[5], const double gU[5], const Params& prm, double gW[5]) {
+    const double q2 = W[1] * W[1] + W[2] * W[2] + W[3] * W[3];
+    gW[0] += gU[0] + gU[1] * W[1] + gU[2] * W[2] + gU[3] * W[3] + gU[4] * 0.5 * q2;
+#pragma unroll
+    for (int j = 1; j < 4; ++j) gW[j] += (gU[j] + gU[4] * W[j]) * W[0];
+    gW[4] += gU[4] * prm.inv_gm1;
+}
+
+// W = prim(U) (equation_manager.py:164-171): returns (dW/dU)^T gW, written with W.
+__device__ __forceinline__ void prim_vjp(const double W[5], const double gW[5], const Params& prm, double gU[5]) {
+    const double ir = 1.0 / W[0];
+    const double q2 = W[1] * W[1] + W[2] * W[2] + W[3] * W[3];
+    gU[0] = gW[0] - (W[1] * gW[1] + W[2] * gW[2] + W[3] * gW[3]) * ir + prm.gm1 * 0.5 * q2 * gW[4];
+#pragma unroll
+    for (int j = 1; j < 4; ++j) gU[j] = gW[j] * ir - prm.gm1 * W[j] * gW[4];
+    gU[4] = prm.gm1 * gW[4];
+}
+
+// physical flux along AXIS (equation_manager.py:237-252): adds its VJP to gW, gU.
+template <int AXIS>
+__device__ __forceinline__ void physical_flux_vjp(const double W[5], const double U[5], const double gF[5],
+                                                  double gW[5], double gU[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    gU[v] += gF[0] + gF[1] * W[1] + gF[2] * W[2] + gF[3] * W[3];
+#pragma unroll
+    for (int j = 1; j < 4; ++j) gW[j] += gF[j] * U[v];
+    gW[4] += gF[v];
+    gW[v] += gF[4] * (U[4] + W[4]);
+    gU[4] += gF[4] * W[v];
+    gW[4] += gF[4] * W[v];
+}
+
+// c = sqrt(gamma p / rho) (ideal_gas.py:69-71)
+__device__ __forceinline__ void sound_speed_vjp(double c, double rho, double gamma, double gc, double& grho,
+                                                double& gp) {
+    gp += gc * gamma / (2.0 * c * rho);
+    grho -= gc * c / (2.0 * rho);
+}
+
+// Wave-speed estimates (signal_speeds.py): adds the VJP of (SL, SR) to u, a (sound speed), rho.
+// The branch of each min/max is the one the forward takes (first operand on ties).
+__device__ __forceinline__ void signal_speeds_vjp(int kind, double uL, double uR, double aL, double aR, double rL,
+                                                  double rR, double gSL, double gSR, double& guL, double& guR,
+                                                  double& gaL, double& gaR, double& grL, double& grR) {
+    switch (kind) {
+        case JFA_SS_ARITHMETIC: {
+            const double um = 0.5 * (uL + uR), am = 0.5 * (aL + aR);
+            double gum = 0.0, gam = 0.0;
+            if (um - am <= uL - aL) { gum += gSL; gam -= gSL; } else { guL += gSL; gaL -= gSL; }
+            if (um + am >= uR + aR) { gum += gSR; gam += gSR; } else { guR += gSR; gaR += gSR; }
+            guL += 0.5 * gum;
+            guR += 0.5 * gum;
+            gaL += 0.5 * gam;
+            gaR += 0.5 * gam;
+        } break;
+        case JFA_SS_RUSANOV: {
+            const double gsp = gSR - gSL;
+            if (fabs(uL) + aL >= fabs(uR) + aR) {
+                guL += gsp * (uL > 0.0 ? 1.0 : (uL < 0.0 ? -1.0 : 0.0));
+                gaL += gsp;
+            } else {
+                guR += gsp * (uR > 0.0 ? 1.0 : (uR < 0.0 ? -1.0 : 0.0));
+                gaR += gsp;
+            }
+        } break;
+        case JFA_SS_DAVIS:
+            if (uL - aL <= uR - aR) { guL += gSL; gaL -= gSL; } else { guR += gSL; gaR -= gSL; }
+            if (uL + aL >= uR + aR) { guL += gSR; gaL += gSR; } else { guR += gSR; gaR += gSR; }
+            break;
+        default: {  // EINFELDT
+            const double sL = sqrt(rL), sR = sqrt(rR);
+            const double od = 1.0 / (sL + sR);
+            const double eta2 = 0.5 * sL * sR * od * od;
+            const double t2 = sL * uL + sR * uR;
+            const double ub = t2 * od;
+            const double du = uR - uL;
+            const double t1 = sL * aL * aL + sR * aR * aR;
+            const double db = sqrt(t1 * od + eta2 * du * du);
+            double gub = 0.0, gdb = 0.0;
+            if (ub - db <= uL - aL) { gub += gSL; gdb -= gSL; } else { guL += gSL; gaL -= gSL; }
+            if (ub + db >= uR + aR) { gub += gSR; gdb += gSR; } else { guR += gSR; gaR += gSR; }
+            const double gQ = db > 0.0 ? gdb / (2.0 * db) : 0.0;
+            const double gt1 = gQ * od;
+            double god = gQ * t1;
+            const double geta2 = gQ * du * du;
+            const double gdu = gQ * 2.0 * eta2 * du;
+            guR += gdu;
+            guL -= gdu;
+            double gsL = gt1 * aL * aL, gsR = gt1 * aR * aR;
+            gaL += gt1 * 2.0 * sL * aL;
+            gaR += gt1 * 2.0 * sR * aR;
+            const double gt2 = gub * od;
+            god += gub * t2;
+            gsL += gt2 * uL;
+            guL += gt2 * sL;
+            gsR += gt2 * uR;
+            guR += gt2 * sR;
+            gsL += geta2 * 0.5 * sR * od * od;
+            gsR += geta2 * 0.5 * sL * od * od;
+            god += geta2 * sL * sR * od;
+            const double gsum = -god * od * od;
+            gsL += gsum;
+            gsR += gsum;
+            grL += gsL / (2.0 * sL);
+            grR += gsR / (2.0 * sR);
+        } break;
+    }
+}
+
+// HLLC star flux of side K (HLLC.py:41-78, the form of hllc_star_flux):
+//   F = phys(W, U) + Sc (u* - U),  d = SK - u, inv = 1/(SK - Ss), pre = rho d inv,
+//   u* = (pre, pre Ss | pre w_minor, inv (d E + (Ss - u)(rho d Ss + p)))
+// Adds its VJP for cotangent mu to gW, gU, gSK, gSs.
+template <int AXIS, bool LEFT>
+__device__ __forceinline__ void hllc_star_vjp(const double W[5], const double U[5], double SK, double Ss,
+                                              const double mu[5], double gW[5], double gU[5], double& gSK,
+                                              double& gSs) {
+    constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
+    const double u = W[v];
+    const double d = SK - u;
+    const double inv = 1.0 / (SK - Ss);
+    const double rd = W[0] * d;
+    const double pre = rd * inv;
+    const double A = Ss - u;
+    const double B = rd * Ss + W[4];
+    const double Q = d * U[4] + A * B;
+    double us[5];
+    us[0] = pre;
+    us[v] = pre * Ss;
+    us[m0] = pre * W[m0];
+    us[m1] = pre * W[m1];
+    us[4] = inv * Q;
+    const double Sc = LEFT ? fmin(SK, 0.0) : fmax(SK, 0.0);
+    double gSc = 0.0, gus[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        gSc += mu[i] * (us[i] - U[i]);
+        gus[i] = Sc * mu[i];
+        gU[i] -= Sc * mu[i];
+    }
+    if (LEFT ? SK < 0.0 : SK > 0.0) gSK += gSc;
+    const double gpre = gus[0] + gus[v] * Ss + gus[m0] * W[m0] + gus[m1] * W[m1];
+    gSs += gus[v] * pre;
+    gW[m0] += gus[m0] * pre;
+    gW[m1] += gus[m1] * pre;
+    double ginv = gus[4] * Q;
+    const double gQ = gus[4] * inv;
+    double gd = gQ * U[4];
+    gU[4] += gQ * d;
+    const double gA = gQ * B, gB = gQ * A;
+    double grd = gB * Ss;
+    gSs += gB * rd;
+    gW[4] += gB;
+    gSs += gA;
+    double gu = -gA;
+    grd += gpre * inv;
+    ginv += gpre * rd;
+    gW[0] += grd * d;
+    gd += grd * W[0];
+    const double gM = -ginv * inv * inv;
+    gSK += gM;
+    gSs -= gM;
+    gSK += gd;
+    gu -= gd;
+    gW[v] += gu;
+    physical_flux_vjp<AXIS>(W, U, mu, gW, gU);
+}
+
+// HLLC (HLLC.py:80-126): adds the VJP of the face flux for cotangent mu to gWL, gWR.
+template <int AXIS>
+__device__ __forceinline__ void hllc_vjp(const double WL[5], const double WR[5], const double UL[5],
+                                         const double UR[5], const double mu[5], const Params& prm, double gWL[5],
+                                         double gWR[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    const double g = prm.gamma;
+    const double cL = sqrt(g * WL[4] / WL[0]), cR = sqrt(g * WR[4] / WR[0]);
+    double SL, SR;
+    signal_speeds(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], WL[4], WR[4], g, SL, SR);
+    const double aL = WL[0] * (SL - WL[v]), aR = WR[0] * (SR - WR[v]);
+    const double num = (WR[4] - WL[4]) + (WL[v] * aL - WR[v] * aR);
+    const double den = aL - aR;
+    const double Ss = num / den;
+    double gUL[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gUR[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double gSL = 0.0, gSR = 0.0, gSs = 0.0;
+    if (Ss >= 0.0) {
+        double m[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) m[i] = Ss == 0.0 ? 0.5 * mu[i] : mu[i];
+        hllc_star_vjp<AXIS, true>(WL, UL, SL, Ss, m, gWL, gUL, gSL, gSs);
+    }
+    if (Ss <= 0.0) {
+        double m[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) m[i] = Ss == 0.0 ? 0.5 * mu[i] : mu[i];
+        hllc_star_vjp<AXIS, false>(WR, UR, SR, Ss, m, gWR, gUR, gSR, gSs);
+    }
+    // S* = num / den (signal_speeds.py:194-197)
+    const double gnum = gSs / den, gden = -gSs * Ss / den;
+    gWR[4] += gnum;
+    gWL[4] -= gnum;
+    gWL[v] += gnum * aL;
+    gWR[v] -= gnum * aR;
+    const double gaL = gnum * WL[v] + gden, gaR = -gnum * WR[v] - gden;
+    gWL[0] += gaL * (SL - WL[v]);
+    gSL += gaL * WL[0];
+    gWL[v] -= gaL * WL[0];
+    gWR[0] += gaR * (SR - WR[v]);
+    gSR += gaR * WR[0];
+    gWR[v] -= gaR * WR[0];
+    double gcL = 0.0, gcR = 0.0;
+    signal_speeds_vjp(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], gSL, gSR, gWL[v], gWR[v], gcL, gcR,
+                      gWL[0], gWR[0]);
+    sound_speed_vjp(cL, WL[0], g, gcL, gWL[0], gWL[4]);
+    sound_speed_vjp(cR, WR[0], g, gcR, gWR[0], gWR[4]);
+    cons_vjp(WL, gUL, prm, gWL);
+    cons_vjp(WR, gUR, prm, gWR);
+}
+
+// Rusanov (Rusanov.py:35-45)
+template <int AXIS>
+__device__ __forceinline__ void rusanov_vjp(const double WL[5], const double WR[5], const double UL[5],
+                                            const double UR[5], const double mu[5], const Params& prm,
+                                            double gWL[5], double gWR[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    const double g = prm.gamma;
+    const double cL = sqrt(g * WL[4] / WL[0]), cR = sqrt(g * WR[4] / WR[0]);
+    const bool left = fabs(WL[v]) + cL >= fabs(WR[v]) + cR;
+    const double alpha = left ? fabs(WL[v]) + cL : fabs(WR[v]) + cR;
+    double gUL[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gUR[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, half[5];
+    double galpha = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        half[i] = 0.5 * mu[i];
+        galpha -= 0.5 * mu[i] * (UR[i] - UL[i]);
+        gUR[i] -= 0.5 * alpha * mu[i];
+        gUL[i] += 0.5 * alpha * mu[i];
+    }
+    physical_flux_vjp<AXIS>(WL, UL, half, gWL, gUL);
+    physical_flux_vjp<AXIS>(WR, UR, half, gWR, gUR);
+    double gcL = 0.0, gcR = 0.0;
+    if (left) {
+        gWL[v] += galpha * (WL[v] > 0.0 ? 1.0 : (WL[v] < 0.0 ? -1.0 : 0.0));
+        gcL += galpha;
+    } else {
+        gWR[v] += galpha * (WR[v] > 0.0 ? 1.0 : (WR[v] < 0.0 ? -1.0 : 0.0));
+        gcR += galpha;
+    }
+    sound_speed_vjp(cL, WL[0], g, gcL, gWL[0], gWL[4]);
+    sound_speed_vjp(cR, WR[0], g, gcR, gWR[0], gWR[4]);
+    cons_vjp(WL, gUL, prm, gWL);
+    cons_vjp(WR, gUR, prm, gWR);
+}
+
+// VJP of one face flux, first part: cotangents of the reconstructed face states WL, WR.
+template <int AXIS, int STENCIL, int RIEMANN>
+__device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const double mu[5], const Params& prm,
+                                               double gWL[5], double gWR[5]) {
+    double wl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, wr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double WL[5], WR[5], UL[5], UR[5];
+    face_states<AXIS, JFA_PRIMITIVE, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        gWL[i] = 0.0;
+        gWR[i] = 0.0;
+    }
+    if constexpr (RIEMANN == JFA_HLLC) hllc_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
+    else rusanov_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
+}
+
+// Second part, one variable at a time: cotangents of the six window cells x[0..5] of that
+// variable (the left state reads cells -3..1 = window 0..4, the right state cells 2..-2 =
+// window 5..1).
+template <int STENCIL>
+__device__ __forceinline__ void window_vjp_var(const double x[6], double gl, double gr, double eps, double out[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = 0.0;
+    weno5_vjp<STENCIL>(x[0], x[1], x[2], x[3], x[4], eps, gl, out[0], out[1], out[2], out[3], out[4]);
+    weno5_vjp<STENCIL>(x[5], x[4], x[3], x[2], x[1], eps, gr, out[5], out[4], out[3], out[2], out[1]);
+}
+
+// Face f (right cell at buffer index br, stride sa along the face normal): cotangents of WL, WR
+// from a window loaded here, so the window is not held across the Riemann VJP.
+template <int AXIS, int STENCIL, int RIEMANN>
+__device__ __forceinline__ void face_state_vjp_at(const double* __restrict__ W, int64_t br, int64_t sa, int64_t fs,
+                                                  const double mu[5], const Params& prm, double gl[5], double gr[5]) {
+    double win[6][5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) win[k][i] = ld(W, i * fs + br + (int64_t)(k - 3) * sa);
+    face_state_vjp<AXIS, STENCIL, RIEMANN>(win, mu, prm, gl, gr);
+}
+
+// The six window cells of variable i of that face (reloaded; they are in cache).
+__device__ __forceinline__ void load_var6(const double* __restrict__ W, int64_t br, int64_t sa, int64_t fs, int i,
+                                          double x[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) x[k] = ld(W, i * fs + br + (int64_t)(k - 3) * sa);
+}
+
+// ------------------------------------------------------------------ kernels
+constexpr int kAdjBlock = 256;
+
+// cotangent of the face between cells f-1 and f along an axis of n cells: lam outside the
+// interior is zero (those cells are not updated by the stage)
+__device__ __forceinline__ void face_mu(const double* __restrict__ lam, int64_t b_right, int64_t sa, int64_t fs,
+                                        int f, int n, double scale, double mu[5]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const double lr = f < n ? ld(lam, i * fs + b_right) : 0.0;
+        const double ll = f > 0 ? ld(lam, i * fs + b_right - sa) : 0.0;
+        mu[i] = scale * (lr - ll);
+    }
+}
+
+__device__ __forceinline__ void add_cell(double* __restrict__ gW, int64_t fs, int64_t b, const double g[5]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        JFA_ACCESS(gW + i * fs + b, sizeof(double));
+        gW[i * fs + b] += g[i];
+    }
+}
+
+// Marching transpose sweep along A (not the innermost axis IN).  Chunk z of length L owns the
+// cells [-3 + zL, min(-3 + (z+1)L, N+3)) of its column (halo cells included) and evaluates
+// the faces f in [o0-2, o1+2] (clipped to 0..N) whose windows touch them; each face adds its
+// window cotangents to the owned cells in place (the six contributions a cell receives come
+// from consecutive faces of the same thread, so they hit cache and never race).
+template <int A, int IN, int STENCIL, int RIEMANN>
+__global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __restrict__ W,
+                                                              const double* __restrict__ lam, Geo G, Params prm,
+                                                              double ts, int L, double* __restrict__ gW) {
+    constexpr int O = 3 - A - IN;
+    const int iin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iin >= G.n[IN]) return;  // no barriers in this kernel
+    const int io = blockIdx.y;
+    const int N = G.n[A];
+    const int o0 = -3 + (int)blockIdx.z * L;
+    const int o1 = min(o0 + L, N + 3);
+    const int fa = max(o0 - 2, 0), fb = min(o1 + 2, N);
+    const int64_t sa = G.s[A], fs = G.fs;
+    const int64_t bcol = (int64_t)(io + G.off[O]) * G.s[O] + (int64_t)(iin + G.off[IN]) * G.s[IN] +
+                         (int64_t)G.off[A] * sa;  // cell 0 of the column along A
+    const double scale = ts * G.one_dx[A];
+    for (int f = fa; f <= fb; ++f) {
+        int64_t br = bcol + (int64_t)f * sa;
+        JFA_OPAQUE(br);  // keeps per-tap addresses from becoming loop-carried registers
+        double mu[5], gl[5], gr[5];
+        face_mu(lam, br, sa, fs, f, N, scale, mu);
+        face_state_vjp_at<A, STENCIL, RIEMANN>(W, br, sa, fs, mu, prm, gl, gr);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            double x6[6], o6[6];
+            load_var6(W, br, sa, fs, i, x6);
+            window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int cell = f - 3 + k;
+                if (cell >= o0 && cell < o1) {
+                    double* p = gW + i * fs + br + (int64_t)(k - 3) * sa;
+                    JFA_ACCESS(p, sizeof(double));
+                    *p += o6[k];
+                }
+            }
+        }
+    }
+}
+
+// Transpose sweep along the innermost axis IN (unit stride).  Wave w of a row evaluates the
+// faces F0 .. F0+63, F0 = -5 + 59 w (lane l: face F0 + l, zero outside 0..N) and lanes 5..63
+// own the cells F0 + l - 3, i.e. [-3 + 59 w, -3 + 59 w + 59) clipped to the row's N + 6 cells.
+template <int IN, int STENCIL, int RIEMANN>
+__global__ __launch_bounds__(kAdjBlock) void adj_inner_kernel(const double* __restrict__ W,
+                                                              const double* __restrict__ lam, Geo G, Params prm,
+                                                              double ts, int waves_per_row, int64_t nwaves,
+                                                              double* __restrict__ gW) {
+    constexpr int P = IN == 0 ? 1 : 0;
+    constexpr int Q = IN == 2 ? 1 : 2;
+    const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const bool vwave = gwave < nwaves;  // whole waves; shuffles below run in every lane of a wave
+    const int64_t row = vwave ? gwave / waves_per_row : 0;
+    const int w = vwave ? (int)(gwave - row * waves_per_row) : 0;
+    const int N = G.n[IN];
+    const int cP = (int)(row / G.n[Q]);
+    const int cQ = (int)(row - (int64_t)cP * G.n[Q]);
+    const int64_t fs = G.fs;
+    const int64_t brow = (int64_t)(cP + G.off[P]) * G.s[P] + (int64_t)(cQ + G.off[Q]) * G.s[Q] + G.off[IN];
+    const int f = -5 + 59 * w + lane;
+    const bool vface = vwave && f >= 0 && f <= N;
+    const int64_t br = brow + (vface ? f : 0);
+    double gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (vface) {
+        double mu[5];
+        face_mu(lam, br, 1, fs, f, N, ts * G.one_dx[IN], mu);
+        face_state_vjp_at<IN, STENCIL, RIEMANN>(W, br, 1, fs, mu, prm, gl, gr);
+    }
+    // cell c = f - 3 gathers window slot k of face c + 3 - k, held by lane - k
+    double s[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double x6[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0}, o6[6];  // lanes without a face: zero cotangents
+        if (vface) load_var6(W, br, 1, fs, i, x6);
+        window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
+        s[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s[i] += __shfl_up(o6[k], (unsigned)k, 64);
+    }
+    const int cell = f - 3;
+    if (vwave && lane >= 5 && cell <= N + 2) add_cell(gW, fs, brow + cell, s);
+}
+
+// Per interior cell: halo^T, (dW/dU)^T and the stage blend.
+__device__ __forceinline__ void halo_gather(const double* __restrict__ gW, const Geo& G, const BC6& bc,
+                                            const int c[3], int64_t b, double g[5]) {
+    const int nh = G.nh;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!G.active[a]) continue;
+        const int N = G.n[a];
+        const int ci = c[a];
+        const int64_t sa = G.s[a];
+        const int64_t bc0 = b - (int64_t)(ci + nh) * sa;
+        const int tl = bc.t[2 * a + 1], th = bc.t[2 * a];
+        // low halo (west/south/bottom) -- transpose of halo_scatter's low side
+        if (tl == JFA_BC_PERIODIC) {
+            if (ci >= N - nh)
+                for (int i = 0; i < 5; ++i) g[i] += ld(gW, i * G.fs + bc0 + (int64_t)(ci - N + nh) * sa);
+        } else if (tl == JFA_BC_ZEROGRADIENT) {
+            if (ci == 0)
+                for (int m = 0; m < nh; ++m)
+                    for (int i = 0; i < 5; ++i) g[i] += ld(gW, i * G.fs + bc0 + (int64_t)m * sa);
+        } else if (ci < nh) {
+            for (int i = 0; i < 5; ++i) {
+                const double h = ld(gW, i * G.fs + bc0 + (int64_t)(nh - 1 - ci) * sa);
+                g[i] += i == a + 1 ? -h : h;
+            }
+        }
+        // high halo (east/north/top)
+        if (th == JFA_BC_PERIODIC) {
+            if (ci < nh)
+                for (int i = 0; i < 5; ++i) g[i] += ld(gW, i * G.fs + bc0 + (int64_t)(N + nh + ci) * sa);
+        } else if (th == JFA_BC_ZEROGRADIENT) {
+            if (ci == N - 1)
+                for (int m = 0; m < nh; ++m)
+                    for (int i = 0; i < 5; ++i) g[i] += ld(gW, i * G.fs + bc0 + (int64_t)(N + nh + m) * sa);
+        } else if (ci >= N - nh) {
+            for (int i = 0; i < 5; ++i) {
+                const double h = ld(gW, i * G.fs + bc0 + (int64_t)(N + nh + (N - 1 - ci)) * sa);
+                g[i] += i == a + 1 ? -h : h;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void adj_update_kernel(const double* __restrict__ W,
+                                                            const double* __restrict__ lam,
+                                                            const double* __restrict__ gW, Geo G, Params prm,
+                                                            BC6 bc, double c0, double c1, double* __restrict__ lam_in,
+                                                            double* __restrict__ lam_n) {
+    int c[3];
+    if (!flat_index((int64_t)G.n[0] * G.n[1] * G.n[2], G.n[1], G.n[2], c[0], c[1], c[2])) return;
+    const int64_t b = (int64_t)(c[0] + G.off[0]) * G.s[0] + (int64_t)(c[1] + G.off[1]) * G.s[1] + (c[2] + G.off[2]);
+    const int64_t fs = G.fs;
+    double g[5], w[5], l[5], gu[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        g[i] = ld(gW, i * fs + b);
+        w[i] = ld(W, i * fs + b);
+        l[i] = ld(lam, i * fs + b);
+    }
+    halo_gather(gW, G, bc, c, b, g);
+    prim_vjp(w, g, prm, gu);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        JFA_ACCESS(lam_in + i * fs + b, sizeof(double));
+        lam_in[i * fs + b] += c0 * l[i] + gu[i];
+    }
+    if (lam_n) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            JFA_ACCESS(lam_n + i * fs + b, sizeof(double));
+            lam_n[i * fs + b] += c1 * l[i];
+        }
+    }
+}
+
+template <int UNUSED = 0>
+__global__ __launch_bounds__(kBlock) void adj_zero_kernel(double* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+// ------------------------------------------------------------------ host side
+template <int A, int IN, int STENCIL, int RIEMANN>
+static void launch_adj_march(const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
+                             double* gW, hipStream_t st) {
+    constexpr int O = 3 - A - IN;
+    int bx = ((G.n[IN] + 63) / 64) * 64;
+    if (bx > kAdjBlock) bx = kAdjBlock;
+    const int L = 16;
+    const int ncell = G.n[A] + 6;
+    dim3 grid((unsigned)((G.n[IN] + bx - 1) / bx), (unsigned)G.n[O], (unsigned)((ncell + L - 1) / L));
+    hipLaunchKernelGGL((adj_march_kernel<A, IN, STENCIL, RIEMANN>), grid, dim3(bx), 0, st, W, lam, G, prm, ts, L,
+                       gW);
+}
+
+template <int IN, int STENCIL, int RIEMANN>
+static void launch_adj_inner(const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
+                             double* gW, hipStream_t st) {
+    constexpr int P = IN == 0 ? 1 : 0;
+    constexpr int Q = IN == 2 ? 1 : 2;
+    const int waves_per_row = (G.n[IN] + 6 + 58) / 59;
+    const int64_t nwaves = (int64_t)waves_per_row * G.n[P] * G.n[Q];
+    const int64_t blocks = (nwaves + kAdjBlock / 64 - 1) / (kAdjBlock / 64);
+    hipLaunchKernelGGL((adj_inner_kernel<IN, STENCIL, RIEMANN>), dim3((unsigned)blocks), dim3(kAdjBlock), 0, st, W,
+                       lam, G, prm, ts, waves_per_row, nwaves, gW);
+}
+
+template <int STENCIL, int RIEMANN>
+static void adj_sweeps(const double* W, const double* lam, const Geo& G, const Params& prm, double ts, double* gW,
+                       hipStream_t st) {
+    const int IN = innermost_axis(G);
+    for (int A = 0; A < IN; ++A) {
+        if (!G.active[A]) continue;
+        if (A == 0 && IN == 1) launch_adj_march<0, 1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+        else if (A == 0) launch_adj_march<0, 2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+        else launch_adj_march<1, 2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+    }
+    if (IN == 0) launch_adj_inner<0, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+    else if (IN == 1) launch_adj_inner<1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+    else launch_adj_inner<2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+}
+
+}  // namespace jfa

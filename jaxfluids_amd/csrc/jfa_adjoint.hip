@@ -1,0 +1,67 @@
+// jfa_adjoint.hip -- C ABI of the reverse-mode RK3 stage (FP64).
+#include "jfa_adjoint.h"
+
+namespace jfa {
+
+template <int STENCIL>
+static void adj_sweeps_r(int riemann, const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
+                         double* gW, hipStream_t st) {
+    if (riemann == JFA_HLLC) adj_sweeps<STENCIL, JFA_HLLC>(W, lam, G, prm, ts, gW, st);
+    else adj_sweeps<STENCIL, JFA_RUSANOV>(W, lam, G, prm, ts, gW, st);
+}
+
+}  // namespace jfa
+
+using namespace jfa;
+
+extern "C" {
+
+size_t jfa_stage_vjp_workspace_bytes(const jfa_grid* grid) {
+    Geo G;
+    if (make_geo(grid, G)) return 0;
+    return (size_t)(5 * G.fs) * sizeof(double);
+}
+
+int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void* lam_n, const jfa_grid* grid,
+                      const jfa_scheme* scheme, const int32_t bc[6], int32_t stage, double dt, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+    Geo G;
+    int rc = make_geo(grid, G);
+    if (rc) return rc;
+    rc = check_scheme(scheme);
+    if (rc) return rc;
+    if (scheme->dtype != JFA_F64 || scheme->recon_var != JFA_PRIMITIVE) return JFA_EUNSUPPORTED;
+    if (scheme->riemann == JFA_HLLC && scheme->signal_speed == JFA_SS_TORO) return JFA_EUNSUPPORTED;
+    if (!W_in || !lam_out || !lam_in || !bc || !workspace || stage < 0 || stage > 2 || (stage > 0 && !lam_n))
+        return JFA_EINVAL;
+    if (lam_in == lam_out || lam_n == lam_out || (stage > 0 && lam_n == lam_in)) return JFA_EINVAL;
+    if (!(dt > 0.0) || workspace_bytes < (size_t)(5 * G.fs) * sizeof(double)) return JFA_EINVAL;
+    BC6 b6;
+    for (int f = 0; f < 6; ++f) {
+        const int a = f / 2;
+        b6.t[f] = bc[f];
+        if (!G.active[a]) continue;
+        if (bc[f] < 0 || bc[f] > 2) return JFA_EUNSUPPORTED;
+        if (bc[f] != JFA_BC_ZEROGRADIENT && G.nh > G.n[a]) return JFA_EINVAL;
+    }
+    // stage blend U_out = c0 U_in + c1 U_n + m dt R(U_in) (RK3.py:27-62)
+    const double c0 = stage == 0 ? 1.0 : (stage == 1 ? 0.25 : 2.0 / 3.0);
+    const double c1 = stage == 0 ? 0.0 : (stage == 1 ? 0.75 : 1.0 / 3.0);
+    const double m = stage == 0 ? 1.0 : (stage == 1 ? 0.25 : 2.0 / 3.0);
+    const Params prm = make_params(scheme);
+    hipStream_t st = (hipStream_t)stream;
+    double* gW = static_cast<double*>(workspace);
+    const int64_t nz = 5 * G.fs;
+    const int64_t zb = (nz + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(adj_zero_kernel<0>, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kBlock), 0, st, gW, nz);
+    const double* W = static_cast<const double*>(W_in);
+    const double* lam = static_cast<const double*>(lam_out);
+    if (scheme->stencil == JFA_WENO5_JS) adj_sweeps_r<JFA_WENO5_JS>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+    else adj_sweeps_r<JFA_WENO5_Z>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    hipLaunchKernelGGL(adj_update_kernel, flat_grid(ncell), dim3(kBlock), 0, st, W, lam, gW, G, prm, b6, c0, c1,
+                       static_cast<double*>(lam_in), stage > 0 ? static_cast<double*>(lam_n) : nullptr);
+    return check_launch();
+}
+
+}  // extern "C"

@@ -105,6 +105,9 @@ __device__ __forceinline__ bool flat_index(int64_t total, int e1, int e2, int& i
     return true;
 }
 
+// innermost active axis: the one with unit stride among the active axes
+static inline int innermost_axis(const Geo& G) { return G.active[2] ? 2 : (G.active[1] ? 1 : 0); }
+
 static inline dim3 flat_grid(int64_t total) { return dim3((unsigned)((total + kBlock - 1) / kBlock)); }
 
 // RK3.py:27-29 tables

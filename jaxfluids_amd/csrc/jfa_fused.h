@@ -229,8 +229,6 @@ __global__ __launch_bounds__(1024) void dt_from_parts_kernel(const double* __res
 }
 
 // ------------------------------------------------------------------ host side
-static inline int innermost_axis(const Geo& G) { return G.active[2] ? 2 : (G.active[1] ? 1 : 0); }
-
 static inline int64_t inner_blocks(const Geo& G) {
     const int IN = innermost_axis(G);
     const int P = IN == 0 ? 1 : 0, Q = IN == 2 ? 1 : 2;

@@ -1,4 +1,4 @@
-"""Builds tests/emu/_build/libjfa_emu.so: the library's batch-stage sources compiled for the
+"""Builds tests/emu/_build/libjfa_emu.so: the library's batch-stage and adjoint sources compiled for the
 host with g++ against the emulation header in tests/emu/hip/ (test infrastructure only; the
 product library is built by jaxfluids_amd/build.py with hipcc for gfx950)."""
 import os
@@ -9,8 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "jaxfluids_amd", "csrc")
 OUT = os.path.join(HERE, "_build", "libjfa_emu.so")
-SOURCES = [os.path.join(CSRC, "jfa_batch_f64.hip"), os.path.join(CSRC, "jfa_batch_f32.hip")]
-DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("jfa_batch.h", "jfa_common.h", "jfa_device.h")] + [
+SOURCES = [os.path.join(CSRC, f) for f in ("jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip")]
+DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("jfa_batch.h", "jfa_common.h", "jfa_device.h", "jfa_adjoint.h")] + [
     os.path.join(HERE, "emu_runtime.cpp"), os.path.join(HERE, "hip", "hip_runtime.h"), __file__]
 
 

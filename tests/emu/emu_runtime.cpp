@@ -145,6 +145,8 @@ uint64_t exchange(uint64_t v, int mode, int arg) {
         r = w->slot[(t_lane ^ arg) & 63];
     } else if (mode == 2) {
         r = w->slot[arg & 63];
+    } else if (mode == 4) {
+        if (t_lane - arg >= 0) r = w->slot[t_lane - arg];
     } else {
         for (int l = 0; l < 64; ++l)
             if (w->alive[l]) {

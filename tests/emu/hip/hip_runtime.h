@@ -7,6 +7,7 @@
 // against the ranges the test registered.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -46,7 +47,7 @@ hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int*, const void*, int, 
 
 namespace emu {
 void launch(dim3 grid, dim3 block, const std::function<void()>& body);
-uint64_t exchange(uint64_t v, int mode, int arg);  // mode 0 down, 1 xor, 2 index, 3 first live
+uint64_t exchange(uint64_t v, int mode, int arg);  // mode 0 down, 1 xor, 2 index, 3 first live, 4 up
 void syncthreads();
 void access(const void* p, size_t bytes);
 double rcp(double x);
@@ -75,6 +76,10 @@ inline T __shfl_down(T v, unsigned d, int = 64) {
     return emu::from_bits<T>(emu::exchange(emu::to_bits(v), 0, (int)d));
 }
 template <class T>
+inline T __shfl_up(T v, unsigned d, int = 64) {
+    return emu::from_bits<T>(emu::exchange(emu::to_bits(v), 4, (int)d));
+}
+template <class T>
 inline T __shfl_xor(T v, int m, int = 64) {
     return emu::from_bits<T>(emu::exchange(emu::to_bits(v), 1, m));
 }
@@ -94,5 +99,7 @@ inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v)
     }
     return old;
 }
+using std::max;  // HIP provides integer min/max in the global namespace
+using std::min;
 inline long long __double_as_longlong(double x) { return emu::from_bits<long long>(emu::to_bits(x)); }
 inline double __longlong_as_double(long long x) { return emu::from_bits<double>(emu::to_bits(x)); }

@@ -1,0 +1,87 @@
+"""Binomial checkpointing schedule of the rollout adjoint (host logic, no GPU).
+
+The dynamic-programming table is checked against brute-force recursion; the reverse sweep is
+run on a scalar toy map x -> f(x), whose exact derivative of x_N with respect to x_0 is the
+product of f'(x_n), and checked for the gradient, the number of forward steps it takes
+(T[n, s] from the table) and the number of states alive at once (<= s + 1).
+"""
+import functools
+import math
+
+import pytest
+
+from jaxfluids_amd.adjoint import binomial_table, first_level_snapshots, reverse_rollout
+
+
+@functools.lru_cache(maxsize=None)
+def brute(t, s):
+    if t <= 1:
+        return 0
+    if s == 0:
+        return t * (t - 1) // 2
+    return min(m + brute(t - m, s - 1) + brute(m, s) for m in range(1, t))
+
+
+def test_table_matches_brute_force():
+    T, M = binomial_table(40, 6)
+    for t in range(1, 41):
+        for s in range(0, 7):
+            assert T[t, s] == brute(t, s), (t, s)
+            if t > 1 and s > 0:
+                m = M[t, s]
+                assert m + T[t - m, s - 1] + T[m, s] == T[t, s]
+
+
+def test_table_is_binomial():
+    """Griewank & Walther's closed form: with S checkpoints (here the start state plus s free
+    slots, S = s + 1) the fewest forward steps to reverse t steps are r t - C(S + r, S + 1) for the
+    smallest r with C(S + r, S) >= t."""
+    T, _ = binomial_table(120, 5)
+    for s in range(1, 6):
+        S = s + 1
+        for t in range(1, 121):
+            r = 0
+            while math.comb(S + r, S) < t:
+                r += 1
+            assert T[t, s] == r * t - math.comb(S + r, S + 1), (t, s)
+
+
+f = lambda x: math.sin(x) + 0.5 * x
+df = lambda x: math.cos(x) + 0.5
+
+
+@pytest.mark.parametrize("n,s,prestore", [(1, 0, False), (7, 0, False), (20, 1, False), (37, 3, False),
+                                          (64, 4, True), (50, 50, True), (23, 2, True)])
+def test_reverse_sweep_on_a_scalar_map(n, s, prestore):
+    xs = [0.3]
+    for _ in range(n):
+        xs.append(f(xs[-1]))
+    exact = 1.0
+    for x in xs[:-1]:
+        exact *= df(x)
+    snaps = {0: xs[0]}
+    if prestore:
+        for p in first_level_snapshots(n, s):
+            snaps[p] = xs[p]
+    pre = len(snaps) - 1
+    steps = [0]
+    alive = [len(snaps)]
+
+    def advance(x, k):
+        steps[0] += k
+        for _ in range(k):
+            x = f(x)
+        return x
+
+    def store(x):
+        alive[0] = max(alive[0], len(snaps) + 1)
+        return x
+
+    grad = reverse_rollout(snaps, n, s, 1.0, advance, lambda x, lam: lam * df(x), store)
+    assert abs(grad - exact) <= 1e-12 * abs(exact)
+    assert alive[0] <= s + 1
+    T, _ = binomial_table(n, s)
+    if not prestore:
+        assert steps[0] == T[n, s]
+    else:
+        assert steps[0] <= T[n, s] and pre == len(first_level_snapshots(n, s))

@@ -1,0 +1,173 @@
+"""GPU checks of the reverse mode (jfa_rk3_stage_vjp, FixedDtStepper, RK3Rollout).
+
+* stage VJP on the device: dot-product identity against central differences of the oracle's
+  stage map (as the CPU emulation test does for the same source);
+* rollout gradient (torch.autograd through RK3Rollout) against Richardson central differences
+  of the oracle rollout (small case) and of the device forward (Kelvin-Helmholtz 64^2, 40 steps,
+  the 1e-6 bar of the adjoint workload);
+* binomial checkpointing: the gradient with 3 snapshot slots equals the stored-every-step
+  gradient bit for bit, and the forward steps taken equal the schedule's count.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import jxf_oracle as o
+from test_emu_adjoint import CASES, FACES, _rollout_states, richardson, smooth_state, stage_map
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def dev(x):
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device="cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("case", [0, 1, 2, 6])
+@pytest.mark.parametrize("stage", [0, 1, 2])
+def test_stage_vjp_device_dot_product(case, stage):
+    from jaxfluids_amd import _native
+
+    say(f"[stage vjp] case {case} stage {stage}")
+    cells, bcs, (stencil, riemann, speed) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann=riemann, signal_speed=speed,
+                bcs=dict(zip(FACES, bcs)))
+    rng = np.random.default_rng(2000 + 10 * case + stage)
+    W0 = smooth_state(dom, 60 + case)
+    Uint = o.cons_from_prim(W0, s.gamma)
+    Un_int = Uint * (1.0 + 0.01 * rng.standard_normal(Uint.shape))
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    f = stage_map(dom, s, stage, dt, Un_int)
+    _, W_in = f(Uint)
+    lam_int = rng.standard_normal(Uint.shape)
+    lam_out = np.zeros((5,) + dom.shape)
+    lam_out[dom.interior] = lam_int
+    grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
+    scheme = _native.make_scheme("PRIMITIVE", stencil, riemann, speed, "ARITHMETIC", _native.F64, 1.4,
+                                 s.eps_stencil)
+    lib = _native.load()
+    ws = torch.empty(lib.jfa_stage_vjp_workspace_bytes(grid), dtype=torch.uint8, device="cuda")
+    Wd, ld = dev(W_in), dev(lam_out)
+    li, ln = torch.zeros_like(ld), torch.zeros_like(ld)
+    P = _native.ptr
+    rc = lib.jfa_rk3_stage_vjp(P(Wd), P(ld), P(li), P(ln if stage else None), grid, scheme,
+                               _native.bc_array(s.bcs), stage, dt, P(ws), ws.numel(), _native.stream_ptr())
+    _native.check(rc, "jfa_rk3_stage_vjp")
+    g = host(li)[dom.interior]
+    for trial in range(2):
+        d = rng.standard_normal(Uint.shape) * np.abs(Uint).mean(axis=tuple(range(1, Uint.ndim)), keepdims=True)
+        fd = richardson(f, Uint, d, 1e-7)
+        lhs, rhs = float(np.sum(lam_int * fd)), float(np.sum(g * d))
+        assert abs(lhs - rhs) <= 1e-8 * np.sum(np.abs(lam_int * fd)), (trial, lhs, rhs)
+    if stage:
+        c1 = o.RK3_CONS_MULTIPLIER[stage - 1][1]
+        assert np.allclose(host(ln)[dom.interior], c1 * lam_int, rtol=0, atol=0)
+
+
+def _kh_stepper(n, dt):
+    from jaxfluids_amd.adjoint import FixedDtStepper
+
+    return FixedDtStepper((n, n, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), {f: "PERIODIC" for f in FACES}, dt,
+                          nh=3, stencil="WENO5-JS", riemann="HLLC", signal_speed="EINFELDT")
+
+
+def test_rollout_gradient_against_oracle_fd():
+    """The small rollout of the CPU chain test, now through RK3Rollout on the device."""
+    from jaxfluids_amd.adjoint import kinetic_energy, rollout
+    from jaxfluids_amd.workloads import kelvin_helmholtz_primitives
+
+    say("[rollout vs oracle fd] 12^2 x 3 steps")
+    n, nsteps = 12, 3
+    dom = o.Domain((n, n, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), nh=3)
+    s = o.Setup(gamma=1.4, stencil="WENO5-JS", variable="PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT",
+                bcs={f: "PERIODIC" for f in FACES})
+    W0 = kelvin_helmholtz_primitives(n, seed=3, thickness=0.08)
+    W0[3] = 0.05 * np.sin(2 * np.pi * dom.cell_centers()[0])[:, None, None] + 0.02
+    dt = 0.5 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    dV = dom.dx[0] * dom.dx[1]
+    st = _kh_stepper(n, dt)
+    W0d = dev(W0).requires_grad_(True)
+    L = kinetic_energy(rollout(W0d, st, nsteps), dV)
+    L.backward()
+    g = host(W0d.grad)
+
+    def loss(Wi):
+        WT, _ = _rollout_states(Wi, dom, s, dt, nsteps)
+        Wt = WT[dom.interior]
+        return 0.5 * dV * float(np.sum(Wt[0] * (Wt[1] ** 2 + Wt[2] ** 2 + Wt[3] ** 2)))
+
+    assert abs(float(L) - loss(W0)) <= 1e-12 * abs(float(L))
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal(W0.shape) * np.array([0.1, 0.05, 0.05, 0.05, 0.0])[:, None, None, None]
+    D = lambda h: (loss(W0 + h * d) - loss(W0 - h * d)) / (2 * h)
+    fd = (4.0 * D(0.5e-6) - D(1e-6)) / 3.0
+    an = float(np.sum(g * d))
+    assert abs(fd - an) <= 1e-7 * abs(an), (fd, an)
+
+
+def test_kelvin_helmholtz_gradient_against_device_fd():
+    """Kelvin-Helmholtz 64^2, 40 fixed-dt steps: d KE / d (initial velocity perturbation)
+    against Richardson central differences of the device forward, relative error <= 1e-6."""
+    from jaxfluids_amd.adjoint import kinetic_energy, rollout
+    from jaxfluids_amd.workloads import kelvin_helmholtz_primitives
+
+    say("[kh gradient vs device fd] 64^2 x 40 steps")
+    n, nsteps = 64, 40
+    W0 = kelvin_helmholtz_primitives(n, seed=1)
+    dom = o.Domain((n, n, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), nh=3)
+    s = o.Setup(gamma=1.4, stencil="WENO5-JS", variable="PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT",
+                bcs={f: "PERIODIC" for f in FACES})
+    dt = 0.5 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    dV = 1.0 / n ** 2
+    st = _kh_stepper(n, dt)
+    W0d = dev(W0).requires_grad_(True)
+    L = kinetic_energy(rollout(W0d, st, nsteps, snapshots=8), dV)
+    L.backward()
+    g = W0d.grad.detach()
+    rng = np.random.default_rng(9)
+    d = np.zeros_like(W0)
+    d[1:3] = 1e-2 * rng.standard_normal((2,) + W0.shape[1:])  # velocity perturbation field
+    dd = dev(d)
+
+    def loss(h):
+        with torch.no_grad():
+            return float(kinetic_energy(rollout(dev(W0) + h * dd, st, nsteps, snapshots=0), dV))
+
+    D = lambda h: (loss(h) - loss(-h)) / (2 * h)
+    fd = (4.0 * D(0.5e-4) - D(1e-4)) / 3.0
+    an = float(torch.sum(g * dd))
+    say(f"  fd {fd:.15e} adjoint {an:.15e} rel {abs(fd - an) / abs(an):.2e}")
+    assert abs(fd - an) <= 1e-6 * abs(an)
+
+
+def test_binomial_checkpointing_is_exact():
+    """3 snapshot slots vs one per step: the same gradient bit for bit (the recomputed forward
+    steps are the same kernels on the same inputs) and the schedule's forward-step count."""
+    from jaxfluids_amd.adjoint import binomial_table, first_level_snapshots, kinetic_energy, rollout
+    from jaxfluids_amd.workloads import kelvin_helmholtz_primitives
+
+    say("[checkpointing] 32^2 x 30 steps, 3 slots vs all")
+    n, nsteps = 32, 30
+    W0 = kelvin_helmholtz_primitives(n, seed=2)
+    grads = []
+    for snaps in (nsteps, 3):
+        st = _kh_stepper(n, 2e-3)
+        W0d = dev(W0).requires_grad_(True)
+        L = kinetic_energy(rollout(W0d, st, nsteps, snapshots=snaps), 1.0 / n ** 2)
+        before = st.forward_steps
+        L.backward()
+        grads.append(host(W0d.grad))
+        T, _ = binomial_table(nsteps, snaps)
+        chain = first_level_snapshots(nsteps, snaps)
+        # the forward sweep already made the first-level snapshots (the last at chain[-1])
+        assert st.forward_steps - before == T[nsteps, snaps] - (chain[-1] if chain else 0)
+    assert np.array_equal(grads[0], grads[1])
