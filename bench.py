@@ -16,6 +16,8 @@ cpu_baseline: the C/OpenMP restatement (oracle/c) of the same step on the host c
 timed on a bounded sample of the same workload.
 ensemble: the batched-ensemble workload (64 uneven 2-D Riemann problems, one launch per RK3
 stage for all cases) against plain per-case launches of the same stages.
+adjoint: the Kelvin-Helmholtz 256^2 adjoint (500 fixed-dt steps): backward / forward wall time
+and a finite-difference check of the gradient.
 """
 from __future__ import annotations
 
@@ -268,6 +270,83 @@ def bench_ensemble(steps: int, warmup: int, ncases: int = 64, seed: int = 0, nh:
     }
 
 
+def bench_adjoint(nsteps: int = 500, n: int = 256, slots: int = 20, fd_check: bool = True):
+    """Adjoint of the 2-D Kelvin-Helmholtz workload: 256^2 periodic, WENO5-JS + HLLC-Einfeldt + RK3,
+    primitive reconstruction, fixed dt (CFL 0.5 of the initial state), 500 steps, FP64; loss =
+    domain-integrated kinetic energy at t_end; gradient w.r.t. the initial state (its velocity
+    part is the gradient w.r.t. the seeded perturbation field).  Reports forward and backward
+    wall times with every step stored and with binomial checkpointing in `slots` slots, and the
+    gradient along a random velocity perturbation against Richardson central differences of the
+    forward."""
+    import torch
+
+    from jaxfluids_amd.adjoint import FixedDtStepper, binomial_table, kinetic_energy, rollout
+    from jaxfluids_amd.workloads import kelvin_helmholtz_primitives
+    from jaxfluids_amd import _native
+
+    W0 = kelvin_helmholtz_primitives(n, seed=0)
+    faces = {f: "PERIODIC" for f in _native.FACE_LOCATIONS}
+    # fixed dt: CFL 0.5 of the initial state, sum over the two axes of (|u_i| + c) (time_step_size.py:103-109)
+    c = np.sqrt(1.4 * W0[4] / W0[0])
+    dt = 0.5 * (1.0 / n) / float(np.max(np.abs(W0[1]) + c + np.abs(W0[2]) + c))
+    st = FixedDtStepper((n, n, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), faces, dt, nh=3, stencil="WENO5-JS",
+                        riemann="HLLC", signal_speed="EINFELDT")
+    dV = 1.0 / n ** 2
+    W0d = torch.as_tensor(W0, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+
+    # warm up kernels and caches
+    with torch.no_grad():
+        rollout(W0d, st, 5, 0)
+    x = W0d.clone().requires_grad_(True)
+    kinetic_energy(rollout(x, st, 5), dV).backward()
+
+    with torch.no_grad():
+        t_fwd, WT = timed(lambda: rollout(W0d, st, nsteps, 0))
+    L0 = float(kinetic_energy(WT, dV))
+    res = {}
+    grads = {}
+    for name, sl in (("store_every_step", nsteps), ("binomial", slots)):
+        x = W0d.clone().requires_grad_(True)
+        f0 = st.forward_steps
+        t_f, out = timed(lambda: kinetic_energy(rollout(x, st, nsteps, sl), dV))
+        f1 = st.forward_steps
+        t_b, _ = timed(lambda: out.backward())
+        T, _ = binomial_table(nsteps, sl)
+        res[name] = {"snapshot_slots": sl, "forward_s": round(t_f, 4), "backward_s": round(t_b, 4),
+                     "backward_over_forward": round(t_b / t_fwd, 3),
+                     "recomputed_steps": int(st.forward_steps - f1), "schedule_steps": int(T[nsteps, sl])}
+        grads[name] = x.grad.detach()
+    same = bool(torch.equal(grads["store_every_step"], grads["binomial"]))
+    out = {"workload": f"adjoint of 2-D Kelvin-Helmholtz {n}^2 periodic, WENO5-JS PRIMITIVE + HLLC-Einfeldt + RK3, "
+                       f"fixed dt {dt:.6e}, {nsteps} steps, FP64, loss = kinetic energy at t_end",
+           "forward_s": round(t_fwd, 4), "ms_per_forward_step": round(t_fwd / nsteps * 1e3, 4),
+           "adjoint_over_forward": res["store_every_step"]["backward_over_forward"],
+           "modes": res, "checkpointed_gradient_identical": same, "loss": L0}
+    if fd_check:
+        rng = np.random.default_rng(1)
+        d = np.zeros_like(W0)
+        d[1:3] = 1e-2 * rng.standard_normal((2,) + W0.shape[1:])
+        dd = torch.as_tensor(d, dtype=torch.float64, device="cuda")
+
+        def loss(h):
+            with torch.no_grad():
+                return float(kinetic_energy(rollout(W0d + h * dd, st, nsteps, 0), dV))
+
+        D = lambda h: (loss(h) - loss(-h)) / (2 * h)
+        fd = (4.0 * D(0.5e-4) - D(1e-4)) / 3.0
+        an = float(torch.sum(grads["store_every_step"] * dd))
+        out["fd_check"] = {"direction": "random velocity perturbation field (u, v), amplitude 1e-2",
+                           "richardson_fd": fd, "adjoint": an, "rel_error": abs(fd - an) / abs(an)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,9 +358,10 @@ def main():
     ap.add_argument("--variable", default="CHAR-PRIMITIVE")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("tgv", "ensemble", "all"), default="all",
-                    help="tgv: the headline line only; ensemble: only the batched-ensemble measurement; "
-                         "all: headline line with the ensemble measurement attached")
+    ap.add_argument("--workload", choices=("tgv", "ensemble", "adjoint", "all"), default="all",
+                    help="tgv: the headline line only; ensemble / adjoint: only that measurement; "
+                         "all: headline line with the ensemble and adjoint measurements attached")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device-to-device copy of GIB GiB (5 launches): the reference "
@@ -309,6 +389,9 @@ def main():
         return
     if args.workload == "ensemble":
         print(json.dumps(bench_ensemble(args.ensemble_steps, 5)), flush=True)
+        return
+    if args.workload == "adjoint":
+        print(json.dumps(bench_adjoint(args.adjoint_steps)), flush=True)
         return
     case, num = tgv_setups(args.n, args.nh, args.dtype, args.variable)
     im = InputManager(case, num)
@@ -379,6 +462,8 @@ def main():
     torch.cuda.empty_cache()
     if args.workload == "all":
         out["ensemble"] = bench_ensemble(args.ensemble_steps, 5)
+        torch.cuda.empty_cache()
+        out["adjoint"] = bench_adjoint(args.adjoint_steps)
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.n, args.nh, args.variable if args.variable != "CHAR-CONSERVATIVE"
