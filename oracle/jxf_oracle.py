@@ -508,6 +508,33 @@ RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov}
 # Setup + high-order Godunov flux
 # --------------------------------------------------------------------------
 @dataclass
+class Transport:
+    """Transport properties of the dissipative fluxes (materials/single_materials/material.py:46-130).
+
+    Dynamic viscosity mu(T) = mu_ref (T / T_ref)^mu_exp -- the power law a CUSTOM lambda
+    expresses (material.py:96-97); mu_exp = 0 is a constant viscosity.  Thermal conductivity
+    kappa = cp mu / Pr (PRANDTL model, material.py:115-116) with cp = gamma R / (gamma - 1)
+    (ideal_gas.py:32); prandtl None switches the heat flux off.  T = p / (rho R) (ideal_gas.py:81-83).
+    """
+
+    R: float = 1.0
+    mu_ref: float = 0.0
+    T_ref: float = 1.0
+    mu_exp: float = 0.0
+    bulk_viscosity: float = 0.0
+    prandtl: Optional[float] = None
+
+    def mu(self, T):
+        return self.mu_ref * (T / self.T_ref) ** self.mu_exp if self.mu_exp != 0.0 else self.mu_ref * np.ones_like(T)
+
+    def cp(self, gamma):
+        return gamma / (gamma - 1.0) * self.R
+
+    def kappa(self, T, gamma):
+        return self.cp(gamma) * self.mu(T) / self.prandtl
+
+
+@dataclass
 class Setup:
     """Subset of NumericalSetup / CaseSetup on the hot path."""
 
@@ -521,6 +548,7 @@ class Setup:
     fixed_dt: Optional[float] = None
     bcs: Dict[str, str] = field(default_factory=lambda: {f: "PERIODIC" for f in FACE_LOCATIONS})
     eps_stencil: float = SPATIAL_STENCIL_EPS
+    transport: Optional[Transport] = None     # viscous (and heat) fluxes when set
 
 
 def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
@@ -581,11 +609,96 @@ def flux_divergence(F, one_dx, axis):
     return 0.0 + one_dx * (F[lo] - F[hi])
 
 
+# --------------------------------------------------------------------------
+# Dissipative fluxes, second-order central (solvers/source_term_solver.py:188-350,414-448,
+# 504-580; stencils central_2.py, deriv_face_2.py, deriv_center_2.py).  Periodic domains: the
+# primitive buffer is rebuilt from the interior with wrap padding, which fills face, edge and
+# vertex halos exactly as the reference's periodic halo update does for viscous runs
+# (halo_manager.py:125-129).
+# --------------------------------------------------------------------------
+def wrap_halos(W, dom: Domain):
+    """Buffer whose halos (faces, edges, vertices) are periodic copies of the interior."""
+    Wi = W[dom.interior]
+    pad = [(0, 0)] + [(dom.nh, dom.nh) if i in dom.active else (0, 0) for i in range(3)]
+    return np.pad(Wi, pad, mode="wrap")
+
+
+def _face_cells(buf, dom: Domain, axis: int, k: int, other: Optional[int] = None, ok: int = 0):
+    """Cells f + k (k = -1: left of face f, 0: right) of every face f along axis, shifted by ok
+    along `other`; face-array shape (shift_slice with an extra transverse offset)."""
+    n = dom.nh
+    sl = [dom.dom(0), dom.dom(1), dom.dom(2)]
+    hi = -n + k + 1
+    sl[axis] = slice(n + k, None if hi == 0 else hi)
+    if other is not None and ok != 0:
+        sl[other] = slice(n + ok, -n + ok)
+    return buf[(Ellipsis,) + tuple(sl)]
+
+
+def velocity_gradient_at_faces(W, dom: Domain, axis: int):
+    """compute_velocity_gradient (source_term_solver.py:414-448) with
+    compute_xi_derivatives_at_xj (:538-580): g[k][i] = d u_k / d x_i at the faces along axis;
+    i == axis: (u_R - u_L)/dx (deriv_face_2); i != axis: the central derivative at the two
+    adjacent cells (deriv_center_2), averaged to the face (central_2); inactive i: 0."""
+    L = _face_cells(W, dom, axis, -1)
+    g = [[None] * 3 for _ in range(3)]
+    for i in range(3):
+        for k in range(3):
+            if i not in dom.active:
+                g[k][i] = np.zeros_like(L[0])
+            elif i == axis:
+                g[k][i] = (_face_cells(W, dom, axis, 0)[1 + k] - L[1 + k]) * dom.one_dx[i]
+            else:
+                h = 0.5 * dom.one_dx[i]
+                dL = (_face_cells(W, dom, axis, -1, i, 1)[1 + k] - _face_cells(W, dom, axis, -1, i, -1)[1 + k]) * h
+                dR = (_face_cells(W, dom, axis, 0, i, 1)[1 + k] - _face_cells(W, dom, axis, 0, i, -1)[1 + k]) * h
+                g[k][i] = 0.5 * (dL + dR)
+    return g
+
+
+def dissipative_flux_xi(W, dom: Domain, axis: int, s: Setup):
+    """Viscous flux (compute_viscous_flux_xi :264-350, compute_tau :504-536) and heat flux
+    (compute_heat_flux_xi :188-262) on the faces along axis, as the amounts added to the
+    convective face flux (space_solver.py:565-587): momentum -tau_axis,k, energy
+    -sum_k tau_axis,k u_k + q with q = -kappa dT/dx_axis."""
+    tr = s.transport
+    assert all(s.bcs[f] == "PERIODIC" for f in FACE_LOCATIONS if FACE_AXIS[f] in dom.active), \
+        "dissipative fluxes: periodic domains"
+    Wp = wrap_halos(W, dom)
+    L, R = _face_cells(Wp, dom, axis, -1), _face_cells(Wp, dom, axis, 0)
+    T_L, T_R = L[4] / (L[0] * tr.R), R[4] / (R[0] * tr.R)
+    T_f = 0.5 * (T_L + T_R)                                    # central_2.py reconstruct_xi
+    T_f = np.where(T_f <= 0.0, EPS64, T_f)                     # source_term_solver.py:301
+    mu1 = tr.mu(T_f)
+    mu2 = tr.bulk_viscosity - 2.0 / 3.0 * mu1                  # :527
+    g = velocity_gradient_at_faces(Wp, dom, axis)
+    tau = [np.zeros_like(T_f) for _ in range(3)]
+    for k in range(3):
+        if axis in dom.active and k in dom.active:
+            tau[k] = mu1 * (g[axis][k] + g[k][axis])
+    tau[axis] = tau[axis] + mu2 * sum(g[k][k] for k in dom.active)
+    u_f = [0.5 * (L[1 + k] + R[1 + k]) for k in range(3)]     # central_2 of the velocity
+    vel_tau = 0.0
+    for k in dom.active:
+        vel_tau = vel_tau + tau[k] * u_f[k]
+    D = np.zeros((5,) + T_f.shape)
+    for k in range(3):
+        D[1 + k] = -tau[k]
+    D[4] = -vel_tau
+    if tr.prandtl is not None:
+        q = -tr.kappa(T_f, s.gamma) * ((T_R - T_L) * dom.one_dx[axis])   # deriv_face_2
+        D[4] = D[4] + q
+    return D
+
+
 def compute_rhs(W, U, dom: Domain, s: Setup):
-    """SpaceSolver.compute_rhs (space_solver.py:266,289-314): 0.0 + rhs_x + rhs_y + rhs_z."""
+    """SpaceSolver.compute_rhs (space_solver.py:266,289-314): 0.0 + rhs_x + rhs_y + rhs_z;
+    with transport set, each axis' face flux carries the dissipative part (:565-587)."""
     rhs = 0.0
     for axis in dom.active:
         F = compute_flux_xi(W, U, dom, axis, s)
+        if s.transport is not None:
+            F = F + dissipative_flux_xi(W, dom, axis, s)
         rhs = rhs + flux_divergence(F, dom.one_dx[axis], axis)
     return rhs
 
@@ -675,6 +788,12 @@ def compute_dt(W, dom: Domain, s: Setup, eps=EPS64):
     for i in dom.active:
         abs_velocity = abs_velocity + (np.abs(Wi[i + 1]) + c)
     dt = dom.dx_min / (np.max(abs_velocity) + eps)
+    tr = s.transport
+    if tr is not None:  # time_step_size.py:112-135
+        T = Wi[4] / (Wi[0] * tr.R)
+        dt = min(dt, 3.0 / 14.0 * dom.dx_min ** 2 / (np.max(tr.mu(T) / Wi[0]) + eps))
+        if tr.prandtl is not None:
+            dt = min(dt, 0.1 * dom.dx_min ** 2 / (np.max(tr.kappa(T, s.gamma) / (Wi[0] * tr.cp(s.gamma))) + eps))
     return dt * s.cfl
 
 

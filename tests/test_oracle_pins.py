@@ -407,3 +407,135 @@ def test_sod_against_exact_solution(variable):
     # wave-speed mistake anywhere on the path moves them by orders of magnitude.
     l1 = [np.mean(np.abs(Wi[i] - ex[k])) for i, k in ((0, 0), (1, 1), (4, 2))]
     assert l1[0] < 2.5e-3 and l1[1] < 5e-3 and l1[2] < 2e-3, l1
+
+
+# ------------------------------------------------------------------ dissipative fluxes (config 5)
+def _visc_rhs(W, dom, s):
+    """momentum/energy rhs of the dissipative face fluxes alone"""
+    rhs = 0.0
+    for axis in dom.active:
+        rhs = rhs + o.flux_divergence(o.dissipative_flux_xi(W, dom, axis, s), dom.one_dx[axis], axis)
+    return rhs
+
+
+def _periodic_setup(transport):
+    return o.Setup(bcs={f: "PERIODIC" for f in o.FACE_LOCATIONS}, transport=transport)
+
+
+def test_viscous_momentum_matches_fourier_symbols():
+    """Constant mu and bulk viscosity, periodic 3-D grid with unequal spacings: the discrete
+    momentum operator equals the product of the stencils' Fourier symbols -- face derivative
+    (1 - e^{-i t})/dx, central derivative i sin(t)/dx, face average (1 + e^{-i t})/2, flux
+    divergence (1 - e^{i t})/dx -- computed independently with FFTs."""
+    dom = o.Domain((8, 6, 10), ((0, 1.0), (0, 0.7), (0, 1.3)), nh=3)
+    rng = np.random.default_rng(11)
+    W = np.ones((5,) + dom.shape)
+    W[1:4][(slice(None),) + dom.interior[1:]] = rng.standard_normal((3,) + dom.cells)
+    tr = o.Transport(R=1.0, mu_ref=0.37, bulk_viscosity=0.21)
+    rhs = _visc_rhs(W, dom, _periodic_setup(tr))
+    uh = [np.fft.fftn(W[1 + k][dom.interior[1:]]) for k in range(3)]
+    th = np.meshgrid(*[2 * np.pi * np.fft.fftfreq(n) for n in dom.cells], indexing="ij")
+    e = [np.exp(1j * t) for t in th]
+    dx = dom.dx
+    Fd = [(1 - 1 / e[j]) / dx[j] for j in range(3)]
+    C = [1j * np.sin(th[i]) / dx[i] for i in range(3)]
+    A = [(1 + 1 / e[j]) / 2 for j in range(3)]
+    Dv = [(1 - e[j]) / dx[j] for j in range(3)]
+    mu, mu2 = 0.37, 0.21 - 2.0 / 3.0 * 0.37
+    for k in range(3):
+        acc = 0.0
+        for j in range(3):
+            G = lambda a, b: (Fd[j] if b == j else A[j] * C[b]) * uh[a]  # d u_a / d x_b at j-faces
+            tau = mu * (G(j, k) + G(k, j)) + (mu2 * sum(G(m, m) for m in range(3)) if j == k else 0.0)
+            acc = acc + Dv[j] * (-tau)
+        ref = np.real(np.fft.ifftn(acc))
+        assert np.max(np.abs(rhs[1 + k] - ref)) <= 1e-12 * np.max(np.abs(ref)), k
+
+
+def test_heat_flux_is_discrete_laplacian():
+    """u = 0, rho = 1, constant conductivity: the energy rhs is kappa times the discrete Laplacian
+    of T = p / (rho R), kappa = cp mu / Pr, cp = gamma R / (gamma - 1)."""
+    dom = o.Domain((9, 7, 1), ((0, 1.0), (0, 0.8), (0, 1.0)), nh=3)
+    rng = np.random.default_rng(12)
+    W = np.zeros((5,) + dom.shape)
+    W[0] = 1.0
+    W[4] = 1.0
+    W[4][dom.interior[1:]] = 1.0 + 0.1 * rng.standard_normal(dom.cells)
+    tr = o.Transport(R=0.8, mu_ref=0.05, prandtl=0.71)
+    rhs = _visc_rhs(W, dom, _periodic_setup(tr))
+    T = W[4][dom.interior[1:]] / 0.8
+    kappa = 1.4 / 0.4 * 0.8 * 0.05 / 0.71
+    lap = sum((np.roll(T, -1, a) - 2 * T + np.roll(T, 1, a)) / dom.dx[a] ** 2 for a in (0, 1))
+    assert np.max(np.abs(rhs[4] - kappa * lap)) <= 1e-12 * np.max(np.abs(kappa * lap))
+    assert np.max(np.abs(rhs[1:4])) == 0.0
+
+
+def test_viscous_work_of_a_shear_is_laplacian_of_half_v_squared():
+    """v(x) only, constant mu: momentum rhs_v = mu Lap(v), energy rhs = mu Lap(v^2 / 2) exactly
+    at the discrete level (the face work tau_xy v_f is a face derivative of v^2/2)."""
+    dom = o.Domain((12, 5, 1), ((0, 1.0), (0, 1.0), (0, 1.0)), nh=3)
+    rng = np.random.default_rng(13)
+    v = rng.standard_normal(12)
+    W = np.zeros((5,) + dom.shape)
+    W[0] = 1.0
+    W[4] = 1.0
+    W[2][dom.interior[1:]] = v[:, None, None]
+    mu = 0.2
+    rhs = _visc_rhs(W, dom, _periodic_setup(o.Transport(mu_ref=mu)))
+    lap = lambda f: (np.roll(f, -1) - 2 * f + np.roll(f, 1)) / dom.dx[0] ** 2
+    assert np.allclose(rhs[2][:, :, 0], (mu * lap(v))[:, None], rtol=0, atol=1e-12 * np.max(np.abs(mu * lap(v))))
+    assert np.allclose(rhs[4][:, :, 0], (mu * lap(0.5 * v * v))[:, None], rtol=0,
+                       atol=1e-12 * np.max(np.abs(mu * lap(0.5 * v * v))))
+
+
+def test_variable_viscosity_at_face_averaged_temperature():
+    """v(x), p(x) with rho = 1: tau_xy = mu(T_f) (v_R - v_L)/dx with T_f the average of the two
+    cells' p/(rho R) and mu = mu_ref (T_f / T_ref)^n."""
+    dom = o.Domain((10, 4, 1), ((0, 1.0), (0, 1.0), (0, 1.0)), nh=3)
+    rng = np.random.default_rng(14)
+    v = rng.standard_normal(10)
+    p = 1.0 + 0.3 * rng.random(10)
+    W = np.zeros((5,) + dom.shape)
+    W[0] = 1.0
+    W[2][dom.interior[1:]] = v[:, None, None]
+    W[4][dom.interior[1:]] = p[:, None, None]
+    tr = o.Transport(R=0.9, mu_ref=0.3, T_ref=1.2, mu_exp=0.76)
+    rhs = _visc_rhs(W, dom, _periodic_setup(tr))
+    T = p / 0.9
+    Tf = 0.5 * (np.roll(T, 1) + T)                  # face m between cells m-1 and m
+    tauf = 0.3 * (Tf / 1.2) ** 0.76 * (v - np.roll(v, 1)) / dom.dx[0]
+    ref = (np.roll(tauf, -1) - tauf) / dom.dx[0]    # rhs = (D_m - D_{m+1})/dx with D = -tau
+    assert np.allclose(rhs[2][:, :, 0], ref[:, None], rtol=0, atol=1e-12 * np.max(np.abs(ref)))
+
+
+def test_viscous_operator_second_order():
+    """u = sin x cos y, v = -cos x sin y on [0, 2pi)^2, constant mu: the discrete momentum rhs
+    converges to mu Lap(u) (the field is divergence free) at second order."""
+    errs = []
+    for n in (16, 32, 64):
+        dom = o.Domain((n, n, 1), ((0, 2 * np.pi), (0, 2 * np.pi), (0, 1.0)), nh=3)
+        X, Y = np.meshgrid(*dom.cell_centers()[:2], indexing="ij")
+        W = np.zeros((5,) + dom.shape)
+        W[0] = 1.0
+        W[4] = 1.0
+        W[1][dom.interior[1:]] = (np.sin(X) * np.cos(Y))[..., None]
+        W[2][dom.interior[1:]] = (-np.cos(X) * np.sin(Y))[..., None]
+        rhs = _visc_rhs(W, dom, _periodic_setup(o.Transport(mu_ref=0.1)))
+        exact = -2 * 0.1 * np.sin(X) * np.cos(Y)
+        errs.append(np.max(np.abs(rhs[1][..., 0] - exact)))
+    rates = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(1.9 < r < 2.1 for r in rates), rates
+
+
+def test_viscous_time_step_limit():
+    """dt = CFL min(dx/max sum(|u|+c), 3/14 dx^2 / max(mu/rho), 0.1 dx^2 / max(kappa/(rho cp)))."""
+    dom = o.Domain((8, 8, 1), ((0, 1.0), (0, 1.0), (0, 1.0)), nh=3)
+    W = np.ones((5,) + dom.shape)
+    W[1:4] = 0.0
+    tr = o.Transport(R=1.0, mu_ref=0.05, prandtl=0.71)
+    s = o.Setup(cfl=0.5, transport=tr)
+    dx = 1 / 8
+    conv = dx / (2 * np.sqrt(1.4))
+    visc = 3 / 14 * dx ** 2 / 0.05
+    therm = 0.1 * dx ** 2 / (1.4 / 0.4 * 0.05 / 0.71 / (1.4 / 0.4))
+    assert abs(o.compute_dt(W, dom, s) - 0.5 * min(conv, visc, therm)) <= 1e-15
