@@ -65,6 +65,25 @@ typedef struct jfa_grid {
     double dx[3];   /* cell size per axis, (b - a) / n (domain/mesh_creation/homogenous.py:13) */
 } jfa_grid;
 
+/* Transport properties of the dissipative fluxes, second-order central (solvers/source_term_solver.py:
+ * 188-350; flux assembly space_solver.py:565-587; dt limits time_integration/time_step_size.py:112-135):
+ *   mu(T) = mu_ref (T / T_ref)^mu_exp   (the power law a CUSTOM viscosity lambda expresses,
+ *                                         materials/single_materials/material.py:96-97; mu_exp = 0: constant)
+ *   kappa = cp mu / prandtl              (PRANDTL model, material.py:115-116), cp = gamma R / (gamma - 1)
+ *   T = p / (rho R)                      (ideal_gas.py:81-83)
+ * The dissipative fluxes use central stencils across face-adjacent cells and their transverse
+ * neighbours; they require PERIODIC boundaries on every active axis. */
+typedef struct jfa_transport {
+    int32_t viscous;        /* 1: viscous stresses tau and work u.tau (active_physics.is_viscous_flux) */
+    int32_t heat;           /* 1: heat flux q = -kappa dT/dx (active_physics.is_heat_flux) */
+    double R;               /* specific gas constant */
+    double mu_ref;
+    double T_ref;
+    double mu_exp;
+    double bulk_viscosity;
+    double prandtl;
+} jfa_transport;
+
 /* Numerical scheme of the high-order Godunov flux (data_types/numerical_setup/conservatives.py:35-42). */
 typedef struct jfa_scheme {
     int32_t recon_var;     /* JFA_PRIMITIVE .. JFA_CHAR_CONSERVATIVE */
@@ -75,6 +94,7 @@ typedef struct jfa_scheme {
     int32_t dtype;         /* JFA_F64 (float64 buffers) or JFA_F32 (float32 buffers) */
     double gamma;          /* ideal-gas specific heat ratio (materials/single_materials/ideal_gas.py:29) */
     double eps_stencil;    /* WENO epsilon, 1e-30 (config/precision.py:39,53) */
+    const jfa_transport* transport;  /* dissipative fluxes of the fused stage / RHS; NULL: inviscid */
 } jfa_scheme;
 
 /* Library version string; never NULL. */
@@ -106,10 +126,11 @@ int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void
 int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme,
                         int32_t axis, void* flux, void* stream);
 
-/* Convective right-hand side 0.0 + rhs_x + rhs_y + rhs_z, rhs_xi = (1/dx)(F[:-1]-F[1:]).
+/* Right-hand side 0.0 + rhs_x + rhs_y + rhs_z, rhs_xi = (1/dx)(F[:-1]-F[1:]).
  * rhs: (5, Nx, Ny, Nz).  workspace: >= jfa_rhs_workspace_bytes(grid, dtype) bytes (FP64 partial RHS).
- * Replaces SpaceSolver.compute_rhs (solvers/space_solver.py:151-453) and compute_rhs_xi (:456-674),
- * convective flux only. */
+ * Replaces SpaceSolver.compute_rhs (solvers/space_solver.py:151-453) and compute_rhs_xi (:456-674):
+ * the convective flux, plus the dissipative fluxes when scheme->transport is set (the caller's
+ * boundaries must then be periodic). */
 int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme,
                     void* rhs, void* workspace, size_t workspace_bytes, void* stream);
 size_t jfa_rhs_workspace_bytes(const jfa_grid* grid, int32_t dtype);
@@ -131,10 +152,13 @@ int jfa_halo_update(void* W, void* U, const jfa_grid* grid, const int32_t bc[6],
                     int32_t dtype, void* stream);
 
 /* CFL time step dt = CFL * dx_min / (max_cells sum_active(|u_i| + c) + eps), written to dt_dev[0]
- * (a device double).  workspace: >= 16 bytes of device memory.
- * Replaces compute_time_step_size, convective part (time_integration/time_step_size.py:15-157). */
+ * (a device double); with a transport that has viscous (heat) fluxes, the minimum with
+ * 3/14 dx_min^2 / (max mu/rho + eps) (and 0.1 dx_min^2 / (max kappa/(rho cp) + eps)) before the
+ * CFL factor.  transport may be NULL.  workspace: >= 32 bytes of device memory.
+ * Replaces compute_time_step_size (time_integration/time_step_size.py:15-157). */
 int jfa_compute_dt(const void* W, const jfa_grid* grid, double gamma, double cfl, double eps,
-                   int32_t dtype, double* dt_dev, void* workspace, void* stream);
+                   int32_t dtype, const jfa_transport* transport, double* dt_dev, void* workspace,
+                   void* stream);
 
 /* Whole-buffer conversions (every cell, halos included).
  * Replace EquationManager.get_conservatives_from_primitives (equation_manager.py:44-106) and

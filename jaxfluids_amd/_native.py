@@ -31,6 +31,12 @@ class JfaGrid(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32 * 3), ("nh", ctypes.c_int32), ("dx", ctypes.c_double * 3)]
 
 
+class JfaTransport(ctypes.Structure):
+    _fields_ = [("viscous", ctypes.c_int32), ("heat", ctypes.c_int32), ("R", ctypes.c_double),
+                ("mu_ref", ctypes.c_double), ("T_ref", ctypes.c_double), ("mu_exp", ctypes.c_double),
+                ("bulk_viscosity", ctypes.c_double), ("prandtl", ctypes.c_double)]
+
+
 class JfaScheme(ctypes.Structure):
     _fields_ = [
         ("recon_var", ctypes.c_int32),
@@ -41,6 +47,7 @@ class JfaScheme(ctypes.Structure):
         ("dtype", ctypes.c_int32),
         ("gamma", ctypes.c_double),
         ("eps_stencil", ctypes.c_double),
+        ("transport", ctypes.POINTER(JfaTransport)),
     ]
 
 
@@ -74,7 +81,7 @@ SIGNATURES = {
     "jfa_rhs_workspace_bytes": (_SZ, [_PG, _I32]),
     "jfa_rk3_stage": (_I32, [_P, _P, _P, _P, _PG, _PS, _I32, _P, _D, _P]),
     "jfa_halo_update": (_I32, [_P, _P, _PG, _PI32, _D, _I32, _P]),
-    "jfa_compute_dt": (_I32, [_P, _PG, _D, _D, _D, _I32, _P, _P, _P]),
+    "jfa_compute_dt": (_I32, [_P, _PG, _D, _D, _D, _I32, ctypes.POINTER(JfaTransport), _P, _P, _P]),
     "jfa_cons_from_prim": (_I32, [_P, _P, _I64, _D, _I32, _P]),
     "jfa_prim_from_cons": (_I32, [_P, _P, _I64, _D, _I32, _P]),
     "jfa_rk3_stage_fused": (_I32, [_P, _P, _P, _P, _P, _PG, _PS, _PI32, _I32, _P, _D, _D, _D, _P, _P, _SZ, _P]),
@@ -145,8 +152,21 @@ def make_grid(cells: Sequence[int], nh: int, dx: Sequence[float]) -> JfaGrid:
     return g
 
 
+def make_transport(R: float = 1.0, mu_ref: float = 0.0, T_ref: float = 1.0, mu_exp: float = 0.0,
+                   bulk_viscosity: float = 0.0, prandtl: Optional[float] = None, viscous: bool = True) -> JfaTransport:
+    """Dissipative-flux transport: mu(T) = mu_ref (T/T_ref)^mu_exp, kappa = cp mu / prandtl (heat flux
+    when prandtl is given)."""
+    t = JfaTransport()
+    t.viscous = 1 if viscous else 0
+    t.heat = 1 if prandtl is not None else 0
+    t.R, t.mu_ref, t.T_ref, t.mu_exp = float(R), float(mu_ref), float(T_ref), float(mu_exp)
+    t.bulk_viscosity = float(bulk_viscosity)
+    t.prandtl = float(prandtl) if prandtl is not None else 1.0
+    return t
+
+
 def make_scheme(recon_var: str, stencil: str, riemann: str, signal_speed: Optional[str], frozen_state: str,
-                dtype: int, gamma: float, eps_stencil: float) -> JfaScheme:
+                dtype: int, gamma: float, eps_stencil: float, transport: Optional[JfaTransport] = None) -> JfaScheme:
     s = JfaScheme()
     try:
         s.recon_var = RECON_VAR[recon_var]
@@ -159,6 +179,9 @@ def make_scheme(recon_var: str, stencil: str, riemann: str, signal_speed: Option
     s.dtype = int(dtype)
     s.gamma = float(gamma)
     s.eps_stencil = float(eps_stencil)
+    if transport is not None:
+        s._transport_ref = transport  # keeps the struct alive as long as the scheme
+        s.transport = ctypes.pointer(transport)
     return s
 
 

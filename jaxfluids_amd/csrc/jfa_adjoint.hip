@@ -31,6 +31,7 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     rc = check_scheme(scheme);
     if (rc) return rc;
     if (scheme->dtype != JFA_F64 || scheme->recon_var != JFA_PRIMITIVE) return JFA_EUNSUPPORTED;
+    if (scheme->transport && (scheme->transport->viscous || scheme->transport->heat)) return JFA_EUNSUPPORTED;
     if (scheme->riemann == JFA_HLLC && scheme->signal_speed == JFA_SS_TORO) return JFA_EUNSUPPORTED;
     if (!W_in || !lam_out || !lam_in || !bc || !workspace || stage < 0 || stage > 2 || (stage > 0 && !lam_n))
         return JFA_EINVAL;

@@ -236,7 +236,35 @@ static inline Params make_params(const jfa_scheme* s) {
     p.eps_js = (144.0 / 169.0) * s->eps_stencil;
     p.signal_speed = s->signal_speed;
     p.frozen = s->frozen_state;
+    p.dissip = 0;
+    p.R = 1.0;
+    p.mu_ref = p.inv_T_ref = p.mu_exp = p.bulk = p.kappa_fac = p.inv_cp = 0.0;
+    const jfa_transport* t = s->transport;
+    if (t && (t->viscous || t->heat)) {
+        p.dissip = (t->viscous ? 1 : 0) | (t->heat ? 2 : 0);
+        p.R = t->R;
+        p.mu_ref = t->mu_ref;
+        p.inv_T_ref = 1.0 / t->T_ref;
+        p.mu_exp = t->mu_exp;
+        p.bulk = t->bulk_viscosity;
+        const double cp = s->gamma / (s->gamma - 1.0) * t->R;  // ideal_gas.py:32
+        p.kappa_fac = t->heat ? cp / t->prandtl : 0.0;          // material.py:115-116
+        p.inv_cp = 1.0 / cp;
+    }
     return p;
+}
+
+// A transport is usable when its constants are finite and positive where needed; dissipative
+// fluxes need PERIODIC boundaries on every active axis (their central stencils wrap).
+static inline int check_transport(const jfa_scheme* s, const Geo& G, const int32_t* bc) {
+    const jfa_transport* t = s->transport;
+    if (!t || !(t->viscous || t->heat)) return JFA_OK;
+    if (!(t->R > 0.0) || !(t->T_ref > 0.0) || !(t->mu_ref >= 0.0) || !(t->bulk_viscosity >= 0.0)) return JFA_EINVAL;
+    if (t->heat && !(t->prandtl > 0.0)) return JFA_EINVAL;
+    if (bc)
+        for (int f = 0; f < 6; ++f)
+            if (G.active[f / 2] && bc[f] != JFA_BC_PERIODIC) return JFA_EUNSUPPORTED;
+    return JFA_OK;
 }
 
 static inline Params gas_params(double gamma) {
@@ -249,6 +277,9 @@ static inline Params gas_params(double gamma) {
     p.eps_js = (144.0 / 169.0) * 1e-30;
     p.signal_speed = JFA_SS_EINFELDT;
     p.frozen = JFA_FROZEN_ARITHMETIC;
+    p.dissip = 0;
+    p.R = 1.0;
+    p.mu_ref = p.inv_T_ref = p.mu_exp = p.bulk = p.kappa_fac = p.inv_cp = 0.0;
     return p;
 }
 

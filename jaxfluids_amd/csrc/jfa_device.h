@@ -39,6 +39,15 @@ struct Params {
     double eps_js;     // (12/13)^2 eps: epsilon of the scaled WENO-JS indicators (squared)
     int signal_speed;  // JFA_SS_*
     int frozen;        // JFA_FROZEN_*
+    // dissipative fluxes (jfa_transport); dissip = 0: inviscid
+    int dissip;        // bit 0: viscous stresses, bit 1: heat flux
+    double R;          // specific gas constant
+    double mu_ref;     // mu(T) = mu_ref (T / T_ref)^mu_exp
+    double inv_T_ref;
+    double mu_exp;
+    double bulk;       // bulk viscosity
+    double kappa_fac;  // kappa = kappa_fac mu, kappa_fac = cp / Pr
+    double inv_cp;     // 1 / cp
 };
 
 // ---------------------------------------------------------------- refined hardware estimates

@@ -1,0 +1,125 @@
+// jfa_dissipative.h -- second-order central viscous and heat fluxes at one face, added to the
+// convective face flux inside the fused sweeps (solvers/source_term_solver.py:188-350,414-448,
+// 504-580; flux assembly space_solver.py:565-587).
+//
+// Face f along AXIS lies between the cells L = f-1 and R = f:
+//   T_f  = (T_L + T_R)/2,  T = p/(rho R)                          (central_2, ideal_gas.py:81-83)
+//   mu   = mu_ref (T_f/T_ref)^mu_exp,  mu2 = mu_b - 2/3 mu          (material.py:96-97, :527)
+//   g[k][i] = d u_k / d x_i at the face: i == AXIS: (u_R - u_L)/dx (deriv_face_2);
+//            i != AXIS: the central derivative (u(+e_i) - u(-e_i))/(2 dx_i) at L and at R
+//            (deriv_center_2), averaged (central_2); inactive i: 0
+//   tau_k = mu (g[AXIS][k] + g[k][AXIS]) (active k), tau_AXIS += mu2 sum_k g[k][k]
+//   F[1+k] += -tau_k;  F[4] += -sum_k tau_k u_f,k + q,  q = -kappa (T_R - T_L)/dx, kappa = cp mu/Pr
+// All accesses wrap periodically on the active axes (the reference fills edge halos with the
+// periodic copies for viscous runs), so no edge halo is read.
+#pragma once
+
+#include "jfa_common.h"
+
+namespace jfa {
+
+__device__ __forceinline__ double viscosity(double T, const Params& prm) {
+    return prm.mu_exp == 0.0 ? prm.mu_ref : prm.mu_ref * pow(T * prm.inv_T_ref, prm.mu_exp);
+}
+
+// buffer index of interior cell c, coordinates wrapped onto [0, n) on the active axes
+__device__ __forceinline__ int64_t wrapped_index(const Geo& G, const int c[3]) {
+    int64_t b = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        int ci = c[i];
+        if (G.active[i]) ci = ci < 0 ? ci + G.n[i] : (ci >= G.n[i] ? ci - G.n[i] : ci);
+        b += (int64_t)(ci + G.off[i]) * G.s[i];
+    }
+    return b;
+}
+
+template <class T>
+__device__ __forceinline__ void load_velocity(const T* __restrict__ W, int64_t fs, int64_t b, double u[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) u[k] = ld(W, (k + 1) * fs + b);
+}
+
+// adds the dissipative flux of the face whose right cell has interior coordinates cr
+// (cr[AXIS] in 0..n, the left cell is cr - e_AXIS) to F
+template <class T, int AXIS>
+__device__ __forceinline__ void dissipative_face(const T* __restrict__ W, const Geo& G, const Params& prm,
+                                                 const int cr[3], double F[5]) {
+    const int64_t fs = G.fs;
+    int cl[3] = {cr[0], cr[1], cr[2]};
+    cl[AXIS] -= 1;
+    const int64_t bl = wrapped_index(G, cl), br = wrapped_index(G, cr);
+    double wl[5], wr[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        wl[i] = ld(W, i * fs + bl);
+        wr[i] = ld(W, i * fs + br);
+    }
+    const double TL = wl[4] / (wl[0] * prm.R), TR = wr[4] / (wr[0] * prm.R);
+    double Tf = 0.5 * (TL + TR);
+    if (Tf <= 0.0) Tf = 2.220446049250313e-16;  // source_term_solver.py:301 (precision eps)
+    const double mu = viscosity(Tf, prm);
+    double d4 = 0.0;
+    if (prm.dissip & 1) {
+        double g[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (!G.active[i]) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) g[k][i] = 0.0;
+            } else if (i == AXIS) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) g[k][i] = (wr[1 + k] - wl[1 + k]) * G.one_dx[i];
+            } else {
+                const double h = 0.5 * G.one_dx[i];
+                int c[3];
+                double up[3], um[3], dL[3], dR[3];
+                c[0] = cl[0]; c[1] = cl[1]; c[2] = cl[2];
+                c[i] += 1;
+                load_velocity(W, fs, wrapped_index(G, c), up);
+                c[i] -= 2;
+                load_velocity(W, fs, wrapped_index(G, c), um);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) dL[k] = (up[k] - um[k]) * h;
+                c[0] = cr[0]; c[1] = cr[1]; c[2] = cr[2];
+                c[i] += 1;
+                load_velocity(W, fs, wrapped_index(G, c), up);
+                c[i] -= 2;
+                load_velocity(W, fs, wrapped_index(G, c), um);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    dR[k] = (up[k] - um[k]) * h;
+                    g[k][i] = 0.5 * (dL[k] + dR[k]);
+                }
+            }
+        }
+        const double mu2 = prm.bulk - 2.0 / 3.0 * mu;
+        double tau[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) tau[k] = G.active[k] ? mu * (g[AXIS][k] + g[k][AXIS]) : 0.0;
+        double div = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (G.active[k]) div = div + g[k][k];
+        tau[AXIS] = tau[AXIS] + mu2 * div;
+        double vel_tau = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (G.active[k]) vel_tau = vel_tau + tau[k] * (0.5 * (wl[1 + k] + wr[1 + k]));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) F[1 + k] = F[1 + k] + (-tau[k]);
+        d4 = -vel_tau;
+    }
+    if (prm.dissip & 2) d4 = d4 + (-(prm.kappa_fac * mu) * ((TR - TL) * G.one_dx[AXIS]));
+    F[4] = F[4] + d4;
+}
+
+// cell diffusivities for the time-step limits: nu = mu/rho, alpha = kappa/(rho cp)
+__device__ __forceinline__ void diffusivities(const double w[5], const Params& prm, double& nu, double& alpha) {
+    const double T = w[4] / (w[0] * prm.R);
+    const double mu = viscosity(T, prm);
+    nu = (prm.dissip & 1) ? mu / w[0] : 0.0;
+    alpha = (prm.dissip & 2) ? prm.kappa_fac * mu / w[0] * prm.inv_cp : 0.0;
+}
+
+}  // namespace jfa

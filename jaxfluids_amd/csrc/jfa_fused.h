@@ -19,6 +19,7 @@
 #pragma once
 
 #include "jfa_common.h"
+#include "jfa_dissipative.h"
 
 namespace jfa {
 
@@ -45,7 +46,7 @@ __device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* 
 
 // ------------------------------------------------------------------ MARCH kernel
 // grid: x over IN (lanes), y over the third axis O, z over chunks of L cells along A.
-template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN>
+template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS>
 __global__ __launch_bounds__(kMarchBlockMax) void march_kernel(const T* __restrict__ W, const T* __restrict__ U,
                                                                Geo G, Params prm, int L, int accumulate,
                                                                double* __restrict__ R) {
@@ -69,6 +70,11 @@ __global__ __launch_bounds__(kMarchBlockMax) void march_kernel(const T* __restri
         for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bcol + (int64_t)(a0 - 3 + k) * sa);
     double Fp[5];
     window_flux<T, A, VAR, STENCIL, RIEMANN>(win, W, bcol + (int64_t)a0 * sa, sa, fs, prm, Fp);
+    int cr[3];  // right cell of the current face (dissipative fluxes)
+    cr[A] = a0;
+    cr[O] = io;
+    cr[IN] = iin;
+    if constexpr (DISS) dissipative_face<T, A>(W, G, prm, cr, Fp);
     // R layout (5, N0, N1, N2)
     int c[3];
     c[A] = a0;
@@ -95,6 +101,10 @@ __global__ __launch_bounds__(kMarchBlockMax) void march_kernel(const T* __restri
         }
         double Fn[5];
         window_flux<T, A, VAR, STENCIL, RIEMANN>(win, W, bcol + (int64_t)(a + 1) * sa, sa, fs, prm, Fn);
+        if constexpr (DISS) {
+            cr[A] = a + 1;
+            dissipative_face<T, A>(W, G, prm, cr, Fn);
+        }
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             const double d = odx * (Fp[i] - Fn[i]);  // space_solver.py:597-599
@@ -136,7 +146,31 @@ __device__ __forceinline__ void block_max_part(double* part, double v) {
     }
 }
 
-template <class T, int IN, int VAR, int STENCIL, int RIEMANN>
+// Block maxima of three quantities written to part[b], part[nb + b], part[2 nb + b].
+__device__ __forceinline__ void block_max_part3(double* part, double v0, double v1, double v2) {
+    __shared__ double sm[3][kInnerBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        v0 = fmax(v0, __shfl_xor(v0, o, 64));
+        v1 = fmax(v1, __shfl_xor(v1, o, 64));
+        v2 = fmax(v2, __shfl_xor(v2, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sm[0][threadIdx.x >> 6] = v0;
+        sm[1][threadIdx.x >> 6] = v1;
+        sm[2][threadIdx.x >> 6] = v2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nb = (int)gridDim.x;
+        for (int q = 0; q < 3; ++q) {
+            double m = sm[q][0];
+            for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sm[q][i]);
+            part[q * nb + blockIdx.x] = m;
+        }
+    }
+}
+
+template <class T, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS>
 __global__ __launch_bounds__(kInnerBlock) void inner_kernel(const T* __restrict__ W, const T* __restrict__ U,
                                                             Geo G, Params prm, InnerArgs<T> args) {
     constexpr int P = IN == 0 ? 1 : 0;
@@ -166,12 +200,19 @@ __global__ __launch_bounds__(kInnerBlock) void inner_kernel(const T* __restrict_
 #pragma unroll
             for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bf + (k - 3));
         window_flux<T, IN, VAR, STENCIL, RIEMANN>(win, W, bf, 1, fs, prm, F);
+        if constexpr (DISS) {
+            int cr[3];
+            cr[IN] = (int)f;
+            cr[P] = cP;
+            cr[Q] = cQ;
+            dissipative_face<T, IN>(W, G, prm, cr, F);
+        }
     }
     double Fu[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) Fu[i] = __shfl_down(F[i], 1, 64);
     const bool vcell = vface && lane < 63u && f < NIN;
-    double vmax = 0.0;
+    double vmax = 0.0, nu_max = 0.0, al_max = 0.0;
     if (vcell) {
         int c[3];
         c[IN] = (int)f;
@@ -205,26 +246,45 @@ __global__ __launch_bounds__(kInnerBlock) void inner_kernel(const T* __restrict_
             double uh[5];
             cons_from_prim(w, prm, uh);
             halo_scatter(args.Wout, args.Uout, G, args.bc, c, bf, w, uh, prm);
-            if (args.cfl_part) vmax = wave_sum(w, G.active, prm.gamma);
+            if (args.cfl_part) {
+                vmax = wave_sum(w, G.active, prm.gamma);
+                if constexpr (DISS) diffusivities(w, prm, nu_max, al_max);
+            }
         }
     }
-    if (args.cfl_part) block_max_part(args.cfl_part, vmax);
+    if constexpr (DISS) {
+        if (args.cfl_part) block_max_part3(args.cfl_part, vmax, nu_max, al_max);
+    } else {
+        if (args.cfl_part) block_max_part(args.cfl_part, vmax);
+    }
 }
 
-// dt = CFL dx_min / (max + eps) from the per-block partials (time_step_size.py:109,154-155)
+// dt from the per-block partials (time_step_size.py:103-135,154-155): dx_min / (max sum(|u|+c) + eps),
+// and with dissipative fluxes the minimum with 3/14 dx_min^2 / (max mu/rho + eps) (viscous) and
+// 0.1 dx_min^2 / (max kappa/(rho cp) + eps) (heat), times CFL.  part holds 1 or 3 arrays of n.
 template <int UNUSED = 0>  // template: one definition shared by both storage-type translation units
 __global__ __launch_bounds__(1024) void dt_from_parts_kernel(const double* __restrict__ part, int n, double dx_min,
-                                                             double cfl, double eps, double* __restrict__ dt) {
-    __shared__ double sm[16];
-    double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, part[i]);
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+                                                             double cfl, double eps, int dissip,
+                                                             double* __restrict__ dt) {
+    __shared__ double sm[3][16];
+    const int nq = dissip ? 3 : 1;
+    for (int q = 0; q < nq; ++q) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, part[q * n + i]);
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0) sm[q][threadIdx.x >> 6] = v;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double mx = sm[0];
-        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx = fmax(mx, sm[i]);
-        dt[0] = dx_min / (mx + eps) * cfl;
+        double mx[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < nq; ++q) {
+            mx[q] = sm[q][0];
+            for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx[q] = fmax(mx[q], sm[q][i]);
+        }
+        double d = dx_min / (mx[0] + eps);
+        if (dissip & 1) d = fmin(d, 3.0 / 14.0 * dx_min * dx_min / (mx[1] + eps));
+        if (dissip & 2) d = fmin(d, 0.1 * dx_min * dx_min / (mx[2] + eps));
+        dt[0] = d * cfl;
     }
 }
 
@@ -275,7 +335,7 @@ static inline int64_t device_wave_slots(const void* kernel, int block) {
     return (int64_t)num_cu * blocks * ((block + 63) / 64);
 }
 
-template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN>
+template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS>
 static void launch_march(const void* W, const void* U, const Geo& G, const Params& prm, int accumulate, double* R,
                          hipStream_t st) {
     constexpr int O = 3 - A - IN;
@@ -283,46 +343,51 @@ static void launch_march(const void* W, const void* U, const Geo& G, const Param
     if (bx > kMarchBlockMax) bx = kMarchBlockMax;
     static int64_t slots[kMarchBlockMax / 64 + 1] = {0};
     int64_t& sl = slots[bx / 64];
-    if (sl == 0) sl = device_wave_slots((const void*)march_kernel<T, A, IN, VAR, STENCIL, RIEMANN>, bx);
+    if (sl == 0) sl = device_wave_slots((const void*)march_kernel<T, A, IN, VAR, STENCIL, RIEMANN, DISS>, bx);
     const int L = march_chunk(G, A, IN, sl);
     dim3 grid((unsigned)((G.n[IN] + bx - 1) / bx), (unsigned)G.n[O], (unsigned)((G.n[A] + L - 1) / L));
-    hipLaunchKernelGGL((march_kernel<T, A, IN, VAR, STENCIL, RIEMANN>), grid, dim3(bx), 0, st,
+    hipLaunchKernelGGL((march_kernel<T, A, IN, VAR, STENCIL, RIEMANN, DISS>), grid, dim3(bx), 0, st,
                        static_cast<const T*>(W), static_cast<const T*>(U), G, prm, L, accumulate, R);
 }
 
-template <class T, int IN, int VAR, int STENCIL, int RIEMANN>
+template <class T, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS>
 static void launch_inner(const void* W, const void* U, const Geo& G, const Params& prm, const InnerArgs<T>& a,
                          hipStream_t st) {
     dim3 grid((unsigned)inner_blocks(G));
-    hipLaunchKernelGGL((inner_kernel<T, IN, VAR, STENCIL, RIEMANN>), grid, dim3(kInnerBlock), 0, st,
+    hipLaunchKernelGGL((inner_kernel<T, IN, VAR, STENCIL, RIEMANN, DISS>), grid, dim3(kInnerBlock), 0, st,
                        static_cast<const T*>(W), static_cast<const T*>(U), G, prm, a);
 }
 
 // All sweeps of one stage (or of one RHS evaluation) for fixed scheme templates.
-template <class T, int VAR, int STENCIL, int RIEMANN>
+template <class T, int VAR, int STENCIL, int RIEMANN, bool DISS>
 static void sweeps(const void* W, const void* U, const Geo& G, const Params& prm, double* R, InnerArgs<T> ia,
                    hipStream_t st) {
     const int IN = innermost_axis(G);
     int acc = 0;
     for (int A = 0; A < IN; ++A) {
         if (!G.active[A]) continue;
-        if (A == 0 && IN == 1) launch_march<T, 0, 1, VAR, STENCIL, RIEMANN>(W, U, G, prm, acc, R, st);
-        else if (A == 0) launch_march<T, 0, 2, VAR, STENCIL, RIEMANN>(W, U, G, prm, acc, R, st);
-        else launch_march<T, 1, 2, VAR, STENCIL, RIEMANN>(W, U, G, prm, acc, R, st);
+        if (A == 0 && IN == 1) launch_march<T, 0, 1, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, acc, R, st);
+        else if (A == 0) launch_march<T, 0, 2, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, acc, R, st);
+        else launch_march<T, 1, 2, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, acc, R, st);
         acc = 1;
     }
     ia.R = R;
     ia.accumulate = acc;
-    if (IN == 0) launch_inner<T, 0, VAR, STENCIL, RIEMANN>(W, U, G, prm, ia, st);
-    else if (IN == 1) launch_inner<T, 1, VAR, STENCIL, RIEMANN>(W, U, G, prm, ia, st);
-    else launch_inner<T, 2, VAR, STENCIL, RIEMANN>(W, U, G, prm, ia, st);
+    if (IN == 0) launch_inner<T, 0, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, ia, st);
+    else if (IN == 1) launch_inner<T, 1, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, ia, st);
+    else launch_inner<T, 2, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, ia, st);
 }
 
 template <class T, int VAR, int STENCIL>
 static void sweeps_r(int riemann, const void* W, const void* U, const Geo& G, const Params& prm, double* R,
                      const InnerArgs<T>& ia, hipStream_t st) {
-    if (riemann == JFA_HLLC) sweeps<T, VAR, STENCIL, JFA_HLLC>(W, U, G, prm, R, ia, st);
-    else sweeps<T, VAR, STENCIL, JFA_RUSANOV>(W, U, G, prm, R, ia, st);
+    if (prm.dissip) {
+        if (riemann == JFA_HLLC) sweeps<T, VAR, STENCIL, JFA_HLLC, true>(W, U, G, prm, R, ia, st);
+        else sweeps<T, VAR, STENCIL, JFA_RUSANOV, true>(W, U, G, prm, R, ia, st);
+    } else {
+        if (riemann == JFA_HLLC) sweeps<T, VAR, STENCIL, JFA_HLLC, false>(W, U, G, prm, R, ia, st);
+        else sweeps<T, VAR, STENCIL, JFA_RUSANOV, false>(W, U, G, prm, R, ia, st);
+    }
 }
 template <class T, int VAR>
 static void sweeps_s(int stencil, int riemann, const void* W, const void* U, const Geo& G, const Params& prm,
@@ -366,7 +431,7 @@ int fused_stage_impl(const void* W_in, const void* U_in, const void* U_n, void* 
     if (rc) return rc;
     if (dt_next) {
         hipLaunchKernelGGL(dt_from_parts_kernel<0>, dim3(1), dim3(1024), 0, st, part, (int)inner_blocks(G), G.dx_min,
-                           cfl, eps, dt_next);
+                           cfl, eps, make_params(s).dissip, dt_next);
         rc = check_launch();
     }
     return rc;

@@ -15,6 +15,6 @@ int fused_rhs_f64(const void* W, const void* U, const Geo& G, const jfa_scheme* 
     return fused_rhs_impl<double>(W, U, G, s, R, rhs, st);
 }
 
-int64_t fused_partials(const Geo& G) { return inner_blocks(G); }
+int64_t fused_partials(const Geo& G) { return 3 * inner_blocks(G); }  // convective, viscous, thermal
 
 }  // namespace jfa

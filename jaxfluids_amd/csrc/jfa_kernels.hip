@@ -10,6 +10,7 @@
 #include <cstring>
 
 #include "jfa_common.h"
+#include "jfa_dissipative.h"
 
 namespace jfa {
 
@@ -181,12 +182,13 @@ __global__ __launch_bounds__(kBlock) void halo_kernel(T* __restrict__ W, T* __re
 }
 
 
-// grid-stride over the interior cells: at most kCflBlocks atomics per reduction
+// grid-stride over the interior cells: at most kCflBlocks atomics per reduction and slot.
+// slot[0]: max sum(|u_i| + c); with dissipative fluxes slot[1]: max mu/rho, slot[2]: max kappa/(rho cp).
 template <class T>
-__global__ __launch_bounds__(kBlock) void cfl_max_kernel(const T* __restrict__ W, Geo G, double gamma,
+__global__ __launch_bounds__(kBlock) void cfl_max_kernel(const T* __restrict__ W, Geo G, Params prm,
                                                          unsigned long long* slot) {
     const uint32_t total = (uint32_t)G.n[0] * G.n[1] * G.n[2];
-    double v = 0.0;
+    double v = 0.0, vn = 0.0, va = 0.0;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         const uint32_t q = t / (uint32_t)G.n[2];
         const int c2i = (int)(t - q * (uint32_t)G.n[2]);
@@ -197,17 +199,36 @@ __global__ __launch_bounds__(kBlock) void cfl_max_kernel(const T* __restrict__ W
         double w[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) w[i] = ld(W, i * G.fs + b);
-        v = fmax(v, wave_sum(w, G.active, gamma));
+        v = fmax(v, wave_sum(w, G.active, prm.gamma));
+        if (prm.dissip) {
+            double nu, al;
+            diffusivities(w, prm, nu, al);
+            vn = fmax(vn, nu);
+            va = fmax(va, al);
+        }
     }
     block_max_to(slot, v);
+    if (prm.dissip) {
+        __syncthreads();  // block_max_to's shared partials are reused
+        block_max_to(slot + 1, vn);
+        __syncthreads();
+        block_max_to(slot + 2, va);
+    }
 }
 
-__global__ void zero_slot_kernel(unsigned long long* slot) { slot[0] = 0ull; }
+__global__ void zero_slot_kernel(unsigned long long* slot) {
+    slot[0] = 0ull;
+    slot[1] = 0ull;
+    slot[2] = 0ull;
+}
 
-__global__ void dt_finalize_kernel(const unsigned long long* slot, double dx_min, double cfl, double eps,
+// time_step_size.py:103-135,154-155
+__global__ void dt_finalize_kernel(const unsigned long long* slot, double dx_min, double cfl, double eps, int dissip,
                                    double* dt) {
-    const double mx = __longlong_as_double((long long)slot[0]);
-    dt[0] = dx_min / (mx + eps) * cfl;  // time_step_size.py:109,154-155
+    double d = dx_min / (__longlong_as_double((long long)slot[0]) + eps);
+    if (dissip & 1) d = fmin(d, 3.0 / 14.0 * dx_min * dx_min / (__longlong_as_double((long long)slot[1]) + eps));
+    if (dissip & 2) d = fmin(d, 0.1 * dx_min * dx_min / (__longlong_as_double((long long)slot[2]) + eps));
+    dt[0] = d * cfl;
 }
 
 // ------------------------------------------------------------------ whole-buffer conversions
@@ -363,6 +384,8 @@ int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jf
     if (rc) return rc;
     if (!W || !U || !rhs || !workspace) return JFA_EINVAL;
     if (workspace_bytes < jfa_rhs_workspace_bytes(grid, scheme->dtype)) return JFA_EINVAL;
+    rc = check_transport(scheme, G, nullptr);  // dissipative fluxes: the caller's boundaries are periodic
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     if (scheme->dtype == JFA_F32) return fused_rhs_f32(W, U, G, scheme, (double*)workspace, rhs, st);
     return fused_rhs_f64(W, U, G, scheme, (double*)workspace, rhs, st);
@@ -421,12 +444,29 @@ int jfa_halo_update(void* W, void* U, const jfa_grid* grid, const int32_t bc[6],
 }
 
 int jfa_compute_dt(const void* W, const jfa_grid* grid, double gamma, double cfl, double eps, int32_t dtype,
-                   double* dt_dev, void* workspace, void* stream) {
+                   const jfa_transport* transport, double* dt_dev, void* workspace, void* stream) {
     Geo G;
     int rc = make_geo(grid, G);
     if (rc) return rc;
-    if (!W || !dt_dev || !workspace) return JFA_EINVAL;
+    if (!W || !dt_dev || !workspace || !(gamma > 1.0)) return JFA_EINVAL;
     if (dtype != JFA_F64 && dtype != JFA_F32) return JFA_EUNSUPPORTED;
+    jfa_scheme s = {};
+    s.gamma = gamma;
+    s.transport = transport;
+    rc = check_transport(&s, G, nullptr);
+    if (rc) return rc;
+    Params prm = gas_params(gamma);
+    if (transport) {
+        const Params t = make_params(&s);
+        prm.dissip = t.dissip;
+        prm.R = t.R;
+        prm.mu_ref = t.mu_ref;
+        prm.inv_T_ref = t.inv_T_ref;
+        prm.mu_exp = t.mu_exp;
+        prm.bulk = t.bulk;
+        prm.kappa_fac = t.kappa_fac;
+        prm.inv_cp = t.inv_cp;
+    }
     hipStream_t st = (hipStream_t)stream;
     unsigned long long* slot = (unsigned long long*)workspace;
     hipLaunchKernelGGL(zero_slot_kernel, dim3(1), dim3(1), 0, st, slot);
@@ -434,10 +474,10 @@ int jfa_compute_dt(const void* W, const jfa_grid* grid, double gamma, double cfl
     const int64_t nblk = (ncell + kBlock - 1) / kBlock;
     dim3 g((unsigned)(nblk < kCflBlocks ? nblk : kCflBlocks));
     if (dtype == JFA_F32)
-        hipLaunchKernelGGL((cfl_max_kernel<float>), g, dim3(kBlock), 0, st, (const float*)W, G, gamma, slot);
+        hipLaunchKernelGGL((cfl_max_kernel<float>), g, dim3(kBlock), 0, st, (const float*)W, G, prm, slot);
     else
-        hipLaunchKernelGGL((cfl_max_kernel<double>), g, dim3(kBlock), 0, st, (const double*)W, G, gamma, slot);
-    hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, st, slot, G.dx_min, cfl, eps, dt_dev);
+        hipLaunchKernelGGL((cfl_max_kernel<double>), g, dim3(kBlock), 0, st, (const double*)W, G, prm, slot);
+    hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(1), 0, st, slot, G.dx_min, cfl, eps, prm.dissip, dt_dev);
     return check_launch();
 }
 
@@ -497,6 +537,8 @@ int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, voi
         const int axis = i / 2;
         if (G.active[axis] && G.nh > G.n[axis] && bc[i] != JFA_BC_ZEROGRADIENT) return JFA_EINVAL;
     }
+    rc = check_transport(scheme, G, bc);
+    if (rc) return rc;
     const size_t ncell = (size_t)G.n[0] * G.n[1] * G.n[2];
     double* R = (double*)workspace;
     double* part = R + 5 * ncell;

@@ -107,7 +107,7 @@ class Ensemble2D:
         face halos, first CFL dt (material_fields_initializer.py:590-690)."""
         lib = _native.load()
         eps_fill = precision.get_eps()
-        ws16 = torch.empty(16, dtype=torch.uint8, device=self.device)
+        ws32 = torch.empty(32, dtype=torch.uint8, device=self.device)
         for c, prim in enumerate(primitives):
             W, U = self.state(c)
             W.fill_(eps_fill)
@@ -119,7 +119,7 @@ class Ensemble2D:
                                      self.scheme.dtype, _native.stream_ptr())
             _native.check(rc, "jfa_halo_update")
             rc = lib.jfa_compute_dt(_native.ptr(W), self.grids[c], self.gamma, self.cfl, self.eps, self.scheme.dtype,
-                                    _native.ptr(self.dt[c:c + 1]), _native.ptr(ws16), _native.stream_ptr())
+                                    None, _native.ptr(self.dt[c:c + 1]), _native.ptr(ws32), _native.stream_ptr())
             _native.check(rc, "jfa_compute_dt")
         self._graph = None
         self.steps_done = 0

@@ -75,7 +75,7 @@ class SimulationManager:
             "B": (torch.empty_like(W), torch.empty_like(W)),
             "C": (torch.empty_like(W), torch.empty_like(W)),
             "ws": torch.empty(ws_bytes, dtype=torch.uint8, device=W.device),
-            "dt_ws": torch.empty(16, dtype=torch.uint8, device=W.device),
+            "dt_ws": torch.empty(32, dtype=torch.uint8, device=W.device),
         }
         self._graph = None
         return self._bufs

@@ -109,11 +109,12 @@ def compute_time_step_size(primitives: torch.Tensor, temperature=None, levelset=
         return torch.full((1,), float(ti.fixed_timestep), dtype=torch.float64, device=primitives.device)
     _native.require_device(primitives, "primitives")
     dt = out if out is not None else torch.empty(1, dtype=torch.float64, device=primitives.device)
-    ws = workspace if workspace is not None else torch.empty(16, dtype=torch.uint8, device=primitives.device)
+    ws = workspace if workspace is not None else torch.empty(32, dtype=torch.uint8, device=primitives.device)
     eps = precision.get_eps() if primitives.dtype == torch.float64 else float(torch.finfo(torch.float32).eps)
     rc = _native.load().jfa_compute_dt(_native.ptr(primitives), domain_information.grid,
                                        material_manager.get_gamma(), float(ti.CFL), eps,
-                                       _native.dtype_code(primitives), _native.ptr(dt), _native.ptr(ws),
+                                       _native.dtype_code(primitives), getattr(domain_information, "transport", None),
+                                       _native.ptr(dt), _native.ptr(ws),
                                        _native.stream_ptr())
     _native.check(rc, "jfa_compute_dt")
     return dt

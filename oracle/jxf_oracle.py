@@ -523,6 +523,7 @@ class Transport:
     mu_exp: float = 0.0
     bulk_viscosity: float = 0.0
     prandtl: Optional[float] = None
+    viscous: bool = True                  # active_physics.is_viscous_flux; is_heat_flux: prandtl set
 
     def mu(self, T):
         return self.mu_ref * (T / self.T_ref) ** self.mu_exp if self.mu_exp != 0.0 else self.mu_ref * np.ones_like(T)
@@ -673,10 +674,11 @@ def dissipative_flux_xi(W, dom: Domain, axis: int, s: Setup):
     mu2 = tr.bulk_viscosity - 2.0 / 3.0 * mu1                  # :527
     g = velocity_gradient_at_faces(Wp, dom, axis)
     tau = [np.zeros_like(T_f) for _ in range(3)]
-    for k in range(3):
-        if axis in dom.active and k in dom.active:
-            tau[k] = mu1 * (g[axis][k] + g[k][axis])
-    tau[axis] = tau[axis] + mu2 * sum(g[k][k] for k in dom.active)
+    if tr.viscous:
+        for k in range(3):
+            if axis in dom.active and k in dom.active:
+                tau[k] = mu1 * (g[axis][k] + g[k][axis])
+        tau[axis] = tau[axis] + mu2 * sum(g[k][k] for k in dom.active)
     u_f = [0.5 * (L[1 + k] + R[1 + k]) for k in range(3)]     # central_2 of the velocity
     vel_tau = 0.0
     for k in dom.active:
@@ -791,7 +793,8 @@ def compute_dt(W, dom: Domain, s: Setup, eps=EPS64):
     tr = s.transport
     if tr is not None:  # time_step_size.py:112-135
         T = Wi[4] / (Wi[0] * tr.R)
-        dt = min(dt, 3.0 / 14.0 * dom.dx_min ** 2 / (np.max(tr.mu(T) / Wi[0]) + eps))
+        if tr.viscous:
+            dt = min(dt, 3.0 / 14.0 * dom.dx_min ** 2 / (np.max(tr.mu(T) / Wi[0]) + eps))
         if tr.prandtl is not None:
             dt = min(dt, 0.1 * dom.dx_min ** 2 / (np.max(tr.kappa(T, s.gamma) / (Wi[0] * tr.cp(s.gamma))) + eps))
     return dt * s.cfl

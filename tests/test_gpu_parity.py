@@ -217,8 +217,8 @@ def test_halo_and_dt_parity(bc):
             assert np.array_equal(Wh[sl], W[sl]), (a, side)
             assert rel(Uh[sl], U[sl]) < 1e-15, (a, side)
     dt = torch.empty(1, dtype=torch.float64, device="cuda")
-    ws = torch.empty(16, dtype=torch.uint8, device="cuda")
-    rc = lib.jfa_compute_dt(_native.ptr(Wd), native_grid(dom), 1.4, 0.7, o.EPS64, 0, _native.ptr(dt),
+    ws = torch.empty(32, dtype=torch.uint8, device="cuda")
+    rc = lib.jfa_compute_dt(_native.ptr(Wd), native_grid(dom), 1.4, 0.7, o.EPS64, 0, None, _native.ptr(dt),
                             _native.ptr(ws), _native.stream_ptr())
     _native.check(rc, "dt")
     assert abs(host(dt)[0] - o.compute_dt(W, dom, s)) <= 1e-15 * o.compute_dt(W, dom, s)

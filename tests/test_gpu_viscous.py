@@ -1,0 +1,142 @@
+"""GPU parity of the dissipative (viscous + heat) fluxes against the oracle (FP64, relative L-inf
+<= 1e-12): RHS, fused RK3 steps with the viscous/thermal time-step limits, and the standalone
+time step.  Periodic domains (the path's requirement)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import jxf_oracle as o
+from test_gpu_parity import dev, host, native_grid, random_state, rel, say
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+FACES = ("east", "west", "north", "south", "top", "bottom")
+
+TRANSPORTS = {
+    "powerlaw-heat": o.Transport(R=1.0, mu_ref=2e-3, T_ref=1.0, mu_exp=0.76, bulk_viscosity=0.0, prandtl=0.71),
+    "const-bulk": o.Transport(R=0.8, mu_ref=5e-3, bulk_viscosity=1e-3),
+    "heat-only": o.Transport(R=1.2, mu_ref=3e-3, prandtl=0.9, viscous=False),
+}
+
+
+def native_transport(tr):
+    from jaxfluids_amd import _native
+
+    return _native.make_transport(tr.R, tr.mu_ref, tr.T_ref, tr.mu_exp, tr.bulk_viscosity, tr.prandtl, tr.viscous)
+
+
+def native_scheme_tr(s, tr):
+    from jaxfluids_amd import _native
+
+    return _native.make_scheme(s.variable, s.stencil, s.riemann, s.signal_speed, s.frozen_state, 0, s.gamma,
+                               s.eps_stencil, native_transport(tr))
+
+
+def _setup(tr, stencil="WENO5-Z", variable="CHAR-PRIMITIVE"):
+    return o.Setup(gamma=1.4, stencil=stencil, variable=variable, riemann="HLLC", signal_speed="EINFELDT",
+                   bcs={f: "PERIODIC" for f in FACES}, transport=tr)
+
+
+@pytest.mark.parametrize("name", list(TRANSPORTS))
+@pytest.mark.parametrize("cells", [(11, 9, 8), (13, 10, 1)], ids=["3d", "2d"])
+def test_dissipative_rhs(name, cells):
+    from jaxfluids_amd import _native
+
+    say(f"[visc rhs] {name} {cells}")
+    tr = TRANSPORTS[name]
+    s = _setup(tr)
+    dom, W0 = random_state(cells, 3, seed=41, jump=False)
+    if cells[2] == 1:
+        W0 = W0[..., :1]
+    W, U = o.create_buffers(W0, dom, s)
+    ref = o.compute_rhs(W, U, dom, s)
+    lib = _native.load()
+    grid = native_grid(dom)
+    scheme = native_scheme_tr(s, tr)
+    ws = torch.empty(lib.jfa_rhs_workspace_bytes(grid, 0), dtype=torch.uint8, device="cuda")
+    out = torch.empty((5,) + dom.cells, dtype=torch.float64, device="cuda")
+    Wd, Ud = dev(W), dev(U)
+    rc = lib.jfa_compute_rhs(_native.ptr(Wd), _native.ptr(Ud), grid, scheme, _native.ptr(out), _native.ptr(ws),
+                             ws.numel(), _native.stream_ptr())
+    _native.check(rc, "jfa_compute_rhs")
+    got = host(out)
+    assert rel(got, ref) < TOL, rel(got, ref)
+    # the dissipative part is not negligible here
+    s0 = _setup(None)
+    conv = o.compute_rhs(W, U, dom, s0)
+    assert np.max(np.abs(ref - conv)) > 1e-6 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", ["powerlaw-heat", "const-bulk"])
+@pytest.mark.parametrize("cells", [(10, 8, 9), (12, 11, 1)], ids=["3d", "2d"])
+def test_dissipative_rk3_step_and_dt(name, cells):
+    """One fused RK3 step (3 stages, halos, next dt with the viscous / thermal limits)."""
+    from jaxfluids_amd import _native
+
+    say(f"[visc step] {name} {cells}")
+    tr = TRANSPORTS[name]
+    s = _setup(tr, stencil="WENO5-JS", variable="PRIMITIVE")
+    dom, W0 = random_state(cells, 4, seed=43, jump=False)
+    if cells[2] == 1:
+        W0 = W0[..., :1]
+    W, U = o.create_buffers(W0, dom, s)
+    dt = o.compute_dt(W, dom, s)
+    Wr, Ur = o.rk3_step(W, U, dt, dom, s)
+    lib = _native.load()
+    grid, scheme = native_grid(dom), native_scheme_tr(s, tr)
+    bcs = _native.bc_array({f: "PERIODIC" for f in FACES})
+    ws = torch.empty(lib.jfa_stage_workspace_bytes(grid, scheme), dtype=torch.uint8, device="cuda")
+    Wd, Ud = dev(W), dev(U)
+    Wb, Ub, Wc, Uc = (torch.empty_like(Wd) for _ in range(4))
+    dtd = torch.full((1,), dt, dtype=torch.float64, device="cuda")
+    P = _native.ptr
+    chain = [(Wd, Ud, Wb, Ub), (Wb, Ub, Wc, Uc), (Wc, Uc, Wd, Ud)]
+    for k, (wi, ui, wo, uo) in enumerate(chain):
+        rc = lib.jfa_rk3_stage_fused(P(wi), P(ui), P(Ud if k else None), P(wo), P(uo), grid, scheme, bcs, k, P(dtd),
+                                     0.0, s.cfl, o.EPS64, P(dtd) if k == 2 else P(None), P(ws), ws.numel(),
+                                     _native.stream_ptr())
+        _native.check(rc, "jfa_rk3_stage_fused")
+    I = dom.interior
+    assert rel(host(Wd)[I], Wr[I]) < TOL and rel(host(Ud)[I], Ur[I]) < TOL
+    assert abs(host(dtd)[0] - o.compute_dt(Wr, dom, s)) <= 1e-14 * dt
+
+
+def test_dissipative_dt_limits():
+    """jfa_compute_dt with a transport: convective, viscous and thermal limits (a viscosity large
+    enough that the viscous limit binds, then a conductivity large enough that the thermal one does)."""
+    from jaxfluids_amd import _native
+
+    say("[visc dt]")
+    dom, W0 = random_state((12, 10, 8), 3, seed=45, jump=False)
+    for tr in (o.Transport(mu_ref=0.5), o.Transport(mu_ref=0.05, prandtl=0.01), o.Transport(mu_ref=1e-6, prandtl=0.7)):
+        s = _setup(tr)
+        W, _ = o.create_buffers(W0, dom, s)
+        ref = o.compute_dt(W, dom, s)
+        dt = torch.empty(1, dtype=torch.float64, device="cuda")
+        ws = torch.empty(32, dtype=torch.uint8, device="cuda")
+        lib = _native.load()
+        rc = lib.jfa_compute_dt(_native.ptr(dev(W)), native_grid(dom), 1.4, s.cfl, o.EPS64, 0,
+                                native_transport(tr), _native.ptr(dt), _native.ptr(ws), _native.stream_ptr())
+        _native.check(rc, "jfa_compute_dt")
+        assert abs(host(dt)[0] - ref) <= 1e-14 * ref, (tr, host(dt)[0], ref)
+
+
+def test_dissipative_requires_periodic():
+    from jaxfluids_amd import _native
+
+    say("[visc periodic only]")
+    dom, W0 = random_state((8, 8, 8), 3, seed=47, jump=False)
+    s = _setup(TRANSPORTS["const-bulk"])
+    W, U = o.create_buffers(W0, dom, s)
+    lib = _native.load()
+    grid, scheme = native_grid(dom), native_scheme_tr(s, TRANSPORTS["const-bulk"])
+    bcs = _native.bc_array({**{f: "PERIODIC" for f in FACES}, "east": "ZEROGRADIENT", "west": "ZEROGRADIENT"})
+    ws = torch.empty(lib.jfa_stage_workspace_bytes(grid, scheme), dtype=torch.uint8, device="cuda")
+    Wd, Ud = dev(W), dev(U)
+    Wb, Ub = torch.empty_like(Wd), torch.empty_like(Ud)
+    dtd = torch.full((1,), 1e-3, dtype=torch.float64, device="cuda")
+    P = _native.ptr
+    rc = lib.jfa_rk3_stage_fused(P(Wd), P(Ud), P(None), P(Wb), P(Ub), grid, scheme, bcs, 0, P(dtd), 0.0, 0.5, o.EPS64,
+                                 P(None), P(ws), ws.numel(), _native.stream_ptr())
+    assert rc == _native.JFA_EUNSUPPORTED

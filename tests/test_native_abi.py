@@ -33,9 +33,11 @@ def test_library_exports_all_header_symbols():
 
 
 def test_struct_layouts():
-    # jfa_grid: 3 int32 + int32 + 3 double = 40 bytes; jfa_scheme: 6 int32 + 2 double = 40 bytes
+    # jfa_grid: 3 int32 + int32 + 3 double = 40 bytes; jfa_scheme: 6 int32 + 2 double + pointer = 48 bytes;
+    # jfa_transport: 2 int32 + 6 double = 56 bytes
     assert ctypes.sizeof(_native.JfaGrid) == 40
-    assert ctypes.sizeof(_native.JfaScheme) == 40
+    assert ctypes.sizeof(_native.JfaScheme) == 48
+    assert ctypes.sizeof(_native.JfaTransport) == 56
 
 
 def test_safe_lambda_parser():
