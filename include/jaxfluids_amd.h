@@ -65,11 +65,19 @@ typedef struct jfa_grid {
     double dx[3];   /* cell size per axis, (b - a) / n (domain/mesh_creation/homogenous.py:13) */
 } jfa_grid;
 
+/* Laws of the transport properties (materials/single_materials/material.py:82-124). */
+enum {
+    JFA_LAW_POWER = 0,       /* ref (T / T_ref)^param; param = 0: the constant ref (CUSTOM value,
+                                or a CUSTOM lambda that is a power law, e.g. "lambda T: 0.0028 * T**0.76") */
+    JFA_LAW_SUTHERLAND = 1,  /* ref ((T_ref + param) / (T + param)) (T / T_ref)^(3/2) (SUTHERLAND, :94-96) */
+    JFA_LAW_PRANDTL = 2      /* thermal conductivity only: cp mu / prandtl (PRANDTL, :115-116) */
+};
+
 /* Transport properties of the dissipative fluxes, second-order central (solvers/source_term_solver.py:
  * 188-350; flux assembly space_solver.py:565-587; dt limits time_integration/time_step_size.py:112-135):
- *   mu(T) = mu_ref (T / T_ref)^mu_exp   (the power law a CUSTOM viscosity lambda expresses,
- *                                         materials/single_materials/material.py:96-97; mu_exp = 0: constant)
- *   kappa = cp mu / prandtl              (PRANDTL model, material.py:115-116), cp = gamma R / (gamma - 1)
+ *   mu(T)    by mu_law    from (mu_ref, T_ref, mu_param)
+ *   kappa(T) by kappa_law from (kappa_ref, kappa_T_ref, kappa_param), or cp mu / prandtl;
+ *            cp = gamma R / (gamma - 1) (ideal_gas.py:32)
  *   T = p / (rho R)                      (ideal_gas.py:81-83)
  * The dissipative fluxes use central stencils across face-adjacent cells and their transverse
  * neighbours; they require PERIODIC boundaries on every active axis. */
@@ -77,11 +85,16 @@ typedef struct jfa_transport {
     int32_t viscous;        /* 1: viscous stresses tau and work u.tau (active_physics.is_viscous_flux) */
     int32_t heat;           /* 1: heat flux q = -kappa dT/dx (active_physics.is_heat_flux) */
     double R;               /* specific gas constant */
+    int32_t mu_law;         /* JFA_LAW_POWER / JFA_LAW_SUTHERLAND */
+    int32_t kappa_law;      /* JFA_LAW_POWER / JFA_LAW_SUTHERLAND / JFA_LAW_PRANDTL */
     double mu_ref;
     double T_ref;
-    double mu_exp;
-    double bulk_viscosity;
-    double prandtl;
+    double mu_param;        /* POWER: exponent; SUTHERLAND: constant C */
+    double bulk_viscosity;  /* mu_b of tau's (mu_b - 2/3 mu) div u term (source_term_solver.py:527) */
+    double prandtl;         /* JFA_LAW_PRANDTL */
+    double kappa_ref;
+    double kappa_T_ref;
+    double kappa_param;
 } jfa_transport;
 
 /* Numerical scheme of the high-order Godunov flux (data_types/numerical_setup/conservatives.py:35-42). */

@@ -33,8 +33,10 @@ class JfaGrid(ctypes.Structure):
 
 class JfaTransport(ctypes.Structure):
     _fields_ = [("viscous", ctypes.c_int32), ("heat", ctypes.c_int32), ("R", ctypes.c_double),
-                ("mu_ref", ctypes.c_double), ("T_ref", ctypes.c_double), ("mu_exp", ctypes.c_double),
-                ("bulk_viscosity", ctypes.c_double), ("prandtl", ctypes.c_double)]
+                ("mu_law", ctypes.c_int32), ("kappa_law", ctypes.c_int32),
+                ("mu_ref", ctypes.c_double), ("T_ref", ctypes.c_double), ("mu_param", ctypes.c_double),
+                ("bulk_viscosity", ctypes.c_double), ("prandtl", ctypes.c_double),
+                ("kappa_ref", ctypes.c_double), ("kappa_T_ref", ctypes.c_double), ("kappa_param", ctypes.c_double)]
 
 
 class JfaScheme(ctypes.Structure):
@@ -152,16 +154,35 @@ def make_grid(cells: Sequence[int], nh: int, dx: Sequence[float]) -> JfaGrid:
     return g
 
 
+LAW_POWER, LAW_SUTHERLAND, LAW_PRANDTL = 0, 1, 2
+
+
 def make_transport(R: float = 1.0, mu_ref: float = 0.0, T_ref: float = 1.0, mu_exp: float = 0.0,
-                   bulk_viscosity: float = 0.0, prandtl: Optional[float] = None, viscous: bool = True) -> JfaTransport:
-    """Dissipative-flux transport: mu(T) = mu_ref (T/T_ref)^mu_exp, kappa = cp mu / prandtl (heat flux
-    when prandtl is given)."""
+                   bulk_viscosity: float = 0.0, prandtl: Optional[float] = None, viscous: bool = True,
+                   mu_sutherland: Optional[float] = None, kappa_ref: Optional[float] = None,
+                   kappa_T_ref: float = 1.0, kappa_exp: float = 0.0,
+                   kappa_sutherland: Optional[float] = None) -> JfaTransport:
+    """Dissipative-flux transport (material.py:82-124).
+
+    Viscosity: mu_ref (T/T_ref)^mu_exp, or SUTHERLAND with constant mu_sutherland.  Heat flux when
+    prandtl (kappa = cp mu / prandtl) or kappa_ref (kappa_ref (T/kappa_T_ref)^kappa_exp, or
+    SUTHERLAND with kappa_sutherland) is given."""
     t = JfaTransport()
     t.viscous = 1 if viscous else 0
-    t.heat = 1 if prandtl is not None else 0
-    t.R, t.mu_ref, t.T_ref, t.mu_exp = float(R), float(mu_ref), float(T_ref), float(mu_exp)
+    t.heat = 1 if (prandtl is not None or kappa_ref is not None) else 0
+    t.R, t.mu_ref, t.T_ref = float(R), float(mu_ref), float(T_ref)
+    t.mu_law = LAW_SUTHERLAND if mu_sutherland is not None else LAW_POWER
+    t.mu_param = float(mu_sutherland if mu_sutherland is not None else mu_exp)
     t.bulk_viscosity = float(bulk_viscosity)
     t.prandtl = float(prandtl) if prandtl is not None else 1.0
+    if prandtl is not None:
+        t.kappa_law = LAW_PRANDTL
+        t.kappa_ref, t.kappa_T_ref, t.kappa_param = 0.0, 1.0, 0.0
+    else:
+        t.kappa_law = LAW_SUTHERLAND if kappa_sutherland is not None else LAW_POWER
+        t.kappa_ref = float(kappa_ref) if kappa_ref is not None else 0.0
+        t.kappa_T_ref = float(kappa_T_ref)
+        t.kappa_param = float(kappa_sutherland if kappa_sutherland is not None else kappa_exp)
     return t
 
 

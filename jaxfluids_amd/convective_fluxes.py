@@ -9,7 +9,7 @@ problem is solved -- the reference's three steps without intermediate arrays.
 """
 from __future__ import annotations
 
-from typing import Dict, Tuple
+from typing import Dict, Optional, Tuple
 
 import torch
 
@@ -61,11 +61,15 @@ class HighOrderGodunov(ConvectiveFluxSolver):
         self._names = (self.reconstruction_variable, stencil_name, riemann_name, ss_name, self.frozen_state)
         self._schemes = {}
 
-    def scheme(self, dtype_code: int) -> _native.JfaScheme:
-        if dtype_code not in self._schemes:
-            self._schemes[dtype_code] = _native.make_scheme(
-                *self._names, dtype_code, self.material_manager.get_gamma(), precision.get_spatial_stencil_eps())
-        return self._schemes[dtype_code]
+    def scheme(self, dtype_code: int, transport: Optional[_native.JfaTransport] = None) -> _native.JfaScheme:
+        """Native scheme of this flux; with a transport, the fused stage and the RHS add the
+        dissipative fluxes to it."""
+        key = (dtype_code, None if transport is None else id(transport))
+        if key not in self._schemes:
+            self._schemes[key] = _native.make_scheme(
+                *self._names, dtype_code, self.material_manager.get_gamma(), precision.get_spatial_stencil_eps(),
+                transport)
+        return self._schemes[key]
 
     def compute_flux_xi(self, primitives: torch.Tensor, conservatives: torch.Tensor, axis: int,
                         curvature=None, volume_fraction=None, apertures=None, ml_setup=None

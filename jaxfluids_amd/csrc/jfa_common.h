@@ -226,6 +226,33 @@ static inline int check_scheme(const jfa_scheme* s) {
     return JFA_OK;
 }
 
+static inline Law make_law(int kind, double ref, double T_ref, double param) {
+    Law l;
+    l.kind = kind;
+    l.ref = ref;
+    l.T_ref = T_ref;
+    l.inv_T_ref = T_ref > 0.0 ? 1.0 / T_ref : 0.0;
+    l.param = param;
+    return l;
+}
+
+// Dissipative-flux fields of p from a transport (NULL or neither flux: inviscid).
+static inline void set_transport(Params& p, const jfa_transport* t, double gamma) {
+    p.dissip = 0;
+    p.R = 1.0;
+    p.mu = p.kappa = make_law(JFA_LAW_POWER, 0.0, 1.0, 0.0);
+    p.bulk = p.kappa_fac = p.inv_cp = 0.0;
+    if (!t || !(t->viscous || t->heat)) return;
+    p.dissip = (t->viscous ? 1 : 0) | (t->heat ? 2 : 0);
+    p.R = t->R;
+    p.mu = make_law(t->mu_law, t->mu_ref, t->T_ref, t->mu_param);
+    p.kappa = make_law(t->kappa_law, t->kappa_ref, t->kappa_T_ref, t->kappa_param);
+    p.bulk = t->bulk_viscosity;
+    const double cp = gamma / (gamma - 1.0) * t->R;  // ideal_gas.py:32
+    p.kappa_fac = (t->heat && t->kappa_law == JFA_LAW_PRANDTL) ? cp / t->prandtl : 0.0;  // material.py:115-116
+    p.inv_cp = 1.0 / cp;
+}
+
 static inline Params make_params(const jfa_scheme* s) {
     Params p;
     p.gamma = s->gamma;
@@ -236,22 +263,16 @@ static inline Params make_params(const jfa_scheme* s) {
     p.eps_js = (144.0 / 169.0) * s->eps_stencil;
     p.signal_speed = s->signal_speed;
     p.frozen = s->frozen_state;
-    p.dissip = 0;
-    p.R = 1.0;
-    p.mu_ref = p.inv_T_ref = p.mu_exp = p.bulk = p.kappa_fac = p.inv_cp = 0.0;
-    const jfa_transport* t = s->transport;
-    if (t && (t->viscous || t->heat)) {
-        p.dissip = (t->viscous ? 1 : 0) | (t->heat ? 2 : 0);
-        p.R = t->R;
-        p.mu_ref = t->mu_ref;
-        p.inv_T_ref = 1.0 / t->T_ref;
-        p.mu_exp = t->mu_exp;
-        p.bulk = t->bulk_viscosity;
-        const double cp = s->gamma / (s->gamma - 1.0) * t->R;  // ideal_gas.py:32
-        p.kappa_fac = t->heat ? cp / t->prandtl : 0.0;          // material.py:115-116
-        p.inv_cp = 1.0 / cp;
-    }
+    set_transport(p, s->transport, s->gamma);
     return p;
+}
+
+// POWER or SUTHERLAND law with finite constants, ref >= 0, T_ref > 0 (and T_ref + C > 0).
+static inline bool valid_law(int kind, double ref, double T_ref, double param) {
+    if (kind != JFA_LAW_POWER && kind != JFA_LAW_SUTHERLAND) return false;
+    if (!(ref >= 0.0) || !(ref < INFINITY) || !(T_ref > 0.0) || !(T_ref < INFINITY)) return false;
+    if (!(param > -INFINITY) || !(param < INFINITY)) return false;
+    return kind == JFA_LAW_POWER || T_ref + param > 0.0;
 }
 
 // A transport is usable when its constants are finite and positive where needed; dissipative
@@ -259,8 +280,15 @@ static inline Params make_params(const jfa_scheme* s) {
 static inline int check_transport(const jfa_scheme* s, const Geo& G, const int32_t* bc) {
     const jfa_transport* t = s->transport;
     if (!t || !(t->viscous || t->heat)) return JFA_OK;
-    if (!(t->R > 0.0) || !(t->T_ref > 0.0) || !(t->mu_ref >= 0.0) || !(t->bulk_viscosity >= 0.0)) return JFA_EINVAL;
-    if (t->heat && !(t->prandtl > 0.0)) return JFA_EINVAL;
+    if (!(t->R > 0.0) || !(t->bulk_viscosity >= 0.0)) return JFA_EINVAL;
+    if (!valid_law(t->mu_law, t->mu_ref, t->T_ref, t->mu_param)) return JFA_EINVAL;
+    if (t->heat) {
+        if (t->kappa_law == JFA_LAW_PRANDTL) {
+            if (!(t->prandtl > 0.0) || !(t->prandtl < INFINITY)) return JFA_EINVAL;
+        } else if (!valid_law(t->kappa_law, t->kappa_ref, t->kappa_T_ref, t->kappa_param)) {
+            return JFA_EINVAL;
+        }
+    }
     if (bc)
         for (int f = 0; f < 6; ++f)
             if (G.active[f / 2] && bc[f] != JFA_BC_PERIODIC) return JFA_EUNSUPPORTED;
@@ -277,9 +305,7 @@ static inline Params gas_params(double gamma) {
     p.eps_js = (144.0 / 169.0) * 1e-30;
     p.signal_speed = JFA_SS_EINFELDT;
     p.frozen = JFA_FROZEN_ARITHMETIC;
-    p.dissip = 0;
-    p.R = 1.0;
-    p.mu_ref = p.inv_T_ref = p.mu_exp = p.bulk = p.kappa_fac = p.inv_cp = 0.0;
+    set_transport(p, nullptr, gamma);
     return p;
 }
 

@@ -30,6 +30,16 @@ struct Ax {
     static constexpr int m1 = AXIS == 0 ? 3 : (AXIS == 1 ? 1 : 2);
 };
 
+// Temperature law of one transport property (jfa_transport's JFA_LAW_*):
+// POWER ref (T / T_ref)^param, SUTHERLAND ref ((T_ref + param) / (T + param)) (T / T_ref)^(3/2).
+struct Law {
+    int kind;
+    double ref;
+    double T_ref;
+    double inv_T_ref;
+    double param;
+};
+
 struct Params {
     double gamma;
     double gm1;        // gamma - 1
@@ -42,11 +52,10 @@ struct Params {
     // dissipative fluxes (jfa_transport); dissip = 0: inviscid
     int dissip;        // bit 0: viscous stresses, bit 1: heat flux
     double R;          // specific gas constant
-    double mu_ref;     // mu(T) = mu_ref (T / T_ref)^mu_exp
-    double inv_T_ref;
-    double mu_exp;
+    Law mu;            // dynamic viscosity mu(T)
+    Law kappa;         // thermal conductivity kappa(T); kind JFA_LAW_PRANDTL: kappa_fac mu
     double bulk;       // bulk viscosity
-    double kappa_fac;  // kappa = kappa_fac mu, kappa_fac = cp / Pr
+    double kappa_fac;  // cp / Pr (JFA_LAW_PRANDTL)
     double inv_cp;     // 1 / cp
 };
 

@@ -4,12 +4,13 @@
 //
 // Face f along AXIS lies between the cells L = f-1 and R = f:
 //   T_f  = (T_L + T_R)/2,  T = p/(rho R)                          (central_2, ideal_gas.py:81-83)
-//   mu   = mu_ref (T_f/T_ref)^mu_exp,  mu2 = mu_b - 2/3 mu          (material.py:96-97, :527)
+//   mu   = mu(T_f) (POWER or SUTHERLAND law),  mu2 = mu_b - 2/3 mu   (material.py:82-101, :527)
 //   g[k][i] = d u_k / d x_i at the face: i == AXIS: (u_R - u_L)/dx (deriv_face_2);
 //            i != AXIS: the central derivative (u(+e_i) - u(-e_i))/(2 dx_i) at L and at R
 //            (deriv_center_2), averaged (central_2); inactive i: 0
 //   tau_k = mu (g[AXIS][k] + g[k][AXIS]) (active k), tau_AXIS += mu2 sum_k g[k][k]
-//   F[1+k] += -tau_k;  F[4] += -sum_k tau_k u_f,k + q,  q = -kappa (T_R - T_L)/dx, kappa = cp mu/Pr
+//   F[1+k] += -tau_k;  F[4] += -sum_k tau_k u_f,k + q,  q = -kappa(T_f) (T_R - T_L)/dx,
+//   kappa = cp mu / Pr (PRANDTL) or its own POWER / SUTHERLAND law      (material.py:103-124)
 // All accesses wrap periodically on the active axes (the reference fills edge halos with the
 // periodic copies for viscous runs), so no edge halo is read.
 #pragma once
@@ -18,8 +19,19 @@
 
 namespace jfa {
 
-__device__ __forceinline__ double viscosity(double T, const Params& prm) {
-    return prm.mu_exp == 0.0 ? prm.mu_ref : prm.mu_ref * pow(T * prm.inv_T_ref, prm.mu_exp);
+__device__ __forceinline__ double law_value(double T, const Law& l) {
+    if (l.kind == JFA_LAW_SUTHERLAND) {  // material.py:94-96, power3_2 (math/power_functions.py:29-30)
+        const double x = T * l.inv_T_ref;
+        return l.ref * ((l.T_ref + l.param) / (T + l.param)) * (x * sqrt(x));
+    }
+    return l.param == 0.0 ? l.ref : l.ref * pow(T * l.inv_T_ref, l.param);
+}
+
+__device__ __forceinline__ double viscosity(double T, const Params& prm) { return law_value(T, prm.mu); }
+
+// thermal conductivity at temperature T, where mu = viscosity(T)
+__device__ __forceinline__ double conductivity(double T, double mu, const Params& prm) {
+    return prm.kappa.kind == JFA_LAW_PRANDTL ? prm.kappa_fac * mu : law_value(T, prm.kappa);
 }
 
 // buffer index of interior cell c, coordinates wrapped onto [0, n) on the active axes
@@ -110,7 +122,7 @@ __device__ __forceinline__ void dissipative_face(const T* __restrict__ W, const 
         for (int k = 0; k < 3; ++k) F[1 + k] = F[1 + k] + (-tau[k]);
         d4 = -vel_tau;
     }
-    if (prm.dissip & 2) d4 = d4 + (-(prm.kappa_fac * mu) * ((TR - TL) * G.one_dx[AXIS]));
+    if (prm.dissip & 2) d4 = d4 + (-conductivity(Tf, mu, prm) * ((TR - TL) * G.one_dx[AXIS]));
     F[4] = F[4] + d4;
 }
 
@@ -119,7 +131,7 @@ __device__ __forceinline__ void diffusivities(const double w[5], const Params& p
     const double T = w[4] / (w[0] * prm.R);
     const double mu = viscosity(T, prm);
     nu = (prm.dissip & 1) ? mu / w[0] : 0.0;
-    alpha = (prm.dissip & 2) ? prm.kappa_fac * mu / w[0] * prm.inv_cp : 0.0;
+    alpha = (prm.dissip & 2) ? conductivity(T, mu, prm) / w[0] * prm.inv_cp : 0.0;
 }
 
 }  // namespace jfa

@@ -456,17 +456,7 @@ int jfa_compute_dt(const void* W, const jfa_grid* grid, double gamma, double cfl
     rc = check_transport(&s, G, nullptr);
     if (rc) return rc;
     Params prm = gas_params(gamma);
-    if (transport) {
-        const Params t = make_params(&s);
-        prm.dissip = t.dissip;
-        prm.R = t.R;
-        prm.mu_ref = t.mu_ref;
-        prm.inv_T_ref = t.inv_T_ref;
-        prm.mu_exp = t.mu_exp;
-        prm.bulk = t.bulk;
-        prm.kappa_fac = t.kappa_fac;
-        prm.inv_cp = t.inv_cp;
-    }
+    set_transport(prm, transport, gamma);
     hipStream_t st = (hipStream_t)stream;
     unsigned long long* slot = (unsigned long long*)workspace;
     hipLaunchKernelGGL(zero_slot_kernel, dim3(1), dim3(1), 0, st, slot);

@@ -42,6 +42,15 @@ class PositivitySetup(NamedTuple):
     flux_limiter: Any = False
 
 
+class DissipativeFluxesSetup(NamedTuple):
+    """conservatives.py:69-74 (stencil names; the built path is second-order central)"""
+    reconstruction_stencil: str = "CENTRAL2"
+    derivative_stencil_center: str = "CENTRAL2"
+    derivative_stencil_face: str = "CENTRAL2"
+    is_laplacian: bool = False
+    second_derivative_stencil_center: str = "CENTRAL2"
+
+
 class ConservativesSetup(NamedTuple):
     halo_cells: int
     time_integration: TimeIntegrationSetup
@@ -84,10 +93,28 @@ class DomainSetup(NamedTuple):
     ranges: Tuple[Tuple[float, float], ...]
 
 
+class TransportSetup(NamedTuple):
+    """Transport properties (data_types/case_setup/material_properties.py, TransportPropertiesSetup),
+    resolved to laws: viscosity mu_ref (T/T_ref)^mu_exp or Sutherland (mu_sutherland = C);
+    conductivity cp mu / prandtl, or kappa_ref (T/kappa_T_ref)^kappa_exp or Sutherland
+    (kappa_sutherland = C); None where a property is not given."""
+    mu_ref: float = 0.0
+    T_ref: float = 1.0
+    mu_exp: float = 0.0
+    mu_sutherland: Optional[float] = None
+    bulk_viscosity: float = 0.0
+    prandtl: Optional[float] = None
+    kappa_ref: Optional[float] = None
+    kappa_T_ref: float = 1.0
+    kappa_exp: float = 0.0
+    kappa_sutherland: Optional[float] = None
+
+
 class MaterialSetup(NamedTuple):
     model: str = "IdealGas"
     specific_heat_ratio: float = 1.4
     specific_gas_constant: float = 1.0
+    transport: Optional[TransportSetup] = None
 
 
 class CaseSetup(NamedTuple):

@@ -19,9 +19,9 @@ import numpy as np
 
 from .config import precision
 from .convective_fluxes import DICT_CONVECTIVE_SOLVER
-from .data_types import (ActivePhysicsSetup, CaseSetup, ConservativesSetup, ConvectiveFluxesSetup, DomainSetup,
-                         GeneralSetup, HighOrderGodunovSetup, MaterialSetup, NumericalSetup, PositivitySetup,
-                         PrecisionSetup, TimeIntegrationSetup)
+from .data_types import (ActivePhysicsSetup, CaseSetup, ConservativesSetup, ConvectiveFluxesSetup,
+                         DissipativeFluxesSetup, DomainSetup, GeneralSetup, HighOrderGodunovSetup, MaterialSetup,
+                         NumericalSetup, PositivitySetup, PrecisionSetup, TimeIntegrationSetup, TransportSetup)
 from .riemann_solvers import DICT_RIEMANN_SOLVER, DICT_SIGNAL_SPEEDS
 from .stencils import DICT_SPATIAL_RECONSTRUCTION
 from .time_integration import DICT_TIME_INTEGRATION
@@ -98,6 +98,25 @@ def _load(setup: Union[str, Dict]) -> Dict:
 
 
 # ------------------------------------------------------------------ readers
+def read_dissipative_fluxes(d: Dict) -> DissipativeFluxesSetup:
+    """read_conservatives.py:374-430 with the reference's defaults (CENTRAL4); the stencils built
+    here are second-order central: reconstruction central_2, face derivative deriv_face_2,
+    centre derivative deriv_center_2, no Laplacian form."""
+    setup = DissipativeFluxesSetup(
+        reconstruction_stencil=d.get("reconstruction_stencil", "CENTRAL4"),
+        derivative_stencil_center=d.get("derivative_stencil_center", "CENTRAL4"),
+        derivative_stencil_face=d.get("derivative_stencil_face", "CENTRAL4"),
+        is_laplacian=bool(d.get("is_laplacian", False)),
+        second_derivative_stencil_center=d.get("second_derivative_stencil_center", "CENTRAL4"))
+    first = (setup.reconstruction_stencil, setup.derivative_stencil_center, setup.derivative_stencil_face)
+    if any(v != "CENTRAL2" for v in first) or setup.is_laplacian:
+        raise NotImplementedError(
+            "dissipative_fluxes: the second-order central stencils are built (reconstruction_stencil, "
+            f"derivative_stencil_center, derivative_stencil_face = 'CENTRAL2', is_laplacian false); got {first}"
+            f"{', is_laplacian' if setup.is_laplacian else ''}")
+    return setup
+
+
 def read_numerical_setup(d: Dict) -> NumericalSetup:
     """read_conservatives.py:20-203 (convective-flux keys)."""
     assert "conservatives" in d, "numerical_setup: key 'conservatives' is required"
@@ -140,19 +159,86 @@ def read_numerical_setup(d: Dict) -> NumericalSetup:
     pos = c.get("positivity", {})
     if pos.get("is_interpolation_limiter") or pos.get("flux_limiter"):
         raise NotImplementedError("positivity limiters are not on this path")
-    cons = ConservativesSetup(halo_cells, TimeIntegrationSetup(DICT_TIME_INTEGRATION[integrator], cfl, fixed), conv)
     ap = d.get("active_physics", {})
     active = ActivePhysicsSetup(
         is_convective_flux=bool(ap.get("is_convective_flux", True)),
         is_viscous_flux=bool(ap.get("is_viscous_flux", False)),
         is_heat_flux=bool(ap.get("is_heat_flux", False)),
         is_volume_force=bool(ap.get("is_volume_force", False)))
-    if active.is_viscous_flux or active.is_heat_flux or active.is_volume_force or not active.is_convective_flux:
-        raise NotImplementedError("only the convective flux is built on this path")
+    if active.is_volume_force or not active.is_convective_flux:
+        raise NotImplementedError("the convective flux (with optional viscous and heat fluxes) is built on this path")
+    dissipative = read_dissipative_fluxes(c.get("dissipative_fluxes", {})) \
+        if (active.is_viscous_flux or active.is_heat_flux) else None
+    cons = ConservativesSetup(halo_cells, TimeIntegrationSetup(DICT_TIME_INTEGRATION[integrator], cfl, fixed), conv,
+                              dissipative)
     pr = d.get("precision", {})
     prec = PrecisionSetup(bool(pr.get("is_double_precision_compute", True)),
                           bool(pr.get("is_double_precision_output", True)))
     return NumericalSetup(cons, active, prec)
+
+
+_PROBE_T = np.array([0.05, 0.3, 0.7, 1.0, 1.3, 2.0, 3.7, 11.0, 260.0])
+
+
+def power_law(value: Any, what: str):
+    """(ref, T_ref, exponent) of a CUSTOM property value (read_material_manager.py:239-252,
+    :304-316): a number is the constant ref; a lambda of T is accepted when it is a power law
+    ref T^n, checked at probe temperatures from 0.05 to 260 (relative 1e-12), as the built
+    transport laws evaluate it."""
+    if isinstance(value, (int, float)) and not isinstance(value, bool):
+        if not value >= 0.0:
+            raise ValueError(f"{what}: value must be >= 0")
+        return float(value), 1.0, 0.0
+    f = parse_lambda(value) if isinstance(value, str) else value
+    y = np.asarray(f(_PROBE_T), dtype=np.float64) * np.ones_like(_PROBE_T)
+    ref = float(np.asarray(f(np.array([1.0])), dtype=np.float64).reshape(-1)[0])
+    if not (ref >= 0.0 and np.all(np.isfinite(y))):
+        raise ValueError(f"{what}: {value!r} must be finite and >= 0")
+    if ref == 0.0:
+        n = 0.0
+    else:
+        n = math.log2(float(np.asarray(f(np.array([2.0])), dtype=np.float64).reshape(-1)[0]) / ref)
+        n = float(f"{n:.12g}")  # the exponent the lambda was written with
+    if not np.allclose(y, ref * _PROBE_T ** n, rtol=1e-12, atol=0.0):
+        raise NotImplementedError(
+            f"{what}: CUSTOM lambdas are built as power laws ref * T**n (or constants); {value!r} is not one")
+    return ref, 1.0, n
+
+
+def read_transport(t: Any) -> Any:
+    """material_properties/transport (read_material_manager.py:201-355) resolved to laws; None
+    when absent.  dynamic_viscosity: CUSTOM (number or power-law lambda of T) or SUTHERLAND
+    [mu_ref, T_ref, C]; bulk_viscosity; thermal_conductivity: CUSTOM (number or power-law lambda),
+    PRANDTL (prandtl_number) or SUTHERLAND [kappa_ref, T_ref, C]."""
+    if not t:
+        return None
+    kw = {}
+    dv = t.get("dynamic_viscosity", {})
+    model = dv.get("model", "CUSTOM")
+    if model == "CUSTOM":
+        kw["mu_ref"], kw["T_ref"], kw["mu_exp"] = power_law(dv.get("value", 0.0), "dynamic_viscosity")
+    elif model == "SUTHERLAND":
+        mu_ref, T_ref, C = (float(v) for v in dv["sutherland_parameters"])
+        kw.update(mu_ref=mu_ref, T_ref=T_ref, mu_sutherland=C)
+    else:
+        raise ValueError(f"dynamic_viscosity model {model!r}: one of ('CUSTOM', 'SUTHERLAND')")
+    kw["bulk_viscosity"] = float(t.get("bulk_viscosity", 0.0))
+    if kw["bulk_viscosity"] < 0.0:
+        raise ValueError("bulk_viscosity must be >= 0")
+    tc = t.get("thermal_conductivity", {})
+    model = tc.get("model", "CUSTOM")
+    if model == "CUSTOM":
+        kw["kappa_ref"], kw["kappa_T_ref"], kw["kappa_exp"] = power_law(tc.get("value", 0.0), "thermal_conductivity")
+    elif model == "PRANDTL":
+        kw["prandtl"] = float(tc["prandtl_number"])
+        if not kw["prandtl"] > 0.0:
+            raise ValueError("prandtl_number must be > 0")
+    elif model == "SUTHERLAND":
+        k_ref, T_ref, C = (float(v) for v in tc["sutherland_parameters"])
+        kw.update(kappa_ref=k_ref, kappa_T_ref=T_ref, kappa_sutherland=C)
+    else:
+        raise ValueError(f"thermal_conductivity model {model!r}: one of ('CUSTOM', 'PRANDTL', 'SUTHERLAND')")
+    return TransportSetup(**kw)
 
 
 def read_case_setup(d: Dict) -> CaseSetup:
@@ -179,8 +265,9 @@ def read_case_setup(d: Dict) -> CaseSetup:
     model = mp.get("model", "IdealGas")
     if model != "IdealGas":
         raise NotImplementedError(f"equation of state {model!r} is not on this path")
+    transport = read_transport(d.get("material_properties", {}).get("transport"))
     material = MaterialSetup(model, float(mp.get("specific_heat_ratio", 1.4)),
-                             float(mp.get("specific_gas_constant", 1.0)))
+                             float(mp.get("specific_gas_constant", 1.0)), transport)
     return CaseSetup(general, DomainSetup(cells, ranges), bcs, ic_calls, material)
 
 
@@ -206,5 +293,31 @@ class InputManager:
             assert cs.boundary_conditions[loc] != "INACTIVE", f"boundary {loc} of an active axis cannot be INACTIVE"
         self.material_manager = MaterialManager(IdealGas(cs.material.specific_heat_ratio,
                                                          cs.material.specific_gas_constant))
+        self.material_manager.transport = self._native_transport()
+        # compute_time_step_size reads the dissipative limits from here
+        self.domain_information.transport = self.material_manager.transport
         self.equation_information = EquationInformation(self.domain_information.active_axes_indices)
         self.equation_manager = EquationManager(self.material_manager, self.equation_information)
+
+    def _native_transport(self):
+        """jfa_transport of the active dissipative fluxes (None: inviscid).  The transport
+        properties are required for them (read_material_manager.py:211-221), and the built
+        central stencils wrap periodically, so every active boundary must be PERIODIC."""
+        from . import _native
+
+        ap = self.numerical_setup.active_physics
+        if not (ap.is_viscous_flux or ap.is_heat_flux):
+            return None
+        cs = self.case_setup
+        tr = cs.material.transport
+        assert tr is not None, "material_properties/transport is required for viscous or heat fluxes"
+        assert not ap.is_heat_flux or tr.prandtl is not None or tr.kappa_ref is not None, \
+            "material_properties/transport/thermal_conductivity is required for the heat flux"
+        for loc in self.domain_information.active_face_locations:
+            if cs.boundary_conditions[loc] != "PERIODIC":
+                raise NotImplementedError(
+                    f"dissipative fluxes are built for periodic domains; boundary {loc} is {cs.boundary_conditions[loc]}")
+        kw = tr._asdict()
+        if not ap.is_heat_flux:
+            kw["prandtl"] = kw["kappa_ref"] = None
+        return _native.make_transport(R=cs.material.specific_gas_constant, viscous=ap.is_viscous_flux, **kw)

@@ -68,7 +68,7 @@ class SimulationManager:
         if self._bufs is not None and self._bufs["key"] == key:
             return self._bufs
         lib = _native.load()
-        scheme = self.convective_flux_solver.scheme(_native.dtype_code(W))
+        scheme = self.convective_flux_solver.scheme(_native.dtype_code(W), self.space_solver.transport)
         ws_bytes = lib.jfa_stage_workspace_bytes(self.domain_information.grid, scheme)
         self._bufs = {
             "key": key,
@@ -84,7 +84,7 @@ class SimulationManager:
     def _fused_stage(self, W_in, U_in, U_n, W_out, U_out, stage: int, dt: torch.Tensor,
                      dt_next: Optional[torch.Tensor] = None) -> None:
         b = self._bufs
-        scheme = self.convective_flux_solver.scheme(_native.dtype_code(W_in))
+        scheme = self.convective_flux_solver.scheme(_native.dtype_code(W_in), self.space_solver.transport)
         eps = precision.get_eps() if W_in.dtype == torch.float64 else float(torch.finfo(torch.float32).eps)
         cfl = float(self.numerical_setup.conservatives.time_integration.CFL)
         rc = _native.load().jfa_rk3_stage_fused(

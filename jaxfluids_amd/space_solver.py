@@ -25,8 +25,10 @@ class SpaceSolver:
         self.convective_flux_solver = convective_flux_solver
         self.numerical_setup = numerical_setup
         ap = getattr(numerical_setup, "active_physics", None)
-        if ap is not None and (ap.is_viscous_flux or ap.is_heat_flux or ap.is_volume_force):
-            raise NotImplementedError("only the convective flux is built on this path")
+        if ap is not None and ap.is_volume_force:
+            raise NotImplementedError("volume forces are not on this path")
+        # viscous / heat fluxes (source_term_solver.py:188-350) when the material carries a transport
+        self.transport = getattr(material_manager, "transport", None)
         self._ws = {}
 
     def _workspace(self, like: torch.Tensor) -> torch.Tensor:
@@ -39,13 +41,14 @@ class SpaceSolver:
     def compute_rhs(self, conservatives: torch.Tensor, primitives: torch.Tensor, temperature=None,
                     physical_simulation_time: float = 0.0, physical_timestep_size: float = 0.0, *args,
                     **kwargs) -> Tuple[IntegrationBuffers, PositivityCounter, DiscretizationCounter]:
-        """Convective RHS of the interior cells, (5, Nx, Ny, Nz)."""
+        """RHS of the interior cells, (5, Nx, Ny, Nz): convective flux, plus the viscous and heat
+        fluxes when the material has a transport."""
         _native.require_device(primitives, "primitives")
         _native.require_device(conservatives, "conservatives")
         dom = self.domain_information
         rhs = torch.empty((5, *dom.global_number_of_cells), dtype=primitives.dtype, device=primitives.device)
         ws = self._workspace(primitives)
-        scheme = self.convective_flux_solver.scheme(_native.dtype_code(primitives))
+        scheme = self.convective_flux_solver.scheme(_native.dtype_code(primitives), self.transport)
         rc = _native.load().jfa_compute_rhs(_native.ptr(primitives), _native.ptr(conservatives), dom.grid, scheme,
                                            _native.ptr(rhs), _native.ptr(ws), ws.numel(), _native.stream_ptr())
         _native.check(rc, "jfa_compute_rhs")

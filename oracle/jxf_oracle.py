@@ -511,10 +511,14 @@ RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov}
 class Transport:
     """Transport properties of the dissipative fluxes (materials/single_materials/material.py:46-130).
 
-    Dynamic viscosity mu(T) = mu_ref (T / T_ref)^mu_exp -- the power law a CUSTOM lambda
-    expresses (material.py:96-97); mu_exp = 0 is a constant viscosity.  Thermal conductivity
-    kappa = cp mu / Pr (PRANDTL model, material.py:115-116) with cp = gamma R / (gamma - 1)
-    (ideal_gas.py:32); prandtl None switches the heat flux off.  T = p / (rho R) (ideal_gas.py:81-83).
+    Dynamic viscosity (get_dynamic_viscosity, material.py:82-101): with mu_sutherland None, the
+    power law mu_ref (T / T_ref)^mu_exp that a CUSTOM value or lambda expresses (mu_exp = 0: the
+    constant mu_ref); with mu_sutherland = C, SUTHERLAND mu_ref ((T_ref + C)/(T + C)) (T/T_ref)^1.5
+    (:94-96, power3_2 = x sqrt(x), math/power_functions.py:29-30).
+    Thermal conductivity (get_thermal_conductivity, :103-124): with prandtl set, PRANDTL
+    kappa = cp mu / Pr (:115-116), cp = gamma R / (gamma - 1) (ideal_gas.py:32); else with kappa_ref
+    set, the same two laws from (kappa_ref, kappa_T_ref, kappa_exp | kappa_sutherland); neither:
+    no heat flux.  T = p / (rho R) (ideal_gas.py:81-83).
     """
 
     R: float = 1.0
@@ -523,16 +527,34 @@ class Transport:
     mu_exp: float = 0.0
     bulk_viscosity: float = 0.0
     prandtl: Optional[float] = None
-    viscous: bool = True                  # active_physics.is_viscous_flux; is_heat_flux: prandtl set
+    viscous: bool = True                  # active_physics.is_viscous_flux
+    mu_sutherland: Optional[float] = None
+    kappa_ref: Optional[float] = None
+    kappa_T_ref: float = 1.0
+    kappa_exp: float = 0.0
+    kappa_sutherland: Optional[float] = None
+
+    @property
+    def heat(self) -> bool:               # active_physics.is_heat_flux
+        return self.prandtl is not None or self.kappa_ref is not None
+
+    @staticmethod
+    def _law(T, ref, T_ref, exp, C):
+        if C is not None:
+            x = T / T_ref
+            return ref * ((T_ref + C) / (T + C)) * (x * np.sqrt(x))
+        return ref * (T / T_ref) ** exp if exp != 0.0 else ref * np.ones_like(T)
 
     def mu(self, T):
-        return self.mu_ref * (T / self.T_ref) ** self.mu_exp if self.mu_exp != 0.0 else self.mu_ref * np.ones_like(T)
+        return self._law(T, self.mu_ref, self.T_ref, self.mu_exp, self.mu_sutherland)
 
     def cp(self, gamma):
         return gamma / (gamma - 1.0) * self.R
 
     def kappa(self, T, gamma):
-        return self.cp(gamma) * self.mu(T) / self.prandtl
+        if self.prandtl is not None:
+            return self.cp(gamma) * self.mu(T) / self.prandtl
+        return self._law(T, self.kappa_ref, self.kappa_T_ref, self.kappa_exp, self.kappa_sutherland)
 
 
 @dataclass
@@ -687,7 +709,7 @@ def dissipative_flux_xi(W, dom: Domain, axis: int, s: Setup):
     for k in range(3):
         D[1 + k] = -tau[k]
     D[4] = -vel_tau
-    if tr.prandtl is not None:
+    if tr.heat:
         q = -tr.kappa(T_f, s.gamma) * ((T_R - T_L) * dom.one_dx[axis])   # deriv_face_2
         D[4] = D[4] + q
     return D
@@ -795,7 +817,7 @@ def compute_dt(W, dom: Domain, s: Setup, eps=EPS64):
         T = Wi[4] / (Wi[0] * tr.R)
         if tr.viscous:
             dt = min(dt, 3.0 / 14.0 * dom.dx_min ** 2 / (np.max(tr.mu(T) / Wi[0]) + eps))
-        if tr.prandtl is not None:
+        if tr.heat:
             dt = min(dt, 0.1 * dom.dx_min ** 2 / (np.max(tr.kappa(T, s.gamma) / (Wi[0] * tr.cp(s.gamma))) + eps))
     return dt * s.cfl
 

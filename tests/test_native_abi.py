@@ -32,12 +32,35 @@ def test_library_exports_all_header_symbols():
     assert lib.jfa_version().decode().startswith("jaxfluids_amd")
 
 
-def test_struct_layouts():
-    # jfa_grid: 3 int32 + int32 + 3 double = 40 bytes; jfa_scheme: 6 int32 + 2 double + pointer = 48 bytes;
-    # jfa_transport: 2 int32 + 6 double = 56 bytes
-    assert ctypes.sizeof(_native.JfaGrid) == 40
-    assert ctypes.sizeof(_native.JfaScheme) == 48
-    assert ctypes.sizeof(_native.JfaTransport) == 56
+def _c_layout(struct, fields):
+    """sizeof and offsetof of a header struct, as the C compiler lays it out."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ["#include <stddef.h>", "#include <stdio.h>", f'#include "{HEADER}"', "int main(void) {",
+             f'    printf("%zu\\n", sizeof({struct}));']
+    lines += [f'    printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    lines += ["    return 0;", "}"]
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "layout.c"), os.path.join(d, "layout")
+        open(src, "w").write("\n".join(lines) + "\n")
+        subprocess.run(["gcc", "-std=c99", "-o", exe, src], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    return int(out[0]), [int(v) for v in out[1:]]
+
+
+@pytest.mark.parametrize("struct,cls", [("jfa_grid", "JfaGrid"), ("jfa_scheme", "JfaScheme"),
+                                        ("jfa_transport", "JfaTransport"), ("jfa_case2d", "JfaCase2D")])
+def test_struct_layouts(struct, cls):
+    """Every ctypes structure has the size and field offsets the C compiler gives the header's."""
+    ct = getattr(_native, cls)
+    fields = [f[0] for f in ct._fields_]
+    size, offsets = _c_layout(struct, fields)
+    assert ctypes.sizeof(ct) == size
+    assert [getattr(ct, f).offset for f in fields] == offsets
 
 
 def test_safe_lambda_parser():

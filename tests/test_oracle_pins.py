@@ -539,3 +539,47 @@ def test_viscous_time_step_limit():
     visc = 3 / 14 * dx ** 2 / 0.05
     therm = 0.1 * dx ** 2 / (1.4 / 0.4 * 0.05 / 0.71 / (1.4 / 0.4))
     assert abs(o.compute_dt(W, dom, s) - 0.5 * min(conv, visc, therm)) <= 1e-15
+
+
+def test_transport_law_special_cases():
+    """Sutherland's law: equals mu_ref at T = T_ref; with C = 0 it is the power law of exponent 1/2;
+    with air's constants (1.716e-5 Pa s, 273.15 K, 110.4 K) it gives the tabulated 1.846e-5 Pa s
+    at 300 K.  A PRANDTL conductivity is cp mu / Pr; a kappa law without prandtl is its own law."""
+    T = np.array([0.3, 0.9, 1.7, 4.0])
+    sl = o.Transport(mu_ref=0.02, T_ref=0.9, mu_sutherland=0.35)
+    assert abs(sl.mu(np.array([0.9]))[0] - 0.02) <= 1e-17
+    s0 = o.Transport(mu_ref=0.02, T_ref=0.9, mu_sutherland=0.0)
+    pw = o.Transport(mu_ref=0.02, T_ref=0.9, mu_exp=0.5)
+    assert np.allclose(s0.mu(T), pw.mu(T), rtol=1e-15, atol=0)
+    air = o.Transport(mu_ref=1.716e-5, T_ref=273.15, mu_sutherland=110.4)
+    assert abs(air.mu(np.array([300.0]))[0] - 1.846e-5) <= 1e-3 * 1.846e-5
+    # monotone increasing in T for C > 0, slower than T^1.5
+    m = sl.mu(T)
+    assert np.all(np.diff(m) > 0) and np.all(np.diff(m / T ** 1.5) < 0)
+    pr = o.Transport(R=0.7, mu_ref=0.1, mu_exp=0.6, prandtl=0.8)
+    assert np.allclose(pr.kappa(T, 1.4), 1.4 / 0.4 * 0.7 * 0.1 * T ** 0.6 / 0.8, rtol=1e-15, atol=0)
+    kl = o.Transport(mu_ref=0.1, kappa_ref=0.3, kappa_T_ref=2.0, kappa_exp=0.25)
+    assert kl.heat and np.allclose(kl.kappa(T, 1.4), 0.3 * (T / 2.0) ** 0.25, rtol=1e-15, atol=0)
+    assert not o.Transport(mu_ref=0.1).heat
+
+
+def test_heat_flux_with_variable_conductivity_law():
+    """u = 0, rho = 1, p(x): the energy rhs is (q_{m} - q_{m+1})/dx with q = -kappa(T_f) dT/dx
+    at the faces, kappa from its own (Sutherland) law at the face-averaged temperature; with a
+    constant kappa law it is kappa times the discrete Laplacian of T."""
+    dom = o.Domain((11, 3, 1), ((0, 1.0), (0, 1.0), (0, 1.0)), nh=3)
+    rng = np.random.default_rng(15)
+    p = 1.0 + 0.4 * rng.random(11)
+    W = np.zeros((5,) + dom.shape)
+    W[0] = 1.0
+    W[4][dom.interior[1:]] = p[:, None, None]
+    T = p / 1.3
+    Tf = 0.5 * (np.roll(T, 1) + T)
+    for tr, kf in ((o.Transport(R=1.3, kappa_ref=0.2), lambda t: 0.2 + 0 * t),
+                   (o.Transport(R=1.3, kappa_ref=0.2, kappa_T_ref=0.7, kappa_sutherland=0.5),
+                    lambda t: 0.2 * (1.2 / (t + 0.5)) * (t / 0.7) ** 1.5)):
+        rhs = _visc_rhs(W, dom, _periodic_setup(tr))
+        q = -kf(Tf) * (T - np.roll(T, 1)) / dom.dx[0]
+        ref = (q - np.roll(q, -1)) / dom.dx[0]
+        assert np.allclose(rhs[4][:, :, 0], ref[:, None], rtol=0, atol=1e-12 * np.max(np.abs(ref)))
+        assert np.max(np.abs(rhs[1:4])) == 0.0
