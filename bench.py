@@ -18,6 +18,8 @@ ensemble: the batched-ensemble workload (64 uneven 2-D Riemann problems, one lau
 stage for all cases) against plain per-case launches of the same stages.
 adjoint: the Kelvin-Helmholtz 256^2 adjoint (500 fixed-dt steps): backward / forward wall time
 and a finite-difference check of the gradient.
+adjoint_viscous: the viscous Taylor-Green 128^3 adjoint (Re 1600, 1000 fixed-dt steps, binomial
+checkpointing in 20 slots, enstrophy loss): the same measurements.
 """
 from __future__ import annotations
 
@@ -281,7 +283,7 @@ def bench_adjoint(nsteps: int = 500, n: int = 256, slots: int = 20, fd_check: bo
     import torch
 
     from jaxfluids_amd.adjoint import FixedDtStepper, binomial_table, kinetic_energy, rollout
-    from jaxfluids_amd.workloads import kelvin_helmholtz_primitives
+    from jaxfluids_amd.workloads import fourier_velocity_direction, kelvin_helmholtz_primitives
     from jaxfluids_amd import _native
 
     W0 = kelvin_helmholtz_primitives(n, seed=0)
@@ -330,20 +332,108 @@ def bench_adjoint(nsteps: int = 500, n: int = 256, slots: int = 20, fd_check: bo
            "adjoint_over_forward": res["store_every_step"]["backward_over_forward"],
            "modes": res, "checkpointed_gradient_identical": same, "loss": L0}
     if fd_check:
-        rng = np.random.default_rng(1)
-        d = np.zeros_like(W0)
-        d[1:3] = 1e-2 * rng.standard_normal((2,) + W0.shape[1:])
-        dd = torch.as_tensor(d, dtype=torch.float64, device="cuda")
+        out["fd_check"] = fd_check_along(lambda x: kinetic_energy(rollout(x, st, nsteps, 0), dV), W0d,
+                                         grads["store_every_step"], fourier_velocity_direction((n, n, 1), seed=1))
+    return out
 
-        def loss(h):
-            with torch.no_grad():
-                return float(kinetic_energy(rollout(W0d + h * dd, st, nsteps, 0), dV))
 
-        D = lambda h: (loss(h) - loss(-h)) / (2 * h)
-        fd = (4.0 * D(0.5e-4) - D(1e-4)) / 3.0
-        an = float(torch.sum(grads["store_every_step"] * dd))
-        out["fd_check"] = {"direction": "random velocity perturbation field (u, v), amplitude 1e-2",
-                           "richardson_fd": fd, "adjoint": an, "rel_error": abs(fd - an) / abs(an)}
+def fd_check_along(loss_fn, W0d, grad, direction, steps=(1e-4, 5e-5, 2.5e-5, 1.25e-5)):
+    """Central differences of loss_fn along `direction` at several steps, against the adjoint
+    gradient's projection.  The rollouts' losses are only piecewise smooth (Riemann-solver branch
+    switches, WENO weights near flat data), so the differences are reported at every step with
+    their relative spread; rel_error is that of the smallest step."""
+    import torch
+
+    dd = torch.as_tensor(direction, dtype=torch.float64, device=W0d.device)
+
+    def loss(h):
+        with torch.no_grad():
+            return float(loss_fn(W0d + h * dd))
+
+    D = {h: (loss(h) - loss(-h)) / (2 * h) for h in steps}
+    an = float(torch.sum(grad * dd))
+    vals = list(D.values())
+    return {"direction": "seeded (u, v) field on Fourier modes k = 1..8, amplitude 1e-2",
+            "central_fd": {f"{h:g}": v for h, v in D.items()}, "adjoint": an,
+            "rel_error": abs(D[min(steps)] - an) / abs(an),
+            "fd_spread": (max(vals) - min(vals)) / abs(an)}
+
+
+def bench_adjoint_viscous(nsteps: int = 1000, n: int = 128, slots: int = 20, store_steps: int = 100,
+                          fd_check: bool = True):
+    """Adjoint of the 3-D viscous Taylor-Green vortex: 128^3 periodic [0, 2pi)^3, Re 1600, Mach 0.1,
+    Prandtl 0.71, power-law viscosity mu = mu_ref (T / T0)^0.76, WENO5-Z PRIMITIVE + HLLC-Einfeldt
+    + RK3 with second-order central viscous and heat fluxes, fixed dt (CFL 0.5 of the initial
+    state), 1000 steps with binomial checkpointing in 20 snapshot slots (one per 50 steps), FP64;
+    loss = enstrophy at t_end; gradient w.r.t. the initial velocity field.  The adjoint/forward
+    ratio with every step stored is measured over the first `store_steps` steps (a full store of
+    1000 steps would hold 190 GB)."""
+    import torch
+
+    from jaxfluids_amd import _native
+    from jaxfluids_amd.adjoint import FixedDtStepper, binomial_table, enstrophy, rollout
+    from jaxfluids_amd.workloads import fourier_velocity_direction
+
+    W0 = tgv_primitives(n)
+    gamma, mach = 1.4, 0.1
+    p0 = 1.0 / (gamma * mach ** 2)
+    transport = _native.make_transport(R=1.0, mu_ref=1.0 / 1600.0, T_ref=p0, mu_exp=0.76, prandtl=0.71)
+    h = 2 * np.pi / n
+    c = np.sqrt(gamma * W0[4] / W0[0])
+    conv = h / float(np.max(np.abs(W0[1]) + np.abs(W0[2]) + np.abs(W0[3]) + 3 * c))
+    mu_max = float(np.max(1.0 / 1600.0 * (W0[4] / W0[0] / p0) ** 0.76 / W0[0]))
+    dt = 0.5 * min(conv, 3.0 / 14.0 * h * h / mu_max)  # time_step_size.py:103-121 at t = 0
+    faces = {f: "PERIODIC" for f in _native.FACE_LOCATIONS}
+    st = FixedDtStepper((n, n, n), ((0.0, 2 * np.pi),) * 3, faces, dt, nh=3, stencil="WENO5-Z", riemann="HLLC",
+                        signal_speed="EINFELDT", transport=transport)
+    dx = (h, h, h)
+    W0d = torch.as_tensor(W0, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    with torch.no_grad():
+        rollout(W0d, st, 3, 0)
+    x = W0d.clone().requires_grad_(True)
+    enstrophy(rollout(x, st, 3), dx).backward()
+
+    res = {}
+    with torch.no_grad():
+        t_fwd_s, _ = timed(lambda: rollout(W0d, st, store_steps, 0))
+    x = W0d.clone().requires_grad_(True)
+    t_f, lo = timed(lambda: enstrophy(rollout(x, st, store_steps, store_steps), dx))
+    t_b, _ = timed(lambda: lo.backward())
+    res["store_every_step"] = {"steps": store_steps, "forward_s": round(t_fwd_s, 4), "backward_s": round(t_b, 4),
+                               "backward_over_forward": round(t_b / t_fwd_s, 3)}
+    del x, lo
+    torch.cuda.empty_cache()
+    with torch.no_grad():
+        t_fwd, WT = timed(lambda: rollout(W0d, st, nsteps, 0))
+    L0 = float(enstrophy(WT, dx))
+    x = W0d.clone().requires_grad_(True)
+    t_f, lo = timed(lambda: enstrophy(rollout(x, st, nsteps, slots), dx))
+    f1 = st.forward_steps
+    t_b, _ = timed(lambda: lo.backward())
+    T, _ = binomial_table(nsteps, slots)
+    res["binomial"] = {"steps": nsteps, "snapshot_slots": slots, "forward_s": round(t_f, 4),
+                       "backward_s": round(t_b, 4), "backward_over_forward": round(t_b / t_fwd, 3),
+                       "recomputed_steps": int(st.forward_steps - f1), "schedule_steps": int(T[nsteps, slots])}
+    g = x.grad.detach()
+    out = {"workload": f"adjoint of 3-D viscous Taylor-Green {n}^3 periodic, Re 1600, Mach 0.1, Pr 0.71, power-law "
+                       f"viscosity (0.76), WENO5-Z PRIMITIVE + HLLC-Einfeldt + RK3 + CENTRAL2 viscous/heat fluxes, "
+                       f"fixed dt {dt:.6e}, {nsteps} steps, FP64, loss = enstrophy at t_end",
+           "forward_s": round(t_fwd, 4), "ms_per_forward_step": round(t_fwd / nsteps * 1e3, 4),
+           "forward_mcell_updates_per_s": round(n ** 3 * 3 * nsteps / t_fwd / 1e6, 1),
+           "adjoint_over_forward": res["store_every_step"]["backward_over_forward"],
+           "modes": res, "loss": L0, "grad_finite": bool(torch.isfinite(g).all().item())}
+    if fd_check:
+        out["fd_check"] = fd_check_along(lambda y: enstrophy(rollout(y, st, nsteps, 0), dx), W0d, g,
+                                         fourier_velocity_direction((n, n, n), seed=1),
+                                         steps=(4e-5, 2e-5, 1e-5))
     return out
 
 
@@ -358,10 +448,11 @@ def main():
     ap.add_argument("--variable", default="CHAR-PRIMITIVE")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("tgv", "ensemble", "adjoint", "all"), default="all",
+    ap.add_argument("--workload", choices=("tgv", "ensemble", "adjoint", "adjoint-viscous", "all"), default="all",
                     help="tgv: the headline line only; ensemble / adjoint: only that measurement; "
                          "all: headline line with the ensemble and adjoint measurements attached")
     ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--viscous-adjoint-steps", type=int, default=1000)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device-to-device copy of GIB GiB (5 launches): the reference "
@@ -392,6 +483,9 @@ def main():
         return
     if args.workload == "adjoint":
         print(json.dumps(bench_adjoint(args.adjoint_steps)), flush=True)
+        return
+    if args.workload == "adjoint-viscous":
+        print(json.dumps(bench_adjoint_viscous(args.viscous_adjoint_steps)), flush=True)
         return
     case, num = tgv_setups(args.n, args.nh, args.dtype, args.variable)
     im = InputManager(case, num)
@@ -464,6 +558,8 @@ def main():
         out["ensemble"] = bench_ensemble(args.ensemble_steps, 5)
         torch.cuda.empty_cache()
         out["adjoint"] = bench_adjoint(args.adjoint_steps)
+        torch.cuda.empty_cache()
+        out["adjoint_viscous"] = bench_adjoint_viscous(args.viscous_adjoint_steps)
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.n, args.nh, args.variable if args.variable != "CHAR-CONSERVATIVE"

@@ -199,7 +199,8 @@ int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, voi
 size_t jfa_stage_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme);
 
 /* ---- Reverse mode of the RK3 stage at fixed dt (FP64; PRIMITIVE reconstruction, WENO5-JS/-Z,
- * HLLC with ARITHMETIC/RUSANOV/DAVIS/EINFELDT speeds or Rusanov).  The reference obtains this
+ * HLLC with ARITHMETIC/RUSANOV/DAVIS/EINFELDT speeds or Rusanov; with scheme->transport, the
+ * viscous and heat fluxes too, on periodic domains).  The reference obtains this
  * derivative by jax.grad through SimulationManager.do_runge_kutta_stages
  * (simulation_manager.py:770-1047, RK3.py:27-62); here it is a hand-written VJP.
  * For the forward stage U_out = c0 U_in + c1 U_n + m dt R(prim(U_in)) with W_in = prim(U_in)
@@ -207,8 +208,9 @@ size_t jfa_stage_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme)
  *   lam_in += c0 lam_out + (dW/dU)^T halo^T (dR/dW)^T (m dt lam_out)      (c0 = 1 at stage 0)
  *   lam_n  += c1 lam_out                                                  (stages 1, 2)
  * on interior cells; halos of lam_* are neither read nor written.  lam_in, lam_n and lam_out
- * must be distinct.  workspace: >= jfa_stage_vjp_workspace_bytes(grid) bytes. */
-size_t jfa_stage_vjp_workspace_bytes(const jfa_grid* grid);
+ * must be distinct.  workspace: >= jfa_stage_vjp_workspace_bytes(grid, scheme) bytes (the face-halo
+ * cotangents, plus 48 doubles per cell with a transport). */
+size_t jfa_stage_vjp_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme);
 int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void* lam_n, const jfa_grid* grid,
                       const jfa_scheme* scheme, const int32_t bc[6], int32_t stage, double dt, void* workspace,
                       size_t workspace_bytes, void* stream);

@@ -88,7 +88,7 @@ SIGNATURES = {
     "jfa_prim_from_cons": (_I32, [_P, _P, _I64, _D, _I32, _P]),
     "jfa_rk3_stage_fused": (_I32, [_P, _P, _P, _P, _P, _PG, _PS, _PI32, _I32, _P, _D, _D, _D, _P, _P, _SZ, _P]),
     "jfa_stage_workspace_bytes": (_SZ, [_PG, _PS]),
-    "jfa_stage_vjp_workspace_bytes": (_SZ, [_PG]),
+    "jfa_stage_vjp_workspace_bytes": (_SZ, [_PG, _PS]),
     "jfa_rk3_stage_vjp": (_I32, [_P, _P, _P, _P, _PG, _PS, _PI32, _I32, _D, _P, _SZ, _P]),
     "jfa_batch2d_workspace_bytes": (_SZ, [_P, _I32, _I32]),
     "jfa_batch2d_plan": (_I32, [_P, _I32, _I32, _P, _SZ, _P]),

@@ -115,18 +115,22 @@ def cons_vjp(W: torch.Tensor, gU: torch.Tensor, gamma: float) -> torch.Tensor:
 class FixedDtStepper:
     """Fixed-dt RK3 steps through the fused HIP stage and their VJPs, FP64, primitive
     reconstruction.  ``cells``/``ranges``/``bcs`` as in the case setup; the scheme strings are
-    the reference's registry names."""
+    the reference's registry names; ``transport`` (``_native.make_transport``) adds the viscous
+    and heat fluxes on periodic domains."""
 
     def __init__(self, cells, ranges, bcs: Dict[str, str], dt: float, *, nh: int = 3, stencil: str = "WENO5-JS",
-                 riemann: str = "HLLC", signal_speed: str = "EINFELDT", gamma: float = 1.4, device=None) -> None:
+                 riemann: str = "HLLC", signal_speed: str = "EINFELDT", gamma: float = 1.4,
+                 transport: Optional[_native.JfaTransport] = None, device=None) -> None:
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
         self.cells = tuple(int(c) for c in cells)
         self.nh = int(nh)
         self.gamma = float(gamma)
         dx = [(r[1] - r[0]) / n for r, n in zip(ranges, self.cells)]
         self.grid = _native.make_grid(self.cells, self.nh, dx)
+        self.transport = transport  # viscous / heat fluxes (periodic domains); None: inviscid
         self.scheme = _native.make_scheme("PRIMITIVE", stencil, riemann, signal_speed if riemann == "HLLC" else None,
-                                          "ARITHMETIC", _native.F64, gamma, precision.get_spatial_stencil_eps())
+                                          "ARITHMETIC", _native.F64, gamma, precision.get_spatial_stencil_eps(),
+                                          transport)
         full = {f: bcs.get(f, "PERIODIC") for f in _native.FACE_LOCATIONS}
         self.bc = _native.bc_array(full)
         self.shape = (5,) + tuple(n + 2 * self.nh if n > 1 else 1 for n in self.cells)
@@ -137,7 +141,7 @@ class FixedDtStepper:
         self._C = (e(), e())
         self._ws = torch.empty(max(int(lib.jfa_stage_workspace_bytes(self.grid, self.scheme)), 16), dtype=torch.uint8,
                                device=self.device)
-        nb = int(lib.jfa_stage_vjp_workspace_bytes(self.grid))
+        nb = int(lib.jfa_stage_vjp_workspace_bytes(self.grid, self.scheme))
         if nb == 0:
             raise _native.JFAError("invalid grid for the adjoint")
         self._aws = torch.empty(nb, dtype=torch.uint8, device=self.device)
@@ -256,3 +260,21 @@ def rollout(W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: O
 def kinetic_energy(W: torch.Tensor, cell_volume: float) -> torch.Tensor:
     """Domain-integrated kinetic energy sum(0.5 rho |u|^2) dV of interior primitives."""
     return 0.5 * cell_volume * torch.sum(W[0] * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]))
+
+
+def enstrophy(W: torch.Tensor, dx) -> torch.Tensor:
+    """Domain-integrated enstrophy sum(0.5 |curl u|^2) dV of interior primitives on a periodic
+    grid, with second-order central derivatives (the derivative_stencil_center CENTRAL2 form,
+    stencils/derivative/deriv_center_2.py); inactive axes (extent 1) contribute no derivative."""
+    u = W[1:4]
+
+    def d(f, axis):
+        if f.shape[axis] == 1:
+            return torch.zeros_like(f)
+        return (torch.roll(f, -1, axis) - torch.roll(f, 1, axis)) / (2.0 * dx[axis])
+
+    wx = d(u[2], 1) - d(u[1], 2)
+    wy = d(u[0], 2) - d(u[2], 0)
+    wz = d(u[1], 0) - d(u[0], 1)
+    dV = float(np.prod([h for h in dx]))
+    return 0.5 * dV * torch.sum(wx * wx + wy * wy + wz * wz)

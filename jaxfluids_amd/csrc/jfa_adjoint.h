@@ -17,12 +17,15 @@
 // * adj_inner_kernel (innermost axis): one face per lane; a cell gathers the six window
 //   cotangents of the faces f = c-2..c+3 from lanes l..l-5 by wave shuffles; a wave owns 59
 //   cells (lanes 5..63).
+// * adj_dissip_face_kernel (with viscous / heat fluxes): the VJP of each cell's lower face on
+//   every axis into a face buffer, gathered per cell by adj_update_kernel.
 // * adj_update_kernel: halo^T, (dW/dU)^T and the stage blend per interior cell.
 // Face states are recomputed with the forward's own face_states, so branch choices (the sign
 // of S*, the min/max of the wave-speed estimates) are those of the forward evaluation.
 #pragma once
 
 #include "jfa_common.h"
+#include "jfa_dissipative.h"
 
 namespace jfa {
 
@@ -398,6 +401,203 @@ __device__ __forceinline__ void load_var6(const double* __restrict__ W, int64_t 
     for (int k = 0; k < 6; ++k) x[k] = ld(W, i * fs + br + (int64_t)(k - 3) * sa);
 }
 
+// ------------------------------------------------------------------ dissipative fluxes
+// d law / dT at T, where value = law_value(T, l): POWER n value / T; SUTHERLAND
+// value (3/2 / T - 1 / (T + C)).
+__device__ __forceinline__ double law_slope(double T, double value, const Law& l) {
+    if (l.kind == JFA_LAW_SUTHERLAND) return value * (1.5 / T - 1.0 / (T + l.param));
+    return l.param == 0.0 ? 0.0 : l.param * value / T;
+}
+
+// VJP of the dissipative flux (jfa_dissipative.h) of the face whose right cell is cr, for the
+// cotangent m of the face flux.  Adds the cotangents of the left and right cells' primitives to
+// gl, gr and returns Gt[j][k], j-th axis i != AXIS (ascending), velocity component k: the face
+// sends +Gt to u_k at L + e_i and R + e_i and -Gt to u_k at L - e_i and R - e_i (the transverse
+// central derivatives).  The clamp of T_f <= 0 has zero derivative.
+template <int AXIS>
+__device__ __forceinline__ void dissipative_face_vjp(const double* __restrict__ W, const Geo& G, const Params& prm,
+                                                     const int cr[3], const double m[5], double gl[5], double gr[5],
+                                                     double Gt[2][3]) {
+    DissState s;
+    dissipative_state<double, AXIS>(W, G, prm, cr, s);
+    double mub = 0.0, Tfb = 0.0, TLb = 0.0, TRb = 0.0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Gt[j][k] = 0.0;
+    if (prm.dissip & 1) {
+        // forward: tau_k = mu (g[A][k] + g[k][A]) (+ mu2 div for k = A), D[1+k] = -tau_k,
+        // D[4] = -sum_k tau_k u_f,k
+        const double mu2 = prm.bulk - 2.0 / 3.0 * s.mu;
+        double tau[3], uf[3], div = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            tau[k] = G.active[k] ? s.mu * (s.g[AXIS][k] + s.g[k][AXIS]) : 0.0;
+            uf[k] = 0.5 * (s.wl[1 + k] + s.wr[1 + k]);
+            if (G.active[k]) div = div + s.g[k][k];
+        }
+        tau[AXIS] = tau[AXIS] + mu2 * div;
+        double tb[3], gb[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            tb[k] = G.active[k] ? -m[1 + k] - m[4] * uf[k] : 0.0;
+            const double ufb = G.active[k] ? -m[4] * tau[k] : 0.0;
+            gl[1 + k] += 0.5 * ufb;
+            gr[1 + k] += 0.5 * ufb;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) gb[k][i] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (!G.active[k]) continue;
+            mub += tb[k] * (s.g[AXIS][k] + s.g[k][AXIS]);
+            gb[AXIS][k] += tb[k] * s.mu;
+            gb[k][AXIS] += tb[k] * s.mu;
+            gb[k][k] += tb[AXIS] * mu2;
+        }
+        mub += -2.0 / 3.0 * (tb[AXIS] * div);
+        // g[k][AXIS] = (u_R,k - u_L,k) / dx_AXIS
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            gr[1 + k] += gb[k][AXIS] * G.one_dx[AXIS];
+            gl[1 + k] -= gb[k][AXIS] * G.one_dx[AXIS];
+        }
+        // g[k][i] = (dL + dR) / 2, dX = (u(X + e_i) - u(X - e_i)) / (2 dx_i)
+        int j = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (i == AXIS) continue;
+            if (G.active[i])
+#pragma unroll
+                for (int k = 0; k < 3; ++k) Gt[j][k] = gb[k][i] * (0.25 * G.one_dx[i]);
+            ++j;
+        }
+    }
+    if (prm.dissip & 2) {
+        // q = -kappa(T_f) (T_R - T_L) / dx
+        const double kappa = conductivity(s.Tf, s.mu, prm);
+        const double qb = m[4];
+        const double kb = -qb * ((s.TR - s.TL) * G.one_dx[AXIS]);
+        TRb += -qb * kappa * G.one_dx[AXIS];
+        TLb += qb * kappa * G.one_dx[AXIS];
+        if (prm.kappa.kind == JFA_LAW_PRANDTL) mub += prm.kappa_fac * kb;
+        else Tfb += kb * law_slope(s.Tf, kappa, prm.kappa);
+    }
+    Tfb += mub * law_slope(s.Tf, s.mu, prm.mu);
+    if (!s.clamped) {
+        TLb += 0.5 * Tfb;
+        TRb += 0.5 * Tfb;
+    }
+    // T = p / (rho R)
+    gl[4] += TLb / (s.wl[0] * prm.R);
+    gl[0] -= TLb * s.TL / s.wl[0];
+    gr[4] += TRb / (s.wr[0] * prm.R);
+    gr[0] -= TRb * s.TR / s.wr[0];
+}
+
+// Face buffer of the dissipative VJP: per active axis A and interior cell c, the VJP of the
+// face between c - e_A and c (periodic: face n is face 0) as kDissSlots arrays of ncell doubles:
+// gl[5], gr[5], Gt[2][3].
+constexpr int kDissSlots = 16;
+
+__device__ __forceinline__ int64_t interior_flat(const Geo& G, const int c[3]) {
+    int w[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        w[i] = c[i];
+        if (G.active[i]) w[i] = w[i] < 0 ? w[i] + G.n[i] : (w[i] >= G.n[i] ? w[i] - G.n[i] : w[i]);
+    }
+    return ((int64_t)w[0] * G.n[1] + w[1]) * G.n[2] + w[2];
+}
+
+template <int A>
+__device__ __forceinline__ void dissip_face_store(const double* __restrict__ W, const double* __restrict__ lam,
+                                                  const Geo& G, const Params& prm, double ts, const int c[3],
+                                                  int64_t idx, int64_t ncell, double* __restrict__ FB) {
+    // face cotangent ts (lam_R - lam_L) / dx with the left cell wrapped: the forward's faces 0
+    // and n evaluate the same wrapped cells, so their two cotangents add up to this one
+    int cl[3] = {c[0], c[1], c[2]};
+    cl[A] -= 1;
+    const int64_t br = wrapped_index(G, c), bl = wrapped_index(G, cl);
+    double m[5], gl[5], gr[5], Gt[2][3];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        m[i] = ts * ((ld(lam, i * G.fs + br) - ld(lam, i * G.fs + bl)) * G.one_dx[A]);
+        gl[i] = 0.0;
+        gr[i] = 0.0;
+    }
+    dissipative_face_vjp<A>(W, G, prm, c, m, gl, gr, Gt);
+    double* out = FB + (int64_t)A * kDissSlots * ncell + idx;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        JFA_ACCESS(out + i * ncell, sizeof(double));
+        out[i * ncell] = gl[i];
+        JFA_ACCESS(out + (5 + i) * ncell, sizeof(double));
+        out[(5 + i) * ncell] = gr[i];
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            JFA_ACCESS(out + (10 + 3 * j + k) * ncell, sizeof(double));
+            out[(10 + 3 * j + k) * ncell] = Gt[j][k];
+        }
+}
+
+// One thread per interior cell: the VJPs of the cell's lower face on every active axis.
+template <int UNUSED = 0>
+__global__ __launch_bounds__(kBlock) void adj_dissip_face_kernel(const double* __restrict__ W,
+                                                                 const double* __restrict__ lam, Geo G, Params prm,
+                                                                 double ts, double* __restrict__ FB) {
+    int c[3];
+    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    if (!flat_index(ncell, G.n[1], G.n[2], c[0], c[1], c[2])) return;
+    const int64_t idx = ((int64_t)c[0] * G.n[1] + c[1]) * G.n[2] + c[2];
+    if (G.active[0]) dissip_face_store<0>(W, lam, G, prm, ts, c, idx, ncell, FB);
+    if (G.active[1]) dissip_face_store<1>(W, lam, G, prm, ts, c, idx, ncell, FB);
+    if (G.active[2]) dissip_face_store<2>(W, lam, G, prm, ts, c, idx, ncell, FB);
+}
+
+// Cotangent of cell c's primitives from the face buffer: gr of its lower faces, gl of its
+// upper faces (c + e_A, wrapped), and the transverse terms S(c - e_i) - S(c + e_i) with
+// S(x) = Gt(face x) + Gt(face x + e_A).
+__device__ __forceinline__ void dissip_gather(const double* __restrict__ FB, const Geo& G, const int c[3],
+                                              double g[5]) {
+    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+#pragma unroll
+    for (int A = 0; A < 3; ++A) {
+        if (!G.active[A]) continue;
+        const double* fa = FB + (int64_t)A * kDissSlots * ncell;
+        int x[3] = {c[0], c[1], c[2]};
+        const int64_t i0 = interior_flat(G, x);
+        x[A] += 1;
+        const int64_t i1 = interior_flat(G, x);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) g[i] += ld(fa, (5 + i) * ncell + i0) + ld(fa, i * ncell + i1);
+        int j = 0;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            if (t == A) continue;
+            if (G.active[t]) {
+#pragma unroll
+                for (int sgn = -1; sgn <= 1; sgn += 2) {
+                    int y[3] = {c[0], c[1], c[2]};
+                    y[t] += sgn;
+                    const int64_t j0 = interior_flat(G, y);
+                    y[A] += 1;
+                    const int64_t j1 = interior_flat(G, y);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const double S = ld(fa, (10 + 3 * j + k) * ncell + j0) + ld(fa, (10 + 3 * j + k) * ncell + j1);
+                        g[1 + k] += sgn < 0 ? S : -S;
+                    }
+                }
+            }
+            ++j;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ kernels
 constexpr int kAdjBlock = 256;
 
@@ -557,7 +757,8 @@ __global__ __launch_bounds__(kBlock) void adj_update_kernel(const double* __rest
                                                             const double* __restrict__ lam,
                                                             const double* __restrict__ gW, Geo G, Params prm,
                                                             BC6 bc, double c0, double c1, double* __restrict__ lam_in,
-                                                            double* __restrict__ lam_n) {
+                                                            double* __restrict__ lam_n,
+                                                            const double* __restrict__ FB) {
     int c[3];
     if (!flat_index((int64_t)G.n[0] * G.n[1] * G.n[2], G.n[1], G.n[2], c[0], c[1], c[2])) return;
     const int64_t b = (int64_t)(c[0] + G.off[0]) * G.s[0] + (int64_t)(c[1] + G.off[1]) * G.s[1] + (c[2] + G.off[2]);
@@ -570,6 +771,7 @@ __global__ __launch_bounds__(kBlock) void adj_update_kernel(const double* __rest
         l[i] = ld(lam, i * fs + b);
     }
     halo_gather(gW, G, bc, c, b, g);
+    if (FB) dissip_gather(FB, G, c, g);  // dissipative fluxes read interior cells only (wrapped)
     prim_vjp(w, g, prm, gu);
 #pragma unroll
     for (int i = 0; i < 5; ++i) {

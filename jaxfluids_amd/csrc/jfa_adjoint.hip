@@ -16,10 +16,17 @@ using namespace jfa;
 
 extern "C" {
 
-size_t jfa_stage_vjp_workspace_bytes(const jfa_grid* grid) {
+static size_t vjp_workspace_bytes(const Geo& G, const jfa_scheme* scheme) {
+    size_t n = (size_t)(5 * G.fs);
+    if (scheme && scheme->transport && (scheme->transport->viscous || scheme->transport->heat))
+        n += (size_t)3 * kDissSlots * ((size_t)G.n[0] * G.n[1] * G.n[2]);
+    return n * sizeof(double);
+}
+
+size_t jfa_stage_vjp_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme) {
     Geo G;
     if (make_geo(grid, G)) return 0;
-    return (size_t)(5 * G.fs) * sizeof(double);
+    return vjp_workspace_bytes(G, scheme);
 }
 
 int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void* lam_n, const jfa_grid* grid,
@@ -31,12 +38,11 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     rc = check_scheme(scheme);
     if (rc) return rc;
     if (scheme->dtype != JFA_F64 || scheme->recon_var != JFA_PRIMITIVE) return JFA_EUNSUPPORTED;
-    if (scheme->transport && (scheme->transport->viscous || scheme->transport->heat)) return JFA_EUNSUPPORTED;
     if (scheme->riemann == JFA_HLLC && scheme->signal_speed == JFA_SS_TORO) return JFA_EUNSUPPORTED;
     if (!W_in || !lam_out || !lam_in || !bc || !workspace || stage < 0 || stage > 2 || (stage > 0 && !lam_n))
         return JFA_EINVAL;
     if (lam_in == lam_out || lam_n == lam_out || (stage > 0 && lam_n == lam_in)) return JFA_EINVAL;
-    if (!(dt > 0.0) || workspace_bytes < (size_t)(5 * G.fs) * sizeof(double)) return JFA_EINVAL;
+    if (!(dt > 0.0) || workspace_bytes < vjp_workspace_bytes(G, scheme)) return JFA_EINVAL;
     BC6 b6;
     for (int f = 0; f < 6; ++f) {
         const int a = f / 2;
@@ -45,6 +51,8 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
         if (bc[f] < 0 || bc[f] > 2) return JFA_EUNSUPPORTED;
         if (bc[f] != JFA_BC_ZEROGRADIENT && G.nh > G.n[a]) return JFA_EINVAL;
     }
+    rc = check_transport(scheme, G, bc);  // dissipative fluxes: periodic boundaries
+    if (rc) return rc;
     // stage blend U_out = c0 U_in + c1 U_n + m dt R(U_in) (RK3.py:27-62)
     const double c0 = stage == 0 ? 1.0 : (stage == 1 ? 0.25 : 2.0 / 3.0);
     const double c1 = stage == 0 ? 0.0 : (stage == 1 ? 0.75 : 1.0 / 3.0);
@@ -60,8 +68,14 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     if (scheme->stencil == JFA_WENO5_JS) adj_sweeps_r<JFA_WENO5_JS>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
     else adj_sweeps_r<JFA_WENO5_Z>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
     const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    double* FB = nullptr;
+    if (prm.dissip) {
+        FB = gW + nz;  // every slot is written before it is read: no clearing
+        hipLaunchKernelGGL(adj_dissip_face_kernel<0>, flat_grid(ncell), dim3(kBlock), 0, st, W, lam, G, prm, m * dt,
+                           FB);
+    }
     hipLaunchKernelGGL(adj_update_kernel, flat_grid(ncell), dim3(kBlock), 0, st, W, lam, gW, G, prm, b6, c0, c1,
-                       static_cast<double*>(lam_in), stage > 0 ? static_cast<double*>(lam_n) : nullptr);
+                       static_cast<double*>(lam_in), stage > 0 ? static_cast<double*>(lam_n) : nullptr, FB);
     return check_launch();
 }
 

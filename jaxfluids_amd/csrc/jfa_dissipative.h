@@ -52,78 +52,103 @@ __device__ __forceinline__ void load_velocity(const T* __restrict__ W, int64_t f
     for (int k = 0; k < 3; ++k) u[k] = ld(W, (k + 1) * fs + b);
 }
 
-// adds the dissipative flux of the face whose right cell has interior coordinates cr
-// (cr[AXIS] in 0..n, the left cell is cr - e_AXIS) to F
+// Face state of the dissipative flux: the two cells' primitives, their temperatures, the face
+// temperature (clamped: T_f <= 0 replaced by the precision eps), the viscosity there and, for
+// viscous fluxes, the velocity-gradient tensor g[k][i] = d u_k / d x_i at the face.
+struct DissState {
+    double wl[5], wr[5];
+    double TL, TR, Tf;
+    bool clamped;
+    double mu;
+    double g[3][3];
+};
+
+// face whose right cell has interior coordinates cr (cr[AXIS] in 0..n, the left cell is cr - e_AXIS)
 template <class T, int AXIS>
-__device__ __forceinline__ void dissipative_face(const T* __restrict__ W, const Geo& G, const Params& prm,
-                                                 const int cr[3], double F[5]) {
+__device__ __forceinline__ void dissipative_state(const T* __restrict__ W, const Geo& G, const Params& prm,
+                                                  const int cr[3], DissState& s) {
     const int64_t fs = G.fs;
     int cl[3] = {cr[0], cr[1], cr[2]};
     cl[AXIS] -= 1;
     const int64_t bl = wrapped_index(G, cl), br = wrapped_index(G, cr);
-    double wl[5], wr[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-        wl[i] = ld(W, i * fs + bl);
-        wr[i] = ld(W, i * fs + br);
+        s.wl[i] = ld(W, i * fs + bl);
+        s.wr[i] = ld(W, i * fs + br);
     }
-    const double TL = wl[4] / (wl[0] * prm.R), TR = wr[4] / (wr[0] * prm.R);
-    double Tf = 0.5 * (TL + TR);
-    if (Tf <= 0.0) Tf = 2.220446049250313e-16;  // source_term_solver.py:301 (precision eps)
-    const double mu = viscosity(Tf, prm);
-    double d4 = 0.0;
-    if (prm.dissip & 1) {
-        double g[3][3];
+    s.TL = s.wl[4] / (s.wl[0] * prm.R);
+    s.TR = s.wr[4] / (s.wr[0] * prm.R);
+    s.Tf = 0.5 * (s.TL + s.TR);
+    s.clamped = s.Tf <= 0.0;
+    if (s.clamped) s.Tf = 2.220446049250313e-16;  // source_term_solver.py:301 (precision eps)
+    s.mu = viscosity(s.Tf, prm);
+    if (!(prm.dissip & 1)) return;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            if (!G.active[i]) {
+    for (int i = 0; i < 3; ++i) {
+        if (!G.active[i]) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) g[k][i] = 0.0;
-            } else if (i == AXIS) {
+            for (int k = 0; k < 3; ++k) s.g[k][i] = 0.0;
+        } else if (i == AXIS) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) g[k][i] = (wr[1 + k] - wl[1 + k]) * G.one_dx[i];
-            } else {
-                const double h = 0.5 * G.one_dx[i];
-                int c[3];
-                double up[3], um[3], dL[3], dR[3];
-                c[0] = cl[0]; c[1] = cl[1]; c[2] = cl[2];
-                c[i] += 1;
-                load_velocity(W, fs, wrapped_index(G, c), up);
-                c[i] -= 2;
-                load_velocity(W, fs, wrapped_index(G, c), um);
+            for (int k = 0; k < 3; ++k) s.g[k][i] = (s.wr[1 + k] - s.wl[1 + k]) * G.one_dx[i];
+        } else {
+            const double h = 0.5 * G.one_dx[i];
+            int c[3];
+            double up[3], um[3], dL[3], dR[3];
+            c[0] = cl[0]; c[1] = cl[1]; c[2] = cl[2];
+            c[i] += 1;
+            load_velocity(W, fs, wrapped_index(G, c), up);
+            c[i] -= 2;
+            load_velocity(W, fs, wrapped_index(G, c), um);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) dL[k] = (up[k] - um[k]) * h;
-                c[0] = cr[0]; c[1] = cr[1]; c[2] = cr[2];
-                c[i] += 1;
-                load_velocity(W, fs, wrapped_index(G, c), up);
-                c[i] -= 2;
-                load_velocity(W, fs, wrapped_index(G, c), um);
+            for (int k = 0; k < 3; ++k) dL[k] = (up[k] - um[k]) * h;
+            c[0] = cr[0]; c[1] = cr[1]; c[2] = cr[2];
+            c[i] += 1;
+            load_velocity(W, fs, wrapped_index(G, c), up);
+            c[i] -= 2;
+            load_velocity(W, fs, wrapped_index(G, c), um);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    dR[k] = (up[k] - um[k]) * h;
-                    g[k][i] = 0.5 * (dL[k] + dR[k]);
-                }
+            for (int k = 0; k < 3; ++k) {
+                dR[k] = (up[k] - um[k]) * h;
+                s.g[k][i] = 0.5 * (dL[k] + dR[k]);
             }
         }
-        const double mu2 = prm.bulk - 2.0 / 3.0 * mu;
+    }
+}
+
+// adds the dissipative flux of a face state to F
+template <int AXIS>
+__device__ __forceinline__ void dissipative_flux(const DissState& s, const Geo& G, const Params& prm, double F[5]) {
+    double d4 = 0.0;
+    if (prm.dissip & 1) {
+        const double mu2 = prm.bulk - 2.0 / 3.0 * s.mu;
         double tau[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) tau[k] = G.active[k] ? mu * (g[AXIS][k] + g[k][AXIS]) : 0.0;
+        for (int k = 0; k < 3; ++k) tau[k] = G.active[k] ? s.mu * (s.g[AXIS][k] + s.g[k][AXIS]) : 0.0;
         double div = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-            if (G.active[k]) div = div + g[k][k];
+            if (G.active[k]) div = div + s.g[k][k];
         tau[AXIS] = tau[AXIS] + mu2 * div;
         double vel_tau = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-            if (G.active[k]) vel_tau = vel_tau + tau[k] * (0.5 * (wl[1 + k] + wr[1 + k]));
+            if (G.active[k]) vel_tau = vel_tau + tau[k] * (0.5 * (s.wl[1 + k] + s.wr[1 + k]));
 #pragma unroll
         for (int k = 0; k < 3; ++k) F[1 + k] = F[1 + k] + (-tau[k]);
         d4 = -vel_tau;
     }
-    if (prm.dissip & 2) d4 = d4 + (-conductivity(Tf, mu, prm) * ((TR - TL) * G.one_dx[AXIS]));
+    if (prm.dissip & 2) d4 = d4 + (-conductivity(s.Tf, s.mu, prm) * ((s.TR - s.TL) * G.one_dx[AXIS]));
     F[4] = F[4] + d4;
+}
+
+// adds the dissipative flux of the face whose right cell has interior coordinates cr to F
+template <class T, int AXIS>
+__device__ __forceinline__ void dissipative_face(const T* __restrict__ W, const Geo& G, const Params& prm,
+                                                 const int cr[3], double F[5]) {
+    DissState s;
+    dissipative_state<T, AXIS>(W, G, prm, cr, s);
+    dissipative_flux<AXIS>(s, G, prm, F);
 }
 
 // cell diffusivities for the time-step limits: nu = mu/rho, alpha = kappa/(rho cp)

@@ -93,3 +93,23 @@ def kelvin_helmholtz_primitives(n: int = 256, seed: int = 0, amplitude: float = 
     W[2, ..., 0] = v
     W[4, ..., 0] = p0
     return W
+
+
+def fourier_velocity_direction(cells, seed: int = 0, modes: int = 8, amplitude: float = 1e-2) -> np.ndarray:
+    """A seeded (u, v) field on the periodic unit-spaced grid of `cells` made of the Fourier modes
+    k = 1..modes along every active axis (random phases, amplitude/modes each); density, w and
+    pressure components are zero.  The direction of the finite-difference checks of the adjoint
+    workloads: it lies in the span of their initial perturbations."""
+    rng = np.random.default_rng(seed)
+    cells = tuple(int(c) for c in cells)
+    axes = [(np.arange(n) + 0.5) / n for n in cells]
+    X = np.meshgrid(*axes, indexing="ij")
+    d = np.zeros((5,) + cells)
+    for comp in (1, 2):
+        for a, n in enumerate(cells):
+            if n == 1:
+                continue
+            ph = rng.uniform(0, 2 * np.pi, modes)
+            for k in range(1, modes + 1):
+                d[comp] += amplitude / modes * np.cos(2 * np.pi * k * X[a] + ph[k - 1])
+    return d

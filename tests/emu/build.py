@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "jaxfluids_amd", "csrc")
 OUT = os.path.join(HERE, "_build", "libjfa_emu.so")
 SOURCES = [os.path.join(CSRC, f) for f in ("jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip")]
-DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("jfa_batch.h", "jfa_common.h", "jfa_device.h", "jfa_adjoint.h")] + [
+DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("jfa_batch.h", "jfa_common.h", "jfa_device.h", "jfa_adjoint.h", "jfa_dissipative.h")] + [
     os.path.join(HERE, "emu_runtime.cpp"), os.path.join(HERE, "hip", "hip_runtime.h"), __file__]
 
 

@@ -85,3 +85,29 @@ def test_reverse_sweep_on_a_scalar_map(n, s, prestore):
         assert steps[0] == T[n, s]
     else:
         assert steps[0] <= T[n, s] and pre == len(first_level_snapshots(n, s))
+
+
+def test_enstrophy_of_taylor_green_closed_form():
+    """Taylor-Green u = sin x cos y cos z, v = -cos x sin y cos z, w = 0 on [0, 2pi)^3: central
+    differences scale each derivative of a single mode by s = sin(h)/h, so the discrete
+    vorticity is s (-cos x sin y sin z, -sin x cos y sin z, 2 sin x sin y cos z) and its
+    enstrophy 0.5 * 0.75 s^2 (2pi)^3 (grid means of sin^2, cos^2 are 1/2)."""
+    import numpy as np
+    import torch
+
+    from jaxfluids_amd.adjoint import enstrophy
+
+    n = 12
+    h = 2 * np.pi / n
+    x = (np.arange(n) + 0.5) * h
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    W = np.stack([np.ones_like(X), np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z),
+                  np.zeros_like(X), np.ones_like(X)])
+    got = float(enstrophy(torch.as_tensor(W), (h, h, h)))
+    s = np.sin(h) / h
+    assert abs(got - 0.5 * 0.75 * s * s * (2 * np.pi) ** 3) <= 1e-12 * got
+    # 2-D shear u(y) = sin y: omega_z = -s cos y, enstrophy 0.5 * s^2 / 2 * area
+    W2 = np.zeros((5, n, n, 1))
+    W2[1] = np.sin(Y[:, :, :1])
+    got2 = float(enstrophy(torch.as_tensor(W2), (h, h, 1.0)))
+    assert abs(got2 - 0.5 * 0.5 * s * s * (2 * np.pi) ** 2) <= 1e-12 * got2

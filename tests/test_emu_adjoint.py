@@ -7,6 +7,7 @@ identity  <lam_out, J d> = <lam_in, d>  for any direction d; J d comes from cent
 of the oracle.  Every state load and store of the kernels is bounds-checked.
 """
 import ctypes
+import dataclasses
 import os
 import sys
 
@@ -103,10 +104,11 @@ def run_vjp(emu, dom, s, W_in, lam_out, stage, dt):
     from jaxfluids_amd import _native
 
     grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
+    transport = None if s.transport is None else _native.make_transport(**dataclasses.asdict(s.transport))
     scheme = _native.make_scheme("PRIMITIVE", s.stencil, s.riemann, s.signal_speed, "ARITHMETIC", _native.F64,
-                                 s.gamma, s.eps_stencil)
+                                 s.gamma, s.eps_stencil, transport)
     bcs = _native.bc_array(s.bcs)
-    nbytes = emu.jfa_stage_vjp_workspace_bytes(grid)
+    nbytes = emu.jfa_stage_vjp_workspace_bytes(grid, scheme)
     ws = np.zeros(nbytes // 8)
     W = np.ascontiguousarray(W_in)
     lam = np.ascontiguousarray(lam_out)
@@ -280,3 +282,55 @@ def test_rollout_gradient_emulated_vs_oracle_fd(emu):
         fd = (4.0 * D(0.5e-6) - D(1e-6)) / 3.0
         an = float(np.sum(g0 * d))
         assert abs(fd - an) <= 1e-7 * abs(an), (trial, fd, an)
+
+
+VISCOUS_CASES = [
+    ((7, 6, 8), ("WENO5-Z", "HLLC", "EINFELDT"),
+     o.Transport(R=1.0, mu_ref=0.02, T_ref=1.0, mu_exp=0.76, bulk_viscosity=0.01, prandtl=0.71)),
+    ((10, 9, 1), ("WENO5-JS", "HLLC", "EINFELDT"),
+     o.Transport(R=0.9, mu_ref=0.03, T_ref=0.8, mu_sutherland=0.3, kappa_ref=0.05, kappa_T_ref=1.1, kappa_exp=0.6)),
+    ((9, 1, 10), ("WENO5-Z", "RUSANOV", None),
+     o.Transport(R=1.2, mu_ref=0.02, prandtl=0.9, viscous=False)),
+    ((8, 9, 1), ("WENO5-Z", "HLLC", "DAVIS"),
+     o.Transport(R=1.0, mu_ref=0.025, kappa_ref=0.04, kappa_T_ref=0.9, kappa_sutherland=0.4)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(VISCOUS_CASES)))
+@pytest.mark.parametrize("stage", [0, 1])
+def test_viscous_stage_vjp_against_oracle_fd(emu, case, stage):
+    """Periodic stage with viscous and / or heat fluxes: the dot-product identity of its dissipative
+    part alone (VJP with the transport minus VJP without, against central differences of the
+    oracle's stage with minus without) to 1e-9, and of the whole stage."""
+    cells, (stencil, riemann, speed), tr = VISCOUS_CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    kw = dict(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann=riemann, signal_speed=speed,
+              bcs={f: "PERIODIC" for f in FACES})
+    sv, si = o.Setup(transport=tr, **kw), o.Setup(**kw)
+    rng = np.random.default_rng(3000 + 10 * case + stage)
+    W0 = smooth_state(dom, 80 + case)
+    Uint = o.cons_from_prim(W0, 1.4)
+    Un_int = Uint * (1.0 + 0.01 * rng.standard_normal(Uint.shape))
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, sv)[0], dom, sv)
+    fv, fi = stage_map(dom, sv, stage, dt, Un_int), stage_map(dom, si, stage, dt, Un_int)
+    _, W_in = fv(Uint)
+    lam_int = rng.standard_normal(Uint.shape)
+    lam_out = np.zeros((5,) + dom.shape)
+    lam_out[dom.interior] = lam_int
+    gv = run_vjp(emu, dom, sv, W_in, lam_out, stage, dt)[0][dom.interior]
+    gi = run_vjp(emu, dom, si, W_in, lam_out, stage, dt)[0][dom.interior]
+    for trial in range(2):
+        d = rng.standard_normal(Uint.shape) * np.abs(Uint).mean(axis=tuple(range(1, Uint.ndim)), keepdims=True)
+        # dissipative part: a smooth rational map, so a larger step leaves only rounding
+        fdv, fdi = richardson(fv, Uint, d, 1e-5), richardson(fi, Uint, d, 1e-5)
+        dl = float(np.sum(lam_int * (fdv - fdi)))
+        dr = float(np.sum((gv - gi) * d))
+        scale = np.sum(np.abs(lam_int * (fdv - fdi)))
+        assert scale > 1e-3 * np.sum(np.abs(lam_int * fdv))  # the dissipative part is not negligible
+        assert abs(dl - dr) <= 1e-9 * scale, (trial, dl, dr, scale)
+        # whole stage: the WENO part's differences need the step chosen per state (see richardson)
+        errs = []
+        for h in (1e-5, 1e-6, 1e-7):
+            fd = richardson(fv, Uint, d, h)
+            errs.append(abs(float(np.sum(lam_int * fd)) - float(np.sum(gv * d))) / np.sum(np.abs(lam_int * fd)))
+        assert min(errs) <= 1e-6, (trial, errs)
