@@ -209,8 +209,14 @@ size_t jfa_stage_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme)
  *   lam_n  += c1 lam_out                                                  (stages 1, 2)
  * on interior cells; halos of lam_* are neither read nor written.  lam_in, lam_n and lam_out
  * must be distinct.  workspace: >= jfa_stage_vjp_workspace_bytes(grid, scheme) bytes (the face-halo
- * cotangents, plus 48 doubles per cell with a transport). */
+ * cotangents, plus 9 doubles per interior cell with a transport). */
 size_t jfa_stage_vjp_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme);
+/* Tuning of jfa_rk3_stage_vjp (process-wide): an axis other than the innermost is swept by the
+ * marching transpose kernel (coalesced columns, chunks of 16 cells) when that launch has at least
+ * `waves` wavefronts, else by the lane-parallel one (one face per lane, strided loads).  Default
+ * 2048; 0 forces marching, a huge value lane-parallel sweeps.  No reference counterpart (the
+ * reference differentiates with jax.grad).  Returns JFA_EINVAL for waves < 0. */
+int jfa_adjoint_set_march_min_waves(int64_t waves);
 int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void* lam_n, const jfa_grid* grid,
                       const jfa_scheme* scheme, const int32_t bc[6], int32_t stage, double dt, void* workspace,
                       size_t workspace_bytes, void* stream);

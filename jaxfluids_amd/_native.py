@@ -89,6 +89,7 @@ SIGNATURES = {
     "jfa_rk3_stage_fused": (_I32, [_P, _P, _P, _P, _P, _PG, _PS, _PI32, _I32, _P, _D, _D, _D, _P, _P, _SZ, _P]),
     "jfa_stage_workspace_bytes": (_SZ, [_PG, _PS]),
     "jfa_stage_vjp_workspace_bytes": (_SZ, [_PG, _PS]),
+    "jfa_adjoint_set_march_min_waves": (_I32, [_I64]),
     "jfa_rk3_stage_vjp": (_I32, [_P, _P, _P, _P, _PG, _PS, _PI32, _I32, _D, _P, _SZ, _P]),
     "jfa_batch2d_workspace_bytes": (_SZ, [_P, _I32, _I32]),
     "jfa_batch2d_plan": (_I32, [_P, _I32, _I32, _P, _SZ, _P]),

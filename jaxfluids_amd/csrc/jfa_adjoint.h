@@ -14,11 +14,12 @@
 //   each thread walks the faces of a column and accumulates window cotangents in a 6-cell
 //   register window; each chunk owns a range of cells and evaluates every face that touches
 //   them (5 extra faces per chunk), so all writes are owned -- no atomics.
-// * adj_inner_kernel (innermost axis): one face per lane; a cell gathers the six window
-//   cotangents of the faces f = c-2..c+3 from lanes l..l-5 by wave shuffles; a wave owns 59
-//   cells (lanes 5..63).
-// * adj_dissip_face_kernel (with viscous / heat fluxes): the VJP of each cell's lower face on
-//   every axis into a face buffer, gathered per cell by adj_update_kernel.
+// * adj_lane_kernel (the innermost axis, and any axis whose marching sweep is too small to fill
+//   the device): one face per lane; a cell gathers the six window cotangents of the faces
+//   f = c-2..c+3 from lanes l..l-5 by wave shuffles; a wave owns 59 cells (lanes 5..63).
+// * adj_dissip_cell_kernel (with viscous / heat fluxes): per cell, the VJPs of its two faces on
+//   every axis into gW, and their transverse-derivative terms into X, a 9-field per-cell sum
+//   gathered from the periodic neighbours by adj_update_kernel.
 // * adj_update_kernel: halo^T, (dW/dU)^T and the stage blend per interior cell.
 // Face states are recomputed with the forward's own face_states, so branch choices (the sign
 // of S*, the min/max of the wave-speed estimates) are those of the forward evaluation.
@@ -495,109 +496,6 @@ __device__ __forceinline__ void dissipative_face_vjp(const double* __restrict__ 
     gr[0] -= TRb * s.TR / s.wr[0];
 }
 
-// Face buffer of the dissipative VJP: per active axis A and interior cell c, the VJP of the
-// face between c - e_A and c (periodic: face n is face 0) as kDissSlots arrays of ncell doubles:
-// gl[5], gr[5], Gt[2][3].
-constexpr int kDissSlots = 16;
-
-__device__ __forceinline__ int64_t interior_flat(const Geo& G, const int c[3]) {
-    int w[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        w[i] = c[i];
-        if (G.active[i]) w[i] = w[i] < 0 ? w[i] + G.n[i] : (w[i] >= G.n[i] ? w[i] - G.n[i] : w[i]);
-    }
-    return ((int64_t)w[0] * G.n[1] + w[1]) * G.n[2] + w[2];
-}
-
-template <int A>
-__device__ __forceinline__ void dissip_face_store(const double* __restrict__ W, const double* __restrict__ lam,
-                                                  const Geo& G, const Params& prm, double ts, const int c[3],
-                                                  int64_t idx, int64_t ncell, double* __restrict__ FB) {
-    // face cotangent ts (lam_R - lam_L) / dx with the left cell wrapped: the forward's faces 0
-    // and n evaluate the same wrapped cells, so their two cotangents add up to this one
-    int cl[3] = {c[0], c[1], c[2]};
-    cl[A] -= 1;
-    const int64_t br = wrapped_index(G, c), bl = wrapped_index(G, cl);
-    double m[5], gl[5], gr[5], Gt[2][3];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        m[i] = ts * ((ld(lam, i * G.fs + br) - ld(lam, i * G.fs + bl)) * G.one_dx[A]);
-        gl[i] = 0.0;
-        gr[i] = 0.0;
-    }
-    dissipative_face_vjp<A>(W, G, prm, c, m, gl, gr, Gt);
-    double* out = FB + (int64_t)A * kDissSlots * ncell + idx;
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        JFA_ACCESS(out + i * ncell, sizeof(double));
-        out[i * ncell] = gl[i];
-        JFA_ACCESS(out + (5 + i) * ncell, sizeof(double));
-        out[(5 + i) * ncell] = gr[i];
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            JFA_ACCESS(out + (10 + 3 * j + k) * ncell, sizeof(double));
-            out[(10 + 3 * j + k) * ncell] = Gt[j][k];
-        }
-}
-
-// One thread per interior cell: the VJPs of the cell's lower face on every active axis.
-template <int UNUSED = 0>
-__global__ __launch_bounds__(kBlock) void adj_dissip_face_kernel(const double* __restrict__ W,
-                                                                 const double* __restrict__ lam, Geo G, Params prm,
-                                                                 double ts, double* __restrict__ FB) {
-    int c[3];
-    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
-    if (!flat_index(ncell, G.n[1], G.n[2], c[0], c[1], c[2])) return;
-    const int64_t idx = ((int64_t)c[0] * G.n[1] + c[1]) * G.n[2] + c[2];
-    if (G.active[0]) dissip_face_store<0>(W, lam, G, prm, ts, c, idx, ncell, FB);
-    if (G.active[1]) dissip_face_store<1>(W, lam, G, prm, ts, c, idx, ncell, FB);
-    if (G.active[2]) dissip_face_store<2>(W, lam, G, prm, ts, c, idx, ncell, FB);
-}
-
-// Cotangent of cell c's primitives from the face buffer: gr of its lower faces, gl of its
-// upper faces (c + e_A, wrapped), and the transverse terms S(c - e_i) - S(c + e_i) with
-// S(x) = Gt(face x) + Gt(face x + e_A).
-__device__ __forceinline__ void dissip_gather(const double* __restrict__ FB, const Geo& G, const int c[3],
-                                              double g[5]) {
-    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
-#pragma unroll
-    for (int A = 0; A < 3; ++A) {
-        if (!G.active[A]) continue;
-        const double* fa = FB + (int64_t)A * kDissSlots * ncell;
-        int x[3] = {c[0], c[1], c[2]};
-        const int64_t i0 = interior_flat(G, x);
-        x[A] += 1;
-        const int64_t i1 = interior_flat(G, x);
-#pragma unroll
-        for (int i = 0; i < 5; ++i) g[i] += ld(fa, (5 + i) * ncell + i0) + ld(fa, i * ncell + i1);
-        int j = 0;
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            if (t == A) continue;
-            if (G.active[t]) {
-#pragma unroll
-                for (int sgn = -1; sgn <= 1; sgn += 2) {
-                    int y[3] = {c[0], c[1], c[2]};
-                    y[t] += sgn;
-                    const int64_t j0 = interior_flat(G, y);
-                    y[A] += 1;
-                    const int64_t j1 = interior_flat(G, y);
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const double S = ld(fa, (10 + 3 * j + k) * ncell + j0) + ld(fa, (10 + 3 * j + k) * ncell + j1);
-                        g[1 + k] += sgn < 0 ? S : -S;
-                    }
-                }
-            }
-            ++j;
-        }
-    }
-}
-
 // ------------------------------------------------------------------ kernels
 constexpr int kAdjBlock = 256;
 
@@ -666,48 +564,133 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
     }
 }
 
-// Transpose sweep along the innermost axis IN (unit stride).  Wave w of a row evaluates the
-// faces F0 .. F0+63, F0 = -5 + 59 w (lane l: face F0 + l, zero outside 0..N) and lanes 5..63
-// own the cells F0 + l - 3, i.e. [-3 + 59 w, -3 + 59 w + 59) clipped to the row's N + 6 cells.
-template <int IN, int STENCIL, int RIEMANN>
-__global__ __launch_bounds__(kAdjBlock) void adj_inner_kernel(const double* __restrict__ W,
-                                                              const double* __restrict__ lam, Geo G, Params prm,
-                                                              double ts, int waves_per_row, int64_t nwaves,
-                                                              double* __restrict__ gW) {
-    constexpr int P = IN == 0 ? 1 : 0;
-    constexpr int Q = IN == 2 ? 1 : 2;
+// Lane-parallel transpose sweep along axis A (stride s[A]; unit stride on the innermost axis).
+// Wave w of a row evaluates the faces F0 .. F0+63, F0 = -5 + 59 w (lane l: face F0 + l, zero
+// outside 0..N) and lanes 5..63 own the cells F0 + l - 3, i.e. [-3 + 59 w, -3 + 59 w + 59)
+// clipped to the row's N + 6 cells.  Rows run over the two other axes P, Q.
+template <int A, int STENCIL, int RIEMANN>
+__global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __restrict__ W,
+                                                             const double* __restrict__ lam, Geo G, Params prm,
+                                                             double ts, int waves_per_row, int64_t nwaves,
+                                                             double* __restrict__ gW) {
+    constexpr int P = A == 0 ? 1 : 0;
+    constexpr int Q = A == 2 ? 1 : 2;
     const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const bool vwave = gwave < nwaves;  // whole waves; shuffles below run in every lane of a wave
     const int64_t row = vwave ? gwave / waves_per_row : 0;
     const int w = vwave ? (int)(gwave - row * waves_per_row) : 0;
-    const int N = G.n[IN];
+    const int N = G.n[A];
     const int cP = (int)(row / G.n[Q]);
     const int cQ = (int)(row - (int64_t)cP * G.n[Q]);
-    const int64_t fs = G.fs;
-    const int64_t brow = (int64_t)(cP + G.off[P]) * G.s[P] + (int64_t)(cQ + G.off[Q]) * G.s[Q] + G.off[IN];
+    const int64_t fs = G.fs, sa = G.s[A];
+    const int64_t brow = (int64_t)(cP + G.off[P]) * G.s[P] + (int64_t)(cQ + G.off[Q]) * G.s[Q] +
+                         (int64_t)G.off[A] * sa;  // cell 0 of the row along A
     const int f = -5 + 59 * w + lane;
     const bool vface = vwave && f >= 0 && f <= N;
-    const int64_t br = brow + (vface ? f : 0);
+    const int64_t br = brow + (vface ? f : 0) * sa;
     double gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (vface) {
         double mu[5];
-        face_mu(lam, br, 1, fs, f, N, ts * G.one_dx[IN], mu);
-        face_state_vjp_at<IN, STENCIL, RIEMANN>(W, br, 1, fs, mu, prm, gl, gr);
+        face_mu(lam, br, sa, fs, f, N, ts * G.one_dx[A], mu);
+        face_state_vjp_at<A, STENCIL, RIEMANN>(W, br, sa, fs, mu, prm, gl, gr);
     }
     // cell c = f - 3 gathers window slot k of face c + 3 - k, held by lane - k
     double s[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         double x6[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0}, o6[6];  // lanes without a face: zero cotangents
-        if (vface) load_var6(W, br, 1, fs, i, x6);
+        if (vface) load_var6(W, br, sa, fs, i, x6);
         window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
         s[i] = 0.0;
 #pragma unroll
         for (int k = 0; k < 6; ++k) s[i] += __shfl_up(o6[k], (unsigned)k, 64);
     }
     const int cell = f - 3;
-    if (vwave && lane >= 5 && cell <= N + 2) add_cell(gW, fs, brow + cell, s);
+    if (vwave && lane >= 5 && cell <= N + 2) add_cell(gW, fs, brow + (int64_t)cell * sa, s);
+}
+
+__device__ __forceinline__ int64_t interior_flat(const Geo& G, const int c[3]) {
+    int w[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        w[i] = c[i];
+        if (G.active[i]) w[i] = w[i] < 0 ? w[i] + G.n[i] : (w[i] >= G.n[i] ? w[i] - G.n[i] : w[i]);
+    }
+    return ((int64_t)w[0] * G.n[1] + w[1]) * G.n[2] + w[2];
+}
+
+// Dissipative part, one thread per interior cell c: the VJPs of both faces of c along every
+// active axis (periodic: the forward's faces 0 and n evaluate the same wrapped cells, so their
+// cotangents combine into ts (lam_R - lam_L) / dx with wrapped lam), so c owns its whole
+// dissipative cotangent, added to gW, and the sum X[3 i + k](c) of the transverse-derivative
+// cotangents Gt of its faces (interior layout, ncell per field), gathered by adj_update_kernel.
+template <int UNUSED = 0>
+__global__ __launch_bounds__(kBlock) void adj_dissip_cell_kernel(const double* __restrict__ W,
+                                                                 const double* __restrict__ lam, Geo G, Params prm,
+                                                                 double ts, double* __restrict__ gW,
+                                                                 double* __restrict__ X) {
+    int c[3];
+    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    if (!flat_index(ncell, G.n[1], G.n[2], c[0], c[1], c[2])) return;
+    const int64_t idx = ((int64_t)c[0] * G.n[1] + c[1]) * G.n[2] + c[2];
+    const int64_t b = wrapped_index(G, c), fs = G.fs;
+    double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, x[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double lc[5] = {ld(lam, b), ld(lam, fs + b), ld(lam, 2 * fs + b), ld(lam, 3 * fs + b), ld(lam, 4 * fs + b)};
+#pragma unroll
+    for (int A = 0; A < 3; ++A) {
+        if (!G.active[A]) continue;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {  // 0: lower face (c is its right cell), 1: upper face
+            int cr[3] = {c[0], c[1], c[2]}, co[3] = {c[0], c[1], c[2]};
+            cr[A] += side;
+            co[A] += side ? 1 : -1;  // the face's other cell
+            const int64_t bo = wrapped_index(G, co);
+            double mu[5], gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gr[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Gt[2][3];
+            const double sc = ts * G.one_dx[A];
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                const double lo = ld(lam, i * fs + bo);
+                mu[i] = side ? sc * (lo - lc[i]) : sc * (lc[i] - lo);
+            }
+            if (A == 0) dissipative_face_vjp<0>(W, G, prm, cr, mu, gl, gr, Gt);
+            else if (A == 1) dissipative_face_vjp<1>(W, G, prm, cr, mu, gl, gr, Gt);
+            else dissipative_face_vjp<2>(W, G, prm, cr, mu, gl, gr, Gt);
+#pragma unroll
+            for (int i = 0; i < 5; ++i) g[i] += side ? gl[i] : gr[i];
+            int j = 0;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                if (t == A) continue;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) x[3 * t + k] += Gt[j][k];
+                ++j;
+            }
+        }
+    }
+    add_cell(gW, fs, b, g);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        JFA_ACCESS(X + q * ncell + idx, sizeof(double));
+        X[q * ncell + idx] = x[q];
+    }
+}
+
+// Transverse-derivative cotangents of cell c's velocity: a face adds +Gt to u at L + e_i, R + e_i
+// and -Gt at L - e_i, R - e_i, so u_k(c) gathers X[3 i + k](c - e_i) - X[3 i + k](c + e_i)
+// (periodic neighbours; X in interior layout).
+__device__ __forceinline__ void xbar_gather(const double* __restrict__ X, const Geo& G, const int c[3], double g[5]) {
+    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (!G.active[i]) continue;
+        int ym[3] = {c[0], c[1], c[2]}, yp[3] = {c[0], c[1], c[2]};
+        ym[i] -= 1;
+        yp[i] += 1;
+        const int64_t jm = interior_flat(G, ym), jp = interior_flat(G, yp);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g[1 + k] += ld(X, (3 * i + k) * ncell + jm) - ld(X, (3 * i + k) * ncell + jp);
+    }
 }
 
 // Per interior cell: halo^T, (dW/dU)^T and the stage blend.
@@ -758,7 +741,7 @@ __global__ __launch_bounds__(kBlock) void adj_update_kernel(const double* __rest
                                                             const double* __restrict__ gW, Geo G, Params prm,
                                                             BC6 bc, double c0, double c1, double* __restrict__ lam_in,
                                                             double* __restrict__ lam_n,
-                                                            const double* __restrict__ FB) {
+                                                            const double* __restrict__ X) {
     int c[3];
     if (!flat_index((int64_t)G.n[0] * G.n[1] * G.n[2], G.n[1], G.n[2], c[0], c[1], c[2])) return;
     const int64_t b = (int64_t)(c[0] + G.off[0]) * G.s[0] + (int64_t)(c[1] + G.off[1]) * G.s[1] + (c[2] + G.off[2]);
@@ -771,7 +754,7 @@ __global__ __launch_bounds__(kBlock) void adj_update_kernel(const double* __rest
         l[i] = ld(lam, i * fs + b);
     }
     halo_gather(gW, G, bc, c, b, g);
-    if (FB) dissip_gather(FB, G, c, g);  // dissipative fluxes read interior cells only (wrapped)
+    if (X) xbar_gather(X, G, c, g);
     prim_vjp(w, g, prm, gu);
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
@@ -807,31 +790,50 @@ static void launch_adj_march(const double* W, const double* lam, const Geo& G, c
                        gW);
 }
 
-template <int IN, int STENCIL, int RIEMANN>
-static void launch_adj_inner(const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
+template <int A, int STENCIL, int RIEMANN>
+static void launch_adj_lanes(const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
                              double* gW, hipStream_t st) {
-    constexpr int P = IN == 0 ? 1 : 0;
-    constexpr int Q = IN == 2 ? 1 : 2;
-    const int waves_per_row = (G.n[IN] + 6 + 58) / 59;
+    constexpr int P = A == 0 ? 1 : 0;
+    constexpr int Q = A == 2 ? 1 : 2;
+    const int waves_per_row = (G.n[A] + 6 + 58) / 59;
     const int64_t nwaves = (int64_t)waves_per_row * G.n[P] * G.n[Q];
     const int64_t blocks = (nwaves + kAdjBlock / 64 - 1) / (kAdjBlock / 64);
-    hipLaunchKernelGGL((adj_inner_kernel<IN, STENCIL, RIEMANN>), dim3((unsigned)blocks), dim3(kAdjBlock), 0, st, W,
+    hipLaunchKernelGGL((adj_lane_kernel<A, STENCIL, RIEMANN>), dim3((unsigned)blocks), dim3(kAdjBlock), 0, st, W,
                        lam, G, prm, ts, waves_per_row, nwaves, gW);
 }
 
+// Waves of the marching sweep along A (columns along IN, chunks of 16 cells along A).
+static inline int64_t march_waves(const Geo& G, int A, int IN) {
+    const int O = 3 - A - IN;
+    return (int64_t)((G.n[IN] + 63) / 64) * G.n[O] * ((G.n[A] + 6 + 15) / 16);
+}
+
+// Below this many waves the marching sweep cannot fill the device and the lane-parallel sweep
+// (strided loads along A, every face its own lane) is used for that axis instead.
+int64_t adjoint_march_min_waves();
+
+// Transpose sweeps of the convective flux along every active axis into gW.
 template <int STENCIL, int RIEMANN>
 static void adj_sweeps(const double* W, const double* lam, const Geo& G, const Params& prm, double ts, double* gW,
                        hipStream_t st) {
     const int IN = innermost_axis(G);
+    const int64_t min_waves = adjoint_march_min_waves();
     for (int A = 0; A < IN; ++A) {
         if (!G.active[A]) continue;
-        if (A == 0 && IN == 1) launch_adj_march<0, 1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
-        else if (A == 0) launch_adj_march<0, 2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
-        else launch_adj_march<1, 2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+        if (march_waves(G, A, IN) < min_waves) {
+            if (A == 0) launch_adj_lanes<0, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+            else launch_adj_lanes<1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+        } else if (A == 0 && IN == 1) {
+            launch_adj_march<0, 1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+        } else if (A == 0) {
+            launch_adj_march<0, 2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+        } else {
+            launch_adj_march<1, 2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+        }
     }
-    if (IN == 0) launch_adj_inner<0, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
-    else if (IN == 1) launch_adj_inner<1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
-    else launch_adj_inner<2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+    if (IN == 0) launch_adj_lanes<0, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+    else if (IN == 1) launch_adj_lanes<1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+    else launch_adj_lanes<2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
 }
 
 }  // namespace jfa

@@ -1,7 +1,13 @@
 // jfa_adjoint.hip -- C ABI of the reverse-mode RK3 stage (FP64).
+#include <atomic>
+
 #include "jfa_adjoint.h"
 
 namespace jfa {
+
+static std::atomic<int64_t> g_march_min_waves{2048};
+
+int64_t adjoint_march_min_waves() { return g_march_min_waves.load(std::memory_order_relaxed); }
 
 template <int STENCIL>
 static void adj_sweeps_r(int riemann, const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
@@ -16,10 +22,16 @@ using namespace jfa;
 
 extern "C" {
 
+int jfa_adjoint_set_march_min_waves(int64_t waves) {
+    if (waves < 0) return JFA_EINVAL;
+    g_march_min_waves.store(waves, std::memory_order_relaxed);
+    return JFA_OK;
+}
+
 static size_t vjp_workspace_bytes(const Geo& G, const jfa_scheme* scheme) {
     size_t n = (size_t)(5 * G.fs);
     if (scheme && scheme->transport && (scheme->transport->viscous || scheme->transport->heat))
-        n += (size_t)3 * kDissSlots * ((size_t)G.n[0] * G.n[1] * G.n[2]);
+        n += (size_t)9 * ((size_t)G.n[0] * G.n[1] * G.n[2]);  // X: transverse-derivative cotangent sums
     return n * sizeof(double);
 }
 
@@ -67,15 +79,15 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     const double* lam = static_cast<const double*>(lam_out);
     if (scheme->stencil == JFA_WENO5_JS) adj_sweeps_r<JFA_WENO5_JS>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
     else adj_sweeps_r<JFA_WENO5_Z>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
-    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
-    double* FB = nullptr;
+    double* X = nullptr;
     if (prm.dissip) {
-        FB = gW + nz;  // every slot is written before it is read: no clearing
-        hipLaunchKernelGGL(adj_dissip_face_kernel<0>, flat_grid(ncell), dim3(kBlock), 0, st, W, lam, G, prm, m * dt,
-                           FB);
+        X = gW + nz;  // written whole by the cell kernel: no clearing
+        hipLaunchKernelGGL(adj_dissip_cell_kernel<0>, flat_grid((int64_t)G.n[0] * G.n[1] * G.n[2]), dim3(kBlock), 0,
+                           st, W, lam, G, prm, m * dt, gW, X);
     }
+    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
     hipLaunchKernelGGL(adj_update_kernel, flat_grid(ncell), dim3(kBlock), 0, st, W, lam, gW, G, prm, b6, c0, c1,
-                       static_cast<double*>(lam_in), stage > 0 ? static_cast<double*>(lam_n) : nullptr, FB);
+                       static_cast<double*>(lam_in), stage > 0 ? static_cast<double*>(lam_n) : nullptr, X);
     return check_launch();
 }
 

@@ -22,21 +22,33 @@ def stale() -> bool:
 
 
 def build(force: bool = False) -> str:
+    """Builds when stale.  Concurrent callers (pytest-xdist workers) serialise on a lock file and
+    the library appears by an atomic rename, so no caller loads a half-written file."""
+    import fcntl
+    import tempfile
+
     if not force and not stale():
         return OUT
     cxx = shutil.which("g++")
     if cxx is None:
         raise RuntimeError("g++ not found")
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    flags = ["-std=c++17", "-O2", "-g", "-fPIC", "-pthread", "-Wno-unknown-pragmas", "-Wno-unused-result",
-             f"-I{os.path.join(HERE)}", f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}"]
-    objs = []
-    for src in SOURCES + [os.path.join(HERE, "emu_runtime.cpp")]:
-        obj = os.path.join(os.path.dirname(OUT), os.path.basename(src) + ".o")
-        lang = ["-x", "c++"] if src.endswith(".hip") else []
-        subprocess.run([cxx, *flags, *lang, "-c", src, "-o", obj], check=True)
-        objs.append(obj)
-    subprocess.run([cxx, "-shared", "-pthread", "-o", OUT, *objs], check=True)
+    with open(OUT + ".lock", "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        if not force and not stale():  # built by another process meanwhile
+            return OUT
+        flags = ["-std=c++17", "-O2", "-g", "-fPIC", "-pthread", "-Wno-unknown-pragmas", "-Wno-unused-result",
+                 f"-I{os.path.join(HERE)}", f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}"]
+        with tempfile.TemporaryDirectory(dir=os.path.dirname(OUT)) as tmp:
+            objs = []
+            for src in SOURCES + [os.path.join(HERE, "emu_runtime.cpp")]:
+                obj = os.path.join(tmp, os.path.basename(src) + ".o")
+                lang = ["-x", "c++"] if src.endswith(".hip") else []
+                subprocess.run([cxx, *flags, *lang, "-c", src, "-o", obj], check=True)
+                objs.append(obj)
+            lib = os.path.join(tmp, "lib.so")
+            subprocess.run([cxx, "-shared", "-pthread", "-o", lib, *objs], check=True)
+            os.replace(lib, OUT)
     return OUT
 
 

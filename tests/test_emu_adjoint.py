@@ -27,7 +27,7 @@ def emu():
     from jaxfluids_amd import _native
 
     lib = ctypes.CDLL(emu_build.build())
-    for name in ("jfa_stage_vjp_workspace_bytes", "jfa_rk3_stage_vjp"):
+    for name in ("jfa_stage_vjp_workspace_bytes", "jfa_rk3_stage_vjp", "jfa_adjoint_set_march_min_waves"):
         fn = getattr(lib, name)
         fn.restype, fn.argtypes = _native.SIGNATURES[name]
     lib.emu_add_range.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
@@ -100,8 +100,14 @@ def richardson(f, U, d, h):
     return (4.0 * D(0.5 * h) - D(h)) / 3.0
 
 
-def run_vjp(emu, dom, s, W_in, lam_out, stage, dt):
+# marching transpose sweeps (threshold 0) or lane-parallel ones (threshold above any grid here)
+SWEEPS = {"march": 0, "lanes": 1 << 40}
+
+
+def run_vjp(emu, dom, s, W_in, lam_out, stage, dt, sweep="march"):
     from jaxfluids_amd import _native
+
+    assert emu.jfa_adjoint_set_march_min_waves(SWEEPS[sweep]) == 0
 
     grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
     transport = None if s.transport is None else _native.make_transport(**dataclasses.asdict(s.transport))
@@ -126,9 +132,10 @@ def run_vjp(emu, dom, s, W_in, lam_out, stage, dt):
     return lam_in, lam_n
 
 
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
 @pytest.mark.parametrize("case", range(len(CASES)))
 @pytest.mark.parametrize("stage", [0, 2])
-def test_stage_vjp_dot_product_against_oracle_fd(emu, case, stage):
+def test_stage_vjp_dot_product_against_oracle_fd(emu, case, stage, sweep):
     cells, bcs, (stencil, riemann, speed) = CASES[case]
     ranges = ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9))
     dom = o.Domain(cells, ranges, nh=3)
@@ -144,7 +151,7 @@ def test_stage_vjp_dot_product_against_oracle_fd(emu, case, stage):
     lam_int = rng.standard_normal(Uint.shape)
     lam_out = np.zeros((5,) + dom.shape)
     lam_out[dom.interior] = lam_int
-    lam_in, lam_n = run_vjp(emu, dom, s, W_in, lam_out, stage, dt)
+    lam_in, lam_n = run_vjp(emu, dom, s, W_in, lam_out, stage, dt, sweep)
     halo = np.ones(dom.shape, bool)
     halo[dom.interior[1:]] = False
     assert not np.any(lam_in[:, halo]) and not np.any(lam_n[:, halo])  # halos untouched
@@ -189,7 +196,7 @@ def test_stage_vjp_every_entry_against_oracle_fd(emu, case):
     lam_int = rng.standard_normal(Uint.shape)
     lam_out = np.zeros((5,) + dom.shape)
     lam_out[dom.interior] = lam_int
-    g = run_vjp(emu, dom, s, W_in, lam_out, 1, dt)[0][dom.interior]
+    g = run_vjp(emu, dom, s, W_in, lam_out, 1, dt, "lanes")[0][dom.interior]
     # per entry, the closest of four Richardson estimates: a kink of the scheme (S* = 0, a
     # min/max tie) within a step of the state spoils that step only; a wrong derivative
     # matches at none
@@ -299,6 +306,7 @@ VISCOUS_CASES = [
 @pytest.mark.parametrize("case", range(len(VISCOUS_CASES)))
 @pytest.mark.parametrize("stage", [0, 1])
 def test_viscous_stage_vjp_against_oracle_fd(emu, case, stage):
+    sweep = ("march", "lanes")[(case + stage) % 2]
     """Periodic stage with viscous and / or heat fluxes: the dot-product identity of its dissipative
     part alone (VJP with the transport minus VJP without, against central differences of the
     oracle's stage with minus without) to 1e-9, and of the whole stage."""
@@ -317,8 +325,8 @@ def test_viscous_stage_vjp_against_oracle_fd(emu, case, stage):
     lam_int = rng.standard_normal(Uint.shape)
     lam_out = np.zeros((5,) + dom.shape)
     lam_out[dom.interior] = lam_int
-    gv = run_vjp(emu, dom, sv, W_in, lam_out, stage, dt)[0][dom.interior]
-    gi = run_vjp(emu, dom, si, W_in, lam_out, stage, dt)[0][dom.interior]
+    gv = run_vjp(emu, dom, sv, W_in, lam_out, stage, dt, sweep)[0][dom.interior]
+    gi = run_vjp(emu, dom, si, W_in, lam_out, stage, dt, sweep)[0][dom.interior]
     for trial in range(2):
         d = rng.standard_normal(Uint.shape) * np.abs(Uint).mean(axis=tuple(range(1, Uint.ndim)), keepdims=True)
         # dissipative part: a smooth rational map, so a larger step leaves only rounding
