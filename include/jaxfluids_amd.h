@@ -108,6 +108,16 @@ typedef struct jfa_scheme {
     double gamma;          /* ideal-gas specific heat ratio (materials/single_materials/ideal_gas.py:29) */
     double eps_stencil;    /* WENO epsilon, 1e-30 (config/precision.py:39,53) */
     const jfa_transport* transport;  /* dissipative fluxes of the fused stage / RHS; NULL: inviscid */
+    /* Positivity limiter of the reconstructed face states (positivity.is_interpolation_limiter,
+     * solvers/positivity/limiter_interpolation.py:86-208, single phase): a face state with density
+     * below limiter_eps_density takes the upwind cell's density and pressure (all five primitives
+     * with limit_velocity), then one with pressure below limiter_eps_pressure likewise; the face's
+     * conservatives are recomputed from the limited primitives.  Epsilons: 1e-12 / 1e-10 in double
+     * precision, 1e-6 / 1e-6 in single (config/precision.py:40,54). */
+    int32_t interpolation_limiter;
+    int32_t limit_velocity;
+    double limiter_eps_density;
+    double limiter_eps_pressure;
 } jfa_scheme;
 
 /* Library version string; never NULL. */
@@ -133,11 +143,13 @@ int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void
                    int32_t axis, const jfa_scheme* scheme, void* flux, void* stream);
 
 /* Numerical flux along `axis` on every face: reconstruction (4 variable modes), prim<->cons
- * conversion and Riemann solve.  flux: (5, face shape).
+ * conversion, the positivity limiter when the scheme enables it, and Riemann solve.
+ * flux: (5, face shape).  limiter_count: NULL or one device int64 to which the number of limiter
+ * replacements is added (the count_interpolation_limiter the reference returns).
  * Replaces HighOrderGodunov.compute_flux_xi (solvers/convective_fluxes/high_order_godunov.py:117-231)
  * with reconstruct_xi (:233-419). */
 int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme,
-                        int32_t axis, void* flux, void* stream);
+                        int32_t axis, void* flux, int64_t* limiter_count, void* stream);
 
 /* Right-hand side 0.0 + rhs_x + rhs_y + rhs_z, rhs_xi = (1/dx)(F[:-1]-F[1:]).
  * rhs: (5, Nx, Ny, Nz).  workspace: >= jfa_rhs_workspace_bytes(grid, dtype) bytes (FP64 partial RHS).

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from typing import Optional, Sequence
+from typing import Tuple, Optional, Sequence
 
 import torch
 
@@ -50,6 +50,10 @@ class JfaScheme(ctypes.Structure):
         ("gamma", ctypes.c_double),
         ("eps_stencil", ctypes.c_double),
         ("transport", ctypes.POINTER(JfaTransport)),
+        ("interpolation_limiter", ctypes.c_int32),
+        ("limit_velocity", ctypes.c_int32),
+        ("limiter_eps_density", ctypes.c_double),
+        ("limiter_eps_pressure", ctypes.c_double),
     ]
 
 
@@ -78,7 +82,7 @@ SIGNATURES = {
     "jfa_last_error": (ctypes.c_char_p, []),
     "jfa_weno5_reconstruct_xi": (_I32, [_P, _I32, _PG, _I32, _I32, _I32, _D, _I32, _P, _P]),
     "jfa_riemann_xi": (_I32, [_P, _P, _P, _P, _I64, _I32, _PS, _P, _P]),
-    "jfa_compute_flux_xi": (_I32, [_P, _P, _PG, _PS, _I32, _P, _P]),
+    "jfa_compute_flux_xi": (_I32, [_P, _P, _PG, _PS, _I32, _P, _P, _P]),
     "jfa_compute_rhs": (_I32, [_P, _P, _PG, _PS, _P, _P, _SZ, _P]),
     "jfa_rhs_workspace_bytes": (_SZ, [_PG, _I32]),
     "jfa_rk3_stage": (_I32, [_P, _P, _P, _P, _PG, _PS, _I32, _P, _D, _P]),
@@ -188,7 +192,11 @@ def make_transport(R: float = 1.0, mu_ref: float = 0.0, T_ref: float = 1.0, mu_e
 
 
 def make_scheme(recon_var: str, stencil: str, riemann: str, signal_speed: Optional[str], frozen_state: str,
-                dtype: int, gamma: float, eps_stencil: float, transport: Optional[JfaTransport] = None) -> JfaScheme:
+                dtype: int, gamma: float, eps_stencil: float, transport: Optional[JfaTransport] = None,
+                interpolation_limiter: bool = False, limit_velocity: bool = False,
+                limiter_eps: Tuple[float, float] = (1e-12, 1e-10)) -> JfaScheme:
+    """Native scheme; ``limiter_eps`` = (density, pressure) epsilons of the positivity limiter of
+    the reconstructed face states (used when ``interpolation_limiter``)."""
     s = JfaScheme()
     try:
         s.recon_var = RECON_VAR[recon_var]
@@ -204,6 +212,9 @@ def make_scheme(recon_var: str, stencil: str, riemann: str, signal_speed: Option
     if transport is not None:
         s._transport_ref = transport  # keeps the struct alive as long as the scheme
         s.transport = ctypes.pointer(transport)
+    s.interpolation_limiter = 1 if interpolation_limiter else 0
+    s.limit_velocity = 1 if limit_velocity else 0
+    s.limiter_eps_density, s.limiter_eps_pressure = (float(v) for v in limiter_eps)
     return s
 
 

@@ -20,18 +20,28 @@ class PrecisionConfig:
         self.is_double = True
         self._eps = float(np.finfo(np.float64).eps)
         self._spatial_stencil_eps = 1e-30
+        self._interpolation_limiter_eps = (1e-12, 1e-10)  # (density, pressure), precision.py:54
 
     def enable_single_precision(self) -> None:
         # precision.py:30-42
         self.is_double = False
         self._eps = float(np.finfo(np.float32).eps)
         self._spatial_stencil_eps = 1e-30
+        self._interpolation_limiter_eps = (1e-6, 1e-6)  # precision.py:40
 
     def get_eps(self) -> float:
         return self._eps
 
     def get_spatial_stencil_eps(self) -> float:
         return self._spatial_stencil_eps
+
+    def get_interpolation_limiter_eps(self):
+        """(density, pressure) thresholds of the positivity limiter of the face states."""
+        return self._interpolation_limiter_eps
+
+    def set_interpolation_limiter_eps(self, density: float, pressure: float) -> None:
+        assert density > 0.0 and pressure > 0.0, "limiter epsilons must be positive"
+        self._interpolation_limiter_eps = (float(density), float(pressure))
 
     def set_spatial_stencil_eps(self, eps: float) -> None:
         assert isinstance(eps, float) and eps > 0.0, "spatial stencil eps must be a positive float"

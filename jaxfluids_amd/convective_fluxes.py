@@ -42,8 +42,9 @@ class HighOrderGodunov(ConvectiveFluxSolver):
         super().__init__(convective_fluxes_setup, material_manager, domain_information, equation_manager)
         godunov = convective_fluxes_setup.godunov
         self.reconstruction_variable = godunov.reconstruction_variable
-        if positivity_setup is not None and getattr(positivity_setup, "is_interpolation_limiter", False):
-            raise NotImplementedError("positivity interpolation limiter is not on this path")
+        # positivity limiter of the face states (high_order_godunov.py:67,163-176)
+        self.is_interpolation_limiter = bool(getattr(positivity_setup, "is_interpolation_limiter", False))
+        self.limit_velocity = bool(getattr(positivity_setup, "limit_velocity", False))
         self.riemann_solver = godunov.riemann_solver(
             material_manager=material_manager, equation_manager=equation_manager,
             signal_speed=godunov.signal_speed, catum_setup=godunov.catum_setup)
@@ -68,16 +69,18 @@ class HighOrderGodunov(ConvectiveFluxSolver):
         if key not in self._schemes:
             self._schemes[key] = _native.make_scheme(
                 *self._names, dtype_code, self.material_manager.get_gamma(), precision.get_spatial_stencil_eps(),
-                transport)
+                transport, self.is_interpolation_limiter, self.limit_velocity,
+                precision.get_interpolation_limiter_eps())
         return self._schemes[key]
 
     def compute_flux_xi(self, primitives: torch.Tensor, conservatives: torch.Tensor, axis: int,
                         curvature=None, volume_fraction=None, apertures=None, ml_setup=None
-                        ) -> Tuple[torch.Tensor, None, None, None, None]:
+                        ) -> Tuple[torch.Tensor, None, None, Optional[torch.Tensor], None]:
         """Numerical flux on every face along axis (high_order_godunov.py:117-231).
 
         Returns (fluxes_xi, u_hat, alpha_hat, count_interpolation_limiter,
-        count_interpolation_limiter_thinc) with the single-phase Nones.
+        count_interpolation_limiter_thinc) with the single-phase Nones; the limiter count is a
+        device int64 scalar when the interpolation limiter is enabled, else None.
         """
         _native.require_device(primitives, "primitives")
         _native.require_device(conservatives, "conservatives")
@@ -89,11 +92,14 @@ class HighOrderGodunov(ConvectiveFluxSolver):
         face = list(dom.global_number_of_cells)
         face[axis] += 1
         flux = torch.empty((5, *face), dtype=primitives.dtype, device=primitives.device)
+        count = torch.zeros((), dtype=torch.int64, device=primitives.device) if self.is_interpolation_limiter \
+            else None
         rc = _native.load().jfa_compute_flux_xi(
             _native.ptr(primitives), _native.ptr(conservatives), dom.grid,
-            self.scheme(_native.dtype_code(primitives)), int(axis), _native.ptr(flux), _native.stream_ptr())
+            self.scheme(_native.dtype_code(primitives)), int(axis), _native.ptr(flux), _native.ptr(count),
+            _native.stream_ptr())
         _native.check(rc, "jfa_compute_flux_xi")
-        return flux, None, None, None, None
+        return flux, None, None, count, None
 
 
 # solvers/convective_fluxes/__init__.py:7-14 (the solver built on this path)

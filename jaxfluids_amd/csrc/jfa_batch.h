@@ -160,6 +160,15 @@ __device__ __forceinline__ void face_flux(const T* __restrict__ W, const T* __re
     }
     double WL[5], WR[5], UL[5], UR[5];
     face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
+    if (prm.limiter) {
+        double c[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) c[i] = ldo(W + (rb + i * fs - st), voff);
+        limit_side<VAR>(c, prm, WL, UL);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) c[i] = ldo(W + (rb + i * fs), voff);
+        limit_side<VAR>(c, prm, WR, UR);
+    }
     riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, F);
 }
 
@@ -259,7 +268,11 @@ __device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& pr
 }
 
 template <class T, int VAR, int STENCIL, int RIEMANN>
-__global__ __launch_bounds__(kBatchBlock) void batch2d_stage_kernel(BatchArgs<T> a, Params prm) {
+// Three waves per SIMD (at most 168 VGPRs) for the variants that fit without spilling; the
+// dense 5x5 projections of CHAR-CONSERVATIVE need more registers and keep two.
+__global__ __launch_bounds__(kBatchBlock)
+JFA_WAVES_PER_EU(VAR == JFA_CHAR_CONSERVATIVE ? 1 : 3)
+void batch2d_stage_kernel(BatchArgs<T> a, Params prm) {
     const unsigned idx = blockIdx.x * (kBatchBlock / 64) + (threadIdx.x >> 6);  // one item per wavefront
     if (idx >= (unsigned)a.nitems) return;  // whole wavefronts only; no barriers follow
     const Item2D it = load_item(a.items + idx);

@@ -64,13 +64,17 @@ static inline int check_launch() {
 
 // Hooks for a host build of these sources (the test suite's CPU emulation of the kernels):
 // JFA_ACCESS(p, bytes) sees every state load and store, JFA_OPAQUE(x) hides a value from the
-// optimiser.  In the library JFA_ACCESS is empty and JFA_OPAQUE is an empty asm statement
-// that keeps per-tap address arithmetic from being hoisted out of march loops.
+// optimiser, JFA_WAVES_PER_EU(n) asks the compiler to fit n waves per SIMD.  In the library
+// JFA_ACCESS is empty, JFA_OPAQUE is an empty asm statement that keeps per-tap address
+// arithmetic from being hoisted out of march loops, and JFA_WAVES_PER_EU the amdgpu attribute.
 #ifndef JFA_ACCESS
 #define JFA_ACCESS(p, bytes)
 #endif
 #ifndef JFA_OPAQUE
 #define JFA_OPAQUE(x) asm volatile("" : "+v"(x))
+#endif
+#ifndef JFA_WAVES_PER_EU
+#define JFA_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
 
 template <class T>
@@ -222,6 +226,10 @@ static inline int check_scheme(const jfa_scheme* s) {
     if (s->signal_speed < 0 || s->signal_speed > 4) return JFA_EUNSUPPORTED;
     if (s->frozen_state != JFA_FROZEN_ARITHMETIC && s->frozen_state != JFA_FROZEN_ROE) return JFA_EUNSUPPORTED;
     if (s->dtype != JFA_F64 && s->dtype != JFA_F32) return JFA_EUNSUPPORTED;
+    if (s->interpolation_limiter &&
+        (!(s->limiter_eps_density > 0.0) || !(s->limiter_eps_density < INFINITY) ||
+         !(s->limiter_eps_pressure > 0.0) || !(s->limiter_eps_pressure < INFINITY)))
+        return JFA_EINVAL;
     if (!(s->gamma > 1.0)) return JFA_EINVAL;
     return JFA_OK;
 }
@@ -264,6 +272,10 @@ static inline Params make_params(const jfa_scheme* s) {
     p.signal_speed = s->signal_speed;
     p.frozen = s->frozen_state;
     set_transport(p, s->transport, s->gamma);
+    p.limiter = s->interpolation_limiter ? 1 : 0;
+    p.limit_velocity = s->limit_velocity ? 1 : 0;
+    p.lim_eps_rho = s->limiter_eps_density;
+    p.lim_eps_p = s->limiter_eps_pressure;
     return p;
 }
 
@@ -306,6 +318,8 @@ static inline Params gas_params(double gamma) {
     p.signal_speed = JFA_SS_EINFELDT;
     p.frozen = JFA_FROZEN_ARITHMETIC;
     set_transport(p, nullptr, gamma);
+    p.limiter = p.limit_velocity = 0;
+    p.lim_eps_rho = p.lim_eps_p = 0.0;
     return p;
 }
 

@@ -57,6 +57,11 @@ struct Params {
     double bulk;       // bulk viscosity
     double kappa_fac;  // cp / Pr (JFA_LAW_PRANDTL)
     double inv_cp;     // 1 / cp
+    // positivity limiter of the reconstructed face states (jfa_scheme.interpolation_limiter)
+    int limiter;
+    int limit_velocity;  // replace all five primitives, not only density and pressure
+    double lim_eps_rho;
+    double lim_eps_p;
 };
 
 // ---------------------------------------------------------------- refined hardware estimates
@@ -409,6 +414,32 @@ __device__ __forceinline__ void riemann(const double WL[5], const double WR[5], 
 // CHAR-PRIMITIVE `win` holds primitives; for CONSERVATIVE and CHAR-CONSERVATIVE it holds
 // conservatives and `wl`, `wr` hold the primitives of the two cells adjacent to the face.
 // `win` is read only: the marching sweeps keep it in registers across faces.
+// Positivity limiter of one reconstructed face state (limiter_interpolation.py:86-208, single
+// phase): density below eps takes the first-order (upwind cell) value q1 in the limited fields
+// (density and pressure; all five with limit_velocity), then, on the updated state, pressure
+// below eps likewise (pb = 0 for an ideal gas).  Returns the number of replacements.
+__device__ __forceinline__ int limit_face_state(double Wf[5], const double q1[5], const Params& prm) {
+    int n = 0;
+#pragma unroll
+    for (int check = 0; check < 2; ++check) {
+        const bool low = check == 0 ? Wf[0] < prm.lim_eps_rho : Wf[4] < prm.lim_eps_p;
+        if (low) {
+            ++n;
+            Wf[0] = q1[0];
+            Wf[4] = q1[4];
+            if (prm.limit_velocity) {
+                Wf[1] = q1[1];
+                Wf[2] = q1[2];
+                Wf[3] = q1[3];
+            }
+        }
+    }
+    return n;
+}
+
+// Face states WL, WR, UL, UR of one face from its six-cell window (primitives for PRIMITIVE and
+// CHAR-PRIMITIVE, conservatives otherwise); wl, wr are the primitives of the face's two cells,
+// read for CHAR-CONSERVATIVE.  The positivity limiter is applied by the callers (limit_side).
 template <int AXIS, int VAR, int STENCIL>
 __device__ __forceinline__ void face_states(const double (&win)[6][5], const double wl[5], const double wr[5],
                                             const Params& prm, double WL[5], double WR[5], double UL[5],
@@ -559,6 +590,19 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         prim_from_cons(UL, prm, WL);  // :416-417
         prim_from_cons(UR, prm, WR);
     }
+}
+
+// The limiter on one face state (high_order_godunov.py:163-176), given the primitives q1 of the
+// cell upwind of it on its side (the face's left cell for the left state), then U = cons(W)
+// (limiter_interpolation.py:206; for primitive reconstruction U already is cons(W) where nothing
+// was replaced).  Callers load q1 inside their `prm.limiter` branch, one side at a time, so no
+// register holds cell values across the reconstruction.  Returns the number of replacements.
+template <int VAR>
+__device__ __forceinline__ int limit_side(const double q1[5], const Params& prm, double Wf[5], double Uf[5]) {
+    constexpr bool cons_win = VAR == JFA_CONSERVATIVE || VAR == JFA_CHAR_CONSERVATIVE;
+    const int n = limit_face_state(Wf, q1, prm);
+    if (cons_win || n) cons_from_prim(Wf, prm, Uf);
+    return n;
 }
 
 }  // namespace jfa

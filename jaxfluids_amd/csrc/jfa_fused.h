@@ -41,6 +41,15 @@ __device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* 
     }
     double WL[5], WR[5], UL[5], UR[5];
     face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
+    if (prm.limiter) {
+        double c[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) c[i] = ld(W, i * fs + b_right - sa);
+        limit_side<VAR>(c, prm, WL, UL);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) c[i] = ld(W, i * fs + b_right);
+        limit_side<VAR>(c, prm, WR, UR);
+    }
     riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, F);
 }
 

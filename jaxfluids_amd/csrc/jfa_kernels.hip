@@ -63,8 +63,8 @@ __global__ __launch_bounds__(kBlock) void riemann_kernel(const T* __restrict__ W
 // One thread per face.  Loads the 6-cell window of the face along AXIS and evaluates
 // reconstruction + Riemann in registers.
 template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN>
-__device__ __forceinline__ void face_flux_at(const T* __restrict__ W, const T* __restrict__ U, int64_t base,
-                                             int64_t sa, int64_t fs, const Params& prm, double Fo[5]) {
+__device__ __forceinline__ int face_flux_at(const T* __restrict__ W, const T* __restrict__ U, int64_t base,
+                                            int64_t sa, int64_t fs, const Params& prm, double Fo[5]) {
     double win[6][5];
     double wl[5], wr[5];
     constexpr bool cons_win = (VAR == JFA_CONSERVATIVE || VAR == JFA_CHAR_CONSERVATIVE);
@@ -82,17 +82,31 @@ __device__ __forceinline__ void face_flux_at(const T* __restrict__ W, const T* _
     }
     double WL[5], WR[5], UL[5], UR[5];
     face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
+    int nlim = 0;
+    if (prm.limiter) {
+        double c[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) c[i] = ld(W, i * fs + base - sa);
+        nlim = limit_side<VAR>(c, prm, WL, UL);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) c[i] = ld(W, i * fs + base);
+        nlim += limit_side<VAR>(c, prm, WR, UR);
+    }
     riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, Fo);
+    return nlim;
 }
 
 template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN>
 __global__ __launch_bounds__(kBlock) void flux_xi_kernel(const T* __restrict__ W, const T* __restrict__ U, Geo G,
-                                                         FaceIdx F, Params prm, T* __restrict__ flux) {
+                                                         FaceIdx F, Params prm, T* __restrict__ flux,
+                                                         unsigned long long* __restrict__ nlim_total) {
     int f0, f1, f2;
     if (!flat_index(F.nface, F.fe[1], F.fe[2], f0, f1, f2)) return;
     const int64_t base = (int64_t)(f0 + G.off[0]) * G.s[0] + (int64_t)(f1 + G.off[1]) * G.s[1] + (f2 + G.off[2]);
     double Fo[5];
-    face_flux_at<T, AXIS, VAR, STENCIL, RIEMANN>(W, U, base, G.s[AXIS], G.fs, prm, Fo);
+    const int nlim = face_flux_at<T, AXIS, VAR, STENCIL, RIEMANN>(W, U, base, G.s[AXIS], G.fs, prm, Fo);
+    // limiter replacements are rare: one global atomic per face that has any
+    if (nlim && nlim_total) atomicAdd(nlim_total, (unsigned long long)nlim);
     const int64_t o = ((int64_t)f0 * F.fe[1] + f1) * F.fe[2] + f2;
 #pragma unroll
     for (int i = 0; i < 5; ++i) flux[(int64_t)i * F.nface + o] = static_cast<T>(Fo[i]);
@@ -251,14 +265,15 @@ __global__ __launch_bounds__(kBlock) void convert_kernel(const T* __restrict__ i
 // ------------------------------------------------------------------ fused stage epilogue
 
 // ------------------------------------------------------------------ dispatch tables
-using FluxLaunch = void (*)(const void*, const void*, const Geo&, const FaceIdx&, const Params&, void*, hipStream_t);
+using FluxLaunch = void (*)(const void*, const void*, const Geo&, const FaceIdx&, const Params&, void*,
+                           unsigned long long*, hipStream_t);
 
 template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN>
 static void launch_flux(const void* W, const void* U, const Geo& G, const FaceIdx& F, const Params& prm, void* out,
-                        hipStream_t st) {
+                        unsigned long long* nlim, hipStream_t st) {
     dim3 grid = flat_grid(F.nface);
     hipLaunchKernelGGL((flux_xi_kernel<T, AXIS, VAR, STENCIL, RIEMANN>), grid, dim3(kBlock), 0, st,
-                       static_cast<const T*>(W), static_cast<const T*>(U), G, F, prm, static_cast<T*>(out));
+                       static_cast<const T*>(W), static_cast<const T*>(U), G, F, prm, static_cast<T*>(out), nlim);
 }
 
 template <class T, int AXIS, int VAR, int STENCIL>
@@ -291,12 +306,12 @@ static FluxLaunch pick_flux(int axis, int var, int stencil, int riemann) {
 
 
 static int run_flux(const void* W, const void* U, const Geo& G, const jfa_scheme* s, int axis, void* flux,
-                    hipStream_t st) {
+                    int64_t* nlim, hipStream_t st) {
     const FaceIdx F = face_extents(G, axis);
     const Params prm = make_params(s);
     FluxLaunch fn = s->dtype == JFA_F32 ? pick_flux<float>(axis, s->recon_var, s->stencil, s->riemann)
                                         : pick_flux<double>(axis, s->recon_var, s->stencil, s->riemann);
-    fn(W, U, G, F, prm, flux, st);
+    fn(W, U, G, F, prm, flux, prm.limiter ? reinterpret_cast<unsigned long long*>(nlim) : nullptr, st);
     return check_launch();
 }
 
@@ -359,14 +374,14 @@ int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void
 }
 
 int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme, int32_t axis,
-                        void* flux, void* stream) {
+                        void* flux, int64_t* limiter_count, void* stream) {
     Geo G;
     int rc = make_geo(grid, G);
     if (rc) return rc;
     rc = check_scheme(scheme);
     if (rc) return rc;
     if (!W || !U || !flux || axis < 0 || axis > 2 || !G.active[axis]) return JFA_EINVAL;
-    return run_flux(W, U, G, scheme, axis, flux, (hipStream_t)stream);
+    return run_flux(W, U, G, scheme, axis, flux, limiter_count, (hipStream_t)stream);
 }
 
 size_t jfa_rhs_workspace_bytes(const jfa_grid* grid, int32_t dtype) {

@@ -37,9 +37,11 @@ class TimeIntegrationSetup(NamedTuple):
 
 
 class PositivitySetup(NamedTuple):
+    """conservatives.py PositivitySetup (the single-phase fields)"""
     is_interpolation_limiter: bool = False
     is_thinc_interpolation_limiter: bool = False
     flux_limiter: Any = False
+    limit_velocity: bool = False
 
 
 class DissipativeFluxesSetup(NamedTuple):

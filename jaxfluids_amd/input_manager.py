@@ -156,9 +156,14 @@ def read_numerical_setup(d: Dict) -> NumericalSetup:
         f"halo_cells={halo_cells} < {stencil_cls.required_halos} required by {st}"
     godunov = HighOrderGodunovSetup(DICT_RIEMANN_SOLVER[rs], DICT_SIGNAL_SPEEDS[ss], stencil_cls, None, rv, fz, None)
     conv = ConvectiveFluxesSetup(DICT_CONVECTIVE_SOLVER[solver], godunov)
-    pos = c.get("positivity", {})
-    if pos.get("is_interpolation_limiter") or pos.get("flux_limiter"):
-        raise NotImplementedError("positivity limiters are not on this path")
+    pos = c.get("positivity", {})  # read_positivity.py:8-64
+    if pos.get("flux_limiter"):
+        raise NotImplementedError("the positivity flux limiter is not built (the interpolation limiter is)")
+    for key in ("is_thinc_interpolation_limiter", "is_volume_fraction_limiter", "is_acdi_flux_limiter"):
+        if pos.get(key):
+            raise NotImplementedError(f"positivity/{key} belongs to multiphase models, not on this path")
+    positivity = PositivitySetup(is_interpolation_limiter=bool(pos.get("is_interpolation_limiter", False)),
+                                 limit_velocity=bool(pos.get("limit_velocity", False)))
     ap = d.get("active_physics", {})
     active = ActivePhysicsSetup(
         is_convective_flux=bool(ap.get("is_convective_flux", True)),
@@ -170,7 +175,7 @@ def read_numerical_setup(d: Dict) -> NumericalSetup:
     dissipative = read_dissipative_fluxes(c.get("dissipative_fluxes", {})) \
         if (active.is_viscous_flux or active.is_heat_flux) else None
     cons = ConservativesSetup(halo_cells, TimeIntegrationSetup(DICT_TIME_INTEGRATION[integrator], cfl, fixed), conv,
-                              dissipative)
+                              dissipative, positivity)
     pr = d.get("precision", {})
     prec = PrecisionSetup(bool(pr.get("is_double_precision_compute", True)),
                           bool(pr.get("is_double_precision_output", True)))

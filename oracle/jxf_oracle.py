@@ -572,6 +572,9 @@ class Setup:
     bcs: Dict[str, str] = field(default_factory=lambda: {f: "PERIODIC" for f in FACE_LOCATIONS})
     eps_stencil: float = SPATIAL_STENCIL_EPS
     transport: Optional[Transport] = None     # viscous (and heat) fluxes when set
+    interpolation_limiter: bool = False       # positivity.is_interpolation_limiter
+    limit_velocity: bool = False              # positivity.limit_velocity
+    limiter_eps: Tuple[float, float] = (1e-12, 1e-10)  # (density, pressure), config/precision.py:54
 
 
 def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
@@ -614,11 +617,38 @@ def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
     return U_L, U_R, W_L, W_R
 
 
-def compute_flux_xi(W, U, dom: Domain, axis: int, s: Setup):
-    """HighOrderGodunov.compute_flux_xi (high_order_godunov.py:117-231), single phase,
-    no limiter: reconstruct then solve the Riemann problem."""
+def limit_interpolation_xi(W, W_j, j: int, dom: Domain, axis: int, s: Setup):
+    """PositivityLimiterInterpolation.limit_interpolation_xi, single phase
+    (solvers/positivity/limiter_interpolation.py:86-208): face states with density below
+    eps.density take the first-order value (WENO1, the cell upwind of the face on side j:
+    weno1_js.py:22-29) in the limited fields (density and pressure; all five with
+    limit_velocity, :61-69); then, on the updated states, those with p + pb < eps.pressure
+    (pb = 0 for an ideal gas) likewise.  Returns (U_j, W_j, count) with U_j = cons(W_j) on every
+    face (:206) and count the number of replacements."""
+    first = face_LR_cells(W, dom, axis)[j]
+    ids = [0, 1, 2, 3, 4] if s.limit_velocity else [0, 4]
+    W_j = W_j.copy()
+    count = 0
+    for field_id, eps in ((0, s.limiter_eps[0]), (4, s.limiter_eps[1])):
+        mask = W_j[field_id] < eps
+        count += int(np.sum(mask))
+        for i in ids:
+            W_j[i] = W_j[i] * (1 - mask) + first[i] * mask
+    return cons_from_prim(W_j, s.gamma), W_j, count
+
+
+def compute_flux_xi(W, U, dom: Domain, axis: int, s: Setup, return_count: bool = False):
+    """HighOrderGodunov.compute_flux_xi (high_order_godunov.py:117-231), single phase:
+    reconstruct, apply the interpolation limiter when enabled (:163-176), then solve the
+    Riemann problem."""
     U_L, U_R, W_L, W_R = reconstruct_xi(W, U, dom, axis, s)
-    return RIEMANN[s.riemann](W_L, W_R, U_L, U_R, axis, s.gamma, s.signal_speed)
+    count = None
+    if s.interpolation_limiter:
+        U_L, W_L, cL = limit_interpolation_xi(W, W_L, 0, dom, axis, s)
+        U_R, W_R, cR = limit_interpolation_xi(W, W_R, 1, dom, axis, s)
+        count = cL + cR
+    F = RIEMANN[s.riemann](W_L, W_R, U_L, U_R, axis, s.gamma, s.signal_speed)
+    return (F, count) if return_count else F
 
 
 def flux_divergence(F, one_dx, axis):

@@ -70,6 +70,7 @@ inline T from_bits(uint64_t b) {
 #define hipLaunchKernelGGL(K, G, B, S, ST, ...) emu::launch(dim3(G), dim3(B), [=]() { K(__VA_ARGS__); })
 #define JFA_ACCESS(p, bytes) emu::access((const void*)(p), (bytes))
 #define JFA_OPAQUE(x) ((void)0)
+#define JFA_WAVES_PER_EU(n)
 
 template <class T>
 inline T __shfl_down(T v, unsigned d, int = 64) {

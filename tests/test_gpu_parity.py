@@ -70,7 +70,7 @@ def native_scheme(s, dtype=0):
     from jaxfluids_amd import _native
 
     return _native.make_scheme(s.variable, s.stencil, s.riemann, s.signal_speed, s.frozen_state, dtype, s.gamma,
-                               s.eps_stencil)
+                               s.eps_stencil, None, s.interpolation_limiter, s.limit_velocity, s.limiter_eps)
 
 
 # ------------------------------------------------------------------ reconstruct_xi
@@ -163,7 +163,7 @@ def test_compute_flux_xi_parity(case):
         ref = o.compute_flux_xi(W, U, dom, axis, s)
         out = torch.empty(ref.shape, dtype=torch.float64, device="cuda")
         rc = lib.jfa_compute_flux_xi(_native.ptr(Wd), _native.ptr(Ud), native_grid(dom), native_scheme(s), axis,
-                                     _native.ptr(out), _native.stream_ptr())
+                                     _native.ptr(out), None, _native.stream_ptr())
         _native.check(rc, "flux")
         assert rel(host(out), ref) < TOL, (axis, rel(host(out), ref))
 
@@ -264,7 +264,7 @@ def test_rk3_step_parity(case, bc, fused):
     assert abs(host(dt2)[0] - o.compute_dt(Wr, dom, s)) < 1e-14 * dt
 
 
-def _input_manager(dom, s, end_time=None, end_step=None):
+def _input_manager(dom, s, end_time=None, end_step=None, positivity=None):
     from jaxfluids_amd import InputManager
 
     case = {
@@ -281,7 +281,8 @@ def _input_manager(dom, s, end_time=None, end_step=None):
                              "convective_fluxes": {"convective_solver": "GODUNOV", "godunov": {
                                  "riemann_solver": s.riemann, "signal_speed": s.signal_speed or "EINFELDT",
                                  "reconstruction_stencil": s.stencil, "reconstruction_variable": s.variable,
-                                 "frozen_state": s.frozen_state}}},
+                                 "frozen_state": s.frozen_state}},
+                             "positivity": positivity or {}},
            "active_physics": {"is_convective_flux": True},
            "precision": {"is_double_precision_compute": True}}
     return InputManager(case, num)

@@ -1,0 +1,96 @@
+"""GPU parity of the positivity limiter of the reconstructed face states (interpolation limiter,
+solvers/positivity/limiter_interpolation.py:86-208) against the oracle: face fluxes and the
+limiter count of compute_flux_xi, fused RK3 steps, and the plugin-order step, on strong-jump
+data that makes WENO overshoot to non-positive face states (FP64, relative L-inf <= 1e-12)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import jxf_oracle as o
+from test_gpu_parity import dev, host, native_grid, native_scheme, rel, say
+from test_oracle_pins import _jump_state, _pocket_state
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+FACES = ("east", "west", "north", "south", "top", "bottom")
+ZG2 = {"east": "ZEROGRADIENT", "west": "ZEROGRADIENT", "north": "ZEROGRADIENT", "south": "ZEROGRADIENT"}
+
+
+@pytest.mark.parametrize("var", ["PRIMITIVE", "CHAR-PRIMITIVE", "CONSERVATIVE", "CHAR-CONSERVATIVE"])
+@pytest.mark.parametrize("limit_velocity", [False, True])
+def test_flux_xi_with_limiter(var, limit_velocity):
+    from jaxfluids_amd import _native
+
+    say(f"[limiter flux] {var} limit_velocity={limit_velocity}")
+    dom, W0 = _jump_state(24, 11, dims=2, log_rho=-3.0, log_p=-3.0, umax=0.5)  # well-conditioned
+    s = o.Setup(stencil="WENO5-Z", variable=var, riemann="HLLC", signal_speed="EINFELDT", bcs=ZG2,
+                interpolation_limiter=True, limit_velocity=limit_velocity)
+    W, U = o.create_buffers(W0, dom, s)
+    lib = _native.load()
+    Wd, Ud = dev(W), dev(U)
+    for axis in (0, 1):
+        ref, n_ref = o.compute_flux_xi(W, U, dom, axis, s, return_count=True)
+        assert n_ref > 10
+        out = torch.empty(ref.shape, dtype=torch.float64, device="cuda")
+        cnt = torch.zeros((), dtype=torch.int64, device="cuda")
+        rc = lib.jfa_compute_flux_xi(_native.ptr(Wd), _native.ptr(Ud), native_grid(dom), native_scheme(s), axis,
+                                     _native.ptr(out), _native.ptr(cnt), _native.stream_ptr())
+        _native.check(rc, "jfa_compute_flux_xi")
+        assert int(host(cnt)) == n_ref
+        assert rel(host(out), ref) < TOL, (axis, rel(host(out), ref))
+
+
+@pytest.mark.parametrize("var", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+def test_fused_rk3_step_with_limiter(var):
+    """One fused RK3 step on the near-vacuum pocket, where the step is not finite without the
+    limiter (checked on the oracle in the CPU suite)."""
+    from jaxfluids_amd import _native
+
+    say(f"[limiter step] {var}")
+    dom, W0 = _pocket_state()
+    s = o.Setup(stencil="WENO5-JS", variable=var, riemann="HLLC", signal_speed="EINFELDT", bcs=ZG2,
+                interpolation_limiter=True)
+    W, U = o.create_buffers(W0, dom, s)
+    dt = 0.2 * o.compute_dt(W, dom, s)
+    Wr, Ur = o.rk3_step(W, U, dt, dom, s)
+    lib = _native.load()
+    grid, scheme = native_grid(dom), native_scheme(s)
+    bcs = _native.bc_array({**ZG2, "top": "PERIODIC", "bottom": "PERIODIC"})  # z is inactive
+    ws = torch.empty(lib.jfa_stage_workspace_bytes(grid, scheme), dtype=torch.uint8, device="cuda")
+    Wd, Ud = dev(W), dev(U)
+    Wb, Ub, Wc, Uc = (torch.empty_like(Wd) for _ in range(4))
+    dtd = torch.full((1,), dt, dtype=torch.float64, device="cuda")
+    P = _native.ptr
+    for k, (wi, ui, wo, uo) in enumerate([(Wd, Ud, Wb, Ub), (Wb, Ub, Wc, Uc), (Wc, Uc, Wd, Ud)]):
+        rc = lib.jfa_rk3_stage_fused(P(wi), P(ui), P(Ud if k else None), P(wo), P(uo), grid, scheme, bcs, k, P(dtd),
+                                     0.0, s.cfl, o.EPS64, P(None), P(ws), ws.numel(), _native.stream_ptr())
+        _native.check(rc, "jfa_rk3_stage_fused")
+    I = dom.interior
+    assert np.all(np.isfinite(Wr[I]))
+    assert rel(host(Wd)[I], Wr[I]) < TOL and rel(host(Ud)[I], Ur[I]) < TOL
+
+
+def test_limiter_through_setup_dicts():
+    """positivity/is_interpolation_limiter in the numerical setup: one plugin-order step and the
+    flux's limiter count through SimulationManager / HighOrderGodunov."""
+    from jaxfluids_amd import SimulationManager
+    from test_gpu_parity import _input_manager
+
+    say("[limiter setup dicts]")
+    dom, W0 = _pocket_state()
+    s = o.Setup(stencil="WENO5-JS", variable="CHAR-PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT", bcs=ZG2,
+                interpolation_limiter=True)
+    im = _input_manager(dom, s, positivity={"is_interpolation_limiter": True})
+    sim = SimulationManager(im, fused=False)
+    W, U = o.create_buffers(W0, dom, s)
+    dt = 0.2 * o.compute_dt(W, dom, s)
+    Wr, _ = o.rk3_step(W, U, dt, dom, s)
+    Wd, Ud = dev(W), dev(U)
+    _, _, _, count, _ = sim.convective_flux_solver.compute_flux_xi(Wd, Ud, 0)
+    assert int(host(count)) == o.compute_flux_xi(W, U, dom, 0, s, return_count=True)[1] > 0
+    dtd = torch.full((1,), dt, dtype=torch.float64, device="cuda")
+    Wd, Ud = sim.do_runge_kutta_stages(Wd, Ud, dtd)
+    I = dom.interior
+    assert np.all(np.isfinite(Wr[I]))
+    assert rel(host(Wd)[I], Wr[I]) < TOL
