@@ -251,8 +251,19 @@ def bench_ensemble(steps: int, warmup: int, ncases: int = 64, seed: int = 0, nh:
     per_restore()
     t_per_graph = timed(lambda n: [g.replay() for _ in range(n)], steps)
     finite = int(sum(bool(torch.isfinite(ens.view(W_batched, i)).all().item()) for i in range(ncases)))
-
     rate = lambda t: ncell * 3 * steps / t / 1e6
+
+    # the same cases with the positivity limiter of the face states (positivity.is_interpolation_limiter)
+    lim = Ensemble2D([{"cells": (c.nx, c.ny), "ranges": ((0.0, 1.0), (0.0, 1.0)), "bcs": zg} for c in qc],
+                     nh=nh, cfl=0.5, chunk=chunk, interpolation_limiter=True, **scheme)
+    lim.initialize([quadrant_primitives(c) for c in qc])
+    lim.step(warmup)
+    lim.initialize([quadrant_primitives(c) for c in qc])
+    t_lim = timed(lambda n: lim.step(n), steps)
+    limited = {"batched_eager": round(rate(t_lim), 2),
+               "finite_cases": int(sum(bool(torch.isfinite(lim.state(i)[0]).all().item()) for i in range(ncases)))}
+    del lim
+
     achieved = kern_bytes / (kern_ms * 1e-3) / 1e9
     return {
         "workload": f"{ncases} independent 2-D four-quadrant Riemann problems, sizes from "
@@ -266,6 +277,7 @@ def bench_ensemble(steps: int, warmup: int, ncases: int = 64, seed: int = 0, nh:
         "batched_vs_per_case_max_rel_diff_10_steps": max(d10) if d10 else None,
         "cases_within_1e-10_after_all_steps": int(sum(1 for d in d200 if d == d and d <= 1e-10)),
         "finite_cases": finite,
+        "with_interpolation_limiter": limited,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "batch2d_stage_kernel", "avg_stage_ms": round(kern_ms / len(ev), 4)},

@@ -171,14 +171,20 @@ class FixedDtStepper:
                                                 _native.stream_ptr())
         _native.check(rc, "jfa_rk3_stage_fused")
 
-    def step(self, W: torch.Tensor, U: torch.Tensor, n: int = 1) -> None:
-        """n RK3 steps in place (simulation_manager.py:770-1047 at fixed dt)."""
+    def step(self, W: torch.Tensor, U: torch.Tensor, n: int = 1, keep_stages: bool = False):
+        """n RK3 steps in place (simulation_manager.py:770-1047 at fixed dt).  With keep_stages
+        (n = 1), returns copies of the primitives that enter stages 2 and 3, which let step_vjp
+        skip recomputing them."""
         (Wb, Ub), (Wc, Uc) = self._B, self._C
+        kept = None
         for _ in range(n):
             self._stage(W, U, None, Wb, Ub, 0)
             self._stage(Wb, Ub, U, Wc, Uc, 1)
+            if keep_stages:
+                kept = (Wb.clone(), Wc.clone())
             self._stage(Wc, Uc, U, W, U, 2)  # U_out aliases U_n (cell-local read-before-write)
         self.forward_steps += n
+        return kept
 
     def _vjp(self, W_in, lam_out, lam_in, lam_n, k) -> None:
         P = _native.ptr
@@ -187,12 +193,17 @@ class FixedDtStepper:
                                               _native.stream_ptr())
         _native.check(rc, "jfa_rk3_stage_vjp")
 
-    def step_vjp(self, W: torch.Tensor, U: torch.Tensor, lam: torch.Tensor) -> torch.Tensor:
+    def step_vjp(self, W: torch.Tensor, U: torch.Tensor, lam: torch.Tensor,
+                 stages: Optional[Tuple[torch.Tensor, torch.Tensor]] = None) -> torch.Tensor:
         """dL/dU_n from lam = dL/dU_{n+1} (interior of buffer-layout tensors); (W, U) is the state at
-        the start of the step and is not modified.  The step's stage states are recomputed."""
+        the start of the step and is not modified.  The primitives entering stages 2 and 3 are
+        taken from `stages` (kept by step) or recomputed."""
         (Wb, Ub), (Wc, Uc) = self._B, self._C
-        self._stage(W, U, None, Wb, Ub, 0)
-        self._stage(Wb, Ub, U, Wc, Uc, 1)
+        if stages is not None:
+            Wb, Wc = stages
+        else:
+            self._stage(W, U, None, Wb, Ub, 0)
+            self._stage(Wb, Ub, U, Wc, Uc, 1)
         l2, l1, l0 = self._lam
         for t in (l2, l1, l0):
             t.zero_()  # fill kernels (graph-safe), halos stay zero
@@ -215,8 +226,10 @@ class FixedDtStepper:
             self.step(work[0], work[1], n)
             return work
 
+        # a stored state may carry its step's stage primitives (W, U, W2, W3)
         return reverse_rollout(snaps, nsteps, snapshots, lam, advance,
-                               lambda state, l: self.step_vjp(state[0], state[1], l),
+                               lambda state, l: self.step_vjp(state[0], state[1], l,
+                                                              state[2:] if len(state) == 4 else None),
                                lambda state: (state[0].clone(), state[1].clone()))
 
 
@@ -231,8 +244,13 @@ class RK3Rollout(torch.autograd.Function):
         W, U = stepper.fill(W0.detach())
         snaps = {0: (W.clone(), U.clone())}
         chain = set(first_level_snapshots(nsteps, snapshots))
+        # with a snapshot at every step (snapshots >= nsteps - 1) the stage primitives are kept too,
+        # so the reverse sweep evaluates only the three stage VJPs per step
+        keep = snapshots >= nsteps - 1
         for n in range(1, nsteps + 1):
-            stepper.step(W, U)
+            kept = stepper.step(W, U, keep_stages=keep)
+            if keep:
+                snaps[n - 1] = snaps[n - 1] + kept
             if n in chain:
                 snaps[n] = (W.clone(), U.clone())
         ctx.stepper, ctx.nsteps, ctx.snapshots, ctx.snaps = stepper, nsteps, snapshots, snaps
