@@ -57,6 +57,12 @@ extern "C" {
 /* storage / compute precision */
 #define JFA_F64 0
 #define JFA_F32 1
+/* positivity.flux_limiter and flux_partition (solvers/positivity/__init__.py:1-7) */
+#define JFA_FLUX_LIMITER_NONE 0
+#define JFA_FLUX_LIMITER_SIMPLE 1
+#define JFA_FLUX_LIMITER_NASA 2
+#define JFA_PARTITION_UNIFORM 0
+#define JFA_PARTITION_CELLSIZE 1
 
 /* Uniform grid of one domain (domain/domain_information.py:68-300). */
 typedef struct jfa_grid {
@@ -118,6 +124,18 @@ typedef struct jfa_scheme {
     int32_t limit_velocity;
     double limiter_eps_density;
     double limiter_eps_pressure;
+    /* Positivity flux limiter of the convective face flux (positivity.flux_limiter SIMPLE / NASA,
+     * solvers/positivity/limiter_flux.py:147-316; applied by the stage and RHS entry points, which
+     * know the step's dt, as SpaceSolver.compute_rhs_xi does, space_solver.py:530-541): where the
+     * face's pseudo-updated cell states U_R + 2 lam F, U_L - 2 lam F (NASA: F minus each cell's
+     * physical flux), lam = dt sigma / dx, have a density, then a pressure, below the epsilons, the
+     * flux becomes the first-order HLLC-Einfeldt flux of the two cells.  sigma: the number of
+     * active axes (UNIFORM) or sum(1/dx_a) / (1/dx) (CELLSIZE).  Epsilons 1e-12 / 1e-10 in double
+     * precision, 1e-6 / 1e-6 in single (config/precision.py:41,55). */
+    int32_t flux_limiter;    /* JFA_FLUX_LIMITER_* */
+    int32_t flux_partition;  /* JFA_PARTITION_* */
+    double flux_limiter_eps_density;
+    double flux_limiter_eps_pressure;
 } jfa_scheme;
 
 /* Library version string; never NULL. */
@@ -154,9 +172,11 @@ int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, cons
 /* Right-hand side 0.0 + rhs_x + rhs_y + rhs_z, rhs_xi = (1/dx)(F[:-1]-F[1:]).
  * rhs: (5, Nx, Ny, Nz).  workspace: >= jfa_rhs_workspace_bytes(grid, dtype) bytes (FP64 partial RHS).
  * Replaces SpaceSolver.compute_rhs (solvers/space_solver.py:151-453) and compute_rhs_xi (:456-674):
- * the convective flux, plus the dissipative fluxes when scheme->transport is set (the caller's
- * boundaries must then be periodic). */
-int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme,
+ * the convective flux (positivity-limited with the step's time step dt when
+ * scheme->flux_limiter is set, as compute_rhs's physical_timestep_size; dt is unused otherwise),
+ * plus the dissipative fluxes when scheme->transport is set (the caller's boundaries must then be
+ * periodic). */
+int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme, double dt,
                     void* rhs, void* workspace, size_t workspace_bytes, void* stream);
 size_t jfa_rhs_workspace_bytes(const jfa_grid* grid, int32_t dtype);
 

@@ -54,6 +54,10 @@ class JfaScheme(ctypes.Structure):
         ("limit_velocity", ctypes.c_int32),
         ("limiter_eps_density", ctypes.c_double),
         ("limiter_eps_pressure", ctypes.c_double),
+        ("flux_limiter", ctypes.c_int32),
+        ("flux_partition", ctypes.c_int32),
+        ("flux_limiter_eps_density", ctypes.c_double),
+        ("flux_limiter_eps_pressure", ctypes.c_double),
     ]
 
 
@@ -83,7 +87,7 @@ SIGNATURES = {
     "jfa_weno5_reconstruct_xi": (_I32, [_P, _I32, _PG, _I32, _I32, _I32, _D, _I32, _P, _P]),
     "jfa_riemann_xi": (_I32, [_P, _P, _P, _P, _I64, _I32, _PS, _P, _P]),
     "jfa_compute_flux_xi": (_I32, [_P, _P, _PG, _PS, _I32, _P, _P, _P]),
-    "jfa_compute_rhs": (_I32, [_P, _P, _PG, _PS, _P, _P, _SZ, _P]),
+    "jfa_compute_rhs": (_I32, [_P, _P, _PG, _PS, _D, _P, _P, _SZ, _P]),
     "jfa_rhs_workspace_bytes": (_SZ, [_PG, _I32]),
     "jfa_rk3_stage": (_I32, [_P, _P, _P, _P, _PG, _PS, _I32, _P, _D, _P]),
     "jfa_halo_update": (_I32, [_P, _P, _PG, _PI32, _D, _I32, _P]),
@@ -160,6 +164,8 @@ def make_grid(cells: Sequence[int], nh: int, dx: Sequence[float]) -> JfaGrid:
 
 
 LAW_POWER, LAW_SUTHERLAND, LAW_PRANDTL = 0, 1, 2
+FLUX_LIMITER = {None: 0, "SIMPLE": 1, "NASA": 2}
+FLUX_PARTITION = {"UNIFORM": 0, "CELLSIZE": 1}
 
 
 def make_transport(R: float = 1.0, mu_ref: float = 0.0, T_ref: float = 1.0, mu_exp: float = 0.0,
@@ -194,9 +200,13 @@ def make_transport(R: float = 1.0, mu_ref: float = 0.0, T_ref: float = 1.0, mu_e
 def make_scheme(recon_var: str, stencil: str, riemann: str, signal_speed: Optional[str], frozen_state: str,
                 dtype: int, gamma: float, eps_stencil: float, transport: Optional[JfaTransport] = None,
                 interpolation_limiter: bool = False, limit_velocity: bool = False,
-                limiter_eps: Tuple[float, float] = (1e-12, 1e-10)) -> JfaScheme:
+                limiter_eps: Tuple[float, float] = (1e-12, 1e-10), flux_limiter: Optional[str] = None,
+                flux_partition: str = "UNIFORM",
+                flux_limiter_eps: Tuple[float, float] = (1e-12, 1e-10)) -> JfaScheme:
     """Native scheme; ``limiter_eps`` = (density, pressure) epsilons of the positivity limiter of
-    the reconstructed face states (used when ``interpolation_limiter``)."""
+    the reconstructed face states (used when ``interpolation_limiter``); ``flux_limiter`` "SIMPLE"
+    / "NASA" with ``flux_partition`` "UNIFORM" / "CELLSIZE" and its epsilons, the positivity flux
+    limiter of the stage and RHS entry points."""
     s = JfaScheme()
     try:
         s.recon_var = RECON_VAR[recon_var]
@@ -215,6 +225,12 @@ def make_scheme(recon_var: str, stencil: str, riemann: str, signal_speed: Option
     s.interpolation_limiter = 1 if interpolation_limiter else 0
     s.limit_velocity = 1 if limit_velocity else 0
     s.limiter_eps_density, s.limiter_eps_pressure = (float(v) for v in limiter_eps)
+    try:
+        s.flux_limiter = FLUX_LIMITER[flux_limiter or None]
+        s.flux_partition = FLUX_PARTITION[flux_partition]
+    except KeyError as e:
+        raise NotImplementedError(f"positivity option {e} is not built into jaxfluids_amd") from None
+    s.flux_limiter_eps_density, s.flux_limiter_eps_pressure = (float(v) for v in flux_limiter_eps)
     return s
 
 

@@ -21,6 +21,7 @@ class PrecisionConfig:
         self._eps = float(np.finfo(np.float64).eps)
         self._spatial_stencil_eps = 1e-30
         self._interpolation_limiter_eps = (1e-12, 1e-10)  # (density, pressure), precision.py:54
+        self._flux_limiter_eps = (1e-12, 1e-10)  # precision.py:55
 
     def enable_single_precision(self) -> None:
         # precision.py:30-42
@@ -28,6 +29,7 @@ class PrecisionConfig:
         self._eps = float(np.finfo(np.float32).eps)
         self._spatial_stencil_eps = 1e-30
         self._interpolation_limiter_eps = (1e-6, 1e-6)  # precision.py:40
+        self._flux_limiter_eps = (1e-6, 1e-6)  # precision.py:41
 
     def get_eps(self) -> float:
         return self._eps
@@ -38,6 +40,14 @@ class PrecisionConfig:
     def get_interpolation_limiter_eps(self):
         """(density, pressure) thresholds of the positivity limiter of the face states."""
         return self._interpolation_limiter_eps
+
+    def get_flux_limiter_eps(self):
+        """(density, pressure) thresholds of the positivity flux limiter."""
+        return self._flux_limiter_eps
+
+    def set_flux_limiter_eps(self, density: float, pressure: float) -> None:
+        assert density > 0.0 and pressure > 0.0, "limiter epsilons must be positive"
+        self._flux_limiter_eps = (float(density), float(pressure))
 
     def set_interpolation_limiter_eps(self, density: float, pressure: float) -> None:
         assert density > 0.0 and pressure > 0.0, "limiter epsilons must be positive"

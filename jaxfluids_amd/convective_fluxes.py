@@ -45,6 +45,9 @@ class HighOrderGodunov(ConvectiveFluxSolver):
         # positivity limiter of the face states (high_order_godunov.py:67,163-176)
         self.is_interpolation_limiter = bool(getattr(positivity_setup, "is_interpolation_limiter", False))
         self.limit_velocity = bool(getattr(positivity_setup, "limit_velocity", False))
+        # positivity flux limiter, applied by the stage / RHS entry points (space_solver.py:530-541)
+        self.flux_limiter = getattr(positivity_setup, "flux_limiter", None) or None
+        self.flux_partition = getattr(positivity_setup, "flux_partition", "UNIFORM")
         self.riemann_solver = godunov.riemann_solver(
             material_manager=material_manager, equation_manager=equation_manager,
             signal_speed=godunov.signal_speed, catum_setup=godunov.catum_setup)
@@ -70,7 +73,8 @@ class HighOrderGodunov(ConvectiveFluxSolver):
             self._schemes[key] = _native.make_scheme(
                 *self._names, dtype_code, self.material_manager.get_gamma(), precision.get_spatial_stencil_eps(),
                 transport, self.is_interpolation_limiter, self.limit_velocity,
-                precision.get_interpolation_limiter_eps())
+                precision.get_interpolation_limiter_eps(), self.flux_limiter, self.flux_partition,
+                precision.get_flux_limiter_eps())
         return self._schemes[key]
 
     def compute_flux_xi(self, primitives: torch.Tensor, conservatives: torch.Tensor, axis: int,

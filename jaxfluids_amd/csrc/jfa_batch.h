@@ -140,7 +140,7 @@ __device__ __forceinline__ void sto(T* __restrict__ base, uint32_t voff, double 
 
 // Flux of one face along AXIS (stride st): window cells -3..2 relative to the cell right of
 // the face, which sits at element rb (wave-uniform) + voff bytes (per lane) of the case block.
-template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN>
+template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN, bool FLIM>
 __device__ __forceinline__ void face_flux(const T* __restrict__ W, const T* __restrict__ U, int64_t rb,
                                           uint32_t voff, int64_t st, int64_t fs, const Params& prm, double F[5]) {
     constexpr bool cons_win = (VAR == JFA_CONSERVATIVE || VAR == JFA_CHAR_CONSERVATIVE);
@@ -170,15 +170,34 @@ __device__ __forceinline__ void face_flux(const T* __restrict__ W, const T* __re
         limit_side<VAR>(c, prm, WR, UR);
     }
     riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, F);
+    if constexpr (FLIM) {  // positivity flux limiter with the case's dt (set per item)
+        double wl_[5], wr_[5], ul_[5], ur_[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            wl_[i] = ldo(W + (rb + i * fs - st), voff);
+            wr_[i] = ldo(W + (rb + i * fs), voff);
+            ul_[i] = ldo(U + (rb + i * fs - st), voff);
+            ur_[i] = ldo(U + (rb + i * fs), voff);
+        }
+        limit_flux<AXIS>(F, wl_, wr_, ul_, ur_, prm.dt_val * prm.flim_fac[AXIS], prm);
+    }
 }
 
 // One work item.  Only flux terms are carried from plane to plane (the x-face below the
 // current plane and, in lane k, the extra y-face of plane x0+k); each face reloads its
 // six-cell window, which the previous plane's loads leave in cache.  This keeps the
 // register footprint near that of a single face evaluation.
-template <class T, int VAR, int STENCIL, int RIEMANN>
-__device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& prm, const Item2D it) {
+template <class T, int VAR, int STENCIL, int RIEMANN, bool FLIM>
+__device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& prm_in, const Item2D it) {
     const Case2D cs = load_case(a.cases + it.c);
+    // the flux limiter's lam = dt (1/dx) sigma uses this case's dt and cell sizes
+    Params prm = prm_in;
+    if constexpr (FLIM) {
+        prm.dt_val = uni(a.dt[it.c]);
+        const double sum = cs.one_dx0 + cs.one_dx1;
+        prm.flim_fac[0] = cs.one_dx0 * (prm.flux_partition == JFA_PARTITION_CELLSIZE ? sum / cs.one_dx0 : 2.0);
+        prm.flim_fac[1] = cs.one_dx1 * (prm.flux_partition == JFA_PARTITION_CELLSIZE ? sum / cs.one_dx1 : 2.0);
+    }
     const int lane = threadIdx.x & 63;
     const int y = it.y0 + lane;               // this lane's cell / lower face along y
     const int64_t fs = cs.fs;
@@ -195,11 +214,11 @@ __device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& pr
     double Fext[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (it.y0 + 64 <= cs.n1 && lane < it.x1 - it.x0) {
         const uint32_t vext = ((uint32_t)lane * (uint32_t)sx + (uint32_t)(it.y0 + 64 + cs.nh)) * (uint32_t)sizeof(T);
-        face_flux<T, 1, VAR, STENCIL, RIEMANN>(W, U, (int64_t)(it.x0 + cs.nh) * sx, vext, 1, fs, prm, Fext);
+        face_flux<T, 1, VAR, STENCIL, RIEMANN, FLIM>(W, U, (int64_t)(it.x0 + cs.nh) * sx, vext, 1, fs, prm, Fext);
     }
     // x-face x0 - 1/2
     double Fxp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (has_cell) face_flux<T, 0, VAR, STENCIL, RIEMANN>(W, U, (int64_t)(it.x0 + cs.nh) * sx, voff, sx, fs, prm, Fxp);
+    if (has_cell) face_flux<T, 0, VAR, STENCIL, RIEMANN, FLIM>(W, U, (int64_t)(it.x0 + cs.nh) * sx, voff, sx, fs, prm, Fxp);
     double c0, c1, m;
     rk3_coeffs(a.stage, c0, c1, m);
     const double ts = uni(a.dt[it.c]) * m;
@@ -215,7 +234,7 @@ __device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& pr
         double dyt[5];
         {
             double Fy[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            if (has_face) face_flux<T, 1, VAR, STENCIL, RIEMANN>(W, U, rb, vo, 1, fs, prm, Fy);
+            if (has_face) face_flux<T, 1, VAR, STENCIL, RIEMANN, FLIM>(W, U, rb, vo, 1, fs, prm, Fy);
             const int src_lane = x - it.x0;
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
@@ -227,7 +246,7 @@ __device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& pr
         }
         if (has_cell) {
             double Fxn[5];
-            face_flux<T, 0, VAR, STENCIL, RIEMANN>(W, U, rb + sx, vo, sx, fs, prm, Fxn);
+            face_flux<T, 0, VAR, STENCIL, RIEMANN, FLIM>(W, U, rb + sx, vo, sx, fs, prm, Fxn);
             double u[5], w[5];
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
@@ -267,7 +286,7 @@ __device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& pr
     }
 }
 
-template <class T, int VAR, int STENCIL, int RIEMANN>
+template <class T, int VAR, int STENCIL, int RIEMANN, bool FLIM>
 // Three waves per SIMD (at most 168 VGPRs) for the variants that fit without spilling; the
 // dense 5x5 projections of CHAR-CONSERVATIVE need more registers and keep two.
 __global__ __launch_bounds__(kBatchBlock)
@@ -278,7 +297,7 @@ void batch2d_stage_kernel(BatchArgs<T> a, Params prm) {
     const Item2D it = load_item(a.items + idx);
     BatchArgs<T> aa = a;
     if (a.part) aa.part = a.part + idx;
-    run_item<T, VAR, STENCIL, RIEMANN>(aa, prm, it);
+    run_item<T, VAR, STENCIL, RIEMANN, FLIM>(aa, prm, it);
 }
 
 // Per-case CFL time step from the per-item partials (time_step_size.py:109,154-155).
@@ -327,8 +346,13 @@ template <class T, int VAR, int STENCIL, int RIEMANN>
 static void launch_batch(const BatchArgs<T>& a, const Params& prm, hipStream_t st) {
     const int waves_per_block = kBatchBlock / 64;
     const int blocks = (a.nitems + waves_per_block - 1) / waves_per_block;
-    hipLaunchKernelGGL((batch2d_stage_kernel<T, VAR, STENCIL, RIEMANN>), dim3(blocks), dim3(kBatchBlock), 0, st, a,
-                       prm);
+    // the positivity flux limiter is a compile-time variant, so the plain kernel keeps its registers
+    if (prm.flux_limiter)
+        hipLaunchKernelGGL((batch2d_stage_kernel<T, VAR, STENCIL, RIEMANN, true>), dim3(blocks), dim3(kBatchBlock), 0,
+                           st, a, prm);
+    else
+        hipLaunchKernelGGL((batch2d_stage_kernel<T, VAR, STENCIL, RIEMANN, false>), dim3(blocks), dim3(kBatchBlock),
+                           0, st, a, prm);
 }
 
 template <class T, int VAR, int STENCIL>

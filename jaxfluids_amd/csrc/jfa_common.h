@@ -230,6 +230,12 @@ static inline int check_scheme(const jfa_scheme* s) {
         (!(s->limiter_eps_density > 0.0) || !(s->limiter_eps_density < INFINITY) ||
          !(s->limiter_eps_pressure > 0.0) || !(s->limiter_eps_pressure < INFINITY)))
         return JFA_EINVAL;
+    if (s->flux_limiter < JFA_FLUX_LIMITER_NONE || s->flux_limiter > JFA_FLUX_LIMITER_NASA) return JFA_EUNSUPPORTED;
+    if (s->flux_limiter &&
+        ((s->flux_partition != JFA_PARTITION_UNIFORM && s->flux_partition != JFA_PARTITION_CELLSIZE) ||
+         !(s->flux_limiter_eps_density > 0.0) || !(s->flux_limiter_eps_density < INFINITY) ||
+         !(s->flux_limiter_eps_pressure > 0.0) || !(s->flux_limiter_eps_pressure < INFINITY)))
+        return JFA_EINVAL;
     if (!(s->gamma > 1.0)) return JFA_EINVAL;
     return JFA_OK;
 }
@@ -276,7 +282,34 @@ static inline Params make_params(const jfa_scheme* s) {
     p.limit_velocity = s->limit_velocity ? 1 : 0;
     p.lim_eps_rho = s->limiter_eps_density;
     p.lim_eps_p = s->limiter_eps_pressure;
+    p.flux_limiter = s->flux_limiter;
+    p.flux_partition = s->flux_partition;
+    p.flim_eps_rho = s->flux_limiter_eps_density;
+    p.flim_eps_p = s->flux_limiter_eps_pressure;
+    p.flim_fac[0] = p.flim_fac[1] = p.flim_fac[2] = 0.0;
+    p.dt_ptr = nullptr;
+    p.dt_val = 0.0;
     return p;
+}
+
+// (1/dx_axis) sigma of the flux limiter's partition (limiter_flux.py:696-731) for a grid:
+// sigma = the number of active axes (UNIFORM) or sum_active(1/dx) / (1/dx_axis) (CELLSIZE).
+static inline void set_flux_partition(Params& p, const jfa_scheme* s, const double one_dx[3], const int active[3]) {
+    int dim = 0;
+    double sum = 0.0;
+    for (int a = 0; a < 3; ++a)
+        if (active[a]) {
+            ++dim;
+            sum += one_dx[a];
+        }
+    for (int a = 0; a < 3; ++a) {
+        if (!active[a]) {
+            p.flim_fac[a] = 0.0;
+            continue;
+        }
+        const double one_sigma = s->flux_partition == JFA_PARTITION_CELLSIZE ? sum / one_dx[a] : (double)dim;
+        p.flim_fac[a] = one_dx[a] * one_sigma;
+    }
 }
 
 // POWER or SUTHERLAND law with finite constants, ref >= 0, T_ref > 0 (and T_ref + C > 0).
@@ -320,6 +353,11 @@ static inline Params gas_params(double gamma) {
     set_transport(p, nullptr, gamma);
     p.limiter = p.limit_velocity = 0;
     p.lim_eps_rho = p.lim_eps_p = 0.0;
+    p.flux_limiter = p.flux_partition = 0;
+    p.flim_eps_rho = p.flim_eps_p = 0.0;
+    p.flim_fac[0] = p.flim_fac[1] = p.flim_fac[2] = 0.0;
+    p.dt_ptr = nullptr;
+    p.dt_val = 0.0;
     return p;
 }
 
@@ -332,9 +370,9 @@ int fused_stage_f64(const void* W_in, const void* U_in, const void* U_n, void* W
 int fused_stage_f32(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out, const Geo& G,
                     const jfa_scheme* s, const BC6& bc, int stage, const double* dt_dev, double dt_host, double cfl,
                     double eps, double* dt_next, double* R, double* part, hipStream_t st);
-int fused_rhs_f64(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double* R, void* rhs,
+int fused_rhs_f64(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double dt, double* R, void* rhs,
                   hipStream_t st);
-int fused_rhs_f32(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double* R, void* rhs,
+int fused_rhs_f32(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double dt, double* R, void* rhs,
                   hipStream_t st);
 // number of per-block CFL partials of the inner sweep
 int64_t fused_partials(const Geo& G);

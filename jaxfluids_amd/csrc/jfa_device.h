@@ -62,6 +62,14 @@ struct Params {
     int limit_velocity;  // replace all five primitives, not only density and pressure
     double lim_eps_rho;
     double lim_eps_p;
+    // positivity flux limiter (jfa_scheme.flux_limiter): JFA_FLUX_LIMITER_SIMPLE / _NASA, 0 off
+    int flux_limiter;
+    int flux_partition;     // JFA_PARTITION_*
+    double flim_eps_rho;
+    double flim_eps_p;
+    double flim_fac[3];     // (1/dx_axis) sigma of the flux partition, per axis
+    const double* dt_ptr;   // the step's time step in device memory, or null: dt_val
+    double dt_val;
 };
 
 // ---------------------------------------------------------------- refined hardware estimates
@@ -405,6 +413,64 @@ __device__ __forceinline__ void riemann(const double WL[5], const double WR[5], 
         hllc<AXIS>(WL, WR, UL, UR, prm, F);
     else
         rusanov<AXIS>(WL, WR, UL, UR, prm, F);
+}
+
+// Positivity flux limiter of one face (limiter_flux.py:147-316,452-640,733-774; SIMPLE / NASA,
+// single phase).  wl, wr, ul, ur: primitives and conservatives of the face's left and right
+// cells; lam = dt (1/dx) sigma.  The pseudo-updated states are U_minus = ur + 2 lam (F - Fs_r)
+// and U_plus = ul - 2 lam (F - Fs_l), Fs the cells' physical flux (NASA) or 0 (SIMPLE); where the
+// smaller density is below eps the flux becomes the first-order one (HLLC with Einfeldt speeds
+// on the cells' primitives, the reference's WENO1 Godunov flux, :57-98), then, with the updated
+// flux, likewise where the smaller pressure is.  Returns the number of switches.
+template <int AXIS>
+__device__ __forceinline__ int limit_flux(double F[5], const double wl[5], const double wr[5], const double ul[5],
+                                          const double ur[5], double lam, const Params& prm) {
+    double fsl[5], fsr[5];
+    if (prm.flux_limiter == JFA_FLUX_LIMITER_NASA) {
+        physical_flux<AXIS>(wl, ul, fsl);
+        physical_flux<AXIS>(wr, ur, fsr);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) fsl[i] = fsr[i] = 0.0;
+    }
+    const double l2 = 2.0 * lam;
+    int n = 0;
+    bool pos = false;
+    double Fp[5];
+    auto first_order = [&]() {
+        Params pe = prm;
+        pe.signal_speed = JFA_SS_EINFELDT;
+        double cl[5], cr[5];
+        cons_from_prim(wl, pe, cl);
+        cons_from_prim(wr, pe, cr);
+        riemann<AXIS, JFA_HLLC>(wl, wr, cl, cr, pe, Fp);
+        pos = true;
+    };
+    // density of the pseudo-updated states (:570-575)
+    const double rm = ur[0] + l2 * (F[0] - fsr[0]);
+    const double rp = ul[0] - l2 * (F[0] - fsl[0]);
+    if (fmin(rm, rp) < prm.flim_eps_rho) {
+        first_order();
+#pragma unroll
+        for (int i = 0; i < 5; ++i) F[i] = Fp[i];
+        ++n;
+    }
+    // pressure, with the flux after the density check (:619-627)
+    double um[5], up[5], wm[5], wp[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        um[i] = ur[i] + l2 * (F[i] - fsr[i]);
+        up[i] = ul[i] - l2 * (F[i] - fsl[i]);
+    }
+    prim_from_cons(um, prm, wm);
+    prim_from_cons(up, prm, wp);
+    if (fmin(wm[4], wp[4]) < prm.flim_eps_p) {
+        if (!pos) first_order();
+#pragma unroll
+        for (int i = 0; i < 5; ++i) F[i] = Fp[i];
+        ++n;
+    }
+    return n;
 }
 
 // ---------------------------------------------------------------- reconstruction

@@ -28,9 +28,10 @@ constexpr int kMarchBlockMax = 256;
 
 // Face flux from a 6-cell register window (cells k-3 relative to the cell right of the face).
 // CHAR-CONSERVATIVE also needs the primitives of the two cells adjacent to the face.
-template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN>
-__device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* __restrict__ W, int64_t b_right,
-                                            int64_t sa, int64_t fs, const Params& prm, double F[5]) {
+template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN, bool FLIM>
+__device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* __restrict__ W,
+                                            const T* __restrict__ U, int64_t b_right, int64_t sa, int64_t fs,
+                                            const Params& prm, double F[5]) {
     double wl[5], wr[5];
     if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
 #pragma unroll
@@ -51,11 +52,23 @@ __device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* 
         limit_side<VAR>(c, prm, WR, UR);
     }
     riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, F);
+    if constexpr (FLIM) {  // positivity flux limiter, with the step's dt (space_solver.py:530-541)
+        double wl_[5], wr_[5], ul_[5], ur_[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            wl_[i] = ld(W, i * fs + b_right - sa);
+            wr_[i] = ld(W, i * fs + b_right);
+            ul_[i] = ld(U, i * fs + b_right - sa);
+            ur_[i] = ld(U, i * fs + b_right);
+        }
+        const double dt = prm.dt_ptr ? *prm.dt_ptr : prm.dt_val;
+        limit_flux<AXIS>(F, wl_, wr_, ul_, ur_, dt * prm.flim_fac[AXIS], prm);
+    }
 }
 
 // ------------------------------------------------------------------ MARCH kernel
 // grid: x over IN (lanes), y over the third axis O, z over chunks of L cells along A.
-template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS>
+template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
 __global__ __launch_bounds__(kMarchBlockMax) void march_kernel(const T* __restrict__ W, const T* __restrict__ U,
                                                                Geo G, Params prm, int L, int accumulate,
                                                                double* __restrict__ R) {
@@ -78,7 +91,7 @@ __global__ __launch_bounds__(kMarchBlockMax) void march_kernel(const T* __restri
 #pragma unroll
         for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bcol + (int64_t)(a0 - 3 + k) * sa);
     double Fp[5];
-    window_flux<T, A, VAR, STENCIL, RIEMANN>(win, W, bcol + (int64_t)a0 * sa, sa, fs, prm, Fp);
+    window_flux<T, A, VAR, STENCIL, RIEMANN, FLIM>(win, W, U, bcol + (int64_t)a0 * sa, sa, fs, prm, Fp);
     int cr[3];  // right cell of the current face (dissipative fluxes)
     cr[A] = a0;
     cr[O] = io;
@@ -109,7 +122,7 @@ __global__ __launch_bounds__(kMarchBlockMax) void march_kernel(const T* __restri
             for (int i = 0; i < 5; ++i) nxt[i] = ld(src, i * fs + bcol + (int64_t)(a + 4) * sa);
         }
         double Fn[5];
-        window_flux<T, A, VAR, STENCIL, RIEMANN>(win, W, bcol + (int64_t)(a + 1) * sa, sa, fs, prm, Fn);
+        window_flux<T, A, VAR, STENCIL, RIEMANN, FLIM>(win, W, U, bcol + (int64_t)(a + 1) * sa, sa, fs, prm, Fn);
         if constexpr (DISS) {
             cr[A] = a + 1;
             dissipative_face<T, A>(W, G, prm, cr, Fn);
@@ -179,7 +192,7 @@ __device__ __forceinline__ void block_max_part3(double* part, double v0, double 
     }
 }
 
-template <class T, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS>
+template <class T, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
 __global__ __launch_bounds__(kInnerBlock) void inner_kernel(const T* __restrict__ W, const T* __restrict__ U,
                                                             Geo G, Params prm, InnerArgs<T> args) {
     constexpr int P = IN == 0 ? 1 : 0;
@@ -208,7 +221,7 @@ __global__ __launch_bounds__(kInnerBlock) void inner_kernel(const T* __restrict_
         for (int k = 0; k < 6; ++k)
 #pragma unroll
             for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bf + (k - 3));
-        window_flux<T, IN, VAR, STENCIL, RIEMANN>(win, W, bf, 1, fs, prm, F);
+        window_flux<T, IN, VAR, STENCIL, RIEMANN, FLIM>(win, W, U, bf, 1, fs, prm, F);
         if constexpr (DISS) {
             int cr[3];
             cr[IN] = (int)f;
@@ -344,7 +357,7 @@ static inline int64_t device_wave_slots(const void* kernel, int block) {
     return (int64_t)num_cu * blocks * ((block + 63) / 64);
 }
 
-template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS>
+template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
 static void launch_march(const void* W, const void* U, const Geo& G, const Params& prm, int accumulate, double* R,
                          hipStream_t st) {
     constexpr int O = 3 - A - IN;
@@ -352,51 +365,59 @@ static void launch_march(const void* W, const void* U, const Geo& G, const Param
     if (bx > kMarchBlockMax) bx = kMarchBlockMax;
     static int64_t slots[kMarchBlockMax / 64 + 1] = {0};
     int64_t& sl = slots[bx / 64];
-    if (sl == 0) sl = device_wave_slots((const void*)march_kernel<T, A, IN, VAR, STENCIL, RIEMANN, DISS>, bx);
+    if (sl == 0) sl = device_wave_slots((const void*)march_kernel<T, A, IN, VAR, STENCIL, RIEMANN, DISS, FLIM>, bx);
     const int L = march_chunk(G, A, IN, sl);
     dim3 grid((unsigned)((G.n[IN] + bx - 1) / bx), (unsigned)G.n[O], (unsigned)((G.n[A] + L - 1) / L));
-    hipLaunchKernelGGL((march_kernel<T, A, IN, VAR, STENCIL, RIEMANN, DISS>), grid, dim3(bx), 0, st,
+    hipLaunchKernelGGL((march_kernel<T, A, IN, VAR, STENCIL, RIEMANN, DISS, FLIM>), grid, dim3(bx), 0, st,
                        static_cast<const T*>(W), static_cast<const T*>(U), G, prm, L, accumulate, R);
 }
 
-template <class T, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS>
+template <class T, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
 static void launch_inner(const void* W, const void* U, const Geo& G, const Params& prm, const InnerArgs<T>& a,
                          hipStream_t st) {
     dim3 grid((unsigned)inner_blocks(G));
-    hipLaunchKernelGGL((inner_kernel<T, IN, VAR, STENCIL, RIEMANN, DISS>), grid, dim3(kInnerBlock), 0, st,
+    hipLaunchKernelGGL((inner_kernel<T, IN, VAR, STENCIL, RIEMANN, DISS, FLIM>), grid, dim3(kInnerBlock), 0, st,
                        static_cast<const T*>(W), static_cast<const T*>(U), G, prm, a);
 }
 
 // All sweeps of one stage (or of one RHS evaluation) for fixed scheme templates.
-template <class T, int VAR, int STENCIL, int RIEMANN, bool DISS>
+template <class T, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
 static void sweeps(const void* W, const void* U, const Geo& G, const Params& prm, double* R, InnerArgs<T> ia,
                    hipStream_t st) {
     const int IN = innermost_axis(G);
     int acc = 0;
     for (int A = 0; A < IN; ++A) {
         if (!G.active[A]) continue;
-        if (A == 0 && IN == 1) launch_march<T, 0, 1, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, acc, R, st);
-        else if (A == 0) launch_march<T, 0, 2, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, acc, R, st);
-        else launch_march<T, 1, 2, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, acc, R, st);
+        if (A == 0 && IN == 1) launch_march<T, 0, 1, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, acc, R, st);
+        else if (A == 0) launch_march<T, 0, 2, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, acc, R, st);
+        else launch_march<T, 1, 2, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, acc, R, st);
         acc = 1;
     }
     ia.R = R;
     ia.accumulate = acc;
-    if (IN == 0) launch_inner<T, 0, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, ia, st);
-    else if (IN == 1) launch_inner<T, 1, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, ia, st);
-    else launch_inner<T, 2, VAR, STENCIL, RIEMANN, DISS>(W, U, G, prm, ia, st);
+    if (IN == 0) launch_inner<T, 0, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, ia, st);
+    else if (IN == 1) launch_inner<T, 1, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, ia, st);
+    else launch_inner<T, 2, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, ia, st);
 }
 
+template <class T, int VAR, int STENCIL, int RIEMANN>
+static void sweeps_f(const void* W, const void* U, const Geo& G, const Params& prm, double* R,
+                     const InnerArgs<T>& ia, hipStream_t st) {
+    // dissipative fluxes and the positivity flux limiter are compile-time variants, so the plain
+    // convective sweeps keep their register budget
+    if (prm.dissip) {
+        if (prm.flux_limiter) sweeps<T, VAR, STENCIL, RIEMANN, true, true>(W, U, G, prm, R, ia, st);
+        else sweeps<T, VAR, STENCIL, RIEMANN, true, false>(W, U, G, prm, R, ia, st);
+    } else {
+        if (prm.flux_limiter) sweeps<T, VAR, STENCIL, RIEMANN, false, true>(W, U, G, prm, R, ia, st);
+        else sweeps<T, VAR, STENCIL, RIEMANN, false, false>(W, U, G, prm, R, ia, st);
+    }
+}
 template <class T, int VAR, int STENCIL>
 static void sweeps_r(int riemann, const void* W, const void* U, const Geo& G, const Params& prm, double* R,
                      const InnerArgs<T>& ia, hipStream_t st) {
-    if (prm.dissip) {
-        if (riemann == JFA_HLLC) sweeps<T, VAR, STENCIL, JFA_HLLC, true>(W, U, G, prm, R, ia, st);
-        else sweeps<T, VAR, STENCIL, JFA_RUSANOV, true>(W, U, G, prm, R, ia, st);
-    } else {
-        if (riemann == JFA_HLLC) sweeps<T, VAR, STENCIL, JFA_HLLC, false>(W, U, G, prm, R, ia, st);
-        else sweeps<T, VAR, STENCIL, JFA_RUSANOV, false>(W, U, G, prm, R, ia, st);
-    }
+    if (riemann == JFA_HLLC) sweeps_f<T, VAR, STENCIL, JFA_HLLC>(W, U, G, prm, R, ia, st);
+    else sweeps_f<T, VAR, STENCIL, JFA_RUSANOV>(W, U, G, prm, R, ia, st);
 }
 template <class T, int VAR>
 static void sweeps_s(int stencil, int riemann, const void* W, const void* U, const Geo& G, const Params& prm,
@@ -407,8 +428,11 @@ static void sweeps_s(int stencil, int riemann, const void* W, const void* U, con
 
 template <class T>
 static int run_sweeps(const jfa_scheme* s, const void* W, const void* U, const Geo& G, double* R,
-                      const InnerArgs<T>& ia, hipStream_t st) {
-    const Params prm = make_params(s);
+                      const InnerArgs<T>& ia, const double* dt_dev, double dt_host, hipStream_t st) {
+    Params prm = make_params(s);
+    set_flux_partition(prm, s, G.one_dx, G.active);
+    prm.dt_ptr = dt_dev;  // the flux limiter's step dt
+    prm.dt_val = dt_host;
     switch (s->recon_var) {
         case JFA_PRIMITIVE: sweeps_s<T, JFA_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
         case JFA_CONSERVATIVE: sweeps_s<T, JFA_CONSERVATIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
@@ -436,7 +460,7 @@ int fused_stage_impl(const void* W_in, const void* U_in, const void* U_n, void* 
     ia.dt_dev = dt_dev;
     ia.dt_host = dt_host;
     ia.cfl_part = dt_next ? part : nullptr;
-    int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, st);
+    int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, dt_dev, dt_host, st);
     if (rc) return rc;
     if (dt_next) {
         hipLaunchKernelGGL(dt_from_parts_kernel<0>, dim3(1), dim3(1024), 0, st, part, (int)inner_blocks(G), G.dx_min,
@@ -447,12 +471,12 @@ int fused_stage_impl(const void* W_in, const void* U_in, const void* U_n, void* 
 }
 
 template <class T>
-int fused_rhs_impl(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double* R, void* rhs,
-                   hipStream_t st) {
+int fused_rhs_impl(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double dt, double* R,
+                   void* rhs, hipStream_t st) {
     InnerArgs<T> ia = {};
     ia.mode = 0;
     ia.rhs_out = static_cast<T*>(rhs);
-    return run_sweeps<T>(s, W, U, G, R, ia, st);
+    return run_sweeps<T>(s, W, U, G, R, ia, nullptr, dt, st);
 }
 
 }  // namespace jfa

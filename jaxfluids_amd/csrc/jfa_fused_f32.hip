@@ -10,9 +10,9 @@ int fused_stage_f32(const void* W_in, const void* U_in, const void* U_n, void* W
                                 part, st);
 }
 
-int fused_rhs_f32(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double* R, void* rhs,
+int fused_rhs_f32(const void* W, const void* U, const Geo& G, const jfa_scheme* s, double dt, double* R, void* rhs,
                   hipStream_t st) {
-    return fused_rhs_impl<float>(W, U, G, s, R, rhs, st);
+    return fused_rhs_impl<float>(W, U, G, s, dt, R, rhs, st);
 }
 
 }  // namespace jfa

@@ -390,8 +390,8 @@ size_t jfa_rhs_workspace_bytes(const jfa_grid* grid, int32_t dtype) {
     return 5 * (size_t)G.n[0] * G.n[1] * G.n[2] * sizeof(double);
 }
 
-int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme, void* rhs,
-                    void* workspace, size_t workspace_bytes, void* stream) {
+int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme, double dt,
+                    void* rhs, void* workspace, size_t workspace_bytes, void* stream) {
     Geo G;
     int rc = make_geo(grid, G);
     if (rc) return rc;
@@ -402,8 +402,9 @@ int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jf
     rc = check_transport(scheme, G, nullptr);  // dissipative fluxes: the caller's boundaries are periodic
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
-    if (scheme->dtype == JFA_F32) return fused_rhs_f32(W, U, G, scheme, (double*)workspace, rhs, st);
-    return fused_rhs_f64(W, U, G, scheme, (double*)workspace, rhs, st);
+    if (scheme->flux_limiter && !(dt > 0.0)) return JFA_EINVAL;  // the flux limiter needs the step's dt
+    if (scheme->dtype == JFA_F32) return fused_rhs_f32(W, U, G, scheme, dt, (double*)workspace, rhs, st);
+    return fused_rhs_f64(W, U, G, scheme, dt, (double*)workspace, rhs, st);
 }
 
 int jfa_rk3_stage(void* U, const void* U_n, const void* rhs, void* W, const jfa_grid* grid, const jfa_scheme* scheme,

@@ -42,6 +42,7 @@ class PositivitySetup(NamedTuple):
     is_thinc_interpolation_limiter: bool = False
     flux_limiter: Any = False
     limit_velocity: bool = False
+    flux_partition: str = "UNIFORM"
 
 
 class DissipativeFluxesSetup(NamedTuple):

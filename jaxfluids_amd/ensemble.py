@@ -40,7 +40,8 @@ class Ensemble2D:
     def __init__(self, cases: Sequence[dict], *, stencil: str = "WENO5-JS", variable: str = "PRIMITIVE",
                  riemann: str = "HLLC", signal_speed: str = "EINFELDT", frozen_state: str = "ARITHMETIC",
                  gamma: float = 1.4, cfl: float = 0.5, nh: int = 3, dtype=torch.float64, chunk: int = 32,
-                 interpolation_limiter: bool = False, limit_velocity: bool = False, device=None) -> None:
+                 interpolation_limiter: bool = False, limit_velocity: bool = False,
+                 flux_limiter: Optional[str] = None, flux_partition: str = "UNIFORM", device=None) -> None:
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
         self.dtype = dtype
         self.gamma, self.cfl, self.nh = float(gamma), float(cfl), int(nh)
@@ -49,7 +50,7 @@ class Ensemble2D:
         self.scheme = _native.make_scheme(variable, stencil, riemann, signal_speed, frozen_state,
                                           _native.dtype_code(torch.empty(0, dtype=dtype)), gamma,
                                           precision.get_spatial_stencil_eps(), None, interpolation_limiter,
-                                          limit_velocity, lim_eps)
+                                          limit_velocity, lim_eps, flux_limiter, flux_partition, lim_eps)
         self.eps = float(torch.finfo(dtype).eps)
         self.ncases = len(cases)
         self.table = (_native.JfaCase2D * self.ncases)()

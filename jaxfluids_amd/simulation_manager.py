@@ -111,7 +111,7 @@ class SimulationManager:
         init = IntegrationBuffers(EulerIntegrationBuffers(U))
         integ = init
         for stage in range(self.time_integrator.no_stages):
-            rhs, _, _ = self.space_solver.compute_rhs(U, W)
+            rhs, _, _ = self.space_solver.compute_rhs(U, W, physical_timestep_size=dt)
             W_new = torch.empty_like(W)
             W_new.copy_(W)
             integ = self.time_integrator.perform_stage_integration(

@@ -41,15 +41,20 @@ class SpaceSolver:
     def compute_rhs(self, conservatives: torch.Tensor, primitives: torch.Tensor, temperature=None,
                     physical_simulation_time: float = 0.0, physical_timestep_size: float = 0.0, *args,
                     **kwargs) -> Tuple[IntegrationBuffers, PositivityCounter, DiscretizationCounter]:
-        """RHS of the interior cells, (5, Nx, Ny, Nz): convective flux, plus the viscous and heat
-        fluxes when the material has a transport."""
+        """RHS of the interior cells, (5, Nx, Ny, Nz): convective flux (positivity-limited with
+        physical_timestep_size when the flux limiter is on), plus the viscous and heat fluxes when
+        the material has a transport."""
         _native.require_device(primitives, "primitives")
         _native.require_device(conservatives, "conservatives")
         dom = self.domain_information
         rhs = torch.empty((5, *dom.global_number_of_cells), dtype=primitives.dtype, device=primitives.device)
         ws = self._workspace(primitives)
         scheme = self.convective_flux_solver.scheme(_native.dtype_code(primitives), self.transport)
+        dt = 0.0
+        if scheme.flux_limiter:
+            dt = float(physical_timestep_size.reshape(-1)[0]) if torch.is_tensor(physical_timestep_size) \
+                else float(physical_timestep_size)
         rc = _native.load().jfa_compute_rhs(_native.ptr(primitives), _native.ptr(conservatives), dom.grid, scheme,
-                                           _native.ptr(rhs), _native.ptr(ws), ws.numel(), _native.stream_ptr())
+                                           dt, _native.ptr(rhs), _native.ptr(ws), ws.numel(), _native.stream_ptr())
         _native.check(rc, "jfa_compute_rhs")
         return (IntegrationBuffers(EulerIntegrationBuffers(rhs)), PositivityCounter(), DiscretizationCounter())

@@ -575,6 +575,9 @@ class Setup:
     interpolation_limiter: bool = False       # positivity.is_interpolation_limiter
     limit_velocity: bool = False              # positivity.limit_velocity
     limiter_eps: Tuple[float, float] = (1e-12, 1e-10)  # (density, pressure), config/precision.py:54
+    flux_limiter: Optional[str] = None        # positivity.flux_limiter: "SIMPLE" / "NASA"
+    flux_partition: str = "UNIFORM"           # positivity.flux_partition: "UNIFORM" / "CELLSIZE"
+    flux_limiter_eps: Tuple[float, float] = (1e-12, 1e-10)  # config/precision.py:55
 
 
 def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
@@ -745,12 +748,64 @@ def dissipative_flux_xi(W, dom: Domain, axis: int, s: Setup):
     return D
 
 
-def compute_rhs(W, U, dom: Domain, s: Setup):
+def positivity_flux_xi(F, W, U, dom: Domain, axis: int, s: Setup, dt: float, return_count: bool = False):
+    """PositivityLimiterFlux.compute_positivity_preserving_flux, SIMPLE / NASA, single phase
+    (solvers/positivity/limiter_flux.py:147-316,452-562,563-640,696-774).
+
+    lambda = dt (1/dx_axis) sigma with sigma = dim (UNIFORM) or sum_active(1/dx) / (1/dx_axis)
+    (CELLSIZE).  Face f lies between cells f-1 and f; the pseudo-updated states are
+    U_minus = U_f + 2 lambda (F - Fs_f) and U_plus = U_{f-1} - 2 lambda (F - Fs_{f-1}), Fs the
+    cells' own physical flux (NASA) or 0 (SIMPLE).  Where min(rho_minus, rho_plus) < eps_rho the
+    flux becomes the first-order one -- HLLC with Einfeldt speeds on the two cells' primitives
+    (a HighOrderGodunov with WENO1, :57-98) -- then, with the updated flux, likewise where
+    min(p_minus, p_plus) < eps_p.  count: faces switched, summed over the two checks."""
+    one_dx = dom.one_dx[axis]
+    if s.flux_partition == "UNIFORM":                                     # :711-712
+        one_sigma = float(len(dom.active))
+    elif s.flux_partition == "CELLSIZE":                                  # :714-715
+        one_sigma = sum(dom.one_dx[a] for a in dom.active) / one_dx
+    else:
+        raise NotImplementedError(s.flux_partition)
+    lam = dt * one_dx * one_sigma                                         # :204
+    W_L, W_R = face_LR_cells(W, dom, axis)
+    U_L, U_R = face_LR_cells(U, dom, axis)
+    F_pos = hllc(W_L, W_R, cons_from_prim(W_L, s.gamma), cons_from_prim(W_R, s.gamma), axis, s.gamma,
+                 "EINFELDT")
+    if s.flux_limiter == "NASA":                                          # :528-541
+        Fs_L, Fs_R = physical_flux_xi(W_L, U_L, axis), physical_flux_xi(W_R, U_R, axis)
+    elif s.flux_limiter == "SIMPLE":                                      # :520-527
+        Fs_L = Fs_R = 0.0
+    else:
+        raise NotImplementedError(s.flux_limiter)
+
+    def integrate(F):
+        return U_R + 2.0 * lam * (F - Fs_R), U_L - 2.0 * lam * (F - Fs_L)
+
+    count = 0
+    with np.errstate(all="ignore"):
+        U_minus, U_plus = integrate(F)
+        theta = np.minimum(U_minus[0], U_plus[0]) < s.flux_limiter_eps[0]          # :570-575
+        count += int(np.sum(theta))
+        F = np.where(theta, F_pos, F)                                              # :764
+        U_minus, U_plus = integrate(F)
+        p_minus = prim_from_cons(U_minus, s.gamma)[4]
+        p_plus = prim_from_cons(U_plus, s.gamma)[4]
+        theta = np.minimum(p_minus, p_plus) < s.flux_limiter_eps[1]                # :619-627
+        count += int(np.sum(theta))
+        F = np.where(theta, F_pos, F)
+    return (F, count) if return_count else F
+
+
+def compute_rhs(W, U, dom: Domain, s: Setup, dt: Optional[float] = None):
     """SpaceSolver.compute_rhs (space_solver.py:266,289-314): 0.0 + rhs_x + rhs_y + rhs_z;
-    with transport set, each axis' face flux carries the dissipative part (:565-587)."""
+    with the positivity flux limiter, each axis' convective flux is limited with the step's
+    time step dt (:530-541); with transport set, each axis' face flux carries the dissipative
+    part (:565-587)."""
     rhs = 0.0
     for axis in dom.active:
         F = compute_flux_xi(W, U, dom, axis, s)
+        if s.flux_limiter:
+            F = positivity_flux_xi(F, W, U, dom, axis, s, dt)
         if s.transport is not None:
             F = F + dissipative_flux_xi(W, dom, axis, s)
         rhs = rhs + flux_divergence(F, dom.one_dx[axis], axis)
@@ -856,7 +911,7 @@ def rk3_step(W, U, dt, dom: Domain, s: Setup, rhs_fn: Callable = None):
     """One TVD-RK3 step in the order of SimulationManager.do_runge_kutta_stages
     (simulation_manager.py:770-1047): rhs -> stage integration -> W = prim(U) on the
     whole buffer -> face halo update with U halo = cons(W halo)."""
-    rhs_fn = rhs_fn or (lambda W_, U_: compute_rhs(W_, U_, dom, s))
+    rhs_fn = rhs_fn or (lambda W_, U_: compute_rhs(W_, U_, dom, s, dt))
     U_n = U
     for stage in range(3):
         rhs = rhs_fn(W, U)

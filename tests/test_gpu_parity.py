@@ -70,7 +70,8 @@ def native_scheme(s, dtype=0):
     from jaxfluids_amd import _native
 
     return _native.make_scheme(s.variable, s.stencil, s.riemann, s.signal_speed, s.frozen_state, dtype, s.gamma,
-                               s.eps_stencil, None, s.interpolation_limiter, s.limit_velocity, s.limiter_eps)
+                               s.eps_stencil, None, s.interpolation_limiter, s.limit_velocity, s.limiter_eps,
+                               s.flux_limiter, s.flux_partition, s.flux_limiter_eps)
 
 
 # ------------------------------------------------------------------ reconstruct_xi
@@ -182,7 +183,7 @@ def test_compute_rhs_parity(case):
     ws = torch.empty(lib.jfa_rhs_workspace_bytes(grid, 0), dtype=torch.uint8, device="cuda")
     out = torch.empty(ref.shape, dtype=torch.float64, device="cuda")
     Wd, Ud = dev(W), dev(U)
-    rc = lib.jfa_compute_rhs(_native.ptr(Wd), _native.ptr(Ud), grid, native_scheme(s), _native.ptr(out),
+    rc = lib.jfa_compute_rhs(_native.ptr(Wd), _native.ptr(Ud), grid, native_scheme(s), 0.0, _native.ptr(out),
                              _native.ptr(ws), ws.numel(), _native.stream_ptr())
     _native.check(rc, "rhs")
     assert rel(host(out), ref) < TOL

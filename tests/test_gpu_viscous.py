@@ -63,7 +63,7 @@ def test_dissipative_rhs(name, cells):
     ws = torch.empty(lib.jfa_rhs_workspace_bytes(grid, 0), dtype=torch.uint8, device="cuda")
     out = torch.empty((5,) + dom.cells, dtype=torch.float64, device="cuda")
     Wd, Ud = dev(W), dev(U)
-    rc = lib.jfa_compute_rhs(_native.ptr(Wd), _native.ptr(Ud), grid, scheme, _native.ptr(out), _native.ptr(ws),
+    rc = lib.jfa_compute_rhs(_native.ptr(Wd), _native.ptr(Ud), grid, scheme, 0.0, _native.ptr(out), _native.ptr(ws),
                              ws.numel(), _native.stream_ptr())
     _native.check(rc, "jfa_compute_rhs")
     got = host(out)
@@ -124,7 +124,8 @@ def test_dissipative_dt_limits():
         dt = torch.empty(1, dtype=torch.float64, device="cuda")
         ws = torch.empty(32, dtype=torch.uint8, device="cuda")
         lib = _native.load()
-        rc = lib.jfa_compute_dt(_native.ptr(dev(W)), native_grid(dom), 1.4, s.cfl, o.EPS64, 0,
+        Wd = dev(W)
+        rc = lib.jfa_compute_dt(_native.ptr(Wd), native_grid(dom), 1.4, s.cfl, o.EPS64, 0,
                                 native_transport(tr), _native.ptr(dt), _native.ptr(ws), _native.stream_ptr())
         _native.check(rc, "jfa_compute_dt")
         assert abs(host(dt)[0] - ref) <= 1e-14 * ref, (tr, host(dt)[0], ref)

@@ -684,3 +684,71 @@ def test_interpolation_limiter_keeps_the_step_finite(var):
     with np.errstate(invalid="ignore"):
         assert not np.all(np.isfinite(o.rk3_step(W, U, dt, dom, s0)[0][I]))
     assert np.all(np.isfinite(o.rk3_step(W, U, dt, dom, s1)[0][I]))
+
+
+# ------------------------------------------------------------------ positivity flux limiter
+ZG2_ = {"east": "ZEROGRADIENT", "west": "ZEROGRADIENT", "north": "ZEROGRADIENT", "south": "ZEROGRADIENT"}
+
+
+def test_nasa_flux_limiter_is_exact_for_uniform_flow():
+    """Uniform supersonic flow, a time step 100x the CFL limit: every face flux equals each cell's
+    physical flux, so NASA's pseudo-updated states are the cells themselves and nothing is
+    switched, while SIMPLE's U -/+ 2 lam F go negative and every face is switched to the
+    first-order flux -- which for uniform data equals the high-order one."""
+    dom = o.Domain((12, 10, 1), ((0, 1), (0, 1), (0, 1)), nh=3)
+    W0 = np.zeros((5,) + dom.cells)
+    W0[0], W0[1], W0[2], W0[4] = 1.0, 3.0, -2.0, 1.0
+    for fl, expect_switch in (("NASA", False), ("SIMPLE", True)):
+        s = o.Setup(stencil="WENO5-JS", variable="PRIMITIVE", bcs=ZG2_, flux_limiter=fl)
+        W, U = o.create_buffers(W0, dom, s)
+        dt = 100 * o.compute_dt(W, dom, s)
+        for axis in (0, 1):
+            F = o.compute_flux_xi(W, U, dom, axis, s)
+            G, n = o.positivity_flux_xi(F, W, U, dom, axis, s, dt, return_count=True)
+            assert (n > 0) == expect_switch, (fl, axis, n)
+            assert np.allclose(G, F, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("fl", ["SIMPLE", "NASA"])
+@pytest.mark.parametrize("partition", ["UNIFORM", "CELLSIZE"])
+def test_flux_limiter_switches_to_the_first_order_flux(fl, partition):
+    """On the near-vacuum pocket at CFL 0.9 (unequal cell sizes): every face's limited flux is bit
+    for bit its high-order flux or the first-order HLLC-Einfeldt flux of its two cells; where it
+    cannot be the latter, both pseudo-updated states are admissible; the count covers every face
+    changed; and the RK3 step, negative in pressure with the interpolation limiter alone, stays
+    positive."""
+    dom, W0 = _pocket_state()
+    s = o.Setup(stencil="WENO5-JS", variable="CHAR-PRIMITIVE", bcs=ZG2_, interpolation_limiter=True,
+                flux_limiter=fl, flux_partition=partition)
+    W, U = o.create_buffers(W0, dom, s)
+    dt = 0.9 / 0.5 * o.compute_dt(W, dom, s)
+    one_sigma = 2.0 if partition == "UNIFORM" else None
+    changed = 0
+    for axis in (0, 1):
+        F = o.compute_flux_xi(W, U, dom, axis, s)
+        G, n = o.positivity_flux_xi(F, W, U, dom, axis, s, dt, return_count=True)
+        WL, WR = o.face_LR_cells(W, dom, axis)
+        UL, UR = o.face_LR_cells(U, dom, axis)
+        Fpos = o.hllc(WL, WR, o.cons_from_prim(WL, 1.4), o.cons_from_prim(WR, 1.4), axis, 1.4, "EINFELDT")
+        high, first = np.all(G == F, axis=0), np.all(G == Fpos, axis=0)
+        assert np.all(high | first)                            # one flux or the other, bit for bit
+        assert n >= int(np.sum(~high))
+        changed += int(np.sum(~high))
+        keep = ~first                                          # certainly the high-order flux
+        sig = one_sigma if one_sigma else (dom.one_dx[0] + dom.one_dx[1]) / dom.one_dx[axis]
+        lam = dt * dom.one_dx[axis] * sig
+        Fs_L = o.physical_flux_xi(WL, UL, axis) if fl == "NASA" else 0.0
+        Fs_R = o.physical_flux_xi(WR, UR, axis) if fl == "NASA" else 0.0
+        Um, Up = UR + 2 * lam * (G - Fs_R), UL - 2 * lam * (G - Fs_L)
+        assert np.all(np.minimum(Um[0], Up[0])[keep] >= s.flux_limiter_eps[0])
+        with np.errstate(all="ignore"):
+            pm, pp = o.prim_from_cons(Um, 1.4)[4], o.prim_from_cons(Up, 1.4)[4]
+        assert np.all(np.minimum(pm, pp)[keep] >= s.flux_limiter_eps[1])
+    assert changed > 0
+    I = dom.interior
+    s0 = o.Setup(stencil="WENO5-JS", variable="CHAR-PRIMITIVE", bcs=ZG2_, interpolation_limiter=True)
+    with np.errstate(all="ignore"):
+        W0s, _ = o.rk3_step(W, U, dt, dom, s0)
+    assert not np.all(np.nan_to_num(W0s[4][I], nan=-1.0) > 0)
+    W1, _ = o.rk3_step(W, U, dt, dom, s)
+    assert np.all(W1[0][I] > 0) and np.all(W1[4][I] > 0)
