@@ -273,8 +273,8 @@ static inline Params make_params(const jfa_scheme* s) {
     p.gm1 = s->gamma - 1.0;
     p.inv_gm1 = 1.0 / (s->gamma - 1.0);
     p.eps = s->eps_stencil;
-    p.eps_z = (12.0 / 13.0) * s->eps_stencil;
-    p.eps_js = (144.0 / 169.0) * s->eps_stencil;
+    p.eps_z = 4.0 * s->eps_stencil;
+    p.eps_js = 16.0 * s->eps_stencil;
     p.signal_speed = s->signal_speed;
     p.frozen = s->frozen_state;
     set_transport(p, s->transport, s->gamma);
@@ -346,8 +346,8 @@ static inline Params gas_params(double gamma) {
     p.gm1 = gamma - 1.0;
     p.inv_gm1 = 1.0 / (gamma - 1.0);
     p.eps = 1e-30;
-    p.eps_z = (12.0 / 13.0) * 1e-30;
-    p.eps_js = (144.0 / 169.0) * 1e-30;
+    p.eps_z = 4.0 * 1e-30;
+    p.eps_js = 16.0 * 1e-30;
     p.signal_speed = JFA_SS_EINFELDT;
     p.frozen = JFA_FROZEN_ARITHMETIC;
     set_transport(p, nullptr, gamma);

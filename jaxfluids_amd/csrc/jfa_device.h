@@ -45,8 +45,8 @@ struct Params {
     double gm1;        // gamma - 1
     double inv_gm1;    // 1 / (gamma - 1)
     double eps;        // WENO epsilon (config/precision.py:39,53)
-    double eps_z;      // (12/13) eps: epsilon of the scaled WENO-Z smoothness indicators
-    double eps_js;     // (12/13)^2 eps: epsilon of the scaled WENO-JS indicators (squared)
+    double eps_z;      // 4 eps: epsilon of the scaled WENO-Z smoothness indicators (4 beta)
+    double eps_js;     // 16 eps: epsilon of the scaled WENO-JS indicators (squared)
     int signal_speed;  // JFA_SS_*
     int frozen;        // JFA_FROZEN_*
     // dissipative fluxes (jfa_transport); dissip = 0: inviscid
@@ -104,88 +104,83 @@ __device__ __forceinline__ double fsqrt(double x) {
 }
 
 // ---------------------------------------------------------------- WENO5
-// weno5_base.py:34-50 smoothness / candidate polynomials; weno5_js.py:39-58; weno5_z.py:152-166.
+// weno5_base.py:34-50 smoothness / candidate polynomials; weno5_js.py:39-58; weno5_z.py:39-49.
 //
-// beta_k = 13/12 D_k^2 + 1/4 S_k^2 with D_k the second difference and S_k the one-sided
-// slope of candidate k.  Scaled by 12/13: beta'_k = D_k^2 + (k S_k)^2, k = sqrt(3/13); the
-// weights are unchanged if the epsilon is scaled alike (WENO-Z: tau5/(beta_k + eps) ->
-// (12/13)-scaled; WENO-JS: 1/(beta_k^2 + eps) -> (12/13)^2-scaled).  The linear weights
-// d = (1/10, 6/10, 3/10) are folded into the candidate coefficients.
-constexpr double kS = 0.4803844614152614;    // sqrt(3/13)
-constexpr double kS3 = 1.4411533842457842;   // 3 sqrt(3/13)
-constexpr double kS4 = -1.9215378456610457;  // -4 sqrt(3/13)
+// Written in the first differences d_j = u_{j+1} - u_j of the stencil u_imm..u_ipp (c = u_i):
+//   beta_k = 13/12 D_k^2 + 1/4 S_k^2 with the second differences D = (d1-d0, d2-d1, d3-d2) and
+//   the one-sided slopes S = (3 d1 - d0, d1 + d2, d3 - 3 d2), evaluated as
+//   4 beta_k = (13/3) D_k^2 + S_k^2; the weights are unchanged if the epsilon is scaled alike
+//   (WENO-Z: tau5/(beta_k + eps) -> 4 eps; WENO-JS: 1/(beta_k^2 + eps) -> 16 eps).
+//   Candidate k is p_k = c + delta_k, delta = ((5 d1 - 2 d0)/6, (d1 + 2 d2)/6, (4 d2 - d3)/6),
+//   so the state is c + sum_k omega_k delta_k; the linear weights (1/10, 6/10, 3/10) are folded
+//   into g_k = d_k delta_k.
+constexpr double kQ = 13.0 / 3.0;
 
-// Nonlinear combination of the candidates: b0..b2 scaled indicators, g0..g2 = d_k p_k.
+// c + sum_k omega_k delta_k from the scaled indicators b0..b2 and g0..g2.  With q_k = b_k + eps
+// (WENO-Z) the weights are omega_k ~ d_k (1 + tau5/q_k) ~ d_k e_k, e_k = P + tau5 prod_{j != k} q_j
+// and P = q0 q1 q2; with q_k = b_k^2 + eps (WENO-JS) omega_k ~ d_k / q_k ~ d_k prod_{j != k} q_j.
 template <int STENCIL>
 __device__ __forceinline__ double weno5_combine(double b0, double b1, double b2, double g0, double g1, double g2,
-                                                const double eps_z, const double eps_js) {
+                                                double c, const double eps_z, const double eps_js) {
     double e0, e1, e2;
     if constexpr (STENCIL == JFA_WENO5_JS) {
-        // alpha_k = d_k / (beta_k^2 + eps): e_k = prod_{j != k} (beta_j^2 + eps)
         const double q0 = fma(b0, b0, eps_js), q1 = fma(b1, b1, eps_js), q2 = fma(b2, b2, eps_js);
         e0 = q1 * q2;
         e1 = q0 * q2;
         e2 = q0 * q1;
     } else {
-        // alpha_k = d_k (1 + tau5/(beta_k + eps)): e_k = (beta_k + eps + tau5) prod_{j != k} (beta_j + eps)
         const double tau5 = fabs(b0 - b2);
         const double q0 = b0 + eps_z, q1 = b1 + eps_z, q2 = b2 + eps_z;
-        e0 = (q0 + tau5) * (q1 * q2);
-        e1 = (q1 + tau5) * (q0 * q2);
-        e2 = (q2 + tau5) * (q0 * q1);
+        const double p12 = q1 * q2, p02 = q0 * q2, p01 = q0 * q1;
+        const double P = p01 * q2;
+        e0 = fma(tau5, p12, P);
+        e1 = fma(tau5, p02, P);
+        e2 = fma(tau5, p01, P);
     }
     const double num = fma(e2, g2, fma(e1, g1, e0 * g0));
     const double den = fma(0.3, e2, fma(0.6, e1, 0.1 * e0));
-    return num * rcp1(den);
+    return fma(num, rcp1(den), c);
 }
 
 // One side: (u_imm, u_im, u_i, u_ip, u_ipp).
 template <int STENCIL>
 __device__ __forceinline__ double weno5(double a, double b, double c, double d, double e, const double eps_z,
                                         const double eps_js) {
-    const double t0 = a - 2.0 * b + c, t2 = b - 2.0 * c + d, t4 = c - 2.0 * d + e;
-    const double s0 = fma(kS3, c, fma(kS4, b, kS * a));
-    const double s1 = kS * (b - d);
-    const double s2 = fma(kS3, c, fma(kS4, d, kS * e));
-    const double b0 = fma(s0, s0, t0 * t0), b1 = fma(s1, s1, t2 * t2), b2 = fma(s2, s2, t4 * t4);
-    const double g0 = fma(11.0 / 60.0, c, fma(-7.0 / 60.0, b, (1.0 / 30.0) * a));
-    const double g1 = fma(0.2, d, fma(0.5, c, -0.1 * b));
-    const double g2 = fma(-0.05, e, fma(0.25, d, 0.1 * c));
-    return weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, eps_z, eps_js);
+    const double d0 = b - a, d1 = c - b, d2 = d - c, d3 = e - d;
+    const double D0 = d1 - d0, D1 = d2 - d1, D2 = d3 - d2;
+    const double S0 = fma(3.0, d1, -d0), S1 = d1 + d2, S2 = fma(-3.0, d2, d3);
+    const double b0 = fma(S0, S0, (kQ * D0) * D0), b1 = fma(S1, S1, (kQ * D1) * D1), b2 = fma(S2, S2, (kQ * D2) * D2);
+    const double g0 = fma(1.0 / 12.0, d1, (-1.0 / 30.0) * d0);
+    const double g1 = fma(0.2, d2, 0.1 * d1);
+    const double g2 = fma(0.2, d2, -0.05 * d3);
+    return weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, c, eps_z, eps_js);
 }
 
 // Left (j = 0: cells -3..1) and right (j = 1: cells 2..-2) states of one face from its
-// 6-cell window v[0..5] = cells -3..2 (weno5_base.py:30-31).  The second differences of
-// cells -1 and 0 and their squares are shared by the two sides.
+// 6-cell window v[0..5] = cells -3..2 (weno5_base.py:30-31).  The first differences, the
+// second differences of cells -1 and 0 and their scaled squares are shared by the two sides;
+// the right side's stencil is the window reversed, whose first differences are -d4..-d1.
 template <int STENCIL>
 __device__ __forceinline__ void weno5_pair(const double v[6], const double eps_z, const double eps_js, double& L,
                                            double& R) {
-    const double Dm2 = v[0] - 2.0 * v[1] + v[2];
-    const double Dm1 = v[1] - 2.0 * v[2] + v[3];
-    const double D0 = v[2] - 2.0 * v[3] + v[4];
-    const double D1 = v[3] - 2.0 * v[4] + v[5];
-    const double Qm1 = Dm1 * Dm1, Q0 = D0 * D0;
-    // left: a..e = v0..v4
-    {
-        const double s0 = fma(kS3, v[2], fma(kS4, v[1], kS * v[0]));
-        const double s1 = kS * (v[1] - v[3]);
-        const double s2 = fma(kS3, v[2], fma(kS4, v[3], kS * v[4]));
-        const double b0 = fma(s0, s0, Dm2 * Dm2), b1 = fma(s1, s1, Qm1), b2 = fma(s2, s2, Q0);
-        const double g0 = fma(11.0 / 60.0, v[2], fma(-7.0 / 60.0, v[1], (1.0 / 30.0) * v[0]));
-        const double g1 = fma(0.2, v[3], fma(0.5, v[2], -0.1 * v[1]));
-        const double g2 = fma(-0.05, v[4], fma(0.25, v[3], 0.1 * v[2]));
-        L = weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, eps_z, eps_js);
+    const double d0 = v[1] - v[0], d1 = v[2] - v[1], d2 = v[3] - v[2], d3 = v[4] - v[3], d4 = v[5] - v[4];
+    const double E0 = d1 - d0, E1 = d2 - d1, E2 = d3 - d2, E3 = d4 - d3;
+    const double Q1 = (kQ * E1) * E1, Q2 = (kQ * E2) * E2;
+    {  // left: stencil v0..v4, centre v2
+        const double S0 = fma(3.0, d1, -d0), S1 = d1 + d2, S2 = fma(-3.0, d2, d3);
+        const double b0 = fma(S0, S0, (kQ * E0) * E0), b1 = fma(S1, S1, Q1), b2 = fma(S2, S2, Q2);
+        const double g0 = fma(1.0 / 12.0, d1, (-1.0 / 30.0) * d0);
+        const double g1 = fma(0.2, d2, 0.1 * d1);
+        const double g2 = fma(0.2, d2, -0.05 * d3);
+        L = weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, v[2], eps_z, eps_js);
     }
-    // right: a..e = v5..v1
-    {
-        const double s0 = fma(kS3, v[3], fma(kS4, v[4], kS * v[5]));
-        const double s1 = kS * (v[4] - v[2]);
-        const double s2 = fma(kS3, v[3], fma(kS4, v[2], kS * v[1]));
-        const double b0 = fma(s0, s0, D1 * D1), b1 = fma(s1, s1, Q0), b2 = fma(s2, s2, Qm1);
-        const double g0 = fma(11.0 / 60.0, v[3], fma(-7.0 / 60.0, v[4], (1.0 / 30.0) * v[5]));
-        const double g1 = fma(0.2, v[2], fma(0.5, v[3], -0.1 * v[4]));
-        const double g2 = fma(-0.05, v[1], fma(0.25, v[2], 0.1 * v[3]));
-        R = weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, eps_z, eps_js);
+    {  // right: stencil v5..v1, centre v3; its differences are (-d4, -d3, -d2, -d1)
+        const double S0 = fma(-3.0, d3, d4), S1 = d3 + d2, S2 = fma(3.0, d2, -d1);
+        const double b0 = fma(S0, S0, (kQ * E3) * E3), b1 = fma(S1, S1, Q2), b2 = fma(S2, S2, Q1);
+        const double g0 = fma(-1.0 / 12.0, d3, (1.0 / 30.0) * d4);
+        const double g1 = fma(-0.2, d2, -0.1 * d3);
+        const double g2 = fma(-0.2, d2, 0.05 * d1);
+        R = weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, v[3], eps_z, eps_js);
     }
 }
 

@@ -68,10 +68,13 @@ __device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* 
 
 // ------------------------------------------------------------------ MARCH kernel
 // grid: x over IN (lanes), y over the third axis O, z over chunks of L cells along A.
+// The plain convective variants are held to 3 waves per SIMD (168 VGPRs): one register more
+// costs a third of the resident waves of this VALU-bound loop.
 template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
-__global__ __launch_bounds__(kMarchBlockMax) void march_kernel(const T* __restrict__ W, const T* __restrict__ U,
-                                                               Geo G, Params prm, int L, int accumulate,
-                                                               double* __restrict__ R) {
+__global__ __launch_bounds__(kMarchBlockMax)
+JFA_WAVES_PER_EU((DISS || FLIM || VAR == JFA_CHAR_CONSERVATIVE) ? 1 : 3) void march_kernel(
+    const T* __restrict__ W, const T* __restrict__ U, Geo G, Params prm, int L, int accumulate,
+    double* __restrict__ R) {
     constexpr int O = 3 - A - IN;
     const int iin = blockIdx.x * blockDim.x + threadIdx.x;
     if (iin >= G.n[IN]) return;  // no barriers in this kernel
@@ -155,8 +158,8 @@ struct InnerArgs {
     double* cfl_part;       // per-block partial max of the new state's wave speed, or null
 };
 
-// Block max written to part[blockIdx.x] (no atomics); every thread must call it.
-__device__ __forceinline__ void block_max_part(double* part, double v) {
+// Block max written to part[b] (no atomics); every thread must call it.
+__device__ __forceinline__ void block_max_part(double* part, double v, uint32_t b) {
     __shared__ double sm[kInnerBlock / 64];
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
@@ -164,12 +167,13 @@ __device__ __forceinline__ void block_max_part(double* part, double v) {
     if (threadIdx.x == 0) {
         double m = sm[0];
         for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sm[i]);
-        part[blockIdx.x] = m;
+        part[b] = m;
     }
 }
 
 // Block maxima of three quantities written to part[b], part[nb + b], part[2 nb + b].
-__device__ __forceinline__ void block_max_part3(double* part, double v0, double v1, double v2) {
+__device__ __forceinline__ void block_max_part3(double* part, double v0, double v1, double v2, uint32_t b,
+                                                uint32_t nb) {
     __shared__ double sm[3][kInnerBlock / 64];
     for (int o = 32; o > 0; o >>= 1) {
         v0 = fmax(v0, __shfl_xor(v0, o, 64));
@@ -183,11 +187,51 @@ __device__ __forceinline__ void block_max_part3(double* part, double v0, double 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int nb = (int)gridDim.x;
         for (int q = 0; q < 3; ++q) {
             double m = sm[q][0];
             for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sm[q][i]);
-            part[q * nb + blockIdx.x] = m;
+            part[q * nb + b] = m;
+        }
+    }
+}
+
+// Stage finish of interior cell c (buffer index b, interior index rc) from its RHS.  Mode 0
+// stores the RHS; mode 1 does U = c0 U + c1 U_n + dt m rhs (RK3.py:27-62), W = prim(U),
+// stores both, scatters the cell into the halo positions the face-halo fill copies it to
+// (U halo = cons(W halo), outer/material.py:248-250), and raises the running maxima of the
+// CFL wave speed (and, with dissipative fluxes, of the diffusivities) of the new state.
+template <class T, bool DISS>
+__device__ __forceinline__ void finish_cell(const Geo& G, const Params& prm, const InnerArgs<T>& args, const int c[3],
+                                            int64_t b, int64_t rc, int64_t ncell, const double rhs[5],
+                                            double& vmax, double& nu_max, double& al_max) {
+    const int64_t fs = G.fs;
+    if (args.mode == 0) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) args.rhs_out[i * ncell + rc] = static_cast<T>(rhs[i]);
+        return;
+    }
+    double c0, c1, m;
+    rk3_coeffs(args.stage, c0, c1, m);
+    const double ts = (args.dt_dev ? args.dt_dev[0] : args.dt_host) * m;  // RK3.py:60
+    double u[5], w[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double x = ld(args.Uin, i * fs + b);
+        if (args.stage > 0) x = c0 * x + c1 * ld(args.Un, i * fs + b);  // RK3.py:49-50
+        u[i] = x + ts * rhs[i];                                          // time_integrator.py:57
+    }
+    prim_from_cons(u, prm, w);
+    store_cell(args.Wout, args.Uout, fs, b, w, u);
+    double uh[5];
+    cons_from_prim(w, prm, uh);
+    halo_scatter(args.Wout, args.Uout, G, args.bc, c, b, w, uh, prm);
+    if (args.cfl_part) {
+        vmax = fmax(vmax, wave_sum(w, G.active, prm.gamma));
+        if constexpr (DISS) {
+            double nu, al;
+            diffusivities(w, prm, nu, al);
+            nu_max = fmax(nu_max, nu);
+            al_max = fmax(al_max, al);
         }
     }
 }
@@ -249,35 +293,12 @@ __global__ __launch_bounds__(kInnerBlock) void inner_kernel(const T* __restrict_
             const double d = odx * (F[i] - Fu[i]);
             rhs[i] = args.accumulate ? args.R[i * ncell + rc] + d : 0.0 + d;  // space_solver.py:314
         }
-        if (args.mode == 0) {
-#pragma unroll
-            for (int i = 0; i < 5; ++i) args.rhs_out[i * ncell + rc] = static_cast<T>(rhs[i]);
-        } else {
-            double c0, c1, m;
-            rk3_coeffs(args.stage, c0, c1, m);
-            const double ts = (args.dt_dev ? args.dt_dev[0] : args.dt_host) * m;  // RK3.py:60
-            double u[5], w[5];
-#pragma unroll
-            for (int i = 0; i < 5; ++i) {
-                double x = ld(args.Uin, i * fs + bf);
-                if (args.stage > 0) x = c0 * x + c1 * ld(args.Un, i * fs + bf);  // RK3.py:49-50
-                u[i] = x + ts * rhs[i];                                           // time_integrator.py:57
-            }
-            prim_from_cons(u, prm, w);
-            store_cell(args.Wout, args.Uout, fs, bf, w, u);
-            double uh[5];
-            cons_from_prim(w, prm, uh);
-            halo_scatter(args.Wout, args.Uout, G, args.bc, c, bf, w, uh, prm);
-            if (args.cfl_part) {
-                vmax = wave_sum(w, G.active, prm.gamma);
-                if constexpr (DISS) diffusivities(w, prm, nu_max, al_max);
-            }
-        }
+        finish_cell<T, DISS>(G, prm, args, c, bf, rc, ncell, rhs, vmax, nu_max, al_max);
     }
     if constexpr (DISS) {
-        if (args.cfl_part) block_max_part3(args.cfl_part, vmax, nu_max, al_max);
+        if (args.cfl_part) block_max_part3(args.cfl_part, vmax, nu_max, al_max, blockIdx.x, gridDim.x);
     } else {
-        if (args.cfl_part) block_max_part(args.cfl_part, vmax);
+        if (args.cfl_part) block_max_part(args.cfl_part, vmax, blockIdx.x);
     }
 }
 
@@ -322,14 +343,12 @@ static inline int64_t inner_blocks(const Geo& G) {
 // Chunk length L along A: every wave marches L cells (L+1 faces).  The waves of one launch
 // run in ceil(waves / slots) rounds, slots = CUs x resident waves per CU of this kernel, so
 // L is chosen to minimise rounds x (L + 1) -- short chunks cost an extra face each, a
-// part-filled last round idles the chip.
-static inline int march_chunk(const Geo& G, int A, int IN, int64_t slots) {
-    const int O = 3 - A - IN;
-    const int64_t waves_per_layer = ((int64_t)G.n[IN] + 63) / 64 * G.n[O];
+// part-filled last round idles the chip.  waves_per_layer: wavefronts of one chunk layer.
+static inline int march_chunk(int nA, int64_t waves_per_layer, int64_t slots) {
     int best_L = 4;
     double best = 1e300;
     for (int L = 4; L <= 64; ++L) {
-        const int64_t chunks = ((int64_t)G.n[A] + L - 1) / L;
+        const int64_t chunks = ((int64_t)nA + L - 1) / L;
         const int64_t waves = chunks * waves_per_layer;
         const int64_t rounds = (waves + slots - 1) / slots;
         const double cost = (double)rounds * (L + 1);
@@ -337,7 +356,7 @@ static inline int march_chunk(const Geo& G, int A, int IN, int64_t slots) {
             best = cost;
             best_L = L;
         }
-        if (L >= G.n[A]) break;
+        if (L >= nA) break;
     }
     return best_L;
 }
@@ -366,24 +385,27 @@ static void launch_march(const void* W, const void* U, const Geo& G, const Param
     static int64_t slots[kMarchBlockMax / 64 + 1] = {0};
     int64_t& sl = slots[bx / 64];
     if (sl == 0) sl = device_wave_slots((const void*)march_kernel<T, A, IN, VAR, STENCIL, RIEMANN, DISS, FLIM>, bx);
-    const int L = march_chunk(G, A, IN, sl);
+    const int L = march_chunk(G.n[A], ((int64_t)G.n[IN] + 63) / 64 * G.n[O], sl);
     dim3 grid((unsigned)((G.n[IN] + bx - 1) / bx), (unsigned)G.n[O], (unsigned)((G.n[A] + L - 1) / L));
     hipLaunchKernelGGL((march_kernel<T, A, IN, VAR, STENCIL, RIEMANN, DISS, FLIM>), grid, dim3(bx), 0, st,
                        static_cast<const T*>(W), static_cast<const T*>(U), G, prm, L, accumulate, R);
 }
 
 template <class T, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
-static void launch_inner(const void* W, const void* U, const Geo& G, const Params& prm, const InnerArgs<T>& a,
-                         hipStream_t st) {
+static int64_t launch_inner(const void* W, const void* U, const Geo& G, const Params& prm, const InnerArgs<T>& a,
+                            hipStream_t st) {
     dim3 grid((unsigned)inner_blocks(G));
     hipLaunchKernelGGL((inner_kernel<T, IN, VAR, STENCIL, RIEMANN, DISS, FLIM>), grid, dim3(kInnerBlock), 0, st,
                        static_cast<const T*>(W), static_cast<const T*>(U), G, prm, a);
+    return grid.x;
 }
 
-// All sweeps of one stage (or of one RHS evaluation) for fixed scheme templates.
+// All sweeps of one stage (or of one RHS evaluation) for fixed scheme templates: a MARCH
+// kernel per active axis below IN, then the INNER kernel.  Returns the number of blocks of the
+// INNER launch (its CFL partials per quantity).
 template <class T, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
-static void sweeps(const void* W, const void* U, const Geo& G, const Params& prm, double* R, InnerArgs<T> ia,
-                   hipStream_t st) {
+static int64_t sweeps(const void* W, const void* U, const Geo& G, const Params& prm, double* R, InnerArgs<T> ia,
+                      hipStream_t st) {
     const int IN = innermost_axis(G);
     int acc = 0;
     for (int A = 0; A < IN; ++A) {
@@ -395,50 +417,53 @@ static void sweeps(const void* W, const void* U, const Geo& G, const Params& prm
     }
     ia.R = R;
     ia.accumulate = acc;
-    if (IN == 0) launch_inner<T, 0, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, ia, st);
-    else if (IN == 1) launch_inner<T, 1, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, ia, st);
-    else launch_inner<T, 2, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, ia, st);
+    if (IN == 0) return launch_inner<T, 0, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, ia, st);
+    if (IN == 1) return launch_inner<T, 1, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, ia, st);
+    return launch_inner<T, 2, VAR, STENCIL, RIEMANN, DISS, FLIM>(W, U, G, prm, ia, st);
 }
 
 template <class T, int VAR, int STENCIL, int RIEMANN>
-static void sweeps_f(const void* W, const void* U, const Geo& G, const Params& prm, double* R,
-                     const InnerArgs<T>& ia, hipStream_t st) {
+static int64_t sweeps_f(const void* W, const void* U, const Geo& G, const Params& prm, double* R,
+                        const InnerArgs<T>& ia, hipStream_t st) {
     // dissipative fluxes and the positivity flux limiter are compile-time variants, so the plain
     // convective sweeps keep their register budget
     if (prm.dissip) {
-        if (prm.flux_limiter) sweeps<T, VAR, STENCIL, RIEMANN, true, true>(W, U, G, prm, R, ia, st);
-        else sweeps<T, VAR, STENCIL, RIEMANN, true, false>(W, U, G, prm, R, ia, st);
-    } else {
-        if (prm.flux_limiter) sweeps<T, VAR, STENCIL, RIEMANN, false, true>(W, U, G, prm, R, ia, st);
-        else sweeps<T, VAR, STENCIL, RIEMANN, false, false>(W, U, G, prm, R, ia, st);
+        if (prm.flux_limiter) return sweeps<T, VAR, STENCIL, RIEMANN, true, true>(W, U, G, prm, R, ia, st);
+        return sweeps<T, VAR, STENCIL, RIEMANN, true, false>(W, U, G, prm, R, ia, st);
     }
+    if (prm.flux_limiter) return sweeps<T, VAR, STENCIL, RIEMANN, false, true>(W, U, G, prm, R, ia, st);
+    return sweeps<T, VAR, STENCIL, RIEMANN, false, false>(W, U, G, prm, R, ia, st);
 }
 template <class T, int VAR, int STENCIL>
-static void sweeps_r(int riemann, const void* W, const void* U, const Geo& G, const Params& prm, double* R,
-                     const InnerArgs<T>& ia, hipStream_t st) {
-    if (riemann == JFA_HLLC) sweeps_f<T, VAR, STENCIL, JFA_HLLC>(W, U, G, prm, R, ia, st);
-    else sweeps_f<T, VAR, STENCIL, JFA_RUSANOV>(W, U, G, prm, R, ia, st);
+static int64_t sweeps_r(int riemann, const void* W, const void* U, const Geo& G, const Params& prm, double* R,
+                        const InnerArgs<T>& ia, hipStream_t st) {
+    if (riemann == JFA_HLLC) return sweeps_f<T, VAR, STENCIL, JFA_HLLC>(W, U, G, prm, R, ia, st);
+    return sweeps_f<T, VAR, STENCIL, JFA_RUSANOV>(W, U, G, prm, R, ia, st);
 }
 template <class T, int VAR>
-static void sweeps_s(int stencil, int riemann, const void* W, const void* U, const Geo& G, const Params& prm,
-                     double* R, const InnerArgs<T>& ia, hipStream_t st) {
-    if (stencil == JFA_WENO5_JS) sweeps_r<T, VAR, JFA_WENO5_JS>(riemann, W, U, G, prm, R, ia, st);
-    else sweeps_r<T, VAR, JFA_WENO5_Z>(riemann, W, U, G, prm, R, ia, st);
+static int64_t sweeps_s(int stencil, int riemann, const void* W, const void* U, const Geo& G, const Params& prm,
+                        double* R, const InnerArgs<T>& ia, hipStream_t st) {
+    if (stencil == JFA_WENO5_JS) return sweeps_r<T, VAR, JFA_WENO5_JS>(riemann, W, U, G, prm, R, ia, st);
+    return sweeps_r<T, VAR, JFA_WENO5_Z>(riemann, W, U, G, prm, R, ia, st);
 }
 
+// Launches the sweeps; *nparts receives the CFL partials per quantity of the finishing launch.
 template <class T>
 static int run_sweeps(const jfa_scheme* s, const void* W, const void* U, const Geo& G, double* R,
-                      const InnerArgs<T>& ia, const double* dt_dev, double dt_host, hipStream_t st) {
+                      const InnerArgs<T>& ia, const double* dt_dev, double dt_host, hipStream_t st,
+                      int64_t* nparts) {
     Params prm = make_params(s);
     set_flux_partition(prm, s, G.one_dx, G.active);
     prm.dt_ptr = dt_dev;  // the flux limiter's step dt
     prm.dt_val = dt_host;
+    int64_t nb = 0;
     switch (s->recon_var) {
-        case JFA_PRIMITIVE: sweeps_s<T, JFA_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
-        case JFA_CONSERVATIVE: sweeps_s<T, JFA_CONSERVATIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
-        case JFA_CHAR_PRIMITIVE: sweeps_s<T, JFA_CHAR_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
-        default: sweeps_s<T, JFA_CHAR_CONSERVATIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
+        case JFA_PRIMITIVE: nb = sweeps_s<T, JFA_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
+        case JFA_CONSERVATIVE: nb = sweeps_s<T, JFA_CONSERVATIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
+        case JFA_CHAR_PRIMITIVE: nb = sweeps_s<T, JFA_CHAR_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
+        default: nb = sweeps_s<T, JFA_CHAR_CONSERVATIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
     }
+    if (nparts) *nparts = nb;
     return check_launch();
 }
 
@@ -460,10 +485,11 @@ int fused_stage_impl(const void* W_in, const void* U_in, const void* U_n, void* 
     ia.dt_dev = dt_dev;
     ia.dt_host = dt_host;
     ia.cfl_part = dt_next ? part : nullptr;
-    int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, dt_dev, dt_host, st);
+    int64_t nparts = 0;
+    int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, dt_dev, dt_host, st, &nparts);
     if (rc) return rc;
     if (dt_next) {
-        hipLaunchKernelGGL(dt_from_parts_kernel<0>, dim3(1), dim3(1024), 0, st, part, (int)inner_blocks(G), G.dx_min,
+        hipLaunchKernelGGL(dt_from_parts_kernel<0>, dim3(1), dim3(1024), 0, st, part, (int)nparts, G.dx_min,
                            cfl, eps, make_params(s).dissip, dt_next);
         rc = check_launch();
     }
@@ -476,7 +502,7 @@ int fused_rhs_impl(const void* W, const void* U, const Geo& G, const jfa_scheme*
     InnerArgs<T> ia = {};
     ia.mode = 0;
     ia.rhs_out = static_cast<T*>(rhs);
-    return run_sweeps<T>(s, W, U, G, R, ia, nullptr, dt, st);
+    return run_sweeps<T>(s, W, U, G, R, ia, nullptr, dt, st, nullptr);
 }
 
 }  // namespace jfa

@@ -1,6 +1,6 @@
-"""Builds tests/emu/_build/libjfa_emu.so: the library's batch-stage and adjoint sources compiled for the
-host with g++ against the emulation header in tests/emu/hip/ (test infrastructure only; the
-product library is built by jaxfluids_amd/build.py with hipcc for gfx950)."""
+"""Builds tests/emu/_build/libjfa_emu.so: the library's sources compiled for the host with g++
+against the emulation header in tests/emu/hip/ (test infrastructure only; the product library is
+built by jaxfluids_amd/build.py with hipcc for gfx950).  The result exports the library's C ABI."""
 import os
 import shutil
 import subprocess
@@ -9,8 +9,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "jaxfluids_amd", "csrc")
 OUT = os.path.join(HERE, "_build", "libjfa_emu.so")
-SOURCES = [os.path.join(CSRC, f) for f in ("jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip")]
-DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("jfa_batch.h", "jfa_common.h", "jfa_device.h", "jfa_adjoint.h", "jfa_dissipative.h")] + [
+SOURCES = [os.path.join(CSRC, f) for f in ("jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip", "jfa_kernels.hip",
+                                            "jfa_fused_f64.hip", "jfa_fused_f32.hip")]
+DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("jfa_batch.h", "jfa_common.h", "jfa_device.h", "jfa_adjoint.h",
+                                                  "jfa_dissipative.h", "jfa_fused.h")] + [
     os.path.join(HERE, "emu_runtime.cpp"), os.path.join(HERE, "hip", "hip_runtime.h"), __file__]
 
 
@@ -40,12 +42,17 @@ def build(force: bool = False) -> str:
         flags = ["-std=c++17", "-O2", "-g", "-fPIC", "-pthread", "-Wno-unknown-pragmas", "-Wno-unused-result",
                  f"-I{os.path.join(HERE)}", f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}"]
         with tempfile.TemporaryDirectory(dir=os.path.dirname(OUT)) as tmp:
-            objs = []
-            for src in SOURCES + [os.path.join(HERE, "emu_runtime.cpp")]:
+            from concurrent.futures import ThreadPoolExecutor
+
+            def compile_one(src):
                 obj = os.path.join(tmp, os.path.basename(src) + ".o")
                 lang = ["-x", "c++"] if src.endswith(".hip") else []
                 subprocess.run([cxx, *flags, *lang, "-c", src, "-o", obj], check=True)
-                objs.append(obj)
+                return obj
+
+            srcs = SOURCES + [os.path.join(HERE, "emu_runtime.cpp")]
+            with ThreadPoolExecutor(max_workers=min(len(srcs), 4)) as ex:
+                objs = list(ex.map(compile_one, srcs))
             lib = os.path.join(tmp, "lib.so")
             subprocess.run([cxx, "-shared", "-pthread", "-o", lib, *objs], check=True)
             os.replace(lib, OUT)

@@ -14,10 +14,6 @@
 thread_local emu_uint3 threadIdx, blockIdx;
 thread_local dim3 blockDim, gridDim;
 
-namespace jfa {
-thread_local char g_last_error[256] = "";
-}
-
 namespace {
 
 // Barrier over the live threads of a group; a thread that leaves the kernel drops out.

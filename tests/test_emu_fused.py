@@ -1,0 +1,161 @@
+"""CPU checks of the fused RK3 stage (jfa_fused.h: the MARCH and INNER kernels) through the
+host build of the library sources (tests/emu): one RK3 step of three jfa_rk3_stage_fused calls on
+host buffers, with the next CFL time step, against the oracle's step (FP64, relative L-inf
+<= 1e-12).
+
+Every state, workspace and time-step load and store of the kernels is checked against the
+registered buffers, so an out-of-bounds index shows up as a violation here instead of a fault on
+the GPU.  The grids cover every kernel combination: 3-D (MARCH x and y, INNER z), the three
+2-D orientations (one MARCH, INNER) and 1-D (INNER alone), with odd extents so the 63-face
+wavefront tiling of the INNER kernel crosses rows.
+"""
+import ctypes
+import dataclasses
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import jxf_oracle as o
+from test_gpu_parity import random_state, rel
+from test_oracle_pins import _pocket_state
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "emu"))
+
+TOL = 1e-12
+FACES = ("east", "west", "north", "south", "top", "bottom")
+SCHEMES = {
+    "z-charprim-hllc": dict(stencil="WENO5-Z", variable="CHAR-PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT",
+                            frozen_state="ROE"),
+    "js-prim-rusanov": dict(stencil="WENO5-JS", variable="PRIMITIVE", riemann="RUSANOV", signal_speed="EINFELDT",
+                            frozen_state="ARITHMETIC"),
+    "z-charcons-hllc": dict(stencil="WENO5-Z", variable="CHAR-CONSERVATIVE", riemann="HLLC", signal_speed="DAVIS",
+                            frozen_state="ARITHMETIC"),
+    "js-cons-hllc": dict(stencil="WENO5-JS", variable="CONSERVATIVE", riemann="HLLC", signal_speed="TORO",
+                         frozen_state="ROE"),
+}
+
+
+@pytest.fixture(scope="module")
+def emu():
+    import build as emu_build
+    from jaxfluids_amd import _native
+
+    lib = ctypes.CDLL(emu_build.build())
+    for name in ("jfa_stage_workspace_bytes", "jfa_rk3_stage_fused"):
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = _native.SIGNATURES[name]
+    lib.emu_add_range.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+    lib.emu_violations.restype = ctypes.c_longlong
+    lib.emu_accesses.restype = ctypes.c_longlong
+    lib.emu_first_violation.restype = ctypes.c_ulonglong
+    return lib
+
+
+def fused_step(emu, dom, s, W, U, dt, transport=None):
+    """Three fused stages as SimulationManager.do_runge_kutta_stages runs them (the last one
+    writes into the step's input buffers and the next dt); returns (W, U, dt_next)."""
+    from jaxfluids_amd import _native
+
+    grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
+    tr = None if transport is None else _native.make_transport(**dataclasses.asdict(transport))
+    scheme = _native.make_scheme(s.variable, s.stencil, s.riemann, s.signal_speed, s.frozen_state, 0, s.gamma,
+                                 s.eps_stencil, tr, s.interpolation_limiter, s.limit_velocity, s.limiter_eps,
+                                 s.flux_limiter, s.flux_partition, s.flux_limiter_eps)
+    bcs = _native.bc_array({f: (s.bcs[f] if o.FACE_AXIS[f] in dom.active else "PERIODIC") for f in FACES})
+    nbytes = emu.jfa_stage_workspace_bytes(grid, scheme)
+    assert nbytes > 0
+    Wa, Ua = np.ascontiguousarray(W.copy()), np.ascontiguousarray(U.copy())
+    Wb, Ub, Wc, Uc = (np.full_like(Wa, np.nan) for _ in range(4))
+    ws = np.zeros(nbytes, dtype=np.uint8)
+    dta = np.array([dt])
+    dtn = np.array([np.nan])
+    emu.emu_clear_ranges()
+    for a in (Wa, Ua, Wb, Ub, Wc, Uc, ws, dta, dtn):
+        emu.emu_add_range(a.ctypes.data, a.nbytes)
+    P = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+    for k, (wi, ui, wo, uo) in enumerate([(Wa, Ua, Wb, Ub), (Wb, Ub, Wc, Uc), (Wc, Uc, Wa, Ua)]):
+        rc = emu.jfa_rk3_stage_fused(P(wi), P(ui), P(Ua if k else None), P(wo), P(uo), grid, scheme, bcs, k, P(dta),
+                                     0.0, s.cfl, o.EPS64, P(dtn if k == 2 else None), P(ws), nbytes, None)
+        assert rc == 0, rc
+    assert emu.emu_violations() == 0, hex(emu.emu_first_violation())
+    assert emu.emu_accesses() > 0
+    return Wa, Ua, float(dtn[0])
+
+
+def check_step(emu, dom, s, W0, transport=None, dt_scale=1.0):
+    W, U = o.create_buffers(W0, dom, s)
+    dt = dt_scale * o.compute_dt(W, dom, s)
+    Wr, Ur = o.rk3_step(W, U, dt, dom, s)
+    I = dom.interior
+    assert np.all(np.isfinite(Wr[I])) and np.all(Wr[4][I] > 0)
+    Wg, Ug, dtn = fused_step(emu, dom, s, W, U, dt, transport)
+    assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL, (rel(Wg[I], Wr[I]), rel(Ug[I], Ur[I]))
+    # face halos written by the last stage (edge halos are not part of the state)
+    for a in dom.active:
+        sl = [dom.dom(0), dom.dom(1), dom.dom(2)]
+        for part in (slice(0, dom.nh), slice(-dom.nh, None)):
+            sl[a] = part
+            t = (Ellipsis,) + tuple(sl)
+            assert rel(Wg[t], Wr[t]) < TOL and rel(Ug[t], Ur[t]) < TOL, (a, part)
+    assert abs(dtn - o.compute_dt(Wr, dom, s)) <= 1e-14 * dt
+
+
+GRIDS = {
+    "3d": ((9, 8, 7), ((0, 1.0), (0, 0.8), (0, 1.1))),
+    "3d-wide": ((20, 6, 30), ((0, 1.0), (0, 0.5), (0, 1.4))),  # 60 wavefronts of z faces, 15 blocks
+    "2d-xy": ((13, 11, 1), ((0, 1.0), (0, 1.3), (0, 1.0))),
+    "2d-xz": ((10, 1, 12), ((0, 1.0), (0, 1.0), (0, 0.9))),
+    "2d-yz": ((1, 9, 11), ((0, 1.0), (0, 1.2), (0, 1.0))),
+    "1d": ((17, 1, 1), ((0, 1.0), (0, 1.0), (0, 1.0))),
+}
+
+
+def grid_state(name, seed, nh=3):
+    cells, ranges = GRIDS[name]
+    full = tuple(max(c, 2) for c in cells)  # random_state needs a mesh on every axis
+    _, W0 = random_state(full, nh, seed=seed, ranges=ranges)
+    W0 = np.ascontiguousarray(W0[:, :cells[0], :cells[1], :cells[2]])
+    return o.Domain(cells, ranges, nh=nh), W0
+
+
+@pytest.mark.parametrize("grid", list(GRIDS))
+@pytest.mark.parametrize("bc", ["PERIODIC", "ZEROGRADIENT", "SYMMETRY"])
+def test_fused_step_grids_and_boundaries(emu, grid, bc):
+    dom, W0 = grid_state(grid, seed=3)
+    s = o.Setup(gamma=1.4, bcs={f: bc for f in FACES}, cfl=0.6, **SCHEMES["z-charprim-hllc"])
+    check_step(emu, dom, s, W0)
+
+
+@pytest.mark.parametrize("scheme", list(SCHEMES)[1:])
+@pytest.mark.parametrize("grid", ["3d", "2d-xy"])
+def test_fused_step_schemes(emu, scheme, grid):
+    dom, W0 = grid_state(grid, seed=7)
+    s = o.Setup(gamma=1.4, bcs={f: "ZEROGRADIENT" for f in FACES}, cfl=0.5, **SCHEMES[scheme])
+    check_step(emu, dom, s, W0)
+
+
+@pytest.mark.parametrize("grid", ["3d", "2d-yz"])
+def test_fused_step_viscous(emu, grid):
+    """Dissipative fluxes through the MARCH and INNER kernels (periodic)."""
+    dom, W0 = grid_state(grid, seed=5)
+    tr = o.Transport(R=1.0, mu_ref=2e-2, T_ref=1.0, mu_exp=0.76, bulk_viscosity=1e-2, prandtl=0.71)
+    s = o.Setup(gamma=1.4, bcs={f: "PERIODIC" for f in FACES}, cfl=0.5, transport=tr,
+                **SCHEMES["z-charprim-hllc"])
+    check_step(emu, dom, s, W0, transport=tr)
+
+
+@pytest.mark.parametrize("fl", ["SIMPLE", "NASA"])
+def test_fused_step_with_limiters(emu, fl):
+    """Both positivity limiters on the near-vacuum pocket (2-D: MARCH x, INNER y)."""
+    dom, W0 = _pocket_state()
+    zg = {f: "ZEROGRADIENT" for f in ("east", "west", "north", "south")}
+    s = o.Setup(stencil="WENO5-JS", variable="CHAR-PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT", bcs=zg,
+                interpolation_limiter=True, flux_limiter=fl, flux_partition="CELLSIZE")
+    W, U = o.create_buffers(W0, dom, s)
+    dt = 0.9 / 0.5 * o.compute_dt(W, dom, s)
+    Wr, Ur = o.rk3_step(W, U, dt, dom, s)
+    Wg, Ug, _ = fused_step(emu, dom, s, W, U, dt)
+    I = dom.interior
+    assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL
