@@ -35,9 +35,16 @@ extern "C" {
 #define JFA_CONSERVATIVE 1
 #define JFA_CHAR_PRIMITIVE 2
 #define JFA_CHAR_CONSERVATIVE 3
-/* reconstruction_stencil (stencils/reconstruction/shock_capturing/weno/__init__.py:26-41) */
+/* reconstruction_stencil (stencils/reconstruction/shock_capturing/weno/__init__.py:26-41,
+ * .../teno/__init__.py:15-25).  Every stencil reads at most the 6-cell window of a face, so all
+ * run on buffers with nh >= 3. */
 #define JFA_WENO5_JS 0
 #define JFA_WENO5_Z 1
+#define JFA_WENO1 2
+#define JFA_WENO3_JS 3
+#define JFA_WENO3_Z 4
+#define JFA_TENO5 5
+#define JFA_TENO5_A 6
 /* riemann_solver (solvers/riemann_solvers/__init__.py:16-25) */
 #define JFA_HLLC 0
 #define JFA_RUSANOV 1
@@ -106,7 +113,7 @@ typedef struct jfa_transport {
 /* Numerical scheme of the high-order Godunov flux (data_types/numerical_setup/conservatives.py:35-42). */
 typedef struct jfa_scheme {
     int32_t recon_var;     /* JFA_PRIMITIVE .. JFA_CHAR_CONSERVATIVE */
-    int32_t stencil;       /* JFA_WENO5_JS / JFA_WENO5_Z */
+    int32_t stencil;       /* JFA_WENO5_JS ... JFA_TENO5_A */
     int32_t riemann;       /* JFA_HLLC / JFA_RUSANOV */
     int32_t signal_speed;  /* JFA_SS_* (HLLC only) */
     int32_t frozen_state;  /* JFA_FROZEN_* (characteristic variables only) */
@@ -144,11 +151,14 @@ const char* jfa_version(void);
 /* Last HIP error string seen by this thread (for diagnostics after JFA_ELAUNCH). */
 const char* jfa_last_error(void);
 
-/* Cell-face WENO5 reconstruction of every variable of `buffer` along `axis`:
+/* Cell-face reconstruction of every variable of `buffer` along `axis` with any JFA_* stencil:
  * j = 0 left state, j = 1 right state of each face.
- * Replaces WENO5JS.reconstruct_xi (stencils/reconstruction/shock_capturing/weno/weno5_js.py:73-115)
- * and WENO5Z.reconstruct_xi (.../weno/weno5_z.py:138-173) with the slice tables of
- * WENO5Base (.../weno5_base.py:30) and SpatialStencil.array_slices (stencils/spatial_stencil.py:45-112).
+ * Replaces WENO5JS.reconstruct_xi (stencils/reconstruction/shock_capturing/weno/weno5_js.py:73-115),
+ * WENO5Z.reconstruct_xi (.../weno/weno5_z.py:138-173), WENO1 / WENO3JS / WENO3Z
+ * (.../weno/weno1_js.py:25-30, weno3_js.py:15-42, weno3_z.py:21-41), TENO5 / TENO5A
+ * (.../teno/teno5.py:32-73, teno5_a.py:42-99) with the slice tables of WENO5Base / WENO3Base / WENO1
+ * (.../weno5_base.py:30, weno3_base.py:29, weno/weno1_js.py:22) and SpatialStencil.array_slices
+ * (stencils/spatial_stencil.py:45-112).
  * buffer: (nvar, buffer shape) of `dtype`; out: (nvar, face shape). */
 int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* grid, int32_t axis,
                              int32_t j, int32_t stencil, double eps, int32_t dtype, void* out,

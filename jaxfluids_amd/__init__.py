@@ -1,6 +1,6 @@
 """jaxfluids_amd: MI355X-native convective-flux engine with the JAX-Fluids plugin API.
 
-The hot path -- WENO5-JS/-Z reconstruction (primitive, conservative or
+The hot path -- WENO5-JS/-Z (and WENO1, WENO3-JS/-Z, TENO5, TENO5-A) reconstruction (primitive, conservative or
 characteristic variables), HLLC / Rusanov Riemann solvers, flux divergence,
 TVD-RK3 stage update, face-halo fill and the CFL time step -- runs as
 hand-written HIP kernels for gfx950 in ``libjaxfluids_amd.so``, called through a
@@ -19,7 +19,7 @@ from .input_manager import InputManager
 from .riemann_solvers import DICT_RIEMANN_SOLVER, DICT_SIGNAL_SPEEDS, HLLC, Rusanov
 from .simulation_manager import SimulationManager
 from .space_solver import SpaceSolver
-from .stencils import DICT_SPATIAL_RECONSTRUCTION, WENO5JS, WENO5Z
+from .stencils import DICT_SPATIAL_RECONSTRUCTION, TENO5, TENO5A, WENO1, WENO3JS, WENO3Z, WENO5JS, WENO5Z
 from .time_integration import DICT_TIME_INTEGRATION, RungeKutta3, compute_time_step_size
 
 __version__ = "0.1.0"

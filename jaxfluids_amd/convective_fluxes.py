@@ -4,7 +4,7 @@ Mirrors solvers/convective_fluxes/convective_flux_solver.py (ABC, :17-67) and
 high_order_godunov.py (HighOrderGodunov, :29-419).  ``compute_flux_xi`` runs one
 HIP kernel per call (jfa_compute_flux_xi): per face, the 6-cell window is loaded,
 reconstructed in the configured variables (PRIMITIVE, CONSERVATIVE,
-CHAR-PRIMITIVE, CHAR-CONSERVATIVE) with WENO5-JS/-Z, converted, and the Riemann
+CHAR-PRIMITIVE, CHAR-CONSERVATIVE) with WENO1/3/5 or TENO5 stencils, converted, and the Riemann
 problem is solved -- the reference's three steps without intermediate arrays.
 """
 from __future__ import annotations
@@ -16,7 +16,7 @@ import torch
 from . import _native
 from .config import precision
 from .riemann_solvers import HLLC, Rusanov, SignalSpeed
-from .stencils import WENO5JS, WENO5Z
+from .stencils import DICT_SPATIAL_RECONSTRUCTION
 
 
 class ConvectiveFluxSolver:
@@ -56,10 +56,12 @@ class HighOrderGodunov(ConvectiveFluxSolver):
         if self.reconstruction_variable in ("CHAR-PRIMITIVE", "CHAR-CONSERVATIVE"):
             self.reconstruction_stencil.set_slices_stencil()
         self.frozen_state = godunov.frozen_state
-        stencil_name = {WENO5JS: "WENO5-JS", WENO5Z: "WENO5-Z"}.get(godunov.reconstruction_stencil)
+        stencil_name = {cls: name for name, cls in DICT_SPATIAL_RECONSTRUCTION.items()}.get(
+            godunov.reconstruction_stencil)
         riemann_name = {HLLC: "HLLC", Rusanov: "RUSANOV"}.get(godunov.riemann_solver)
         if stencil_name is None or riemann_name is None:
-            raise NotImplementedError("only WENO5-JS/-Z with HLLC or Rusanov are built on this path")
+            raise NotImplementedError(
+                f"built on this path: stencils {tuple(DICT_SPATIAL_RECONSTRUCTION)} with HLLC or Rusanov")
         ss = godunov.signal_speed
         ss_name = ss.name if isinstance(ss, SignalSpeed) else (ss if isinstance(ss, str) else "EINFELDT")
         self._names = (self.reconstruction_variable, stencil_name, riemann_name, ss_name, self.frozen_state)

@@ -363,7 +363,8 @@ static void launch_batch_r(int riemann, const BatchArgs<T>& a, const Params& prm
 template <class T, int VAR>
 static void launch_batch_s(int stencil, int riemann, const BatchArgs<T>& a, const Params& prm, hipStream_t st) {
     if (stencil == JFA_WENO5_JS) launch_batch_r<T, VAR, JFA_WENO5_JS>(riemann, a, prm, st);
-    else launch_batch_r<T, VAR, JFA_WENO5_Z>(riemann, a, prm, st);
+    else if (stencil == JFA_WENO5_Z) launch_batch_r<T, VAR, JFA_WENO5_Z>(riemann, a, prm, st);
+    else launch_batch_r<T, VAR, kStencilExt>(riemann, a, prm, st);
 }
 
 template <class T>

@@ -47,6 +47,7 @@ struct Params {
     double eps;        // WENO epsilon (config/precision.py:39,53)
     double eps_z;      // 4 eps: epsilon of the scaled WENO-Z smoothness indicators (4 beta)
     double eps_js;     // 16 eps: epsilon of the scaled WENO-JS indicators (squared)
+    int stencil;       // JFA_WENO5_JS ... JFA_TENO5_A (read by the kStencilExt kernels)
     int signal_speed;  // JFA_SS_*
     int frozen;        // JFA_FROZEN_*
     // dissipative fluxes (jfa_transport); dissip = 0: inviscid
@@ -181,6 +182,121 @@ __device__ __forceinline__ void weno5_pair(const double v[6], const double eps_z
         const double g1 = fma(-0.2, d2, -0.1 * d3);
         const double g2 = fma(-0.2, d2, 0.05 * d1);
         R = weno5_combine<STENCIL>(b0, b1, b2, g0, g1, g2, v[3], eps_z, eps_js);
+    }
+}
+
+// ---------------------------------------------------------------- further stencils
+// Stencils that read a sub-window of the same 6 cells, selected at run time (prm.stencil) in
+// the kStencilExt instantiation of the kernels, so the WENO5 kernels keep their code.
+//   WENO1 (weno/weno1_js.py:25-30): the adjacent cell.
+//   WENO3-JS / WENO3-Z (weno3_base.py:20-45, weno/weno3_js.py:15-42, weno/weno3_z.py:21-41):
+//     beta = ((u_i - u_im)^2, (u_ip - u_i)^2), d = (1/3, 2/3), p = (u_i + dm/2, u_i + dp/2).
+//   TENO5 (teno/teno5.py:32-73): WENO5 indicators and candidates, gamma_k = (1 + tau5/(beta_k + eps))^6,
+//     sharp cutoff gamma_k / sum gamma < CT = 1e-5, d = (0.05, 0.55, 0.40).
+//   TENO5-A (teno/teno5_a.py:42-99): the same with the adaptive cutoff CT = 10^-(ceil(10 - 5 (1 - g)) - 1),
+//     g from the smoothness ratio eta of neighbouring differences, and WENO5's d = (1/10, 6/10, 3/10).
+constexpr int kStencilExt = 100;
+
+// One WENO3 side: stencil (u_im, u_i, u_ip); Z selects WENO3-Z.
+template <bool Z>
+__device__ __forceinline__ double weno3(double um, double ui, double up, double eps) {
+    const double dm = ui - um, dp = up - ui;
+    const double b0 = dm * dm, b1 = dp * dp;
+    double e0, e1;  // d_k alpha_k q0 q1
+    if constexpr (Z) {
+        const double tau3 = fabs(b0 - b1);
+        const double q0 = b0 + eps, q1 = b1 + eps;
+        e0 = (1.0 / 3.0) * ((q0 + tau3) * q1);
+        e1 = (2.0 / 3.0) * ((q1 + tau3) * q0);
+    } else {
+        e0 = (1.0 / 3.0) * fma(b1, b1, eps);
+        e1 = (2.0 / 3.0) * fma(b0, b0, eps);
+    }
+    return fma(0.5 * fma(e1, dp, e0 * dm), rcp1(e0 + e1), ui);
+}
+
+// One TENO5 side: stencil (u_imm, u_im, u_i, u_ip, u_ipp); A selects TENO5-A.
+template <bool A>
+__device__ __forceinline__ double teno5(double a, double b, double c, double d, double e, const Params& prm) {
+    const double d0 = b - a, d1 = c - b, d2 = d - c, d3 = e - d;
+    const double D0 = d1 - d0, D1 = d2 - d1, D2 = d3 - d2;
+    const double S0 = fma(3.0, d1, -d0), S1 = d1 + d2, S2 = fma(-3.0, d2, d3);
+    // 4 beta_k and 4 eps: tau5 / (beta_k + eps) is unchanged
+    const double b0 = fma(S0, S0, (kQ * D0) * D0), b1 = fma(S1, S1, (kQ * D1) * D1), b2 = fma(S2, S2, (kQ * D2) * D2);
+    const double tau5 = fabs(b0 - b2);
+    double x0 = fma(tau5, rcp(b0 + prm.eps_z), 1.0), x1 = fma(tau5, rcp(b1 + prm.eps_z), 1.0),
+           x2 = fma(tau5, rcp(b2 + prm.eps_z), 1.0);
+    x0 = x0 * x0;
+    x1 = x1 * x1;
+    x2 = x2 * x2;
+    const double g0 = x0 * x0 * x0, g1 = x1 * x1 * x1, g2 = x2 * x2 * x2;
+    double CT = 1e-5;
+    if constexpr (A) {
+        const double ed = 0.9 * 0.24 / (1.0 - 0.24) * 1e-6;  // eps_dissipation = 0.9 Cr/(1-Cr) xi^2
+        const double eta0 = fdiv(fabs(2.0 * d1 * d0) + ed, fma(d1, d1, d0 * d0) + ed);
+        const double eta1 = fdiv(fabs(2.0 * d2 * d1) + ed, fma(d2, d2, d1 * d1) + ed);
+        const double eta2 = fdiv(fabs(2.0 * d3 * d2) + ed, fma(d3, d3, d2 * d2) + ed);
+        const double eta = fmin(eta0, fmin(eta1, eta2));
+        const double m = 1.0 - fmin(1.0, fdiv(eta, 0.24));
+        const double om = 1.0 - m, om2 = om * om;
+        const double g = om2 * om2 * fma(4.0, m, 1.0);
+        const int bb = (int)ceil(10.0 - 5.0 * (1.0 - g)) - 1;  // 4..9
+        constexpr double kPow10[10] = {1.0, 1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9};
+        CT = kPow10[bb < 0 ? 0 : (bb > 9 ? 9 : bb)];
+    }
+    const double thr = CT * (g0 + g1 + g2);
+    constexpr double w_[3] = {A ? 0.1 : 0.05, A ? 0.6 : 0.55, A ? 0.3 : 0.40};
+    const double w0 = g0 < thr ? 0.0 : w_[0], w1 = g1 < thr ? 0.0 : w_[1], w2 = g2 < thr ? 0.0 : w_[2];
+    const double del0 = (5.0 * d1 - 2.0 * d0) * (1.0 / 6.0), del1 = fma(2.0, d2, d1) * (1.0 / 6.0),
+                 del2 = fma(4.0, d2, -d3) * (1.0 / 6.0);
+    const double num = fma(w2, del2, fma(w1, del1, w0 * del0));
+    return fma(num, rcp(w0 + w1 + w2 + prm.eps), c);
+}
+
+// Left and right states of one face from its 6-cell window (cells -3..2).
+template <int STENCIL>
+__device__ __forceinline__ void recon_pair(const double v[6], const Params& prm, double& L, double& R) {
+    if constexpr (STENCIL != kStencilExt) {
+        weno5_pair<STENCIL>(v, prm.eps_z, prm.eps_js, L, R);
+    } else {
+        switch (prm.stencil) {
+            case JFA_WENO1:
+                L = v[2];
+                R = v[3];
+                break;
+            case JFA_WENO3_JS:  // weno3_base.py:29: cells -2..0 and 1..-1
+                L = weno3<false>(v[1], v[2], v[3], prm.eps);
+                R = weno3<false>(v[4], v[3], v[2], prm.eps);
+                break;
+            case JFA_WENO3_Z:
+                L = weno3<true>(v[1], v[2], v[3], prm.eps);
+                R = weno3<true>(v[4], v[3], v[2], prm.eps);
+                break;
+            case JFA_TENO5:
+                L = teno5<false>(v[0], v[1], v[2], v[3], v[4], prm);
+                R = teno5<false>(v[5], v[4], v[3], v[2], v[1], prm);
+                break;
+            default:  // JFA_TENO5_A
+                L = teno5<true>(v[0], v[1], v[2], v[3], v[4], prm);
+                R = teno5<true>(v[5], v[4], v[3], v[2], v[1], prm);
+                break;
+        }
+    }
+}
+
+// One side from the 5-point stencil (u_imm, u_im, u_i, u_ip, u_ipp) of WENO5Base's slices.
+template <int STENCIL>
+__device__ __forceinline__ double recon_side(double a, double b, double c, double d, double e, const Params& prm) {
+    if constexpr (STENCIL != kStencilExt) {
+        return weno5<STENCIL>(a, b, c, d, e, prm.eps_z, prm.eps_js);
+    } else {
+        switch (prm.stencil) {
+            case JFA_WENO1: return c;
+            case JFA_WENO3_JS: return weno3<false>(b, c, d, prm.eps);
+            case JFA_WENO3_Z: return weno3<true>(b, c, d, prm.eps);
+            case JFA_TENO5: return teno5<false>(a, b, c, d, e, prm);
+            default: return teno5<true>(a, b, c, d, e, prm);
+        }
     }
 }
 
@@ -506,7 +622,6 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
                                             const Params& prm, double WL[5], double WR[5], double UL[5],
                                             double UR[5]) {
     constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
-    const double ez = prm.eps_z, ej = prm.eps_js;
     if constexpr (VAR == JFA_PRIMITIVE || VAR == JFA_CONSERVATIVE) {
         double* QL = VAR == JFA_PRIMITIVE ? WL : UL;
         double* QR = VAR == JFA_PRIMITIVE ? WR : UR;
@@ -514,7 +629,7 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         for (int i = 0; i < 5; ++i) {
             // WENO5Base slices j=0: cells -3..1, j=1: cells 2..-2 (weno5_base.py:30)
             const double c6[6] = {win[0][i], win[1][i], win[2][i], win[3][i], win[4][i], win[5][i]};
-            weno5_pair<STENCIL>(c6, ez, ej, QL[i], QR[i]);
+            recon_pair<STENCIL>(c6, prm, QL[i], QR[i]);
         }
         if constexpr (VAR == JFA_PRIMITIVE) {
             cons_from_prim(WL, prm, UL);  // :279-280
@@ -537,19 +652,19 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         double cLv[5], cRv[5], t[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] - k1 * win[k][v];
-        weno5_pair<STENCIL>(t, ez, ej, cLv[0], cRv[0]);
+        recon_pair<STENCIL>(t, prm, cLv[0], cRv[0]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = win[k][0] - k3 * win[k][4];
-        weno5_pair<STENCIL>(t, ez, ej, cLv[1], cRv[1]);
+        recon_pair<STENCIL>(t, prm, cLv[1], cRv[1]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = win[k][m0];
-        weno5_pair<STENCIL>(t, ez, ej, cLv[2], cRv[2]);
+        recon_pair<STENCIL>(t, prm, cLv[2], cRv[2]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = win[k][m1];
-        weno5_pair<STENCIL>(t, ez, ej, cLv[3], cRv[3]);
+        recon_pair<STENCIL>(t, prm, cLv[3], cRv[3]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] + k1 * win[k][v];
-        weno5_pair<STENCIL>(t, ez, ej, cLv[4], cRv[4]);
+        recon_pair<STENCIL>(t, prm, cLv[4], cRv[4]);
         // get_primitives_from_characteristics (eigendecomposition.py:517-521)
         const double sL = cLv[0] + cLv[4], sR = cRv[0] + cRv[4];
         WL[0] = f.P[0] * sL + cLv[1];
@@ -634,7 +749,7 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             const double c6[6] = {ch[0][i], ch[1][i], ch[2][i], ch[3][i], ch[4][i], ch[5][i]};
-            weno5_pair<STENCIL>(c6, ez, ej, cLv[i], cRv[i]);
+            recon_pair<STENCIL>(c6, prm, cLv[i], cRv[i]);
         }
         // transformtophysical (eigendecomposition.py:742)
 #pragma unroll

@@ -444,7 +444,8 @@ template <class T, int VAR>
 static int64_t sweeps_s(int stencil, int riemann, const void* W, const void* U, const Geo& G, const Params& prm,
                         double* R, const InnerArgs<T>& ia, hipStream_t st) {
     if (stencil == JFA_WENO5_JS) return sweeps_r<T, VAR, JFA_WENO5_JS>(riemann, W, U, G, prm, R, ia, st);
-    return sweeps_r<T, VAR, JFA_WENO5_Z>(riemann, W, U, G, prm, R, ia, st);
+    if (stencil == JFA_WENO5_Z) return sweeps_r<T, VAR, JFA_WENO5_Z>(riemann, W, U, G, prm, R, ia, st);
+    return sweeps_r<T, VAR, kStencilExt>(riemann, W, U, G, prm, R, ia, st);
 }
 
 // Launches the sweeps; *nparts receives the CFL partials per quantity of the finishing launch.

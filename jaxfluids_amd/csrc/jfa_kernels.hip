@@ -17,11 +17,10 @@ namespace jfa {
 thread_local char g_last_error[256] = "";
 
 
-// ------------------------------------------------------------------ WENO5 reconstruct_xi
+// ------------------------------------------------------------------ reconstruct_xi
 template <class T, int STENCIL>
-__global__ __launch_bounds__(kBlock) void weno5_reconstruct_kernel(const T* __restrict__ buf, int nvar, Geo G,
-                                                                   int axis, int j, double eps, FaceIdx F,
-                                                                   T* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) void reconstruct_kernel(const T* __restrict__ buf, int nvar, Geo G, int axis,
+                                                             int j, Params prm, FaceIdx F, T* __restrict__ out) {
     int f0, f1, f2;
     if (!flat_index(F.nface, F.fe[1], F.fe[2], f0, f1, f2)) return;
     const int64_t base = (int64_t)(f0 + G.off[0]) * G.s[0] + (int64_t)(f1 + G.off[1]) * G.s[1] + (f2 + G.off[2]);
@@ -30,11 +29,10 @@ __global__ __launch_bounds__(kBlock) void weno5_reconstruct_kernel(const T* __re
     for (int v = 0; v < nvar; ++v) {
         const T* b = buf + v * G.fs + base;
         double r;
-        const double ez = (12.0 / 13.0) * eps, ej = (144.0 / 169.0) * eps;
-        if (j == 0)  // cells -3..1 (weno5_base.py:30)
-            r = weno5<STENCIL>(ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), ez, ej);
+        if (j == 0)  // cells -3..1 (weno5_base.py:30; the narrower stencils use the inner cells)
+            r = recon_side<STENCIL>(ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), prm);
         else  // cells 2..-2
-            r = weno5<STENCIL>(ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), ez, ej);
+            r = recon_side<STENCIL>(ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), prm);
         out[(int64_t)v * F.nface + o] = static_cast<T>(r);
     }
 }
@@ -283,8 +281,9 @@ static FluxLaunch pick_r(int riemann) {
 }
 template <class T, int AXIS, int VAR>
 static FluxLaunch pick_s(int stencil, int riemann) {
-    return stencil == JFA_WENO5_JS ? pick_r<T, AXIS, VAR, JFA_WENO5_JS>(riemann)
-                                   : pick_r<T, AXIS, VAR, JFA_WENO5_Z>(riemann);
+    if (stencil == JFA_WENO5_JS) return pick_r<T, AXIS, VAR, JFA_WENO5_JS>(riemann);
+    if (stencil == JFA_WENO5_Z) return pick_r<T, AXIS, VAR, JFA_WENO5_Z>(riemann);
+    return pick_r<T, AXIS, VAR, kStencilExt>(riemann);
 }
 template <class T, int AXIS>
 static FluxLaunch pick_v(int var, int stencil, int riemann) {
@@ -333,19 +332,27 @@ int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* g
     if (rc) return rc;
     if (!buffer || !out || nvar < 1 || axis < 0 || axis > 2 || !G.active[axis] || (j != 0 && j != 1))
         return JFA_EINVAL;
-    if (stencil != JFA_WENO5_JS && stencil != JFA_WENO5_Z) return JFA_EUNSUPPORTED;
+    if (stencil < JFA_WENO5_JS || stencil > JFA_TENO5_A) return JFA_EUNSUPPORTED;
     const FaceIdx F = face_extents(G, axis);
     dim3 g = flat_grid(F.nface);
     hipStream_t st = (hipStream_t)stream;
-#define JFA_W(T, S) \
-    hipLaunchKernelGGL((weno5_reconstruct_kernel<T, S>), g, dim3(kBlock), 0, st, (const T*)buffer, nvar, G, axis, j, eps, F, (T*)out)
-    if (dtype == JFA_F32) {
-        if (stencil == JFA_WENO5_JS) JFA_W(float, JFA_WENO5_JS); else JFA_W(float, JFA_WENO5_Z);
-    } else if (dtype == JFA_F64) {
-        if (stencil == JFA_WENO5_JS) JFA_W(double, JFA_WENO5_JS); else JFA_W(double, JFA_WENO5_Z);
-    } else {
-        return JFA_EUNSUPPORTED;
-    }
+    Params prm = gas_params(1.4);  // only the stencil fields are read
+    prm.eps = eps;
+    prm.eps_z = 4.0 * eps;
+    prm.eps_js = 16.0 * eps;
+    prm.stencil = stencil;
+#define JFA_W(T) \
+    do { \
+        if (stencil == JFA_WENO5_JS) \
+            hipLaunchKernelGGL((reconstruct_kernel<T, JFA_WENO5_JS>), g, dim3(kBlock), 0, st, (const T*)buffer, nvar, G, axis, j, prm, F, (T*)out); \
+        else if (stencil == JFA_WENO5_Z) \
+            hipLaunchKernelGGL((reconstruct_kernel<T, JFA_WENO5_Z>), g, dim3(kBlock), 0, st, (const T*)buffer, nvar, G, axis, j, prm, F, (T*)out); \
+        else \
+            hipLaunchKernelGGL((reconstruct_kernel<T, kStencilExt>), g, dim3(kBlock), 0, st, (const T*)buffer, nvar, G, axis, j, prm, F, (T*)out); \
+    } while (0)
+    if (dtype == JFA_F32) JFA_W(float);
+    else if (dtype == JFA_F64) JFA_W(double);
+    else return JFA_EUNSUPPORTED;
 #undef JFA_W
     return check_launch();
 }

@@ -23,7 +23,7 @@ from .data_types import (ActivePhysicsSetup, CaseSetup, ConservativesSetup, Conv
                          DissipativeFluxesSetup, DomainSetup, GeneralSetup, HighOrderGodunovSetup, MaterialSetup,
                          NumericalSetup, PositivitySetup, PrecisionSetup, TimeIntegrationSetup, TransportSetup)
 from .riemann_solvers import DICT_RIEMANN_SOLVER, DICT_SIGNAL_SPEEDS
-from .stencils import DICT_SPATIAL_RECONSTRUCTION
+from .stencils import DICT_SPATIAL_RECONSTRUCTION, KERNEL_HALOS
 from .time_integration import DICT_TIME_INTEGRATION
 
 TUPLE_RECONSTRUCTION_VARIABLES = ("PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE", "CHAR-CONSERVATIVE")
@@ -154,6 +154,8 @@ def read_numerical_setup(d: Dict) -> NumericalSetup:
     stencil_cls = DICT_SPATIAL_RECONSTRUCTION[st]
     assert halo_cells >= stencil_cls.required_halos, \
         f"halo_cells={halo_cells} < {stencil_cls.required_halos} required by {st}"
+    if halo_cells < KERNEL_HALOS:
+        raise NotImplementedError(f"the HIP kernels need halo_cells >= {KERNEL_HALOS} for every stencil ({st}: {halo_cells})")
     godunov = HighOrderGodunovSetup(DICT_RIEMANN_SOLVER[rs], DICT_SIGNAL_SPEEDS[ss], stencil_cls, None, rv, fz, None)
     conv = ConvectiveFluxesSetup(DICT_CONVECTIVE_SOLVER[solver], godunov)
     pos = c.get("positivity", {})  # read_positivity.py:8-64

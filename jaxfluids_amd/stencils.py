@@ -1,10 +1,12 @@
 """Cell-face reconstruction stencils on the HIP path.
 
 Mirrors stencils/spatial_stencil.py, stencils/spatial_reconstruction.py and the
-WENO5 classes (stencils/reconstruction/shock_capturing/weno5_base.py,
-weno/weno5_js.py, weno/weno5_z.py).  ``reconstruct_xi`` evaluates the stencil on
-the GPU (jfa_weno5_reconstruct_xi) and returns a new face tensor, as the
-reference returns a new array.
+shock-capturing classes WENO1, WENO3-JS/-Z, WENO5-JS/-Z, TENO5 and TENO5-A
+(stencils/reconstruction/shock_capturing/weno5_base.py, weno3_base.py, weno/*.py, teno/*.py).
+``reconstruct_xi`` evaluates the stencil on the GPU (jfa_weno5_reconstruct_xi, which serves
+every stencil) and returns a new face tensor, as the reference returns a new array.  The HIP
+kernels read the 6-cell window of a face for every stencil, so buffers need 3 halo cells
+whatever the stencil's own requirement.
 """
 from __future__ import annotations
 
@@ -46,8 +48,11 @@ class SpatialReconstruction(SpatialStencil):
         raise NotImplementedError
 
 
-class WENO5Base(SpatialReconstruction):
-    """weno5_base.py:10-50: d = (1/10, 6/10, 3/10), 6-cell window, 3 halos."""
+KERNEL_HALOS = 3  # the HIP kernels read cells -3..2 of every face
+
+
+class ShockCapturingStencil(SpatialReconstruction):
+    """Common host side of the shock-capturing stencils: shape checks and the GPU call."""
 
     required_halos = 3
     _variant = None
@@ -56,10 +61,10 @@ class WENO5Base(SpatialReconstruction):
         super().__init__(nh, inactive_axes, offset)
         if self.n < self.required_halos:
             raise AssertionError(
-                f"Stencil with left-shift -3 is too wide for domain with {self.n} halo cells.")
-        self._dr = (1 / 10, 6 / 10, 3 / 10)
-        self._cr = ((1 / 3, -7 / 6, 11 / 6), (-1 / 6, 5 / 6, 1 / 3), (1 / 3, 5 / 6, -1 / 6))
-        self._stencil_size = 6
+                f"Stencil with left-shift -{self.required_halos} is too wide for domain with {self.n} halo cells.")
+        if self.n < KERNEL_HALOS:
+            raise NotImplementedError(
+                f"the HIP kernels read 3 cells on each side of a face: halo_cells >= 3 (got {self.n})")
 
     def _grid_for(self, buffer: torch.Tensor):
         if buffer.dim() != 4:
@@ -94,6 +99,18 @@ class WENO5Base(SpatialReconstruction):
         return out
 
 
+class WENO5Base(ShockCapturingStencil):
+    """weno5_base.py:10-50: d = (1/10, 6/10, 3/10), 6-cell window, 3 halos."""
+
+    required_halos = 3
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self._dr = (1 / 10, 6 / 10, 3 / 10)
+        self._cr = ((1 / 3, -7 / 6, 11 / 6), (-1 / 6, 5 / 6, 1 / 3), (1 / 3, 5 / 6, -1 / 6))
+        self._stencil_size = 6
+
+
 class WENO5JS(WENO5Base):
     """Jiang & Shu (1996) weights (weno/weno5_js.py:10-65)."""
 
@@ -106,8 +123,67 @@ class WENO5Z(WENO5Base):
     _variant = _native.STENCIL["WENO5-Z"]
 
 
-# stencils/__init__.py:15-19 (the reconstruction stencils built on this path)
+class TENO5(WENO5Base):
+    """Fu et al. (2016): d = (0.05, 0.55, 0.40), q = 6, CT = 1e-5 (teno/teno5.py:12-73)."""
+
+    _variant = _native.STENCIL["TENO5"]
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self._dr = (0.05, 0.55, 0.40)
+        self.C, self.q, self.CT = 1.0, 6, 1e-5
+
+
+class TENO5A(WENO5Base):
+    """TENO5 with the adaptive cutoff of teno/teno5_a.py:12-99 (xi = 1e-3, Cr = 0.24)."""
+
+    _variant = _native.STENCIL["TENO5-A"]
+
+
+class WENO3Base(ShockCapturingStencil):
+    """weno3_base.py:10-45: d = (1/3, 2/3), 4-cell window, 2 halos."""
+
+    required_halos = 2
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self._dr = (1 / 3, 2 / 3)
+        self._cr = ((-0.5, 1.5), (0.5, 0.5))
+        self._stencil_size = 4
+
+
+class WENO3JS(WENO3Base):
+    """Jiang & Shu weights on the 3-point stencils (weno/weno3_js.py:10-42)."""
+
+    _variant = _native.STENCIL["WENO3-JS"]
+
+
+class WENO3Z(WENO3Base):
+    """Don & Borges (2013) weights (weno/weno3_z.py:10-41)."""
+
+    _variant = _native.STENCIL["WENO3-Z"]
+
+
+class WENO1(ShockCapturingStencil):
+    """First-order upwind states (weno/weno1_js.py:10-30)."""
+
+    required_halos = 1
+    is_for_adaptive_mesh = True
+    _variant = _native.STENCIL["WENO1"]
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self._stencil_size = 2
+
+
+# stencils/__init__.py:15-19, reconstruction/shock_capturing/weno/__init__.py:26-41 and
+# teno/__init__.py:15-25 (the reconstruction stencils built on this path)
 DICT_SPATIAL_RECONSTRUCTION: Dict[str, type] = {
+    "WENO1": WENO1,
+    "WENO3-JS": WENO3JS,
+    "WENO3-Z": WENO3Z,
     "WENO5-JS": WENO5JS,
     "WENO5-Z": WENO5Z,
+    "TENO5": TENO5,
+    "TENO5-A": TENO5A,
 }

@@ -172,7 +172,98 @@ def weno5_z_point(u_imm, u_im, u_i, u_ip, u_ipp, eps=SPATIAL_STENCIL_EPS):
     return omega_z_0 * p_0 + omega_z_1 * p_1 + omega_z_2 * p_2
 
 
-WENO5_POINT = {"WENO5-JS": weno5_js_point, "WENO5-Z": weno5_z_point}
+# --------------------------------------------------------------------------
+# Narrower and TENO stencils on the same operands.  Each takes WENO5's five operands
+# (WENO5_SHIFTS / WENO5_WINDOW_IDX) and reads the ones of its own slice table: WENO3's
+# array_slices([range(-2,1,1), range(1,-2,-1)]) (weno3_base.py:28) are the middle three
+# (u_im, u_i, u_ip), WENO1's ([range(-1,0,1), range(0,-1,-1)]) (weno/weno1_js.py:22) is u_i;
+# the characteristic windows of sizes 4 and 2 (eigendecomposition.py:75-95, half size 2 and 1)
+# are the middle cells of the 6-cell one.
+# --------------------------------------------------------------------------
+_DR3 = (1 / 3, 2 / 3)                           # weno3_base.py:24
+_CR3 = ((-0.5, 1.5), (0.5, 0.5))                # weno3_base.py:25
+
+
+def weno1_point(u_imm, u_im, u_i, u_ip, u_ipp, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno1_js.py:25-30: the adjacent cell's value."""
+    return np.array(u_i, copy=True)
+
+
+def weno3_smoothness(u_im, u_i, u_ip):
+    """weno3_base.py:32-37."""
+    return np.square(u_i - u_im), np.square(u_ip - u_i)
+
+
+def weno3_polynomials(u_im, u_i, u_ip):
+    """weno3_base.py:40-45."""
+    return _CR3[0][0] * u_im + _CR3[0][1] * u_i, _CR3[1][0] * u_i + _CR3[1][1] * u_ip
+
+
+def weno3_js_point(u_imm, u_im, u_i, u_ip, u_ipp, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno3_js.py:27-40."""
+    beta_0, beta_1 = weno3_smoothness(u_im, u_i, u_ip)
+    one_beta_0_sq = 1.0 / (beta_0 * beta_0 + eps)
+    one_beta_1_sq = 1.0 / (beta_1 * beta_1 + eps)
+    alpha_0 = _DR3[0] * one_beta_0_sq
+    alpha_1 = _DR3[1] * one_beta_1_sq
+    one_alpha = 1.0 / (alpha_0 + alpha_1)
+    p_0, p_1 = weno3_polynomials(u_im, u_i, u_ip)
+    return alpha_0 * one_alpha * p_0 + alpha_1 * one_alpha * p_1
+
+
+def weno3_z_point(u_imm, u_im, u_i, u_ip, u_ipp, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno3_z.py:27-39 (Don & Borges 2013)."""
+    beta_0, beta_1 = weno3_smoothness(u_im, u_i, u_ip)
+    tau_3 = np.abs(beta_0 - beta_1)
+    alpha_z_0 = _DR3[0] * (1.0 + tau_3 / (beta_0 + eps))
+    alpha_z_1 = _DR3[1] * (1.0 + tau_3 / (beta_1 + eps))
+    one_alpha_z = 1.0 / (alpha_z_0 + alpha_z_1)
+    p_0, p_1 = weno3_polynomials(u_im, u_i, u_ip)
+    return alpha_z_0 * one_alpha_z * p_0 + alpha_z_1 * one_alpha_z * p_1
+
+
+def _teno5_weights(beta, CT, dr, eps):
+    """teno/teno5.py:44-68: gamma_k = (C + tau5/(beta_k + eps))^q, C = 1, q = 6; sharp cutoff
+    gamma_k / sum gamma < CT; omega_k = d_k delta_k / (sum d delta + eps)."""
+    beta_0, beta_1, beta_2 = beta
+    tau_5 = np.abs(beta_0 - beta_2)
+    gamma = [np.power(1.0 + tau_5 / (b + eps), 6) for b in beta]
+    one_gamma_sum = 1.0 / (gamma[0] + gamma[1] + gamma[2])
+    w = [dr[k] * np.where(gamma[k] * one_gamma_sum < CT, 0.0, 1.0) for k in range(3)]
+    one_dk = 1.0 / (w[0] + w[1] + w[2] + eps)
+    return [wk * one_dk for wk in w]
+
+
+def teno5_point(u_imm, u_im, u_i, u_ip, u_ipp, eps=SPATIAL_STENCIL_EPS):
+    """teno/teno5.py:32-73 (Fu et al. 2016): d = (0.05, 0.55, 0.40) (:26), CT = 1e-5 (:30)."""
+    beta = weno5_smoothness(u_imm, u_im, u_i, u_ip, u_ipp)
+    om = _teno5_weights(beta, 1e-5, (0.05, 0.55, 0.40), eps)
+    p = weno5_polynomials(u_imm, u_im, u_i, u_ip, u_ipp)
+    return om[0] * p[0] + om[1] * p[1] + om[2] * p[2]
+
+
+def teno5_a_point(u_imm, u_im, u_i, u_ip, u_ipp, eps=SPATIAL_STENCIL_EPS):
+    """teno/teno5_a.py:42-99: adaptive cutoff (:66-78) with xi = 1e-3, Cr = 0.24, alpha = (10, 5)
+    (:29-33); the weights are WENO5Base's _dr (the class sets an unused dr_, :23)."""
+    beta = weno5_smoothness(u_imm, u_im, u_i, u_ip, u_ipp)
+    xi, Cr = 1e-3, 0.24
+    ed = 0.9 * Cr / (1 - Cr) * xi ** 2
+    eta_0 = (np.abs(2 * (u_i - u_im) * (u_im - u_imm)) + ed) / ((u_i - u_im) ** 2 + (u_im - u_imm) ** 2 + ed)
+    eta_1 = (np.abs(2 * (u_ip - u_i) * (u_i - u_im)) + ed) / ((u_ip - u_i) ** 2 + (u_i - u_im) ** 2 + ed)
+    eta_2 = (np.abs(2 * (u_ipp - u_ip) * (u_ip - u_i)) + ed) / ((u_ipp - u_ip) ** 2 + (u_ip - u_i) ** 2 + ed)
+    eta = np.minimum(eta_0, np.minimum(eta_1, eta_2))
+    m = 1 - np.minimum(1.0, eta / Cr)
+    g = np.power(1 - m, 4) * (1 + 4 * m)
+    beta_bar = np.ceil(10.0 - 5.0 * (1 - g)) - 1.0
+    CT = np.power(10.0, -beta_bar)
+    om = _teno5_weights(beta, CT, _DR, eps)
+    p = weno5_polynomials(u_imm, u_im, u_i, u_ip, u_ipp)
+    return om[0] * p[0] + om[1] * p[1] + om[2] * p[2]
+
+
+WENO5_POINT = {"WENO5-JS": weno5_js_point, "WENO5-Z": weno5_z_point, "WENO1": weno1_point,
+               "WENO3-JS": weno3_js_point, "WENO3-Z": weno3_z_point, "TENO5": teno5_point,
+               "TENO5-A": teno5_a_point}
 
 
 def weno5_reconstruct_xi(buffer, dom: Domain, axis: int, j: int, stencil: str = "WENO5-JS",

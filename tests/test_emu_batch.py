@@ -153,7 +153,16 @@ SCHEMES = [
 ]
 
 
-@pytest.mark.parametrize("scheme,nh", [(s, 3) for s in SCHEMES] + [(SCHEMES[1], 5), (SCHEMES[0], 4)],
+FURTHER = [  # the run-time stencil switch of the kernel
+    ("TENO5", "CHAR-PRIMITIVE", "HLLC", "EINFELDT", "ROE"),
+    ("TENO5-A", "PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
+    ("WENO3-Z", "CHAR-PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
+    ("WENO3-JS", "CONSERVATIVE", "RUSANOV", "EINFELDT", "ARITHMETIC"),
+    ("WENO1", "PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
+]
+
+
+@pytest.mark.parametrize("scheme,nh", [(s, 3) for s in SCHEMES + FURTHER] + [(SCHEMES[1], 5), (SCHEMES[0], 4)],
                          ids=lambda v: "-".join(v) if isinstance(v, tuple) else f"nh{v}")
 def test_emulated_batch_step_matches_oracle(emu, scheme, nh):
     """One RK3 step of 5 uneven cases (ragged x chunks and y tiles, every boundary type):

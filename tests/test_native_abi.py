@@ -95,3 +95,41 @@ def test_numerical_setup_reader_rejects_options_off_the_path():
     bad["conservatives"]["halo_cells"] = 2
     with pytest.raises(AssertionError):
         read_numerical_setup(bad)
+
+
+def test_setup_reader_accepts_further_stencils():
+    """WENO1, WENO3-JS/-Z, TENO5 and TENO5-A are read from the setup dicts; the kernels read
+    3 cells on each side of a face, so fewer halo cells are refused even where the stencil's
+    own requirement is smaller."""
+    import copy
+
+    from jaxfluids_amd.input_manager import read_numerical_setup
+    from jaxfluids_amd.stencils import DICT_SPATIAL_RECONSTRUCTION
+
+    base = {"conservatives": {"halo_cells": 3, "time_integration": {"integrator": "RK3", "CFL": 0.5},
+                              "convective_fluxes": {"convective_solver": "GODUNOV",
+                                                    "godunov": {"riemann_solver": "HLLC", "signal_speed": "EINFELDT",
+                                                                "reconstruction_stencil": "WENO5-JS",
+                                                                "reconstruction_variable": "CHAR-PRIMITIVE"}}}}
+    for name in ("WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A"):
+        d = copy.deepcopy(base)
+        d["conservatives"]["convective_fluxes"]["godunov"]["reconstruction_stencil"] = name
+        ns = read_numerical_setup(d)
+        assert ns.conservatives.convective_fluxes.godunov.reconstruction_stencil is DICT_SPATIAL_RECONSTRUCTION[name]
+    d["conservatives"]["halo_cells"] = 2
+    d["conservatives"]["convective_fluxes"]["godunov"]["reconstruction_stencil"] = "WENO3-Z"
+    with pytest.raises(NotImplementedError):
+        read_numerical_setup(d)
+
+
+@pytest.mark.skipif(not os.path.exists(_native.LIB_PATH), reason="library not built")
+def test_stage_vjp_refuses_stencils_without_a_reverse_mode():
+    """jfa_rk3_stage_vjp validates the scheme before any device call: the stencils other than
+    WENO5-JS/-Z have no reverse mode and return JFA_EUNSUPPORTED."""
+    lib = _native.load()
+    grid = _native.make_grid((8, 8, 8), 3, (0.1, 0.1, 0.1))
+    bc = _native.bc_array({f: "PERIODIC" for f in ("east", "west", "north", "south", "top", "bottom")})
+    for name in ("WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A"):
+        scheme = _native.make_scheme("PRIMITIVE", name, "HLLC", "EINFELDT", "ARITHMETIC", _native.F64, 1.4, 1e-30)
+        rc = lib.jfa_rk3_stage_vjp(None, None, None, None, grid, scheme, bc, 0, 1e-3, None, 0, None)
+        assert rc == _native.JFA_EUNSUPPORTED, (name, rc)

@@ -752,3 +752,91 @@ def test_flux_limiter_switches_to_the_first_order_flux(fl, partition):
     assert not np.all(np.nan_to_num(W0s[4][I], nan=-1.0) > 0)
     W1, _ = o.rk3_step(W, U, dt, dom, s)
     assert np.all(W1[0][I] > 0) and np.all(W1[4][I] > 0)
+
+
+# ------------------------------------------------------------------ WENO1, WENO3, TENO5 (further stencils)
+def _left_states(stencil, F, f, N, periodic):
+    """Left face states of cell averages of f (antiderivative F) on [0, 1] with N cells, from
+    the stencil's operands (WENO5_SHIFTS[0] around each face's left cell); returns (states, exact)."""
+    x = np.linspace(0.0, 1.0, N + 1)
+    a = (F(x[1:]) - F(x[:-1])) * N
+    if periodic:
+        ap = np.concatenate([a[-3:], a, a[:3]])
+        i = np.arange(3, N + 3)
+        faces = x[1:]
+    else:
+        ap, i = a, np.arange(2, N - 3)
+        faces = x[i + 1]
+    ops = [ap[i + k] for k in (-2, -1, 0, 1, 2)]
+    return o.WENO5_POINT[stencil](*ops), f(faces)
+
+
+@pytest.mark.parametrize("stencil", ["WENO5-JS", "WENO5-Z", "TENO5", "TENO5-A"])
+def test_five_point_stencils_reproduce_quadratics(stencil):
+    """Every 3-cell candidate reproduces the face value of a quadratic from its cell averages and
+    the weights sum to one, so any weighting (and any cutoff) gives the exact face value."""
+    F = lambda x: 0.3 * x + 0.7 * x ** 2 / 2 - 1.9 * x ** 3 / 3  # noqa: E731
+    f = lambda x: 0.3 + 0.7 * x - 1.9 * x ** 2  # noqa: E731
+    L, exact = _left_states(stencil, F, f, 40, periodic=False)
+    assert np.max(np.abs(L - exact)) < 1e-13
+
+
+@pytest.mark.parametrize("stencil", ["WENO3-JS", "WENO3-Z"])
+def test_three_point_stencils_reproduce_lines(stencil):
+    F = lambda x: -0.4 * x + 2.5 * x ** 2 / 2  # noqa: E731
+    f = lambda x: -0.4 + 2.5 * x  # noqa: E731
+    L, exact = _left_states(stencil, F, f, 40, periodic=False)
+    assert np.max(np.abs(L - exact)) < 1e-13
+
+
+@pytest.mark.parametrize("stencil,order", [("TENO5", 4.0), ("TENO5-A", 5.0), ("WENO5-Z", 5.0)])
+def test_teno_convergence_orders(stencil, order):
+    """Smooth periodic sine: no cutoff fires, so TENO5 is its linear scheme.  With
+    d = (0.05, 0.55, 0.40) the third-order errors of the candidates (-1/4, 1/12, -1/12) h^3 u'''
+    cancel (0.05 (-1/4) + 0.55/12 - 0.40/12 = 0) but the fourth-order ones do not: order 4.
+    TENO5-A keeps WENO5's optimal d: order 5, with the error of WENO5-Z's near-linear weights."""
+    F = lambda x: -np.cos(2 * np.pi * x) / (2 * np.pi)  # noqa: E731
+    f = lambda x: np.sin(2 * np.pi * x)  # noqa: E731
+    errs = []
+    for N in (64, 128, 256):
+        L, exact = _left_states(stencil, F, f, N, periodic=True)
+        errs.append(np.max(np.abs(L - exact)))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(np.abs(rates - order) < 0.05), rates
+    if stencil == "TENO5-A":
+        Lz, _ = _left_states("WENO5-Z", F, f, 256, periodic=True)
+        L, _ = _left_states(stencil, F, f, 256, periodic=True)
+        assert np.max(np.abs(L - Lz)) < 1e-3 * errs[-1]
+
+
+@pytest.mark.parametrize("stencil", ["WENO3-JS", "WENO3-Z"])
+def test_weno3_third_order_on_monotone_data(stencil):
+    """x + 0.1 sin(2 pi x) has no critical point; WENO3's weights then approach d = (1/3, 2/3),
+    whose 3-point combination is third order."""
+    F = lambda x: x * x / 2 - 0.1 * np.cos(2 * np.pi * x) / (2 * np.pi)  # noqa: E731
+    f = lambda x: x + 0.1 * np.sin(2 * np.pi * x)  # noqa: E731
+    errs = [np.max(np.abs(np.subtract(*_left_states(stencil, F, f, N, periodic=False)))) for N in (64, 128, 256)]
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(np.abs(rates - 3.0) < 0.1), rates
+
+
+@pytest.mark.parametrize("stencil", ["TENO5", "TENO5-A"])
+def test_teno_cutoff_drops_stencils_across_a_step(stencil):
+    """Left state of the face just before a unit step (cells 0 0 0 | 1 1): the candidates that
+    cross the step have gamma_k / sum gamma ~ 1e-177 and are cut, so the state is the smooth
+    candidate's value exactly (0 on the zero side); WENO5-JS only makes their weights small.
+    Mirrored for the right state of the face just after the step."""
+    z, one = np.zeros(1), np.ones(1)
+    assert o.WENO5_POINT[stencil](z, z, z, one, one)[0] == 0.0
+    assert abs(o.WENO5_POINT[stencil](one, one, one, z, z)[0] - 1.0) < 1e-15  # candidate coefficients' rounding
+    assert o.WENO5_POINT["WENO5-JS"](z, z, z, one, one)[0] != 0.0
+
+
+def test_weno1_and_weno3_step_states():
+    """WENO1 is the adjacent cell; WENO3 at a step weights the smooth 2-cell candidate by
+    ~ 1 - 1e-30 and returns its value to rounding."""
+    v = [np.array([x]) for x in (3.0, 5.0, 7.0, 11.0, 13.0)]
+    assert o.WENO5_POINT["WENO1"](*v)[0] == 7.0
+    z, one = np.zeros(1), np.ones(1)
+    for st in ("WENO3-JS", "WENO3-Z"):
+        assert abs(o.WENO5_POINT[st](z, z, z, one, one)[0]) < 1e-25
