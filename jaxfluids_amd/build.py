@@ -14,7 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjaxfluids_amd.so")
-SOURCES = ["jfa_kernels.hip", "jfa_fused_f64.hip", "jfa_fused_f32.hip", "jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip"]
+SOURCES = ["jfa_fused_charprim.hip", "jfa_fused_charcons.hip", "jfa_fused_prim.hip", "jfa_fused_cons.hip",
+           "jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_kernels.hip", "jfa_adjoint.hip", "jfa_fused_f64.hip",
+           "jfa_fused_f32.hip"]
 ARCH = os.environ.get("JFA_OFFLOAD_ARCH", "gfx950")
 
 
@@ -56,7 +58,7 @@ def build(verbose: bool = True) -> str:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
 
-    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 8)) as ex:
         list(ex.map(run, jobs))
     objs = [obj for _, obj in jobs]
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]

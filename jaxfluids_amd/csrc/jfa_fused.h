@@ -441,12 +441,28 @@ static int64_t sweeps_r(int riemann, const void* W, const void* U, const Geo& G,
     return sweeps_f<T, VAR, STENCIL, JFA_RUSANOV>(W, U, G, prm, R, ia, st);
 }
 template <class T, int VAR>
-static int64_t sweeps_s(int stencil, int riemann, const void* W, const void* U, const Geo& G, const Params& prm,
-                        double* R, const InnerArgs<T>& ia, hipStream_t st) {
+int64_t sweeps_s(int stencil, int riemann, const void* W, const void* U, const Geo& G, const Params& prm, double* R,
+                 const InnerArgs<T>& ia, hipStream_t st) {
     if (stencil == JFA_WENO5_JS) return sweeps_r<T, VAR, JFA_WENO5_JS>(riemann, W, U, G, prm, R, ia, st);
     if (stencil == JFA_WENO5_Z) return sweeps_r<T, VAR, JFA_WENO5_Z>(riemann, W, U, G, prm, R, ia, st);
     return sweeps_r<T, VAR, kStencilExt>(riemann, W, U, G, prm, R, ia, st);
 }
+
+// The sweeps of each reconstruction variable (both storage types) are instantiated in a
+// translation unit of their own, jfa_fused_<variable>.hip, so the kernels compile in parallel.
+#define JFA_FUSED_SWEEPS(PREFIX, T, VAR)                                                                    \
+    PREFIX template int64_t sweeps_s<T, VAR>(int, int, const void*, const void*, const Geo&, const Params&, \
+                                             double*, const InnerArgs<T>&, hipStream_t);
+#ifndef JFA_FUSED_INSTANTIATE
+JFA_FUSED_SWEEPS(extern, double, JFA_PRIMITIVE)
+JFA_FUSED_SWEEPS(extern, float, JFA_PRIMITIVE)
+JFA_FUSED_SWEEPS(extern, double, JFA_CONSERVATIVE)
+JFA_FUSED_SWEEPS(extern, float, JFA_CONSERVATIVE)
+JFA_FUSED_SWEEPS(extern, double, JFA_CHAR_PRIMITIVE)
+JFA_FUSED_SWEEPS(extern, float, JFA_CHAR_PRIMITIVE)
+JFA_FUSED_SWEEPS(extern, double, JFA_CHAR_CONSERVATIVE)
+JFA_FUSED_SWEEPS(extern, float, JFA_CHAR_CONSERVATIVE)
+#endif
 
 // Launches the sweeps; *nparts receives the CFL partials per quantity of the finishing launch.
 template <class T>

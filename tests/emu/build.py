@@ -10,7 +10,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "jaxfluids_amd", "csrc")
 OUT = os.path.join(HERE, "_build", "libjfa_emu.so")
 SOURCES = [os.path.join(CSRC, f) for f in ("jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip", "jfa_kernels.hip",
-                                            "jfa_fused_f64.hip", "jfa_fused_f32.hip")]
+                                            "jfa_fused_f64.hip", "jfa_fused_f32.hip", "jfa_fused_prim.hip",
+                                            "jfa_fused_cons.hip", "jfa_fused_charprim.hip", "jfa_fused_charcons.hip")]
 DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("jfa_batch.h", "jfa_common.h", "jfa_device.h", "jfa_adjoint.h",
                                                   "jfa_dissipative.h", "jfa_fused.h")] + [
     os.path.join(HERE, "emu_runtime.cpp"), os.path.join(HERE, "hip", "hip_runtime.h"), __file__]
@@ -51,7 +52,7 @@ def build(force: bool = False) -> str:
                 return obj
 
             srcs = SOURCES + [os.path.join(HERE, "emu_runtime.cpp")]
-            with ThreadPoolExecutor(max_workers=min(len(srcs), 4)) as ex:
+            with ThreadPoolExecutor(max_workers=min(len(srcs), 6)) as ex:
                 objs = list(ex.map(compile_one, srcs))
             lib = os.path.join(tmp, "lib.so")
             subprocess.run([cxx, "-shared", "-pthread", "-o", lib, *objs], check=True)
