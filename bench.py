@@ -81,20 +81,25 @@ def tgv_primitives(n: int, gamma=1.4, mach=0.1):
     return W
 
 
-def measured_stage_traffic(n: int, dtype: str, variable: str, nh: int):
-    """HBM bytes per RK3 stage from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of
-    this workload (profiles/tgv256_f64_stage_kernels.md), or None for other workloads."""
+def measured_stage_figure(n: int, dtype: str, variable: str, nh: int, key: str = "stage_hbm_bytes"):
+    """A per-stage figure of this workload from the committed rocprofv3 summary
+    (profiles/tgv256_f64_stage_kernels.md): ``stage_hbm_bytes`` (FETCH_SIZE + WRITE_SIZE, scaled
+    by a plain copy) or ``stage_fp64_flop`` (SQ FP64 instruction counters); None for other
+    workloads."""
     if (n, dtype, variable, nh) != (256, "f64", "CHAR-PRIMITIVE", 5):
         return None
     path = os.path.join(ROOT, "profiles", "tgv256_f64_stage_kernels.md")
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("stage_hbm_bytes:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1])
     except OSError:
         return None
     return None
+
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (and matrix) peak
 
 
 def cpu_baseline(n: int, nh: int, variable: str, seconds: float):
@@ -559,11 +564,18 @@ def main():
                    "cells": ncell, "halo_cells": args.nh, "storage": args.dtype, "finite": finite},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_stage_traffic(args.n, args.dtype, args.variable, args.nh),
+                     "traffic": measured_stage_figure(args.n, args.dtype, args.variable, args.nh),
                      "kernel": "RK3 stage (jfa_rk3_stage_fused: x/y marching sweeps + z sweep with update)",
                      "avg_stage_ms": round(tot_ms / len(events), 4),
                      "alg_bytes_per_cell_stage": round(sum(byt) / 3, 2)},
     }
+    flop = measured_stage_figure(args.n, args.dtype, args.variable, args.nh, "stage_fp64_flop")
+    if flop:
+        # the stage's actual bound: FP64 arithmetic issue (flop count from the SQ counters of the
+        # committed profile, time from this run's events)
+        tflops = flop / (tot_ms / len(events) * 1e-3) / 1e12
+        out["roofline"]["fp64"] = {"achieved": round(tflops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                   "frac": round(tflops / FP64_PEAK_TFLOPS, 4), "flop_per_stage": flop}
     del sim, bufs, W, U, Wb, Ub, Wc, Uc, W_h
     torch.cuda.empty_cache()
     if args.workload == "all":

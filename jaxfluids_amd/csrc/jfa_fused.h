@@ -68,8 +68,8 @@ __device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* 
 
 // ------------------------------------------------------------------ MARCH kernel
 // grid: x over IN (lanes), y over the third axis O, z over chunks of L cells along A.
-// The plain convective variants are held to 3 waves per SIMD (168 VGPRs): one register more
-// costs a third of the resident waves of this VALU-bound loop.
+// The plain convective variants are held to 3 waves per SIMD (168 VGPRs; they need 162): one
+// register more costs a third of the resident waves of this VALU-bound loop.
 template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
 __global__ __launch_bounds__(kMarchBlockMax)
 JFA_WAVES_PER_EU((DISS || FLIM || VAR == JFA_CHAR_CONSERVATIVE) ? 1 : 3) void march_kernel(
@@ -109,21 +109,16 @@ JFA_WAVES_PER_EU((DISS || FLIM || VAR == JFA_CHAR_CONSERVATIVE) ? 1 : 3) void ma
     int64_t r = ((int64_t)c[0] * G.n[1] + c[1]) * G.n[2] + c[2];
     const int64_t rs = A == 0 ? (int64_t)G.n[1] * G.n[2] : (A == 1 ? (int64_t)G.n[2] : 1);
     const double odx = G.one_dx[A];
-    // next cell to enter the window, loaded one step ahead
-    double nxt[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) nxt[i] = ld(src, i * fs + bcol + (int64_t)(a0 + 3) * sa);
+    // The new cell of the window is loaded at the top of its step, not a step ahead: the five
+    // registers of a prefetch push the plain variants past 168 VGPRs into spills, which cost
+    // more than the load latency that three resident waves already cover.
     for (int a = a0; a < a1; ++a, r += rs) {
 #pragma unroll
         for (int k = 0; k < 5; ++k)
 #pragma unroll
             for (int i = 0; i < 5; ++i) win[k][i] = win[k + 1][i];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) win[5][i] = nxt[i];
-        if (a + 1 < a1) {
-#pragma unroll
-            for (int i = 0; i < 5; ++i) nxt[i] = ld(src, i * fs + bcol + (int64_t)(a + 4) * sa);
-        }
+        for (int i = 0; i < 5; ++i) win[5][i] = ld(src, i * fs + bcol + (int64_t)(a + 3) * sa);
         double Fn[5];
         window_flux<T, A, VAR, STENCIL, RIEMANN, FLIM>(win, W, U, bcol + (int64_t)(a + 1) * sa, sa, fs, prm, Fn);
         if constexpr (DISS) {
