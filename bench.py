@@ -391,6 +391,8 @@ def bench_adjoint_viscous(nsteps: int = 1000, n: int = 128, slots: int = 20, sto
     from jaxfluids_amd.adjoint import FixedDtStepper, binomial_table, enstrophy, rollout
     from jaxfluids_amd.workloads import fourier_velocity_direction
 
+    slots = min(slots, nsteps)
+    store_steps = min(store_steps, nsteps)
     W0 = tgv_primitives(n)
     gamma, mach = 1.4, 0.1
     p0 = 1.0 / (gamma * mach ** 2)
