@@ -53,6 +53,7 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     if (scheme->riemann == JFA_HLLC && scheme->signal_speed == JFA_SS_TORO) return JFA_EUNSUPPORTED;
     if (scheme->interpolation_limiter || scheme->flux_limiter) return JFA_EUNSUPPORTED;
     if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z) return JFA_EUNSUPPORTED;
+    if (scheme->riemann != JFA_HLLC && scheme->riemann != JFA_RUSANOV) return JFA_EUNSUPPORTED;
     if (!W_in || !lam_out || !lam_in || !bc || !workspace || stage < 0 || stage > 2 || (stage > 0 && !lam_n))
         return JFA_EINVAL;
     if (lam_in == lam_out || lam_n == lam_out || (stage > 0 && lam_n == lam_in)) return JFA_EINVAL;

@@ -48,6 +48,7 @@ struct Params {
     double eps_z;      // 4 eps: epsilon of the scaled WENO-Z smoothness indicators (4 beta)
     double eps_js;     // 16 eps: epsilon of the scaled WENO-JS indicators (squared)
     int stencil;       // JFA_WENO5_JS ... JFA_TENO5_A (read by the kStencilExt kernels)
+    double eps_precision;  // machine epsilon of the working precision (config/precision.py:36,50)
     int signal_speed;  // JFA_SS_*
     int frozen;        // JFA_FROZEN_*
     // dissipative fluxes (jfa_transport); dissip = 0: inviscid
@@ -517,11 +518,35 @@ __device__ __forceinline__ void rusanov(const double WL[5], const double WR[5], 
     for (int i = 0; i < 5; ++i) F[i] = 0.5 * (FL[i] + FR[i]) - 0.5 * alpha * (UR[i] - UL[i]);
 }
 
+// HLL._solve_riemann_problem_xi_single_phase (HLL.py:29-61): with S_L^- = min(S_L, 0) and
+// S_R^+ = max(S_R, 0) of the configured signal speeds,
+//   F = (S_R^+ F_L - S_L^- F_R + S_L^- S_R^+ (U_R - U_L)) / (S_R^+ - S_L^- + eps),
+// eps the working precision's machine epsilon (riemann_solver.py:28).
+template <int AXIS>
+__device__ __forceinline__ void hll(const double WL[5], const double WR[5], const double UL[5],
+                                    const double UR[5], const Params& prm, double F[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    const double cL = sound_speed(WL[4], WL[0], prm.gamma);
+    const double cR = sound_speed(WR[4], WR[0], prm.gamma);
+    double SL, SR;
+    signal_speeds(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], WL[4], WR[4], prm.gamma, SL, SR);
+    const double sl = fmin(SL, 0.0), sr = fmax(SR, 0.0);
+    const double r = rcp(sr - sl + prm.eps_precision);
+    const double slsr = sl * sr;
+    double FL[5], FR[5];
+    physical_flux<AXIS>(WL, UL, FL);
+    physical_flux<AXIS>(WR, UR, FR);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) F[i] = fma(slsr, UR[i] - UL[i], fma(sr, FL[i], -sl * FR[i])) * r;
+}
+
 template <int AXIS, int RIEMANN>
 __device__ __forceinline__ void riemann(const double WL[5], const double WR[5], const double UL[5],
                                         const double UR[5], const Params& prm, double F[5]) {
     if constexpr (RIEMANN == JFA_HLLC)
         hllc<AXIS>(WL, WR, UL, UR, prm, F);
+    else if constexpr (RIEMANN == JFA_HLL)
+        hll<AXIS>(WL, WR, UL, UR, prm, F);
     else
         rusanov<AXIS>(WL, WR, UL, UR, prm, F);
 }

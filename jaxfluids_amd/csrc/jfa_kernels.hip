@@ -276,8 +276,9 @@ static void launch_flux(const void* W, const void* U, const Geo& G, const FaceId
 
 template <class T, int AXIS, int VAR, int STENCIL>
 static FluxLaunch pick_r(int riemann) {
-    return riemann == JFA_HLLC ? &launch_flux<T, AXIS, VAR, STENCIL, JFA_HLLC>
-                               : &launch_flux<T, AXIS, VAR, STENCIL, JFA_RUSANOV>;
+    if (riemann == JFA_HLLC) return &launch_flux<T, AXIS, VAR, STENCIL, JFA_HLLC>;
+    if (riemann == JFA_HLL) return &launch_flux<T, AXIS, VAR, STENCIL, JFA_HLL>;
+    return &launch_flux<T, AXIS, VAR, STENCIL, JFA_RUSANOV>;
 }
 template <class T, int AXIS, int VAR>
 static FluxLaunch pick_s(int stencil, int riemann) {
@@ -371,9 +372,13 @@ int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void
 #define JFA_RA(T, R) \
     do { if (axis == 0) JFA_R(T, 0, R); else if (axis == 1) JFA_R(T, 1, R); else JFA_R(T, 2, R); } while (0)
     if (scheme->dtype == JFA_F32) {
-        if (scheme->riemann == JFA_HLLC) JFA_RA(float, JFA_HLLC); else JFA_RA(float, JFA_RUSANOV);
+        if (scheme->riemann == JFA_HLLC) JFA_RA(float, JFA_HLLC);
+        else if (scheme->riemann == JFA_HLL) JFA_RA(float, JFA_HLL);
+        else JFA_RA(float, JFA_RUSANOV);
     } else {
-        if (scheme->riemann == JFA_HLLC) JFA_RA(double, JFA_HLLC); else JFA_RA(double, JFA_RUSANOV);
+        if (scheme->riemann == JFA_HLLC) JFA_RA(double, JFA_HLLC);
+        else if (scheme->riemann == JFA_HLL) JFA_RA(double, JFA_HLL);
+        else JFA_RA(double, JFA_RUSANOV);
     }
 #undef JFA_RA
 #undef JFA_R

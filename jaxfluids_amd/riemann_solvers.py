@@ -1,7 +1,7 @@
 """Riemann solvers and signal speeds on the HIP path.
 
 Mirrors solvers/riemann_solvers/riemann_solver.py (RiemannSolver, :13-186),
-HLLC.py (:13-126), Rusanov.py (:12-47) and the DICT_RIEMANN_SOLVER /
+HLLC.py (:13-126), HLL.py (:11-61), Rusanov.py (:12-47) and the DICT_RIEMANN_SOLVER /
 DICT_SIGNAL_SPEEDS registries (solvers/riemann_solvers/__init__.py:16-34).
 The Riemann problem of every face is solved in one HIP kernel
 (jfa_riemann_xi); signal speeds are evaluated inside that kernel and are
@@ -96,6 +96,17 @@ class HLLC(RiemannSolver):
             raise AssertionError("HLLC requires a signal_speed (read_conservatives.py:146-150)")
 
 
+class HLL(RiemannSolver):
+    """HLL (Harten, Lax & van Leer 1983), single phase (HLL.py:11-61)."""
+
+    _kind = "HLL"
+
+    def __init__(self, material_manager, equation_manager, signal_speed, **kwargs) -> None:
+        super().__init__(material_manager, equation_manager, signal_speed)
+        if self.signal_speed is None:
+            raise AssertionError("HLL requires a signal_speed (read_conservatives.py:145-151)")
+
+
 class Rusanov(RiemannSolver):
     """Rusanov / local Lax-Friedrichs (Rusanov.py:12-47)."""
 
@@ -104,5 +115,6 @@ class Rusanov(RiemannSolver):
 
 DICT_RIEMANN_SOLVER: Dict[str, type] = {
     "HLLC": HLLC,
+    "HLL": HLL,
     "RUSANOV": Rusanov,
 }

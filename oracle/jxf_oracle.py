@@ -592,7 +592,23 @@ def rusanov(W_L, W_R, U_L, U_R, axis, gamma, signal_speed=None):
     return 0.5 * (F_L + F_R) - 0.5 * alpha * (U_R - U_L)
 
 
-RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov}
+def hll(W_L, W_R, U_L, U_R, axis, gamma, signal_speed="EINFELDT", eps=EPS64):
+    """HLL._solve_riemann_problem_xi_single_phase (HLL.py:29-61); eps = precision.get_eps()
+    (riemann_solver.py:28)."""
+    v = axis + 1
+    c_L = speed_of_sound(W_L[4], W_L[0], gamma)
+    c_R = speed_of_sound(W_R[4], W_R[0], gamma)
+    S_L, S_R = SIGNAL_SPEEDS[signal_speed](W_L[v], W_R[v], c_L, c_R, rho_L=W_L[0], rho_R=W_R[0],
+                                            p_L=W_L[4], p_R=W_R[4], gamma=gamma)
+    wave_speed_L = np.minimum(S_L, 0.0)
+    wave_speed_R = np.maximum(S_R, 0.0)
+    F_L = physical_flux_xi(W_L, U_L, axis)
+    F_R = physical_flux_xi(W_R, U_R, axis)
+    return (wave_speed_R * F_L - wave_speed_L * F_R + wave_speed_L * wave_speed_R * (U_R - U_L)) \
+        / (wave_speed_R - wave_speed_L + eps)
+
+
+RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll}
 
 
 # --------------------------------------------------------------------------

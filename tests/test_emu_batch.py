@@ -159,6 +159,7 @@ FURTHER = [  # the run-time stencil switch of the kernel
     ("WENO3-Z", "CHAR-PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
     ("WENO3-JS", "CONSERVATIVE", "RUSANOV", "EINFELDT", "ARITHMETIC"),
     ("WENO1", "PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
+    ("WENO5-JS", "CHAR-PRIMITIVE", "HLL", "EINFELDT", "ARITHMETIC"),  # HLL Riemann solver
 ]
 
 

@@ -173,3 +173,13 @@ def test_fused_step_further_stencils(emu, stencil, variable, grid):
     s = o.Setup(gamma=1.4, bcs={f: "ZEROGRADIENT" for f in FACES}, cfl=0.5, stencil=stencil, variable=variable,
                 riemann="HLLC", signal_speed="EINFELDT", frozen_state="ROE")
     check_step(emu, dom, s, W0)
+
+
+@pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "TORO"])
+@pytest.mark.parametrize("variable,grid", [("CHAR-PRIMITIVE", "3d"), ("PRIMITIVE", "2d-xz")])
+def test_fused_step_hll(emu, speed, variable, grid):
+    """The HLL Riemann solver (HLL.py:29-61) in the fused sweeps."""
+    dom, W0 = grid_state(grid, seed=13)
+    s = o.Setup(gamma=1.4, bcs={f: "ZEROGRADIENT" for f in FACES}, cfl=0.5, stencil="WENO5-Z", variable=variable,
+                riemann="HLL", signal_speed=speed, frozen_state="ROE")
+    check_step(emu, dom, s, W0)

@@ -840,3 +840,40 @@ def test_weno1_and_weno3_step_states():
     z, one = np.zeros(1), np.ones(1)
     for st in ("WENO3-JS", "WENO3-Z"):
         assert abs(o.WENO5_POINT[st](z, z, z, one, one)[0]) < 1e-25
+
+
+# ------------------------------------------------------------------ HLL
+def _riemann_pair(n=64, seed=3, axis=0):
+    rng = np.random.default_rng(seed)
+    WL = np.stack([rng.uniform(0.2, 2, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n),
+                   rng.uniform(0.1, 3, n)])
+    WR = np.stack([rng.uniform(0.2, 2, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n),
+                   rng.uniform(0.1, 3, n)])
+    return WL, WR, o.cons_from_prim(WL, 1.4), o.cons_from_prim(WR, 1.4)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_hll_with_rusanov_speeds_is_rusanov(axis):
+    """With S_L = -S, S_R = +S (the RUSANOV signal speeds) HLL's flux is
+    (S F_L + S F_R - S^2 (U_R - U_L)) / (2S + eps) = Rusanov's flux up to eps / 2S."""
+    WL, WR, UL, UR = _riemann_pair(axis=axis)
+    F_hll = o.hll(WL, WR, UL, UR, axis, 1.4, "RUSANOV")
+    F_rus = o.rusanov(WL, WR, UL, UR, axis, 1.4)
+    assert np.max(np.abs(F_hll - F_rus)) < 1e-14 * np.max(np.abs(F_rus))
+
+
+@pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "ARITHMETIC", "TORO"])
+def test_hll_consistency_and_upwinding(speed):
+    """Equal states give the physical flux (to eps); supersonic states to the right (S_L >= 0)
+    give F_L and to the left (S_R <= 0) give F_R (to eps) -- the two limits of the HLL fan."""
+    WL, WR, UL, UR = _riemann_pair()
+    F_same = o.hll(WL, WL, UL, UL, 0, 1.4, speed)
+    assert np.max(np.abs(F_same - o.physical_flux_xi(WL, UL, 0))) < 1e-14 * np.max(np.abs(F_same))
+    for sign in (1.0, -1.0):
+        WLs, WRs = WL.copy(), WR.copy()
+        WLs[1] = sign * 8.0  # Mach > 3 in the x direction for every pair
+        WRs[1] = sign * 8.0
+        ULs, URs = o.cons_from_prim(WLs, 1.4), o.cons_from_prim(WRs, 1.4)
+        F = o.hll(WLs, WRs, ULs, URs, 0, 1.4, speed)
+        up = o.physical_flux_xi(WLs, ULs, 0) if sign > 0 else o.physical_flux_xi(WRs, URs, 0)
+        assert np.max(np.abs(F - up)) < 1e-13 * np.max(np.abs(up)), (speed, sign)
