@@ -358,7 +358,7 @@ static void launch_batch(const BatchArgs<T>& a, const Params& prm, hipStream_t s
 template <class T, int VAR, int STENCIL>
 static void launch_batch_r(int riemann, const BatchArgs<T>& a, const Params& prm, hipStream_t st) {
     if (riemann == JFA_HLLC) launch_batch<T, VAR, STENCIL, JFA_HLLC>(a, prm, st);
-    else if (riemann == JFA_HLL) launch_batch<T, VAR, STENCIL, JFA_HLL>(a, prm, st);
+    else if (riemann != JFA_RUSANOV) launch_batch<T, VAR, STENCIL, kRiemannExt>(a, prm, st);
     else launch_batch<T, VAR, STENCIL, JFA_RUSANOV>(a, prm, st);
 }
 template <class T, int VAR>

@@ -222,7 +222,7 @@ static inline int check_scheme(const jfa_scheme* s) {
     if (!s) return JFA_EINVAL;
     if (s->recon_var < 0 || s->recon_var > 3) return JFA_EUNSUPPORTED;
     if (s->stencil < JFA_WENO5_JS || s->stencil > JFA_TENO5_A) return JFA_EUNSUPPORTED;
-    if (s->riemann != JFA_HLLC && s->riemann != JFA_RUSANOV && s->riemann != JFA_HLL) return JFA_EUNSUPPORTED;
+    if (s->riemann < JFA_HLLC || s->riemann > JFA_HLLC_LM) return JFA_EUNSUPPORTED;
     if (s->signal_speed < 0 || s->signal_speed > 4) return JFA_EUNSUPPORTED;
     if (s->frozen_state != JFA_FROZEN_ARITHMETIC && s->frozen_state != JFA_FROZEN_ROE) return JFA_EUNSUPPORTED;
     if (s->dtype != JFA_F64 && s->dtype != JFA_F32) return JFA_EUNSUPPORTED;
@@ -277,6 +277,7 @@ static inline Params make_params(const jfa_scheme* s) {
     p.eps_js = 16.0 * s->eps_stencil;
     p.stencil = s->stencil;
     p.eps_precision = s->dtype == JFA_F32 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
+    p.riemann = s->riemann;
     p.signal_speed = s->signal_speed;
     p.frozen = s->frozen_state;
     set_transport(p, s->transport, s->gamma);
@@ -352,6 +353,7 @@ static inline Params gas_params(double gamma) {
     p.eps_js = 16.0 * 1e-30;
     p.stencil = JFA_WENO5_JS;
     p.eps_precision = 2.220446049250313e-16;
+    p.riemann = JFA_HLLC;
     p.signal_speed = JFA_SS_EINFELDT;
     p.frozen = JFA_FROZEN_ARITHMETIC;
     set_transport(p, nullptr, gamma);

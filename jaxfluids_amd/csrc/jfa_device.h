@@ -49,6 +49,7 @@ struct Params {
     double eps_js;     // 16 eps: epsilon of the scaled WENO-JS indicators (squared)
     int stencil;       // JFA_WENO5_JS ... JFA_TENO5_A (read by the kStencilExt kernels)
     double eps_precision;  // machine epsilon of the working precision (config/precision.py:36,50)
+    int riemann;           // JFA_HLL / JFA_HLLC_LM (read by the kRiemannExt kernels)
     int signal_speed;  // JFA_SS_*
     int frozen;        // JFA_FROZEN_*
     // dissipative fluxes (jfa_transport); dissip = 0: inviscid
@@ -540,15 +541,74 @@ __device__ __forceinline__ void hll(const double WL[5], const double WR[5], cons
     for (int i = 0; i < 5; ++i) F[i] = fma(slsr, UR[i] - UL[i], fma(sr, FL[i], -sl * FR[i])) * r;
 }
 
+// HLLC star state of side K (HLLCLM.py:95-108, the same as HLLC.py:52-63).
+template <int AXIS>
+__device__ __forceinline__ void hllc_star_state(const double W[5], const double U[5], double SK, double Ss,
+                                                double us[5]) {
+    constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
+    const double d = SK - W[v];
+    const double inv = rcp1(SK - Ss);
+    const double rd = W[0] * d;
+    const double pre = rd * inv;
+    us[0] = pre;
+    us[v] = pre * Ss;
+    us[m0] = pre * W[m0];
+    us[m1] = pre * W[m1];
+    us[4] = inv * (d * U[4] + (Ss - W[v]) * (rd * Ss + W[4]));
+}
+
+__device__ __forceinline__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+
+// HLLC-LM (HLLCLM.py:59-136; Fleischmann et al. 2020): the HLLC star states, outer wave speeds
+// scaled by phi = sin(min(1, Ma / 0.1) pi / 2), Ma = max(|u_L / c_L|, |u_R / c_R|) (:37-57),
+// the central star flux
+//   F* = (F_L + F_R)/2 + (phi S_L (u*_L - U_L) + |S*| (u*_L - u*_R) + phi S_R (u*_R - U_R))/2
+// and the blend (1 + sgn S_L)/2 F_L + (1 - sgn S_R)/2 F_R + (1 - sgn S_L)(1 + sgn S_R)/4 F*.
+template <int AXIS>
+__device__ __forceinline__ void hllc_lm(const double WL[5], const double WR[5], const double UL[5],
+                                        const double UR[5], const Params& prm, double F[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    const double cL = sound_speed(WL[4], WL[0], prm.gamma);
+    const double cR = sound_speed(WR[4], WR[0], prm.gamma);
+    double SL, SR;
+    signal_speeds(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], WL[4], WR[4], prm.gamma, SL, SR);
+    const double rdL = WL[0] * (SL - WL[v]);
+    const double rdR = WR[0] * (SR - WR[v]);
+    const double Ss = fdiv((WR[4] - WL[4]) + (WL[v] * rdL - WR[v] * rdR), rdL - rdR);
+    double usL[5], usR[5];
+    hllc_star_state<AXIS>(WL, UL, SL, Ss, usL);
+    hllc_star_state<AXIS>(WR, UR, SR, Ss, usR);
+    const double Ma = fmax(fabs(fdiv(WL[v], cL)), fabs(fdiv(WR[v], cR)));
+    const double phi = sin(fmin(1.0, Ma / 0.1) * (0.5 * 3.141592653589793));
+    const double wL = phi * SL, wR = phi * SR, aS = fabs(Ss);
+    const double bL = 0.5 * (1.0 + sgn(SL)), bR = 0.5 * (1.0 - sgn(SR));
+    const double bS = 0.25 * (1.0 - sgn(SL)) * (1.0 + sgn(SR));
+    double FL[5], FR[5];
+    physical_flux<AXIS>(WL, UL, FL);
+    physical_flux<AXIS>(WR, UR, FR);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const double Fs = 0.5 * (FL[i] + FR[i]) +
+                          0.5 * (wL * (usL[i] - UL[i]) + aS * (usL[i] - usR[i]) + wR * (usR[i] - UR[i]));
+        F[i] = bL * FL[i] + bR * FR[i] + bS * Fs;
+    }
+}
+
+// The further Riemann solvers, selected at run time (prm.riemann) in one instantiation of the
+// face kernels, so the HLLC and Rusanov kernels keep their code.
+constexpr int kRiemannExt = 100;
+
 template <int AXIS, int RIEMANN>
 __device__ __forceinline__ void riemann(const double WL[5], const double WR[5], const double UL[5],
                                         const double UR[5], const Params& prm, double F[5]) {
-    if constexpr (RIEMANN == JFA_HLLC)
+    if constexpr (RIEMANN == JFA_HLLC) {
         hllc<AXIS>(WL, WR, UL, UR, prm, F);
-    else if constexpr (RIEMANN == JFA_HLL)
-        hll<AXIS>(WL, WR, UL, UR, prm, F);
-    else
+    } else if constexpr (RIEMANN == kRiemannExt) {
+        if (prm.riemann == JFA_HLL) hll<AXIS>(WL, WR, UL, UR, prm, F);
+        else hllc_lm<AXIS>(WL, WR, UL, UR, prm, F);
+    } else {
         rusanov<AXIS>(WL, WR, UL, UR, prm, F);
+    }
 }
 
 // Positivity flux limiter of one face (limiter_flux.py:147-316,452-640,733-774; SIMPLE / NASA,

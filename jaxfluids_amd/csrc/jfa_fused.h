@@ -433,7 +433,7 @@ template <class T, int VAR, int STENCIL>
 static int64_t sweeps_r(int riemann, const void* W, const void* U, const Geo& G, const Params& prm, double* R,
                         const InnerArgs<T>& ia, hipStream_t st) {
     if (riemann == JFA_HLLC) return sweeps_f<T, VAR, STENCIL, JFA_HLLC>(W, U, G, prm, R, ia, st);
-    if (riemann == JFA_HLL) return sweeps_f<T, VAR, STENCIL, JFA_HLL>(W, U, G, prm, R, ia, st);
+    if (riemann != JFA_RUSANOV) return sweeps_f<T, VAR, STENCIL, kRiemannExt>(W, U, G, prm, R, ia, st);
     return sweeps_f<T, VAR, STENCIL, JFA_RUSANOV>(W, U, G, prm, R, ia, st);
 }
 template <class T, int VAR>

@@ -277,7 +277,7 @@ static void launch_flux(const void* W, const void* U, const Geo& G, const FaceId
 template <class T, int AXIS, int VAR, int STENCIL>
 static FluxLaunch pick_r(int riemann) {
     if (riemann == JFA_HLLC) return &launch_flux<T, AXIS, VAR, STENCIL, JFA_HLLC>;
-    if (riemann == JFA_HLL) return &launch_flux<T, AXIS, VAR, STENCIL, JFA_HLL>;
+    if (riemann != JFA_RUSANOV) return &launch_flux<T, AXIS, VAR, STENCIL, kRiemannExt>;
     return &launch_flux<T, AXIS, VAR, STENCIL, JFA_RUSANOV>;
 }
 template <class T, int AXIS, int VAR>
@@ -373,11 +373,11 @@ int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void
     do { if (axis == 0) JFA_R(T, 0, R); else if (axis == 1) JFA_R(T, 1, R); else JFA_R(T, 2, R); } while (0)
     if (scheme->dtype == JFA_F32) {
         if (scheme->riemann == JFA_HLLC) JFA_RA(float, JFA_HLLC);
-        else if (scheme->riemann == JFA_HLL) JFA_RA(float, JFA_HLL);
+        else if (scheme->riemann != JFA_RUSANOV) JFA_RA(float, kRiemannExt);
         else JFA_RA(float, JFA_RUSANOV);
     } else {
         if (scheme->riemann == JFA_HLLC) JFA_RA(double, JFA_HLLC);
-        else if (scheme->riemann == JFA_HLL) JFA_RA(double, JFA_HLL);
+        else if (scheme->riemann != JFA_RUSANOV) JFA_RA(double, kRiemannExt);
         else JFA_RA(double, JFA_RUSANOV);
     }
 #undef JFA_RA

@@ -1,7 +1,8 @@
 """Riemann solvers and signal speeds on the HIP path.
 
 Mirrors solvers/riemann_solvers/riemann_solver.py (RiemannSolver, :13-186),
-HLLC.py (:13-126), HLL.py (:11-61), Rusanov.py (:12-47) and the DICT_RIEMANN_SOLVER /
+HLLC.py (:13-126), HLL.py (:11-61), HLLCLM.py (:13-136), Rusanov.py (:12-47) and the
+DICT_RIEMANN_SOLVER /
 DICT_SIGNAL_SPEEDS registries (solvers/riemann_solvers/__init__.py:16-34).
 The Riemann problem of every face is solved in one HIP kernel
 (jfa_riemann_xi); signal speeds are evaluated inside that kernel and are
@@ -107,6 +108,19 @@ class HLL(RiemannSolver):
             raise AssertionError("HLL requires a signal_speed (read_conservatives.py:145-151)")
 
 
+class HLLCLM(RiemannSolver):
+    """HLLC with the low-Mach correction of Fleischmann et al. (2020), Ma_limit = 0.1
+    (HLLCLM.py:13-136)."""
+
+    _kind = "HLLC-LM"
+
+    def __init__(self, material_manager, equation_manager, signal_speed, **kwargs) -> None:
+        super().__init__(material_manager, equation_manager, signal_speed)
+        if self.signal_speed is None:
+            raise AssertionError("HLLC-LM requires a signal_speed (read_conservatives.py:145-151)")
+        self.Ma_limit = 0.1
+
+
 class Rusanov(RiemannSolver):
     """Rusanov / local Lax-Friedrichs (Rusanov.py:12-47)."""
 
@@ -116,5 +130,6 @@ class Rusanov(RiemannSolver):
 DICT_RIEMANN_SOLVER: Dict[str, type] = {
     "HLLC": HLLC,
     "HLL": HLL,
+    "HLLC-LM": HLLCLM,
     "RUSANOV": Rusanov,
 }

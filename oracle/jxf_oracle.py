@@ -608,7 +608,40 @@ def hll(W_L, W_R, U_L, U_R, axis, gamma, signal_speed="EINFELDT", eps=EPS64):
         / (wave_speed_R - wave_speed_L + eps)
 
 
-RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll}
+def hllc_lm(W_L, W_R, U_L, U_R, axis, gamma, signal_speed="EINFELDT"):
+    """HLLCLM._solve_riemann_problem_xi_single_phase (HLLCLM.py:59-136; Fleischmann et al. 2020):
+    HLLC star states, wave speeds scaled by phi = sin(min(1, Ma / 0.1) pi / 2) with
+    Ma = max(|u_L / c_L|, |u_R / c_R|) (:37-57), the central star flux of Eq. (18) and the
+    sign blend of the outer waves."""
+    v = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    c_L = speed_of_sound(W_L[4], W_L[0], gamma)
+    c_R = speed_of_sound(W_R[4], W_R[0], gamma)
+    S_L, S_R = SIGNAL_SPEEDS[signal_speed](W_L[v], W_R[v], c_L, c_R, rho_L=W_L[0], rho_R=W_R[0],
+                                            p_L=W_L[4], p_R=W_R[4], gamma=gamma)
+    S_star = compute_sstar(W_L[v], W_R[v], W_L[4], W_R[4], W_L[0], W_R[0], S_L, S_R)
+
+    def star(W, U, S_K):  # :95-108
+        pre = (S_K - W[v]) / (S_K - S_star) * W[0]
+        u = [pre, pre, pre, pre, pre * (U[4] / U[0] + (S_star - W[v]) * (S_star + W[4] / W[0] / (S_K - W[v])))]
+        u[v] = u[v] * S_star
+        u[m0] = u[m0] * W[m0]
+        u[m1] = u[m1] * W[m1]
+        return np.stack(u)
+
+    u_star_L, u_star_R = star(W_L, U_L, S_L), star(W_R, U_R, S_R)
+    Ma_local = np.maximum(np.abs(W_L[v] / c_L), np.abs(W_R[v] / c_R))
+    phi = np.sin(np.minimum(1.0, Ma_local / 0.1) * np.pi * 0.5)
+    ws_L, ws_R = phi * S_L, phi * S_R
+    F_L = physical_flux_xi(W_L, U_L, axis)
+    F_R = physical_flux_xi(W_R, U_R, axis)
+    flux_star = 0.5 * (F_L + F_R) + 0.5 * (ws_L * (u_star_L - U_L) + np.abs(S_star) * (u_star_L - u_star_R)
+                                           + ws_R * (u_star_R - U_R))
+    return (0.5 * (1 + np.sign(S_L)) * F_L + 0.5 * (1 - np.sign(S_R)) * F_R
+            + 0.25 * (1 - np.sign(S_L)) * (1 + np.sign(S_R)) * flux_star)
+
+
+RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll, "HLLC-LM": hllc_lm}
 
 
 # --------------------------------------------------------------------------

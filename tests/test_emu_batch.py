@@ -160,6 +160,7 @@ FURTHER = [  # the run-time stencil switch of the kernel
     ("WENO3-JS", "CONSERVATIVE", "RUSANOV", "EINFELDT", "ARITHMETIC"),
     ("WENO1", "PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
     ("WENO5-JS", "CHAR-PRIMITIVE", "HLL", "EINFELDT", "ARITHMETIC"),  # HLL Riemann solver
+    ("TENO5", "PRIMITIVE", "HLLC-LM", "DAVIS", "ARITHMETIC"),  # HLLC-LM Riemann solver
 ]
 
 

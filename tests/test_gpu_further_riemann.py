@@ -1,8 +1,9 @@
-"""GPU parity of the HLL Riemann solver (solvers/riemann_solvers/HLL.py:29-61) against the oracle:
-the plugin's solve_riemann_problem_xi for every signal speed, compute_flux_xi for every
-reconstruction variable, fused RK3 steps through the setup dicts, and the batched ensemble stage.
-FP64, relative L-inf <= 1e-12.  The stage VJP has no HLL reverse mode and refuses it (CPU suite).
-"""
+"""GPU parity of the further Riemann solvers -- HLL (solvers/riemann_solvers/HLL.py:29-61) and
+HLLC-LM (HLLCLM.py:59-136) -- against the oracle: the plugin's solve_riemann_problem_xi for every
+signal speed, compute_flux_xi for every reconstruction variable, fused RK3 steps through the setup
+dicts, and the batched ensemble stage.  FP64, relative L-inf <= 1e-12.  The states' Mach numbers
+straddle HLLC-LM's limit of 0.1.  The stage VJP has no reverse mode for them and refuses them
+(CPU suite)."""
 import numpy as np
 import pytest
 import torch
@@ -17,12 +18,16 @@ FACES = ("east", "west", "north", "south", "top", "bottom")
 SPEEDS = ["EINFELDT", "DAVIS", "ARITHMETIC", "TORO", "RUSANOV"]
 
 
+SOLVERS = ["HLL", "HLLC-LM"]
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("speed", SPEEDS)
 @pytest.mark.parametrize("axis", [0, 2])
-def test_riemann_plugin(speed, axis):
+def test_riemann_plugin(solver, speed, axis):
     from jaxfluids_amd import DICT_RIEMANN_SOLVER, EquationManager, IdealGas, MaterialManager
 
-    say(f"[hll riemann] {speed} axis={axis}")
+    say(f"[riemann] {solver} {speed} axis={axis}")
     rng = np.random.default_rng(11 + axis)
     n = 4096
 
@@ -33,20 +38,23 @@ def test_riemann_plugin(speed, axis):
     WL, WR = prims(), prims()
     WR[:, :64] = WL[:, :64]  # equal states
     UL, UR = o.cons_from_prim(WL, 1.4), o.cons_from_prim(WR, 1.4)
-    ref = o.hll(WL, WR, UL, UR, axis, 1.4, speed)
+    WL[axis + 1, 64:1024] *= 0.02  # low Mach numbers (HLLC-LM's phi < 1)
+    UL, UR = o.cons_from_prim(WL, 1.4), o.cons_from_prim(WR, 1.4)
+    ref = o.RIEMANN[solver](WL, WR, UL, UR, axis, 1.4, speed)
     mm = MaterialManager(IdealGas(1.4))
-    rs = DICT_RIEMANN_SOLVER["HLL"](mm, EquationManager(mm), speed)
+    rs = DICT_RIEMANN_SOLVER[solver](mm, EquationManager(mm), speed)
     got, _, _ = rs.solve_riemann_problem_xi(dev(WL), dev(WR), dev(UL), dev(UR), axis)
     assert rel(host(got), ref) < TOL
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE", "CHAR-CONSERVATIVE"])
 @pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS"])
-def test_compute_flux_xi(variable, speed):
+def test_compute_flux_xi(solver, variable, speed):
     from jaxfluids_amd import _native
 
-    say(f"[hll flux_xi] {variable} {speed}")
-    s = o.Setup(gamma=1.4, stencil="WENO5-Z", variable=variable, riemann="HLL", signal_speed=speed,
+    say(f"[flux_xi] {solver} {variable} {speed}")
+    s = o.Setup(gamma=1.4, stencil="WENO5-Z", variable=variable, riemann=solver, signal_speed=speed,
                 frozen_state="ROE", bcs={f: "PERIODIC" for f in FACES})
     dom, W0 = random_state((10, 9, 8), 3, seed=23)
     W, U = o.create_buffers(W0, dom, s)
@@ -61,18 +69,19 @@ def test_compute_flux_xi(variable, speed):
         assert rel(host(out), ref) < TOL, (axis, rel(host(out), ref))
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
 @pytest.mark.parametrize("cells,bc,stencil", [((11, 9, 8), "ZEROGRADIENT", "WENO5-Z"), ((37, 70, 1), "PERIODIC", "TENO5")],
                          ids=["3d-weno5z", "2d-teno5"])
-def test_rk3_step_through_setup_dicts(cells, bc, stencil):
+def test_rk3_step_through_setup_dicts(solver, cells, bc, stencil):
     from jaxfluids_amd import SimulationManager
 
-    say(f"[hll rk3 step] {cells} {bc} {stencil}")
+    say(f"[rk3 step] {solver} {cells} {bc} {stencil}")
     ranges = ((0, 1.0), (0, 1.3), (0, 1.0))
     full = tuple(max(c, 2) for c in cells)
     _, W0 = random_state(full, 3, seed=19, ranges=ranges)
     W0 = np.ascontiguousarray(W0[:, :cells[0], :cells[1], :cells[2]])
     dom = o.Domain(cells, ranges, nh=3)
-    s = o.Setup(gamma=1.4, stencil=stencil, variable="CHAR-PRIMITIVE", riemann="HLL", signal_speed="EINFELDT",
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="CHAR-PRIMITIVE", riemann=solver, signal_speed="EINFELDT",
                 frozen_state="ARITHMETIC",
                 bcs={f: (bc if o.FACE_AXIS[f] in dom.active else "PERIODIC") for f in FACES})
     sim = SimulationManager(_input_manager(dom, s))
@@ -88,11 +97,12 @@ def test_rk3_step_through_setup_dicts(cells, bc, stencil):
     assert abs(host(dtd)[0] - o.compute_dt(Wr, dom, s)) < 1e-14 * dt
 
 
-def test_ensemble_step_matches_oracle():
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_ensemble_step_matches_oracle(solver):
     from test_gpu_ensemble import CASES, case_state, face_regions, make_ensemble, oracle_case
 
-    scheme = ("WENO5-JS", "CHAR-PRIMITIVE", "HLL", "EINFELDT", "ARITHMETIC")
-    say(f"[hll ensemble] {scheme}")
+    scheme = ("WENO5-JS", "CHAR-PRIMITIVE", solver, "EINFELDT", "ARITHMETIC")
+    say(f"[ensemble] {scheme}")
     prims = [case_state(c[0], c[1], seed=300 + i) for i, c in enumerate(CASES)]
     ens = make_ensemble(scheme, chunk=16)
     ens.initialize(prims)

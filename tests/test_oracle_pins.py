@@ -877,3 +877,36 @@ def test_hll_consistency_and_upwinding(speed):
         F = o.hll(WLs, WRs, ULs, URs, 0, 1.4, speed)
         up = o.physical_flux_xi(WLs, ULs, 0) if sign > 0 else o.physical_flux_xi(WRs, URs, 0)
         assert np.max(np.abs(F - up)) < 1e-13 * np.max(np.abs(up)), (speed, sign)
+
+
+# ------------------------------------------------------------------ HLLC-LM
+def test_hllc_lm_reduces_to_hllc_above_the_mach_limit():
+    """For Ma >= 0.1, phi = 1 and the central star flux of Fleischmann et al. Eq. (18) is the
+    HLLC flux of the side of the contact: with the Rankine-Hugoniot relation across it,
+    F*_R = F*_L + S*(u*_R - u*_L), (F*_L + F*_R)/2 + |S*|(u*_L - u*_R)/2 = F*_L (S* > 0) or F*_R.
+    Checked against the independent HLLC restatement; below the limit the two differ."""
+    WL, WR, UL, UR = _riemann_pair(n=2000, seed=3)
+    c_L, c_R = o.speed_of_sound(WL[4], WL[0], 1.4), o.speed_of_sound(WR[4], WR[0], 1.4)
+    ma = np.maximum(np.abs(WL[1] / c_L), np.abs(WR[1] / c_R))
+    for speed in ("EINFELDT", "DAVIS"):
+        F_lm = o.hllc_lm(WL, WR, UL, UR, 0, 1.4, speed)
+        F_c = o.hllc(WL, WR, UL, UR, 0, 1.4, speed)
+        hi = ma >= 0.1
+        assert hi.sum() > 1000 and (~hi).sum() > 10
+        assert np.max(np.abs(F_lm - F_c)[:, hi]) < 1e-13 * np.max(np.abs(F_c))
+        assert np.max(np.abs(F_lm - F_c)[:, ~hi]) > 1e-3
+
+
+def test_hllc_lm_consistency_and_upwinding():
+    """Equal states at low Mach give the physical flux; supersonic states give the upwind flux."""
+    WL, WR, UL, UR = _riemann_pair()
+    WL[1] *= 0.01  # Ma < 0.1: phi < 1
+    UL = o.cons_from_prim(WL, 1.4)
+    F = o.hllc_lm(WL, WL, UL, UL, 0, 1.4)
+    assert np.max(np.abs(F - o.physical_flux_xi(WL, UL, 0))) < 1e-14 * np.max(np.abs(F))
+    for sign in (1.0, -1.0):
+        WLs, WRs = WL.copy(), WR.copy()
+        WLs[1] = WRs[1] = sign * 8.0
+        ULs, URs = o.cons_from_prim(WLs, 1.4), o.cons_from_prim(WRs, 1.4)
+        up = o.physical_flux_xi(WLs, ULs, 0) if sign > 0 else o.physical_flux_xi(WRs, URs, 0)
+        assert np.max(np.abs(o.hllc_lm(WLs, WRs, ULs, URs, 0, 1.4) - up)) < 1e-13 * np.max(np.abs(up))
