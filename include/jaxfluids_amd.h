@@ -50,6 +50,7 @@ extern "C" {
 #define JFA_RUSANOV 1
 #define JFA_HLL 2
 #define JFA_HLLC_LM 3
+#define JFA_AUSMP 4
 /* signal_speed (solvers/riemann_solvers/__init__.py:27-34; DAVIS2 is not callable from HLLC there) */
 #define JFA_SS_ARITHMETIC 0
 #define JFA_SS_RUSANOV 1
@@ -116,7 +117,7 @@ typedef struct jfa_transport {
 typedef struct jfa_scheme {
     int32_t recon_var;     /* JFA_PRIMITIVE .. JFA_CHAR_CONSERVATIVE */
     int32_t stencil;       /* JFA_WENO5_JS ... JFA_TENO5_A */
-    int32_t riemann;       /* JFA_HLLC / JFA_RUSANOV / JFA_HLL / JFA_HLLC_LM */
+    int32_t riemann;       /* JFA_HLLC / JFA_RUSANOV / JFA_HLL / JFA_HLLC_LM / JFA_AUSMP */
     int32_t signal_speed;  /* JFA_SS_* (HLLC only) */
     int32_t frozen_state;  /* JFA_FROZEN_* (characteristic variables only) */
     int32_t dtype;         /* JFA_F64 (float64 buffers) or JFA_F32 (float32 buffers) */

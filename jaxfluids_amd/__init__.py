@@ -16,7 +16,7 @@ from .equation_manager import EquationInformation, EquationManager, IdealGas, Ma
 from .halos import HaloManager
 from .initialization import InitializationManager
 from .input_manager import InputManager
-from .riemann_solvers import DICT_RIEMANN_SOLVER, DICT_SIGNAL_SPEEDS, HLL, HLLC, HLLCLM, Rusanov
+from .riemann_solvers import AUSMP, DICT_RIEMANN_SOLVER, DICT_SIGNAL_SPEEDS, HLL, HLLC, HLLCLM, Rusanov
 from .simulation_manager import SimulationManager
 from .space_solver import SpaceSolver
 from .stencils import DICT_SPATIAL_RECONSTRUCTION, TENO5, TENO5A, WENO1, WENO3JS, WENO3Z, WENO5JS, WENO5Z

@@ -15,7 +15,7 @@ import torch
 
 from . import _native
 from .config import precision
-from .riemann_solvers import HLL, HLLC, HLLCLM, Rusanov, SignalSpeed
+from .riemann_solvers import AUSMP, HLL, HLLC, HLLCLM, Rusanov, SignalSpeed
 from .stencils import DICT_SPATIAL_RECONSTRUCTION
 
 
@@ -58,10 +58,11 @@ class HighOrderGodunov(ConvectiveFluxSolver):
         self.frozen_state = godunov.frozen_state
         stencil_name = {cls: name for name, cls in DICT_SPATIAL_RECONSTRUCTION.items()}.get(
             godunov.reconstruction_stencil)
-        riemann_name = {HLLC: "HLLC", HLL: "HLL", HLLCLM: "HLLC-LM", Rusanov: "RUSANOV"}.get(godunov.riemann_solver)
+        riemann_name = {HLLC: "HLLC", HLL: "HLL", HLLCLM: "HLLC-LM", AUSMP: "AUSMP", Rusanov: "RUSANOV"}.get(
+            godunov.riemann_solver)
         if stencil_name is None or riemann_name is None:
             raise NotImplementedError(
-                f"built on this path: stencils {tuple(DICT_SPATIAL_RECONSTRUCTION)} with HLLC, HLLC-LM, HLL or Rusanov")
+                f"built on this path: stencils {tuple(DICT_SPATIAL_RECONSTRUCTION)} with HLLC, HLLC-LM, HLL, AUSM+ or Rusanov")
         ss = godunov.signal_speed
         ss_name = ss.name if isinstance(ss, SignalSpeed) else (ss if isinstance(ss, str) else "EINFELDT")
         self._names = (self.reconstruction_variable, stencil_name, riemann_name, ss_name, self.frozen_state)

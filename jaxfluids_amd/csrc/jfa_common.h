@@ -222,7 +222,7 @@ static inline int check_scheme(const jfa_scheme* s) {
     if (!s) return JFA_EINVAL;
     if (s->recon_var < 0 || s->recon_var > 3) return JFA_EUNSUPPORTED;
     if (s->stencil < JFA_WENO5_JS || s->stencil > JFA_TENO5_A) return JFA_EUNSUPPORTED;
-    if (s->riemann < JFA_HLLC || s->riemann > JFA_HLLC_LM) return JFA_EUNSUPPORTED;
+    if (s->riemann < JFA_HLLC || s->riemann > JFA_AUSMP) return JFA_EUNSUPPORTED;
     if (s->signal_speed < 0 || s->signal_speed > 4) return JFA_EUNSUPPORTED;
     if (s->frozen_state != JFA_FROZEN_ARITHMETIC && s->frozen_state != JFA_FROZEN_ROE) return JFA_EUNSUPPORTED;
     if (s->dtype != JFA_F64 && s->dtype != JFA_F32) return JFA_EUNSUPPORTED;

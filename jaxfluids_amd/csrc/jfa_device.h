@@ -49,7 +49,7 @@ struct Params {
     double eps_js;     // 16 eps: epsilon of the scaled WENO-JS indicators (squared)
     int stencil;       // JFA_WENO5_JS ... JFA_TENO5_A (read by the kStencilExt kernels)
     double eps_precision;  // machine epsilon of the working precision (config/precision.py:36,50)
-    int riemann;           // JFA_HLL / JFA_HLLC_LM (read by the kRiemannExt kernels)
+    int riemann;           // JFA_HLL / JFA_HLLC_LM / JFA_AUSMP (read by the kRiemannExt kernels)
     int signal_speed;  // JFA_SS_*
     int frozen;        // JFA_FROZEN_*
     // dissipative fluxes (jfa_transport); dissip = 0: inviscid
@@ -594,6 +594,37 @@ __device__ __forceinline__ void hllc_lm(const double WL[5], const double WR[5], 
     }
 }
 
+// AUSM+ (AUSMP.py:30-103; Liou 1996) with the arithmetic interface speed of sound
+// a = (c_L + c_R)/2 (:26,54-55), alpha = 3/16, beta = 1/8 (:27-28): split Mach numbers
+// M+(M_L), M-(M_R) and pressures P+(M_L), P-(M_R) (:65-75), and
+//   F = a (max(M, 0) phi_L + min(M, 0) phi_R) + (P+ p_L + P- p_R) e_n,  M = M+ + M-,
+// phi = (rho, rho u, rho v, rho w, E + p) (:85-103).
+template <int AXIS>
+__device__ __forceinline__ void ausmp(const double WL[5], const double WR[5], const double UL[5],
+                                      const double UR[5], const Params& prm, double F[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    constexpr double alpha = 3.0 / 16.0, beta = 1.0 / 8.0;
+    const double a = 0.5 * (sound_speed(WL[4], WL[0], prm.gamma) + sound_speed(WR[4], WR[0], prm.gamma));
+    const double Ml = fdiv(WL[v], a), Mr = fdiv(WR[v], a);
+    const double ql = Ml * Ml - 1.0, qr = Mr * Mr - 1.0;
+    const double lp = Ml + 1.0, rm = Mr - 1.0;
+    const bool sl = fabs(Ml) >= 1.0, sr = fabs(Mr) >= 1.0;
+    const double Mp = sl ? 0.5 * (Ml + fabs(Ml)) : 0.25 * (lp * lp) + beta * (ql * ql);
+    const double Mm = sr ? 0.5 * (Mr - fabs(Mr)) : -0.25 * (rm * rm) - beta * (qr * qr);
+    const double M = Mp + Mm;
+    const double Map = 0.5 * (M + fabs(M)), Mam = 0.5 * (M - fabs(M));
+    const double Pp = sl ? 0.5 * (1.0 + sgn(Ml)) : 0.25 * (lp * lp) * (2.0 - Ml) + alpha * Ml * (ql * ql);
+    const double Pm = sr ? 0.5 * (1.0 - sgn(Mr)) : 0.25 * (rm * rm) * (2.0 + Mr) - alpha * Mr * (qr * qr);
+    const double pa = Pp * WL[4] + Pm * WR[4];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const double phl = i == 4 ? UL[4] + WL[4] : UL[i];
+        const double phr = i == 4 ? UR[4] + WR[4] : UR[i];
+        F[i] = a * (Map * phl + Mam * phr);
+    }
+    F[v] += pa;
+}
+
 // The further Riemann solvers, selected at run time (prm.riemann) in one instantiation of the
 // face kernels, so the HLLC and Rusanov kernels keep their code.
 constexpr int kRiemannExt = 100;
@@ -605,7 +636,8 @@ __device__ __forceinline__ void riemann(const double WL[5], const double WR[5], 
         hllc<AXIS>(WL, WR, UL, UR, prm, F);
     } else if constexpr (RIEMANN == kRiemannExt) {
         if (prm.riemann == JFA_HLL) hll<AXIS>(WL, WR, UL, UR, prm, F);
-        else hllc_lm<AXIS>(WL, WR, UL, UR, prm, F);
+        else if (prm.riemann == JFA_HLLC_LM) hllc_lm<AXIS>(WL, WR, UL, UR, prm, F);
+        else ausmp<AXIS>(WL, WR, UL, UR, prm, F);
     } else {
         rusanov<AXIS>(WL, WR, UL, UR, prm, F);
     }

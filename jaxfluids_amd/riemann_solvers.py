@@ -1,7 +1,8 @@
 """Riemann solvers and signal speeds on the HIP path.
 
 Mirrors solvers/riemann_solvers/riemann_solver.py (RiemannSolver, :13-186),
-HLLC.py (:13-126), HLL.py (:11-61), HLLCLM.py (:13-136), Rusanov.py (:12-47) and the
+HLLC.py (:13-126), HLL.py (:11-61), HLLCLM.py (:13-136), AUSMP.py (:11-103), Rusanov.py
+(:12-47) and the
 DICT_RIEMANN_SOLVER /
 DICT_SIGNAL_SPEEDS registries (solvers/riemann_solvers/__init__.py:16-34).
 The Riemann problem of every face is solved in one HIP kernel
@@ -121,6 +122,18 @@ class HLLCLM(RiemannSolver):
         self.Ma_limit = 0.1
 
 
+class AUSMP(RiemannSolver):
+    """AUSM+ (Liou 1996) with the arithmetic interface speed of sound, alpha = 3/16, beta = 1/8
+    (AUSMP.py:11-103); the signal speed is not used."""
+
+    _kind = "AUSMP"
+
+    def __init__(self, material_manager, equation_manager, signal_speed=None, **kwargs) -> None:
+        super().__init__(material_manager, equation_manager, signal_speed)
+        self.interface_speed_of_sound = "ARITHMETIC"
+        self.alpha, self.beta = 3.0 / 16.0, 1.0 / 8.0
+
+
 class Rusanov(RiemannSolver):
     """Rusanov / local Lax-Friedrichs (Rusanov.py:12-47)."""
 
@@ -131,5 +144,6 @@ DICT_RIEMANN_SOLVER: Dict[str, type] = {
     "HLLC": HLLC,
     "HLL": HLL,
     "HLLC-LM": HLLCLM,
+    "AUSMP": AUSMP,
     "RUSANOV": Rusanov,
 }

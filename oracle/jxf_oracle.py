@@ -641,7 +641,37 @@ def hllc_lm(W_L, W_R, U_L, U_R, axis, gamma, signal_speed="EINFELDT"):
             + 0.25 * (1 - np.sign(S_L)) * (1 + np.sign(S_R)) * flux_star)
 
 
-RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll, "HLLC-LM": hllc_lm}
+def ausmp(W_L, W_R, U_L, U_R, axis, gamma, signal_speed=None):
+    """AUSMP._solve_riemann_problem_xi_single_phase (AUSMP.py:30-83; Liou 1996) with the
+    arithmetic interface speed of sound (:26,54-55), alpha = 3/16, beta = 1/8 (:27-28) and
+    phi = (rho, rho u, rho v, rho w, E + p) (get_phi, :85-103); signal speeds are not used."""
+    v = axis + 1
+    alpha, beta = 3.0 / 16.0, 1.0 / 8.0
+    phi_L = np.stack([U_L[0], U_L[1], U_L[2], U_L[3], U_L[4] + W_L[4]])
+    phi_R = np.stack([U_R[0], U_R[1], U_R[2], U_R[3], U_R[4] + W_R[4]])
+    c_L = speed_of_sound(W_L[4], W_L[0], gamma)
+    c_R = speed_of_sound(W_R[4], W_R[0], gamma)
+    a = 0.5 * (c_L + c_R)
+    M_l = W_L[v] / a
+    M_r = W_R[v] / a
+    M_plus = np.where(np.abs(M_l) >= 1, 0.5 * (M_l + np.abs(M_l)),
+                      0.25 * np.square(M_l + 1.0) + beta * np.square(M_l * M_l - 1.0))
+    M_minus = np.where(np.abs(M_r) >= 1, 0.5 * (M_r - np.abs(M_r)),
+                       -0.25 * np.square(M_r - 1.0) - beta * np.square(M_r * M_r - 1.0))
+    M_ausm = M_plus + M_minus
+    M_ausm_plus = 0.5 * (M_ausm + np.abs(M_ausm))
+    M_ausm_minus = 0.5 * (M_ausm - np.abs(M_ausm))
+    P_plus = np.where(np.abs(M_l) >= 1.0, 0.5 * (1 + np.sign(M_l)),
+                      0.25 * np.square(M_l + 1.0) * (2.0 - M_l) + alpha * M_l * np.square(M_l * M_l - 1.0))
+    P_minus = np.where(np.abs(M_r) >= 1.0, 0.5 * (1 - np.sign(M_r)),
+                       0.25 * np.square(M_r - 1.0) * (2.0 + M_r) - alpha * M_r * np.square(M_r * M_r - 1.0))
+    pressure_ausm = P_plus * W_L[4] + P_minus * W_R[4]
+    F = a * (M_ausm_plus * phi_L + M_ausm_minus * phi_R)
+    F[v] = F[v] + pressure_ausm
+    return F
+
+
+RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll, "HLLC-LM": hllc_lm, "AUSMP": ausmp}
 
 
 # --------------------------------------------------------------------------

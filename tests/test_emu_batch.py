@@ -161,6 +161,7 @@ FURTHER = [  # the run-time stencil switch of the kernel
     ("WENO1", "PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
     ("WENO5-JS", "CHAR-PRIMITIVE", "HLL", "EINFELDT", "ARITHMETIC"),  # HLL Riemann solver
     ("TENO5", "PRIMITIVE", "HLLC-LM", "DAVIS", "ARITHMETIC"),  # HLLC-LM Riemann solver
+    ("WENO5-Z", "CONSERVATIVE", "AUSMP", "EINFELDT", "ARITHMETIC"),  # AUSM+ Riemann solver
 ]
 
 

@@ -175,12 +175,12 @@ def test_fused_step_further_stencils(emu, stencil, variable, grid):
     check_step(emu, dom, s, W0)
 
 
-@pytest.mark.parametrize("riemann", ["HLL", "HLLC-LM"])
+@pytest.mark.parametrize("riemann", ["HLL", "HLLC-LM", "AUSMP"])
 @pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "TORO"])
 @pytest.mark.parametrize("variable,grid", [("CHAR-PRIMITIVE", "3d"), ("PRIMITIVE", "2d-xz")])
 def test_fused_step_further_riemann_solvers(emu, riemann, speed, variable, grid):
-    """HLL (HLL.py:29-61) and HLLC-LM (HLLCLM.py:59-136) in the fused sweeps; the state's Mach
-    numbers straddle HLLC-LM's limit of 0.1."""
+    """HLL (HLL.py:29-61), HLLC-LM (HLLCLM.py:59-136) and AUSM+ (AUSMP.py:30-83) in the fused
+    sweeps; the state's Mach numbers straddle HLLC-LM's limit of 0.1."""
     dom, W0 = grid_state(grid, seed=13)
     s = o.Setup(gamma=1.4, bcs={f: "ZEROGRADIENT" for f in FACES}, cfl=0.5, stencil="WENO5-Z", variable=variable,
                 riemann=riemann, signal_speed=speed, frozen_state="ROE")

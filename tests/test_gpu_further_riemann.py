@@ -1,5 +1,5 @@
-"""GPU parity of the further Riemann solvers -- HLL (solvers/riemann_solvers/HLL.py:29-61) and
-HLLC-LM (HLLCLM.py:59-136) -- against the oracle: the plugin's solve_riemann_problem_xi for every
+"""GPU parity of the further Riemann solvers -- HLL (solvers/riemann_solvers/HLL.py:29-61),
+HLLC-LM (HLLCLM.py:59-136) and AUSM+ (AUSMP.py:30-83) -- against the oracle: the plugin's solve_riemann_problem_xi for every
 signal speed, compute_flux_xi for every reconstruction variable, fused RK3 steps through the setup
 dicts, and the batched ensemble stage.  FP64, relative L-inf <= 1e-12.  The states' Mach numbers
 straddle HLLC-LM's limit of 0.1.  The stage VJP has no reverse mode for them and refuses them
@@ -18,7 +18,7 @@ FACES = ("east", "west", "north", "south", "top", "bottom")
 SPEEDS = ["EINFELDT", "DAVIS", "ARITHMETIC", "TORO", "RUSANOV"]
 
 
-SOLVERS = ["HLL", "HLLC-LM"]
+SOLVERS = ["HLL", "HLLC-LM", "AUSMP"]
 
 
 @pytest.mark.parametrize("solver", SOLVERS)

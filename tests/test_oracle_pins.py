@@ -910,3 +910,38 @@ def test_hllc_lm_consistency_and_upwinding():
         ULs, URs = o.cons_from_prim(WLs, 1.4), o.cons_from_prim(WRs, 1.4)
         up = o.physical_flux_xi(WLs, ULs, 0) if sign > 0 else o.physical_flux_xi(WRs, URs, 0)
         assert np.max(np.abs(o.hllc_lm(WLs, WRs, ULs, URs, 0, 1.4) - up)) < 1e-13 * np.max(np.abs(up))
+
+
+# ------------------------------------------------------------------ AUSM+
+def test_ausmp_consistency_and_upwinding():
+    """The AUSM+ splittings satisfy M+(M) + M-(M) = M and P+(M) + P-(M) = 1 for every M (the
+    beta and alpha terms cancel between the halves), so equal states give the physical flux at
+    any Mach number; supersonic states (|M| >= 1 on both sides, same sign) give the upwind
+    flux."""
+    WL, WR, UL, UR = _riemann_pair()
+    for scale in (0.05, 0.7, 3.0):  # Mach numbers below, near and above 1
+        W = WL.copy()
+        W[1] *= scale
+        U = o.cons_from_prim(W, 1.4)
+        F = o.ausmp(W, W, U, U, 0, 1.4)
+        assert np.max(np.abs(F - o.physical_flux_xi(W, U, 0))) < 1e-13 * np.max(np.abs(F)), scale
+    for sign in (1.0, -1.0):
+        WLs, WRs = WL.copy(), WR.copy()
+        WLs[1] = WRs[1] = sign * 8.0
+        ULs, URs = o.cons_from_prim(WLs, 1.4), o.cons_from_prim(WRs, 1.4)
+        up = o.physical_flux_xi(WLs, ULs, 0) if sign > 0 else o.physical_flux_xi(WRs, URs, 0)
+        assert np.max(np.abs(o.ausmp(WLs, WRs, ULs, URs, 0, 1.4) - up)) < 1e-13 * np.max(np.abs(up))
+
+
+def test_ausmp_splitting_polynomials():
+    """Liou's split Mach number: the subsonic M+(M) = (M+1)^2/4 + beta (M^2-1)^2 meets the
+    supersonic branch M at M = 1, and M-(0) = -1/4 - beta.  With c_L = c_R = 1 (a = 1), rho = 1,
+    u_R = 0 and M_L at (or just below) 1, the interface Mach number is 1 - 1/4 - 1/8 > 0 and the
+    mass flux a M rho_L = 0.625 -- a wrong beta or a discontinuous M+ at 1 changes it."""
+    for Ml in (0.999999, 1.0):
+        WL = np.array([[1.0], [Ml], [0.0], [0.0], [1.0 / 1.4]])  # c = 1
+        WR = np.array([[1.0], [0.0], [0.0], [0.0], [1.0 / 1.4]])
+        UL, UR = o.cons_from_prim(WL, 1.4), o.cons_from_prim(WR, 1.4)
+        F = o.ausmp(WL, WR, UL, UR, 0, 1.4)
+        # M_R = 0: M-(0) = -1/4 - beta, so the interface Mach number is M+(M_L) - 1/4 - 1/8
+        assert abs(F[0, 0] - (1.0 - 0.375)) < 1e-5
