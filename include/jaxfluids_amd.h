@@ -45,6 +45,8 @@ extern "C" {
 #define JFA_WENO3_Z 4
 #define JFA_TENO5 5
 #define JFA_TENO5_A 6
+#define JFA_WENO6_CU 7
+#define JFA_TENO6 8
 /* riemann_solver (solvers/riemann_solvers/__init__.py:16-25) */
 #define JFA_HLLC 0
 #define JFA_RUSANOV 1
@@ -116,7 +118,7 @@ typedef struct jfa_transport {
 /* Numerical scheme of the high-order Godunov flux (data_types/numerical_setup/conservatives.py:35-42). */
 typedef struct jfa_scheme {
     int32_t recon_var;     /* JFA_PRIMITIVE .. JFA_CHAR_CONSERVATIVE */
-    int32_t stencil;       /* JFA_WENO5_JS ... JFA_TENO5_A */
+    int32_t stencil;       /* JFA_WENO5_JS ... JFA_TENO6 */
     int32_t riemann;       /* JFA_HLLC / JFA_RUSANOV / JFA_HLL / JFA_HLLC_LM / JFA_AUSMP */
     int32_t signal_speed;  /* JFA_SS_* (HLLC only) */
     int32_t frozen_state;  /* JFA_FROZEN_* (characteristic variables only) */

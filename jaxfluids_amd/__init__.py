@@ -19,7 +19,8 @@ from .input_manager import InputManager
 from .riemann_solvers import AUSMP, DICT_RIEMANN_SOLVER, DICT_SIGNAL_SPEEDS, HLL, HLLC, HLLCLM, Rusanov
 from .simulation_manager import SimulationManager
 from .space_solver import SpaceSolver
-from .stencils import DICT_SPATIAL_RECONSTRUCTION, TENO5, TENO5A, WENO1, WENO3JS, WENO3Z, WENO5JS, WENO5Z
+from .stencils import (DICT_SPATIAL_RECONSTRUCTION, TENO5, TENO5A, TENO6, WENO1, WENO3JS, WENO3Z, WENO5JS, WENO5Z,
+                       WENO6CU)
 from .time_integration import DICT_TIME_INTEGRATION, RungeKutta3, compute_time_step_size
 
 __version__ = "0.1.0"

@@ -255,6 +255,73 @@ __device__ __forceinline__ double teno5(double a, double b, double c, double d, 
     return fma(num, rcp(w0 + w1 + w2 + prm.eps), c);
 }
 
+// Six-point stencils (weno6_base.py:20-59, teno6_base.py:20-64): the 6-point smoothness indicator is
+// a quadratic form with 8-digit integer coefficients whose terms cancel, so these functions keep
+// the reference's operation order, without contraction into FMAs and with IEEE divisions.  They are
+// called, not inlined, so the many kernel instantiations that can reach them compile quickly.
+__device__ __forceinline__ double beta_six_point(double a, double b, double c, double d, double e, double f) {
+#pragma clang fp contract(off)
+    return 1.0 / 10080 / 12 *
+           (271779 * a * a + a * (-2380800 * b + 4086352 * c - 3462252 * d + 1458762 * e - 245620 * f) +
+            b * (5653317 * b - 20427884 * c + 17905032 * d - 7727988 * e + 1325006 * f) +
+            c * (19510972 * c - 35817664 * d + 15929912 * e - 2792660 * f) +
+            d * (17195652 * d - 15880404 * e + 2863984 * f) + e * (3824847 * e - 1429976 * f) + 139633 * f * f);
+}
+
+__device__ __forceinline__ void beta_three_point(double a, double b, double c, double d, double e, double& b0,
+                                                 double& b1, double& b2) {
+#pragma clang fp contract(off)
+    const double s0 = a - 2 * b + c, t0 = a - 4 * b + 3 * c;
+    const double s1 = b - 2 * c + d, t1 = b - d;
+    const double s2 = c - 2 * d + e, t2 = 3 * c - 4 * d + e;
+    b0 = 13.0 / 12.0 * (s0 * s0) + 1.0 / 4.0 * (t0 * t0);
+    b1 = 13.0 / 12.0 * (s1 * s1) + 1.0 / 4.0 * (t1 * t1);
+    b2 = 13.0 / 12.0 * (s2 * s2) + 1.0 / 4.0 * (t2 * t2);
+}
+
+// WENO6-CU (weno/weno6_cu.py:24-61; Hu, Wang & Adams 2010): d = (1/20, 9/20, 9/20, 1/20), C = 20,
+// tau_6 = beta_3 - (beta_0 + 4 beta_1 + beta_2)/6, alpha_k = d_k (C + tau_6/(beta_k + eps)).
+__device__ __noinline__ double weno6_cu(double a, double b, double c, double d, double e, double f, double eps) {
+#pragma clang fp contract(off)
+    double b0, b1, b2;
+    beta_three_point(a, b, c, d, e, b0, b1, b2);
+    const double b3 = beta_six_point(a, b, c, d, e, f);
+    const double tau6 = b3 - 1.0 / 6 * (b0 + 4 * b1 + b2);
+    const double a0 = 1.0 / 20 * (20 + tau6 / (b0 + eps)), a1 = 9.0 / 20 * (20 + tau6 / (b1 + eps));
+    const double a2 = 9.0 / 20 * (20 + tau6 / (b2 + eps)), a3 = 1.0 / 20 * (20 + tau6 / (b3 + eps));
+    const double one_alpha = 1.0 / (a0 + a1 + a2 + a3);
+    const double p0 = 1.0 / 3 * a + -7.0 / 6 * b + 11.0 / 6 * c;
+    const double p1 = -1.0 / 6 * b + 5.0 / 6 * c + 1.0 / 3 * d;
+    const double p2 = 1.0 / 3 * c + 5.0 / 6 * d + -1.0 / 6 * e;
+    const double p3 = 11.0 / 6 * d + -7.0 / 6 * e + 1.0 / 3 * f;
+    return a0 * one_alpha * p0 + a1 * one_alpha * p1 + a2 * one_alpha * p2 + a3 * one_alpha * p3;
+}
+
+// TENO6 (teno/teno6.py:27-67, teno6_base.py:20-64): three 3-point candidates and a 4-point one
+// (3/12, 13/12, -5/12, 1/12) with its own indicator, tau_6 = |beta_6 - (beta_0 + 4 beta_1 + beta_2)/6|,
+// gamma_k = (1 + tau_6/(beta_k + eps))^6, sharp cutoff 1e-7, d = (0.05, 0.45, 0.30, 0.20).
+__device__ __noinline__ double teno6(double a, double b, double c, double d, double e, double f, double eps) {
+#pragma clang fp contract(off)
+    double b0, b1, b2;
+    beta_three_point(a, b, c, d, e, b0, b1, b2);
+    const double b3 = 1.0 / 240.0 *
+                      (c * (2107 * c - 9402 * d + 7042 * e - 1854 * f) + d * (11003 * d - 17246 * e + 4642 * f) +
+                       e * (7043 * e - 3882 * f) + 547 * f * f);
+    const double b6 = beta_six_point(a, b, c, d, e, f);
+    const double tau6 = fabs(b6 - 1.0 / 6 * (b0 + 4 * b1 + b2));
+    const double g0 = pow(1.0 + tau6 / (b0 + eps), 6.0), g1 = pow(1.0 + tau6 / (b1 + eps), 6.0);
+    const double g2 = pow(1.0 + tau6 / (b2 + eps), 6.0), g3 = pow(1.0 + tau6 / (b3 + eps), 6.0);
+    const double one_gamma_sum = 1.0 / (g0 + g1 + g2 + g3);
+    const double w0 = g0 * one_gamma_sum < 1e-7 ? 0.0 : 0.050, w1 = g1 * one_gamma_sum < 1e-7 ? 0.0 : 0.450;
+    const double w2 = g2 * one_gamma_sum < 1e-7 ? 0.0 : 0.300, w3 = g3 * one_gamma_sum < 1e-7 ? 0.0 : 0.200;
+    const double one_dk = 1.0 / (w0 + w1 + w2 + w3 + eps);
+    const double p0 = 1.0 / 3 * a + -7.0 / 6 * b + 11.0 / 6 * c;
+    const double p1 = -1.0 / 6 * b + 5.0 / 6 * c + 1.0 / 3 * d;
+    const double p2 = 1.0 / 3 * c + 5.0 / 6 * d + -1.0 / 6 * e;
+    const double p3 = 3.0 / 12 * c + 13.0 / 12 * d + -5.0 / 12 * e + 1.0 / 12 * f;
+    return w0 * one_dk * p0 + w1 * one_dk * p1 + w2 * one_dk * p2 + w3 * one_dk * p3;
+}
+
 // Left and right states of one face from its 6-cell window (cells -3..2).
 template <int STENCIL>
 __device__ __forceinline__ void recon_pair(const double v[6], const Params& prm, double& L, double& R) {
@@ -278,6 +345,14 @@ __device__ __forceinline__ void recon_pair(const double v[6], const Params& prm,
                 L = teno5<false>(v[0], v[1], v[2], v[3], v[4], prm);
                 R = teno5<false>(v[5], v[4], v[3], v[2], v[1], prm);
                 break;
+            case JFA_WENO6_CU:  // weno6_base.py:27: cells -3..2 and 2..-3
+                L = weno6_cu(v[0], v[1], v[2], v[3], v[4], v[5], prm.eps);
+                R = weno6_cu(v[5], v[4], v[3], v[2], v[1], v[0], prm.eps);
+                break;
+            case JFA_TENO6:
+                L = teno6(v[0], v[1], v[2], v[3], v[4], v[5], prm.eps);
+                R = teno6(v[5], v[4], v[3], v[2], v[1], v[0], prm.eps);
+                break;
             default:  // JFA_TENO5_A
                 L = teno5<true>(v[0], v[1], v[2], v[3], v[4], prm);
                 R = teno5<true>(v[5], v[4], v[3], v[2], v[1], prm);
@@ -286,9 +361,11 @@ __device__ __forceinline__ void recon_pair(const double v[6], const Params& prm,
     }
 }
 
-// One side from the 5-point stencil (u_imm, u_im, u_i, u_ip, u_ipp) of WENO5Base's slices.
+// One side from the 6-point stencil (u_imm, u_im, u_i, u_ip, u_ipp, u_ippp) of WENO6Base's slices
+// (the 5-point stencils read the first five).
 template <int STENCIL>
-__device__ __forceinline__ double recon_side(double a, double b, double c, double d, double e, const Params& prm) {
+__device__ __forceinline__ double recon_side(double a, double b, double c, double d, double e, double f,
+                                             const Params& prm) {
     if constexpr (STENCIL != kStencilExt) {
         return weno5<STENCIL>(a, b, c, d, e, prm.eps_z, prm.eps_js);
     } else {
@@ -297,6 +374,8 @@ __device__ __forceinline__ double recon_side(double a, double b, double c, doubl
             case JFA_WENO3_JS: return weno3<false>(b, c, d, prm.eps);
             case JFA_WENO3_Z: return weno3<true>(b, c, d, prm.eps);
             case JFA_TENO5: return teno5<false>(a, b, c, d, e, prm);
+            case JFA_WENO6_CU: return weno6_cu(a, b, c, d, e, f, prm.eps);
+            case JFA_TENO6: return teno6(a, b, c, d, e, f, prm.eps);
             default: return teno5<true>(a, b, c, d, e, prm);
         }
     }

@@ -29,10 +29,10 @@ __global__ __launch_bounds__(kBlock) void reconstruct_kernel(const T* __restrict
     for (int v = 0; v < nvar; ++v) {
         const T* b = buf + v * G.fs + base;
         double r;
-        if (j == 0)  // cells -3..1 (weno5_base.py:30; the narrower stencils use the inner cells)
-            r = recon_side<STENCIL>(ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), prm);
-        else  // cells 2..-2
-            r = recon_side<STENCIL>(ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), prm);
+        if (j == 0)  // cells -3..2 (weno6_base.py:27; the 5-point and narrower stencils use -3..1 and inner cells)
+            r = recon_side<STENCIL>(ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), ld(b, 2 * sa), prm);
+        else  // cells 2..-3
+            r = recon_side<STENCIL>(ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), ld(b, -3 * sa), prm);
         out[(int64_t)v * F.nface + o] = static_cast<T>(r);
     }
 }
@@ -333,7 +333,7 @@ int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* g
     if (rc) return rc;
     if (!buffer || !out || nvar < 1 || axis < 0 || axis > 2 || !G.active[axis] || (j != 0 && j != 1))
         return JFA_EINVAL;
-    if (stencil < JFA_WENO5_JS || stencil > JFA_TENO5_A) return JFA_EUNSUPPORTED;
+    if (stencil < JFA_WENO5_JS || stencil > JFA_TENO6) return JFA_EUNSUPPORTED;
     const FaceIdx F = face_extents(G, axis);
     dim3 g = flat_grid(F.nface);
     hipStream_t st = (hipStream_t)stream;

@@ -140,6 +140,33 @@ class TENO5A(WENO5Base):
     _variant = _native.STENCIL["TENO5-A"]
 
 
+class WENO6CU(ShockCapturingStencil):
+    """Central-upwind WENO6 (Hu, Wang & Adams 2010): d = (1/20, 9/20, 9/20, 1/20), C = 20, on the
+    6-cell window (weno6_base.py:10-59, weno/weno6_cu.py:10-61)."""
+
+    required_halos = 3
+    _variant = _native.STENCIL["WENO6-CU"]
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self._dr = (1 / 20, 9 / 20, 9 / 20, 1 / 20)
+        self._stencil_size = 6
+        self.C_ = 20
+
+
+class TENO6(ShockCapturingStencil):
+    """TENO6 (teno6_base.py:10-64, teno/teno6.py:10-67): d = (0.05, 0.45, 0.30, 0.20), CT = 1e-7."""
+
+    required_halos = 3
+    _variant = _native.STENCIL["TENO6"]
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self._dr = (0.050, 0.450, 0.300, 0.200)
+        self._stencil_size = 6
+        self.C, self.q, self.CT = 1.0, 6, 1e-7
+
+
 class WENO3Base(ShockCapturingStencil):
     """weno3_base.py:10-45: d = (1/3, 2/3), 4-cell window, 2 halos."""
 
@@ -186,4 +213,6 @@ DICT_SPATIAL_RECONSTRUCTION: Dict[str, type] = {
     "WENO5-Z": WENO5Z,
     "TENO5": TENO5,
     "TENO5-A": TENO5A,
+    "WENO6-CU": WENO6CU,
+    "TENO6": TENO6,
 }

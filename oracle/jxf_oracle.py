@@ -266,6 +266,82 @@ WENO5_POINT = {"WENO5-JS": weno5_js_point, "WENO5-Z": weno5_z_point, "WENO1": we
                "TENO5-A": teno5_a_point}
 
 
+# --------------------------------------------------------------------------
+# Six-point stencils (weno6_base.py, teno6_base.py): array_slices([range(-3,3,1),
+# range(2,-4,-1)]) -- the whole 6-cell window, mirrored for the right state.  The smoothness
+# indicator of the 6-point stencil is a quadratic form with 8-digit integer coefficients whose
+# terms cancel; it is evaluated in the reference's operation order.
+# --------------------------------------------------------------------------
+WENO6_SHIFTS = ((-3, -2, -1, 0, 1, 2), (2, 1, 0, -1, -2, -3))
+WENO6_WINDOW_IDX = ((0, 1, 2, 3, 4, 5), (5, 4, 3, 2, 1, 0))
+_CR6 = ((1 / 3, -7 / 6, 11 / 6), (-1 / 6, 5 / 6, 1 / 3), (1 / 3, 5 / 6, -1 / 6), (11 / 6, -7 / 6, 1 / 3))  # weno6_base.py:23-26
+
+
+def _beta_six_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp):
+    """weno6_base.py:37-45 (beta_3) = teno6_base.py:45-53 (beta_6)."""
+    return 1.0 / 10080 / 12 * (
+        271779 * u_imm * u_imm +
+        u_imm * (-2380800 * u_im + 4086352 * u_i - 3462252 * u_ip + 1458762 * u_ipp - 245620 * u_ippp) +
+        u_im * (5653317 * u_im - 20427884 * u_i + 17905032 * u_ip - 7727988 * u_ipp + 1325006 * u_ippp) +
+        u_i * (19510972 * u_i - 35817664 * u_ip + 15929912 * u_ipp - 2792660 * u_ippp) +
+        u_ip * (17195652 * u_ip - 15880404 * u_ipp + 2863984 * u_ippp) +
+        u_ipp * (3824847 * u_ipp - 1429976 * u_ippp) +
+        139633 * u_ippp * u_ippp)
+
+
+def _beta_three_point(u_imm, u_im, u_i, u_ip, u_ipp):
+    """weno6_base.py:34-36 (the WENO5 indicators in the reference's form)."""
+    beta_0 = 13.0 / 12.0 * np.square(u_imm - 2 * u_im + u_i) + 1.0 / 4.0 * np.square(u_imm - 4 * u_im + 3 * u_i)
+    beta_1 = 13.0 / 12.0 * np.square(u_im - 2 * u_i + u_ip) + 1.0 / 4.0 * np.square(u_im - u_ip)
+    beta_2 = 13.0 / 12.0 * np.square(u_i - 2 * u_ip + u_ipp) + 1.0 / 4.0 * np.square(3 * u_i - 4 * u_ip + u_ipp)
+    return beta_0, beta_1, beta_2
+
+
+def weno6_cu_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno6_cu.py:24-61 (Hu, Wang & Adams 2010): d = (1/20, 9/20, 9/20, 1/20)
+    (weno6_base.py:22), C = 20 (:16), tau_6 = beta_3 - (beta_0 + 4 beta_1 + beta_2)/6."""
+    beta_0, beta_1, beta_2 = _beta_three_point(u_imm, u_im, u_i, u_ip, u_ipp)
+    beta_3 = _beta_six_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp)
+    tau_6 = beta_3 - 1 / 6 * (beta_0 + 4 * beta_1 + beta_2)
+    dr, C = (1 / 20, 9 / 20, 9 / 20, 1 / 20), 20
+    alpha = [dr[k] * (C + tau_6 / (b + eps)) for k, b in enumerate((beta_0, beta_1, beta_2, beta_3))]
+    one_alpha = 1.0 / (alpha[0] + alpha[1] + alpha[2] + alpha[3])
+    p_0 = _CR6[0][0] * u_imm + _CR6[0][1] * u_im + _CR6[0][2] * u_i
+    p_1 = _CR6[1][0] * u_im + _CR6[1][1] * u_i + _CR6[1][2] * u_ip
+    p_2 = _CR6[2][0] * u_i + _CR6[2][1] * u_ip + _CR6[2][2] * u_ipp
+    p_3 = _CR6[3][0] * u_ip + _CR6[3][1] * u_ipp + _CR6[3][2] * u_ippp
+    return (alpha[0] * one_alpha * p_0 + alpha[1] * one_alpha * p_1 + alpha[2] * one_alpha * p_2
+            + alpha[3] * one_alpha * p_3)
+
+
+def teno6_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS):
+    """teno/teno6.py:27-67 with teno6_base.py:20-64: three 3-point candidates and one 4-point
+    candidate (3/12, 13/12, -5/12, 1/12) with its own indicator beta_3 (:40-44), the 6-point
+    indicator beta_6 for tau_6 = |beta_6 - (beta_0 + 4 beta_1 + beta_2)/6|, gamma_k =
+    (1 + tau_6/(beta_k + eps))^6, cutoff CT = 1e-7, d = (0.05, 0.45, 0.30, 0.20)."""
+    beta_0, beta_1, beta_2 = _beta_three_point(u_imm, u_im, u_i, u_ip, u_ipp)
+    beta_3 = 1.0 / 240.0 * (
+        u_i * (2107 * u_i - 9402 * u_ip + 7042 * u_ipp - 1854 * u_ippp)
+        + u_ip * (11003 * u_ip - 17246 * u_ipp + 4642 * u_ippp)
+        + u_ipp * (7043 * u_ipp - 3882 * u_ippp)
+        + 547 * u_ippp * u_ippp)
+    beta_6 = _beta_six_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp)
+    tau_6 = np.abs(beta_6 - 1 / 6 * (beta_0 + 4 * beta_1 + beta_2))
+    gamma = [(1.0 + tau_6 / (b + eps)) ** 6 for b in (beta_0, beta_1, beta_2, beta_3)]
+    one_gamma_sum = 1.0 / (gamma[0] + gamma[1] + gamma[2] + gamma[3])
+    dr = (0.050, 0.450, 0.300, 0.200)
+    w = [dr[k] * np.where(gamma[k] * one_gamma_sum < 1e-7, 0.0, 1.0) for k in range(4)]
+    one_dk = 1.0 / (w[0] + w[1] + w[2] + w[3] + eps)
+    p_0 = _CR6[0][0] * u_imm + _CR6[0][1] * u_im + _CR6[0][2] * u_i
+    p_1 = _CR6[1][0] * u_im + _CR6[1][1] * u_i + _CR6[1][2] * u_ip
+    p_2 = _CR6[2][0] * u_i + _CR6[2][1] * u_ip + _CR6[2][2] * u_ipp
+    p_3 = 3 / 12 * u_i + 13 / 12 * u_ip + -5 / 12 * u_ipp + 1 / 12 * u_ippp
+    return (w[0] * one_dk * p_0 + w[1] * one_dk * p_1 + w[2] * one_dk * p_2 + w[3] * one_dk * p_3)
+
+
+WENO6_POINT = {"WENO6-CU": weno6_cu_point, "TENO6": teno6_point}
+
+
 def weno5_reconstruct_xi(buffer, dom: Domain, axis: int, j: int, stencil: str = "WENO5-JS",
                          eps=SPATIAL_STENCIL_EPS):
     """SpatialReconstruction.reconstruct_xi for WENO5 on a cell buffer.
@@ -273,6 +349,9 @@ def weno5_reconstruct_xi(buffer, dom: Domain, axis: int, j: int, stencil: str = 
     weno/weno5_js.py:23-37 / weno5_z.py:24-36: the five operands are the buffer
     shifted by WENO5_SHIFTS[j] along axis (spatial_stencil.py:45-112).
     """
+    if stencil in WENO6_POINT:
+        ops = [buffer[shift_slice(dom, axis, k)] for k in WENO6_SHIFTS[j]]
+        return WENO6_POINT[stencil](*ops, eps=eps)
     ops = [buffer[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[j]]
     return WENO5_POINT[stencil](*ops, eps=eps)
 
@@ -282,6 +361,9 @@ def weno5_reconstruct_window(window, j: int, stencil: str, eps=SPATIAL_STENCIL_E
 
     ``window`` has the 6-cell stencil window on axis 1: shape (nvar, 6, *face).
     """
+    if stencil in WENO6_POINT:
+        ops = [window[:, i] for i in WENO6_WINDOW_IDX[j]]
+        return WENO6_POINT[stencil](*ops, eps=eps)
     ops = [window[:, i] for i in WENO5_WINDOW_IDX[j]]
     return WENO5_POINT[stencil](*ops, eps=eps)
 

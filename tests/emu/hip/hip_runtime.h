@@ -18,6 +18,7 @@
 #define __device__
 #define __host__
 #define __forceinline__ inline
+#define __noinline__ inline __attribute__((noinline))
 #define __launch_bounds__(...)
 #define __shared__ static
 

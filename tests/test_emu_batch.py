@@ -159,6 +159,8 @@ FURTHER = [  # the run-time stencil switch of the kernel
     ("WENO3-Z", "CHAR-PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
     ("WENO3-JS", "CONSERVATIVE", "RUSANOV", "EINFELDT", "ARITHMETIC"),
     ("WENO1", "PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
+    ("WENO6-CU", "CHAR-PRIMITIVE", "HLLC", "EINFELDT", "ROE"),
+    ("TENO6", "PRIMITIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
     ("WENO5-JS", "CHAR-PRIMITIVE", "HLL", "EINFELDT", "ARITHMETIC"),  # HLL Riemann solver
     ("TENO5", "PRIMITIVE", "HLLC-LM", "DAVIS", "ARITHMETIC"),  # HLLC-LM Riemann solver
     ("WENO5-Z", "CONSERVATIVE", "AUSMP", "EINFELDT", "ARITHMETIC"),  # AUSM+ Riemann solver

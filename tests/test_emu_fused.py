@@ -161,14 +161,15 @@ def test_fused_step_with_limiters(emu, fl):
     assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL
 
 
-NEW_STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A"]
+NEW_STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6"]
 
 
 @pytest.mark.parametrize("stencil", NEW_STENCILS)
 @pytest.mark.parametrize("variable,grid", [("PRIMITIVE", "3d"), ("CHAR-PRIMITIVE", "2d-xy"),
                                            ("CHAR-CONSERVATIVE", "2d-yz"), ("CONSERVATIVE", "1d")])
 def test_fused_step_further_stencils(emu, stencil, variable, grid):
-    """WENO1, WENO3-JS/-Z, TENO5 and TENO5-A (the run-time stencil switch of the kernels)."""
+    """WENO1, WENO3-JS/-Z, TENO5, TENO5-A, WENO6-CU and TENO6 (the run-time stencil switch of the
+    kernels)."""
     dom, W0 = grid_state(grid, seed=11)
     s = o.Setup(gamma=1.4, bcs={f: "ZEROGRADIENT" for f in FACES}, cfl=0.5, stencil=stencil, variable=variable,
                 riemann="HLLC", signal_speed="EINFELDT", frozen_state="ROE")

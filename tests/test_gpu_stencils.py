@@ -1,6 +1,6 @@
-"""GPU parity of the further reconstruction stencils -- WENO1, WENO3-JS, WENO3-Z, TENO5, TENO5-A
-(stencils/reconstruction/shock_capturing/weno/weno1_js.py, weno3_js.py, weno3_z.py,
-teno/teno5.py, teno5_a.py) -- against the oracle: the plugin's reconstruct_xi, compute_flux_xi for
+"""GPU parity of the further reconstruction stencils -- WENO1, WENO3-JS, WENO3-Z, TENO5, TENO5-A,
+WENO6-CU, TENO6 (stencils/reconstruction/shock_capturing/weno/weno1_js.py, weno3_js.py, weno3_z.py,
+weno6_cu.py, teno/teno5.py, teno5_a.py, teno6.py) -- against the oracle: the plugin's reconstruct_xi, compute_flux_xi for
 every reconstruction variable, RK3 steps of the fused stage through the setup dicts (3-D and 2-D),
 and the batched ensemble stage.  FP64, relative L-inf <= 1e-12.
 
@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 FACES = ("east", "west", "north", "south", "top", "bottom")
-STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A"]
+STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6"]
 
 
 @pytest.mark.parametrize("stencil", STENCILS)

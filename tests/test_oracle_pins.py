@@ -945,3 +945,41 @@ def test_ausmp_splitting_polynomials():
         F = o.ausmp(WL, WR, UL, UR, 0, 1.4)
         # M_R = 0: M-(0) = -1/4 - beta, so the interface Mach number is M+(M_L) - 1/4 - 1/8
         assert abs(F[0, 0] - (1.0 - 0.375)) < 1e-5
+
+
+# ------------------------------------------------------------------ WENO6-CU, TENO6
+def _left_states6(stencil, F, f, N, periodic):
+    x = np.linspace(0.0, 1.0, N + 1)
+    a = (F(x[1:]) - F(x[:-1])) * N
+    if periodic:
+        ap, i, faces = np.concatenate([a[-3:], a, a[:3]]), np.arange(3, N + 3), x[1:]
+    else:
+        ap, i = a, np.arange(2, N - 3)
+        faces = x[i + 1]
+    ops = [ap[i + k] for k in (-2, -1, 0, 1, 2, 3)]  # WENO6_SHIFTS[0] around the face's left cell
+    return o.WENO6_POINT[stencil](*ops), f(faces)
+
+
+@pytest.mark.parametrize("stencil", ["WENO6-CU", "TENO6"])
+def test_six_point_stencils_reproduce_quadratics_and_are_sixth_order(stencil):
+    """Every candidate reproduces quadratics (so any normalised weighting does); on a smooth sine
+    the weights approach d, and both d = (1/20, 9/20, 9/20, 1/20) of WENO6-CU and
+    (0.05, 0.45, 0.30, 0.20) of TENO6 (with its 4-point candidate) make the sixth-order central
+    reconstruction: order 6 and the same error."""
+    Fq = lambda x: 0.3 * x + 0.7 * x ** 2 / 2 - 1.9 * x ** 3 / 3  # noqa: E731
+    fq = lambda x: 0.3 + 0.7 * x - 1.9 * x ** 2  # noqa: E731
+    L, exact = _left_states6(stencil, Fq, fq, 40, periodic=False)
+    assert np.max(np.abs(L - exact)) < 1e-13
+    F = lambda x: -np.cos(2 * np.pi * x) / (2 * np.pi)  # noqa: E731
+    f = lambda x: np.sin(2 * np.pi * x)  # noqa: E731
+    errs = [np.max(np.abs(np.subtract(*_left_states6(stencil, F, f, N, periodic=True)))) for N in (32, 64, 128)]
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(np.abs(rates - 6.0) < 0.05), rates
+
+
+def test_teno6_cutoff_and_weno6_cu_at_a_step():
+    """At a unit step TENO6 cuts every candidate that crosses it and returns the smooth side's
+    value exactly; WENO6-CU's central-upwind weights only make them small."""
+    z, one = np.zeros(1), np.ones(1)
+    assert o.WENO6_POINT["TENO6"](z, z, z, one, one, one)[0] == 0.0
+    assert 0.0 < abs(o.WENO6_POINT["WENO6-CU"](z, z, z, one, one, one)[0]) < 1e-20
