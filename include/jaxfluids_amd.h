@@ -47,12 +47,24 @@ extern "C" {
 #define JFA_TENO5_A 6
 #define JFA_WENO6_CU 7
 #define JFA_TENO6 8
+#define JFA_WENO6_CUM1 9
+#define JFA_WENO6_CUM2 10
 /* riemann_solver (solvers/riemann_solvers/__init__.py:16-25) */
 #define JFA_HLLC 0
 #define JFA_RUSANOV 1
 #define JFA_HLL 2
 #define JFA_HLLC_LM 3
 #define JFA_AUSMP 4
+/* convective_solver FLUX-SPLITTING (solvers/convective_fluxes/flux_splitting_scheme.py:21-111) in
+ * place of a Riemann solver, with the eigenvalues its flux_splitting names
+ * (solvers/__init__.py:1-3; eigendecomposition.py:672-696): the window's conservatives and
+ * physical fluxes are projected on the conservative eigenvectors of the frozen state
+ * (frozen_state), split by those eigenvalues and reconstructed with `stencil` (the positive part
+ * from the left, the negative from the right).  recon_var, signal_speed and the interpolation
+ * limiter do not apply, as in the reference. */
+#define JFA_FS_ROE 5
+#define JFA_FS_CLLF 6
+#define JFA_FS_LLF 7
 /* signal_speed (solvers/riemann_solvers/__init__.py:27-34; DAVIS2 is not callable from HLLC there) */
 #define JFA_SS_ARITHMETIC 0
 #define JFA_SS_RUSANOV 1
@@ -118,8 +130,8 @@ typedef struct jfa_transport {
 /* Numerical scheme of the high-order Godunov flux (data_types/numerical_setup/conservatives.py:35-42). */
 typedef struct jfa_scheme {
     int32_t recon_var;     /* JFA_PRIMITIVE .. JFA_CHAR_CONSERVATIVE */
-    int32_t stencil;       /* JFA_WENO5_JS ... JFA_TENO6 */
-    int32_t riemann;       /* JFA_HLLC / JFA_RUSANOV / JFA_HLL / JFA_HLLC_LM / JFA_AUSMP */
+    int32_t stencil;       /* JFA_WENO5_JS ... JFA_WENO6_CUM2 */
+    int32_t riemann;       /* JFA_HLLC / JFA_RUSANOV / JFA_HLL / JFA_HLLC_LM / JFA_AUSMP, or JFA_FS_* */
     int32_t signal_speed;  /* JFA_SS_* (HLLC only) */
     int32_t frozen_state;  /* JFA_FROZEN_* (characteristic variables only) */
     int32_t dtype;         /* JFA_F64 (float64 buffers) or JFA_F32 (float32 buffers) */
@@ -161,7 +173,9 @@ const char* jfa_last_error(void);
  * Replaces WENO5JS.reconstruct_xi (stencils/reconstruction/shock_capturing/weno/weno5_js.py:73-115),
  * WENO5Z.reconstruct_xi (.../weno/weno5_z.py:138-173), WENO1 / WENO3JS / WENO3Z
  * (.../weno/weno1_js.py:25-30, weno3_js.py:15-42, weno3_z.py:21-41), TENO5 / TENO5A
- * (.../teno/teno5.py:32-73, teno5_a.py:42-99) with the slice tables of WENO5Base / WENO3Base / WENO1
+ * (.../teno/teno5.py:32-73, teno5_a.py:42-99), WENO6CU / WENO6CUM1 / WENO6CUM2 / TENO6
+ * (.../weno/weno6_cu.py:24-61, weno6_cum1.py:28-67, weno6_cum2.py:29-72, teno/teno6.py:27-67; the
+ * CUM stencils read the grid's cell size along `axis` as reconstruct_xi's dx) with the slice tables of WENO5Base / WENO3Base / WENO1
  * (.../weno5_base.py:30, weno3_base.py:29, weno/weno1_js.py:22) and SpatialStencil.array_slices
  * (stencils/spatial_stencil.py:45-112).
  * buffer: (nvar, buffer shape) of `dtype`; out: (nvar, face shape). */
@@ -171,7 +185,8 @@ int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* g
 
 /* Riemann flux from reconstructed face states, all (5, nfaces) contiguous.
  * Replaces RiemannSolver.solve_riemann_problem_xi (solvers/riemann_solvers/riemann_solver.py:52-102)
- * for HLLC (HLLC.py:80-126) and Rusanov (Rusanov.py:25-47), single phase. */
+ * for HLLC (HLLC.py:80-126) and Rusanov (Rusanov.py:25-47), single phase.  A flux-splitting
+ * scheme (JFA_FS_*) has no face states: JFA_EUNSUPPORTED. */
 int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void* U_R, int64_t nfaces,
                    int32_t axis, const jfa_scheme* scheme, void* flux, void* stream);
 
@@ -180,7 +195,8 @@ int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void
  * flux: (5, face shape).  limiter_count: NULL or one device int64 to which the number of limiter
  * replacements is added (the count_interpolation_limiter the reference returns).
  * Replaces HighOrderGodunov.compute_flux_xi (solvers/convective_fluxes/high_order_godunov.py:117-231)
- * with reconstruct_xi (:233-419). */
+ * with reconstruct_xi (:233-419), and for riemann = JFA_FS_* FluxSplittingScheme.compute_flux_xi
+ * (solvers/convective_fluxes/flux_splitting_scheme.py:65-111; limiter_count is then unchanged). */
 int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme,
                         int32_t axis, void* flux, int64_t* limiter_count, void* stream);
 

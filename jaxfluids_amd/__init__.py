@@ -10,7 +10,7 @@ in PyTorch-ROCm device tensors.
 """
 from . import _native
 from .config import precision
-from .convective_fluxes import DICT_CONVECTIVE_SOLVER, HighOrderGodunov
+from .convective_fluxes import DICT_CONVECTIVE_SOLVER, FluxSplittingScheme, HighOrderGodunov
 from .domain import DomainInformation
 from .equation_manager import EquationInformation, EquationManager, IdealGas, MaterialManager
 from .halos import HaloManager

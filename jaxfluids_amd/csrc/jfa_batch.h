@@ -150,26 +150,30 @@ __device__ __forceinline__ void face_flux(const T* __restrict__ W, const T* __re
     for (int k = 0; k < 6; ++k)
 #pragma unroll
         for (int i = 0; i < 5; ++i) win[k][i] = ldo(src + (rb + i * fs + (int64_t)(k - 3) * st), voff);
-    double wl[5], wr[5];
-    if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
+    if constexpr (RIEMANN == kFluxSplit) {
+        flux_split_face<AXIS, STENCIL>(win, prm, F);
+    } else {
+        double wl[5], wr[5];
+        if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            wl[i] = ldo(W + (rb + i * fs - st), voff);
-            wr[i] = ldo(W + (rb + i * fs), voff);
+            for (int i = 0; i < 5; ++i) {
+                wl[i] = ldo(W + (rb + i * fs - st), voff);
+                wr[i] = ldo(W + (rb + i * fs), voff);
+            }
         }
-    }
-    double WL[5], WR[5], UL[5], UR[5];
-    face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
-    if (prm.limiter) {
-        double c[5];
+        double WL[5], WR[5], UL[5], UR[5];
+        face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
+        if (prm.limiter) {
+            double c[5];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) c[i] = ldo(W + (rb + i * fs - st), voff);
-        limit_side<VAR>(c, prm, WL, UL);
+            for (int i = 0; i < 5; ++i) c[i] = ldo(W + (rb + i * fs - st), voff);
+            limit_side<VAR>(c, prm, WL, UL);
 #pragma unroll
-        for (int i = 0; i < 5; ++i) c[i] = ldo(W + (rb + i * fs), voff);
-        limit_side<VAR>(c, prm, WR, UR);
+            for (int i = 0; i < 5; ++i) c[i] = ldo(W + (rb + i * fs), voff);
+            limit_side<VAR>(c, prm, WR, UR);
+        }
+        riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, F);
     }
-    riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, F);
     if constexpr (FLIM) {  // positivity flux limiter with the case's dt (set per item)
         double wl_[5], wr_[5], ul_[5], ur_[5];
 #pragma unroll
@@ -192,6 +196,9 @@ __device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& pr
     const Case2D cs = load_case(a.cases + it.c);
     // the flux limiter's lam = dt (1/dx) sigma uses this case's dt and cell sizes
     Params prm = prm_in;
+    const double dx0 = 1.0 / cs.one_dx0, dx1 = 1.0 / cs.one_dx1;  // this case's cell sizes
+    prm.dx2[0] = dx0 * dx0;
+    prm.dx2[1] = dx1 * dx1;
     if constexpr (FLIM) {
         prm.dt_val = uni(a.dt[it.c]);
         const double sum = cs.one_dx0 + cs.one_dx1;
@@ -290,7 +297,7 @@ template <class T, int VAR, int STENCIL, int RIEMANN, bool FLIM>
 // Three waves per SIMD (at most 168 VGPRs) for the variants that fit without spilling; the
 // dense 5x5 projections of CHAR-CONSERVATIVE need more registers and keep two.
 __global__ __launch_bounds__(kBatchBlock)
-JFA_WAVES_PER_EU(VAR == JFA_CHAR_CONSERVATIVE ? 1 : 3)
+JFA_WAVES_PER_EU((VAR == JFA_CHAR_CONSERVATIVE || RIEMANN == kFluxSplit) ? 1 : 3)
 void batch2d_stage_kernel(BatchArgs<T> a, Params prm) {
     const unsigned idx = blockIdx.x * (kBatchBlock / 64) + (threadIdx.x >> 6);  // one item per wavefront
     if (idx >= (unsigned)a.nitems) return;  // whole wavefronts only; no barriers follow
@@ -358,7 +365,9 @@ static void launch_batch(const BatchArgs<T>& a, const Params& prm, hipStream_t s
 template <class T, int VAR, int STENCIL>
 static void launch_batch_r(int riemann, const BatchArgs<T>& a, const Params& prm, hipStream_t st) {
     if (riemann == JFA_HLLC) launch_batch<T, VAR, STENCIL, JFA_HLLC>(a, prm, st);
-    else if (riemann != JFA_RUSANOV) launch_batch<T, VAR, STENCIL, kRiemannExt>(a, prm, st);
+    else if (riemann >= JFA_FS_ROE) {  // flux splitting reads the primitive window
+        if constexpr (VAR == JFA_PRIMITIVE) launch_batch<T, VAR, STENCIL, kFluxSplit>(a, prm, st);
+    } else if (riemann != JFA_RUSANOV) launch_batch<T, VAR, STENCIL, kRiemannExt>(a, prm, st);
     else launch_batch<T, VAR, STENCIL, JFA_RUSANOV>(a, prm, st);
 }
 template <class T, int VAR>
@@ -385,7 +394,7 @@ int batch2d_stage_impl(const void* W_in, const void* U_in, const void* U_n, void
     a.dt = dt_dev;
     a.part = dt_next ? reinterpret_cast<double*>(ws + L.parts_off) : nullptr;
     const Params prm = make_params(s);
-    switch (s->recon_var) {
+    switch (s->riemann >= JFA_FS_ROE ? JFA_PRIMITIVE : s->recon_var) {
         case JFA_PRIMITIVE: launch_batch_s<T, JFA_PRIMITIVE>(s->stencil, s->riemann, a, prm, st); break;
         case JFA_CONSERVATIVE: launch_batch_s<T, JFA_CONSERVATIVE>(s->stencil, s->riemann, a, prm, st); break;
         case JFA_CHAR_PRIMITIVE: launch_batch_s<T, JFA_CHAR_PRIMITIVE>(s->stencil, s->riemann, a, prm, st); break;

@@ -18,6 +18,7 @@ struct Geo {
     int64_t s[3];    // buffer strides (elements)
     int64_t fs;      // field stride
     double one_dx[3];
+    double dx2[3];   // squared cell sizes (0 on inactive axes)
     double dx_min;
     int active[3];
 };
@@ -37,9 +38,11 @@ static inline int make_geo(const jfa_grid* g, Geo& G) {
         if (G.active[i]) {
             if (!(g->dx[i] > 0.0)) return JFA_EINVAL;
             G.one_dx[i] = 1.0 / g->dx[i];  // domain_information.py:290
+            G.dx2[i] = g->dx[i] * g->dx[i];
             if (g->dx[i] < G.dx_min) G.dx_min = g->dx[i];
         } else {
             G.one_dx[i] = 0.0;
+            G.dx2[i] = 0.0;
         }
     }
     if (nact == 0) return JFA_EINVAL;
@@ -221,8 +224,8 @@ __device__ __forceinline__ void halo_scatter(T* __restrict__ W, T* __restrict__ 
 static inline int check_scheme(const jfa_scheme* s) {
     if (!s) return JFA_EINVAL;
     if (s->recon_var < 0 || s->recon_var > 3) return JFA_EUNSUPPORTED;
-    if (s->stencil < JFA_WENO5_JS || s->stencil > JFA_TENO6) return JFA_EUNSUPPORTED;
-    if (s->riemann < JFA_HLLC || s->riemann > JFA_AUSMP) return JFA_EUNSUPPORTED;
+    if (s->stencil < JFA_WENO5_JS || s->stencil > JFA_WENO6_CUM2) return JFA_EUNSUPPORTED;
+    if (s->riemann < JFA_HLLC || s->riemann > JFA_FS_LLF) return JFA_EUNSUPPORTED;
     if (s->signal_speed < 0 || s->signal_speed > 4) return JFA_EUNSUPPORTED;
     if (s->frozen_state != JFA_FROZEN_ARITHMETIC && s->frozen_state != JFA_FROZEN_ROE) return JFA_EUNSUPPORTED;
     if (s->dtype != JFA_F64 && s->dtype != JFA_F32) return JFA_EUNSUPPORTED;
@@ -290,9 +293,15 @@ static inline Params make_params(const jfa_scheme* s) {
     p.flim_eps_rho = s->flux_limiter_eps_density;
     p.flim_eps_p = s->flux_limiter_eps_pressure;
     p.flim_fac[0] = p.flim_fac[1] = p.flim_fac[2] = 0.0;
+    p.dx2[0] = p.dx2[1] = p.dx2[2] = 0.0;
     p.dt_ptr = nullptr;
     p.dt_val = 0.0;
     return p;
+}
+
+// The squared cell sizes that the WENO6-CUM stencils read (reconstruct_xi's dx = cell_sizes[axis]).
+static inline void set_cell_sizes(Params& p, const Geo& G) {
+    for (int a = 0; a < 3; ++a) p.dx2[a] = G.dx2[a];
 }
 
 // (1/dx_axis) sigma of the flux limiter's partition (limiter_flux.py:696-731) for a grid:
@@ -362,6 +371,7 @@ static inline Params gas_params(double gamma) {
     p.flux_limiter = p.flux_partition = 0;
     p.flim_eps_rho = p.flim_eps_p = 0.0;
     p.flim_fac[0] = p.flim_fac[1] = p.flim_fac[2] = 0.0;
+    p.dx2[0] = p.dx2[1] = p.dx2[2] = 0.0;
     p.dt_ptr = nullptr;
     p.dt_val = 0.0;
     return p;

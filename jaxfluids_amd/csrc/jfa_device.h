@@ -71,6 +71,7 @@ struct Params {
     double flim_eps_rho;
     double flim_eps_p;
     double flim_fac[3];     // (1/dx_axis) sigma of the flux partition, per axis
+    double dx2[3];          // squared cell size per axis (the WENO6-CUM stencils read it)
     const double* dt_ptr;   // the step's time step in device memory, or null: dt_val
     double dt_val;
 };
@@ -297,6 +298,49 @@ __device__ __noinline__ double weno6_cu(double a, double b, double c, double d, 
     return a0 * one_alpha * p0 + a1 * one_alpha * p1 + a2 * one_alpha * p2 + a3 * one_alpha * p3;
 }
 
+// WENO6-CUM1 / WENO6-CUM2 (weno/weno6_cum1.py:28-67, weno6_cum2.py:29-72; Hu & Adams 2011): the
+// WENO6-CU candidates and d with C_q = 1000, q = 4 (the fourth power by squaring, as JAX's
+// integer_pow).  CUM1: alpha_k = d_k (C_q + tau_6/(beta_k + eps dx^2))^4.  CUM2: eps_ = 1e-8,
+// chi = 1/eps_, beta_ave = (beta_0 + beta_2 + 4 beta_1)/6, tau_6 = beta_3 - beta_ave and
+// alpha_k = d_k (C_q + tau_6/(beta_k + eps_ dx^2) (beta_ave + chi dx^2)/(beta_k + chi dx^2))^4.
+// dx2: the squared cell size along the axis.  Reference operation order, no contraction.
+template <bool CUM2>
+__device__ __noinline__ double weno6_cum(double a, double b, double c, double d, double e, double f, double eps,
+                                         double dx2) {
+#pragma clang fp contract(off)
+    double b0, b1, b2;
+    beta_three_point(a, b, c, d, e, b0, b1, b2);
+    const double b3 = beta_six_point(a, b, c, d, e, f);
+    const double bk[4] = {b0, b1, b2, b3};
+    const double dr[4] = {1.0 / 20, 9.0 / 20, 9.0 / 20, 1.0 / 20};
+    double al[4];
+    if constexpr (!CUM2) {
+        const double tau6 = b3 - 1.0 / 6 * (b0 + 4 * b1 + b2);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double x = 1000.0 + tau6 / (bk[k] + eps * dx2);
+            const double x2 = x * x;
+            al[k] = dr[k] * (x2 * x2);
+        }
+    } else {
+        const double eps_ = 1e-8, chi = 1.0 / 1e-8;
+        const double bave = 1.0 / 6 * (b0 + b2 + 4 * b1);
+        const double tau6 = b3 - bave;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double x = 1000.0 + tau6 / (bk[k] + eps_ * dx2) * (bave + chi * dx2) / (bk[k] + chi * dx2);
+            const double x2 = x * x;
+            al[k] = dr[k] * (x2 * x2);
+        }
+    }
+    const double one_alpha = 1.0 / (al[0] + al[1] + al[2] + al[3]);
+    const double p0 = 1.0 / 3 * a + -7.0 / 6 * b + 11.0 / 6 * c;
+    const double p1 = -1.0 / 6 * b + 5.0 / 6 * c + 1.0 / 3 * d;
+    const double p2 = 1.0 / 3 * c + 5.0 / 6 * d + -1.0 / 6 * e;
+    const double p3 = 11.0 / 6 * d + -7.0 / 6 * e + 1.0 / 3 * f;
+    return al[0] * one_alpha * p0 + al[1] * one_alpha * p1 + al[2] * one_alpha * p2 + al[3] * one_alpha * p3;
+}
+
 // TENO6 (teno/teno6.py:27-67, teno6_base.py:20-64): three 3-point candidates and a 4-point one
 // (3/12, 13/12, -5/12, 1/12) with its own indicator, tau_6 = |beta_6 - (beta_0 + 4 beta_1 + beta_2)/6|,
 // gamma_k = (1 + tau_6/(beta_k + eps))^6, sharp cutoff 1e-7, d = (0.05, 0.45, 0.30, 0.20).
@@ -324,7 +368,8 @@ __device__ __noinline__ double teno6(double a, double b, double c, double d, dou
 
 // Left and right states of one face from its 6-cell window (cells -3..2).
 template <int STENCIL>
-__device__ __forceinline__ void recon_pair(const double v[6], const Params& prm, double& L, double& R) {
+__device__ __forceinline__ void recon_pair(const double v[6], const Params& prm, double dx2, double& L,
+                                           double& R) {
     if constexpr (STENCIL != kStencilExt) {
         weno5_pair<STENCIL>(v, prm.eps_z, prm.eps_js, L, R);
     } else {
@@ -353,6 +398,14 @@ __device__ __forceinline__ void recon_pair(const double v[6], const Params& prm,
                 L = teno6(v[0], v[1], v[2], v[3], v[4], v[5], prm.eps);
                 R = teno6(v[5], v[4], v[3], v[2], v[1], v[0], prm.eps);
                 break;
+            case JFA_WENO6_CUM1:
+                L = weno6_cum<false>(v[0], v[1], v[2], v[3], v[4], v[5], prm.eps, dx2);
+                R = weno6_cum<false>(v[5], v[4], v[3], v[2], v[1], v[0], prm.eps, dx2);
+                break;
+            case JFA_WENO6_CUM2:
+                L = weno6_cum<true>(v[0], v[1], v[2], v[3], v[4], v[5], prm.eps, dx2);
+                R = weno6_cum<true>(v[5], v[4], v[3], v[2], v[1], v[0], prm.eps, dx2);
+                break;
             default:  // JFA_TENO5_A
                 L = teno5<true>(v[0], v[1], v[2], v[3], v[4], prm);
                 R = teno5<true>(v[5], v[4], v[3], v[2], v[1], prm);
@@ -365,7 +418,7 @@ __device__ __forceinline__ void recon_pair(const double v[6], const Params& prm,
 // (the 5-point stencils read the first five).
 template <int STENCIL>
 __device__ __forceinline__ double recon_side(double a, double b, double c, double d, double e, double f,
-                                             const Params& prm) {
+                                             const Params& prm, double dx2) {
     if constexpr (STENCIL != kStencilExt) {
         return weno5<STENCIL>(a, b, c, d, e, prm.eps_z, prm.eps_js);
     } else {
@@ -376,6 +429,8 @@ __device__ __forceinline__ double recon_side(double a, double b, double c, doubl
             case JFA_TENO5: return teno5<false>(a, b, c, d, e, prm);
             case JFA_WENO6_CU: return weno6_cu(a, b, c, d, e, f, prm.eps);
             case JFA_TENO6: return teno6(a, b, c, d, e, f, prm.eps);
+            case JFA_WENO6_CUM1: return weno6_cum<false>(a, b, c, d, e, f, prm.eps, dx2);
+            case JFA_WENO6_CUM2: return weno6_cum<true>(a, b, c, d, e, f, prm.eps, dx2);
             default: return teno5<true>(a, b, c, d, e, prm);
         }
     }
@@ -478,6 +533,62 @@ __device__ __forceinline__ void frozen_state(const double WL[5], const double WR
             f.q2 = f.P[1] * f.P[1] + f.P[2] * f.P[2] + f.P[3] * f.P[3];                    // :271
         }
     }
+}
+
+// Right and left eigenvectors of the conservative flux Jacobian at the frozen state f (single
+// phase; eigendecomposition_conservatives, eigendecomposition.py:595-658), indexed [row][col].
+template <int AXIS>
+__device__ __forceinline__ void conservative_eigenvectors(const Frozen& f, double L[5][5], double R[5][5]) {
+    constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
+    const double one_cc = f.inv_c * f.inv_c;
+    const double one_rho = rcp1(f.P[0]);
+    const double* P = f.P;
+    const double g = f.g, H = f.H, c = f.c, q2 = f.q2;
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+        for (int b = 0; b < 5; ++b) {
+            L[a][b] = 0.0;
+            R[a][b] = 0.0;
+        }
+    R[0][0] = 1.0;
+    R[v][0] = P[v] - c;
+    R[m0][0] = P[m0];
+    R[m1][0] = P[m1];
+    R[4][0] = H - P[v] * c;
+    R[0][v] = g;
+    R[1][v] = g * P[1];
+    R[2][v] = g * P[2];
+    R[3][v] = g * P[3];
+    R[4][v] = g * H - f.cc;
+    R[m0][m0] = -P[0];
+    R[4][m0] = -P[0] * P[m0];
+    R[m1][m1] = P[0];
+    R[4][m1] = P[0] * P[m1];
+    R[0][4] = 1.0;
+    R[v][4] = P[v] + c;
+    R[m0][4] = P[m0];
+    R[m1][4] = P[m1];
+    R[4][4] = H + P[v] * c;
+    L[0][0] = 0.5 * one_cc * (g * q2 - g * H + (P[v] + c) * c);
+    L[0][v] = 0.5 * one_cc * (-P[v] * g - c);
+    L[0][m0] = 0.5 * one_cc * (-P[m0] * g);
+    L[0][m1] = 0.5 * one_cc * (-P[m1] * g);
+    L[0][4] = 0.5 * one_cc * g;
+    L[v][0] = one_cc * (H - q2);
+    L[v][1] = P[1] * one_cc;
+    L[v][2] = P[2] * one_cc;
+    L[v][3] = P[3] * one_cc;
+    L[v][4] = -one_cc;
+    L[m0][0] = P[m0] * one_rho;
+    L[m0][m0] = -one_rho;
+    L[m1][0] = -P[m1] * one_rho;
+    L[m1][m1] = one_rho;
+    L[4][0] = 0.5 * one_cc * (g * q2 - g * H - (P[v] - c) * c);
+    L[4][v] = 0.5 * one_cc * (-P[v] * g + c);
+    L[4][m0] = 0.5 * one_cc * (-P[m0] * g);
+    L[4][m1] = 0.5 * one_cc * (-P[m1] * g);
+    L[4][4] = 0.5 * one_cc * g;
 }
 
 // ---------------------------------------------------------------- signal speeds
@@ -825,7 +936,7 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         for (int i = 0; i < 5; ++i) {
             // WENO5Base slices j=0: cells -3..1, j=1: cells 2..-2 (weno5_base.py:30)
             const double c6[6] = {win[0][i], win[1][i], win[2][i], win[3][i], win[4][i], win[5][i]};
-            recon_pair<STENCIL>(c6, prm, QL[i], QR[i]);
+            recon_pair<STENCIL>(c6, prm, prm.dx2[AXIS], QL[i], QR[i]);
         }
         if constexpr (VAR == JFA_PRIMITIVE) {
             cons_from_prim(WL, prm, UL);  // :279-280
@@ -848,19 +959,19 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         double cLv[5], cRv[5], t[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] - k1 * win[k][v];
-        recon_pair<STENCIL>(t, prm, cLv[0], cRv[0]);
+        recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[0], cRv[0]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = win[k][0] - k3 * win[k][4];
-        recon_pair<STENCIL>(t, prm, cLv[1], cRv[1]);
+        recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[1], cRv[1]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = win[k][m0];
-        recon_pair<STENCIL>(t, prm, cLv[2], cRv[2]);
+        recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[2], cRv[2]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = win[k][m1];
-        recon_pair<STENCIL>(t, prm, cLv[3], cRv[3]);
+        recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[3], cRv[3]);
 #pragma unroll
         for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] + k1 * win[k][v];
-        recon_pair<STENCIL>(t, prm, cLv[4], cRv[4]);
+        recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[4], cRv[4]);
         // get_primitives_from_characteristics (eigendecomposition.py:517-521)
         const double sL = cLv[0] + cLv[4], sR = cRv[0] + cRv[4];
         WL[0] = f.P[0] * sL + cLv[1];
@@ -878,57 +989,8 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
     } else {  // CHAR-CONSERVATIVE (:404-417)
         Frozen f;
         frozen_state<true>(wl, wr, prm, f);
-        // eigendecomposition_conservatives (eigendecomposition.py:595-658)
-        const double one_cc = f.inv_c * f.inv_c;
-        const double one_rho = rcp1(f.P[0]);
-        const double* P = f.P;
-        const double g = f.g, H = f.H, c = f.c, q2 = f.q2;
         double L[5][5], R[5][5];
-#pragma unroll
-        for (int a = 0; a < 5; ++a)
-#pragma unroll
-            for (int b = 0; b < 5; ++b) {
-                L[a][b] = 0.0;
-                R[a][b] = 0.0;
-            }
-        R[0][0] = 1.0;
-        R[v][0] = P[v] - c;
-        R[m0][0] = P[m0];
-        R[m1][0] = P[m1];
-        R[4][0] = H - P[v] * c;
-        R[0][v] = g;
-        R[1][v] = g * P[1];
-        R[2][v] = g * P[2];
-        R[3][v] = g * P[3];
-        R[4][v] = g * H - f.cc;
-        R[m0][m0] = -P[0];
-        R[4][m0] = -P[0] * P[m0];
-        R[m1][m1] = P[0];
-        R[4][m1] = P[0] * P[m1];
-        R[0][4] = 1.0;
-        R[v][4] = P[v] + c;
-        R[m0][4] = P[m0];
-        R[m1][4] = P[m1];
-        R[4][4] = H + P[v] * c;
-        L[0][0] = 0.5 * one_cc * (g * q2 - g * H + (P[v] + c) * c);
-        L[0][v] = 0.5 * one_cc * (-P[v] * g - c);
-        L[0][m0] = 0.5 * one_cc * (-P[m0] * g);
-        L[0][m1] = 0.5 * one_cc * (-P[m1] * g);
-        L[0][4] = 0.5 * one_cc * g;
-        L[v][0] = one_cc * (H - q2);
-        L[v][1] = P[1] * one_cc;
-        L[v][2] = P[2] * one_cc;
-        L[v][3] = P[3] * one_cc;
-        L[v][4] = -one_cc;
-        L[m0][0] = P[m0] * one_rho;
-        L[m0][m0] = -one_rho;
-        L[m1][0] = -P[m1] * one_rho;
-        L[m1][m1] = one_rho;
-        L[4][0] = 0.5 * one_cc * (g * q2 - g * H - (P[v] - c) * c);
-        L[4][v] = 0.5 * one_cc * (-P[v] * g + c);
-        L[4][m0] = 0.5 * one_cc * (-P[m0] * g);
-        L[4][m1] = 0.5 * one_cc * (-P[m1] * g);
-        L[4][4] = 0.5 * one_cc * g;
+        conservative_eigenvectors<AXIS>(f, L, R);
         // transformtochar: einsum over j (eigendecomposition.py:729-730)
         double ch[6][5];
 #pragma unroll
@@ -945,7 +1007,7 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             const double c6[6] = {ch[0][i], ch[1][i], ch[2][i], ch[3][i], ch[4][i], ch[5][i]};
-            recon_pair<STENCIL>(c6, prm, cLv[i], cRv[i]);
+            recon_pair<STENCIL>(c6, prm, prm.dx2[AXIS], cLv[i], cRv[i]);
         }
         // transformtophysical (eigendecomposition.py:742)
 #pragma unroll
@@ -975,6 +1037,78 @@ __device__ __forceinline__ int limit_side(const double q1[5], const Params& prm,
     const int n = limit_face_state(Wf, q1, prm);
     if (cons_win || n) cons_from_prim(Wf, prm, Uf);
     return n;
+}
+
+// ---------------------------------------------------------------- flux splitting
+// FluxSplittingScheme.compute_flux_xi (solvers/convective_fluxes/flux_splitting_scheme.py:65-111),
+// single phase, on one face.  win[k] holds the primitives of cell k-3 relative to the cell right of
+// the face.  With the conservative eigenvectors L, R of the frozen state of the face's two cells
+// and the eigenvalue magnitudes g_a of the splitting (eigendecomposition.py:672-696), each
+// characteristic field a of the window splits into f+ = (L F + g L U)/2 and f- = (L F - g L U)/2
+// (flux_splitting_scheme.py:91-92; F the cells' physical flux, U their conservatives); f+ is
+// reconstructed from the left (j = 0), f- from the right (j = 1), and the face flux is R (f+ + f-).
+// Instantiated as the kFluxSplit "Riemann solver" of the PRIMITIVE-window kernels; prm.riemann
+// names the eigenvalues (JFA_FS_ROE / JFA_FS_CLLF / JFA_FS_LLF).
+constexpr int kFluxSplit = 200;
+
+template <int AXIS, int STENCIL>
+__device__ __forceinline__ void flux_split_face(const double (&win)[6][5], const Params& prm, double F[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    Frozen f;
+    frozen_state<true>(win[2], win[3], prm, f);
+    double L[5][5], R[5][5];
+    conservative_eigenvectors<AXIS>(f, L, R);
+    double g[5];
+    if (prm.riemann == JFA_FS_ROE) {  // :674-677, at the frozen state
+        g[0] = fabs(f.P[v] - f.c);
+        g[1] = g[2] = g[3] = fabs(f.P[v]);
+        g[4] = fabs(f.P[v] + f.c);
+    } else {
+        const double uL = win[2][v], uR = win[3][v];
+        const double cL = sound_speed(win[2][4], win[2][0], prm.gamma);
+        const double cR = sound_speed(win[3][4], win[3][0], prm.gamma);
+        if (prm.riemann == JFA_FS_CLLF) {  // :680-688
+            g[0] = fmax(fabs(uL - cL), fabs(uR - cR));
+            g[1] = g[2] = g[3] = fmax(fabs(uL), fabs(uR));
+            g[4] = fmax(fabs(uL + cL), fabs(uR + cR));
+        } else {  // LLF :690-695
+            g[0] = g[1] = g[2] = g[3] = g[4] = fmax(fabs(uL) + cL, fabs(uR) + cR);
+        }
+    }
+    double Uk[6][5], Fk[6][5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        cons_from_prim(win[k], prm, Uk[k]);
+        physical_flux<AXIS>(win[k], Uk[k], Fk[k]);
+    }
+    double fc[5];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+        // transformtochar of the window's conservatives and fluxes (eigendecomposition.py:717-730)
+        double pos[6], neg[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double cu = L[a][0] * Uk[k][0], cf = L[a][0] * Fk[k][0];
+#pragma unroll
+            for (int b = 1; b < 5; ++b) {
+                cu = cu + L[a][b] * Uk[k][b];
+                cf = cf + L[a][b] * Fk[k][b];
+            }
+            const double gc = g[a] * cu;
+            pos[k] = 0.5 * (cf + gc);
+            neg[k] = 0.5 * (cf - gc);
+        }
+        fc[a] = recon_side<STENCIL>(pos[0], pos[1], pos[2], pos[3], pos[4], pos[5], prm, prm.dx2[AXIS]) +
+                recon_side<STENCIL>(neg[5], neg[4], neg[3], neg[2], neg[1], neg[0], prm, prm.dx2[AXIS]);
+    }
+    // transformtophysical (eigendecomposition.py:732-742)
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+        double acc = R[a][0] * fc[0];
+#pragma unroll
+        for (int b = 1; b < 5; ++b) acc = acc + R[a][b] * fc[b];
+        F[a] = acc;
+    }
 }
 
 }  // namespace jfa

@@ -32,26 +32,30 @@ template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN, bool FLIM>
 __device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* __restrict__ W,
                                             const T* __restrict__ U, int64_t b_right, int64_t sa, int64_t fs,
                                             const Params& prm, double F[5]) {
-    double wl[5], wr[5];
-    if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
+    if constexpr (RIEMANN == kFluxSplit) {
+        flux_split_face<AXIS, STENCIL>(win, prm, F);
+    } else {
+        double wl[5], wr[5];
+        if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            wl[i] = ld(W, i * fs + b_right - sa);
-            wr[i] = ld(W, i * fs + b_right);
+            for (int i = 0; i < 5; ++i) {
+                wl[i] = ld(W, i * fs + b_right - sa);
+                wr[i] = ld(W, i * fs + b_right);
+            }
         }
-    }
-    double WL[5], WR[5], UL[5], UR[5];
-    face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
-    if (prm.limiter) {
-        double c[5];
+        double WL[5], WR[5], UL[5], UR[5];
+        face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
+        if (prm.limiter) {
+            double c[5];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) c[i] = ld(W, i * fs + b_right - sa);
-        limit_side<VAR>(c, prm, WL, UL);
+            for (int i = 0; i < 5; ++i) c[i] = ld(W, i * fs + b_right - sa);
+            limit_side<VAR>(c, prm, WL, UL);
 #pragma unroll
-        for (int i = 0; i < 5; ++i) c[i] = ld(W, i * fs + b_right);
-        limit_side<VAR>(c, prm, WR, UR);
+            for (int i = 0; i < 5; ++i) c[i] = ld(W, i * fs + b_right);
+            limit_side<VAR>(c, prm, WR, UR);
+        }
+        riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, F);
     }
-    riemann<AXIS, RIEMANN>(WL, WR, UL, UR, prm, F);
     if constexpr (FLIM) {  // positivity flux limiter, with the step's dt (space_solver.py:530-541)
         double wl_[5], wr_[5], ul_[5], ur_[5];
 #pragma unroll
@@ -72,7 +76,7 @@ __device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* 
 // register more costs a third of the resident waves of this VALU-bound loop.
 template <class T, int A, int IN, int VAR, int STENCIL, int RIEMANN, bool DISS, bool FLIM>
 __global__ __launch_bounds__(kMarchBlockMax)
-JFA_WAVES_PER_EU((DISS || FLIM || VAR == JFA_CHAR_CONSERVATIVE) ? 1 : 3) void march_kernel(
+JFA_WAVES_PER_EU((DISS || FLIM || VAR == JFA_CHAR_CONSERVATIVE || RIEMANN == kFluxSplit) ? 1 : 3) void march_kernel(
     const T* __restrict__ W, const T* __restrict__ U, Geo G, Params prm, int L, int accumulate,
     double* __restrict__ R) {
     constexpr int O = 3 - A - IN;
@@ -429,10 +433,33 @@ static int64_t sweeps_f(const void* W, const void* U, const Geo& G, const Params
     if (prm.flux_limiter) return sweeps<T, VAR, STENCIL, RIEMANN, false, true>(W, U, G, prm, R, ia, st);
     return sweeps<T, VAR, STENCIL, RIEMANN, false, false>(W, U, G, prm, R, ia, st);
 }
+// The flux-splitting sweeps (the kFluxSplit face flux on the primitive window), instantiated in
+// jfa_fused_fsplit.hip.
+template <class T, int STENCIL>
+int64_t sweeps_fs(const void* W, const void* U, const Geo& G, const Params& prm, double* R, const InnerArgs<T>& ia,
+                  hipStream_t st) {
+    return sweeps_f<T, JFA_PRIMITIVE, STENCIL, kFluxSplit>(W, U, G, prm, R, ia, st);
+}
+#define JFA_FUSED_SWEEPS_FS(PREFIX, T, STENCIL)                                                                \
+    PREFIX template int64_t sweeps_fs<T, STENCIL>(const void*, const void*, const Geo&, const Params&, double*, \
+                                                  const InnerArgs<T>&, hipStream_t);
+#ifndef JFA_FUSED_INSTANTIATE_FS
+JFA_FUSED_SWEEPS_FS(extern, double, JFA_WENO5_JS)
+JFA_FUSED_SWEEPS_FS(extern, double, JFA_WENO5_Z)
+JFA_FUSED_SWEEPS_FS(extern, double, kStencilExt)
+JFA_FUSED_SWEEPS_FS(extern, float, JFA_WENO5_JS)
+JFA_FUSED_SWEEPS_FS(extern, float, JFA_WENO5_Z)
+JFA_FUSED_SWEEPS_FS(extern, float, kStencilExt)
+#endif
+
 template <class T, int VAR, int STENCIL>
 static int64_t sweeps_r(int riemann, const void* W, const void* U, const Geo& G, const Params& prm, double* R,
                         const InnerArgs<T>& ia, hipStream_t st) {
     if (riemann == JFA_HLLC) return sweeps_f<T, VAR, STENCIL, JFA_HLLC>(W, U, G, prm, R, ia, st);
+    if (riemann >= JFA_FS_ROE) {  // flux splitting reads the primitive window: run_sweeps sends it there
+        if constexpr (VAR == JFA_PRIMITIVE) return sweeps_fs<T, STENCIL>(W, U, G, prm, R, ia, st);
+        return -1;
+    }
     if (riemann != JFA_RUSANOV) return sweeps_f<T, VAR, STENCIL, kRiemannExt>(W, U, G, prm, R, ia, st);
     return sweeps_f<T, VAR, STENCIL, JFA_RUSANOV>(W, U, G, prm, R, ia, st);
 }
@@ -467,10 +494,11 @@ static int run_sweeps(const jfa_scheme* s, const void* W, const void* U, const G
                       int64_t* nparts) {
     Params prm = make_params(s);
     set_flux_partition(prm, s, G.one_dx, G.active);
+    set_cell_sizes(prm, G);
     prm.dt_ptr = dt_dev;  // the flux limiter's step dt
     prm.dt_val = dt_host;
     int64_t nb = 0;
-    switch (s->recon_var) {
+    switch (s->riemann >= JFA_FS_ROE ? JFA_PRIMITIVE : s->recon_var) {  // flux splitting: primitive window
         case JFA_PRIMITIVE: nb = sweeps_s<T, JFA_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
         case JFA_CONSERVATIVE: nb = sweeps_s<T, JFA_CONSERVATIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
         case JFA_CHAR_PRIMITIVE: nb = sweeps_s<T, JFA_CHAR_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;

@@ -30,9 +30,9 @@ __global__ __launch_bounds__(kBlock) void reconstruct_kernel(const T* __restrict
         const T* b = buf + v * G.fs + base;
         double r;
         if (j == 0)  // cells -3..2 (weno6_base.py:27; the 5-point and narrower stencils use -3..1 and inner cells)
-            r = recon_side<STENCIL>(ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), ld(b, 2 * sa), prm);
+            r = recon_side<STENCIL>(ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), ld(b, 2 * sa), prm, prm.dx2[axis]);
         else  // cells 2..-3
-            r = recon_side<STENCIL>(ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), ld(b, -3 * sa), prm);
+            r = recon_side<STENCIL>(ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), ld(b, -3 * sa), prm, prm.dx2[axis]);
         out[(int64_t)v * F.nface + o] = static_cast<T>(r);
     }
 }
@@ -71,6 +71,10 @@ __device__ __forceinline__ int face_flux_at(const T* __restrict__ W, const T* __
     for (int k = 0; k < 6; ++k)
 #pragma unroll
         for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + base + (k - 3) * sa);
+    if constexpr (RIEMANN == kFluxSplit) {
+        flux_split_face<AXIS, STENCIL>(win, prm, Fo);
+        return 0;
+    }
     if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
@@ -277,6 +281,10 @@ static void launch_flux(const void* W, const void* U, const Geo& G, const FaceId
 template <class T, int AXIS, int VAR, int STENCIL>
 static FluxLaunch pick_r(int riemann) {
     if (riemann == JFA_HLLC) return &launch_flux<T, AXIS, VAR, STENCIL, JFA_HLLC>;
+    if (riemann >= JFA_FS_ROE) {  // flux splitting reads the primitive window: run_flux sends it there
+        if constexpr (VAR == JFA_PRIMITIVE) return &launch_flux<T, AXIS, VAR, STENCIL, kFluxSplit>;
+        return nullptr;
+    }
     if (riemann != JFA_RUSANOV) return &launch_flux<T, AXIS, VAR, STENCIL, kRiemannExt>;
     return &launch_flux<T, AXIS, VAR, STENCIL, JFA_RUSANOV>;
 }
@@ -308,9 +316,11 @@ static FluxLaunch pick_flux(int axis, int var, int stencil, int riemann) {
 static int run_flux(const void* W, const void* U, const Geo& G, const jfa_scheme* s, int axis, void* flux,
                     int64_t* nlim, hipStream_t st) {
     const FaceIdx F = face_extents(G, axis);
-    const Params prm = make_params(s);
-    FluxLaunch fn = s->dtype == JFA_F32 ? pick_flux<float>(axis, s->recon_var, s->stencil, s->riemann)
-                                        : pick_flux<double>(axis, s->recon_var, s->stencil, s->riemann);
+    Params prm = make_params(s);
+    set_cell_sizes(prm, G);
+    const int var = s->riemann >= JFA_FS_ROE ? JFA_PRIMITIVE : s->recon_var;
+    FluxLaunch fn = s->dtype == JFA_F32 ? pick_flux<float>(axis, var, s->stencil, s->riemann)
+                                        : pick_flux<double>(axis, var, s->stencil, s->riemann);
     fn(W, U, G, F, prm, flux, prm.limiter ? reinterpret_cast<unsigned long long*>(nlim) : nullptr, st);
     return check_launch();
 }
@@ -333,7 +343,7 @@ int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* g
     if (rc) return rc;
     if (!buffer || !out || nvar < 1 || axis < 0 || axis > 2 || !G.active[axis] || (j != 0 && j != 1))
         return JFA_EINVAL;
-    if (stencil < JFA_WENO5_JS || stencil > JFA_TENO6) return JFA_EUNSUPPORTED;
+    if (stencil < JFA_WENO5_JS || stencil > JFA_WENO6_CUM2) return JFA_EUNSUPPORTED;
     const FaceIdx F = face_extents(G, axis);
     dim3 g = flat_grid(F.nface);
     hipStream_t st = (hipStream_t)stream;
@@ -342,6 +352,7 @@ int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* g
     prm.eps_z = 4.0 * eps;
     prm.eps_js = 16.0 * eps;
     prm.stencil = stencil;
+    set_cell_sizes(prm, G);
 #define JFA_W(T) \
     do { \
         if (stencil == JFA_WENO5_JS) \
@@ -362,6 +373,7 @@ int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void
                    const jfa_scheme* scheme, void* flux, void* stream) {
     int rc = check_scheme(scheme);
     if (rc) return rc;
+    if (scheme->riemann >= JFA_FS_ROE) return JFA_EUNSUPPORTED;  // flux splitting has no face states
     if (!W_L || !W_R || !U_L || !U_R || !flux || nfaces < 0 || axis < 0 || axis > 2) return JFA_EINVAL;
     if (nfaces == 0) return JFA_OK;
     const Params prm = make_params(scheme);

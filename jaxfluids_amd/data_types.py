@@ -23,10 +23,18 @@ class HighOrderGodunovSetup(NamedTuple):
     catum_setup: Any = None
 
 
+class FluxSplittingSetup(NamedTuple):
+    """conservatives.py:44-48"""
+    flux_splitting: str
+    reconstruction_stencil: type
+    split_reconstruction: Any = None
+    frozen_state: str = "ARITHMETIC"
+
+
 class ConvectiveFluxesSetup(NamedTuple):
     convective_solver: type
-    godunov: HighOrderGodunovSetup
-    flux_splitting: Any = None
+    godunov: Optional[HighOrderGodunovSetup]
+    flux_splitting: Optional[FluxSplittingSetup] = None
 
 
 class TimeIntegrationSetup(NamedTuple):

@@ -17,9 +17,10 @@ from typing import Any, Callable, Dict, Union
 
 import numpy as np
 
+from . import _native
 from .config import precision
 from .convective_fluxes import DICT_CONVECTIVE_SOLVER
-from .data_types import (ActivePhysicsSetup, CaseSetup, ConservativesSetup, ConvectiveFluxesSetup,
+from .data_types import (ActivePhysicsSetup, CaseSetup, ConservativesSetup, ConvectiveFluxesSetup, FluxSplittingSetup,
                          DissipativeFluxesSetup, DomainSetup, GeneralSetup, HighOrderGodunovSetup, MaterialSetup,
                          NumericalSetup, PositivitySetup, PrecisionSetup, TimeIntegrationSetup, TransportSetup)
 from .riemann_solvers import DICT_RIEMANN_SOLVER, DICT_SIGNAL_SPEEDS
@@ -117,6 +118,52 @@ def read_dissipative_fluxes(d: Dict) -> DissipativeFluxesSetup:
     return setup
 
 
+def _check_stencil(st: str, halo_cells: int):
+    if st not in DICT_SPATIAL_RECONSTRUCTION:
+        raise NotImplementedError(f"reconstruction_stencil {st!r} is not built (built: {tuple(DICT_SPATIAL_RECONSTRUCTION)})")
+    stencil_cls = DICT_SPATIAL_RECONSTRUCTION[st]
+    assert halo_cells >= stencil_cls.required_halos, \
+        f"halo_cells={halo_cells} < {stencil_cls.required_halos} required by {st}"
+    if halo_cells < KERNEL_HALOS:
+        raise NotImplementedError(f"the HIP kernels need halo_cells >= {KERNEL_HALOS} for every stencil ({st}: {halo_cells})")
+    return stencil_cls
+
+
+def _read_godunov(cf: Dict, halo_cells: int) -> ConvectiveFluxesSetup:
+    """read_godunov (read_conservatives.py:125-203)."""
+    g = cf.get("godunov", {})
+    rs = g.get("riemann_solver", "HLLC")
+    if rs not in DICT_RIEMANN_SOLVER:
+        raise NotImplementedError(f"riemann_solver {rs!r} is not built (built: {tuple(DICT_RIEMANN_SOLVER)})")
+    ss = g.get("signal_speed", "EINFELDT")
+    if ss not in DICT_SIGNAL_SPEEDS:
+        raise NotImplementedError(f"signal_speed {ss!r} is not built (built: {tuple(DICT_SIGNAL_SPEEDS)})")
+    stencil_cls = _check_stencil(g.get("reconstruction_stencil", "WENO5-Z"), halo_cells)
+    rv = g.get("reconstruction_variable", "PRIMITIVE")
+    assert rv in TUPLE_RECONSTRUCTION_VARIABLES, f"reconstruction_variable must be one of {TUPLE_RECONSTRUCTION_VARIABLES}"
+    fz = g.get("frozen_state", "ARITHMETIC")
+    assert fz in TUPLE_FROZEN_STATE, f"frozen_state must be one of {TUPLE_FROZEN_STATE}"
+    godunov = HighOrderGodunovSetup(DICT_RIEMANN_SOLVER[rs], DICT_SIGNAL_SPEEDS[ss], stencil_cls, None, rv, fz, None)
+    return ConvectiveFluxesSetup(DICT_CONVECTIVE_SOLVER["GODUNOV"], godunov)
+
+
+def _read_flux_splitting(cf: Dict, halo_cells: int) -> ConvectiveFluxesSetup:
+    """read_flux_splitting (read_conservatives.py:205-243): flux_splitting one of
+    TUPLE_FLUX_SPLITTING (solvers/__init__.py:1-3; CLF has no eigenvalues in
+    eigendecomposition.py:672-699), the reconstruction stencil and the frozen state."""
+    d = cf.get("flux_splitting")
+    assert isinstance(d, dict), "convective_fluxes: key 'flux_splitting' is required for FLUX-SPLITTING"
+    fsp = d.get("flux_splitting", "ROE")
+    assert fsp in ("ROE", "CLLF", "LLF", "CLF"), f"flux_splitting must be one of ('ROE', 'CLLF', 'LLF', 'CLF')"
+    if fsp not in _native.FLUX_SPLITTING:
+        raise NotImplementedError(f"flux_splitting {fsp!r} has no eigenvalues in the reference (built: ROE, CLLF, LLF)")
+    stencil_cls = _check_stencil(d.get("reconstruction_stencil", "WENO5-Z"), halo_cells)
+    fz = d.get("frozen_state", "ARITHMETIC")
+    assert fz in TUPLE_FROZEN_STATE, f"frozen_state must be one of {TUPLE_FROZEN_STATE}"
+    return ConvectiveFluxesSetup(DICT_CONVECTIVE_SOLVER["FLUX-SPLITTING"], None,
+                                 FluxSplittingSetup(fsp, stencil_cls, None, fz))
+
+
 def read_numerical_setup(d: Dict) -> NumericalSetup:
     """read_conservatives.py:20-203 (convective-flux keys)."""
     assert "conservatives" in d, "numerical_setup: key 'conservatives' is required"
@@ -136,28 +183,11 @@ def read_numerical_setup(d: Dict) -> NumericalSetup:
     cf = c.get("convective_fluxes", {})
     solver = cf.get("convective_solver", "GODUNOV")
     if solver not in DICT_CONVECTIVE_SOLVER:
-        raise NotImplementedError(f"convective_solver {solver!r} is not on this path (built: GODUNOV)")
-    g = cf.get("godunov", {})
-    rs = g.get("riemann_solver", "HLLC")
-    if rs not in DICT_RIEMANN_SOLVER:
-        raise NotImplementedError(f"riemann_solver {rs!r} is not built (built: {tuple(DICT_RIEMANN_SOLVER)})")
-    ss = g.get("signal_speed", "EINFELDT")
-    if ss not in DICT_SIGNAL_SPEEDS:
-        raise NotImplementedError(f"signal_speed {ss!r} is not built (built: {tuple(DICT_SIGNAL_SPEEDS)})")
-    st = g.get("reconstruction_stencil", "WENO5-Z")
-    if st not in DICT_SPATIAL_RECONSTRUCTION:
-        raise NotImplementedError(f"reconstruction_stencil {st!r} is not built (built: {tuple(DICT_SPATIAL_RECONSTRUCTION)})")
-    rv = g.get("reconstruction_variable", "PRIMITIVE")
-    assert rv in TUPLE_RECONSTRUCTION_VARIABLES, f"reconstruction_variable must be one of {TUPLE_RECONSTRUCTION_VARIABLES}"
-    fz = g.get("frozen_state", "ARITHMETIC")
-    assert fz in TUPLE_FROZEN_STATE, f"frozen_state must be one of {TUPLE_FROZEN_STATE}"
-    stencil_cls = DICT_SPATIAL_RECONSTRUCTION[st]
-    assert halo_cells >= stencil_cls.required_halos, \
-        f"halo_cells={halo_cells} < {stencil_cls.required_halos} required by {st}"
-    if halo_cells < KERNEL_HALOS:
-        raise NotImplementedError(f"the HIP kernels need halo_cells >= {KERNEL_HALOS} for every stencil ({st}: {halo_cells})")
-    godunov = HighOrderGodunovSetup(DICT_RIEMANN_SOLVER[rs], DICT_SIGNAL_SPEEDS[ss], stencil_cls, None, rv, fz, None)
-    conv = ConvectiveFluxesSetup(DICT_CONVECTIVE_SOLVER[solver], godunov)
+        raise NotImplementedError(f"convective_solver {solver!r} is not on this path (built: {tuple(DICT_CONVECTIVE_SOLVER)})")
+    if solver == "FLUX-SPLITTING":
+        conv = _read_flux_splitting(cf, halo_cells)
+    else:
+        conv = _read_godunov(cf, halo_cells)
     pos = c.get("positivity", {})  # read_positivity.py:8-64
     flux_limiter = pos.get("flux_limiter") or None
     if flux_limiter not in (None, "SIMPLE", "NASA", "HAS"):

@@ -66,7 +66,7 @@ class ShockCapturingStencil(SpatialReconstruction):
             raise NotImplementedError(
                 f"the HIP kernels read 3 cells on each side of a face: halo_cells >= 3 (got {self.n})")
 
-    def _grid_for(self, buffer: torch.Tensor):
+    def _grid_for(self, buffer: torch.Tensor, axis: int = 0, dx=None):
         if buffer.dim() != 4:
             raise ValueError("buffer must have shape (nvar, Nx+2nh, Ny+2nh, Nz+2nh)")
         cells = []
@@ -78,14 +78,18 @@ class ShockCapturingStencil(SpatialReconstruction):
                 cells.append(1)
             else:
                 cells.append(b - 2 * self.n)
-        return cells, _native.make_grid(cells, self.n, (1.0, 1.0, 1.0))
+        # dx: the cell size along `axis` (cell_sizes[axis]; the WENO6-CUM stencils read it)
+        dxs = [1.0, 1.0, 1.0]
+        if dx is not None:
+            dxs[axis] = float(dx)
+        return cells, _native.make_grid(cells, self.n, tuple(dxs))
 
     def reconstruct_xi(self, buffer: torch.Tensor, axis: int, j: int, dx=None, **kwargs) -> torch.Tensor:
         """Left (j=0) or right (j=1) face states along axis (weno5_js.py:73-115)."""
         _native.require_device(buffer, "buffer")
         if self.offset != 0:
             raise NotImplementedError("offset != 0 is not used on the convective-flux path")
-        cells, grid = self._grid_for(buffer)
+        cells, grid = self._grid_for(buffer, axis, dx)
         if AXES[axis] in self.inactive_axes:
             raise ValueError(f"axis {axis} is inactive")
         face = list(cells)
@@ -154,6 +158,30 @@ class WENO6CU(ShockCapturingStencil):
         self.C_ = 20
 
 
+class WENO6CUM1(WENO6CU):
+    """WENO6-CUM1 (weno/weno6_cum1.py:10-67; Hu & Adams 2011, scale separation for implicit LES):
+    C_q = 1000, q = 4, alpha_k = d_k (C_q + tau_6/(beta_k + eps dx^2))^q; reconstruct_xi's dx is
+    the cell size along the axis."""
+
+    _variant = _native.STENCIL["WENO6-CUM1"]
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self.Cq_, self.q_, self.eps_ = 1000, 4, 1e-8
+
+
+class WENO6CUM2(WENO6CU):
+    """WENO6-CUM2 (weno/weno6_cum2.py:10-72): C_q = 1000, q = 4, eps_ = 1e-8, chi = 1/eps_, the
+    ratio (beta_ave + chi dx^2)/(beta_k + chi dx^2) on tau_6/(beta_k + eps_ dx^2)."""
+
+    _variant = _native.STENCIL["WENO6-CUM2"]
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self.Cq_, self.q_, self.eps_ = 1000, 4, 1e-8
+        self.chi = 1.0 / self.eps_
+
+
 class TENO6(ShockCapturingStencil):
     """TENO6 (teno6_base.py:10-64, teno/teno6.py:10-67): d = (0.05, 0.45, 0.30, 0.20), CT = 1e-7."""
 
@@ -215,4 +243,6 @@ DICT_SPATIAL_RECONSTRUCTION: Dict[str, type] = {
     "TENO5-A": TENO5A,
     "WENO6-CU": WENO6CU,
     "TENO6": TENO6,
+    "WENO6-CUM1": WENO6CUM1,
+    "WENO6-CUM2": WENO6CUM2,
 }

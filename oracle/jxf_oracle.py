@@ -297,7 +297,7 @@ def _beta_three_point(u_imm, u_im, u_i, u_ip, u_ipp):
     return beta_0, beta_1, beta_2
 
 
-def weno6_cu_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS):
+def weno6_cu_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS, dx=None):
     """weno/weno6_cu.py:24-61 (Hu, Wang & Adams 2010): d = (1/20, 9/20, 9/20, 1/20)
     (weno6_base.py:22), C = 20 (:16), tau_6 = beta_3 - (beta_0 + 4 beta_1 + beta_2)/6."""
     beta_0, beta_1, beta_2 = _beta_three_point(u_imm, u_im, u_i, u_ip, u_ipp)
@@ -314,7 +314,7 @@ def weno6_cu_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EP
             + alpha[3] * one_alpha * p_3)
 
 
-def teno6_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS):
+def teno6_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS, dx=None):
     """teno/teno6.py:27-67 with teno6_base.py:20-64: three 3-point candidates and one 4-point
     candidate (3/12, 13/12, -5/12, 1/12) with its own indicator beta_3 (:40-44), the 6-point
     indicator beta_6 for tau_6 = |beta_6 - (beta_0 + 4 beta_1 + beta_2)/6|, gamma_k =
@@ -339,7 +339,56 @@ def teno6_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS):
     return (w[0] * one_dk * p_0 + w[1] * one_dk * p_1 + w[2] * one_dk * p_2 + w[3] * one_dk * p_3)
 
 
-WENO6_POINT = {"WENO6-CU": weno6_cu_point, "TENO6": teno6_point}
+def _integer_pow4(x):
+    """jnp.power(x, 4) with an integer exponent: lax.integer_pow, by squaring."""
+    x2 = x * x
+    return x2 * x2
+
+
+def _weno6_cum(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, alpha_of):
+    beta_0, beta_1, beta_2 = _beta_three_point(u_imm, u_im, u_i, u_ip, u_ipp)
+    beta_3 = _beta_six_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp)
+    dr = (1 / 20, 9 / 20, 9 / 20, 1 / 20)                                  # weno6_base.py:22
+    alpha = [dr[k] * _integer_pow4(alpha_of(beta_0, beta_1, beta_2, beta_3, b))
+             for k, b in enumerate((beta_0, beta_1, beta_2, beta_3))]
+    one_alpha = 1.0 / (alpha[0] + alpha[1] + alpha[2] + alpha[3])
+    p_0 = _CR6[0][0] * u_imm + _CR6[0][1] * u_im + _CR6[0][2] * u_i
+    p_1 = _CR6[1][0] * u_im + _CR6[1][1] * u_i + _CR6[1][2] * u_ip
+    p_2 = _CR6[2][0] * u_i + _CR6[2][1] * u_ip + _CR6[2][2] * u_ipp
+    p_3 = _CR6[3][0] * u_ip + _CR6[3][1] * u_ipp + _CR6[3][2] * u_ippp
+    return (alpha[0] * one_alpha * p_0 + alpha[1] * one_alpha * p_1 + alpha[2] * one_alpha * p_2
+            + alpha[3] * one_alpha * p_3)
+
+
+def weno6_cum1_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS, dx=1.0):
+    """weno/weno6_cum1.py:28-67 (Hu & Adams 2011): C_q = 1000, q = 4, tau_6 as WENO6-CU,
+    alpha_k = d_k (C_q + tau_6 / (beta_k + eps dx^2))^4 with the stencil epsilon and the cell size
+    along the axis (its eps_ = 1e-8 is not read)."""
+    dx2 = dx * dx
+
+    def alpha_of(beta_0, beta_1, beta_2, beta_3, b):
+        tau_6 = beta_3 - 1 / 6 * (beta_0 + 4 * beta_1 + beta_2)
+        return 1000 + tau_6 / (b + eps * dx2)
+    return _weno6_cum(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, alpha_of)
+
+
+def weno6_cum2_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS, dx=1.0):
+    """weno/weno6_cum2.py:29-72: C_q = 1000, q = 4, eps_ = 1e-8, chi = 1/eps_,
+    beta_ave = (beta_0 + beta_2 + 4 beta_1)/6, tau_6 = beta_3 - beta_ave,
+    alpha_k = d_k (C_q + tau_6/(beta_k + eps_ dx^2) (beta_ave + chi dx^2)/(beta_k + chi dx^2))^4."""
+    dx2 = dx * dx
+    eps_ = 1e-8
+    chi = 1.0 / eps_
+
+    def alpha_of(beta_0, beta_1, beta_2, beta_3, b):
+        beta_ave = 1 / 6 * (beta_0 + beta_2 + 4 * beta_1)
+        tau_6 = beta_3 - beta_ave
+        return 1000 + tau_6 / (b + eps_ * dx2) * (beta_ave + chi * dx2) / (b + chi * dx2)
+    return _weno6_cum(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, alpha_of)
+
+
+WENO6_POINT = {"WENO6-CU": weno6_cu_point, "TENO6": teno6_point, "WENO6-CUM1": weno6_cum1_point,
+               "WENO6-CUM2": weno6_cum2_point}
 
 
 def weno5_reconstruct_xi(buffer, dom: Domain, axis: int, j: int, stencil: str = "WENO5-JS",
@@ -351,19 +400,20 @@ def weno5_reconstruct_xi(buffer, dom: Domain, axis: int, j: int, stencil: str = 
     """
     if stencil in WENO6_POINT:
         ops = [buffer[shift_slice(dom, axis, k)] for k in WENO6_SHIFTS[j]]
-        return WENO6_POINT[stencil](*ops, eps=eps)
+        return WENO6_POINT[stencil](*ops, eps=eps, dx=dom.dx[axis])  # dx=cell_sizes[axis]
     ops = [buffer[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[j]]
     return WENO5_POINT[stencil](*ops, eps=eps)
 
 
-def weno5_reconstruct_window(window, j: int, stencil: str, eps=SPATIAL_STENCIL_EPS):
+def weno5_reconstruct_window(window, j: int, stencil: str, eps=SPATIAL_STENCIL_EPS, dx=1.0):
     """reconstruct_xi after set_slices_stencil (spatial_reconstruction.py:133-153).
 
-    ``window`` has the 6-cell stencil window on axis 1: shape (nvar, 6, *face).
+    ``window`` has the 6-cell stencil window on axis 1: shape (nvar, 6, *face); dx is the cell
+    size along the axis (read by WENO6-CUM1/-CUM2).
     """
     if stencil in WENO6_POINT:
         ops = [window[:, i] for i in WENO6_WINDOW_IDX[j]]
-        return WENO6_POINT[stencil](*ops, eps=eps)
+        return WENO6_POINT[stencil](*ops, eps=eps, dx=dx)
     ops = [window[:, i] for i in WENO5_WINDOW_IDX[j]]
     return WENO5_POINT[stencil](*ops, eps=eps)
 
@@ -819,6 +869,8 @@ class Setup:
     riemann: str = "HLLC"                     # riemann_solver
     signal_speed: str = "EINFELDT"            # signal_speed
     frozen_state: str = "ARITHMETIC"          # frozen_state
+    convective_solver: str = "GODUNOV"        # convective_solver: "GODUNOV" / "FLUX-SPLITTING"
+    flux_splitting: str = "ROE"               # flux_splitting/flux_splitting: "ROE" / "CLLF" / "LLF"
     cfl: float = 0.5
     fixed_dt: Optional[float] = None
     bcs: Dict[str, str] = field(default_factory=lambda: {f: "PERIODIC" for f in FACE_LOCATIONS})
@@ -830,6 +882,13 @@ class Setup:
     flux_limiter: Optional[str] = None        # positivity.flux_limiter: "SIMPLE" / "NASA"
     flux_partition: str = "UNIFORM"           # positivity.flux_partition: "UNIFORM" / "CELLSIZE"
     flux_limiter_eps: Tuple[float, float] = (1e-12, 1e-10)  # config/precision.py:55
+
+    def __post_init__(self):
+        # riemann "FLUX-SPLITTING-<eigenvalues>" (the native library's face-flux name) selects the
+        # flux-splitting convective solver
+        if self.riemann.startswith("FLUX-SPLITTING-"):
+            self.convective_solver = "FLUX-SPLITTING"
+            self.flux_splitting = self.riemann[len("FLUX-SPLITTING-"):]
 
 
 def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
@@ -852,8 +911,8 @@ def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
         P, _, _, c, cc, _ = frozen_state(cl, cr, g, s.frozen_state)
         # expand_dims(axis=axis-3) of the frozen state == broadcast over the window axis
         char = char_from_prim(win, P[:, None], c[None], cc[None], axis)
-        cL = weno5_reconstruct_window(char, 0, s.stencil, s.eps_stencil)
-        cR = weno5_reconstruct_window(char, 1, s.stencil, s.eps_stencil)
+        cL = weno5_reconstruct_window(char, 0, s.stencil, s.eps_stencil, dom.dx[axis])
+        cR = weno5_reconstruct_window(char, 1, s.stencil, s.eps_stencil, dom.dx[axis])
         W_L = prim_from_char(cL, P, c, cc, axis)
         W_R = prim_from_char(cR, P, c, cc, axis)
         U_L, U_R = cons_from_prim(W_L, g), cons_from_prim(W_R, g)
@@ -863,8 +922,8 @@ def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
         fs = frozen_state(cl, cr, g, s.frozen_state)
         R, L = eigen_conservatives(*fs, axis=axis)
         char = transform(L[:, :, None], win)
-        cL = weno5_reconstruct_window(char, 0, s.stencil, s.eps_stencil)
-        cR = weno5_reconstruct_window(char, 1, s.stencil, s.eps_stencil)
+        cL = weno5_reconstruct_window(char, 0, s.stencil, s.eps_stencil, dom.dx[axis])
+        cR = weno5_reconstruct_window(char, 1, s.stencil, s.eps_stencil, dom.dx[axis])
         U_L, U_R = transform(R, cL), transform(R, cR)
         W_L, W_R = prim_from_cons(U_L, g), prim_from_cons(U_R, g)
     else:
@@ -892,10 +951,58 @@ def limit_interpolation_xi(W, W_j, j: int, dom: Domain, axis: int, s: Setup):
     return cons_from_prim(W_j, s.gamma), W_j, count
 
 
+def flux_splitting_eigenvalues(W, dom: Domain, axis: int, P, c_ave, gamma, flux_splitting: str):
+    """Eigenvalue magnitudes of the flux splitting (eigendecomposition.py:672-696): the diagonal
+    (gamma_1, gamma_234 x3, gamma_5) on every face."""
+    v = axis + 1
+    if flux_splitting == "ROE":                                              # :674-677
+        g1, g234, g5 = np.abs(P[v] - c_ave), np.abs(P[v]), np.abs(P[v] + c_ave)
+    elif flux_splitting in ("CLLF", "LLF"):
+        W_L, W_R = face_LR_cells(W, dom, axis)
+        c_L = speed_of_sound(W_L[4], W_L[0], gamma)
+        c_R = speed_of_sound(W_R[4], W_R[0], gamma)
+        if flux_splitting == "CLLF":                                         # :680-688
+            g1 = np.maximum(np.abs(W_L[v] - c_L), np.abs(W_R[v] - c_R))
+            g234 = np.maximum(np.abs(W_L[v]), np.abs(W_R[v]))
+            g5 = np.maximum(np.abs(W_L[v] + c_L), np.abs(W_R[v] + c_R))
+        else:                                                                # :690-695
+            g1 = g234 = g5 = np.maximum(np.abs(W_L[v]) + c_L, np.abs(W_R[v]) + c_R)
+    else:
+        raise NotImplementedError(flux_splitting)                            # :698-699
+    return np.stack([g1, g234, g234, g234, g5], axis=0)
+
+
+def flux_splitting_flux_xi(W, U, dom: Domain, axis: int, s: Setup):
+    """FluxSplittingScheme.compute_flux_xi (flux_splitting_scheme.py:65-111), single phase.
+
+    The window of conservatives and of physical fluxes is projected on the left eigenvectors of
+    the frozen state of each face's two cells; with the diagonal eigenvalue matrix Lambda,
+    f+- = (L F +- Lambda L U) / 2 (:91-92); f+ is reconstructed with j = 0, f- with j = 1
+    (:94-105), and the sum is projected back with the right eigenvectors (:107-109)."""
+    cl, cr = face_LR_cells(W, dom, axis)
+    fs = frozen_state(cl, cr, s.gamma, s.frozen_state)
+    R, L = eigen_conservatives(*fs, axis=axis)
+    lam = flux_splitting_eigenvalues(W, dom, axis, fs[0], fs[3], s.gamma, s.flux_splitting)
+    char = transform(L[:, :, None], stencil_window(U, dom, axis))
+    F = physical_flux_xi(W, U, axis)                                         # equation_manager.get_fluxes_xi
+    char_flux = transform(L[:, :, None], stencil_window(F, dom, axis))
+    # transformtochar(char, eigvals): the diagonal matrix times each window cell
+    lam_char = lam[:, None] * char
+    positive = 0.5 * (char_flux + lam_char)
+    negative = 0.5 * (char_flux - lam_char)
+    f_L = weno5_reconstruct_window(positive, 0, s.stencil, s.eps_stencil, dom.dx[axis])
+    f_R = weno5_reconstruct_window(negative, 1, s.stencil, s.eps_stencil, dom.dx[axis])
+    return transform(R, f_L + f_R)
+
+
 def compute_flux_xi(W, U, dom: Domain, axis: int, s: Setup, return_count: bool = False):
     """HighOrderGodunov.compute_flux_xi (high_order_godunov.py:117-231), single phase:
     reconstruct, apply the interpolation limiter when enabled (:163-176), then solve the
-    Riemann problem."""
+    Riemann problem.  With convective_solver FLUX-SPLITTING, flux_splitting_flux_xi (no
+    interpolation limiter there)."""
+    if s.convective_solver == "FLUX-SPLITTING":
+        F = flux_splitting_flux_xi(W, U, dom, axis, s)
+        return (F, None) if return_count else F
     U_L, U_R, W_L, W_R = reconstruct_xi(W, U, dom, axis, s)
     count = None
     if s.interpolation_limiter:

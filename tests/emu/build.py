@@ -11,7 +11,8 @@ CSRC = os.path.join(ROOT, "jaxfluids_amd", "csrc")
 OUT = os.path.join(HERE, "_build", "libjfa_emu.so")
 SOURCES = [os.path.join(CSRC, f) for f in ("jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip", "jfa_kernels.hip",
                                             "jfa_fused_f64.hip", "jfa_fused_f32.hip", "jfa_fused_prim.hip",
-                                            "jfa_fused_cons.hip", "jfa_fused_charprim.hip", "jfa_fused_charcons.hip")]
+                                            "jfa_fused_cons.hip", "jfa_fused_charprim.hip", "jfa_fused_charcons.hip",
+                                            "jfa_fused_fsplit.hip")]
 DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("jfa_batch.h", "jfa_common.h", "jfa_device.h", "jfa_adjoint.h",
                                                   "jfa_dissipative.h", "jfa_fused.h")] + [
     os.path.join(HERE, "emu_runtime.cpp"), os.path.join(HERE, "hip", "hip_runtime.h"), __file__]

@@ -164,6 +164,11 @@ FURTHER = [  # the run-time stencil switch of the kernel
     ("WENO5-JS", "CHAR-PRIMITIVE", "HLL", "EINFELDT", "ARITHMETIC"),  # HLL Riemann solver
     ("TENO5", "PRIMITIVE", "HLLC-LM", "DAVIS", "ARITHMETIC"),  # HLLC-LM Riemann solver
     ("WENO5-Z", "CONSERVATIVE", "AUSMP", "EINFELDT", "ARITHMETIC"),  # AUSM+ Riemann solver
+    ("WENO5-Z", "PRIMITIVE", "FLUX-SPLITTING-ROE", "EINFELDT", "ROE"),  # flux splitting
+    ("TENO5", "CHAR-CONSERVATIVE", "FLUX-SPLITTING-LLF", "EINFELDT", "ARITHMETIC"),
+    ("WENO5-JS", "PRIMITIVE", "FLUX-SPLITTING-CLLF", "EINFELDT", "ARITHMETIC"),
+    ("WENO6-CUM1", "CHAR-PRIMITIVE", "HLLC", "EINFELDT", "ROE"),  # the cell size per case
+    ("WENO6-CUM2", "CONSERVATIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
 ]
 
 

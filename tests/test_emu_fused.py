@@ -161,7 +161,7 @@ def test_fused_step_with_limiters(emu, fl):
     assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL
 
 
-NEW_STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6"]
+NEW_STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6", "WENO6-CUM1", "WENO6-CUM2"]
 
 
 @pytest.mark.parametrize("stencil", NEW_STENCILS)
@@ -186,3 +186,29 @@ def test_fused_step_further_riemann_solvers(emu, riemann, speed, variable, grid)
     s = o.Setup(gamma=1.4, bcs={f: "ZEROGRADIENT" for f in FACES}, cfl=0.5, stencil="WENO5-Z", variable=variable,
                 riemann=riemann, signal_speed=speed, frozen_state="ROE")
     check_step(emu, dom, s, W0)
+
+
+@pytest.mark.parametrize("fs", ["ROE", "CLLF", "LLF"])
+@pytest.mark.parametrize("stencil,grid", [("WENO5-Z", "3d"), ("WENO5-JS", "2d-xy"), ("TENO6", "2d-yz"),
+                                          ("WENO3-JS", "1d"), ("WENO6-CUM2", "3d-wide")])
+def test_fused_step_flux_splitting(emu, fs, stencil, grid):
+    """The flux-splitting scheme (flux_splitting_scheme.py:65-111) as the face flux of the fused
+    sweeps (the kFluxSplit instantiation of the PRIMITIVE-window kernels)."""
+    dom, W0 = grid_state(grid, seed=13)
+    s = o.Setup(gamma=1.4, bcs={f: "ZEROGRADIENT" for f in FACES}, cfl=0.5, stencil=stencil,
+                riemann="FLUX-SPLITTING-" + fs, frozen_state="ROE", variable="CHAR-CONSERVATIVE")
+    check_step(emu, dom, s, W0)
+
+
+@pytest.mark.parametrize("fl", ["SIMPLE", "NASA"])
+def test_fused_step_flux_splitting_with_flux_limiter(emu, fl):
+    dom, W0 = _pocket_state()
+    zg = {f: "ZEROGRADIENT" for f in ("east", "west", "north", "south")}
+    s = o.Setup(stencil="WENO5-Z", riemann="FLUX-SPLITTING-LLF", frozen_state="ARITHMETIC", bcs=zg,
+                flux_limiter=fl)
+    W, U = o.create_buffers(W0, dom, s)
+    dt = 0.9 / 0.5 * o.compute_dt(W, dom, s)
+    Wr, Ur = o.rk3_step(W, U, dt, dom, s)
+    Wg, Ug, _ = fused_step(emu, dom, s, W, U, dt)
+    I = dom.interior
+    assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL

@@ -1,0 +1,154 @@
+"""GPU parity of the flux-splitting convective solver (FluxSplittingScheme,
+solvers/convective_fluxes/flux_splitting_scheme.py:21-111, with the ROE, CLLF and LLF eigenvalues of
+eigendecomposition.py:672-696) against the oracle: the plugin's compute_flux_xi, fused RK3 steps
+built from setup dicts (3-D, 2-D, with the positivity flux limiter), a whole Sod run, and the
+batched ensemble stage.  FP64, relative L-inf <= 1e-12.
+
+The kernels run the splitting as the kFluxSplit instantiation of the PRIMITIVE-window face
+kernels, for every stencil of the run-time switch.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import jxf_oracle as o
+from test_gpu_parity import _input_manager, dev, host, native_grid, native_scheme, random_state, rel, say
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+FACES = ("east", "west", "north", "south", "top", "bottom")
+SPLITS = ["ROE", "CLLF", "LLF"]
+
+
+@pytest.mark.parametrize("fs", SPLITS)
+@pytest.mark.parametrize("stencil", ["WENO5-JS", "WENO5-Z", "TENO5", "WENO3-Z", "WENO6-CU"])
+@pytest.mark.parametrize("frozen", ["ARITHMETIC", "ROE"])
+def test_compute_flux_xi(fs, stencil, frozen):
+    from jaxfluids_amd import _native
+
+    say(f"[fs flux_xi] {fs} {stencil} {frozen}")
+    s = o.Setup(gamma=1.4, stencil=stencil, riemann="FLUX-SPLITTING-" + fs, frozen_state=frozen,
+                bcs={f: "PERIODIC" for f in FACES})
+    dom, W0 = random_state((10, 9, 8), 3, seed=23)
+    W, U = o.create_buffers(W0, dom, s)
+    lib = _native.load()
+    Wd, Ud = dev(W), dev(U)
+    for variable in ("PRIMITIVE", "CHAR-CONSERVATIVE"):  # the reconstruction variable does not apply
+        s.variable = variable
+        for axis in range(3):
+            ref = o.compute_flux_xi(W, U, dom, axis, s)
+            assert np.all(np.isfinite(ref))
+            out = torch.empty(ref.shape, dtype=torch.float64, device="cuda")
+            rc = lib.jfa_compute_flux_xi(_native.ptr(Wd), _native.ptr(Ud), native_grid(dom), native_scheme(s), axis,
+                                         _native.ptr(out), None, _native.stream_ptr())
+            _native.check(rc, "jfa_compute_flux_xi")
+            assert rel(host(out), ref) < TOL, (variable, axis, rel(host(out), ref))
+
+
+def _sim(dom, s, **kw):
+    from jaxfluids_amd import FluxSplittingScheme, SimulationManager
+
+    sim = SimulationManager(_input_manager(dom, s, **kw))
+    assert isinstance(sim.convective_flux_solver, FluxSplittingScheme)
+    return sim
+
+
+@pytest.mark.parametrize("fs", SPLITS)
+@pytest.mark.parametrize("stencil", ["WENO5-Z", "TENO5", "WENO6-CU"])
+@pytest.mark.parametrize("cells,bc", [((11, 9, 8), "ZEROGRADIENT"), ((37, 70, 1), "PERIODIC")], ids=["3d", "2d"])
+def test_rk3_step_through_setup_dicts(fs, stencil, cells, bc):
+    """One fused RK3 step and the next CFL dt of a SimulationManager whose setup dicts name
+    convective_solver FLUX-SPLITTING; and its solver's compute_flux_xi."""
+    say(f"[fs rk3 step] {fs} {stencil} {cells} {bc}")
+    ranges = ((0, 1.0), (0, 1.3), (0, 1.0))
+    full = tuple(max(c, 2) for c in cells)
+    _, W0 = random_state(full, 3, seed=19, ranges=ranges)
+    W0 = np.ascontiguousarray(W0[:, :cells[0], :cells[1], :cells[2]])
+    dom = o.Domain(cells, ranges, nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, riemann="FLUX-SPLITTING-" + fs, frozen_state="ROE",
+                bcs={f: (bc if o.FACE_AXIS[f] in dom.active else "PERIODIC") for f in FACES})
+    sim = _sim(dom, s)
+    W, U = o.create_buffers(W0, dom, s)
+    Wd, Ud = dev(W), dev(U)
+    for axis in dom.active:
+        flux = sim.convective_flux_solver.compute_flux_xi(Wd, Ud, axis)
+        assert flux[1:] == (None, None, None, None)
+        assert rel(host(flux[0]), o.compute_flux_xi(W, U, dom, axis, s)) < TOL
+    dt = o.compute_dt(W, dom, s)
+    Wr, Ur = o.rk3_step(W, U, dt, dom, s)
+    I = dom.interior
+    assert np.all(np.isfinite(Wr[I]))
+    dtd = torch.full((1,), dt, dtype=torch.float64, device="cuda")
+    Wd, Ud = sim.do_runge_kutta_stages(Wd, Ud, dtd, dt_next=dtd)
+    assert rel(host(Wd)[I], Wr[I]) < TOL and rel(host(Ud)[I], Ur[I]) < TOL
+    assert abs(host(dtd)[0] - o.compute_dt(Wr, dom, s)) < 1e-14 * dt
+
+
+@pytest.mark.parametrize("fl", ["SIMPLE", "NASA"])
+def test_rk3_step_with_flux_limiter(fl):
+    """The positivity flux limiter on the split flux (space_solver.py:530-541), on a state with a
+    near-vacuum pocket that makes it switch faces to first order."""
+    from test_oracle_pins import _pocket_state
+
+    say(f"[fs flux limiter] {fl}")
+    dom, W0 = _pocket_state()
+    zg = {f: "ZEROGRADIENT" for f in ("east", "west", "north", "south")}
+    s = o.Setup(gamma=1.4, stencil="WENO5-Z", riemann="FLUX-SPLITTING-LLF", frozen_state="ARITHMETIC", bcs=zg,
+                flux_limiter=fl)
+    sim = _sim(dom, s, positivity={"flux_limiter": fl})
+    W, U = o.create_buffers(W0, dom, s)
+    dt = 0.9 / 0.5 * o.compute_dt(W, dom, s)
+    Wr, Ur = o.rk3_step(W, U, dt, dom, s)
+    I = dom.interior
+    assert np.all(np.isfinite(Wr[I]))
+    Wd, Ud = dev(W), dev(U)
+    dtd = torch.full((1,), dt, dtype=torch.float64, device="cuda")
+    Wd, Ud = sim.do_runge_kutta_stages(Wd, Ud, dtd, dt_next=dtd)
+    assert rel(host(Wd)[I], Wr[I]) < TOL and rel(host(Ud)[I], Ur[I]) < TOL
+
+
+@pytest.mark.parametrize("fs", SPLITS)
+def test_sod_simulation(fs):
+    """Sod (1000 cells, WENO5-Z, RK3, CFL 0.5, t_end 0.2) through the whole step loop."""
+    from jaxfluids_amd import FluxSplittingScheme, InitializationManager, SimulationManager
+
+    say(f"[fs sod] {fs}")
+    dom, W0 = o.sod_initial(1000)
+    s = o.Setup(stencil="WENO5-Z", riemann="FLUX-SPLITTING-" + fs, frozen_state="ROE",
+                bcs={"east": "ZEROGRADIENT", "west": "ZEROGRADIENT"})
+    W, U = o.create_buffers(W0, dom, s)
+    Wr, Ur, tr, nr = o.simulate(W, U, dom, s, end_time=0.2)
+    im = _input_manager(dom, s, end_time=0.2)
+    sim = SimulationManager(im)
+    assert isinstance(sim.convective_flux_solver, FluxSplittingScheme)
+    out = sim.simulate(InitializationManager(im).initialization(W0))
+    tcv = out.time_control_variables
+    assert tcv.simulation_step == nr
+    assert abs(tcv.physical_simulation_time - tr) < 1e-12
+    I = dom.interior
+    Wg = host(out.material_fields.primitives)
+    assert rel(Wg[I], Wr[I]) < 1e-11, rel(Wg[I], Wr[I])
+
+
+@pytest.mark.parametrize("fs", SPLITS)
+def test_ensemble_step_matches_oracle(fs):
+    """The batched stage of 5 uneven cases, one RK3 step each with its own CFL dt."""
+    from test_gpu_ensemble import CASES, case_state, face_regions, make_ensemble, oracle_case
+
+    scheme = ("WENO5-Z", "PRIMITIVE", "FLUX-SPLITTING-" + fs, "EINFELDT", "ROE")
+    say(f"[fs ensemble] {scheme}")
+    prims = [case_state(c[0], c[1], seed=300 + i) for i, c in enumerate(CASES)]
+    ens = make_ensemble(scheme, chunk=16)
+    ens.initialize(prims)
+    refs = [oracle_case(i, scheme, p) for i, p in enumerate(prims)]
+    ens.step(1)
+    got_dt = host(ens.dt)
+    for i, (dom, s, W, U) in enumerate(refs):
+        assert s.convective_solver == "FLUX-SPLITTING"
+        dt = o.compute_dt(W, dom, s)
+        W, U = o.rk3_step(W, U, dt, dom, s)
+        Wg, Ug = host(ens.state(i)[0]), host(ens.state(i)[1])
+        for r in face_regions(dom):
+            assert rel(Wg[r], W[r]) < TOL and rel(Ug[r], U[r]) < TOL, (i, rel(Wg[r], W[r]))
+        assert abs(got_dt[i] - o.compute_dt(W, dom, s)) <= 1e-14 * dt, i

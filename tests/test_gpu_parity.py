@@ -265,6 +265,15 @@ def test_rk3_step_parity(case, bc, fused):
     assert abs(host(dt2)[0] - o.compute_dt(Wr, dom, s)) < 1e-14 * dt
 
 
+def _convective_fluxes_dict(s):
+    if s.convective_solver == "FLUX-SPLITTING":
+        return {"convective_solver": "FLUX-SPLITTING", "flux_splitting": {
+            "flux_splitting": s.flux_splitting, "reconstruction_stencil": s.stencil, "frozen_state": s.frozen_state}}
+    return {"convective_solver": "GODUNOV", "godunov": {
+        "riemann_solver": s.riemann, "signal_speed": s.signal_speed or "EINFELDT",
+        "reconstruction_stencil": s.stencil, "reconstruction_variable": s.variable, "frozen_state": s.frozen_state}}
+
+
 def _input_manager(dom, s, end_time=None, end_step=None, positivity=None):
     from jaxfluids_amd import InputManager
 
@@ -279,10 +288,7 @@ def _input_manager(dom, s, end_time=None, end_step=None, positivity=None):
     }
     num = {"conservatives": {"halo_cells": dom.nh,
                              "time_integration": {"integrator": "RK3", "CFL": s.cfl},
-                             "convective_fluxes": {"convective_solver": "GODUNOV", "godunov": {
-                                 "riemann_solver": s.riemann, "signal_speed": s.signal_speed or "EINFELDT",
-                                 "reconstruction_stencil": s.stencil, "reconstruction_variable": s.variable,
-                                 "frozen_state": s.frozen_state}},
+                             "convective_fluxes": _convective_fluxes_dict(s),
                              "positivity": positivity or {}},
            "active_physics": {"is_convective_flux": True},
            "precision": {"is_double_precision_compute": True}}

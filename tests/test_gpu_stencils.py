@@ -1,6 +1,6 @@
 """GPU parity of the further reconstruction stencils -- WENO1, WENO3-JS, WENO3-Z, TENO5, TENO5-A,
-WENO6-CU, TENO6 (stencils/reconstruction/shock_capturing/weno/weno1_js.py, weno3_js.py, weno3_z.py,
-weno6_cu.py, teno/teno5.py, teno5_a.py, teno6.py) -- against the oracle: the plugin's reconstruct_xi, compute_flux_xi for
+WENO6-CU, TENO6, WENO6-CUM1, WENO6-CUM2 (stencils/reconstruction/shock_capturing/weno/weno1_js.py,
+weno3_js.py, weno3_z.py, weno6_cu.py, weno6_cum1.py, weno6_cum2.py, teno/teno5.py, teno5_a.py, teno6.py) -- against the oracle: the plugin's reconstruct_xi, compute_flux_xi for
 every reconstruction variable, RK3 steps of the fused stage through the setup dicts (3-D and 2-D),
 and the batched ensemble stage.  FP64, relative L-inf <= 1e-12.
 
@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 FACES = ("east", "west", "north", "south", "top", "bottom")
-STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6"]
+STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6", "WENO6-CUM1", "WENO6-CUM2"]
 
 
 @pytest.mark.parametrize("stencil", STENCILS)
@@ -35,7 +35,7 @@ def test_reconstruct_xi_plugin(stencil):
     for axis in range(3):
         for j in (0, 1):
             ref = o.weno5_reconstruct_xi(W, dom, axis, j, stencil)
-            out = plugin.reconstruct_xi(Wd, axis, j)
+            out = plugin.reconstruct_xi(Wd, axis, j, dx=dom.dx[axis])
             assert rel(host(out), ref) < TOL, (axis, j, rel(host(out), ref))
 
 

@@ -111,7 +111,7 @@ def test_setup_reader_accepts_further_stencils():
                                                     "godunov": {"riemann_solver": "HLLC", "signal_speed": "EINFELDT",
                                                                 "reconstruction_stencil": "WENO5-JS",
                                                                 "reconstruction_variable": "CHAR-PRIMITIVE"}}}}
-    for name in ("WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6"):
+    for name in ("WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6", "WENO6-CUM1", "WENO6-CUM2"):
         d = copy.deepcopy(base)
         d["conservatives"]["convective_fluxes"]["godunov"]["reconstruction_stencil"] = name
         ns = read_numerical_setup(d)
@@ -141,7 +141,49 @@ def test_stage_vjp_refuses_further_riemann_solvers():
     lib = _native.load()
     grid = _native.make_grid((8, 8, 8), 3, (0.1, 0.1, 0.1))
     bc = _native.bc_array({f: "PERIODIC" for f in ("east", "west", "north", "south", "top", "bottom")})
-    for rs in ("HLL", "HLLC-LM", "AUSMP"):
+    for rs in ("HLL", "HLLC-LM", "AUSMP", "FLUX-SPLITTING-ROE", "FLUX-SPLITTING-LLF"):
         scheme = _native.make_scheme("PRIMITIVE", "WENO5-JS", rs, "EINFELDT", "ARITHMETIC", _native.F64, 1.4, 1e-30)
         assert lib.jfa_rk3_stage_vjp(None, None, None, None, grid, scheme, bc, 0, 1e-3, None, 0, None) == \
             _native.JFA_EUNSUPPORTED, rs
+
+
+@pytest.mark.skipif(not os.path.exists(_native.LIB_PATH), reason="library not built")
+def test_riemann_xi_refuses_flux_splitting():
+    """A flux-splitting scheme has no face states: jfa_riemann_xi returns JFA_EUNSUPPORTED
+    before any device call; a face-flux value past the last one is refused by every entry point."""
+    lib = _native.load()
+    for fs in _native.FLUX_SPLITTING:
+        scheme = _native.make_scheme("PRIMITIVE", "WENO5-Z", "FLUX-SPLITTING-" + fs, None, "ROE", _native.F64, 1.4,
+                                     1e-30)
+        assert lib.jfa_riemann_xi(None, None, None, None, 4, 0, scheme, None, None) == _native.JFA_EUNSUPPORTED
+    scheme = _native.make_scheme("PRIMITIVE", "WENO5-Z", "HLLC", None, "ROE", _native.F64, 1.4, 1e-30)
+    scheme.riemann = max(_native.FLUX_SPLITTING.values()) + 1
+    assert lib.jfa_riemann_xi(None, None, None, None, 4, 0, scheme, None, None) == _native.JFA_EUNSUPPORTED
+    grid = _native.make_grid((8, 8, 8), 3, (0.1, 0.1, 0.1))
+    assert lib.jfa_compute_flux_xi(None, None, grid, scheme, 0, None, None, None) == _native.JFA_EUNSUPPORTED
+
+
+def test_setup_reader_flux_splitting():
+    """convective_solver FLUX-SPLITTING (read_conservatives.py:205-243) builds FluxSplittingScheme;
+    CLF is an accepted name without eigenvalues in the reference and is refused."""
+    import copy
+
+    from jaxfluids_amd import FluxSplittingScheme
+    from jaxfluids_amd.input_manager import read_numerical_setup
+    from jaxfluids_amd.stencils import DICT_SPATIAL_RECONSTRUCTION
+
+    base = {"conservatives": {"halo_cells": 3, "time_integration": {"integrator": "RK3", "CFL": 0.5},
+                              "convective_fluxes": {"convective_solver": "FLUX-SPLITTING",
+                                                    "flux_splitting": {"flux_splitting": "CLLF",
+                                                                       "reconstruction_stencil": "TENO5"}}}}
+    cf = read_numerical_setup(base).conservatives.convective_fluxes
+    assert cf.convective_solver is FluxSplittingScheme and cf.godunov is None
+    assert cf.flux_splitting.flux_splitting == "CLLF" and cf.flux_splitting.frozen_state == "ARITHMETIC"
+    assert cf.flux_splitting.reconstruction_stencil is DICT_SPATIAL_RECONSTRUCTION["TENO5"]
+    bad = copy.deepcopy(base)
+    bad["conservatives"]["convective_fluxes"]["flux_splitting"]["flux_splitting"] = "CLF"
+    with pytest.raises(NotImplementedError):
+        read_numerical_setup(bad)
+    bad["conservatives"]["convective_fluxes"]["flux_splitting"]["flux_splitting"] = "GLF"
+    with pytest.raises(AssertionError):
+        read_numerical_setup(bad)

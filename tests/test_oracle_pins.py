@@ -960,7 +960,7 @@ def _left_states6(stencil, F, f, N, periodic):
     return o.WENO6_POINT[stencil](*ops), f(faces)
 
 
-@pytest.mark.parametrize("stencil", ["WENO6-CU", "TENO6"])
+@pytest.mark.parametrize("stencil", ["WENO6-CU", "TENO6", "WENO6-CUM1", "WENO6-CUM2"])
 def test_six_point_stencils_reproduce_quadratics_and_are_sixth_order(stencil):
     """Every candidate reproduces quadratics (so any normalised weighting does); on a smooth sine
     the weights approach d, and both d = (1/20, 9/20, 9/20, 1/20) of WENO6-CU and
@@ -983,3 +983,27 @@ def test_teno6_cutoff_and_weno6_cu_at_a_step():
     z, one = np.zeros(1), np.ones(1)
     assert o.WENO6_POINT["TENO6"](z, z, z, one, one, one)[0] == 0.0
     assert 0.0 < abs(o.WENO6_POINT["WENO6-CU"](z, z, z, one, one, one)[0]) < 1e-20
+
+
+def test_weno6_cum_limits_and_cell_size_scaling():
+    """WENO6-CUM2 differs from WENO6-CUM1 only by (beta_ave + chi dx^2)/(beta_k + chi dx^2) and
+    eps_ dx^2 with eps_ = 1/chi = 1e-8: where eps_ dx^2 << beta << chi dx^2 the two agree to
+    O(beta/(chi dx^2)); the reconstruction is homogeneous of degree 1 in (u, dx) together,
+    CUM2(lam u, dx) = lam CUM2(u, dx/lam), since beta scales with lam^2 like dx^2; and on a step
+    a finer grid (smaller chi dx^2) lets the smoothness ratio act, moving the state further
+    towards the smooth side."""
+    rng = np.random.default_rng(5)
+    u = [rng.uniform(0.5, 2.0, 64) for _ in range(6)]
+    c1 = o.weno6_cum1_point(*u, dx=1.0)
+    c2 = o.weno6_cum2_point(*u, dx=1.0)
+    assert 1e-14 < np.max(np.abs(c2 - c1)) and np.max(np.abs(c2 - c1) / np.abs(c1)) < 1e-6
+    for lam in (1e-3, 7.0):
+        a = o.weno6_cum2_point(*[lam * v for v in u], dx=0.02)
+        b = lam * o.weno6_cum2_point(*u, dx=0.02 / lam)
+        assert np.max(np.abs(a - b) / np.abs(b)) < 1e-12
+    # CUM1 is scale invariant (eps dx^2 = 1e-30 dx^2 is negligible), CUM2 is not
+    assert np.max(np.abs(o.weno6_cum1_point(*[3 * v for v in u], dx=0.02) - 3 * o.weno6_cum1_point(*u, dx=0.02))) < 1e-12
+    z, one = np.zeros(1), np.ones(1)
+    coarse = o.weno6_cum2_point(z, z, z, one, one, one, dx=1.0)[0]
+    fine = o.weno6_cum2_point(z, z, z, one, one, one, dx=1e-3)[0]
+    assert 0.0 < fine < coarse < 0.5
