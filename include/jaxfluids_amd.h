@@ -49,6 +49,7 @@ extern "C" {
 #define JFA_TENO6 8
 #define JFA_WENO6_CUM1 9
 #define JFA_WENO6_CUM2 10
+#define JFA_TENO6_A 11
 /* riemann_solver (solvers/riemann_solvers/__init__.py:16-25) */
 #define JFA_HLLC 0
 #define JFA_RUSANOV 1
@@ -130,7 +131,7 @@ typedef struct jfa_transport {
 /* Numerical scheme of the high-order Godunov flux (data_types/numerical_setup/conservatives.py:35-42). */
 typedef struct jfa_scheme {
     int32_t recon_var;     /* JFA_PRIMITIVE .. JFA_CHAR_CONSERVATIVE */
-    int32_t stencil;       /* JFA_WENO5_JS ... JFA_WENO6_CUM2 */
+    int32_t stencil;       /* JFA_WENO5_JS ... JFA_TENO6_A */
     int32_t riemann;       /* JFA_HLLC / JFA_RUSANOV / JFA_HLL / JFA_HLLC_LM / JFA_AUSMP, or JFA_FS_* */
     int32_t signal_speed;  /* JFA_SS_* (HLLC only) */
     int32_t frozen_state;  /* JFA_FROZEN_* (characteristic variables only) */
@@ -173,8 +174,8 @@ const char* jfa_last_error(void);
  * Replaces WENO5JS.reconstruct_xi (stencils/reconstruction/shock_capturing/weno/weno5_js.py:73-115),
  * WENO5Z.reconstruct_xi (.../weno/weno5_z.py:138-173), WENO1 / WENO3JS / WENO3Z
  * (.../weno/weno1_js.py:25-30, weno3_js.py:15-42, weno3_z.py:21-41), TENO5 / TENO5A
- * (.../teno/teno5.py:32-73, teno5_a.py:42-99), WENO6CU / WENO6CUM1 / WENO6CUM2 / TENO6
- * (.../weno/weno6_cu.py:24-61, weno6_cum1.py:28-67, weno6_cum2.py:29-72, teno/teno6.py:27-67; the
+ * (.../teno/teno5.py:32-73, teno5_a.py:42-99), WENO6CU / WENO6CUM1 / WENO6CUM2 / TENO6 / TENO6A
+ * (.../weno/weno6_cu.py:24-61, weno6_cum1.py:28-67, weno6_cum2.py:29-72, teno/teno6.py:27-67, teno6_a.py:40-131; the
  * CUM stencils read the grid's cell size along `axis` as reconstruct_xi's dx) with the slice tables of WENO5Base / WENO3Base / WENO1
  * (.../weno5_base.py:30, weno3_base.py:29, weno/weno1_js.py:22) and SpatialStencil.array_slices
  * (stencils/spatial_stencil.py:45-112).

@@ -19,7 +19,8 @@ LIB_PATH = os.path.join(_HERE, "libjaxfluids_amd.so")
 JFA_OK, JFA_EINVAL, JFA_EUNSUPPORTED, JFA_ELAUNCH = 0, 1, 2, 3
 RECON_VAR = {"PRIMITIVE": 0, "CONSERVATIVE": 1, "CHAR-PRIMITIVE": 2, "CHAR-CONSERVATIVE": 3}
 STENCIL = {"WENO5-JS": 0, "WENO5-Z": 1, "WENO1": 2, "WENO3-JS": 3, "WENO3-Z": 4, "TENO5": 5, "TENO5-A": 6,
-           "WENO6-CU": 7, "TENO6": 8, "WENO6-CUM1": 9, "WENO6-CUM2": 10}
+           "WENO6-CU": 7, "TENO6": 8, "WENO6-CUM1": 9, "WENO6-CUM2": 10,
+           "TENO6-A": 11}
 RIEMANN = {"HLLC": 0, "RUSANOV": 1, "HLL": 2, "HLLC-LM": 3, "AUSMP": 4}
 # convective_solver FLUX-SPLITTING with its eigenvalues (flux_splitting), in place of a Riemann solver
 FLUX_SPLITTING = {"ROE": 5, "CLLF": 6, "LLF": 7}

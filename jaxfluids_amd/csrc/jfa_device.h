@@ -366,6 +366,48 @@ __device__ __noinline__ double teno6(double a, double b, double c, double d, dou
     return w0 * one_dk * p0 + w1 * one_dk * p1 + w2 * one_dk * p2 + w3 * one_dk * p3;
 }
 
+// TENO6-A (teno/teno6_a.py:20-131): TENO6's candidates with |beta_3| and |beta_6|, gamma_k =
+// (1 + tau_6/(beta_k + eps))^6, the spectrally optimised d and the adaptive cutoff
+// CT = 10^-(ceil(10.5 - 4.5 (1 - g)) - 1) (5..10) from eta, the minimum over the four neighbouring
+// difference pairs, with xi = 1e-3 and Cr = 0.17; no eps in the weight normalisation.
+__device__ __noinline__ double teno6_a(double a, double b, double c, double d, double e, double f, double eps) {
+#pragma clang fp contract(off)
+    double b0, b1, b2;
+    beta_three_point(a, b, c, d, e, b0, b1, b2);
+    const double b3 = fabs(1.0 / 240.0 *
+                           (c * (2107 * c - 9402 * d + 7042 * e - 1854 * f) + d * (11003 * d - 17246 * e + 4642 * f) +
+                            e * (7043 * e - 3882 * f) + 547 * f * f));
+    const double b6 = fabs(beta_six_point(a, b, c, d, e, f));
+    const double tau6 = fabs(b6 - 1.0 / 6 * (b0 + 4 * b1 + b2));
+    const double g0 = pow(1.0 + tau6 / (b0 + eps), 6.0), g1 = pow(1.0 + tau6 / (b1 + eps), 6.0);
+    const double g2 = pow(1.0 + tau6 / (b2 + eps), 6.0), g3 = pow(1.0 + tau6 / (b3 + eps), 6.0);
+    const double one_gamma_sum = 1.0 / (g0 + g1 + g2 + g3);
+    const double ed = 0.9 * 0.17 / (1 - 0.17) * (1e-3 * 1e-3);  // eps_dissipation = 0.9 Cr/(1-Cr) xi^2
+    const double dd[5] = {b - a, c - b, d - c, e - d, f - e};
+    double eta = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double ek = (fabs(2 * dd[k + 1] * dd[k]) + ed) / (dd[k + 1] * dd[k + 1] + dd[k] * dd[k] + ed);
+        eta = k == 0 ? ek : fmin(eta, ek);
+    }
+    const double m = 1 - fmin(1.0, eta / 0.17);
+    const double om = 1 - m, om2 = om * om;
+    const double g = om2 * om2 * (1 + 4 * m);
+    const int bb = (int)ceil(10.5 - 4.5 * (1 - g)) - 1;  // 5..10
+    constexpr double kPow10[11] = {1.0, 1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10};
+    const double CT = kPow10[bb < 0 ? 0 : (bb > 10 ? 10 : bb)];
+    const double w0 = g0 * one_gamma_sum < CT ? 0.0 : 0.0855682281039113;
+    const double w1 = g1 * one_gamma_sum < CT ? 0.0 : 0.4294317718960898;
+    const double w2 = g2 * one_gamma_sum < CT ? 0.0 : 0.1727270875843552;
+    const double w3 = g3 * one_gamma_sum < CT ? 0.0 : 0.3122729124156450;
+    const double one_dk = 1.0 / (w0 + w1 + w2 + w3);
+    const double p0 = 1.0 / 3 * a + -7.0 / 6 * b + 11.0 / 6 * c;
+    const double p1 = -1.0 / 6 * b + 5.0 / 6 * c + 1.0 / 3 * d;
+    const double p2 = 1.0 / 3 * c + 5.0 / 6 * d + -1.0 / 6 * e;
+    const double p3 = 3.0 / 12 * c + 13.0 / 12 * d + -5.0 / 12 * e + 1.0 / 12 * f;
+    return w0 * one_dk * p0 + w1 * one_dk * p1 + w2 * one_dk * p2 + w3 * one_dk * p3;
+}
+
 // Left and right states of one face from its 6-cell window (cells -3..2).
 template <int STENCIL>
 __device__ __forceinline__ void recon_pair(const double v[6], const Params& prm, double dx2, double& L,
@@ -398,6 +440,10 @@ __device__ __forceinline__ void recon_pair(const double v[6], const Params& prm,
                 L = teno6(v[0], v[1], v[2], v[3], v[4], v[5], prm.eps);
                 R = teno6(v[5], v[4], v[3], v[2], v[1], v[0], prm.eps);
                 break;
+            case JFA_TENO6_A:
+                L = teno6_a(v[0], v[1], v[2], v[3], v[4], v[5], prm.eps);
+                R = teno6_a(v[5], v[4], v[3], v[2], v[1], v[0], prm.eps);
+                break;
             case JFA_WENO6_CUM1:
                 L = weno6_cum<false>(v[0], v[1], v[2], v[3], v[4], v[5], prm.eps, dx2);
                 R = weno6_cum<false>(v[5], v[4], v[3], v[2], v[1], v[0], prm.eps, dx2);
@@ -429,6 +475,7 @@ __device__ __forceinline__ double recon_side(double a, double b, double c, doubl
             case JFA_TENO5: return teno5<false>(a, b, c, d, e, prm);
             case JFA_WENO6_CU: return weno6_cu(a, b, c, d, e, f, prm.eps);
             case JFA_TENO6: return teno6(a, b, c, d, e, f, prm.eps);
+            case JFA_TENO6_A: return teno6_a(a, b, c, d, e, f, prm.eps);
             case JFA_WENO6_CUM1: return weno6_cum<false>(a, b, c, d, e, f, prm.eps, dx2);
             case JFA_WENO6_CUM2: return weno6_cum<true>(a, b, c, d, e, f, prm.eps, dx2);
             default: return teno5<true>(a, b, c, d, e, prm);

@@ -343,7 +343,7 @@ int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* g
     if (rc) return rc;
     if (!buffer || !out || nvar < 1 || axis < 0 || axis > 2 || !G.active[axis] || (j != 0 && j != 1))
         return JFA_EINVAL;
-    if (stencil < JFA_WENO5_JS || stencil > JFA_WENO6_CUM2) return JFA_EUNSUPPORTED;
+    if (stencil < JFA_WENO5_JS || stencil > JFA_TENO6_A) return JFA_EUNSUPPORTED;
     const FaceIdx F = face_extents(G, axis);
     dim3 g = flat_grid(F.nface);
     hipStream_t st = (hipStream_t)stream;

@@ -158,6 +158,21 @@ class WENO6CU(ShockCapturingStencil):
         self.C_ = 20
 
 
+class TENO6A(ShockCapturingStencil):
+    """TENO6-A (teno/teno6_a.py:10-131): TENO6 with |beta_3|, |beta_6|, the spectrally optimised
+    d (fourth order) and the adaptive cutoff (xi = 1e-3, Cr = 0.17, alpha = (10.5, 4.5))."""
+
+    required_halos = 3
+    _variant = _native.STENCIL["TENO6-A"]
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self._dr = (0.0855682281039113, 0.4294317718960898, 0.1727270875843552, 0.3122729124156450)
+        self._stencil_size = 6
+        self.C, self.q = 1.0, 6
+        self.xi, self.Cr, self.alpha_1, self.alpha_2 = 1e-3, 0.17, 10.5, 4.5
+
+
 class WENO6CUM1(WENO6CU):
     """WENO6-CUM1 (weno/weno6_cum1.py:10-67; Hu & Adams 2011, scale separation for implicit LES):
     C_q = 1000, q = 4, alpha_k = d_k (C_q + tau_6/(beta_k + eps dx^2))^q; reconstruct_xi's dx is
@@ -245,4 +260,5 @@ DICT_SPATIAL_RECONSTRUCTION: Dict[str, type] = {
     "TENO6": TENO6,
     "WENO6-CUM1": WENO6CUM1,
     "WENO6-CUM2": WENO6CUM2,
+    "TENO6-A": TENO6A,
 }

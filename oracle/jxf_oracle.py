@@ -387,8 +387,46 @@ def weno6_cum2_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_
     return _weno6_cum(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, alpha_of)
 
 
+def teno6_a_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS, dx=None):
+    """teno/teno6_a.py:20-131: TENO6's candidates and indicators with |beta_3|, |beta_6|
+    (is_positivity_limiter_smoothness, :36,58-60), C = 1, q = 6, the spectrally optimised
+    d = (0.0855682281039113, 0.4294317718960898, 0.1727270875843552, 0.3122729124156450) (:25-28),
+    and the adaptive cutoff CT = 10^-(ceil(10.5 - 4.5 (1 - g)) - 1) from the minimum eta of the four
+    neighbouring difference pairs (:96-131) with xi = 1e-3, Cr = 0.17 (:31-35); no eps in the
+    weight normalisation (:81)."""
+    beta_0, beta_1, beta_2 = _beta_three_point(u_imm, u_im, u_i, u_ip, u_ipp)
+    beta_3 = np.abs(1.0 / 240.0 * (
+        u_i * (2107 * u_i - 9402 * u_ip + 7042 * u_ipp - 1854 * u_ippp)
+        + u_ip * (11003 * u_ip - 17246 * u_ipp + 4642 * u_ippp)
+        + u_ipp * (7043 * u_ipp - 3882 * u_ippp)
+        + 547 * u_ippp * u_ippp))
+    beta_6 = np.abs(_beta_six_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp))
+    tau_6 = np.abs(beta_6 - 1 / 6 * (beta_0 + 4 * beta_1 + beta_2))
+    gamma = [(1.0 + tau_6 / (b + eps)) ** 6 for b in (beta_0, beta_1, beta_2, beta_3)]
+    one_gamma_sum = 1.0 / (gamma[0] + gamma[1] + gamma[2] + gamma[3])
+    xi, Cr = 1e-3, 0.17
+    ed = 0.9 * Cr / (1 - Cr) * xi ** 2
+    d = (u_im - u_imm, u_i - u_im, u_ip - u_i, u_ipp - u_ip, u_ippp - u_ipp)
+    eta = None
+    for k in range(4):
+        e_k = (np.abs(2 * d[k + 1] * d[k]) + ed) / (d[k + 1] ** 2 + d[k] ** 2 + ed)
+        eta = e_k if eta is None else np.minimum(eta, e_k)
+    m = 1 - np.minimum(1.0, eta / Cr)
+    g = np.power(1 - m, 4) * (1 + 4 * m)
+    beta_bar = np.ceil(10.5 - 4.5 * (1 - g)) - 1.0
+    CT = np.power(10.0, -beta_bar)
+    dr = (0.0855682281039113, 0.4294317718960898, 0.1727270875843552, 0.3122729124156450)
+    w = [dr[k] * np.where(gamma[k] * one_gamma_sum < CT, 0.0, 1.0) for k in range(4)]
+    one_dk = 1.0 / (w[0] + w[1] + w[2] + w[3])
+    p_0 = _CR6[0][0] * u_imm + _CR6[0][1] * u_im + _CR6[0][2] * u_i
+    p_1 = _CR6[1][0] * u_im + _CR6[1][1] * u_i + _CR6[1][2] * u_ip
+    p_2 = _CR6[2][0] * u_i + _CR6[2][1] * u_ip + _CR6[2][2] * u_ipp
+    p_3 = 3 / 12 * u_i + 13 / 12 * u_ip + -5 / 12 * u_ipp + 1 / 12 * u_ippp
+    return (w[0] * one_dk * p_0 + w[1] * one_dk * p_1 + w[2] * one_dk * p_2 + w[3] * one_dk * p_3)
+
+
 WENO6_POINT = {"WENO6-CU": weno6_cu_point, "TENO6": teno6_point, "WENO6-CUM1": weno6_cum1_point,
-               "WENO6-CUM2": weno6_cum2_point}
+               "WENO6-CUM2": weno6_cum2_point, "TENO6-A": teno6_a_point}
 
 
 def weno5_reconstruct_xi(buffer, dom: Domain, axis: int, j: int, stencil: str = "WENO5-JS",

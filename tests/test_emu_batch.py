@@ -169,6 +169,7 @@ FURTHER = [  # the run-time stencil switch of the kernel
     ("WENO5-JS", "PRIMITIVE", "FLUX-SPLITTING-CLLF", "EINFELDT", "ARITHMETIC"),
     ("WENO6-CUM1", "CHAR-PRIMITIVE", "HLLC", "EINFELDT", "ROE"),  # the cell size per case
     ("WENO6-CUM2", "CONSERVATIVE", "HLLC", "EINFELDT", "ARITHMETIC"),
+    ("TENO6-A", "CHAR-CONSERVATIVE", "HLLC", "EINFELDT", "ROE"),
 ]
 
 

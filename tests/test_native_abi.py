@@ -111,7 +111,7 @@ def test_setup_reader_accepts_further_stencils():
                                                     "godunov": {"riemann_solver": "HLLC", "signal_speed": "EINFELDT",
                                                                 "reconstruction_stencil": "WENO5-JS",
                                                                 "reconstruction_variable": "CHAR-PRIMITIVE"}}}}
-    for name in ("WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6", "WENO6-CUM1", "WENO6-CUM2"):
+    for name in ("WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6", "WENO6-CUM1", "WENO6-CUM2", "TENO6-A"):
         d = copy.deepcopy(base)
         d["conservatives"]["convective_fluxes"]["godunov"]["reconstruction_stencil"] = name
         ns = read_numerical_setup(d)

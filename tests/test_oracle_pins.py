@@ -1007,3 +1007,21 @@ def test_weno6_cum_limits_and_cell_size_scaling():
     coarse = o.weno6_cum2_point(z, z, z, one, one, one, dx=1.0)[0]
     fine = o.weno6_cum2_point(z, z, z, one, one, one, dx=1e-3)[0]
     assert 0.0 < fine < coarse < 0.5
+
+
+def test_teno6_a_quadratics_fourth_order_and_step():
+    """TENO6-A: its candidates reproduce quadratics; its spectrally optimised d make a
+    fourth-order scheme on a smooth sine (the class comment's '4-th order convergence'); at a unit
+    step its cutoff removes the candidates that cross it and returns the smooth side's value."""
+    Fq = lambda x: 0.3 * x + 0.7 * x ** 2 / 2 - 1.9 * x ** 3 / 3  # noqa: E731
+    fq = lambda x: 0.3 + 0.7 * x - 1.9 * x ** 2  # noqa: E731
+    L, exact = _left_states6("TENO6-A", Fq, fq, 40, periodic=False)
+    assert np.max(np.abs(L - exact)) < 1e-13
+    F = lambda x: -np.cos(2 * np.pi * x) / (2 * np.pi)  # noqa: E731
+    f = lambda x: np.sin(2 * np.pi * x)  # noqa: E731
+    errs = [np.max(np.abs(np.subtract(*_left_states6("TENO6-A", F, f, N, periodic=True)))) for N in (64, 128, 256)]
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(np.abs(rates - 4.0) < 0.15), rates
+    z, one = np.zeros(1), np.ones(1)
+    assert o.WENO6_POINT["TENO6-A"](z, z, z, one, one, one)[0] == 0.0
+    assert abs(o.WENO6_POINT["TENO6-A"](one, one, one, z, z, z)[0] - 1.0) < 1e-15
