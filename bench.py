@@ -255,7 +255,8 @@ def bench_ensemble(steps: int, warmup: int, ncases: int = 64, seed: int = 0, nh:
     torch.cuda.current_stream().wait_stream(s2)
     per_restore()
     t_per_graph = timed(lambda n: [g.replay() for _ in range(n)], steps)
-    finite = int(sum(bool(torch.isfinite(ens.view(W_batched, i)).all().item()) for i in range(ncases)))
+    nonfinite = [i for i in range(ncases) if not bool(torch.isfinite(ens.view(W_batched, i)).all().item())]
+    nonfinite_per = [i for i, q in enumerate(per) if not bool(torch.isfinite(q["W"]).all().item())]
     rate = lambda t: ncell * 3 * steps / t / 1e6
 
     # the same cases with the positivity limiter of the face states (positivity.is_interpolation_limiter)
@@ -281,7 +282,8 @@ def bench_ensemble(steps: int, warmup: int, ncases: int = 64, seed: int = 0, nh:
         "speedup_vs_per_case_graph": round(t_per_graph / t_graph, 3),
         "batched_vs_per_case_max_rel_diff_10_steps": max(d10) if d10 else None,
         "cases_within_1e-10_after_all_steps": int(sum(1 for d in d200 if d == d and d <= 1e-10)),
-        "finite_cases": finite,
+        "finite_cases": ncases - len(nonfinite),
+        "nonfinite_cases": nonfinite, "nonfinite_cases_per_case_path": nonfinite_per,
         "with_interpolation_limiter": limited,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

@@ -72,6 +72,7 @@ extern "C" {
 #define JFA_SS_DAVIS 2
 #define JFA_SS_EINFELDT 3
 #define JFA_SS_TORO 4
+#define JFA_SS_DAVIS2 5  /* plain function only (jfa_signal_speeds) */
 /* frozen_state (solvers/__init__.py:5-7) */
 #define JFA_FROZEN_ARITHMETIC 0
 #define JFA_FROZEN_ROE 1
@@ -190,6 +191,19 @@ int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* g
  * scheme (JFA_FS_*) has no face states: JFA_EUNSUPPORTED. */
 int jfa_riemann_xi(const void* W_L, const void* W_R, const void* U_L, const void* U_R, int64_t nfaces,
                    int32_t axis, const jfa_scheme* scheme, void* flux, void* stream);
+
+/* Signal-speed estimates (S_L, S_R) of n left/right state pairs, elementwise: the plain functions
+ * signal_speed_Arithmetic, _Rusanov, _Davis, _Davis_2, _Einfeldt, _Toro
+ * (solvers/riemann_solvers/signal_speeds.py:10-157; registry DICT_SIGNAL_SPEEDS,
+ * solvers/riemann_solvers/__init__.py:27-34) that Riemann solvers call, with the same formulas the
+ * face kernels evaluate.  kind: JFA_SS_*.  Every array holds n contiguous values of `dtype`:
+ * u_L, u_R (normal velocities) and a_L, a_R (speeds of sound) always; rho_L, rho_R for EINFELDT,
+ * TORO and DAVIS2; p_L, p_R for TORO; H_L, H_R (total enthalpies) for DAVIS2.  Inputs a kind does
+ * not use may be NULL.  gamma: specific heat ratio (TORO, DAVIS2).  Outputs S_L, S_R: n values. */
+int jfa_signal_speeds(int32_t kind, const void* u_L, const void* u_R, const void* a_L, const void* a_R,
+                      const void* rho_L, const void* rho_R, const void* p_L, const void* p_R, const void* H_L,
+                      const void* H_R, double gamma, int64_t n, int32_t dtype, void* S_L, void* S_R,
+                      void* stream);
 
 /* Numerical flux along `axis` on every face: reconstruction (4 variable modes), prim<->cons
  * conversion, the positivity limiter when the scheme enables it, and Riemann solve.

@@ -25,7 +25,10 @@ RIEMANN = {"HLLC": 0, "RUSANOV": 1, "HLL": 2, "HLLC-LM": 3, "AUSMP": 4}
 # convective_solver FLUX-SPLITTING with its eigenvalues (flux_splitting), in place of a Riemann solver
 FLUX_SPLITTING = {"ROE": 5, "CLLF": 6, "LLF": 7}
 RIEMANN.update({"FLUX-SPLITTING-" + k: v for k, v in FLUX_SPLITTING.items()})
-SIGNAL_SPEED = {"ARITHMETIC": 0, "RUSANOV": 1, "DAVIS": 2, "EINFELDT": 3, "TORO": 4}
+SIGNAL_SPEED = {"ARITHMETIC": 0, "RUSANOV": 1, "DAVIS": 2, "EINFELDT": 3, "TORO": 4, "DAVIS2": 5}
+# Riemann solvers that call their signal speed with (u, a, rho_L=, rho_R=, p_L=, p_R=, gamma=)
+# (HLLC.py:93-102, HLL.py:38-47, HLLCLM.py:72-81)
+_SIGNAL_SPEED_CALLERS = ("HLLC", "HLL", "HLLC-LM")
 FROZEN = {"ARITHMETIC": 0, "ROE": 1}
 BC = {"PERIODIC": 0, "ZEROGRADIENT": 1, "SYMMETRY": 2}
 F64, F32 = 0, 1
@@ -91,6 +94,7 @@ SIGNATURES = {
     "jfa_last_error": (ctypes.c_char_p, []),
     "jfa_weno5_reconstruct_xi": (_I32, [_P, _I32, _PG, _I32, _I32, _I32, _D, _I32, _P, _P]),
     "jfa_riemann_xi": (_I32, [_P, _P, _P, _P, _I64, _I32, _PS, _P, _P]),
+    "jfa_signal_speeds": (_I32, [_I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _D, _I64, _I32, _P, _P, _P]),
     "jfa_compute_flux_xi": (_I32, [_P, _P, _PG, _PS, _I32, _P, _P, _P]),
     "jfa_compute_rhs": (_I32, [_P, _P, _PG, _PS, _D, _P, _P, _SZ, _P]),
     "jfa_rhs_workspace_bytes": (_SZ, [_PG, _I32]),
@@ -213,6 +217,12 @@ def make_scheme(recon_var: str, stencil: str, riemann: str, signal_speed: Option
     / "NASA" with ``flux_partition`` "UNIFORM" / "CELLSIZE" and its epsilons, the positivity flux
     limiter of the stage and RHS entry points."""
     s = JfaScheme()
+    if signal_speed == "DAVIS2":
+        if riemann in _SIGNAL_SPEED_CALLERS:
+            # what the reference raises when its Riemann solver calls signal_speed_Davis_2
+            # (signal_speeds.py:71-73) with p_L/p_R instead of the total enthalpies
+            raise TypeError("signal_speed_Davis_2() missing 2 required positional arguments: 'H_L' and 'H_R'")
+        signal_speed = None  # the Riemann solver does not call a signal speed
     try:
         s.recon_var = RECON_VAR[recon_var]
         s.stencil = STENCIL[stencil]

@@ -17,7 +17,7 @@ import torch
 
 from . import _native
 from .config import precision
-from .riemann_solvers import AUSMP, HLL, HLLC, HLLCLM, Rusanov, SignalSpeed
+from .riemann_solvers import AUSMP, HLL, HLLC, HLLCLM, Rusanov, signal_speed_name
 from .stencils import DICT_SPATIAL_RECONSTRUCTION
 
 
@@ -122,8 +122,7 @@ class HighOrderGodunov(_DeviceFaceFlux):
         if stencil_name is None or riemann_name is None:
             raise NotImplementedError(
                 f"built on this path: stencils {tuple(DICT_SPATIAL_RECONSTRUCTION)} with HLLC, HLLC-LM, HLL, AUSM+ or Rusanov")
-        ss = godunov.signal_speed
-        ss_name = ss.name if isinstance(ss, SignalSpeed) else (ss if isinstance(ss, str) else "EINFELDT")
+        ss_name = signal_speed_name(godunov.signal_speed)
         self._names = (self.reconstruction_variable, stencil_name, riemann_name, ss_name, self.frozen_state)
 
 class FluxSplittingScheme(_DeviceFaceFlux):

@@ -284,11 +284,11 @@ __device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& pr
             bc.t[5] = JFA_BC_PERIODIC;
             const int c3[3] = {x, y, 0};
             halo_scatter(Wo, Uo, G, bc, c3, bcell, w, uh, prm);
-            if (a.part) vmax = fmax(vmax, wave_sum(w, G.active, prm.gamma));
+            if (a.part) vmax = nan_max(vmax, wave_sum(w, G.active, prm.gamma));
         }
     }
     if (a.part) {  // a.part points at this item's slot
-        for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o, 64));
+        for (int o = 32; o > 0; o >>= 1) vmax = nan_max(vmax, __shfl_xor(vmax, o, 64));
         if (lane == 0) a.part[0] = vmax;
     }
 }
@@ -315,12 +315,12 @@ __global__ __launch_bounds__(256) void batch2d_dt_kernel(const Case2D* __restric
     __shared__ double sm[4];
     const Case2D cs = cases[blockIdx.x];
     double v = 0.0;
-    for (int i = cs.item_begin + threadIdx.x; i < cs.item_end; i += blockDim.x) v = fmax(v, part[i]);
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int i = cs.item_begin + threadIdx.x; i < cs.item_end; i += blockDim.x) v = nan_max(v, part[i]);
+    for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 64));
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double mx = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+        const double mx = nan_max(nan_max(sm[0], sm[1]), nan_max(sm[2], sm[3]));
         dt[blockIdx.x] = cs.dx_min / (mx + eps) * cfl;
     }
 }

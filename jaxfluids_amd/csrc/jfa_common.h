@@ -126,8 +126,13 @@ __device__ __forceinline__ void rk3_coeffs(int stage, double& c0, double& c1, do
 
 // ------------------------------------------------------------------ CFL reduction
 // max over interior cells of sum_active(|u_i| + c) (time_step_size.py:103-109).
-// Non-negative doubles order like their bit patterns, so a 64-bit atomicMax on the bits
-// is an exact, order-independent max (a NaN propagates as a huge value).
+// The reference's jnp.max returns NaN as soon as one cell is NaN, so its dt turns NaN and the
+// time loop ends; every maximum of these reductions is therefore taken with nan_max, which
+// returns NaN if either operand is NaN (fmax would drop it).  Non-negative doubles and NaN
+// (positive or negative) order like their bit patterns read as unsigned 64-bit integers, so a
+// 64-bit atomicMax on the bits is an exact, order-independent max that keeps a NaN.
+__device__ __forceinline__ double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
 __device__ __forceinline__ double wave_sum(const double w[5], const int act[3], double gamma) {
     const double c = sound_speed(w[4], w[0], gamma);
     double s = 0.0;
@@ -141,13 +146,13 @@ __device__ __forceinline__ double wave_sum(const double w[5], const int act[3], 
 // atomic per block.  Every thread of the block must call it.
 __device__ __forceinline__ void block_max_to(unsigned long long* slot, double v) {
     __shared__ double part[kBlock / 64];
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 64));
     const int wid = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) part[wid] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = part[0];
-        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, part[i]);
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = nan_max(m, part[i]);
         atomicMax(slot, (unsigned long long)__double_as_longlong(m));
     }
 }

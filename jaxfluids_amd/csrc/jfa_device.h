@@ -682,6 +682,20 @@ __device__ __forceinline__ void signal_speeds(int kind, double uL, double uR, do
     }
 }
 
+// signal_speed_Davis_2 (signal_speeds.py:99-107): u_Roe -/+ a_Roe with the sqrt(rho)-weighted
+// averages of u and of the total enthalpy H.  Only the plain-function entry point calls it: the
+// reference's Riemann solvers pass p_L/p_R, not H_L/H_R, so it cannot run inside them.
+__device__ __forceinline__ void signal_speeds_davis2(double uL, double uR, double rL, double rR, double HL,
+                                                     double HR, double gamma, double& SL, double& SR) {
+    const double sL = rL * rsq1(rL), sR = rR * rsq1(rR);  // sqrt(rho)
+    const double one_dens = rcp1(sL + sR);
+    const double u_roe = (sL * uL + sR * uR) * one_dens;
+    const double H_roe = (sL * HL + sR * HR) * one_dens;
+    const double a_roe = fsqrt((gamma - 1.0) * (H_roe - 0.5 * u_roe * u_roe));
+    SL = u_roe - a_roe;
+    SR = u_roe + a_roe;
+}
+
 // ---------------------------------------------------------------- Riemann solvers
 // HLLC._compute_flux_star_K_single_phase (HLLC.py:41-78) for one side K with wave speed SK,
 // its clipped value S_clip (min(.,0) for L, max(.,0) for R) and contact speed Ss.

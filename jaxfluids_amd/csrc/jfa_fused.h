@@ -160,12 +160,12 @@ struct InnerArgs {
 // Block max written to part[b] (no atomics); every thread must call it.
 __device__ __forceinline__ void block_max_part(double* part, double v, uint32_t b) {
     __shared__ double sm[kInnerBlock / 64];
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 64));
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = sm[0];
-        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sm[i]);
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = nan_max(m, sm[i]);
         part[b] = m;
     }
 }
@@ -175,9 +175,9 @@ __device__ __forceinline__ void block_max_part3(double* part, double v0, double 
                                                 uint32_t nb) {
     __shared__ double sm[3][kInnerBlock / 64];
     for (int o = 32; o > 0; o >>= 1) {
-        v0 = fmax(v0, __shfl_xor(v0, o, 64));
-        v1 = fmax(v1, __shfl_xor(v1, o, 64));
-        v2 = fmax(v2, __shfl_xor(v2, o, 64));
+        v0 = nan_max(v0, __shfl_xor(v0, o, 64));
+        v1 = nan_max(v1, __shfl_xor(v1, o, 64));
+        v2 = nan_max(v2, __shfl_xor(v2, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
         sm[0][threadIdx.x >> 6] = v0;
@@ -188,7 +188,7 @@ __device__ __forceinline__ void block_max_part3(double* part, double v0, double 
     if (threadIdx.x == 0) {
         for (int q = 0; q < 3; ++q) {
             double m = sm[q][0];
-            for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, sm[q][i]);
+            for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = nan_max(m, sm[q][i]);
             part[q * nb + b] = m;
         }
     }
@@ -225,12 +225,12 @@ __device__ __forceinline__ void finish_cell(const Geo& G, const Params& prm, con
     cons_from_prim(w, prm, uh);
     halo_scatter(args.Wout, args.Uout, G, args.bc, c, b, w, uh, prm);
     if (args.cfl_part) {
-        vmax = fmax(vmax, wave_sum(w, G.active, prm.gamma));
+        vmax = nan_max(vmax, wave_sum(w, G.active, prm.gamma));
         if constexpr (DISS) {
             double nu, al;
             diffusivities(w, prm, nu, al);
-            nu_max = fmax(nu_max, nu);
-            al_max = fmax(al_max, al);
+            nu_max = nan_max(nu_max, nu);
+            al_max = nan_max(al_max, al);
         }
     }
 }
@@ -312,8 +312,8 @@ __global__ __launch_bounds__(1024) void dt_from_parts_kernel(const double* __res
     const int nq = dissip ? 3 : 1;
     for (int q = 0; q < nq; ++q) {
         double v = 0.0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, part[q * n + i]);
-        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        for (int i = threadIdx.x; i < n; i += blockDim.x) v = nan_max(v, part[q * n + i]);
+        for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 64));
         if ((threadIdx.x & 63) == 0) sm[q][threadIdx.x >> 6] = v;
     }
     __syncthreads();
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(1024) void dt_from_parts_kernel(const double* __res
         double mx[3] = {0.0, 0.0, 0.0};
         for (int q = 0; q < nq; ++q) {
             mx[q] = sm[q][0];
-            for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx[q] = fmax(mx[q], sm[q][i]);
+            for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx[q] = nan_max(mx[q], sm[q][i]);
         }
         double d = dx_min / (mx[0] + eps);
         if (dissip & 1) d = fmin(d, 3.0 / 14.0 * dx_min * dx_min / (mx[1] + eps));

@@ -57,6 +57,30 @@ __global__ __launch_bounds__(kBlock) void riemann_kernel(const T* __restrict__ W
     for (int i = 0; i < 5; ++i) flux[i * n + t] = static_cast<T>(Fo[i]);
 }
 
+// ------------------------------------------------------------------ signal speeds (plain functions)
+template <class T>
+__global__ __launch_bounds__(kBlock) void signal_speed_kernel(int kind, const T* __restrict__ uL,
+                                                              const T* __restrict__ uR, const T* __restrict__ aL,
+                                                              const T* __restrict__ aR, const T* __restrict__ rL,
+                                                              const T* __restrict__ rR, const T* __restrict__ pL,
+                                                              const T* __restrict__ pR, const T* __restrict__ HL,
+                                                              const T* __restrict__ HR, double gamma, int64_t n,
+                                                              T* __restrict__ SL, T* __restrict__ SR) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double sl, sr;
+    if (kind == JFA_SS_DAVIS2) {
+        signal_speeds_davis2(ld(uL, t), ld(uR, t), ld(rL, t), ld(rR, t), ld(HL, t), ld(HR, t), gamma, sl, sr);
+    } else {
+        const bool rho = kind == JFA_SS_EINFELDT || kind == JFA_SS_TORO;
+        const bool p = kind == JFA_SS_TORO;
+        signal_speeds(kind, ld(uL, t), ld(uR, t), ld(aL, t), ld(aR, t), rho ? ld(rL, t) : 0.0,
+                      rho ? ld(rR, t) : 0.0, p ? ld(pL, t) : 0.0, p ? ld(pR, t) : 0.0, gamma, sl, sr);
+    }
+    SL[t] = static_cast<T>(sl);
+    SR[t] = static_cast<T>(sr);
+}
+
 // ------------------------------------------------------------------ face flux (compute_flux_xi)
 // One thread per face.  Loads the 6-cell window of the face along AXIS and evaluates
 // reconstruction + Riemann in registers.
@@ -215,12 +239,12 @@ __global__ __launch_bounds__(kBlock) void cfl_max_kernel(const T* __restrict__ W
         double w[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) w[i] = ld(W, i * G.fs + b);
-        v = fmax(v, wave_sum(w, G.active, prm.gamma));
+        v = nan_max(v, wave_sum(w, G.active, prm.gamma));
         if (prm.dissip) {
             double nu, al;
             diffusivities(w, prm, nu, al);
-            vn = fmax(vn, nu);
-            va = fmax(va, al);
+            vn = nan_max(vn, nu);
+            va = nan_max(va, al);
         }
     }
     block_max_to(slot, v);
@@ -539,6 +563,34 @@ int jfa_prim_from_cons(const void* U, void* W, int64_t ncells, double gamma, int
     return check_launch();
 }
 
+int jfa_signal_speeds(int32_t kind, const void* u_L, const void* u_R, const void* a_L, const void* a_R,
+                      const void* rho_L, const void* rho_R, const void* p_L, const void* p_R, const void* H_L,
+                      const void* H_R, double gamma, int64_t n, int32_t dtype, void* S_L, void* S_R,
+                      void* stream) {
+    if (kind < JFA_SS_ARITHMETIC || kind > JFA_SS_DAVIS2) return JFA_EUNSUPPORTED;
+    if (n < 0 || !S_L || !S_R || S_L == S_R) return JFA_EINVAL;
+    if (n == 0) return JFA_OK;
+    const bool need_rho = kind == JFA_SS_EINFELDT || kind == JFA_SS_TORO || kind == JFA_SS_DAVIS2;
+    if (!u_L || !u_R || (kind != JFA_SS_DAVIS2 && (!a_L || !a_R)) || (need_rho && (!rho_L || !rho_R)) ||
+        (kind == JFA_SS_TORO && (!p_L || !p_R)) || (kind == JFA_SS_DAVIS2 && (!H_L || !H_R)))
+        return JFA_EINVAL;
+    const dim3 g = flat_grid(n);
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == JFA_F64)
+        hipLaunchKernelGGL(signal_speed_kernel<double>, g, dim3(kBlock), 0, st, kind, (const double*)u_L,
+                           (const double*)u_R, (const double*)a_L, (const double*)a_R, (const double*)rho_L,
+                           (const double*)rho_R, (const double*)p_L, (const double*)p_R, (const double*)H_L,
+                           (const double*)H_R, gamma, n, (double*)S_L, (double*)S_R);
+    else if (dtype == JFA_F32)
+        hipLaunchKernelGGL(signal_speed_kernel<float>, g, dim3(kBlock), 0, st, kind, (const float*)u_L,
+                           (const float*)u_R, (const float*)a_L, (const float*)a_R, (const float*)rho_L,
+                           (const float*)rho_R, (const float*)p_L, (const float*)p_R, (const float*)H_L,
+                           (const float*)H_R, gamma, n, (float*)S_L, (float*)S_R);
+    else
+        return JFA_EUNSUPPORTED;
+    return check_launch();
+}
+
 size_t jfa_stage_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme) {
     Geo G;
     if (!scheme || make_geo(grid, G)) return 0;
@@ -546,10 +598,14 @@ size_t jfa_stage_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme)
     return 5 * (size_t)G.n[0] * G.n[1] * G.n[2] * sizeof(double) + (size_t)fused_partials(G) * sizeof(double) + 256;
 }
 
-int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out,
-                        const jfa_grid* grid, const jfa_scheme* scheme, const int32_t bc[6], int32_t stage,
-                        const double* dt_dev, double dt_host, double cfl, double eps, double* dt_next_dev,
-                        void* workspace, size_t workspace_bytes, void* stream) {
+}  // extern "C"
+
+// Argument checks and launches of the fused stage with its scratch (FP64 RHS, then the CFL
+// partials) at `workspace`.
+static int fused_stage_entry(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out,
+                             const jfa_grid* grid, const jfa_scheme* scheme, const int32_t bc[6], int32_t stage,
+                             const double* dt_dev, double dt_host, double cfl, double eps, double* dt_next_dev,
+                             void* workspace, size_t workspace_bytes, void* stream) {
     Geo G;
     int rc = make_geo(grid, G);
     if (rc) return rc;
@@ -578,6 +634,15 @@ int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, voi
                                dt_next_dev, R, part, st);
     return fused_stage_f64(W_in, U_in, U_n, W_out, U_out, G, scheme, b6, stage, dt_dev, dt_host, cfl, eps, dt_next_dev,
                            R, part, st);
+}
+
+extern "C" {
+int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out,
+                        const jfa_grid* grid, const jfa_scheme* scheme, const int32_t bc[6], int32_t stage,
+                        const double* dt_dev, double dt_host, double cfl, double eps, double* dt_next_dev,
+                        void* workspace, size_t workspace_bytes, void* stream) {
+    return fused_stage_entry(W_in, U_in, U_n, W_out, U_out, grid, scheme, bc, stage, dt_dev, dt_host, cfl, eps,
+                             dt_next_dev, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

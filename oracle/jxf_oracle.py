@@ -676,6 +676,19 @@ def signal_speed_davis(u_L, u_R, a_L, a_R, **kw):
     return np.minimum(u_L - a_L, u_R - a_R), np.maximum(u_L + a_L, u_R + a_R)
 
 
+def signal_speed_davis_2(u_L, u_R, a_L, a_R, rho_L, rho_R, H_L, H_R, gamma, **kw):
+    """signal_speeds.py:71-107: u_Roe -/+ a_Roe from the Roe averages of u and the total
+    enthalpy H.  HLLC/HLL pass p_L/p_R, not H_L/H_R, so the reference cannot use it inside a
+    Riemann solver (its H_L/H_R are missing there); it is a plain function only."""
+    sqrt_rho_L = np.sqrt(rho_L)
+    sqrt_rho_R = np.sqrt(rho_R)
+    one_dens = 1.0 / (sqrt_rho_L + sqrt_rho_R)
+    u_Roe = (sqrt_rho_L * u_L + sqrt_rho_R * u_R) * one_dens
+    H_Roe = (sqrt_rho_L * H_L + sqrt_rho_R * H_R) * one_dens
+    a_Roe = np.sqrt((gamma - 1) * (H_Roe - 0.5 * u_Roe * u_Roe))
+    return u_Roe - a_Roe, u_Roe + a_Roe
+
+
 def signal_speed_einfeldt(u_L, u_R, a_L, a_R, rho_L, rho_R, **kw):
     """signal_speeds.py:109-133 (Toro 10.52-10.54)."""
     sqrt_rho_L = np.sqrt(rho_L)
@@ -698,17 +711,24 @@ def estimate_pressure(u_L, u_R, a_L, a_R, rho_L, rho_R, p_L, p_R):
 def signal_speed_toro(u_L, u_R, a_L, a_R, rho_L, rho_R, p_L, p_R, gamma, **kw):
     """signal_speeds.py:135-157 (Toro 10.59-10.60)."""
     p_star = estimate_pressure(u_L, u_R, a_L, a_R, rho_L, rho_R, p_L, p_R)
+    q_L = toro_q(p_star, p_L, gamma)
+    q_R = toro_q(p_star, p_R, gamma)
+    return u_L - a_L * q_L, u_R + a_R * q_R
+
+
+def toro_q(p_star, p_K, gamma):
+    """signal_speeds.py:152-154: Toro's wave-speed factor q_K (10.61): 1 for a rarefaction
+    (p* <= p_K), sqrt(1 + (gamma+1)/(2 gamma) (p*/p_K - 1)) for a shock."""
     gamma_ = (gamma + 1) * 0.5 / gamma
     with np.errstate(invalid="ignore"):
-        q_L = 1.0 * (p_star <= p_L) + np.sqrt(1 + gamma_ * (p_star / p_L - 1)) * (p_star > p_L)
-        q_R = 1.0 * (p_star <= p_R) + np.sqrt(1 + gamma_ * (p_star / p_R - 1)) * (p_star > p_R)
-    return u_L - a_L * q_L, u_R + a_R * q_R
+        return 1.0 * (p_star <= p_K) + np.sqrt(1 + gamma_ * (p_star / p_K - 1)) * (p_star > p_K)
 
 
 SIGNAL_SPEEDS = {
     "ARITHMETIC": signal_speed_arithmetic,
     "RUSANOV": signal_speed_rusanov,
     "DAVIS": signal_speed_davis,
+    "DAVIS2": signal_speed_davis_2,
     "EINFELDT": signal_speed_einfeldt,
     "TORO": signal_speed_toro,
 }

@@ -238,3 +238,29 @@ def test_emulated_batch_rejects_bad_tables(emu):
     assert emu.jfa_batch2d_workspace_bytes(ctypes.byref(t), 1, 16) == 0
     t[0].n[0], t[0].n[1] = 20000, 20000
     assert emu.jfa_batch2d_workspace_bytes(ctypes.byref(t), 1, 16) == 0
+
+
+def test_emulated_batch_time_step_propagates_nan(emu):
+    """A NaN cell in one case makes that case's next dt NaN (the reference's jnp.max propagates
+    NaN, time_step_size.py:109); the other cases keep the oracle's dt."""
+    scheme = SCHEMES[0]
+    prims = [case_state(c[0], c[1], seed=900 + i) for i, c in enumerate(CASES)]
+    ens = HostEnsemble(emu, scheme, chunk=16)
+    refs = [oracle_case(i, scheme, p) for i, p in enumerate(prims)]
+    bad = 2
+    for i, (dom, s, W, U) in enumerate(refs):
+        ens.view(0, i)[...] = W
+        ens.view(1, i)[...] = U
+        ens.dt[i] = o.compute_dt(W, dom, s)
+    dom = refs[bad][0]
+    ens.view(0, bad)[0, dom.nh + 3, dom.nh + 5, 0] = np.nan
+    ens.view(1, bad)[0, dom.nh + 3, dom.nh + 5, 0] = np.nan
+    dt0 = ens.dt.copy()
+    ens.step()
+    assert emu.emu_violations() == 0, hex(emu.emu_first_violation())
+    for i, (dom, s, W, U) in enumerate(refs):
+        if i == bad:
+            assert np.isnan(ens.dt[i])
+            continue
+        dtr = o.compute_dt(o.rk3_step(W, U, dt0[i], dom, s)[0], dom, s)
+        assert abs(ens.dt[i] - dtr) <= 1e-14 * dtr, (i, ens.dt[i], dtr)

@@ -212,3 +212,20 @@ def test_fused_step_flux_splitting_with_flux_limiter(emu, fl):
     Wg, Ug, _ = fused_step(emu, dom, s, W, U, dt)
     I = dom.interior
     assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL
+
+
+@pytest.mark.parametrize("grid", ["3d", "2d-xy", "1d"])
+def test_fused_cfl_time_step_propagates_nan(emu, grid):
+    """A NaN cell makes the next dt NaN, as the reference's jnp.max (time_step_size.py:109) does:
+    the wave-speed maxima (wave shuffles, block partials and the final reduction) keep a NaN
+    instead of dropping it."""
+    dom, W0 = grid_state(grid, seed=31)
+    s = o.Setup(gamma=1.4, **SCHEMES["z-charprim-hllc"], bcs={f: "PERIODIC" for f in FACES})
+    W, U = o.create_buffers(W0, dom, s)
+    dt = o.compute_dt(W, dom, s)
+    c = tuple(n // 2 + dom.nh if n > 1 else 0 for n in dom.cells)
+    W[(0,) + c] = np.nan
+    U[(0,) + c] = np.nan
+    assert np.isnan(o.compute_dt(W, dom, s))  # the oracle's np.max propagates NaN
+    _, _, dtn = fused_step(emu, dom, s, W, U, dt)
+    assert np.isnan(dtn)

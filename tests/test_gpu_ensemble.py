@@ -290,3 +290,58 @@ def test_ensemble_with_flux_limiter(fl):
         Wg = host(ens.state(i)[0])
         assert np.all(np.isfinite(W[dom.interior]))
         assert rel(Wg[dom.interior], W[dom.interior]) < TOL, (i, rel(Wg[dom.interior], W[dom.interior]))
+
+
+@pytest.mark.parametrize("var", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+def test_four_quadrant_workload_matches_oracle(var):
+    """The four-quadrant Riemann ensemble (seeded quadrant states rho ~ U(0.1, 1.5),
+    p ~ U(0.03, 1.5), u, v ~ U(-1, 1); WENO5-JS + HLLC-Einfeldt, ZEROGRADIENT, nh = 5, per-case CFL
+    dt) at reduced grid sizes drawn from {24, 32, 48}^2: three steps of 8 cases in one ensemble,
+    each against the oracle run on that case alone, state and dt."""
+    from jaxfluids_amd.ensemble import Ensemble2D
+    from jaxfluids_amd.workloads import four_quadrant_cases, quadrant_primitives
+
+    say(f"[four-quadrant] {var}")
+    qc = four_quadrant_cases(8, seed=0, sizes=(24, 32, 48))
+    zg = {f: "ZEROGRADIENT" for f in FACES2}
+    ens = Ensemble2D([{"cells": (c.nx, c.ny), "ranges": ((0.0, 1.0), (0.0, 1.0)), "bcs": zg} for c in qc],
+                     stencil="WENO5-JS", variable=var, riemann="HLLC", signal_speed="EINFELDT", nh=5, chunk=16)
+    prims = [quadrant_primitives(c) for c in qc]
+    ens.initialize(prims)
+    refs = []
+    for c, p in zip(qc, prims):
+        dom = o.Domain((c.nx, c.ny, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), nh=5)
+        s = o.Setup(gamma=1.4, stencil="WENO5-JS", variable=var, riemann="HLLC", signal_speed="EINFELDT",
+                    bcs={**zg, "top": "PERIODIC", "bottom": "PERIODIC"})
+        W, U = o.create_buffers(p, dom, s)
+        refs.append([dom, s, W, U, o.compute_dt(W, dom, s)])
+    for step in range(3):
+        ens.step(1)
+        got_dt = host(ens.dt)
+        for i, r in enumerate(refs):
+            dom, s, W, U, dt = r
+            W, U = o.rk3_step(W, U, dt, dom, s)
+            r[2:] = [W, U, o.compute_dt(W, dom, s)]
+            I = dom.interior
+            assert np.all(np.isfinite(W[I])), (step, i)
+            Wg, Ug = host(ens.state(i)[0]), host(ens.state(i)[1])
+            for reg in face_regions(dom):
+                assert rel(Wg[reg], W[reg]) < TOL, (step, i, rel(Wg[reg], W[reg]))
+                assert rel(Ug[reg], U[reg]) < TOL, (step, i, rel(Ug[reg], U[reg]))
+            assert abs(got_dt[i] - r[4]) <= 1e-14 * r[4], (step, i, got_dt[i], r[4])
+
+
+def test_ensemble_time_step_propagates_nan():
+    """A NaN cell in one case makes that case's dt NaN (the reference's jnp.max propagates NaN,
+    time_step_size.py:109); the other cases keep finite time steps."""
+    say("[ensemble nan dt]")
+    scheme = SCHEMES[1]
+    prims = [case_state(c[0], c[1], seed=600 + i) for i, c in enumerate(CASES)]
+    prims[2][0, 10, 20, 0] = np.nan
+    ens = make_ensemble(scheme)
+    ens.initialize(prims)
+    dt0 = host(ens.dt)
+    assert np.isnan(dt0[2]) and np.all(np.isfinite(np.delete(dt0, 2)))
+    ens.step(1)
+    dt1 = host(ens.dt)
+    assert np.isnan(dt1[2]) and np.all(np.isfinite(np.delete(dt1, 2)))

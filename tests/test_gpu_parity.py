@@ -382,3 +382,32 @@ def test_fp32_storage_error_stated():
     err = rel(host(Wd)[I], Wr[I])
     say(f"    fp32-storage rel Linf vs FP64 oracle: {err:.3e}")
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_cfl_time_step_propagates_nan(fused):
+    """A NaN cell makes dt NaN, as the reference's jnp.max does (time_step_size.py:109): the
+    standalone reduction (jfa_compute_dt) and the fused stage's in-kernel partials; the
+    reference's time loop then ends (NaN < end_time is false), and so does advance()."""
+    from jaxfluids_amd import SimulationManager
+    from jaxfluids_amd.data_types import JaxFluidsBuffers, MaterialFieldBuffers, TimeControlVariables
+
+    say(f"[nan dt] fused={fused}")
+    s = _setup(FLUX_CASES[0], {f: "PERIODIC" for f in FACES})
+    dom, W0 = random_state((11, 9, 8), 4, seed=23)
+    W, U = o.create_buffers(W0, dom, s)
+    dt = o.compute_dt(W, dom, s)
+    c = (0, dom.nh + 4, dom.nh + 3, dom.nh + 2)
+    W[c] = np.nan
+    U[c] = np.nan
+    assert np.isnan(o.compute_dt(W, dom, s))
+    sim = SimulationManager(_input_manager(dom, s, end_step=5), fused=fused)
+    Wd, Ud = dev(W), dev(U)
+    assert math.isnan(float(host(sim.compute_time_step(Wd))[0]))
+    dtd = torch.full((1,), dt, dtype=torch.float64, device="cuda")
+    tcv = TimeControlVariables(physical_simulation_time=0.0, simulation_step=0, physical_timestep_size=dtd,
+                               end_time=1e30, end_step=5)
+    out = sim.advance(JaxFluidsBuffers(MaterialFieldBuffers(Ud, Wd, None), tcv))
+    t = out.time_control_variables
+    assert t.simulation_step == 2 and math.isnan(t.physical_simulation_time), t  # step 1 ran with a finite dt
+    assert math.isnan(float(host(t.physical_timestep_size)[0]))

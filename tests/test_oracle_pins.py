@@ -283,6 +283,192 @@ def test_rusanov_closed_form():
     assert rel(o.rusanov(WL, WR, UL, UR, 1, g), expect) < 1e-14
 
 
+def _states_1d(n, rng, gamma):
+    rho_L, rho_R = rng.uniform(0.2, 2.0, n), rng.uniform(0.2, 2.0, n)
+    u_L, u_R = rng.uniform(-1.0, 1.0, n), rng.uniform(-1.0, 1.0, n)
+    p_L, p_R = rng.uniform(0.2, 3.0, n), rng.uniform(0.2, 3.0, n)
+    a_L, a_R = np.sqrt(gamma * p_L / rho_L), np.sqrt(gamma * p_R / rho_R)
+    return rho_L, rho_R, u_L, u_R, p_L, p_R, a_L, a_R
+
+
+def _roe_average_1d(rho_L, rho_R, u_L, u_R, a_L, a_R, gamma):
+    """Roe's averages (Toro 11.60): sqrt(rho)-weighted velocity and total enthalpy
+    H = a^2/(gamma-1) + u^2/2, sound speed a~^2 = (gamma-1)(H~ - u~^2/2)."""
+    wL, wR = np.sqrt(rho_L), np.sqrt(rho_R)
+    H_L = a_L ** 2 / (gamma - 1) + 0.5 * u_L ** 2
+    H_R = a_R ** 2 / (gamma - 1) + 0.5 * u_R ** 2
+    u = (wL * u_L + wR * u_R) / (wL + wR)
+    H = (wL * H_L + wR * H_R) / (wL + wR)
+    return u, np.sqrt((gamma - 1) * (H - 0.5 * u ** 2)), H_L, H_R
+
+
+def test_einfeldt_speed_is_the_roe_sound_speed_at_gamma_2():
+    """Einfeldt's d_bar^2 = (sqrt(rho_L) a_L^2 + sqrt(rho_R) a_R^2)/(sqrt(rho_L)+sqrt(rho_R))
+    + eta2 (u_R-u_L)^2 equals Roe's a~^2 = (gamma-1)(H~ - u~^2/2) exactly when
+    eta2 = (gamma-1)/2 * sqrt(rho_L rho_R)/(sqrt(rho_L)+sqrt(rho_R))^2, i.e. at gamma = 2 for
+    Einfeldt's eta2 = 1/2 sqrt(rho_L rho_R)/(...)^2 (Toro 10.53-10.54).  On random unequal
+    1-D states the oracle's (S_L, S_R) must be (min(u~-a~, u_L-a_L), max(u~+a~, u_R+a_R))."""
+    g = 2.0
+    rng = np.random.default_rng(77)
+    rho_L, rho_R, u_L, u_R, p_L, p_R, a_L, a_R = _states_1d(400, rng, g)
+    u_L, u_R = u_L + 0.5, u_R - 0.5  # compressive pairs: the Roe branch wins the min/max often
+    S_L, S_R = o.signal_speed_einfeldt(u_L, u_R, a_L, a_R, rho_L=rho_L, rho_R=rho_R, p_L=p_L, p_R=p_R, gamma=g)
+    u, a, _, _ = _roe_average_1d(rho_L, rho_R, u_L, u_R, a_L, a_R, g)
+    assert rel(S_L, np.minimum(u - a, u_L - a_L)) < 1e-14
+    assert rel(S_R, np.maximum(u + a, u_R + a_R)) < 1e-14
+    # the test has teeth only where the Roe branch is taken, by a margin
+    assert np.mean(u - a < u_L - a_L - 1e-3) > 0.3 and np.mean(u + a > u_R + a_R + 1e-3) > 0.3
+
+
+def test_davis2_speeds_satisfy_roes_property():
+    """DAVIS2 returns u~ -/+ a~ from Roe's averages.  Roe's matrix A(u~, H~) -- the Euler
+    Jacobian at any state with velocity u~ and total enthalpy H~ -- satisfies
+    F(U_R) - F(U_L) = A (U_R - U_L) exactly (Roe 1981).  The Jacobian is taken by central
+    differences of the oracle's physical flux, so the check holds to ~1e-7 and a wrong average
+    (arithmetic means, a dropped weight) fails it by O(1)."""
+    g = 1.4
+    rng = np.random.default_rng(78)
+    n = 12
+    rho_L, rho_R, u_L, u_R, p_L, p_R, a_L, a_R = _states_1d(n, rng, g)
+    H_L = a_L ** 2 / (g - 1) + 0.5 * u_L ** 2
+    H_R = a_R ** 2 / (g - 1) + 0.5 * u_R ** 2
+    S_L, S_R = o.signal_speed_davis_2(u_L, u_R, a_L, a_R, rho_L, rho_R, H_L, H_R, g)
+    u_t, a_t = 0.5 * (S_L + S_R), 0.5 * (S_R - S_L)
+    H_t = a_t ** 2 / (g - 1) + 0.5 * u_t ** 2
+    zero = np.zeros(n)
+    WL = np.stack([rho_L, u_L, zero, zero, p_L])
+    WR = np.stack([rho_R, u_R, zero, zero, p_R])
+    UL, UR = o.cons_from_prim(WL, g), o.cons_from_prim(WR, g)
+    dF = o.physical_flux_xi(WR, UR, 0) - o.physical_flux_xi(WL, UL, 0)
+    for k in range(n):
+        rho = 1.0
+        W0 = np.array([rho, u_t[k], 0.0, 0.0, (g - 1) / g * rho * (H_t[k] - 0.5 * u_t[k] ** 2)]).reshape(5, 1)
+        U0 = o.cons_from_prim(W0, g)[:, 0]
+        A = np.zeros((5, 5))
+        for j in (0, 1, 4):
+            h = 1e-6 * max(1.0, abs(U0[j]))
+            Up, Um = U0.copy(), U0.copy()
+            Up[j] += h
+            Um[j] -= h
+            Fp = o.physical_flux_xi(o.prim_from_cons(Up.reshape(5, 1), g), Up.reshape(5, 1), 0)[:, 0]
+            Fm = o.physical_flux_xi(o.prim_from_cons(Um.reshape(5, 1), g), Um.reshape(5, 1), 0)[:, 0]
+            A[:, j] = (Fp - Fm) / (2 * h)
+        lhs, rhs = dF[:, k], A @ (UR[:, k] - UL[:, k])
+        assert np.max(np.abs(lhs - rhs)) < 1e-6 * (1 + np.max(np.abs(lhs))), k
+    # equal states: the Roe averages are the state itself
+    S_L, S_R = o.signal_speed_davis_2(u_L, u_L, a_L, a_L, rho_L, rho_L, H_L, H_L, g)
+    assert rel(S_L, u_L - a_L) < 1e-13 and rel(S_R, u_L + a_L) < 1e-13
+
+
+def _weak_pair(eps, direction):
+    """States (rho, u, p) = base -/+ eps/2 * direction."""
+    base = np.array([1.0, 0.2, 1.0])
+    L = base - 0.5 * eps * np.asarray(direction)
+    R = base + 0.5 * eps * np.asarray(direction)
+    return L, R
+
+
+@pytest.mark.parametrize("direction", [(0.3, -0.5, 0.4), (-0.2, 0.6, -0.3), (0.5, 0.1, -0.7)])
+def test_pvrs_pressure_converges_to_exact_star_pressure_at_second_order(direction):
+    """The PVRS estimate (Toro 9.28) linearises the star-pressure equation about the mean
+    state, so for a jump of size eps it misses the exact p* (Toro ch. 4 Newton solution) by
+    O(eps^2).  A wrong weight on the velocity or pressure term leaves an O(eps) error."""
+    g = 1.4
+    from tests.exact_riemann import star_state
+
+    errs = []
+    for eps in (0.04, 0.02, 0.01):
+        L, R = _weak_pair(eps, direction)
+        a_L, a_R = np.sqrt(g * L[2] / L[0]), np.sqrt(g * R[2] / R[0])
+        p_est = o.estimate_pressure(L[1], R[1], a_L, a_R, L[0], R[0], L[2], R[2])
+        p_ex, _ = star_state(L[0], L[1], L[2], R[0], R[1], R[2], g)
+        errs.append(abs(p_est - p_ex))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.3 < r1 < 4.7 and 3.3 < r2 < 4.7, (errs, r1, r2)
+    assert errs[-1] < 1e-3
+
+
+def _rh_shock_speed(rho_K, u_K, rho_s, u_s):
+    """Rankine-Hugoniot mass condition across a shock: S = [rho u] / [rho]."""
+    return (rho_s * u_s - rho_K * u_K) / (rho_s - rho_K)
+
+
+@pytest.mark.parametrize("case", [(1.0, 0.0, 1.0, 0.125, 0.0, 0.1),     # Sod: right shock
+                                  (1.0, 0.5, 0.4, 0.8, -0.6, 0.6),      # two shocks
+                                  (0.5, 0.9, 0.3, 1.2, -0.2, 2.5),      # left shock
+                                  (2.0, 0.0, 3.0, 1.0, 0.0, 0.2)])
+def test_toro_factor_with_the_exact_star_pressure_gives_the_exact_shock_speed(case):
+    """Toro's factor q_K (10.61) with the exact p* turns u_K -/+ a_K q_K into the exact shock
+    speed, which the Rankine-Hugoniot mass condition fixes from the exact solution's star
+    density (sampled just beside the contact); for a rarefaction (p* <= p_K) q_K = 1, i.e.
+    the head speed u_K -/+ a_K."""
+    g = 1.4
+    rho_L, u_L, p_L, rho_R, u_R, p_R = case
+    from tests.exact_riemann import star_state
+
+    p_s, u_s = star_state(rho_L, u_L, p_L, rho_R, u_R, p_R, g)
+    a_L, a_R = np.sqrt(g * p_L / rho_L), np.sqrt(g * p_R / rho_R)
+    q_L, q_R = o.toro_q(p_s, p_L, g), o.toro_q(p_s, p_R, g)
+    left = exact_sample(np.array([u_s - 1e-9]), rho_L, u_L, p_L, rho_R, u_R, p_R, g)
+    right = exact_sample(np.array([u_s + 1e-9]), rho_L, u_L, p_L, rho_R, u_R, p_R, g)
+    if p_s > p_L:
+        S = _rh_shock_speed(rho_L, u_L, left[0, 0], u_s)
+        assert abs((u_L - a_L * q_L) - S) < 1e-10 * (1 + abs(S))
+    else:
+        assert q_L == 1.0
+    if p_s > p_R:
+        S = _rh_shock_speed(rho_R, u_R, right[0, 0], u_s)
+        assert abs((u_R + a_R * q_R) - S) < 1e-10 * (1 + abs(S))
+    else:
+        assert q_R == 1.0
+    assert (p_s > p_L) or (p_s > p_R)  # every case exercises the shock branch at least once
+
+
+def test_toro_speeds_converge_to_the_exact_weak_shock_speed():
+    """With the PVRS p* the Toro speeds of a weak compressive pair differ from the exact
+    shock speeds (Rankine-Hugoniot from the exact solution) by O(eps^2)."""
+    g = 1.4
+    from tests.exact_riemann import star_state
+
+    errs = []
+    for eps in (0.08, 0.04, 0.02):
+        L, R = _weak_pair(eps, (0.1, -1.0, 0.05))  # u_L > u_R: shocks on both sides
+        a_L, a_R = np.sqrt(g * L[2] / L[0]), np.sqrt(g * R[2] / R[0])
+        S_L, S_R = o.signal_speed_toro(L[1], R[1], a_L, a_R, L[0], R[0], L[2], R[2], g)
+        p_s, u_s = star_state(L[0], L[1], L[2], R[0], R[1], R[2], g)
+        assert p_s > L[2] and p_s > R[2]
+        rl = exact_sample(np.array([u_s - 1e-9]), L[0], L[1], L[2], R[0], R[1], R[2], g)[0, 0]
+        rr = exact_sample(np.array([u_s + 1e-9]), L[0], L[1], L[2], R[0], R[1], R[2], g)[0, 0]
+        errs.append(max(abs(S_L - _rh_shock_speed(L[0], L[1], rl, u_s)),
+                        abs(S_R - _rh_shock_speed(R[0], R[1], rr, u_s))))
+    assert errs[0] / errs[1] > 3.0 and errs[1] / errs[2] > 3.0, errs
+
+
+def test_ausmp_pressure_splitting_is_c2_at_sonic_point():
+    """Liou's alpha = 3/16 is the value that makes the subsonic pressure polynomial
+    P+(M) = (M+1)^2 (2-M)/4 + alpha M (M^2-1)^2 meet the supersonic branch P+ = 1 at M = 1
+    with matching value, slope and curvature (P+''(1) = -3/2 + 8 alpha).  P+ is read off the
+    flux: with a supersonic right state (M_R = 2: P- = M- = 0) and c_L = c_R = 1,
+    F_mom - F_mass u_L = P+(M_L) p_L."""
+    g = 1.4
+
+    def P_plus(M):
+        WL = np.array([[1.0], [M], [0.0], [0.0], [1.0 / g]])
+        WR = np.array([[1.0], [2.0], [0.0], [0.0], [1.0 / g]])
+        F = o.ausmp(WL, WR, o.cons_from_prim(WL, g), o.cons_from_prim(WR, g), 0, g)
+        return (F[1, 0] - F[0, 0] * M) / WL[4, 0]
+
+    h = 1e-3
+    M0 = 1.0 - 2 * h
+    d2 = (P_plus(M0 + h) - 2 * P_plus(M0) + P_plus(M0 - h)) / h ** 2
+    d1 = (P_plus(1.0 - h) - P_plus(1.0 - 3 * h)) / (2 * h)
+    assert abs(P_plus(1.0) - 1.0) < 1e-14
+    assert abs(d1) < 1e-4
+    assert abs(d2) < 0.05, d2
+    # and at M = 0 the split pressure is 1/2 (P+(0) = 1/2 for every alpha)
+    assert abs(P_plus(0.0) - 0.5) < 1e-14
+
+
 # ----------------------------------------------------------------------------- RHS
 SETUPS = [
     o.Setup(stencil=st, variable=var, riemann=rs, signal_speed=sp, frozen_state=fz)
