@@ -431,6 +431,7 @@ def bench_adjoint_viscous(nsteps: int = 1000, n: int = 128, slots: int = 20, sto
     res["store_every_step"] = {"steps": store_steps, "forward_s": round(t_fwd_s, 4), "backward_s": round(t_b, 4),
                                "backward_over_forward": round(t_b / t_fwd_s, 3)}
     del x, lo
+    st.release_buffers()
     torch.cuda.empty_cache()
     with torch.no_grad():
         t_fwd, WT = timed(lambda: rollout(W0d, st, nsteps, 0))

@@ -49,18 +49,21 @@ def binomial_table(nsteps: int, snapshots: int) -> Tuple[np.ndarray, np.ndarray]
     return T, M
 
 
-def reverse_rollout(snaps: dict, nsteps: int, snapshots: int, lam, advance, step_vjp, store):
+def reverse_rollout(snaps: dict, nsteps: int, snapshots: int, lam, advance, step_vjp, store, release=None):
     """Reverse sweep of a rollout.  snaps[p] holds the state at step p (snaps[0] at least);
     advance(state, n) returns a working state n steps later, step_vjp(state, lam) the cotangent
-    one step earlier, store(state) a copy to keep.  At most `snapshots` states beyond snaps[0]
-    are alive at once, and the forward steps taken are T[nsteps, snapshots] of binomial_table
-    (fewer where snaps already holds first-level snapshots)."""
+    one step earlier, store(state) a copy to keep, release(state) (optional) takes back a stored
+    state no longer needed.  At most `snapshots` states beyond snaps[0] are alive at once, and
+    the forward steps taken are T[nsteps, snapshots] of binomial_table (fewer where snaps already
+    holds first-level snapshots)."""
     T, M = binomial_table(nsteps, snapshots)
     stack = [("seg", 0, nsteps, min(snapshots, nsteps))]
     while stack:
         item = stack.pop()
         if item[0] == "free":
-            snaps.pop(item[1], None)
+            state = snaps.pop(item[1], None)
+            if state is not None and release is not None:
+                release(state)
             continue
         _, st, en, s = item
         if en - st == 1:
@@ -88,6 +91,40 @@ def first_level_snapshots(nsteps: int, snapshots: int) -> List[int]:
         out.append(p)
         s -= 1
     return out
+
+
+class StatePool:
+    """Snapshot storage allocated once: `nslots` (W, U) buffer pairs in one device tensor, handed
+    out by store() and taken back by release(), so a rollout's snapshots cost copies, not
+    allocations."""
+
+    def __init__(self, nslots: int, shape, device, fill: float) -> None:
+        self.buf = torch.full((max(1, nslots), 2) + tuple(shape), fill, dtype=torch.float64, device=device)
+        self._free = list(range(self.buf.shape[0] - 1, -1, -1))
+        self._slot = {}
+
+    @property
+    def nslots(self) -> int:
+        return self.buf.shape[0]
+
+    def acquire(self) -> Tuple[torch.Tensor, torch.Tensor]:
+        if not self._free:
+            raise RuntimeError("snapshot pool exhausted")
+        i = self._free.pop()
+        W, U = self.buf[i, 0], self.buf[i, 1]
+        self._slot[W.data_ptr()] = i
+        return W, U
+
+    def store(self, state) -> Tuple[torch.Tensor, torch.Tensor]:
+        W, U = self.acquire()
+        W.copy_(state[0])
+        U.copy_(state[1])
+        return W, U
+
+    def release(self, state) -> None:
+        i = self._slot.pop(state[0].data_ptr(), None)
+        if i is not None:
+            self._free.append(i)
 
 
 # ------------------------------------------------------------------ torch pointwise VJPs
@@ -146,6 +183,11 @@ class FixedDtStepper:
             raise _native.JFAError("invalid grid for the adjoint")
         self._aws = torch.empty(nb, dtype=torch.uint8, device=self.device)
         self._lam = [e(), e(), e()]
+        self._lam_out = [e(), e()]
+        self._nvjp = 0
+        self._work = (e(), e())
+        self._pools: Dict[str, torch.Tensor] = {}
+        self.tape = 0  # id of the taped rollout whose steps the rollout buffers hold
         self.dt_value = float(dt)
         self.dt = torch.full((1,), float(dt), dtype=torch.float64, device=self.device)
         self.forward_steps = 0  # forward steps executed (rollouts and recomputation)
@@ -186,6 +228,31 @@ class FixedDtStepper:
         self.forward_steps += n
         return kept
 
+    def step_into(self, W: torch.Tensor, U: torch.Tensor, W2: torch.Tensor, W3: torch.Tensor, W_out: torch.Tensor,
+                  U_out: torch.Tensor) -> None:
+        """One RK3 step from (W, U) into (W_out, U_out), writing the primitives that enter stages 2
+        and 3 into W2 and W3 (taped forward: nothing is copied or allocated)."""
+        (_, Ub), (_, Uc) = self._B, self._C
+        self._stage(W, U, None, W2, Ub, 0)
+        self._stage(W2, Ub, U, W3, Uc, 1)
+        self._stage(W3, Uc, U, W_out, U_out, 2)
+        self.forward_steps += 1
+
+    def buffers(self, name: str, count: int) -> torch.Tensor:
+        """`count` state-shaped FP64 buffers (first index), kept by the stepper and reused by later
+        rollouts that need no more (grown when they do); edge halos hold the precision eps."""
+        t = self._pools.get(name)
+        if t is None or t.shape[0] < count:
+            self._pools.pop(name, None)
+            t = torch.full((max(1, count),) + self.shape, precision.get_eps(), dtype=torch.float64,
+                           device=self.device)
+            self._pools[name] = t
+        return t[:count]
+
+    def release_buffers(self) -> None:
+        """Frees the rollout buffers kept by `buffers`."""
+        self._pools.clear()
+
     def _vjp(self, W_in, lam_out, lam_in, lam_n, k) -> None:
         P = _native.ptr
         rc = _native.load().jfa_rk3_stage_vjp(P(W_in), P(lam_out), P(lam_in), P(lam_n), self.grid, self.scheme,
@@ -197,7 +264,8 @@ class FixedDtStepper:
                  stages: Optional[Tuple[torch.Tensor, torch.Tensor]] = None) -> torch.Tensor:
         """dL/dU_n from lam = dL/dU_{n+1} (interior of buffer-layout tensors); (W, U) is the state at
         the start of the step and is not modified.  The primitives entering stages 2 and 3 are
-        taken from `stages` (kept by step) or recomputed."""
+        taken from `stages` (kept by step) or recomputed.  The result is one of two buffers the
+        stepper reuses in turn: it stays valid until the second call after this one."""
         (Wb, Ub), (Wc, Uc) = self._B, self._C
         if stages is not None:
             Wb, Wc = stages
@@ -210,15 +278,21 @@ class FixedDtStepper:
         self._vjp(Wc, lam, l2, l0, 2)
         self._vjp(Wb, l2, l1, l0, 1)
         self._vjp(W, l1, l0, None, 0)
-        return l0.clone()
+        # two output buffers in turn: the returned cotangent is the next call's input
+        out = self._lam_out[self._nvjp % 2]
+        if out is lam:
+            out = self._lam_out[(self._nvjp + 1) % 2]
+        self._nvjp += 1
+        out.copy_(l0)
+        return out
 
     # -- rollouts
     def reverse(self, snaps: Dict[int, Tuple[torch.Tensor, torch.Tensor]], nsteps: int, snapshots: int,
-                lam: torch.Tensor) -> torch.Tensor:
+                lam: torch.Tensor, pool: Optional[StatePool] = None) -> torch.Tensor:
         """Reverse nsteps steps from the stored state snaps[0] (plus any first-level snapshots) with
-        at most `snapshots` extra stored states, following the binomial schedule."""
-        work = (torch.empty(self.shape, dtype=torch.float64, device=self.device),
-                torch.empty(self.shape, dtype=torch.float64, device=self.device))
+        at most `snapshots` extra stored states, following the binomial schedule.  States the sweep
+        stores come from `pool` (which must have a free slot for each), else from clones."""
+        work = self._work
 
         def advance(src, n):
             work[0].copy_(src[0])
@@ -226,11 +300,12 @@ class FixedDtStepper:
             self.step(work[0], work[1], n)
             return work
 
+        store = pool.store if pool is not None else (lambda state: (state[0].clone(), state[1].clone()))
         # a stored state may carry its step's stage primitives (W, U, W2, W3)
         return reverse_rollout(snaps, nsteps, snapshots, lam, advance,
                                lambda state, l: self.step_vjp(state[0], state[1], l,
                                                               state[2:] if len(state) == 4 else None),
-                               lambda state: (state[0].clone(), state[1].clone()))
+                               store, None if pool is None else pool.release)
 
 
 class RK3Rollout(torch.autograd.Function):
@@ -242,18 +317,33 @@ class RK3Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: int) -> torch.Tensor:
         W, U = stepper.fill(W0.detach())
-        snaps = {0: (W.clone(), U.clone())}
-        chain = set(first_level_snapshots(nsteps, snapshots))
         # with a snapshot at every step (snapshots >= nsteps - 1) the stage primitives are kept too,
-        # so the reverse sweep evaluates only the three stage VJPs per step
+        # so the reverse sweep evaluates only the three stage VJPs per step; the steps write straight
+        # into the stepper's rollout buffers
         keep = snapshots >= nsteps - 1
-        for n in range(1, nsteps + 1):
-            kept = stepper.step(W, U, keep_stages=keep)
-            if keep:
-                snaps[n - 1] = snaps[n - 1] + kept
-            if n in chain:
-                snaps[n] = (W.clone(), U.clone())
-        ctx.stepper, ctx.nsteps, ctx.snapshots, ctx.snaps = stepper, nsteps, snapshots, snaps
+        pool = None
+        if keep:
+            S = stepper.buffers("states", 2 * (nsteps + 1))   # (W_n, U_n) of every step
+            K = stepper.buffers("stages", 2 * max(nsteps, 1))  # primitives entering stages 2 and 3
+            S[0].copy_(W)
+            S[1].copy_(U)
+            snaps = {}
+            for n in range(nsteps):
+                Wi, Ui, Wo, Uo = S[2 * n], S[2 * n + 1], S[2 * n + 2], S[2 * n + 3]
+                stepper.step_into(Wi, Ui, K[2 * n], K[2 * n + 1], Wo, Uo)
+                snaps[n] = (Wi, Ui, K[2 * n], K[2 * n + 1])
+            W = S[2 * nsteps]
+            stepper.tape += 1
+        else:
+            pool = StatePool(min(snapshots, nsteps) + 1, stepper.shape, stepper.device, precision.get_eps())
+            snaps = {0: pool.store((W, U))}
+            chain = set(first_level_snapshots(nsteps, snapshots))
+            for n in range(1, nsteps + 1):
+                stepper.step(W, U)
+                if n in chain:
+                    snaps[n] = pool.store((W, U))
+        ctx.stepper, ctx.nsteps, ctx.snapshots, ctx.snaps, ctx.pool = stepper, nsteps, snapshots, snaps, pool
+        ctx.tape = stepper.tape if keep else None
         ctx.W0 = W0.detach()
         WT = W[stepper.interior].clone()
         ctx.WT = WT
@@ -262,10 +352,14 @@ class RK3Rollout(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gWT: torch.Tensor):
         st: FixedDtStepper = ctx.stepper
+        if ctx.tape is not None and ctx.tape != st.tape:
+            raise RuntimeError("a later rollout that stores every step reused this stepper's rollout buffers "
+                               "before this rollout's backward; use one stepper per pending rollout")
         lam = torch.zeros(st.shape, dtype=torch.float64, device=st.device)
         lam[st.interior] = prim_vjp(ctx.WT, gWT.to(torch.float64), st.gamma)
-        lam = st.reverse(ctx.snaps, ctx.nsteps, ctx.snapshots, lam)
+        lam = st.reverse(ctx.snaps, ctx.nsteps, ctx.snapshots, lam, ctx.pool)
         ctx.snaps = None
+        ctx.pool = None
         gW0 = cons_vjp(ctx.W0.to(torch.float64), lam[st.interior], st.gamma)
         return gW0.to(ctx.W0.dtype), None, None, None
 

@@ -341,6 +341,13 @@ __device__ __noinline__ double weno6_cum(double a, double b, double c, double d,
     return al[0] * one_alpha * p0 + al[1] * one_alpha * p1 + al[2] * one_alpha * p2 + al[3] * one_alpha * p3;
 }
 
+// x^6 as the reference's x ** q with the Python int q = 6 evaluates it (lax.integer_pow, binary
+// powering): x^2 (x^2 x^2).
+__device__ __forceinline__ double ipow6(double x) {
+    const double x2 = x * x;
+    return x2 * (x2 * x2);
+}
+
 // TENO6 (teno/teno6.py:27-67, teno6_base.py:20-64): three 3-point candidates and a 4-point one
 // (3/12, 13/12, -5/12, 1/12) with its own indicator, tau_6 = |beta_6 - (beta_0 + 4 beta_1 + beta_2)/6|,
 // gamma_k = (1 + tau_6/(beta_k + eps))^6, sharp cutoff 1e-7, d = (0.05, 0.45, 0.30, 0.20).
@@ -353,8 +360,8 @@ __device__ __noinline__ double teno6(double a, double b, double c, double d, dou
                        e * (7043 * e - 3882 * f) + 547 * f * f);
     const double b6 = beta_six_point(a, b, c, d, e, f);
     const double tau6 = fabs(b6 - 1.0 / 6 * (b0 + 4 * b1 + b2));
-    const double g0 = pow(1.0 + tau6 / (b0 + eps), 6.0), g1 = pow(1.0 + tau6 / (b1 + eps), 6.0);
-    const double g2 = pow(1.0 + tau6 / (b2 + eps), 6.0), g3 = pow(1.0 + tau6 / (b3 + eps), 6.0);
+    const double g0 = ipow6(1.0 + tau6 / (b0 + eps)), g1 = ipow6(1.0 + tau6 / (b1 + eps));
+    const double g2 = ipow6(1.0 + tau6 / (b2 + eps)), g3 = ipow6(1.0 + tau6 / (b3 + eps));
     const double one_gamma_sum = 1.0 / (g0 + g1 + g2 + g3);
     const double w0 = g0 * one_gamma_sum < 1e-7 ? 0.0 : 0.050, w1 = g1 * one_gamma_sum < 1e-7 ? 0.0 : 0.450;
     const double w2 = g2 * one_gamma_sum < 1e-7 ? 0.0 : 0.300, w3 = g3 * one_gamma_sum < 1e-7 ? 0.0 : 0.200;
@@ -379,8 +386,8 @@ __device__ __noinline__ double teno6_a(double a, double b, double c, double d, d
                             e * (7043 * e - 3882 * f) + 547 * f * f));
     const double b6 = fabs(beta_six_point(a, b, c, d, e, f));
     const double tau6 = fabs(b6 - 1.0 / 6 * (b0 + 4 * b1 + b2));
-    const double g0 = pow(1.0 + tau6 / (b0 + eps), 6.0), g1 = pow(1.0 + tau6 / (b1 + eps), 6.0);
-    const double g2 = pow(1.0 + tau6 / (b2 + eps), 6.0), g3 = pow(1.0 + tau6 / (b3 + eps), 6.0);
+    const double g0 = ipow6(1.0 + tau6 / (b0 + eps)), g1 = ipow6(1.0 + tau6 / (b1 + eps));
+    const double g2 = ipow6(1.0 + tau6 / (b2 + eps)), g3 = ipow6(1.0 + tau6 / (b3 + eps));
     const double one_gamma_sum = 1.0 / (g0 + g1 + g2 + g3);
     const double ed = 0.9 * 0.17 / (1 - 0.17) * (1e-3 * 1e-3);  // eps_dissipation = 0.9 Cr/(1-Cr) xi^2
     const double dd[5] = {b - a, c - b, d - c, e - d, f - e};

@@ -327,7 +327,7 @@ def teno6_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS, 
         + 547 * u_ippp * u_ippp)
     beta_6 = _beta_six_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp)
     tau_6 = np.abs(beta_6 - 1 / 6 * (beta_0 + 4 * beta_1 + beta_2))
-    gamma = [(1.0 + tau_6 / (b + eps)) ** 6 for b in (beta_0, beta_1, beta_2, beta_3)]
+    gamma = [_integer_pow6(1.0 + tau_6 / (b + eps)) for b in (beta_0, beta_1, beta_2, beta_3)]
     one_gamma_sum = 1.0 / (gamma[0] + gamma[1] + gamma[2] + gamma[3])
     dr = (0.050, 0.450, 0.300, 0.200)
     w = [dr[k] * np.where(gamma[k] * one_gamma_sum < 1e-7, 0.0, 1.0) for k in range(4)]
@@ -343,6 +343,13 @@ def _integer_pow4(x):
     """jnp.power(x, 4) with an integer exponent: lax.integer_pow, by squaring."""
     x2 = x * x
     return x2 * x2
+
+
+def _integer_pow6(x):
+    """x ** 6 with the Python int q = 6 (teno6.py:52-55, teno6_a.py:71-74): lax.integer_pow by
+    binary powering, x^2 * (x^2 * x^2)."""
+    x2 = x * x
+    return x2 * (x2 * x2)
 
 
 def _weno6_cum(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, alpha_of):
@@ -402,7 +409,7 @@ def teno6_a_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp, eps=SPATIAL_STENCIL_EPS
         + 547 * u_ippp * u_ippp))
     beta_6 = np.abs(_beta_six_point(u_imm, u_im, u_i, u_ip, u_ipp, u_ippp))
     tau_6 = np.abs(beta_6 - 1 / 6 * (beta_0 + 4 * beta_1 + beta_2))
-    gamma = [(1.0 + tau_6 / (b + eps)) ** 6 for b in (beta_0, beta_1, beta_2, beta_3)]
+    gamma = [_integer_pow6(1.0 + tau_6 / (b + eps)) for b in (beta_0, beta_1, beta_2, beta_3)]
     one_gamma_sum = 1.0 / (gamma[0] + gamma[1] + gamma[2] + gamma[3])
     xi, Cr = 1e-3, 0.17
     ed = 0.9 * Cr / (1 - Cr) * xi ** 2

@@ -111,3 +111,38 @@ def test_enstrophy_of_taylor_green_closed_form():
     W2[1] = np.sin(Y[:, :, :1])
     got2 = float(enstrophy(torch.as_tensor(W2), (h, h, 1.0)))
     assert abs(got2 - 0.5 * 0.5 * s * s * (2 * np.pi) ** 2) <= 1e-12 * got2
+
+
+@pytest.mark.parametrize("n,s", [(1, 0), (7, 2), (30, 4), (50, 9)])
+def test_reverse_sweep_with_a_state_pool(n, s):
+    """The reverse sweep on pooled snapshot slots (StatePool on the CPU): snapshots + 1 slots
+    suffice (snaps[0] and the first-level snapshots included), every stored state is released,
+    and the gradient is the chain rule's."""
+    import torch
+
+    from jaxfluids_amd.adjoint import StatePool
+
+    pool = StatePool(s + 1, (1,), "cpu", 0.0)
+    xs = [0.3]
+    for _ in range(n):
+        xs.append(f(xs[-1]))
+    exact = 1.0
+    for x in xs[:-1]:
+        exact *= df(x)
+    t = lambda x: (torch.tensor([x], dtype=torch.float64), torch.zeros(1, dtype=torch.float64))  # noqa: E731
+    snaps = {0: pool.store(t(xs[0]))}
+    for p in first_level_snapshots(n, s):
+        snaps[p] = pool.store(t(xs[p]))
+
+    def advance(state, k):
+        x = float(state[0][0])
+        for _ in range(k):
+            x = f(x)
+        return t(x)
+
+    grad = reverse_rollout(snaps, n, s, 1.0, advance, lambda st, lam: lam * df(float(st[0][0])), pool.store,
+                           pool.release)
+    assert abs(grad - exact) <= 1e-12 * abs(exact)
+    assert set(snaps) <= {0}
+    pool.release(snaps.pop(0))
+    assert len(pool._free) == pool.nslots
