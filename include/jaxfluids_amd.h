@@ -276,6 +276,20 @@ int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, voi
                         double* dt_next_dev, void* workspace, size_t workspace_bytes, void* stream);
 size_t jfa_stage_workspace_bytes(const jfa_grid* grid, const jfa_scheme* scheme);
 
+/* jfa_rk3_stage_fused without allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The stage's scratch (FP64 RHS and CFL partials) is a
+ * library-owned device array of JFA_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load
+ * until the process ends; grids whose jfa_stage_workspace_bytes exceed it return JFA_EINVAL
+ * (about 1.6 million interior cells).  Same arguments and results as jfa_rk3_stage_fused without
+ * the workspace.  Calls share that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race; calls in one stream order are safe. */
+#define JFA_NOALLOC_SCRATCH_BYTES (64u << 20)
+int jfa_rk3_stage_fused_noalloc(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out,
+                                const jfa_grid* grid, const jfa_scheme* scheme, const int32_t bc[6],
+                                int32_t stage, const double* dt_dev, double dt_host, double cfl, double eps,
+                                double* dt_next_dev, void* stream);
+
 /* ---- Reverse mode of the RK3 stage at fixed dt (FP64; PRIMITIVE reconstruction, WENO5-JS/-Z,
  * HLLC with ARITHMETIC/RUSANOV/DAVIS/EINFELDT speeds or Rusanov; with scheme->transport, the
  * viscous and heat fluxes too, on periodic domains).  The reference obtains this

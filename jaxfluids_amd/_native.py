@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libjaxfluids_amd.so")
 
 # constants (mirror include/jaxfluids_amd.h)
 JFA_OK, JFA_EINVAL, JFA_EUNSUPPORTED, JFA_ELAUNCH = 0, 1, 2, 3
+NOALLOC_SCRATCH_BYTES = 64 << 20  # JFA_NOALLOC_SCRATCH_BYTES (include/jaxfluids_amd.h)
 RECON_VAR = {"PRIMITIVE": 0, "CONSERVATIVE": 1, "CHAR-PRIMITIVE": 2, "CHAR-CONSERVATIVE": 3}
 STENCIL = {"WENO5-JS": 0, "WENO5-Z": 1, "WENO1": 2, "WENO3-JS": 3, "WENO3-Z": 4, "TENO5": 5, "TENO5-A": 6,
            "WENO6-CU": 7, "TENO6": 8, "WENO6-CUM1": 9, "WENO6-CUM2": 10,
@@ -105,6 +106,7 @@ SIGNATURES = {
     "jfa_prim_from_cons": (_I32, [_P, _P, _I64, _D, _I32, _P]),
     "jfa_rk3_stage_fused": (_I32, [_P, _P, _P, _P, _P, _PG, _PS, _PI32, _I32, _P, _D, _D, _D, _P, _P, _SZ, _P]),
     "jfa_stage_workspace_bytes": (_SZ, [_PG, _PS]),
+    "jfa_rk3_stage_fused_noalloc": (_I32, [_P, _P, _P, _P, _P, _PG, _PS, _PI32, _I32, _P, _D, _D, _D, _P, _P]),
     "jfa_stage_vjp_workspace_bytes": (_SZ, [_PG, _PS]),
     "jfa_adjoint_set_march_min_waves": (_I32, [_I64]),
     "jfa_rk3_stage_vjp": (_I32, [_P, _P, _P, _P, _PG, _PS, _PI32, _I32, _D, _P, _SZ, _P]),

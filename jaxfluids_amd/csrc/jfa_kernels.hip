@@ -8,6 +8,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "jfa_common.h"
 #include "jfa_dissipative.h"
@@ -643,6 +644,68 @@ int jfa_rk3_stage_fused(const void* W_in, const void* U_in, const void* U_n, voi
                         void* workspace, size_t workspace_bytes, void* stream) {
     return fused_stage_entry(W_in, U_in, U_n, W_out, U_out, grid, scheme, bc, stage, dt_dev, dt_host, cfl, eps,
                              dt_next_dev, workspace, workspace_bytes, stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ stage without allocation
+// Scratch of jfa_rk3_stage_fused_noalloc (the FP64 RHS and the CFL partials of one stage): a
+// file-scope device array, so the entry point needs no allocation.  Every GPU the library runs on
+// holds its own copy, from the library's load until the process ends.  Host code never passes
+// the array's name to a kernel (on the host it is a stand-in, not a device address): it uses the
+// per-device address hipGetSymbolAddress returns.
+static __device__ __align__(256) unsigned char scratch[JFA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int noalloc_scratch(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        snprintf(g_last_error, sizeof(g_last_error), "%s", hipGetErrorString(e));
+        return JFA_ELAUNCH;
+    }
+    if (dev < 0 || dev >= kMaxDevices) return JFA_EUNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            snprintf(g_last_error, sizeof(g_last_error), "hipGetSymbolAddress(scratch): %s",
+                     e != hipSuccess ? hipGetErrorString(e) : "null address");
+            return JFA_ELAUNCH;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return JFA_OK;
+}
+}  // namespace
+
+extern "C" {
+int jfa_rk3_stage_fused_noalloc(const void* W_in, const void* U_in, const void* U_n, void* W_out, void* U_out,
+                                const jfa_grid* grid, const jfa_scheme* scheme, const int32_t bc[6], int32_t stage,
+                                const double* dt_dev, double dt_host, double cfl, double eps, double* dt_next_dev,
+                                void* stream) {
+    const size_t need = jfa_stage_workspace_bytes(grid, scheme);
+    if (need == 0) return JFA_EINVAL;
+    if (need > (size_t)JFA_NOALLOC_SCRATCH_BYTES) {
+        snprintf(g_last_error, sizeof(g_last_error),
+                 "jfa_rk3_stage_fused_noalloc: the grid needs %zu bytes of scratch, the limit is %zu", need,
+                 (size_t)JFA_NOALLOC_SCRATCH_BYTES);
+        return JFA_EINVAL;
+    }
+    void* ws = nullptr;
+    const int rc = noalloc_scratch(&ws);
+    if (rc) return rc;
+    // The stage writes the scratch RHS before reading it and one CFL partial per block, so the
+    // previous call's contents need no clearing.
+    return fused_stage_entry(W_in, U_in, U_n, W_out, U_out, grid, scheme, bc, stage, dt_dev, dt_host, cfl, eps,
+                             dt_next_dev, ws, (size_t)JFA_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 }  // extern "C"

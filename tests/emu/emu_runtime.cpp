@@ -85,6 +85,10 @@ hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
     *v = 2;  // compute units of the emulated device
     return hipSuccess;
 }
+hipError_t hipGetSymbolAddress(void** dev_ptr, const void* symbol) {
+    *dev_ptr = const_cast<void*>(symbol);  // host memory is the emulated device memory
+    return hipSuccess;
+}
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, const void*, int, size_t) {
     *n = 1;
     return hipSuccess;

@@ -45,6 +45,9 @@ hipError_t hipStreamSynchronize(hipStream_t);
 hipError_t hipGetDevice(int*);
 hipError_t hipDeviceGetAttribute(int*, hipDeviceAttribute_t, int);
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int*, const void*, int, size_t);
+hipError_t hipGetSymbolAddress(void** dev_ptr, const void* symbol);
+#define HIP_SYMBOL(x) ((const void*)&(x))
+#define __align__(n) alignas(n)
 
 namespace emu {
 void launch(dim3 grid, dim3 block, const std::function<void()>& body);
