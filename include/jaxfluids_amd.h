@@ -87,8 +87,10 @@ extern "C" {
 #define JFA_FLUX_LIMITER_NONE 0
 #define JFA_FLUX_LIMITER_SIMPLE 1
 #define JFA_FLUX_LIMITER_NASA 2
+#define JFA_FLUX_LIMITER_HAS 3
 #define JFA_PARTITION_UNIFORM 0
 #define JFA_PARTITION_CELLSIZE 1
+#define JFA_PARTITION_WAVESPEED 2
 
 /* Uniform grid of one domain (domain/domain_information.py:68-300). */
 typedef struct jfa_grid {
@@ -150,14 +152,19 @@ typedef struct jfa_scheme {
     int32_t limit_velocity;
     double limiter_eps_density;
     double limiter_eps_pressure;
-    /* Positivity flux limiter of the convective face flux (positivity.flux_limiter SIMPLE / NASA,
-     * solvers/positivity/limiter_flux.py:147-316; applied by the stage and RHS entry points, which
-     * know the step's dt, as SpaceSolver.compute_rhs_xi does, space_solver.py:530-541): where the
-     * face's pseudo-updated cell states U_R + 2 lam F, U_L - 2 lam F (NASA: F minus each cell's
+    /* Positivity flux limiter of the convective face flux (positivity.flux_limiter SIMPLE / NASA /
+     * HAS, solvers/positivity/limiter_flux.py:147-316; applied by the stage and RHS entry points,
+     * which know the step's dt, as SpaceSolver.compute_rhs_xi does, space_solver.py:530-541): where
+     * the face's pseudo-updated cell states U_R + 2 lam F, U_L - 2 lam F (NASA: F minus each cell's
      * physical flux), lam = dt sigma / dx, have a density, then a pressure, below the epsilons, the
-     * flux becomes the first-order HLLC-Einfeldt flux of the two cells.  sigma: the number of
-     * active axes (UNIFORM) or sum(1/dx_a) / (1/dx) (CELLSIZE).  Epsilons 1e-12 / 1e-10 in double
-     * precision, 1e-6 / 1e-6 in single (config/precision.py:41,55). */
+     * flux becomes the first-order HLLC-Einfeldt flux F_pos of the two cells (SIMPLE, NASA) or the
+     * blend (1 - theta) F_pos + theta F whose linearised density, then pressure, of the offending
+     * state is the epsilon (HAS, :200-251).  sigma: the number of active axes (UNIFORM),
+     * sum(1/dx_a) / (1/dx) (CELLSIZE), or sum(m_a/dx_a) / (m/dx) with m_a the maximum of |u_a| + c
+     * over the stage's input primitives, halos included (WAVESPEED, :717-726; the halo corners hold
+     * the initialization's machine-epsilon state, so m_a >= eps + sqrt(gamma) on 2-D and 3-D
+     * grids).  Epsilons 1e-12 / 1e-10 in double precision, 1e-6 / 1e-6 in single
+     * (config/precision.py:41,55). */
     int32_t flux_limiter;    /* JFA_FLUX_LIMITER_* */
     int32_t flux_partition;  /* JFA_PARTITION_* */
     double flux_limiter_eps_density;
@@ -216,7 +223,8 @@ int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, cons
                         int32_t axis, void* flux, int64_t* limiter_count, void* stream);
 
 /* Right-hand side 0.0 + rhs_x + rhs_y + rhs_z, rhs_xi = (1/dx)(F[:-1]-F[1:]).
- * rhs: (5, Nx, Ny, Nz).  workspace: >= jfa_rhs_workspace_bytes(grid, dtype) bytes (FP64 partial RHS).
+ * rhs: (5, Nx, Ny, Nz).  workspace: >= jfa_rhs_workspace_bytes(grid, dtype) bytes (FP64 partial RHS and
+ * the WAVESPEED maxima).
  * Replaces SpaceSolver.compute_rhs (solvers/space_solver.py:151-453) and compute_rhs_xi (:456-674):
  * the convective flux (positivity-limited with the step's time step dt when
  * scheme->flux_limiter is set, as compute_rhs's physical_timestep_size; dt is unused otherwise),

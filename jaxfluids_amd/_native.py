@@ -175,8 +175,8 @@ def make_grid(cells: Sequence[int], nh: int, dx: Sequence[float]) -> JfaGrid:
 
 
 LAW_POWER, LAW_SUTHERLAND, LAW_PRANDTL = 0, 1, 2
-FLUX_LIMITER = {None: 0, "SIMPLE": 1, "NASA": 2}
-FLUX_PARTITION = {"UNIFORM": 0, "CELLSIZE": 1}
+FLUX_LIMITER = {None: 0, "SIMPLE": 1, "NASA": 2, "HAS": 3}
+FLUX_PARTITION = {"UNIFORM": 0, "CELLSIZE": 1, "WAVESPEED": 2}
 
 
 def make_transport(R: float = 1.0, mu_ref: float = 0.0, T_ref: float = 1.0, mu_exp: float = 0.0,
@@ -216,7 +216,7 @@ def make_scheme(recon_var: str, stencil: str, riemann: str, signal_speed: Option
                 flux_limiter_eps: Tuple[float, float] = (1e-12, 1e-10)) -> JfaScheme:
     """Native scheme; ``limiter_eps`` = (density, pressure) epsilons of the positivity limiter of
     the reconstructed face states (used when ``interpolation_limiter``); ``flux_limiter`` "SIMPLE"
-    / "NASA" with ``flux_partition`` "UNIFORM" / "CELLSIZE" and its epsilons, the positivity flux
+    / "NASA" / "HAS" with ``flux_partition`` "UNIFORM" / "CELLSIZE" / "WAVESPEED" and its epsilons, the positivity flux
     limiter of the stage and RHS entry points."""
     s = JfaScheme()
     if signal_speed == "DAVIS2":

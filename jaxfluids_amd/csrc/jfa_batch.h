@@ -52,6 +52,7 @@ struct BatchArgs {
     int stage;
     const double* dt;        // per case
     double* part;            // per item CFL partial (nullable)
+    const unsigned long long* wave_max;  // per case, 3 slots: WAVESPEED maxima of W_in (nullable)
 };
 
 // Wave-uniform copies in scalar registers.  The records are read with vector loads (the
@@ -183,7 +184,7 @@ __device__ __forceinline__ void face_flux(const T* __restrict__ W, const T* __re
             ul_[i] = ldo(U + (rb + i * fs - st), voff);
             ur_[i] = ldo(U + (rb + i * fs), voff);
         }
-        limit_flux<AXIS>(F, wl_, wr_, ul_, ur_, prm.dt_val * prm.flim_fac[AXIS], prm);
+        limit_flux<AXIS>(F, wl_, wr_, ul_, ur_, flux_lambda<AXIS>(prm, prm.dt_val), prm);
     }
 }
 
@@ -202,8 +203,12 @@ __device__ __forceinline__ void run_item(const BatchArgs<T>& a, const Params& pr
     if constexpr (FLIM) {
         prm.dt_val = uni(a.dt[it.c]);
         const double sum = cs.one_dx0 + cs.one_dx1;
-        prm.flim_fac[0] = cs.one_dx0 * (prm.flux_partition == JFA_PARTITION_CELLSIZE ? sum / cs.one_dx0 : 2.0);
-        prm.flim_fac[1] = cs.one_dx1 * (prm.flux_partition == JFA_PARTITION_CELLSIZE ? sum / cs.one_dx1 : 2.0);
+        prm.flim_sigma[0] = prm.flux_partition == JFA_PARTITION_CELLSIZE ? sum / cs.one_dx0 : 2.0;
+        prm.flim_sigma[1] = prm.flux_partition == JFA_PARTITION_CELLSIZE ? sum / cs.one_dx1 : 2.0;
+        prm.flim_odx[0] = cs.one_dx0;
+        prm.flim_odx[1] = cs.one_dx1;
+        prm.flim_odx[2] = 0.0;
+        if (a.wave_max) prm.wave_max = a.wave_max + 3 * it.c;
     }
     const int lane = threadIdx.x & 63;
     const int y = it.y0 + lane;               // this lane's cell / lower face along y
@@ -307,6 +312,29 @@ void batch2d_stage_kernel(BatchArgs<T> a, Params prm) {
     run_item<T, VAR, STENCIL, RIEMANN, FLIM>(aa, prm, it);
 }
 
+// WAVESPEED maxima of every case (max over its interior of |u| + c and |v| + c) into
+// slot[3 c], slot[3 c + 1]; grid.y runs over the cases.  slot must be zeroed first.
+template <class T>
+__global__ __launch_bounds__(kBlock) void batch2d_wave_max_kernel(const T* __restrict__ W,
+                                                                  const Case2D* __restrict__ cases, double gamma,
+                                                                  unsigned long long* slot) {
+    const Case2D cs = cases[blockIdx.y];
+    const T* __restrict__ Wc = W + cs.off;
+    const uint32_t total = (uint32_t)cs.n0 * (uint32_t)cs.n1;
+    double v0 = 0.0, v1 = 0.0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const uint32_t i = t / (uint32_t)cs.n1;
+        const uint32_t j = t - i * (uint32_t)cs.n1;
+        const int64_t b = (int64_t)(i + cs.nh) * cs.s0 + (int64_t)(j + cs.nh);
+        const double c = sound_speed(ld(Wc, 4 * cs.fs + b), ld(Wc, b), gamma);
+        v0 = nan_max(v0, fabs(ld(Wc, cs.fs + b)) + c);
+        v1 = nan_max(v1, fabs(ld(Wc, 2 * cs.fs + b)) + c);
+    }
+    block_max_to(slot + 3 * blockIdx.y, v0);
+    __syncthreads();  // block_max_to's shared partials are reused
+    block_max_to(slot + 3 * blockIdx.y + 1, v1);
+}
+
 // Per-case CFL time step from the per-item partials (time_step_size.py:109,154-155).
 template <int UNUSED = 0>  // template: shared by both storage-type translation units
 __global__ __launch_bounds__(256) void batch2d_dt_kernel(const Case2D* __restrict__ cases,
@@ -327,7 +355,7 @@ __global__ __launch_bounds__(256) void batch2d_dt_kernel(const Case2D* __restric
 
 // ------------------------------------------------------------------ host side
 struct BatchLayout {
-    size_t cases_off, items_off, parts_off, total;
+    size_t cases_off, items_off, parts_off, waves_off, total;
     int nitems;
 };
 
@@ -345,7 +373,8 @@ static inline BatchLayout batch_layout(const jfa_case2d* cs, int n, int chunk) {
     L.cases_off = 0;
     L.items_off = L.cases_off + align256(sizeof(Case2D) * (size_t)n);
     L.parts_off = L.items_off + align256(sizeof(Item2D) * (size_t)L.nitems);
-    L.total = L.parts_off + align256(sizeof(double) * (size_t)L.nitems);
+    L.waves_off = L.parts_off + align256(sizeof(double) * (size_t)L.nitems);
+    L.total = L.waves_off + align256(3 * sizeof(unsigned long long) * (size_t)n);
     return L;
 }
 
@@ -393,7 +422,16 @@ int batch2d_stage_impl(const void* W_in, const void* U_in, const void* U_n, void
     a.stage = stage;
     a.dt = dt_dev;
     a.part = dt_next ? reinterpret_cast<double*>(ws + L.parts_off) : nullptr;
-    const Params prm = make_params(s);
+    a.wave_max = nullptr;
+    Params prm = make_params(s);
+    if (s->flux_limiter && s->flux_partition == JFA_PARTITION_WAVESPEED) {
+        unsigned long long* slots = reinterpret_cast<unsigned long long*>(ws + L.waves_off);
+        hipLaunchKernelGGL(zero_slots_kernel<0>, dim3(1), dim3(256), 0, st, slots, 3 * ncases);
+        hipLaunchKernelGGL((batch2d_wave_max_kernel<T>), dim3(64, (unsigned)ncases), dim3(kBlock), 0, st, a.Win,
+                           a.cases, s->gamma, slots);
+        a.wave_max = slots;
+        prm.corner_speed = prm.eps_precision + sqrt(s->gamma);  // 2-D cases: the halo corners' eps state
+    }
     switch (s->riemann >= JFA_FS_ROE ? JFA_PRIMITIVE : s->recon_var) {
         case JFA_PRIMITIVE: launch_batch_s<T, JFA_PRIMITIVE>(s->stencil, s->riemann, a, prm, st); break;
         case JFA_CONSERVATIVE: launch_batch_s<T, JFA_CONSERVATIVE>(s->stencil, s->riemann, a, prm, st); break;

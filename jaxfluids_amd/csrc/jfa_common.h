@@ -159,6 +159,66 @@ __device__ __forceinline__ void block_max_to(unsigned long long* slot, double v)
 
 constexpr int kCflBlocks = 1024;
 
+// lam = dt (1/dx_axis) sigma of the positivity flux limiter (limiter_flux.py:204).  WAVESPEED
+// (:717-726): sigma = sum_active(m_a (1/dx_a)) / (m_axis (1/dx_axis)), m_a the maximum of
+// |u_a| + c over the stage's input buffer -- the interior maxima in p.wave_max and, on 2-D and 3-D
+// grids, the halo corners' constant p.corner_speed.
+template <int AXIS>
+__device__ __forceinline__ double flux_lambda(const Params& p, double dt) {
+    if (p.flux_partition != JFA_PARTITION_WAVESPEED) return (dt * p.flim_odx[AXIS]) * p.flim_sigma[AXIS];
+    double m[3], sum = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        m[a] = nan_max(__longlong_as_double((long long)p.wave_max[a]), p.corner_speed);
+        if (p.flim_odx[a] != 0.0) sum = sum + m[a] * p.flim_odx[a];
+    }
+    return (dt * p.flim_odx[AXIS]) * (sum / (m[AXIS] * p.flim_odx[AXIS]));
+}
+
+// Zeroes n 64-bit slots (a fill kernel, so streams under graph capture get no memset node).
+template <int UNUSED = 0>
+__global__ void zero_slots_kernel(unsigned long long* slot, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) slot[i] = 0ull;
+}
+
+// Maxima over interior cells of |u_a| + c for a = 0, 1, 2 (0 on inactive axes) into slot[0..2]
+// as bit patterns, for the WAVESPEED partition; the face halos hold copies or mirrors of interior
+// cells, so they add nothing.  slot must be zeroed first.
+template <class T>
+__global__ __launch_bounds__(kBlock) void axis_wave_max_kernel(const T* __restrict__ W, Geo G, double gamma,
+                                                               unsigned long long* slot) {
+    const uint32_t total = (uint32_t)G.n[0] * G.n[1] * G.n[2];
+    double v[3] = {0.0, 0.0, 0.0};
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const uint32_t q = t / (uint32_t)G.n[2];
+        const int c2i = (int)(t - q * (uint32_t)G.n[2]);
+        const int c0i = (int)(q / (uint32_t)G.n[1]);
+        const int c1i = (int)(q - (uint32_t)c0i * (uint32_t)G.n[1]);
+        const int64_t b =
+            (int64_t)(c0i + G.off[0]) * G.s[0] + (int64_t)(c1i + G.off[1]) * G.s[1] + (c2i + G.off[2]);
+        const double c = sound_speed(ld(W, 4 * G.fs + b), ld(W, b), gamma);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (G.active[a]) v[a] = nan_max(v[a], fabs(ld(W, (1 + a) * G.fs + b)) + c);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (a) __syncthreads();  // block_max_to's shared partials are reused
+        block_max_to(slot + a, v[a]);
+    }
+}
+
+// Launches the WAVESPEED maxima of W into slot[0..2] on stream st.
+template <class T>
+static inline void launch_axis_wave_max(const T* W, const Geo& G, double gamma, unsigned long long* slot,
+                                        hipStream_t st) {
+    hipLaunchKernelGGL(zero_slots_kernel<0>, dim3(1), dim3(64), 0, st, slot, 3);
+    const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    const int64_t nblk = (ncell + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((axis_wave_max_kernel<T>), dim3((unsigned)(nblk < kCflBlocks ? nblk : kCflBlocks)),
+                       dim3(kBlock), 0, st, W, G, gamma, slot);
+}
+
 // Divergence of the three face fluxes + RK3 blend/update + prim + face-halo writes of the
 // result (+ optional CFL max).  Interior cell c writes W_out/U_out at c and, if c lies in a
 // boundary layer, at the halo positions that the face-halo fill copies it to.
@@ -238,9 +298,9 @@ static inline int check_scheme(const jfa_scheme* s) {
         (!(s->limiter_eps_density > 0.0) || !(s->limiter_eps_density < INFINITY) ||
          !(s->limiter_eps_pressure > 0.0) || !(s->limiter_eps_pressure < INFINITY)))
         return JFA_EINVAL;
-    if (s->flux_limiter < JFA_FLUX_LIMITER_NONE || s->flux_limiter > JFA_FLUX_LIMITER_NASA) return JFA_EUNSUPPORTED;
+    if (s->flux_limiter < JFA_FLUX_LIMITER_NONE || s->flux_limiter > JFA_FLUX_LIMITER_HAS) return JFA_EUNSUPPORTED;
     if (s->flux_limiter &&
-        ((s->flux_partition != JFA_PARTITION_UNIFORM && s->flux_partition != JFA_PARTITION_CELLSIZE) ||
+        ((s->flux_partition < JFA_PARTITION_UNIFORM || s->flux_partition > JFA_PARTITION_WAVESPEED) ||
          !(s->flux_limiter_eps_density > 0.0) || !(s->flux_limiter_eps_density < INFINITY) ||
          !(s->flux_limiter_eps_pressure > 0.0) || !(s->flux_limiter_eps_pressure < INFINITY)))
         return JFA_EINVAL;
@@ -297,7 +357,10 @@ static inline Params make_params(const jfa_scheme* s) {
     p.flux_partition = s->flux_partition;
     p.flim_eps_rho = s->flux_limiter_eps_density;
     p.flim_eps_p = s->flux_limiter_eps_pressure;
-    p.flim_fac[0] = p.flim_fac[1] = p.flim_fac[2] = 0.0;
+    p.flim_sigma[0] = p.flim_sigma[1] = p.flim_sigma[2] = 0.0;
+    p.flim_odx[0] = p.flim_odx[1] = p.flim_odx[2] = 0.0;
+    p.wave_max = nullptr;
+    p.corner_speed = 0.0;
     p.dx2[0] = p.dx2[1] = p.dx2[2] = 0.0;
     p.dt_ptr = nullptr;
     p.dt_val = 0.0;
@@ -309,8 +372,10 @@ static inline void set_cell_sizes(Params& p, const Geo& G) {
     for (int a = 0; a < 3; ++a) p.dx2[a] = G.dx2[a];
 }
 
-// (1/dx_axis) sigma of the flux limiter's partition (limiter_flux.py:696-731) for a grid:
-// sigma = the number of active axes (UNIFORM) or sum_active(1/dx) / (1/dx_axis) (CELLSIZE).
+// 1/sigma of the flux limiter's partition (limiter_flux.py:696-731) for a grid: the number of
+// active axes (UNIFORM) or sum_active(1/dx) / (1/dx_axis) (CELLSIZE), summed from 0 in axis order.
+// WAVESPEED's sigma depends on the state: flux_lambda evaluates it from p.wave_max, and this sets
+// the cell sizes and the halo corners' wave speed it reads.
 static inline void set_flux_partition(Params& p, const jfa_scheme* s, const double one_dx[3], const int active[3]) {
     int dim = 0;
     double sum = 0.0;
@@ -319,13 +384,16 @@ static inline void set_flux_partition(Params& p, const jfa_scheme* s, const doub
             ++dim;
             sum += one_dx[a];
         }
+    // halo corners (2-D, 3-D) keep the initialization's fill, rho = u = p = the precision's eps
+    // (initialization/helper_functions.py:49-60): |u_a| + c = eps + sqrt(gamma eps / eps)
+    p.corner_speed = dim >= 2 ? p.eps_precision + sqrt(s->gamma) : 0.0;
     for (int a = 0; a < 3; ++a) {
+        p.flim_odx[a] = active[a] ? one_dx[a] : 0.0;
         if (!active[a]) {
-            p.flim_fac[a] = 0.0;
+            p.flim_sigma[a] = 0.0;
             continue;
         }
-        const double one_sigma = s->flux_partition == JFA_PARTITION_CELLSIZE ? sum / one_dx[a] : (double)dim;
-        p.flim_fac[a] = one_dx[a] * one_sigma;
+        p.flim_sigma[a] = s->flux_partition == JFA_PARTITION_CELLSIZE ? sum / one_dx[a] : (double)dim;
     }
 }
 
@@ -375,7 +443,7 @@ static inline Params gas_params(double gamma) {
     p.lim_eps_rho = p.lim_eps_p = 0.0;
     p.flux_limiter = p.flux_partition = 0;
     p.flim_eps_rho = p.flim_eps_p = 0.0;
-    p.flim_fac[0] = p.flim_fac[1] = p.flim_fac[2] = 0.0;
+    p.flim_sigma[0] = p.flim_sigma[1] = p.flim_sigma[2] = 0.0;
     p.dx2[0] = p.dx2[1] = p.dx2[2] = 0.0;
     p.dt_ptr = nullptr;
     p.dt_val = 0.0;

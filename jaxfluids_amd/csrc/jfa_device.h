@@ -70,7 +70,10 @@ struct Params {
     int flux_partition;     // JFA_PARTITION_*
     double flim_eps_rho;
     double flim_eps_p;
-    double flim_fac[3];     // (1/dx_axis) sigma of the flux partition, per axis
+    double flim_sigma[3];   // 1/sigma of the flux partition per axis (UNIFORM, CELLSIZE)
+    double flim_odx[3];     // 1/dx of the active axes, 0 on inactive ones (WAVESPEED partition)
+    const unsigned long long* wave_max;  // WAVESPEED: bits of max |u_a| + c over the interior, per axis
+    double corner_speed;    // WAVESPEED: |u_a| + c of the halo corners' eps state (0 on 1-D grids)
     double dx2[3];          // squared cell size per axis (the WENO6-CUM stencils read it)
     const double* dt_ptr;   // the step's time step in device memory, or null: dt_val
     double dt_val;
@@ -908,9 +911,68 @@ __device__ __forceinline__ void riemann(const double WL[5], const double WR[5], 
 // smaller density is below eps the flux becomes the first-order one (HLLC with Einfeldt speeds
 // on the cells' primitives, the reference's WENO1 Godunov flux, :57-98), then, with the updated
 // flux, likewise where the smaller pressure is.  Returns the number of switches.
+// Pressure of prim(U) in the reference's operation order (equation_manager.py:164-171,
+// ideal_gas.py:73-75): 1/rho by IEEE division, e = E/rho - 0.5 ((u^2 + v^2) + w^2), p = ((gamma-1) e) rho.
+__device__ __forceinline__ double prim_pressure(const double U[5], const Params& prm) {
+#pragma clang fp contract(off)
+    const double one_rho = 1.0 / U[0];
+    const double u = U[1] * one_rho, v = U[2] * one_rho, w = U[3] * one_rho;
+    const double e = U[4] * one_rho - 0.5 * ((u * u + v * v) + w * w);
+    return prm.gm1 * e * U[0];
+}
+
+template <int AXIS>
+__device__ __forceinline__ int limit_flux_has(double F[5], const double wl[5], const double wr[5], const double ul[5],
+                                              const double ur[5], double lam, const Params& prm) {
+    // Hu, Adams & Shu (limiter_flux.py:200-251): theta = (eps - q_LF) / (q - q_LF) on a side whose
+    // density (then pressure) q is below eps, q_LF the same side's value with the first-order
+    // flux; F <- (1 - theta) F_pos + theta F with theta the smaller of the two sides'.  IEEE
+    // divisions, as the reference's jnp.where evaluates them (a zero denominator gives +-inf/NaN).
+    // theta divides by the difference of two nearly equal pseudo-updated values wherever the
+    // first-order flux is close to the high-order one, so this follows the reference's operation
+    // order: no contraction, IEEE divisions (prim_pressure).
+#pragma clang fp contract(off)
+    Params pe = prm;
+    pe.signal_speed = JFA_SS_EINFELDT;
+    double cl[5], cr[5], Fp[5];
+    cons_from_prim(wl, pe, cl);
+    cons_from_prim(wr, pe, cr);
+    riemann<AXIS, JFA_HLLC>(wl, wr, cl, cr, pe, Fp);
+    const double l2 = 2.0 * lam;
+    double um[5], up[5], umf[5], upf[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        umf[i] = ur[i] + l2 * Fp[i];
+        upf[i] = ul[i] - l2 * Fp[i];
+    }
+    // density (:220-234)
+    const double rm = ur[0] + l2 * F[0], rp = ul[0] - l2 * F[0];
+    const double er = prm.flim_eps_rho, ep = prm.flim_eps_p;
+    const double tm = rm < er ? (er - umf[0]) / (rm - umf[0]) : 1.0;
+    const double tp = rp < er ? (er - upf[0]) / (rp - upf[0]) : 1.0;
+    const double t_rho = (tm < tp || tm != tm) ? tm : tp;  // jnp.minimum: a NaN operand gives NaN
+#pragma unroll
+    for (int i = 0; i < 5; ++i) F[i] = (1.0 - t_rho) * Fp[i] + t_rho * F[i];
+    // pressure of the updated flux's states and of the first-order states (:237-251)
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        um[i] = ur[i] + l2 * F[i];
+        up[i] = ul[i] - l2 * F[i];
+    }
+    const double pm = prim_pressure(um, prm), pp = prim_pressure(up, prm);
+    const double pmf = prim_pressure(umf, prm), ppf = prim_pressure(upf, prm);
+    const double sm = pm < ep ? (ep - pmf) / (pm - pmf) : 1.0;
+    const double sp = pp < ep ? (ep - ppf) / (pp - ppf) : 1.0;
+    const double t_p = (sm < sp || sm != sm) ? sm : sp;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) F[i] = (1.0 - t_p) * Fp[i] + t_p * F[i];
+    return (t_rho < 1.0 || t_p < 1.0) ? 1 : 0;
+}
+
 template <int AXIS>
 __device__ __forceinline__ int limit_flux(double F[5], const double wl[5], const double wr[5], const double ul[5],
                                           const double ur[5], double lam, const Params& prm) {
+    if (prm.flux_limiter == JFA_FLUX_LIMITER_HAS) return limit_flux_has<AXIS>(F, wl, wr, ul, ur, lam, prm);
     double fsl[5], fsr[5];
     if (prm.flux_limiter == JFA_FLUX_LIMITER_NASA) {
         physical_flux<AXIS>(wl, ul, fsl);

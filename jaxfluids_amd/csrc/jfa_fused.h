@@ -66,7 +66,7 @@ __device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* 
             ur_[i] = ld(U, i * fs + b_right);
         }
         const double dt = prm.dt_ptr ? *prm.dt_ptr : prm.dt_val;
-        limit_flux<AXIS>(F, wl_, wr_, ul_, ur_, dt * prm.flim_fac[AXIS], prm);
+        limit_flux<AXIS>(F, wl_, wr_, ul_, ur_, flux_lambda<AXIS>(prm, dt), prm);
     }
 }
 
@@ -488,15 +488,21 @@ JFA_FUSED_SWEEPS(extern, float, JFA_CHAR_CONSERVATIVE)
 #endif
 
 // Launches the sweeps; *nparts receives the CFL partials per quantity of the finishing launch.
+// wave_slots: 3 device slots for the WAVESPEED partition's maxima of W (used when the scheme's flux
+// limiter has that partition).
 template <class T>
 static int run_sweeps(const jfa_scheme* s, const void* W, const void* U, const Geo& G, double* R,
                       const InnerArgs<T>& ia, const double* dt_dev, double dt_host, hipStream_t st,
-                      int64_t* nparts) {
+                      int64_t* nparts, unsigned long long* wave_slots) {
     Params prm = make_params(s);
     set_flux_partition(prm, s, G.one_dx, G.active);
     set_cell_sizes(prm, G);
     prm.dt_ptr = dt_dev;  // the flux limiter's step dt
     prm.dt_val = dt_host;
+    if (s->flux_limiter && s->flux_partition == JFA_PARTITION_WAVESPEED) {
+        launch_axis_wave_max<T>(static_cast<const T*>(W), G, s->gamma, wave_slots, st);
+        prm.wave_max = wave_slots;
+    }
     int64_t nb = 0;
     switch (s->riemann >= JFA_FS_ROE ? JFA_PRIMITIVE : s->recon_var) {  // flux splitting: primitive window
         case JFA_PRIMITIVE: nb = sweeps_s<T, JFA_PRIMITIVE>(s->stencil, s->riemann, W, U, G, prm, R, ia, st); break;
@@ -527,7 +533,9 @@ int fused_stage_impl(const void* W_in, const void* U_in, const void* U_n, void* 
     ia.dt_host = dt_host;
     ia.cfl_part = dt_next ? part : nullptr;
     int64_t nparts = 0;
-    int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, dt_dev, dt_host, st, &nparts);
+    // the WAVESPEED slots follow the CFL partials (inside the workspace's 256-byte tail)
+    unsigned long long* slots = reinterpret_cast<unsigned long long*>(part + fused_partials(G));
+    int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, dt_dev, dt_host, st, &nparts, slots);
     if (rc) return rc;
     if (dt_next) {
         hipLaunchKernelGGL(dt_from_parts_kernel<0>, dim3(1), dim3(1024), 0, st, part, (int)nparts, G.dx_min,
@@ -543,7 +551,9 @@ int fused_rhs_impl(const void* W, const void* U, const Geo& G, const jfa_scheme*
     InnerArgs<T> ia = {};
     ia.mode = 0;
     ia.rhs_out = static_cast<T*>(rhs);
-    return run_sweeps<T>(s, W, U, G, R, ia, nullptr, dt, st, nullptr);
+    // the WAVESPEED slots follow the partial RHS (inside the workspace's 256-byte tail)
+    unsigned long long* slots = reinterpret_cast<unsigned long long*>(R + 5 * (int64_t)G.n[0] * G.n[1] * G.n[2]);
+    return run_sweeps<T>(s, W, U, G, R, ia, nullptr, dt, st, nullptr, slots);
 }
 
 }  // namespace jfa

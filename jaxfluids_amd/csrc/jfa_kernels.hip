@@ -436,7 +436,7 @@ int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, cons
 size_t jfa_rhs_workspace_bytes(const jfa_grid* grid, int32_t dtype) {
     Geo G;
     if (make_geo(grid, G) || (dtype != JFA_F64 && dtype != JFA_F32)) return 0;
-    return 5 * (size_t)G.n[0] * G.n[1] * G.n[2] * sizeof(double);
+    return 5 * (size_t)G.n[0] * G.n[1] * G.n[2] * sizeof(double) + 256;  // partial RHS + WAVESPEED slots
 }
 
 int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jfa_scheme* scheme, double dt,

@@ -192,13 +192,9 @@ def read_numerical_setup(d: Dict) -> NumericalSetup:
     flux_limiter = pos.get("flux_limiter") or None
     if flux_limiter not in (None, "SIMPLE", "NASA", "HAS"):
         raise AssertionError(f"positivity/flux_limiter must be one of ('SIMPLE', 'NASA', 'HAS'), got {flux_limiter!r}")
-    if flux_limiter == "HAS":
-        raise NotImplementedError("positivity/flux_limiter HAS is not built (SIMPLE and NASA are)")
     flux_partition = pos.get("flux_partition", "UNIFORM")
     if flux_partition not in ("UNIFORM", "CELLSIZE", "WAVESPEED"):
         raise AssertionError(f"positivity/flux_partition must be one of ('UNIFORM', 'CELLSIZE', 'WAVESPEED')")
-    if flux_limiter and flux_partition == "WAVESPEED":
-        raise NotImplementedError("positivity/flux_partition WAVESPEED is not built (UNIFORM and CELLSIZE are)")
     for key in ("is_thinc_interpolation_limiter", "is_volume_fraction_limiter", "is_acdi_flux_limiter"):
         if pos.get(key):
             raise NotImplementedError(f"positivity/{key} belongs to multiphase models, not on this path")

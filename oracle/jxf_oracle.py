@@ -944,8 +944,8 @@ class Setup:
     interpolation_limiter: bool = False       # positivity.is_interpolation_limiter
     limit_velocity: bool = False              # positivity.limit_velocity
     limiter_eps: Tuple[float, float] = (1e-12, 1e-10)  # (density, pressure), config/precision.py:54
-    flux_limiter: Optional[str] = None        # positivity.flux_limiter: "SIMPLE" / "NASA"
-    flux_partition: str = "UNIFORM"           # positivity.flux_partition: "UNIFORM" / "CELLSIZE"
+    flux_limiter: Optional[str] = None        # positivity.flux_limiter: "SIMPLE" / "NASA" / "HAS"
+    flux_partition: str = "UNIFORM"           # positivity.flux_partition: "UNIFORM" / "CELLSIZE" / "WAVESPEED"
     flux_limiter_eps: Tuple[float, float] = (1e-12, 1e-10)  # config/precision.py:55
 
     def __post_init__(self):
@@ -1172,29 +1172,69 @@ def dissipative_flux_xi(W, dom: Domain, axis: int, s: Setup):
     return D
 
 
-def positivity_flux_xi(F, W, U, dom: Domain, axis: int, s: Setup, dt: float, return_count: bool = False):
-    """PositivityLimiterFlux.compute_positivity_preserving_flux, SIMPLE / NASA, single phase
-    (solvers/positivity/limiter_flux.py:147-316,452-562,563-640,696-774).
-
-    lambda = dt (1/dx_axis) sigma with sigma = dim (UNIFORM) or sum_active(1/dx) / (1/dx_axis)
-    (CELLSIZE).  Face f lies between cells f-1 and f; the pseudo-updated states are
-    U_minus = U_f + 2 lambda (F - Fs_f) and U_plus = U_{f-1} - 2 lambda (F - Fs_{f-1}), Fs the
-    cells' own physical flux (NASA) or 0 (SIMPLE).  Where min(rho_minus, rho_plus) < eps_rho the
-    flux becomes the first-order one -- HLLC with Einfeldt speeds on the two cells' primitives
-    (a HighOrderGodunov with WENO1, :57-98) -- then, with the updated flux, likewise where
-    min(p_minus, p_plus) < eps_p.  count: faces switched, summed over the two checks."""
+def flux_partition_sigma(W, dom: Domain, axis: int, s: Setup):
+    """compute_partition (solvers/positivity/limiter_flux.py:696-727): 1/sigma of the axis, i.e.
+    dim (UNIFORM), sum_active(1/dx) / (1/dx_axis) (CELLSIZE), or, with m_a the maximum of
+    |u_a| + c over the whole primitive buffer, halos included (WAVESPEED, :717-726),
+    sum_active(m_a / dx_a) / (m_axis / dx_axis), the sum started at 0 as Python's sum."""
     one_dx = dom.one_dx[axis]
     if s.flux_partition == "UNIFORM":                                     # :711-712
-        one_sigma = float(len(dom.active))
-    elif s.flux_partition == "CELLSIZE":                                  # :714-715
-        one_sigma = sum(dom.one_dx[a] for a in dom.active) / one_dx
-    else:
-        raise NotImplementedError(s.flux_partition)
-    lam = dt * one_dx * one_sigma                                         # :204
+        return float(len(dom.active))
+    if s.flux_partition == "CELLSIZE":                                    # :714-715
+        return sum(dom.one_dx[a] for a in dom.active) / one_dx
+    if s.flux_partition == "WAVESPEED":                                   # :717-726
+        with np.errstate(all="ignore"):
+            c = speed_of_sound(W[4], W[0], s.gamma)
+            m = [np.max(np.abs(W[1 + a]) + c) for a in range(3)]
+        return sum(m[a] * dom.one_dx[a] for a in dom.active) / (m[axis] * one_dx)
+    raise NotImplementedError(s.flux_partition)
+
+
+def positivity_flux_xi(F, W, U, dom: Domain, axis: int, s: Setup, dt: float, return_count: bool = False):
+    """PositivityLimiterFlux.compute_positivity_preserving_flux, SIMPLE / NASA / HAS, single
+    phase (solvers/positivity/limiter_flux.py:147-316,452-562,563-640,696-774).
+
+    lambda = dt (1/dx_axis) sigma, sigma from flux_partition_sigma.  Face f lies between cells f-1
+    and f; the pseudo-updated states are U_minus = U_f + 2 lambda (F - Fs_f) and
+    U_plus = U_{f-1} - 2 lambda (F - Fs_{f-1}), Fs the cells' own physical flux (NASA) or 0
+    (SIMPLE, HAS).  The first-order flux F_pos is HLLC with Einfeldt speeds on the two cells'
+    primitives (a HighOrderGodunov with WENO1, :57-98).
+    SIMPLE / NASA: where min(rho_minus, rho_plus) < eps_rho the flux becomes F_pos, then, with the
+    updated flux, likewise where min(p_minus, p_plus) < eps_p; count: faces switched, summed over
+    the two checks.
+    HAS (Hu, Adams & Shu, :200-246): theta = (eps - q_LF) / (q - q_LF) on each side whose density q
+    is below eps_rho (1 elsewhere), q_LF the same side's density with F_pos; the flux becomes
+    (1 - theta) F_pos + theta F with theta = min over the sides; then likewise with the pressures
+    of the updated flux's states and of the F_pos states; count: sum of 1 - min(theta_rho, theta_p)."""
+    lam = dt * dom.one_dx[axis] * flux_partition_sigma(W, dom, axis, s)   # :204
     W_L, W_R = face_LR_cells(W, dom, axis)
     U_L, U_R = face_LR_cells(U, dom, axis)
     F_pos = hllc(W_L, W_R, cons_from_prim(W_L, s.gamma), cons_from_prim(W_R, s.gamma), axis, s.gamma,
                  "EINFELDT")
+    if s.flux_limiter == "HAS":
+        eps_rho, eps_p = s.flux_limiter_eps
+        with np.errstate(all="ignore"):
+            U_minus = U_R + 2 * lam * F                                   # :220-224
+            U_minus_LF = U_R + 2 * lam * F_pos
+            theta_minus = np.where(U_minus[0] < eps_rho, (eps_rho - U_minus_LF[0]) / (U_minus[0] - U_minus_LF[0]),
+                                   1.0)
+            U_plus = U_L - 2 * lam * F                                    # :226-230
+            U_plus_LF = U_L - 2 * lam * F_pos
+            theta_plus = np.where(U_plus[0] < eps_rho, (eps_rho - U_plus_LF[0]) / (U_plus[0] - U_plus_LF[0]), 1.0)
+            theta_rho = np.minimum(theta_minus, theta_plus)
+            F = (1 - theta_rho) * F_pos + theta_rho * F                   # :234
+            U_minus = U_R + 2 * lam * F                                   # :237-241
+            W_minus = prim_from_cons(U_minus, s.gamma)
+            W_minus_LF = prim_from_cons(U_minus_LF, s.gamma)
+            theta_minus = np.where(W_minus[4] < eps_p, (eps_p - W_minus_LF[4]) / (W_minus[4] - W_minus_LF[4]), 1.0)
+            U_plus = U_L - 2 * lam * F                                    # :243-246
+            W_plus = prim_from_cons(U_plus, s.gamma)
+            W_plus_LF = prim_from_cons(U_plus_LF, s.gamma)
+            theta_plus = np.where(W_plus[4] < eps_p, (eps_p - W_plus_LF[4]) / (W_plus[4] - W_plus_LF[4]), 1.0)
+            theta_p = np.minimum(theta_minus, theta_plus)
+            count = float(np.sum(1 - np.minimum(theta_rho, theta_p)))     # :249
+            F = (1 - theta_p) * F_pos + theta_p * F                       # :251
+        return (F, count) if return_count else F
     if s.flux_limiter == "NASA":                                          # :528-541
         Fs_L, Fs_R = physical_flux_xi(W_L, U_L, axis), physical_flux_xi(W_R, U_R, axis)
     elif s.flux_limiter == "SIMPLE":                                      # :520-527

@@ -146,15 +146,23 @@ def test_fused_step_viscous(emu, grid):
     check_step(emu, dom, s, W0, transport=tr)
 
 
-@pytest.mark.parametrize("fl", ["SIMPLE", "NASA"])
-def test_fused_step_with_limiters(emu, fl):
+# HAS's blend weight theta = (eps - q_LF) / (q - q_LF) is ill-conditioned where the first-order
+# flux's pseudo-updated state is itself inadmissible (on the pocket at CFL 0.9 |theta| reaches 1e14,
+# so the result hangs on the last bit of every operand): HAS is compared at CFL 0.6, where every
+# theta lies in [0, 1].
+HAS_CFL = {"HAS": 0.6}
+
+
+@pytest.mark.parametrize("fl,partition", [("SIMPLE", "CELLSIZE"), ("NASA", "CELLSIZE"), ("HAS", "CELLSIZE"),
+                                          ("HAS", "WAVESPEED"), ("NASA", "WAVESPEED")])
+def test_fused_step_with_limiters(emu, fl, partition):
     """Both positivity limiters on the near-vacuum pocket (2-D: MARCH x, INNER y)."""
     dom, W0 = _pocket_state()
     zg = {f: "ZEROGRADIENT" for f in ("east", "west", "north", "south")}
     s = o.Setup(stencil="WENO5-JS", variable="CHAR-PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT", bcs=zg,
-                interpolation_limiter=True, flux_limiter=fl, flux_partition="CELLSIZE")
+                interpolation_limiter=True, flux_limiter=fl, flux_partition=partition)
     W, U = o.create_buffers(W0, dom, s)
-    dt = 0.9 / 0.5 * o.compute_dt(W, dom, s)
+    dt = HAS_CFL.get(fl, 0.9) / 0.5 * o.compute_dt(W, dom, s)
     Wr, Ur = o.rk3_step(W, U, dt, dom, s)
     Wg, Ug, _ = fused_step(emu, dom, s, W, U, dt)
     I = dom.interior

@@ -115,8 +115,8 @@ def _fused_step(s, dom, W, U, dt):
     return host(Wd), host(Ud)
 
 
-@pytest.mark.parametrize("fl", ["SIMPLE", "NASA"])
-@pytest.mark.parametrize("partition", ["UNIFORM", "CELLSIZE"])
+@pytest.mark.parametrize("fl", ["SIMPLE", "NASA", "HAS"])
+@pytest.mark.parametrize("partition", ["UNIFORM", "CELLSIZE", "WAVESPEED"])
 def test_fused_rk3_step_with_flux_limiter(fl, partition):
     """The near-vacuum pocket at CFL 0.9 with both limiters: one fused RK3 step against the oracle
     (the step is not positive without the flux limiter, checked in the CPU suite)."""
@@ -125,7 +125,7 @@ def test_fused_rk3_step_with_flux_limiter(fl, partition):
     s = o.Setup(stencil="WENO5-JS", variable="CHAR-PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT", bcs=ZG2,
                 interpolation_limiter=True, flux_limiter=fl, flux_partition=partition)
     W, U = o.create_buffers(W0, dom, s)
-    dt = 0.9 / 0.5 * o.compute_dt(W, dom, s)
+    dt = (0.6 if s.flux_limiter == "HAS" else 0.9) / 0.5 * o.compute_dt(W, dom, s)
     Wr, Ur = o.rk3_step(W, U, dt, dom, s)
     Wg, Ug = _fused_step(s, dom, W, U, dt)
     I = dom.interior
@@ -133,17 +133,18 @@ def test_fused_rk3_step_with_flux_limiter(fl, partition):
     assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL, (rel(Wg[I], Wr[I]), rel(Ug[I], Ur[I]))
 
 
-def test_rhs_with_flux_limiter():
+@pytest.mark.parametrize("fl,partition", [("NASA", "UNIFORM"), ("HAS", "CELLSIZE"), ("SIMPLE", "WAVESPEED")])
+def test_rhs_with_flux_limiter(fl, partition):
     """jfa_compute_rhs with the flux limiter and the step's dt (SpaceSolver.compute_rhs's
     physical_timestep_size) against the oracle; dt <= 0 is rejected."""
     from jaxfluids_amd import _native
 
-    say("[flux limiter rhs]")
+    say(f"[flux limiter rhs] {fl} {partition}")
     dom, W0 = _pocket_state()
     s = o.Setup(stencil="WENO5-Z", variable="PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT", bcs=ZG2,
-                interpolation_limiter=True, flux_limiter="NASA")
+                interpolation_limiter=True, flux_limiter=fl, flux_partition=partition)
     W, U = o.create_buffers(W0, dom, s)
-    dt = 0.9 / 0.5 * o.compute_dt(W, dom, s)
+    dt = (0.6 if s.flux_limiter == "HAS" else 0.9) / 0.5 * o.compute_dt(W, dom, s)
     ref = o.compute_rhs(W, U, dom, s, dt)
     lib = _native.load()
     grid = native_grid(dom)
@@ -171,7 +172,7 @@ def test_flux_limiter_through_setup_dicts():
     im = _input_manager(dom, s, positivity={"is_interpolation_limiter": True, "flux_limiter": "NASA",
                                             "flux_partition": "CELLSIZE"})
     W, U = o.create_buffers(W0, dom, s)
-    dt = 0.9 / 0.5 * o.compute_dt(W, dom, s)
+    dt = (0.6 if s.flux_limiter == "HAS" else 0.9) / 0.5 * o.compute_dt(W, dom, s)
     Wr, _ = o.rk3_step(W, U, dt, dom, s)
     for fused in (True, False):
         sim = SimulationManager(im, fused=fused)

@@ -1211,3 +1211,186 @@ def test_teno6_a_quadratics_fourth_order_and_step():
     z, one = np.zeros(1), np.ones(1)
     assert o.WENO6_POINT["TENO6-A"](z, z, z, one, one, one)[0] == 0.0
     assert abs(o.WENO6_POINT["TENO6-A"](one, one, one, z, z, z)[0] - 1.0) < 1e-15
+
+
+# ------------------------------------------------------------------ HAS flux limiter, WAVESPEED partition
+def _rest_1d(limiter, partition="UNIFORM"):
+    """1-D zero-gradient column at rest (rho = 1, p = 1, gamma = 1.4): every face's first-order
+    HLLC-Einfeldt flux is (0, 1, 0, 0, 0)."""
+    dom = o.Domain((4, 1, 1), ((0, 1), (0, 1), (0, 1)), nh=3)
+    W0 = np.zeros((5,) + dom.cells)
+    W0[0], W0[4] = 1.0, 1.0
+    s = o.Setup(stencil="WENO5-JS", variable="PRIMITIVE", bcs={"east": "ZEROGRADIENT", "west": "ZEROGRADIENT"},
+                flux_limiter=limiter, flux_partition=partition)
+    W, U = o.create_buffers(W0, dom, s)
+    return dom, s, W, U
+
+
+def _fpos_rest(dom, U, axis=0):
+    F = np.zeros((5,) + o.face_LR_cells(U, dom, axis)[0].shape[1:])
+    F[1] = 1.0
+    return F
+
+
+@pytest.mark.parametrize("side", ["minus", "plus"])
+def test_has_density_blend_reaches_eps_exactly(side):
+    """HAS on one face whose mass flux makes one pseudo-updated density -0.5 (lam = 1e-6, so the
+    momentum 2 lam p keeps the pressure positive): the limited mass flux brings that side's
+    density to eps_rho (to rounding), the other faces and components keep their fluxes, and
+    the count is 1 - theta with theta the blend weight of the high-order flux."""
+    dom, s, W, U = _rest_1d("HAS")
+    lam = 1e-6
+    dt = lam / dom.one_dx[0]
+    F = _fpos_rest(dom, U)
+    j = 2
+    delta = (-1.5 if side == "minus" else 1.5) / (2 * lam)
+    F[0, j] += delta
+    G, count = o.positivity_flux_xi(F, W, U, dom, 0, s, dt, return_count=True)
+    others = np.arange(F.shape[1]) != j
+    assert np.array_equal(G[:, others], F[:, others])
+    UL, UR = o.face_LR_cells(U, dom, 0)
+    rho = UR[0, j] + 2 * lam * G[0, j] if side == "minus" else UL[0, j] - 2 * lam * G[0, j]
+    assert abs(rho - s.flux_limiter_eps[0]) < 1e-15, rho
+    theta = G[0, j] / delta
+    assert 0.0 < theta < 1.0
+    assert abs(count - (1.0 - theta)) < 1e-14
+    assert np.allclose(G[1:, j], F[1:, j], rtol=1e-15, atol=0)
+    # the admissible other side stays admissible
+    rho_o = UL[0, j] - 2 * lam * G[0, j] if side == "minus" else UR[0, j] + 2 * lam * G[0, j]
+    assert rho_o > s.flux_limiter_eps[0]
+
+
+def test_has_pressure_blend_reaches_eps_exactly():
+    """HAS with an energy-flux defect only: the pseudo-updated pressure is affine in the blend
+    weight (density and momentum do not change with it), so the limited flux puts the offending
+    side's pressure at eps_p to rounding; the density check leaves the flux alone."""
+    dom, s, W, U = _rest_1d("HAS")
+    lam = 1e-6
+    dt = lam / dom.one_dx[0]
+    F = _fpos_rest(dom, U)
+    j = 1
+    delta = -3.0 / (2 * lam)
+    F[4, j] += delta
+    G, count = o.positivity_flux_xi(F, W, U, dom, 0, s, dt, return_count=True)
+    UL, UR = o.face_LR_cells(U, dom, 0)
+    Um = UR[:, j] + 2 * lam * G[:, j]
+    p = o.prim_from_cons(Um[:, None, None, None], 1.4)[4].ravel()[0]
+    assert abs(p - s.flux_limiter_eps[1]) < 1e-15, p
+    theta = G[4, j] / delta
+    assert 0.0 < theta < 1.0 and abs(count - (1.0 - theta)) < 1e-14
+    assert np.array_equal(G[0], F[0])
+
+
+def test_has_takes_the_smaller_theta_of_the_two_sides():
+    """Both pseudo-updated densities negative, the minus side more so per unit of mass flux: the
+    blend weight is the smaller of the two sides', so exactly the more demanding side reaches
+    eps_rho and the other stays above it."""
+    dom, s, W, U = _rest_1d("HAS")
+    lam = 1e-6
+    dt = lam / dom.one_dx[0]
+    # densities 1 +- 2 lam F0 cannot both be negative for one F0: lower the right cell's density
+    W2 = W.copy()
+    W2[0, 6] = 0.2                                      # buffer cell right of face j = 3 (nh = 3)
+    U2 = o.cons_from_prim(W2, 1.4)
+    F = _fpos_rest(dom, U2)
+    j = 3
+    F[0, j] = -0.7 / (2 * lam)                          # rho_minus = 0.2 - 0.7 < 0; rho_plus = 1.7 > 0
+    G, _ = o.positivity_flux_xi(F, W2, U2, dom, 0, s, dt, return_count=True)
+    UL, UR = o.face_LR_cells(U2, dom, 0)
+    assert abs(UR[0, j] + 2 * lam * G[0, j] - s.flux_limiter_eps[0]) < 1e-15
+    F[0, j] = 1.3 / (2 * lam)                           # rho_plus = 1 - 1.3 < 0, rho_minus = 1.5
+    G, _ = o.positivity_flux_xi(F, W2, U2, dom, 0, s, dt, return_count=True)
+    assert abs(UL[0, j] - 2 * lam * G[0, j] - s.flux_limiter_eps[0]) < 1e-15
+    assert UR[0, j] + 2 * lam * G[0, j] > s.flux_limiter_eps[0]
+
+
+def test_has_keeps_admissible_fluxes_bit_for_bit():
+    """Uniform flow, a CFL-sized step: every pseudo-updated state is admissible, theta = 1, and
+    (1 - 1) F_pos + 1 F returns the high-order flux unchanged; count 0."""
+    dom = o.Domain((12, 10, 1), ((0, 1), (0, 1), (0, 1)), nh=3)
+    W0 = np.zeros((5,) + dom.cells)
+    W0[0], W0[1], W0[2], W0[4] = 1.0, 0.3, -0.2, 1.0
+    s = o.Setup(stencil="WENO5-JS", variable="PRIMITIVE", bcs=ZG2_, flux_limiter="HAS")
+    W, U = o.create_buffers(W0, dom, s)
+    dt = o.compute_dt(W, dom, s)
+    for axis in (0, 1):
+        F = o.compute_flux_xi(W, U, dom, axis, s)
+        G, n = o.positivity_flux_xi(F, W, U, dom, axis, s, dt, return_count=True)
+        assert n == 0.0 and np.array_equal(G, F)
+
+
+def _speed_state(u_peak, v_peak, p):
+    """2-D 6 x 4 cells on [0,1] x [0,1] (dx = 1/6, dy = 1/4) at rho = 1, pressure p, at rest
+    except u = u_peak in one cell and v = v_peak in another."""
+    dom = o.Domain((6, 4, 1), ((0, 1), (0, 1), (0, 1)), nh=3)
+    W0 = np.zeros((5,) + dom.cells)
+    W0[0], W0[4] = 1.0, p
+    W0[1, 2, 1, 0] = u_peak
+    W0[2, 4, 2, 0] = v_peak
+    s = o.Setup(stencil="WENO5-JS", variable="PRIMITIVE", bcs=ZG2_, flux_limiter="SIMPLE",
+                flux_partition="WAVESPEED")
+    W, U = o.create_buffers(W0, dom, s)
+    return dom, s, W
+
+
+def test_wavespeed_partition_closed_form():
+    """WAVESPEED: sigma_axis = (A/dx + B/dy) / (M_axis/dx_axis) with A = max(|u| + c), B =
+    max(|v| + c); here A = 2 + c and B = 0.5 + c at rho = 1, p = 1 (c = sqrt(1.4))."""
+    dom, s, W = _speed_state(2.0, -0.5, 1.0)
+    c = np.sqrt(1.4)
+    A, B = 2.0 + c, 0.5 + c
+    sx = o.flux_partition_sigma(W, dom, 0, s)
+    sy = o.flux_partition_sigma(W, dom, 1, s)
+    assert abs(sx - (A * 6 + B * 4) / (A * 6)) < 1e-15 * sx
+    assert abs(sy - (A * 6 + B * 4) / (B * 4)) < 1e-15 * sy
+    # equal maxima and cell sizes: the UNIFORM partition's dim
+    dom2 = o.Domain((5, 5, 1), ((0, 1), (0, 1), (0, 1)), nh=3)
+    W2 = np.ones((5, 5, 5, 1))
+    W2[1:4] = 0.0
+    Wb, _ = o.create_buffers(W2, dom2, s)
+    assert abs(o.flux_partition_sigma(Wb, dom2, 0, s) - 2.0) < 1e-15
+    # 1-D: sigma = 1 for any state
+    dom1 = o.Domain((7, 1, 1), ((0, 1), (0, 1), (0, 1)), nh=3)
+    W1 = np.ones((5, 7, 1, 1))
+    W1[1, 3] = 5.0
+    s1 = o.Setup(stencil="WENO5-JS", variable="PRIMITIVE", bcs={"east": "ZEROGRADIENT", "west": "ZEROGRADIENT"},
+                 flux_limiter="SIMPLE", flux_partition="WAVESPEED")
+    Wb1, _ = o.create_buffers(W1, dom1, s1)
+    assert o.flux_partition_sigma(Wb1, dom1, 0, s1) == 1.0
+
+
+def test_wavespeed_partition_reads_the_whole_buffer():
+    """The maxima run over the whole primitive buffer, halos included (limiter_flux.py:720): a
+    speed placed in a halo cell counts, and on a slow state (|u| + c < 1) the eps-filled halo
+    corners (rho = u = p = eps, so |u| + c = eps + sqrt(1.4)) set both maxima."""
+    dom, s, W = _speed_state(0.3, 0.0, 0.1 / 1.4)      # c = sqrt(0.1): maxima 0.3 + c, c < 1.18
+    corner = o.EPS64 + np.sqrt(1.4)
+    sx = o.flux_partition_sigma(W, dom, 0, s)
+    assert abs(sx - (corner * 6 + corner * 4) / (corner * 6)) < 1e-15
+    W2 = W.copy()
+    W2[1, 0, 5, 0] = 9.0                                # x-velocity in a west halo cell (not a corner)
+    c = np.sqrt(1.4 * W2[4, 0, 5, 0] / W2[0, 0, 5, 0])
+    sx2 = o.flux_partition_sigma(W2, dom, 0, s)
+    assert abs(sx2 - ((9.0 + c) * 6 + corner * 4) / ((9.0 + c) * 6)) < 1e-15
+
+
+@pytest.mark.parametrize("side", ["minus", "plus"])
+def test_has_density_blend_with_a_moving_first_order_flux(side):
+    """As test_has_density_blend_reaches_eps_exactly, across a pressure jump at rest (rho = 1,
+    p = 1 | 0.5), where the first-order mass flux is not zero: the offending side's density
+    still reaches eps_rho, which needs that side's first-order state U -/+ 2 lam F_pos."""
+    dom, s, W, U = _rest_1d("HAS")
+    W2 = W.copy()
+    W2[4, 6:] = 0.5                                     # cells right of face j = 3
+    W2, U2 = o.halo_update(W2, o.cons_from_prim(W2, 1.4), dom, s)
+    lam = 1e-6
+    dt = lam / dom.one_dx[0]
+    WL, WR = o.face_LR_cells(W2, dom, 0)
+    UL, UR = o.face_LR_cells(U2, dom, 0)
+    F = o.hllc(WL, WR, o.cons_from_prim(WL, 1.4), o.cons_from_prim(WR, 1.4), 0, 1.4, "EINFELDT")
+    j = 3
+    assert abs(F[0, j]) > 0.1
+    F[0, j] += (-1.5 if side == "minus" else 1.5) / (2 * lam)
+    G, _ = o.positivity_flux_xi(F, W2, U2, dom, 0, s, dt, return_count=True)
+    rho = UR[0, j] + 2 * lam * G[0, j] if side == "minus" else UL[0, j] - 2 * lam * G[0, j]
+    assert abs(rho - s.flux_limiter_eps[0]) < 1e-15, rho
