@@ -36,8 +36,10 @@ extern "C" {
 #define JFA_CHAR_PRIMITIVE 2
 #define JFA_CHAR_CONSERVATIVE 3
 /* reconstruction_stencil (stencils/reconstruction/shock_capturing/weno/__init__.py:26-41,
- * .../teno/__init__.py:15-25).  Every stencil reads at most the 6-cell window of a face, so all
- * run on buffers with nh >= 3. */
+ * .../teno/__init__.py:15-25).  Every stencil but WENO7-JS reads at most the 6-cell window of a
+ * face and runs on buffers with nh >= 3; WENO7-JS reads 8 cells and needs nh >= 4 (JFA_EINVAL
+ * otherwise).  WENO7-JS is built for the Godunov flux (any Riemann solver), not for flux
+ * splitting or the batched stage (JFA_EUNSUPPORTED). */
 #define JFA_WENO5_JS 0
 #define JFA_WENO5_Z 1
 #define JFA_WENO1 2
@@ -50,6 +52,7 @@ extern "C" {
 #define JFA_WENO6_CUM1 9
 #define JFA_WENO6_CUM2 10
 #define JFA_TENO6_A 11
+#define JFA_WENO7_JS 12
 /* riemann_solver (solvers/riemann_solvers/__init__.py:16-25) */
 #define JFA_HLLC 0
 #define JFA_RUSANOV 1

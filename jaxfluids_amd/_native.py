@@ -21,7 +21,7 @@ NOALLOC_SCRATCH_BYTES = 64 << 20  # JFA_NOALLOC_SCRATCH_BYTES (include/jaxfluids
 RECON_VAR = {"PRIMITIVE": 0, "CONSERVATIVE": 1, "CHAR-PRIMITIVE": 2, "CHAR-CONSERVATIVE": 3}
 STENCIL = {"WENO5-JS": 0, "WENO5-Z": 1, "WENO1": 2, "WENO3-JS": 3, "WENO3-Z": 4, "TENO5": 5, "TENO5-A": 6,
            "WENO6-CU": 7, "TENO6": 8, "WENO6-CUM1": 9, "WENO6-CUM2": 10,
-           "TENO6-A": 11}
+           "TENO6-A": 11, "WENO7-JS": 12}
 RIEMANN = {"HLLC": 0, "RUSANOV": 1, "HLL": 2, "HLLC-LM": 3, "AUSMP": 4}
 # convective_solver FLUX-SPLITTING with its eigenvalues (flux_splitting), in place of a Riemann solver
 FLUX_SPLITTING = {"ROE": 5, "CLLF": 6, "LLF": 7}

@@ -92,6 +92,7 @@ int jfa_batch2d_stage(const void* W_in, const void* U_in, const void* U_n, void*
     if (rc) return rc;
     rc = check_scheme(scheme);
     if (rc) return rc;
+    if (scheme->stencil == JFA_WENO7_JS) return JFA_EUNSUPPORTED;  // the batched stage reads 6-cell windows
     if (scheme->transport && (scheme->transport->viscous || scheme->transport->heat)) return JFA_EUNSUPPORTED;
     if (!W_in || !U_in || !W_out || !U_out || !dt_dev || !workspace || stage < 0 || stage > 2 || (stage > 0 && !U_n))
         return JFA_EINVAL;

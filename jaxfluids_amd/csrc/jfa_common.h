@@ -286,10 +286,16 @@ __device__ __forceinline__ void halo_scatter(T* __restrict__ W, T* __restrict__ 
     }
 }
 
+// Halo cells a stencil's window needs (WENO7-JS: 4; every other stencil reads cells -3..2).
+static inline int check_stencil_halos(const jfa_scheme* s, const Geo& G) {
+    return (s->stencil == JFA_WENO7_JS && G.nh < 4) ? JFA_EINVAL : JFA_OK;
+}
+
 static inline int check_scheme(const jfa_scheme* s) {
     if (!s) return JFA_EINVAL;
     if (s->recon_var < 0 || s->recon_var > 3) return JFA_EUNSUPPORTED;
-    if (s->stencil < JFA_WENO5_JS || s->stencil > JFA_TENO6_A) return JFA_EUNSUPPORTED;
+    if (s->stencil < JFA_WENO5_JS || s->stencil > JFA_WENO7_JS) return JFA_EUNSUPPORTED;
+    if (s->stencil == JFA_WENO7_JS && s->riemann >= JFA_FS_ROE) return JFA_EUNSUPPORTED;
     if (s->riemann < JFA_HLLC || s->riemann > JFA_FS_LLF) return JFA_EUNSUPPORTED;
     if (s->signal_speed < 0 || s->signal_speed > 4) return JFA_EUNSUPPORTED;
     if (s->frozen_state != JFA_FROZEN_ARITHMETIC && s->frozen_state != JFA_FROZEN_ROE) return JFA_EUNSUPPORTED;

@@ -202,6 +202,14 @@ __device__ __forceinline__ void weno5_pair(const double v[6], const double eps_z
 //   TENO5-A (teno/teno5_a.py:42-99): the same with the adaptive cutoff CT = 10^-(ceil(10 - 5 (1 - g)) - 1),
 //     g from the smoothness ratio eta of neighbouring differences, and WENO5's d = (1/10, 6/10, 3/10).
 constexpr int kStencilExt = 100;
+// WENO7-JS reads an 8-cell window (cells -4..3 of a face), so it has instantiations of its own.
+constexpr int kStencilW7 = 101;
+// Face window of a stencil instantiation: n cells, -h..h-1 relative to the cell right of the face.
+template <int STENCIL>
+struct Win {
+    static constexpr int n = STENCIL == kStencilW7 ? 8 : 6;
+    static constexpr int h = n / 2;
+};
 
 // One WENO3 side: stencil (u_im, u_i, u_ip); Z selects WENO3-Z.
 template <bool Z>
@@ -418,11 +426,41 @@ __device__ __noinline__ double teno6_a(double a, double b, double c, double d, d
     return w0 * one_dk * p0 + w1 * one_dk * p1 + w2 * one_dk * p2 + w3 * one_dk * p3;
 }
 
-// Left and right states of one face from its 6-cell window (cells -3..2).
+// WENO7-JS (weno/weno7_js.py:10-90; Balsara & Shu 2000), one side from cells (u_-3 .. u_3) around
+// the upwind cell u_0: four 4-point candidates, indicators with integer coefficients whose terms
+// cancel, alpha_k = d_k / (beta_k^2 + eps) with d = (1, 12, 18, 4)/35.  The reference's operation
+// order, no contraction, IEEE divisions, as the six-point stencils.
+__device__ __noinline__ double weno7_js(double a, double b, double c, double d, double e, double f, double g,
+                                        double eps) {
+#pragma clang fp contract(off)
+    const double b0 = a * (547 * a - 3882 * b + 4642 * c - 1854 * d) + b * (7043 * b - 17246 * c + 7042 * d) +
+                      c * (11003 * c - 9402 * d) + d * (2107 * d);
+    const double b1 = b * (267 * b - 1642 * c + 1602 * d - 494 * e) + c * (2843 * c - 5966 * d + 1922 * e) +
+                      d * (3443 * d - 2522 * e) + e * (547 * e);
+    const double b2 = c * (547 * c - 2522 * d + 1922 * e - 494 * f) + d * (3443 * d - 5966 * e + 1602 * f) +
+                      e * (2843 * e - 1642 * f) + f * (267 * f);
+    const double b3 = d * (2107 * d - 9402 * e + 7042 * f - 1854 * g) + e * (11003 * e - 17246 * f + 4642 * g) +
+                      f * (7043 * f - 3882 * g) + g * (547 * g);
+    const double o0 = 1.0 / (b0 * b0 + eps), o1 = 1.0 / (b1 * b1 + eps);
+    const double o2 = 1.0 / (b2 * b2 + eps), o3 = 1.0 / (b3 * b3 + eps);
+    const double a0 = (1.0 / 35) * o0, a1 = (12.0 / 35) * o1, a2 = (18.0 / 35) * o2, a3 = (4.0 / 35) * o3;
+    const double one_alpha = 1.0 / (a0 + a1 + a2 + a3);
+    const double w0 = a0 * one_alpha, w1 = a1 * one_alpha, w2 = a2 * one_alpha, w3 = a3 * one_alpha;
+    const double p0 = (-1.0 / 4) * a + (13.0 / 12) * b + (-23.0 / 12) * c + (25.0 / 12) * d;
+    const double p1 = (1.0 / 12) * b + (-5.0 / 12) * c + (13.0 / 12) * d + (1.0 / 4) * e;
+    const double p2 = (-1.0 / 12) * c + (7.0 / 12) * d + (7.0 / 12) * e + (-1.0 / 12) * f;
+    const double p3 = (1.0 / 4) * d + (13.0 / 12) * e + (-5.0 / 12) * f + (1.0 / 12) * g;
+    return w0 * p0 + w1 * p1 + w2 * p2 + w3 * p3;
+}
+
+// Left and right states of one face from its window (cells -3..2; WENO7-JS: -4..3).
 template <int STENCIL>
-__device__ __forceinline__ void recon_pair(const double v[6], const Params& prm, double dx2, double& L,
+__device__ __forceinline__ void recon_pair(const double v[Win<STENCIL>::n], const Params& prm, double dx2, double& L,
                                            double& R) {
-    if constexpr (STENCIL != kStencilExt) {
+    if constexpr (STENCIL == kStencilW7) {  // weno7_js.py:31: cells -4..2 and 3..-3
+        L = weno7_js(v[0], v[1], v[2], v[3], v[4], v[5], v[6], prm.eps);
+        R = weno7_js(v[7], v[6], v[5], v[4], v[3], v[2], v[1], prm.eps);
+    } else if constexpr (STENCIL != kStencilExt) {
         weno5_pair<STENCIL>(v, prm.eps_z, prm.eps_js, L, R);
     } else {
         switch (prm.stencil) {
@@ -1055,18 +1093,21 @@ __device__ __forceinline__ int limit_face_state(double Wf[5], const double q1[5]
 // CHAR-PRIMITIVE, conservatives otherwise); wl, wr are the primitives of the face's two cells,
 // read for CHAR-CONSERVATIVE.  The positivity limiter is applied by the callers (limit_side).
 template <int AXIS, int VAR, int STENCIL>
-__device__ __forceinline__ void face_states(const double (&win)[6][5], const double wl[5], const double wr[5],
-                                            const Params& prm, double WL[5], double WR[5], double UL[5],
-                                            double UR[5]) {
+__device__ __forceinline__ void face_states(const double (&win)[Win<STENCIL>::n][5], const double wl[5],
+                                            const double wr[5], const Params& prm, double WL[5], double WR[5],
+                                            double UL[5], double UR[5]) {
     constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
+    constexpr int NW = Win<STENCIL>::n, H = Win<STENCIL>::h;
     if constexpr (VAR == JFA_PRIMITIVE || VAR == JFA_CONSERVATIVE) {
         double* QL = VAR == JFA_PRIMITIVE ? WL : UL;
         double* QR = VAR == JFA_PRIMITIVE ? WR : UR;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             // WENO5Base slices j=0: cells -3..1, j=1: cells 2..-2 (weno5_base.py:30)
-            const double c6[6] = {win[0][i], win[1][i], win[2][i], win[3][i], win[4][i], win[5][i]};
-            recon_pair<STENCIL>(c6, prm, prm.dx2[AXIS], QL[i], QR[i]);
+            double cw[NW];
+#pragma unroll
+            for (int k = 0; k < NW; ++k) cw[k] = win[k][i];
+            recon_pair<STENCIL>(cw, prm, prm.dx2[AXIS], QL[i], QR[i]);
         }
         if constexpr (VAR == JFA_PRIMITIVE) {
             cons_from_prim(WL, prm, UL);  // :279-280
@@ -1078,7 +1119,7 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
     } else if constexpr (VAR == JFA_CHAR_PRIMITIVE) {
         // frozen state from the cells left/right of the face (eigendecomposition.py:139-140)
         Frozen f;
-        frozen_state<false>(win[2], win[3], prm, f);
+        frozen_state<false>(win[H - 1], win[H], prm, f);
         // get_characteristics_from_primitives (eigendecomposition.py:425-431):
         // k1 = 0.5/c, k2 = 0.5/(c^2 rho), k3 = 1/c^2
         const double k1 = 0.5 * f.inv_c;
@@ -1086,21 +1127,21 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         const double k2 = 0.5 * rcp1(f.rcc);
         // one characteristic field at a time (x0 = k2 p - k1 u_n, x1 = rho - k3 p,
         // x2, x3 = tangential velocities, x4 = k2 p + k1 u_n), each reconstructed L and R
-        double cLv[5], cRv[5], t[6];
+        double cLv[5], cRv[5], t[NW];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] - k1 * win[k][v];
+        for (int k = 0; k < NW; ++k) t[k] = k2 * win[k][4] - k1 * win[k][v];
         recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[0], cRv[0]);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) t[k] = win[k][0] - k3 * win[k][4];
+        for (int k = 0; k < NW; ++k) t[k] = win[k][0] - k3 * win[k][4];
         recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[1], cRv[1]);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) t[k] = win[k][m0];
+        for (int k = 0; k < NW; ++k) t[k] = win[k][m0];
         recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[2], cRv[2]);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) t[k] = win[k][m1];
+        for (int k = 0; k < NW; ++k) t[k] = win[k][m1];
         recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[3], cRv[3]);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) t[k] = k2 * win[k][4] + k1 * win[k][v];
+        for (int k = 0; k < NW; ++k) t[k] = k2 * win[k][4] + k1 * win[k][v];
         recon_pair<STENCIL>(t, prm, prm.dx2[AXIS], cLv[4], cRv[4]);
         // get_primitives_from_characteristics (eigendecomposition.py:517-521)
         const double sL = cLv[0] + cLv[4], sR = cRv[0] + cRv[4];
@@ -1122,9 +1163,9 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         double L[5][5], R[5][5];
         conservative_eigenvectors<AXIS>(f, L, R);
         // transformtochar: einsum over j (eigendecomposition.py:729-730)
-        double ch[6][5];
+        double ch[NW][5];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
+        for (int k = 0; k < NW; ++k) {
 #pragma unroll
             for (int a = 0; a < 5; ++a) {
                 double acc = L[a][0] * win[k][0];
@@ -1136,8 +1177,10 @@ __device__ __forceinline__ void face_states(const double (&win)[6][5], const dou
         double cLv[5], cRv[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            const double c6[6] = {ch[0][i], ch[1][i], ch[2][i], ch[3][i], ch[4][i], ch[5][i]};
-            recon_pair<STENCIL>(c6, prm, prm.dx2[AXIS], cLv[i], cRv[i]);
+            double cw[NW];
+#pragma unroll
+            for (int k = 0; k < NW; ++k) cw[k] = ch[k][i];
+            recon_pair<STENCIL>(cw, prm, prm.dx2[AXIS], cLv[i], cRv[i]);
         }
         // transformtophysical (eigendecomposition.py:742)
 #pragma unroll

@@ -26,10 +26,11 @@ namespace jfa {
 constexpr int kInnerBlock = 256;
 constexpr int kMarchBlockMax = 256;
 
-// Face flux from a 6-cell register window (cells k-3 relative to the cell right of the face).
+// Face flux from a register window (cell k-h relative to the cell right of the face, k < n; 6 cells,
+// 8 for WENO7-JS).
 // CHAR-CONSERVATIVE also needs the primitives of the two cells adjacent to the face.
 template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN, bool FLIM>
-__device__ __forceinline__ void window_flux(const double (&win)[6][5], const T* __restrict__ W,
+__device__ __forceinline__ void window_flux(const double (&win)[Win<STENCIL>::n][5], const T* __restrict__ W,
                                             const T* __restrict__ U, int64_t b_right, int64_t sa, int64_t fs,
                                             const Params& prm, double F[5]) {
     if constexpr (RIEMANN == kFluxSplit) {
@@ -91,12 +92,13 @@ JFA_WAVES_PER_EU((DISS || FLIM || VAR == JFA_CHAR_CONSERVATIVE || RIEMANN == kFl
     const int64_t bcol = (int64_t)(io + G.off[O]) * G.s[O] + (int64_t)(iin + G.off[IN]) * G.s[IN] +
                          (int64_t)G.off[A] * sa;
     constexpr bool cons_win = (VAR == JFA_CONSERVATIVE || VAR == JFA_CHAR_CONSERVATIVE);
+    constexpr int NW = Win<STENCIL>::n, H = Win<STENCIL>::h;
     const T* __restrict__ src = cons_win ? U : W;
-    double win[6][5];
+    double win[NW][5];
 #pragma unroll
-    for (int k = 0; k < 6; ++k)
+    for (int k = 0; k < NW; ++k)
 #pragma unroll
-        for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bcol + (int64_t)(a0 - 3 + k) * sa);
+        for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bcol + (int64_t)(a0 - H + k) * sa);
     double Fp[5];
     window_flux<T, A, VAR, STENCIL, RIEMANN, FLIM>(win, W, U, bcol + (int64_t)a0 * sa, sa, fs, prm, Fp);
     int cr[3];  // right cell of the current face (dissipative fluxes)
@@ -118,11 +120,11 @@ JFA_WAVES_PER_EU((DISS || FLIM || VAR == JFA_CHAR_CONSERVATIVE || RIEMANN == kFl
     // more than the load latency that three resident waves already cover.
     for (int a = a0; a < a1; ++a, r += rs) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k)
+        for (int k = 0; k < NW - 1; ++k)
 #pragma unroll
             for (int i = 0; i < 5; ++i) win[k][i] = win[k + 1][i];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) win[5][i] = ld(src, i * fs + bcol + (int64_t)(a + 3) * sa);
+        for (int i = 0; i < 5; ++i) win[NW - 1][i] = ld(src, i * fs + bcol + (int64_t)(a + H) * sa);
         double Fn[5];
         window_flux<T, A, VAR, STENCIL, RIEMANN, FLIM>(win, W, U, bcol + (int64_t)(a + 1) * sa, sa, fs, prm, Fn);
         if constexpr (DISS) {
@@ -259,11 +261,12 @@ __global__ __launch_bounds__(kInnerBlock) void inner_kernel(const T* __restrict_
         bf = (int64_t)(cP + G.off[P]) * G.s[P] + (int64_t)(cQ + G.off[Q]) * G.s[Q] + (int64_t)(f + G.off[IN]);
         constexpr bool cons_win = (VAR == JFA_CONSERVATIVE || VAR == JFA_CHAR_CONSERVATIVE);
         const T* __restrict__ src = cons_win ? U : W;
-        double win[6][5];
+        constexpr int NW = Win<STENCIL>::n, H = Win<STENCIL>::h;
+        double win[NW][5];
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
+        for (int k = 0; k < NW; ++k)
 #pragma unroll
-            for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bf + (k - 3));
+            for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + bf + (k - H));
         window_flux<T, IN, VAR, STENCIL, RIEMANN, FLIM>(win, W, U, bf, 1, fs, prm, F);
         if constexpr (DISS) {
             int cr[3];
@@ -457,8 +460,8 @@ static int64_t sweeps_r(int riemann, const void* W, const void* U, const Geo& G,
                         const InnerArgs<T>& ia, hipStream_t st) {
     if (riemann == JFA_HLLC) return sweeps_f<T, VAR, STENCIL, JFA_HLLC>(W, U, G, prm, R, ia, st);
     if (riemann >= JFA_FS_ROE) {  // flux splitting reads the primitive window: run_sweeps sends it there
-        if constexpr (VAR == JFA_PRIMITIVE) return sweeps_fs<T, STENCIL>(W, U, G, prm, R, ia, st);
-        return -1;
+        if constexpr (VAR == JFA_PRIMITIVE && STENCIL != kStencilW7) return sweeps_fs<T, STENCIL>(W, U, G, prm, R, ia, st);
+        return -1;  // not built (check_stencil refuses it)
     }
     if (riemann != JFA_RUSANOV) return sweeps_f<T, VAR, STENCIL, kRiemannExt>(W, U, G, prm, R, ia, st);
     return sweeps_f<T, VAR, STENCIL, JFA_RUSANOV>(W, U, G, prm, R, ia, st);
@@ -468,6 +471,7 @@ int64_t sweeps_s(int stencil, int riemann, const void* W, const void* U, const G
                  const InnerArgs<T>& ia, hipStream_t st) {
     if (stencil == JFA_WENO5_JS) return sweeps_r<T, VAR, JFA_WENO5_JS>(riemann, W, U, G, prm, R, ia, st);
     if (stencil == JFA_WENO5_Z) return sweeps_r<T, VAR, JFA_WENO5_Z>(riemann, W, U, G, prm, R, ia, st);
+    if (stencil == JFA_WENO7_JS) return sweeps_r<T, VAR, kStencilW7>(riemann, W, U, G, prm, R, ia, st);
     return sweeps_r<T, VAR, kStencilExt>(riemann, W, U, G, prm, R, ia, st);
 }
 

@@ -30,7 +30,12 @@ __global__ __launch_bounds__(kBlock) void reconstruct_kernel(const T* __restrict
     for (int v = 0; v < nvar; ++v) {
         const T* b = buf + v * G.fs + base;
         double r;
-        if (j == 0)  // cells -3..2 (weno6_base.py:27; the 5-point and narrower stencils use -3..1 and inner cells)
+        if constexpr (STENCIL == kStencilW7) {  // weno7_js.py:31: cells -4..2 and 3..-3
+            if (j == 0)
+                r = weno7_js(ld(b, -4 * sa), ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), ld(b, 2 * sa), prm.eps);
+            else
+                r = weno7_js(ld(b, 3 * sa), ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), ld(b, -3 * sa), prm.eps);
+        } else if (j == 0)  // cells -3..2 (weno6_base.py:27; the 5-point and narrower stencils use -3..1 and inner cells)
             r = recon_side<STENCIL>(ld(b, -3 * sa), ld(b, -2 * sa), ld(b, -sa), ld(b, 0), ld(b, sa), ld(b, 2 * sa), prm, prm.dx2[axis]);
         else  // cells 2..-3
             r = recon_side<STENCIL>(ld(b, 2 * sa), ld(b, sa), ld(b, 0), ld(b, -sa), ld(b, -2 * sa), ld(b, -3 * sa), prm, prm.dx2[axis]);
@@ -88,16 +93,17 @@ __global__ __launch_bounds__(kBlock) void signal_speed_kernel(int kind, const T*
 template <class T, int AXIS, int VAR, int STENCIL, int RIEMANN>
 __device__ __forceinline__ int face_flux_at(const T* __restrict__ W, const T* __restrict__ U, int64_t base,
                                             int64_t sa, int64_t fs, const Params& prm, double Fo[5]) {
-    double win[6][5];
+    constexpr int NW = Win<STENCIL>::n, H = Win<STENCIL>::h;
+    double win[NW][5];
     double wl[5], wr[5];
     constexpr bool cons_win = (VAR == JFA_CONSERVATIVE || VAR == JFA_CHAR_CONSERVATIVE);
     const T* src = cons_win ? U : W;
 #pragma unroll
-    for (int k = 0; k < 6; ++k)
+    for (int k = 0; k < NW; ++k)
 #pragma unroll
-        for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + base + (k - 3) * sa);
+        for (int i = 0; i < 5; ++i) win[k][i] = ld(src, i * fs + base + (k - H) * sa);
     if constexpr (RIEMANN == kFluxSplit) {
-        flux_split_face<AXIS, STENCIL>(win, prm, Fo);
+        if constexpr (STENCIL != kStencilW7) flux_split_face<AXIS, STENCIL>(win, prm, Fo);
         return 0;
     }
     if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
@@ -307,7 +313,7 @@ template <class T, int AXIS, int VAR, int STENCIL>
 static FluxLaunch pick_r(int riemann) {
     if (riemann == JFA_HLLC) return &launch_flux<T, AXIS, VAR, STENCIL, JFA_HLLC>;
     if (riemann >= JFA_FS_ROE) {  // flux splitting reads the primitive window: run_flux sends it there
-        if constexpr (VAR == JFA_PRIMITIVE) return &launch_flux<T, AXIS, VAR, STENCIL, kFluxSplit>;
+        if constexpr (VAR == JFA_PRIMITIVE && STENCIL != kStencilW7) return &launch_flux<T, AXIS, VAR, STENCIL, kFluxSplit>;
         return nullptr;
     }
     if (riemann != JFA_RUSANOV) return &launch_flux<T, AXIS, VAR, STENCIL, kRiemannExt>;
@@ -317,6 +323,7 @@ template <class T, int AXIS, int VAR>
 static FluxLaunch pick_s(int stencil, int riemann) {
     if (stencil == JFA_WENO5_JS) return pick_r<T, AXIS, VAR, JFA_WENO5_JS>(riemann);
     if (stencil == JFA_WENO5_Z) return pick_r<T, AXIS, VAR, JFA_WENO5_Z>(riemann);
+    if (stencil == JFA_WENO7_JS) return pick_r<T, AXIS, VAR, kStencilW7>(riemann);
     return pick_r<T, AXIS, VAR, kStencilExt>(riemann);
 }
 template <class T, int AXIS>
@@ -368,7 +375,8 @@ int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* g
     if (rc) return rc;
     if (!buffer || !out || nvar < 1 || axis < 0 || axis > 2 || !G.active[axis] || (j != 0 && j != 1))
         return JFA_EINVAL;
-    if (stencil < JFA_WENO5_JS || stencil > JFA_TENO6_A) return JFA_EUNSUPPORTED;
+    if (stencil < JFA_WENO5_JS || stencil > JFA_WENO7_JS) return JFA_EUNSUPPORTED;
+    if (stencil == JFA_WENO7_JS && G.nh < 4) return JFA_EINVAL;
     const FaceIdx F = face_extents(G, axis);
     dim3 g = flat_grid(F.nface);
     hipStream_t st = (hipStream_t)stream;
@@ -384,6 +392,8 @@ int jfa_weno5_reconstruct_xi(const void* buffer, int32_t nvar, const jfa_grid* g
             hipLaunchKernelGGL((reconstruct_kernel<T, JFA_WENO5_JS>), g, dim3(kBlock), 0, st, (const T*)buffer, nvar, G, axis, j, prm, F, (T*)out); \
         else if (stencil == JFA_WENO5_Z) \
             hipLaunchKernelGGL((reconstruct_kernel<T, JFA_WENO5_Z>), g, dim3(kBlock), 0, st, (const T*)buffer, nvar, G, axis, j, prm, F, (T*)out); \
+        else if (stencil == JFA_WENO7_JS) \
+            hipLaunchKernelGGL((reconstruct_kernel<T, kStencilW7>), g, dim3(kBlock), 0, st, (const T*)buffer, nvar, G, axis, j, prm, F, (T*)out); \
         else \
             hipLaunchKernelGGL((reconstruct_kernel<T, kStencilExt>), g, dim3(kBlock), 0, st, (const T*)buffer, nvar, G, axis, j, prm, F, (T*)out); \
     } while (0)
@@ -430,6 +440,8 @@ int jfa_compute_flux_xi(const void* W, const void* U, const jfa_grid* grid, cons
     rc = check_scheme(scheme);
     if (rc) return rc;
     if (!W || !U || !flux || axis < 0 || axis > 2 || !G.active[axis]) return JFA_EINVAL;
+    rc = check_stencil_halos(scheme, G);
+    if (rc) return rc;
     return run_flux(W, U, G, scheme, axis, flux, limiter_count, (hipStream_t)stream);
 }
 
@@ -448,6 +460,8 @@ int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jf
     if (rc) return rc;
     if (!W || !U || !rhs || !workspace) return JFA_EINVAL;
     if (workspace_bytes < jfa_rhs_workspace_bytes(grid, scheme->dtype)) return JFA_EINVAL;
+    rc = check_stencil_halos(scheme, G);
+    if (rc) return rc;
     rc = check_transport(scheme, G, nullptr);  // dissipative fluxes: the caller's boundaries are periodic
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
@@ -611,6 +625,8 @@ static int fused_stage_entry(const void* W_in, const void* U_in, const void* U_n
     int rc = make_geo(grid, G);
     if (rc) return rc;
     rc = check_scheme(scheme);
+    if (rc) return rc;
+    rc = check_stencil_halos(scheme, G);
     if (rc) return rc;
     if (!W_in || !U_in || !W_out || !U_out || !bc || !workspace || stage < 0 || stage > 2 || (stage > 0 && !U_n))
         return JFA_EINVAL;

@@ -158,6 +158,8 @@ def _read_flux_splitting(cf: Dict, halo_cells: int) -> ConvectiveFluxesSetup:
     if fsp not in _native.FLUX_SPLITTING:
         raise NotImplementedError(f"flux_splitting {fsp!r} has no eigenvalues in the reference (built: ROE, CLLF, LLF)")
     stencil_cls = _check_stencil(d.get("reconstruction_stencil", "WENO5-Z"), halo_cells)
+    if stencil_cls.__name__ == "WENO7JS":
+        raise NotImplementedError("flux splitting with WENO7-JS is not built (the Godunov flux is)")
     fz = d.get("frozen_state", "ARITHMETIC")
     assert fz in TUPLE_FROZEN_STATE, f"frozen_state must be one of {TUPLE_FROZEN_STATE}"
     return ConvectiveFluxesSetup(DICT_CONVECTIVE_SOLVER["FLUX-SPLITTING"], None,

@@ -5,8 +5,8 @@ shock-capturing classes WENO1, WENO3-JS/-Z, WENO5-JS/-Z, TENO5 and TENO5-A
 (stencils/reconstruction/shock_capturing/weno5_base.py, weno3_base.py, weno/*.py, teno/*.py).
 ``reconstruct_xi`` evaluates the stencil on the GPU (jfa_weno5_reconstruct_xi, which serves
 every stencil) and returns a new face tensor, as the reference returns a new array.  The HIP
-kernels read the 6-cell window of a face for every stencil, so buffers need 3 halo cells
-whatever the stencil's own requirement.
+kernels read the 6-cell window of a face for every stencil but WENO7-JS (8 cells), so buffers need
+3 halo cells (WENO7-JS: 4) whatever the stencil's own requirement.
 """
 from __future__ import annotations
 
@@ -210,6 +210,21 @@ class TENO6(ShockCapturingStencil):
         self.C, self.q, self.CT = 1.0, 6, 1e-7
 
 
+class WENO7JS(ShockCapturingStencil):
+    """Balsara & Shu (2000) WENO7-JS (weno/weno7_js.py:10-90): d = (1, 12, 18, 4)/35, four 4-point
+    candidates on the 8-cell window, 4 halo cells."""
+
+    required_halos = 4
+    _variant = _native.STENCIL["WENO7-JS"]
+
+    def __init__(self, nh: int, inactive_axes: Sequence[str], offset: int = 0, **kwargs) -> None:
+        super().__init__(nh, inactive_axes, offset)
+        self._dr = (1 / 35, 12 / 35, 18 / 35, 4 / 35)
+        self._cr = ((-1 / 4, 13 / 12, -23 / 12, 25 / 12), (1 / 12, -5 / 12, 13 / 12, 1 / 4),
+                    (-1 / 12, 7 / 12, 7 / 12, -1 / 12), (1 / 4, 13 / 12, -5 / 12, 1 / 12))
+        self._stencil_size = 8
+
+
 class WENO3Base(ShockCapturingStencil):
     """weno3_base.py:10-45: d = (1/3, 2/3), 4-cell window, 2 halos."""
 
@@ -261,4 +276,5 @@ DICT_SPATIAL_RECONSTRUCTION: Dict[str, type] = {
     "WENO6-CUM1": WENO6CUM1,
     "WENO6-CUM2": WENO6CUM2,
     "TENO6-A": TENO6A,
+    "WENO7-JS": WENO7JS,
 }

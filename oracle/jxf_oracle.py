@@ -267,6 +267,47 @@ WENO5_POINT = {"WENO5-JS": weno5_js_point, "WENO5-Z": weno5_z_point, "WENO1": we
 
 
 # --------------------------------------------------------------------------
+# WENO7-JS (weno/weno7_js.py:10-90; Balsara & Shu 2000): array_slices([range(-4,3,1),
+# range(3,-4,-1)]) -- seven of the eight window cells -4..3, mirrored for the right state; four
+# 4-point candidates; indicators with integer coefficients (no common factor), evaluated in the
+# reference's operation order.
+# --------------------------------------------------------------------------
+WENO7_SHIFTS = ((-4, -3, -2, -1, 0, 1, 2), (3, 2, 1, 0, -1, -2, -3))
+WENO7_WINDOW_IDX = ((0, 1, 2, 3, 4, 5, 6), (7, 6, 5, 4, 3, 2, 1))   # stencil_slices (:31)
+_DR7 = (1 / 35, 12 / 35, 18 / 35, 4 / 35)                          # :20
+_CR7 = ((-1 / 4, 13 / 12, -23 / 12, 25 / 12), (1 / 12, -5 / 12, 13 / 12, 1 / 4),
+        (-1 / 12, 7 / 12, 7 / 12, -1 / 12), (1 / 4, 13 / 12, -5 / 12, 1 / 12))   # :22-27
+
+
+def weno7_js_point(b0_, b1_, b2_, b3_, b4_, b5_, b6_, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno7_js.py:33-90 on the seven operands buffer[s1_[0..6]]."""
+    u = (b0_, b1_, b2_, b3_, b4_, b5_, b6_)
+    beta_0 = u[0] * (547 * u[0] - 3882 * u[1] + 4642 * u[2] - 1854 * u[3]) \
+        + u[1] * (7043 * u[1] - 17246 * u[2] + 7042 * u[3]) \
+        + u[2] * (11003 * u[2] - 9402 * u[3]) \
+        + u[3] * (2107 * u[3])
+    beta_1 = u[1] * (267 * u[1] - 1642 * u[2] + 1602 * u[3] - 494 * u[4]) \
+        + u[2] * (2843 * u[2] - 5966 * u[3] + 1922 * u[4]) \
+        + u[3] * (3443 * u[3] - 2522 * u[4]) \
+        + u[4] * (547 * u[4])
+    beta_2 = u[2] * (547 * u[2] - 2522 * u[3] + 1922 * u[4] - 494 * u[5]) \
+        + u[3] * (3443 * u[3] - 5966 * u[4] + 1602 * u[5]) \
+        + u[4] * (2843 * u[4] - 1642 * u[5]) \
+        + u[5] * (267 * u[5])
+    beta_3 = u[3] * (2107 * u[3] - 9402 * u[4] + 7042 * u[5] - 1854 * u[6]) \
+        + u[4] * (11003 * u[4] - 17246 * u[5] + 4642 * u[6]) \
+        + u[5] * (7043 * u[5] - 3882 * u[6]) \
+        + u[6] * (547 * u[6])
+    one = [1.0 / (b * b + eps) for b in (beta_0, beta_1, beta_2, beta_3)]
+    alpha = [_DR7[k] * one[k] for k in range(4)]
+    one_alpha = 1.0 / (alpha[0] + alpha[1] + alpha[2] + alpha[3])
+    omega = [a * one_alpha for a in alpha]
+    p = [_CR7[k][0] * u[k] + _CR7[k][1] * u[k + 1] + _CR7[k][2] * u[k + 2] + _CR7[k][3] * u[k + 3]
+         for k in range(4)]
+    return omega[0] * p[0] + omega[1] * p[1] + omega[2] * p[2] + omega[3] * p[3]
+
+
+# --------------------------------------------------------------------------
 # Six-point stencils (weno6_base.py, teno6_base.py): array_slices([range(-3,3,1),
 # range(2,-4,-1)]) -- the whole 6-cell window, mirrored for the right state.  The smoothness
 # indicator of the 6-point stencil is a quadratic form with 8-digit integer coefficients whose
@@ -443,6 +484,8 @@ def weno5_reconstruct_xi(buffer, dom: Domain, axis: int, j: int, stencil: str = 
     weno/weno5_js.py:23-37 / weno5_z.py:24-36: the five operands are the buffer
     shifted by WENO5_SHIFTS[j] along axis (spatial_stencil.py:45-112).
     """
+    if stencil == "WENO7-JS":
+        return weno7_js_point(*[buffer[shift_slice(dom, axis, k)] for k in WENO7_SHIFTS[j]], eps=eps)
     if stencil in WENO6_POINT:
         ops = [buffer[shift_slice(dom, axis, k)] for k in WENO6_SHIFTS[j]]
         return WENO6_POINT[stencil](*ops, eps=eps, dx=dom.dx[axis])  # dx=cell_sizes[axis]
@@ -456,6 +499,8 @@ def weno5_reconstruct_window(window, j: int, stencil: str, eps=SPATIAL_STENCIL_E
     ``window`` has the 6-cell stencil window on axis 1: shape (nvar, 6, *face); dx is the cell
     size along the axis (read by WENO6-CUM1/-CUM2).
     """
+    if stencil == "WENO7-JS":  # the 8-cell window
+        return weno7_js_point(*[window[:, i] for i in WENO7_WINDOW_IDX[j]], eps=eps)
     if stencil in WENO6_POINT:
         ops = [window[:, i] for i in WENO6_WINDOW_IDX[j]]
         return WENO6_POINT[stencil](*ops, eps=eps, dx=dx)
@@ -649,12 +694,20 @@ def transform(M, x):
     return np.stack(out, axis=0)
 
 
-def stencil_window(buffer, dom: Domain, axis: int):
-    """get_stencil_window (eigendecomposition.py:75-118): cells f-3..f+2 of face f.
+def stencil_window(buffer, dom: Domain, axis: int, size: int = 6):
+    """get_stencil_window (eigendecomposition.py:75-118): cells f-size/2..f+size/2-1 of face f
+    (half_stencil_size = stencil_size // 2; 6 cells, 8 for WENO7-JS).
 
-    Returned with the window on axis 1: (nvar, 6, *face_shape).
+    Returned with the window on axis 1: (nvar, size, *face_shape).
     """
-    return np.stack([buffer[shift_slice(dom, axis, k)] for k in range(-3, 3)], axis=1)
+    h = size // 2
+    return np.stack([buffer[shift_slice(dom, axis, k)] for k in range(-h, h)], axis=1)
+
+
+def window_size(stencil: str) -> int:
+    """The reconstruction stencil's _stencil_size as the eigendecomposition window uses it: 8 for
+    WENO7-JS (weno7_js.py:29); the narrower stencils read the middle cells of the 6-cell window."""
+    return 8 if stencil == "WENO7-JS" else 6
 
 
 def face_LR_cells(buffer, dom: Domain, axis: int):
@@ -971,7 +1024,7 @@ def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
         U_R = weno5_reconstruct_xi(U, dom, axis, 1, s.stencil, s.eps_stencil)
         W_L, W_R = prim_from_cons(U_L, g), prim_from_cons(U_R, g)
     elif s.variable == "CHAR-PRIMITIVE":                                      # :298-316,401-402
-        win = stencil_window(W, dom, axis)
+        win = stencil_window(W, dom, axis, window_size(s.stencil))
         cl, cr = face_LR_cells(W, dom, axis)
         P, _, _, c, cc, _ = frozen_state(cl, cr, g, s.frozen_state)
         # expand_dims(axis=axis-3) of the frozen state == broadcast over the window axis
@@ -982,7 +1035,7 @@ def reconstruct_xi(W, U, dom: Domain, axis: int, s: Setup):
         W_R = prim_from_char(cR, P, c, cc, axis)
         U_L, U_R = cons_from_prim(W_L, g), cons_from_prim(W_R, g)
     elif s.variable == "CHAR-CONSERVATIVE":                                   # :404-417
-        win = stencil_window(U, dom, axis)
+        win = stencil_window(U, dom, axis, window_size(s.stencil))
         cl, cr = face_LR_cells(W, dom, axis)
         fs = frozen_state(cl, cr, g, s.frozen_state)
         R, L = eigen_conservatives(*fs, axis=axis)

@@ -184,6 +184,40 @@ def test_fused_step_further_stencils(emu, stencil, variable, grid):
     check_step(emu, dom, s, W0)
 
 
+@pytest.mark.parametrize("variable,grid,bc", [("PRIMITIVE", "3d", "ZEROGRADIENT"), ("CHAR-PRIMITIVE", "2d-xy", "PERIODIC"),
+                                              ("CHAR-CONSERVATIVE", "2d-yz", "SYMMETRY"),
+                                              ("CONSERVATIVE", "1d", "ZEROGRADIENT"),
+                                              ("CHAR-PRIMITIVE", "3d-wide", "PERIODIC")])
+@pytest.mark.parametrize("riemann", ["HLLC", "RUSANOV", "HLL"])
+def test_fused_step_weno7(emu, variable, grid, bc, riemann):
+    """WENO7-JS (its own 8-cell-window instantiation of the march and inner kernels), nh = 4."""
+    dom, W0 = grid_state(grid, seed=17, nh=4)
+    s = o.Setup(gamma=1.4, bcs={f: bc for f in FACES}, cfl=0.5, stencil="WENO7-JS", variable=variable,
+                riemann=riemann, signal_speed="EINFELDT", frozen_state="ROE")
+    check_step(emu, dom, s, W0)
+
+
+def test_weno7_refusals(emu):
+    """WENO7-JS needs 4 halo cells (JFA_EINVAL with 3) and is not built for flux splitting
+    (JFA_EUNSUPPORTED)."""
+    from jaxfluids_amd import _native
+
+    for nh, riemann, want in ((3, "HLLC", _native.JFA_EINVAL), (4, "FLUX-SPLITTING-ROE", _native.JFA_EUNSUPPORTED)):
+        dom, W0 = grid_state("2d-xy", seed=1, nh=nh)
+        s = o.Setup(gamma=1.4, bcs={f: "PERIODIC" for f in FACES}, stencil="WENO7-JS", variable="PRIMITIVE",
+                    riemann=riemann, signal_speed="EINFELDT")
+        grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
+        scheme = _native.make_scheme(s.variable, s.stencil, s.riemann, s.signal_speed, s.frozen_state, 0, s.gamma,
+                                     s.eps_stencil)
+        W = np.zeros((5,) + dom.shape)
+        ws = np.zeros(emu.jfa_stage_workspace_bytes(grid, scheme), dtype=np.uint8)
+        bcs = _native.bc_array({f: "PERIODIC" for f in FACES})
+        P = W.ctypes.data
+        rc = emu.jfa_rk3_stage_fused(P, P, None, ws.ctypes.data, ws.ctypes.data + 8, grid, scheme, bcs, 0, None, 1e-3,
+                                     0.5, o.EPS64, None, ws.ctypes.data, ws.nbytes, None)
+        assert rc == want, (nh, riemann, rc)
+
+
 @pytest.mark.parametrize("riemann", ["HLL", "HLLC-LM", "AUSMP"])
 @pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "TORO"])
 @pytest.mark.parametrize("variable,grid", [("CHAR-PRIMITIVE", "3d"), ("PRIMITIVE", "2d-xz")])

@@ -1,12 +1,13 @@
 """GPU parity of the further reconstruction stencils -- WENO1, WENO3-JS, WENO3-Z, TENO5, TENO5-A,
-WENO6-CU, TENO6, WENO6-CUM1, WENO6-CUM2 (stencils/reconstruction/shock_capturing/weno/weno1_js.py,
-weno3_js.py, weno3_z.py, weno6_cu.py, weno6_cum1.py, weno6_cum2.py, teno/teno5.py, teno5_a.py, teno6.py) -- against the oracle: the plugin's reconstruct_xi, compute_flux_xi for
+WENO6-CU, TENO6, WENO6-CUM1, WENO6-CUM2, TENO6-A and WENO7-JS
+(stencils/reconstruction/shock_capturing/weno/weno1_js.py, weno3_js.py, weno3_z.py, weno6_cu.py,
+weno6_cum1.py, weno6_cum2.py, weno7_js.py, teno/teno5.py, teno5_a.py, teno6.py, teno6_a.py) -- against the oracle: the plugin's reconstruct_xi, compute_flux_xi for
 every reconstruction variable, RK3 steps of the fused stage through the setup dicts (3-D and 2-D),
 and the batched ensemble stage.  FP64, relative L-inf <= 1e-12.
 
 The kernels run these stencils through one run-time switch (the kStencilExt instantiation) on the
 same 6-cell face window as WENO5, so every case is also a check that the switch picks the right
-stencil and sub-window.
+stencil and sub-window; WENO7-JS has its own 8-cell-window instantiation (nh = 4).
 """
 import numpy as np
 import pytest
@@ -19,7 +20,9 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 FACES = ("east", "west", "north", "south", "top", "bottom")
-STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6", "WENO6-CUM1", "WENO6-CUM2", "TENO6-A"]
+STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A", "WENO6-CU", "TENO6", "WENO6-CUM1", "WENO6-CUM2", "TENO6-A",
+            "WENO7-JS"]
+NH = {"WENO7-JS": 4}  # WENO7-JS reads the 8-cell window (weno7_js.py:29-31)
 
 
 @pytest.mark.parametrize("stencil", STENCILS)
@@ -27,10 +30,11 @@ def test_reconstruct_xi_plugin(stencil):
     from jaxfluids_amd.stencils import DICT_SPATIAL_RECONSTRUCTION
 
     say(f"[reconstruct_xi] {stencil}")
-    dom, W0 = random_state((10, 9, 8), 3, seed=31)
+    nh = NH.get(stencil, 3)
+    dom, W0 = random_state((10, 9, 8), nh, seed=31)
     s = o.Setup(stencil="WENO5-JS")
     W, _ = o.create_buffers(W0, dom, s)
-    plugin = DICT_SPATIAL_RECONSTRUCTION[stencil](nh=3, inactive_axes=())
+    plugin = DICT_SPATIAL_RECONSTRUCTION[stencil](nh=nh, inactive_axes=())
     Wd = dev(W)
     for axis in range(3):
         for j in (0, 1):
@@ -47,7 +51,7 @@ def test_compute_flux_xi(stencil, variable):
     say(f"[flux_xi] {stencil} {variable}")
     s = o.Setup(gamma=1.4, stencil=stencil, variable=variable, riemann="HLLC", signal_speed="EINFELDT",
                 frozen_state="ROE", bcs={f: "PERIODIC" for f in FACES})
-    dom, W0 = random_state((10, 9, 8), 3, seed=23)
+    dom, W0 = random_state((10, 9, 8), NH.get(stencil, 3), seed=23)
     W, U = o.create_buffers(W0, dom, s)
     lib = _native.load()
     Wd, Ud = dev(W), dev(U)
@@ -79,7 +83,7 @@ def test_rk3_step_through_setup_dicts(stencil, cells, bc):
     full = tuple(max(c, 2) for c in cells)
     _, W0 = random_state(full, 3, seed=19, ranges=ranges)
     W0 = np.ascontiguousarray(W0[:, :cells[0], :cells[1], :cells[2]])
-    dom = o.Domain(cells, ranges, nh=3)
+    dom = o.Domain(cells, ranges, nh=NH.get(stencil, 3))
     s = o.Setup(gamma=1.4, stencil=stencil, variable="CHAR-PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT",
                 frozen_state="ARITHMETIC",
                 bcs={f: (bc if o.FACE_AXIS[f] in dom.active else "PERIODIC") for f in FACES})
@@ -118,3 +122,19 @@ def test_ensemble_step_matches_oracle(scheme):
         for r in face_regions(dom):
             assert rel(Wg[r], W[r]) < TOL and rel(Ug[r], U[r]) < TOL, (i, rel(Wg[r], W[r]))
         assert abs(got_dt[i] - o.compute_dt(W, dom, s)) <= 1e-14 * dt, i
+
+
+def test_weno7_is_refused_by_the_batched_stage():
+    """The batched stage reads 6-cell windows: WENO7-JS returns JFA_EUNSUPPORTED there, before any
+    launch."""
+    from jaxfluids_amd import _native
+    from jaxfluids_amd.ensemble import Ensemble2D
+
+    say("[weno7 batch refusal]")
+    zg = {f: "ZEROGRADIENT" for f in ("east", "west", "north", "south")}
+    ens = Ensemble2D([{"cells": (16, 12), "ranges": ((0.0, 1.0), (0.0, 1.0)), "bcs": zg}], nh=4,
+                     stencil="WENO7-JS", variable="PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT")
+    _, W0 = random_state((16, 12, 2), 4, seed=5)
+    ens.initialize([np.ascontiguousarray(W0[:, :, :, :1])])
+    with pytest.raises(_native.JFAError, match="unsupported"):
+        ens.step(1)

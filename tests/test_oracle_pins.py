@@ -1394,3 +1394,83 @@ def test_has_density_blend_with_a_moving_first_order_flux(side):
     G, _ = o.positivity_flux_xi(F, W2, U2, dom, 0, s, dt, return_count=True)
     rho = UR[0, j] + 2 * lam * G[0, j] if side == "minus" else UL[0, j] - 2 * lam * G[0, j]
     assert abs(rho - s.flux_limiter_eps[0]) < 1e-15, rho
+
+
+# ------------------------------------------------------------------ WENO7-JS
+def _weno7_sides(F, N, k=2.0):
+    """Left (cells i-3..i+3 around the face's left cell) and right (i+4..i-2) WENO7-JS states of the
+    cell averages of an antiderivative F on [0, 1] with N cells, at the faces with a full
+    one-sided window; returns (left, right, face positions)."""
+    x = np.linspace(0.0, 1.0, N + 1)
+    a = (F(x[1:]) - F(x[:-1])) * N
+    i = np.arange(3, N - 4)
+    L = o.weno7_js_point(*[a[i + s] for s in (-3, -2, -1, 0, 1, 2, 3)])
+    R = o.weno7_js_point(*[a[i + 1 + s] for s in (3, 2, 1, 0, -1, -2, -3)])
+    return L, R, x[i + 1]
+
+
+def test_weno7_reproduces_cubics_and_constants():
+    """Cell averages of a cubic: every 4-point candidate is exact, so both face states are exact
+    whatever the weights; constant data stays constant."""
+    P = lambda x: 1.0 + 2.0 * x - 3.0 * x ** 2 + 0.5 * x ** 3          # noqa: E731
+    Pi = lambda x: x + x ** 2 - x ** 3 + 0.125 * x ** 4                 # noqa: E731
+    L, R, xf = _weno7_sides(Pi, 20)
+    assert np.max(np.abs(L - P(xf))) < 1e-13 and np.max(np.abs(R - P(xf))) < 1e-13
+    c = np.full(9, 0.7)
+    assert abs(o.weno7_js_point(*c[:7]) - 0.7) < 1e-15
+
+
+def test_weno7_is_seventh_order_on_smooth_data():
+    """exp(2x) (no critical point, where Jiang-Shu weights keep the optimal order): the face error
+    falls by about 2^7 per halving of the cell size (measured order > 6.5 at N = 24 -> 48)."""
+    errs = []
+    for N in (24, 48):
+        L, R, xf = _weno7_sides(lambda x: np.exp(2.0 * x) / 2.0, N)
+        errs.append(max(np.max(np.abs(L - np.exp(2.0 * xf))), np.max(np.abs(R - np.exp(2.0 * xf)))))
+    assert np.log2(errs[0] / errs[1]) > 6.5, errs
+
+
+def test_weno7_mirror_and_buffer_paths_agree():
+    """The right state of a face is the left state of the mirrored data, and the buffer and
+    window forms of reconstruct_xi give the same faces (cells -4..3, nh = 4)."""
+    rng = np.random.default_rng(3)
+    dom = o.Domain((12, 1, 1), ((0, 1), (0, 1), (0, 1)), nh=4)
+    buf = np.ones((1,) + dom.shape)
+    buf[0, :, 0, 0] = rng.standard_normal(dom.shape[0])
+    Lb = o.weno5_reconstruct_xi(buf, dom, 0, 0, "WENO7-JS")
+    Rb = o.weno5_reconstruct_xi(buf, dom, 0, 1, "WENO7-JS")
+    win = o.stencil_window(buf, dom, 0, 8)
+    assert np.array_equal(Lb, o.weno5_reconstruct_window(win, 0, "WENO7-JS"))
+    assert np.array_equal(Rb, o.weno5_reconstruct_window(win, 1, "WENO7-JS"))
+    rev = buf[:, ::-1].copy()
+    Lr = o.weno5_reconstruct_xi(rev, dom, 0, 0, "WENO7-JS")
+    assert np.allclose(Rb[0, ::-1], Lr[0], rtol=0, atol=1e-14)
+
+
+def test_weno7_weights_are_inverse_squares_of_the_indicators():
+    """On integer data the indicators are exact integers, so the Jiang-Shu weights
+    omega_k = (d_k / beta_k^2) / sum_j (d_j / beta_j^2) (Balsara & Shu 2000, eq. 2.9, with eps
+    below every beta^2) and the face state can be evaluated in exact rational arithmetic; the
+    floating-point stencil agrees to rounding."""
+    from fractions import Fraction as Fr
+
+    u = (1, 3, 2, 5, 4, 4, 7)
+    q = [  # the quadratic forms of weno7_js.py:47-65 as coefficient lists over (u_k, ..., u_k+3)
+        ((547, -3882, 4642, -1854), (7043, -17246, 7042), (11003, -9402), (2107,)),
+        ((267, -1642, 1602, -494), (2843, -5966, 1922), (3443, -2522), (547,)),
+        ((547, -2522, 1922, -494), (3443, -5966, 1602), (2843, -1642), (267,)),
+        ((2107, -9402, 7042, -1854), (11003, -17246, 4642), (7043, -3882), (547,)),
+    ]
+    beta = []
+    for k in range(4):
+        w = u[k:k + 4]
+        beta.append(sum(w[r] * sum(c * w[r + m] for m, c in enumerate(q[k][r])) for r in range(4)))
+    d = (Fr(1, 35), Fr(12, 35), Fr(18, 35), Fr(4, 35))
+    cr = ((Fr(-1, 4), Fr(13, 12), Fr(-23, 12), Fr(25, 12)), (Fr(1, 12), Fr(-5, 12), Fr(13, 12), Fr(1, 4)),
+          (Fr(-1, 12), Fr(7, 12), Fr(7, 12), Fr(-1, 12)), (Fr(1, 4), Fr(13, 12), Fr(-5, 12), Fr(1, 12)))
+    alpha = [d[k] / Fr(beta[k]) ** 2 for k in range(4)]
+    p = [sum(cr[k][m] * u[k + m] for m in range(4)) for k in range(4)]
+    exact = sum(a * pk for a, pk in zip(alpha, p)) / sum(alpha)
+    got = o.weno7_js_point(*[np.float64(v) for v in u])
+    assert all(b > 0 for b in beta)
+    assert abs(float(got) - float(exact)) < 1e-14 * abs(float(exact))
