@@ -39,13 +39,21 @@ class InitializationManager:
         prim = np.stack(fields, axis=0)
         return prim.reshape((5,) + dom.global_number_of_cells)
 
-    def initialization(self, primitives: Optional[np.ndarray] = None) -> JaxFluidsBuffers:
+    def initialization(self, primitives: Optional[np.ndarray] = None,
+                       restart_file_path: Optional[str] = None) -> JaxFluidsBuffers:
         """Creates (W, U) on the device, fills halos, computes the first dt.
 
-        ``primitives`` (5, Nx, Ny, Nz) overrides the case-setup initial condition."""
+        ``primitives`` (5, Nx, Ny, Nz) overrides the case-setup initial condition;
+        ``restart_file_path`` starts from an HDF5 output file instead, at its time
+        (MaterialFieldsInitializer.from_restart_file, material_fields_initializer.py:212-333)."""
         im = self.input_manager
         dom = im.domain_information
         dtype = precision.torch_dtype
+        t0 = 0.0
+        if restart_file_path is not None:
+            from .hdf5 import read_restart
+
+            t0, primitives = read_restart(restart_file_path, dom)
         prim = self.primitives_on_cells() if primitives is None else np.asarray(primitives, dtype=np.float64)
         if prim.shape != (5,) + dom.global_number_of_cells:
             raise ValueError(f"initial primitives must have shape {(5,) + dom.global_number_of_cells}")
@@ -56,6 +64,6 @@ class InitializationManager:
         dt = compute_time_step_size(W, None, None, None, None, dom, im.equation_information, im.material_manager,
                                     None, im.numerical_setup)
         gs = im.case_setup.general
-        tcv = TimeControlVariables(0.0, 0, dt, im.numerical_setup.conservatives.time_integration.fixed_timestep,
+        tcv = TimeControlVariables(t0, 0, dt, im.numerical_setup.conservatives.time_integration.fixed_timestep,
                                    gs.end_time, gs.end_step)
         return JaxFluidsBuffers(MaterialFieldBuffers(U, W, None), tcv)

@@ -19,6 +19,8 @@ Optionally one whole step is captured in a HIP graph and replayed.
 from __future__ import annotations
 
 import time
+
+import numpy as np
 from typing import Optional
 
 import torch
@@ -173,8 +175,9 @@ class SimulationManager:
                            simulation_step=tcv.simulation_step + 1, physical_timestep_size=dt)
         return JaxFluidsBuffers(MaterialFieldBuffers(U, W, mf.temperature), tcv)
 
-    def advance(self, jxf_buffers: JaxFluidsBuffers) -> JaxFluidsBuffers:
-        """Host loop: while t < end_time and step < end_step (simulation_manager.py:325-328)."""
+    def advance(self, jxf_buffers: JaxFluidsBuffers, output=None) -> JaxFluidsBuffers:
+        """Host loop: while t < end_time and step < end_step (simulation_manager.py:325-328);
+        ``output`` (an _Output) writes files after the steps its schedule selects."""
         tcv = jxf_buffers.time_control_variables
         self.wall_clock_times = []
         while tcv.physical_simulation_time < tcv.end_time and tcv.simulation_step < tcv.end_step:
@@ -182,8 +185,66 @@ class SimulationManager:
             jxf_buffers = self.do_integration_step(jxf_buffers)
             tcv = jxf_buffers.time_control_variables
             self.wall_clock_times.append(time.perf_counter() - t0)
+            if output is not None:
+                output.after_step(jxf_buffers, self.wall_clock_times)
         torch.cuda.synchronize()
         return jxf_buffers
 
-    def simulate(self, jxf_buffers: JaxFluidsBuffers) -> JaxFluidsBuffers:
-        return self.advance(jxf_buffers)
+    def simulate(self, jxf_buffers: JaxFluidsBuffers, output_directory: Optional[str] = None) -> JaxFluidsBuffers:
+        """Runs the step loop.  With ``output_directory``, writes HDF5 and XDMF files there as the
+        reference's OutputWriter does (io_utils/output_writer.py:228-286; simulation_manager.py:
+        258,391,411): at the start, after every step that reaches the next multiple of
+        case_setup.general.save_dt (within 1e-12, 1e-6 in single precision), and at the end, then
+        the XDMF time series."""
+        if output_directory is None:
+            return self.advance(jxf_buffers)
+        out = _Output(self, output_directory)
+        out.write(jxf_buffers, [])
+        jxf_buffers = self.advance(jxf_buffers, out)
+        out.write(jxf_buffers, self.wall_clock_times)
+        out.xdmf.write_timeseries(output_directory)
+        return jxf_buffers
+
+
+class _Output:
+    """The save_dt schedule of io_utils/output_writer.py:55-72,261-266 and the HDF5/XDMF writers."""
+
+    def __init__(self, sim: "SimulationManager", directory: str) -> None:
+        import os
+
+        from .hdf5 import HDF5Writer, XDMFWriter
+
+        os.makedirs(directory, exist_ok=True)
+        self.dir = directory
+        im = sim.input_manager
+        gen = im.case_setup.general
+        dom = sim.domain_information
+        is_double = bool(precision.is_double)
+        R = float(im.case_setup.material.specific_gas_constant)
+        self.h5 = HDF5Writer(dom, gas_constant=R, is_double=is_double)
+        self.xdmf = XDMFWriter(dom, self.h5.quantities, is_double)
+        self.save_dt = gen.save_dt
+        self.next_time = self.save_dt  # save_start 0: the t = 0 output is written anyway
+        self.eps_time = 1e-12 if is_double else 1e-6
+        self.last_time = None
+
+    def write(self, jxf_buffers: JaxFluidsBuffers, wall_clock) -> None:
+        mf, tcv = jxf_buffers.material_fields, jxf_buffers.time_control_variables
+        t = float(tcv.physical_simulation_time)
+        if t == self.last_time:
+            return
+        steps = list(wall_clock)
+        ncell = float(np.prod(self.h5.domain.global_number_of_cells))
+        wc = (steps[-1] if steps else 0.0, (steps[-1] / ncell) if steps else 0.0,
+              float(np.mean(steps)) if steps else 0.0, float(np.mean(steps)) / ncell if steps else 0.0)
+        path = self.h5.write_file(self.dir, mf.primitives, mf.conservatives, t, wc)
+        self.xdmf.write_file(path, t)
+        self.last_time = t
+
+    def after_step(self, jxf_buffers: JaxFluidsBuffers, wall_clock) -> None:
+        if not self.save_dt:
+            return
+        t = float(jxf_buffers.time_control_variables.physical_simulation_time)
+        if t - self.next_time >= -self.eps_time:
+            self.next_time += self.save_dt
+            self.write(jxf_buffers, wall_clock)
