@@ -117,7 +117,7 @@ enum {
  *            cp = gamma R / (gamma - 1) (ideal_gas.py:32)
  *   T = p / (rho R)                      (ideal_gas.py:81-83)
  * The dissipative fluxes use central stencils across face-adjacent cells and their transverse
- * neighbours; they require PERIODIC boundaries on every active axis. */
+ * neighbours, halos and edge halos included on non-periodic domains. */
 typedef struct jfa_transport {
     int32_t viscous;        /* 1: viscous stresses tau and work u.tau (active_physics.is_viscous_flux) */
     int32_t heat;           /* 1: heat flux q = -kappa dT/dx (active_physics.is_heat_flux) */
@@ -132,6 +132,10 @@ typedef struct jfa_transport {
     double kappa_ref;
     double kappa_T_ref;
     double kappa_param;
+    int32_t halos;          /* jfa_compute_rhs only: 1 = the stencils read the buffer's face and edge
+                               halos, which the caller filled (non-periodic boundaries; the edge halos
+                               as jfa_edge_halo_update fills them); 0 = they wrap periodically.  The
+                               fused stage decides this from its boundary types itself. */
 } jfa_transport;
 
 /* Numerical scheme of the high-order Godunov flux (data_types/numerical_setup/conservatives.py:35-42). */
@@ -250,6 +254,15 @@ int jfa_rk3_stage(void* U, const void* U_n, const void* rhs, void* W, const jfa_
  * bc[6] in the order east, west, north, south, top, bottom (JFA_BC_*).
  * Replaces HaloManager.perform_halo_update_material (halos/halo_manager.py:146-234) ->
  * BoundaryConditionMaterial.face_halo_update (halos/outer/material.py:94-287, miscellaneous :868-894). */
+/* Edge halos of W, and U = cons(W) there, for viscous or heat fluxes on 2-D and 3-D grids
+ * (halo_manager.py:125-129,183-186; BoundaryConditionMaterial.edge_halo_update,
+ * halos/outer/material.py:289-380): on each active edge the first face (in the reference's edge
+ * order, the lower axis first) that is PERIODIC or SYMMETRY decides -- the periodic image, or the
+ * mirror with that axis' velocity negated, of the other face's halo -- else the mean of the two
+ * face halos moved nh cells into the interior.  The face halos must be filled already.  Vertex
+ * halos are left alone. */
+int jfa_edge_halo_update(void* W, void* U, const jfa_grid* grid, const int32_t bc[6], double gamma,
+                         int32_t dtype, void* stream);
 int jfa_halo_update(void* W, void* U, const jfa_grid* grid, const int32_t bc[6], double gamma,
                     int32_t dtype, void* stream);
 

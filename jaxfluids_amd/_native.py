@@ -45,7 +45,8 @@ class JfaTransport(ctypes.Structure):
                 ("mu_law", ctypes.c_int32), ("kappa_law", ctypes.c_int32),
                 ("mu_ref", ctypes.c_double), ("T_ref", ctypes.c_double), ("mu_param", ctypes.c_double),
                 ("bulk_viscosity", ctypes.c_double), ("prandtl", ctypes.c_double),
-                ("kappa_ref", ctypes.c_double), ("kappa_T_ref", ctypes.c_double), ("kappa_param", ctypes.c_double)]
+                ("kappa_ref", ctypes.c_double), ("kappa_T_ref", ctypes.c_double), ("kappa_param", ctypes.c_double),
+                ("halos", ctypes.c_int32)]
 
 
 class JfaScheme(ctypes.Structure):
@@ -101,6 +102,7 @@ SIGNATURES = {
     "jfa_rhs_workspace_bytes": (_SZ, [_PG, _I32]),
     "jfa_rk3_stage": (_I32, [_P, _P, _P, _P, _PG, _PS, _I32, _P, _D, _P]),
     "jfa_halo_update": (_I32, [_P, _P, _PG, _PI32, _D, _I32, _P]),
+    "jfa_edge_halo_update": (_I32, [_P, _P, _PG, _PI32, _D, _I32, _P]),
     "jfa_compute_dt": (_I32, [_P, _PG, _D, _D, _D, _I32, ctypes.POINTER(JfaTransport), _P, _P, _P]),
     "jfa_cons_from_prim": (_I32, [_P, _P, _I64, _D, _I32, _P]),
     "jfa_prim_from_cons": (_I32, [_P, _P, _I64, _D, _I32, _P]),
@@ -183,13 +185,15 @@ def make_transport(R: float = 1.0, mu_ref: float = 0.0, T_ref: float = 1.0, mu_e
                    bulk_viscosity: float = 0.0, prandtl: Optional[float] = None, viscous: bool = True,
                    mu_sutherland: Optional[float] = None, kappa_ref: Optional[float] = None,
                    kappa_T_ref: float = 1.0, kappa_exp: float = 0.0,
-                   kappa_sutherland: Optional[float] = None) -> JfaTransport:
+                   kappa_sutherland: Optional[float] = None, halos: bool = False) -> JfaTransport:
     """Dissipative-flux transport (material.py:82-124).
 
     Viscosity: mu_ref (T/T_ref)^mu_exp, or SUTHERLAND with constant mu_sutherland.  Heat flux when
     prandtl (kappa = cp mu / prandtl) or kappa_ref (kappa_ref (T/kappa_T_ref)^kappa_exp, or
-    SUTHERLAND with kappa_sutherland) is given."""
+    SUTHERLAND with kappa_sutherland) is given.  halos: jfa_compute_rhs reads the buffer's face
+    and edge halos (a non-periodic domain) instead of wrapping periodically."""
     t = JfaTransport()
+    t.halos = 1 if halos else 0
     t.viscous = 1 if viscous else 0
     t.heat = 1 if (prandtl is not None or kappa_ref is not None) else 0
     t.R, t.mu_ref, t.T_ref = float(R), float(mu_ref), float(T_ref)

@@ -634,7 +634,7 @@ __global__ __launch_bounds__(kBlock) void adj_dissip_cell_kernel(const double* _
     const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
     if (!flat_index(ncell, G.n[1], G.n[2], c[0], c[1], c[2])) return;
     const int64_t idx = ((int64_t)c[0] * G.n[1] + c[1]) * G.n[2] + c[2];
-    const int64_t b = wrapped_index(G, c), fs = G.fs;
+    const int64_t b = wrapped_index(G, c, 0), fs = G.fs;
     double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, x[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const double lc[5] = {ld(lam, b), ld(lam, fs + b), ld(lam, 2 * fs + b), ld(lam, 3 * fs + b), ld(lam, 4 * fs + b)};
 #pragma unroll
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void adj_dissip_cell_kernel(const double* _
             int cr[3] = {c[0], c[1], c[2]}, co[3] = {c[0], c[1], c[2]};
             cr[A] += side;
             co[A] += side ? 1 : -1;  // the face's other cell
-            const int64_t bo = wrapped_index(G, co);
+            const int64_t bo = wrapped_index(G, co, 0);
             double mu[5], gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gr[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Gt[2][3];
             const double sc = ts * G.one_dx[A];
 #pragma unroll

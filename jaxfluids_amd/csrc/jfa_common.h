@@ -286,6 +286,91 @@ __device__ __forceinline__ void halo_scatter(T* __restrict__ W, T* __restrict__ 
     }
 }
 
+// Edge halos of W (and U = cons(W) there) on the active edges, from the face halos already in the
+// buffer (BoundaryConditionMaterial.edge_halo_update, halos/outer/material.py:289-380).  blockIdx.y
+// is the edge: axis pair e / 4 ((0,1), (1,2), (0,2)), high face of the lower axis a when e & 1, of
+// the upper axis b when e & 2; one thread per cell of the nh x nh x n_c block.  The first of the two
+// faces in the reference's edge names (the one on a, boundary_condition.py:128-180) that is
+// PERIODIC or SYMMETRY decides:
+//   PERIODIC on axis x: the cell N_x further in along x (the opposite face's block,
+//                       boundary_condition.py:607-654);
+//   SYMMETRY on axis x: the mirror across the boundary along x, velocity x negated (:657-760);
+//   neither:            0.5 (W[moved nh in along a] + W[moved nh in along b]), both face halos
+//                       (halo_slices.py:77-150, the "_10" and "_01" blocks).
+// Every source cell is interior along the third axis, so no edge reads another edge.
+template <class T>
+__global__ __launch_bounds__(kBlock) void edge_halo_kernel(T* __restrict__ W, T* __restrict__ U, Geo G, BC6 bc,
+                                                           Params prm) {
+    const int e = blockIdx.y;
+    const int pair = e >> 2;
+    const int a = pair == 1 ? 1 : 0, b = pair == 0 ? 1 : 2, c = 3 - a - b;
+    if (!G.active[a] || !G.active[b]) return;
+    const int ha = e & 1, hb = (e >> 1) & 1;
+    const int nh = G.nh;
+    const int64_t total = (int64_t)nh * nh * G.n[c];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int k = (int)(t % G.n[c]);
+    const int ib = (int)((t / G.n[c]) % nh);
+    const int ia = (int)(t / ((int64_t)G.n[c] * nh));
+    const int pa = ha ? G.n[a] + nh + ia : ia, pb = hb ? G.n[b] + nh + ib : ib;  // buffer coordinates
+    const int64_t dst = (int64_t)pa * G.s[a] + (int64_t)pb * G.s[b] + (int64_t)(k + G.off[c]) * G.s[c];
+    const int ta = bc.t[2 * a + (ha ? 0 : 1)], tb = bc.t[2 * b + (hb ? 0 : 1)];
+    const int64_t fs = G.fs;
+    double w[5], u[5];
+    int x = -1, kind = 0;
+    if (ta == JFA_BC_PERIODIC || ta == JFA_BC_SYMMETRY) {
+        x = a;
+        kind = ta;
+    } else if (tb == JFA_BC_PERIODIC || tb == JFA_BC_SYMMETRY) {
+        x = b;
+        kind = tb;
+    }
+    if (x < 0) {
+        const int64_t s1 = dst + (ha ? -(int64_t)nh : (int64_t)nh) * G.s[a];
+        const int64_t s2 = dst + (hb ? -(int64_t)nh : (int64_t)nh) * G.s[b];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) w[i] = 0.5 * (ld(W, i * fs + s1) + ld(W, i * fs + s2));
+    } else {
+        const int hi = x == a ? ha : hb, pos = x == a ? pa : pb, N = G.n[x];
+        int src;
+        if (kind == JFA_BC_PERIODIC) src = hi ? pos - N : pos + N;
+        else src = hi ? 2 * (N + nh) - 1 - pos : 2 * nh - 1 - pos;
+        const int64_t s = dst + (int64_t)(src - pos) * G.s[x];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) w[i] = ld(W, i * fs + s);
+        if (kind == JFA_BC_SYMMETRY) w[x + 1] = -w[x + 1];
+    }
+    cons_from_prim(w, prm, u);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        W[i * fs + dst] = static_cast<T>(w[i]);
+        if (U) U[i * fs + dst] = static_cast<T>(u[i]);
+    }
+}
+
+// Whether the dissipative fluxes read halos (a transport with viscous or heat fluxes and a
+// non-periodic active face); periodic domains wrap onto the interior instead.
+static inline bool dissipative_halos(const Params& p, const Geo& G, const BC6& bc) {
+    if (!p.dissip) return false;
+    for (int f = 0; f < 6; ++f)
+        if (G.active[f / 2] && bc.t[f] != JFA_BC_PERIODIC) return true;
+    return false;
+}
+
+template <class T>
+static void launch_edge_halos(T* W, T* U, const Geo& G, const BC6& bc, const Params& prm, hipStream_t st) {
+    int nact = 0, nmax = 1;
+    for (int i = 0; i < 3; ++i) {
+        nact += G.active[i];
+        if (G.n[i] > nmax) nmax = G.n[i];
+    }
+    if (nact < 2) return;
+    const int64_t total = (int64_t)G.nh * G.nh * nmax;
+    hipLaunchKernelGGL((edge_halo_kernel<T>), dim3((unsigned)((total + kBlock - 1) / kBlock), 12), dim3(kBlock), 0, st,
+                       W, U, G, bc, prm);
+}
+
 // Halo cells a stencil's window needs (WENO7-JS: 4; every other stencil reads cells -3..2).
 static inline int check_stencil_halos(const jfa_scheme* s, const Geo& G) {
     return (s->stencil == JFA_WENO7_JS && G.nh < 4) ? JFA_EINVAL : JFA_OK;
@@ -327,6 +412,7 @@ static inline Law make_law(int kind, double ref, double T_ref, double param) {
 // Dissipative-flux fields of p from a transport (NULL or neither flux: inviscid).
 static inline void set_transport(Params& p, const jfa_transport* t, double gamma) {
     p.dissip = 0;
+    p.diss_halos = 0;
     p.R = 1.0;
     p.mu = p.kappa = make_law(JFA_LAW_POWER, 0.0, 1.0, 0.0);
     p.bulk = p.kappa_fac = p.inv_cp = 0.0;
@@ -411,9 +497,9 @@ static inline bool valid_law(int kind, double ref, double T_ref, double param) {
     return kind == JFA_LAW_POWER || T_ref + param > 0.0;
 }
 
-// A transport is usable when its constants are finite and positive where needed; dissipative
-// fluxes need PERIODIC boundaries on every active axis (their central stencils wrap).
-static inline int check_transport(const jfa_scheme* s, const Geo& G, const int32_t* bc) {
+// A transport is usable when its constants are finite and positive where needed.  periodic_bc
+// (the adjoint's boundaries, when non-null) must be PERIODIC on every active axis.
+static inline int check_transport(const jfa_scheme* s, const Geo& G, const int32_t* periodic_bc) {
     const jfa_transport* t = s->transport;
     if (!t || !(t->viscous || t->heat)) return JFA_OK;
     if (!(t->R > 0.0) || !(t->bulk_viscosity >= 0.0)) return JFA_EINVAL;
@@ -425,9 +511,9 @@ static inline int check_transport(const jfa_scheme* s, const Geo& G, const int32
             return JFA_EINVAL;
         }
     }
-    if (bc)
+    if (periodic_bc)
         for (int f = 0; f < 6; ++f)
-            if (G.active[f / 2] && bc[f] != JFA_BC_PERIODIC) return JFA_EUNSUPPORTED;
+            if (G.active[f / 2] && periodic_bc[f] != JFA_BC_PERIODIC) return JFA_EUNSUPPORTED;
     return JFA_OK;
 }
 

@@ -54,6 +54,7 @@ struct Params {
     int frozen;        // JFA_FROZEN_*
     // dissipative fluxes (jfa_transport); dissip = 0: inviscid
     int dissip;        // bit 0: viscous stresses, bit 1: heat flux
+    int diss_halos;    // 1: the dissipative stencils read face and edge halos; 0: they wrap periodically
     double R;          // specific gas constant
     Law mu;            // dynamic viscosity mu(T)
     Law kappa;         // thermal conductivity kappa(T); kind JFA_LAW_PRANDTL: kappa_fac mu

@@ -11,8 +11,9 @@
 //   tau_k = mu (g[AXIS][k] + g[k][AXIS]) (active k), tau_AXIS += mu2 sum_k g[k][k]
 //   F[1+k] += -tau_k;  F[4] += -sum_k tau_k u_f,k + q,  q = -kappa(T_f) (T_R - T_L)/dx,
 //   kappa = cp mu / Pr (PRANDTL) or its own POWER / SUTHERLAND law      (material.py:103-124)
-// All accesses wrap periodically on the active axes (the reference fills edge halos with the
-// periodic copies for viscous runs), so no edge halo is read.
+// On periodic domains the accesses wrap onto the interior (the halos and edge halos there are
+// periodic copies); otherwise (prm.diss_halos) they read the buffer's face and edge halos, which
+// the stage's halo scatter and edge-halo fill wrote.
 #pragma once
 
 #include "jfa_common.h"
@@ -34,13 +35,14 @@ __device__ __forceinline__ double conductivity(double T, double mu, const Params
     return prm.kappa.kind == JFA_LAW_PRANDTL ? prm.kappa_fac * mu : law_value(T, prm.kappa);
 }
 
-// buffer index of interior cell c, coordinates wrapped onto [0, n) on the active axes
-__device__ __forceinline__ int64_t wrapped_index(const Geo& G, const int c[3]) {
+// buffer index of cell c (interior coordinates, -1 and n are halo cells): wrapped onto [0, n) on
+// the active axes of a periodic domain, the halo cell itself otherwise
+__device__ __forceinline__ int64_t wrapped_index(const Geo& G, const int c[3], int halos) {
     int64_t b = 0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         int ci = c[i];
-        if (G.active[i]) ci = ci < 0 ? ci + G.n[i] : (ci >= G.n[i] ? ci - G.n[i] : ci);
+        if (G.active[i] && !halos) ci = ci < 0 ? ci + G.n[i] : (ci >= G.n[i] ? ci - G.n[i] : ci);
         b += (int64_t)(ci + G.off[i]) * G.s[i];
     }
     return b;
@@ -70,7 +72,7 @@ __device__ __forceinline__ void dissipative_state(const T* __restrict__ W, const
     const int64_t fs = G.fs;
     int cl[3] = {cr[0], cr[1], cr[2]};
     cl[AXIS] -= 1;
-    const int64_t bl = wrapped_index(G, cl), br = wrapped_index(G, cr);
+    const int64_t bl = wrapped_index(G, cl, prm.diss_halos), br = wrapped_index(G, cr, prm.diss_halos);
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         s.wl[i] = ld(W, i * fs + bl);
@@ -97,16 +99,16 @@ __device__ __forceinline__ void dissipative_state(const T* __restrict__ W, const
             double up[3], um[3], dL[3], dR[3];
             c[0] = cl[0]; c[1] = cl[1]; c[2] = cl[2];
             c[i] += 1;
-            load_velocity(W, fs, wrapped_index(G, c), up);
+            load_velocity(W, fs, wrapped_index(G, c, prm.diss_halos), up);
             c[i] -= 2;
-            load_velocity(W, fs, wrapped_index(G, c), um);
+            load_velocity(W, fs, wrapped_index(G, c, prm.diss_halos), um);
 #pragma unroll
             for (int k = 0; k < 3; ++k) dL[k] = (up[k] - um[k]) * h;
             c[0] = cr[0]; c[1] = cr[1]; c[2] = cr[2];
             c[i] += 1;
-            load_velocity(W, fs, wrapped_index(G, c), up);
+            load_velocity(W, fs, wrapped_index(G, c, prm.diss_halos), up);
             c[i] -= 2;
-            load_velocity(W, fs, wrapped_index(G, c), um);
+            load_velocity(W, fs, wrapped_index(G, c, prm.diss_halos), um);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 dR[k] = (up[k] - um[k]) * h;

@@ -497,8 +497,9 @@ JFA_FUSED_SWEEPS(extern, float, JFA_CHAR_CONSERVATIVE)
 template <class T>
 static int run_sweeps(const jfa_scheme* s, const void* W, const void* U, const Geo& G, double* R,
                       const InnerArgs<T>& ia, const double* dt_dev, double dt_host, hipStream_t st,
-                      int64_t* nparts, unsigned long long* wave_slots) {
+                      int64_t* nparts, unsigned long long* wave_slots, int diss_halos) {
     Params prm = make_params(s);
+    prm.diss_halos = diss_halos;
     set_flux_partition(prm, s, G.one_dx, G.active);
     set_cell_sizes(prm, G);
     prm.dt_ptr = dt_dev;  // the flux limiter's step dt
@@ -539,11 +540,21 @@ int fused_stage_impl(const void* W_in, const void* U_in, const void* U_n, void* 
     int64_t nparts = 0;
     // the WAVESPEED slots follow the CFL partials (inside the workspace's 256-byte tail)
     unsigned long long* slots = reinterpret_cast<unsigned long long*>(part + fused_partials(G));
-    int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, dt_dev, dt_host, st, &nparts, slots);
+    // Non-periodic dissipative fluxes read the halos (periodic ones wrap onto the interior).  The
+    // inner kernel's scatter fills the face halos of the new state; with dissipative fluxes the
+    // edge halos follow (fill_edge_halos_material, halo_manager.py:125-129,183-186).
+    const Params base = make_params(s);
+    const int diss_halos = dissipative_halos(base, G, bc) ? 1 : 0;
+    int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, dt_dev, dt_host, st, &nparts, slots, diss_halos);
     if (rc) return rc;
+    if (base.dissip) {
+        launch_edge_halos<T>(static_cast<T*>(W_out), static_cast<T*>(U_out), G, bc, base, st);
+        rc = check_launch();
+        if (rc) return rc;
+    }
     if (dt_next) {
         hipLaunchKernelGGL(dt_from_parts_kernel<0>, dim3(1), dim3(1024), 0, st, part, (int)nparts, G.dx_min,
-                           cfl, eps, make_params(s).dissip, dt_next);
+                           cfl, eps, base.dissip, dt_next);
         rc = check_launch();
     }
     return rc;
@@ -557,7 +568,8 @@ int fused_rhs_impl(const void* W, const void* U, const Geo& G, const jfa_scheme*
     ia.rhs_out = static_cast<T*>(rhs);
     // the WAVESPEED slots follow the partial RHS (inside the workspace's 256-byte tail)
     unsigned long long* slots = reinterpret_cast<unsigned long long*>(R + 5 * (int64_t)G.n[0] * G.n[1] * G.n[2]);
-    return run_sweeps<T>(s, W, U, G, R, ia, nullptr, dt, st, nullptr, slots);
+    const int halos = s->transport && s->transport->halos ? 1 : 0;
+    return run_sweeps<T>(s, W, U, G, R, ia, nullptr, dt, st, nullptr, slots, halos);
 }
 
 }  // namespace jfa

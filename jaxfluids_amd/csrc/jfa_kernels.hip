@@ -462,7 +462,7 @@ int jfa_compute_rhs(const void* W, const void* U, const jfa_grid* grid, const jf
     if (workspace_bytes < jfa_rhs_workspace_bytes(grid, scheme->dtype)) return JFA_EINVAL;
     rc = check_stencil_halos(scheme, G);
     if (rc) return rc;
-    rc = check_transport(scheme, G, nullptr);  // dissipative fluxes: the caller's boundaries are periodic
+    rc = check_transport(scheme, G, nullptr);  // transport->halos says whether the caller's halos are read
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     if (scheme->flux_limiter && !(dt > 0.0)) return JFA_EINVAL;  // the flux limiter needs the step's dt
@@ -520,6 +520,26 @@ int jfa_halo_update(void* W, void* U, const jfa_grid* grid, const int32_t bc[6],
         if (rc) return rc;
     }
     return JFA_OK;
+}
+
+int jfa_edge_halo_update(void* W, void* U, const jfa_grid* grid, const int32_t bc[6], double gamma,
+                         int32_t dtype, void* stream) {
+    Geo G;
+    int rc = make_geo(grid, G);
+    if (rc) return rc;
+    if (!W || !bc || !(gamma > 1.0)) return JFA_EINVAL;
+    if (dtype != JFA_F64 && dtype != JFA_F32) return JFA_EUNSUPPORTED;
+    BC6 b6;
+    for (int f = 0; f < 6; ++f) {
+        b6.t[f] = bc[f];
+        if (!G.active[f / 2]) continue;
+        if (bc[f] < 0 || bc[f] > 2) return JFA_EUNSUPPORTED;
+        if (G.nh > G.n[f / 2] && bc[f] != JFA_BC_ZEROGRADIENT) return JFA_EINVAL;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == JFA_F32) launch_edge_halos<float>((float*)W, (float*)U, G, b6, gas_params(gamma), st);
+    else launch_edge_halos<double>((double*)W, (double*)U, G, b6, gas_params(gamma), st);
+    return check_launch();
 }
 
 int jfa_compute_dt(const void* W, const jfa_grid* grid, double gamma, double cfl, double eps, int32_t dtype,
@@ -640,7 +660,7 @@ static int fused_stage_entry(const void* W_in, const void* U_in, const void* U_n
         const int axis = i / 2;
         if (G.active[axis] && G.nh > G.n[axis] && bc[i] != JFA_BC_ZEROGRADIENT) return JFA_EINVAL;
     }
-    rc = check_transport(scheme, G, bc);
+    rc = check_transport(scheme, G, nullptr);
     if (rc) return rc;
     const size_t ncell = (size_t)G.n[0] * G.n[1] * G.n[2];
     double* R = (double*)workspace;

@@ -1,9 +1,11 @@
-"""Face-halo fill on the HIP path.
+"""Face- and edge-halo fill on the HIP path.
 
 Mirrors halos/halo_manager.py (HaloManager.perform_halo_update_material
 :146-234) -> halos/outer/material.py (face_halo_update :94-287, miscellaneous
-:868-894) for PERIODIC, ZEROGRADIENT and SYMMETRY boundaries; the conservative
-halo is recomputed from the primitive halo (:248-250).
+:868-894; edge_halo_update :289-380) for PERIODIC, ZEROGRADIENT and SYMMETRY
+boundaries; the conservative halo is recomputed from the primitive halo
+(:248-250).  Edge halos are filled when the material has viscous or heat fluxes
+(:125-129), whose central stencils read them on non-periodic domains.
 """
 from __future__ import annotations
 
@@ -36,8 +38,9 @@ class HaloManager:
         self.bc_array = _native.bc_array(
             {loc: (self.boundary_conditions.get(loc) if loc in domain_information.active_face_locations
                    else "PERIODIC") for loc in _native.FACE_LOCATIONS})
-        self.fill_edge_halos_material = False     # inviscid path (halo_manager.py:125-129)
-        self.fill_vertex_halos_material = False
+        # dissipative cell-face fluxes read edge halos (halo_manager.py:125-129)
+        self.fill_edge_halos_material = getattr(material_manager, "transport", None) is not None
+        self.fill_vertex_halos_material = False   # levelset / diffuse-interface only
 
     def perform_halo_update_material(self, primitives: torch.Tensor, physical_simulation_time: float = 0.0,
                                      fill_edge_halos: bool = False, fill_vertex_halos: bool = False,
@@ -47,8 +50,8 @@ class HaloManager:
 
         The reference returns new arrays; with ``inplace=True`` the given tensors are
         updated instead (no copy)."""
-        if fill_edge_halos or fill_vertex_halos:
-            raise NotImplementedError("edge/vertex halos are only needed by viscous fluxes")
+        if fill_vertex_halos:
+            raise NotImplementedError("vertex halos are only needed by levelset and diffuse-interface fluxes")
         _native.require_device(primitives, "primitives")
         W = primitives if inplace else primitives.clone()
         U = None
@@ -60,4 +63,9 @@ class HaloManager:
                                                 self.bc_array, self.material_manager.get_gamma(),
                                                 _native.dtype_code(W), _native.stream_ptr())
             _native.check(rc, "jfa_halo_update")
+        if fill_edge_halos:  # after the face halos they are built from (halo_manager.py:183-186)
+            rc = _native.load().jfa_edge_halo_update(_native.ptr(W), _native.ptr(U), self.domain_information.grid,
+                                                     self.bc_array, self.material_manager.get_gamma(),
+                                                     _native.dtype_code(W), _native.stream_ptr())
+            _native.check(rc, "jfa_edge_halo_update")
         return (W, U) if conservatives is not None else W

@@ -60,7 +60,8 @@ class InitializationManager:
         W = torch.full((5,) + dom.buffer_shape, precision.get_eps(), dtype=dtype, device=self.device)
         W[dom.interior()] = torch.as_tensor(prim, dtype=dtype, device=self.device)
         U = im.equation_manager.get_conservatives_from_primitives(W)
-        W, U = self.halo_manager.perform_halo_update_material(W, 0.0, False, False, U, inplace=True)
+        hm = self.halo_manager
+        W, U = hm.perform_halo_update_material(W, 0.0, hm.fill_edge_halos_material, False, U, inplace=True)
         dt = compute_time_step_size(W, None, None, None, None, dom, im.equation_information, im.material_manager,
                                     None, im.numerical_setup)
         gs = im.case_setup.general

@@ -346,8 +346,9 @@ class InputManager:
 
     def _native_transport(self):
         """jfa_transport of the active dissipative fluxes (None: inviscid).  The transport
-        properties are required for them (read_material_manager.py:211-221), and the built
-        central stencils wrap periodically, so every active boundary must be PERIODIC."""
+        properties are required for them (read_material_manager.py:211-221).  On periodic domains
+        the central stencils wrap onto the interior; with a non-periodic active face they read the
+        face and edge halos (the transport's halos flag, for the unfused RHS)."""
         from . import _native
 
         ap = self.numerical_setup.active_physics
@@ -358,11 +359,9 @@ class InputManager:
         assert tr is not None, "material_properties/transport is required for viscous or heat fluxes"
         assert not ap.is_heat_flux or tr.prandtl is not None or tr.kappa_ref is not None, \
             "material_properties/transport/thermal_conductivity is required for the heat flux"
-        for loc in self.domain_information.active_face_locations:
-            if cs.boundary_conditions[loc] != "PERIODIC":
-                raise NotImplementedError(
-                    f"dissipative fluxes are built for periodic domains; boundary {loc} is {cs.boundary_conditions[loc]}")
+        halos = any(cs.boundary_conditions[loc] != "PERIODIC" for loc in self.domain_information.active_face_locations)
         kw = tr._asdict()
         if not ap.is_heat_flux:
             kw["prandtl"] = kw["kappa_ref"] = None
-        return _native.make_transport(R=cs.material.specific_gas_constant, viscous=ap.is_viscous_flux, **kw)
+        return _native.make_transport(R=cs.material.specific_gas_constant, viscous=ap.is_viscous_flux, halos=halos,
+                                      **kw)

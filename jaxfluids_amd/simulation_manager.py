@@ -120,7 +120,8 @@ class SimulationManager:
                 integ, rhs, init, dt, stage, self.equation_information, self.domain_information,
                 self.material_manager.get_gamma(), primitives_out=W_new)
             U = integ.euler_buffers.conservatives
-            W, U = self.halo_manager.perform_halo_update_material(W_new, 0.0, False, False, U, inplace=True)
+            W, U = self.halo_manager.perform_halo_update_material(W_new, 0.0, self.halo_manager.fill_edge_halos_material,
+                                                                  False, U, inplace=True)
             integ = IntegrationBuffers(EulerIntegrationBuffers(U))
         return W, U
 

@@ -1195,9 +1195,11 @@ def dissipative_flux_xi(W, dom: Domain, axis: int, s: Setup):
     convective face flux (space_solver.py:565-587): momentum -tau_axis,k, energy
     -sum_k tau_axis,k u_k + q with q = -kappa dT/dx_axis."""
     tr = s.transport
-    assert all(s.bcs[f] == "PERIODIC" for f in FACE_LOCATIONS if FACE_AXIS[f] in dom.active), \
-        "dissipative fluxes: periodic domains"
-    Wp = wrap_halos(W, dom)
+    # the cells next to the faces and their transverse neighbours, face and edge halos included
+    # (filled by halo_update with the edge fill); on periodic domains these are the wrapped
+    # interior cells
+    periodic = all(s.bcs[f] == "PERIODIC" for f in FACE_LOCATIONS if FACE_AXIS[f] in dom.active)
+    Wp = wrap_halos(W, dom) if periodic else W
     L, R = _face_cells(Wp, dom, axis, -1), _face_cells(Wp, dom, axis, 0)
     T_L, T_R = L[4] / (L[0] * tr.R), R[4] / (R[0] * tr.R)
     T_f = 0.5 * (T_L + T_R)                                    # central_2.py reconstruct_xi
@@ -1358,11 +1360,98 @@ def _face_retrieve_slice(dom: Domain, loc: str, bc: str):
     return (Ellipsis,) + tuple(s)
 
 
-def halo_update(W, U, dom: Domain, s: Setup):
+# domain/__init__.py:9-15; the first face of each name lies on the lower axis
+EDGE_LOCATIONS = ("west_south", "west_north", "east_north", "east_south", "south_bottom", "north_bottom",
+                  "south_top", "north_top", "east_bottom", "west_bottom", "east_top", "west_top")
+_OPPOSITE = {"east": "west", "west": "east", "north": "south", "south": "north", "top": "bottom", "bottom": "top"}
+
+
+def edge_type(s: Setup, edge: str) -> str:
+    """assign_boundary_type_combinations (outer/boundary_condition.py:128-180): the first face of
+    the edge (in name order) that is PERIODIC or SYMMETRY gives its type, the other face "ANY";
+    with neither, "ANY_ANY"."""
+    out, found = [], False
+    for face in edge.split("_"):
+        b = s.bcs[face]
+        if not found and b in ("PERIODIC", "SYMMETRY"):
+            out.append(b)
+            found = True
+        else:
+            out.append("ANY")
+    return "_".join(out)
+
+
+def _edge_slice(dom: Domain, edge: str, shift: Optional[Tuple[int, int]] = None):
+    """edge_slices of halos/halo_slices.py:77-150: the edge halo block (nh x nh cells at the two
+    faces, interior along the third axis); shift (da, db) in {0, 1} moves it by nh cells into the
+    interior along the first (da) or second (db) face's axis (the "_10" / "_01" blocks), and
+    PERIODIC retrieval uses the opposite face's block."""
+    nh = dom.nh
+    sl = [dom.dom(0), dom.dom(1), dom.dom(2)]
+    for k, face in enumerate(edge.split("_")):
+        a = FACE_AXIS[face]
+        hi = face in ("east", "north", "top")
+        moved = shift is not None and shift[k]
+        if not moved:
+            sl[a] = slice(-nh, None) if hi else slice(None, nh)
+        else:
+            sl[a] = slice(-2 * nh, -nh) if hi else slice(nh, 2 * nh)
+    return (Ellipsis,) + tuple(sl)
+
+
+def edge_halo_update(W, U, dom: Domain, s: Setup):
+    """BoundaryConditionMaterial.edge_halo_update / compute_edge_halos (outer/material.py:289-380)
+    on the active edges (both faces' axes active):
+    ANY_ANY: the mean of the face-halo blocks next to the edge, moved nh cells into the interior
+    along either face's axis ("_10", "_01"), unmirrored (halo_slices.py:77-150);
+    PERIODIC on a face: the block of the opposite face moved along that face's axis ("_10" or
+    "_01" of the edge with that face replaced by its opposite, boundary_condition.py:607-654);
+    SYMMETRY on a face: the block moved along that face's axis, reversed along it, with that
+    axis' velocity negated (:657-760).  U = cons(W) on the edge cells."""
+    W = W.copy()
+    U = None if U is None else U.copy()
+    for edge in EDGE_LOCATIONS:
+        faces = edge.split("_")
+        if not all(FACE_AXIS[f] in dom.active for f in faces):
+            continue
+        t = edge_type(s, edge).split("_")
+        if t == ["ANY", "ANY"]:
+            halos = 0.5 * (W[_edge_slice(dom, edge, (1, 0))] + W[_edge_slice(dom, edge, (0, 1))])
+        else:
+            k = t.index("PERIODIC") if "PERIODIC" in t else t.index("SYMMETRY")
+            shift = (1, 0) if k == 0 else (0, 1)
+            if t[k] == "PERIODIC":
+                src = list(faces)
+                src[k] = _OPPOSITE[faces[k]]
+                halos = W[_edge_slice(dom, "_".join(src), shift)]
+            else:
+                halos = W[_edge_slice(dom, edge, shift)]
+                a = FACE_AXIS[faces[k]]
+                flip = [slice(None)] * 4
+                flip[1 + a] = slice(None, None, -1)
+                halos = halos[tuple(flip)].copy()
+                halos[1 + a] = -halos[1 + a]
+        W[_edge_slice(dom, edge)] = halos
+        if U is not None:
+            U[_edge_slice(dom, edge)] = cons_from_prim(halos, s.gamma)
+    return W, U
+
+
+def halo_update(W, U, dom: Domain, s: Setup, fill_edge: Optional[bool] = None):
     """BoundaryConditionMaterial.face_halo_update for PERIODIC / ZEROGRADIENT /
     SYMMETRY (outer/material.py:94-287, miscellaneous :868-894).  U halo is
-    recomputed from the W halo (:248-250).  Face halos only; edge and vertex
-    halos are not filled on the inviscid path (halo_manager.py:125-129)."""
+    recomputed from the W halo (:248-250).  Edge halos follow with viscous or heat fluxes
+    (fill_edge_halos_material, halo_manager.py:125-129,183-186; 2-D and 3-D), vertex halos never
+    on this path."""
+    if fill_edge is None:
+        fill_edge = s.transport is not None
+    W, U = _face_halo_update(W, U, dom, s)
+    if fill_edge and len(dom.active) > 1:
+        W, U = edge_halo_update(W, U, dom, s)
+    return W, U
+
+
+def _face_halo_update(W, U, dom: Domain, s: Setup):
     W = W.copy()
     U = None if U is None else U.copy()
     for loc in FACE_LOCATIONS:

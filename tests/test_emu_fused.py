@@ -43,7 +43,7 @@ def emu():
     from jaxfluids_amd import _native
 
     lib = ctypes.CDLL(emu_build.build())
-    for name in ("jfa_stage_workspace_bytes", "jfa_rk3_stage_fused"):
+    for name in ("jfa_stage_workspace_bytes", "jfa_rk3_stage_fused", "jfa_edge_halo_update"):
         fn = getattr(lib, name)
         fn.restype, fn.argtypes = _native.SIGNATURES[name]
     lib.emu_add_range.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
@@ -92,14 +92,31 @@ def check_step(emu, dom, s, W0, transport=None, dt_scale=1.0):
     assert np.all(np.isfinite(Wr[I])) and np.all(Wr[4][I] > 0)
     Wg, Ug, dtn = fused_step(emu, dom, s, W, U, dt, transport)
     assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL, (rel(Wg[I], Wr[I]), rel(Ug[I], Ur[I]))
-    # face halos written by the last stage (edge halos are not part of the state)
-    for a in dom.active:
-        sl = [dom.dom(0), dom.dom(1), dom.dom(2)]
-        for part in (slice(0, dom.nh), slice(-dom.nh, None)):
-            sl[a] = part
-            t = (Ellipsis,) + tuple(sl)
-            assert rel(Wg[t], Wr[t]) < TOL and rel(Ug[t], Ur[t]) < TOL, (a, part)
+    # face halos written by the last stage; with dissipative fluxes the edge halos too
+    for t in halo_regions(dom, edges=transport is not None):
+        assert rel(Wg[t], Wr[t]) < TOL and rel(Ug[t], Ur[t]) < TOL, t
     assert abs(dtn - o.compute_dt(Wr, dom, s)) <= 1e-14 * dt
+
+
+def halo_regions(dom, edges=False):
+    """Index tuples of the face-halo slabs (interior along the other axes) and, with edges, of the
+    edge-halo blocks of every pair of active axes (interior along the third)."""
+    parts = (slice(0, dom.nh), slice(-dom.nh, None))
+    out = []
+    for a in dom.active:
+        for part in parts:
+            sl = [dom.dom(0), dom.dom(1), dom.dom(2)]
+            sl[a] = part
+            out.append((Ellipsis,) + tuple(sl))
+    if edges:
+        for i, a in enumerate(dom.active):
+            for b in dom.active[i + 1:]:
+                for pa in parts:
+                    for pb in parts:
+                        sl = [dom.dom(0), dom.dom(1), dom.dom(2)]
+                        sl[a], sl[b] = pa, pb
+                        out.append((Ellipsis,) + tuple(sl))
+    return out
 
 
 GRIDS = {
@@ -144,6 +161,55 @@ def test_fused_step_viscous(emu, grid):
     s = o.Setup(gamma=1.4, bcs={f: "PERIODIC" for f in FACES}, cfl=0.5, transport=tr,
                 **SCHEMES["z-charprim-hllc"])
     check_step(emu, dom, s, W0, transport=tr)
+
+
+VISCOUS_BCS = {
+    "zerogradient": {f: "ZEROGRADIENT" for f in FACES},
+    "symmetry": {f: "SYMMETRY" for f in FACES},
+    # periodic x with walls and outflow: PERIODIC decides the x edges, SYMMETRY the y-z edges
+    "mixed": {"east": "PERIODIC", "west": "PERIODIC", "north": "SYMMETRY", "south": "ZEROGRADIENT",
+              "top": "ZEROGRADIENT", "bottom": "SYMMETRY"},
+    # SYMMETRY on the upper axis only: it decides the edges where the lower axis' face is ANY
+    "upper-symmetry": {"east": "ZEROGRADIENT", "west": "ZEROGRADIENT", "north": "SYMMETRY", "south": "SYMMETRY",
+                       "top": "SYMMETRY", "bottom": "ZEROGRADIENT"},
+}
+
+
+@pytest.mark.parametrize("bcs", list(VISCOUS_BCS))
+@pytest.mark.parametrize("grid", ["3d", "2d-xy", "2d-xz", "1d"])
+def test_fused_step_viscous_boundaries(emu, grid, bcs):
+    """Dissipative fluxes on non-periodic domains: the stencils read the face halos and the edge
+    halos, which every stage refills (the intermediate buffers start as NaN, so a halo read
+    before it is written shows)."""
+    dom, W0 = grid_state(grid, seed=9)
+    tr = o.Transport(R=1.0, mu_ref=2e-2, T_ref=1.0, mu_exp=0.76, bulk_viscosity=1e-2, prandtl=0.71)
+    s = o.Setup(gamma=1.4, bcs=VISCOUS_BCS[bcs], cfl=0.5, transport=tr, **SCHEMES["z-charprim-hllc"])
+    check_step(emu, dom, s, W0, transport=tr)
+
+
+@pytest.mark.parametrize("bcs", list(VISCOUS_BCS))
+@pytest.mark.parametrize("grid", ["3d", "2d-yz"])
+def test_edge_halo_update(emu, grid, bcs):
+    """jfa_edge_halo_update on a buffer with random face halos against the oracle's edge fill:
+    W bit-exact (copies, sign flips and one mean), U = cons(W) to an ulp of the energy; vertex
+    halos stay as they were."""
+    dom, W0 = grid_state(grid, seed=13)
+    s = o.Setup(gamma=1.4, bcs=VISCOUS_BCS[bcs])
+    rng = np.random.default_rng(3)
+    W = np.ascontiguousarray(1.0 + rng.random((5,) + dom.shape))
+    U = np.ascontiguousarray(rng.random((5,) + dom.shape))
+    Wr, Ur = o.edge_halo_update(W, U, dom, s)
+    from jaxfluids_amd import _native
+
+    grid_ = _native.make_grid(dom.cells, dom.nh, dom.dx)
+    bcs_ = _native.bc_array({f: (s.bcs[f] if o.FACE_AXIS[f] in dom.active else "PERIODIC") for f in FACES})
+    emu.emu_clear_ranges()
+    for a in (W, U):
+        emu.emu_add_range(a.ctypes.data, a.nbytes)
+    assert emu.jfa_edge_halo_update(W.ctypes.data, U.ctypes.data, grid_, bcs_, 1.4, 0, None) == 0
+    assert emu.emu_violations() == 0, hex(emu.emu_first_violation())
+    assert np.array_equal(W, Wr)
+    assert np.array_equal(U[:4], Ur[:4]) and rel(U, Ur) <= 4e-16, rel(U, Ur)
 
 
 # HAS's blend weight theta = (eps - q_LF) / (q - q_LF) is ill-conditioned where the first-order

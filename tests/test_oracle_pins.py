@@ -1474,3 +1474,129 @@ def test_weno7_weights_are_inverse_squares_of_the_indicators():
     got = o.weno7_js_point(*[np.float64(v) for v in u])
     assert all(b > 0 for b in beta)
     assert abs(float(got) - float(exact)) < 1e-14 * abs(float(exact))
+
+
+# ------------------------------------------------------------------ edge halos, dissipative fluxes on non-periodic domains
+def _indexed_state(cells, nh=3):
+    """W interior = 1000 + 100 f + 10 i + j (+ k / 10), so every edge value names its sources."""
+    dom = o.Domain(cells, tuple((0.0, 1.0) for _ in cells), nh=nh)
+    i, j, k = np.meshgrid(*[np.arange(c) for c in cells], indexing="ij")
+    W0 = np.stack([1000.0 + 100 * f + 10 * i + j + 0.1 * k for f in range(5)])
+    return dom, W0
+
+
+def test_edge_halos_periodic_are_the_periodic_images():
+    dom, W0 = _indexed_state((6, 5, 1))
+    s = o.Setup(bcs={f: "PERIODIC" for f in o.FACE_LOCATIONS})
+    W, U = o.halo_update(*o.create_buffers(W0, dom, s), dom, s, fill_edge=True)
+    wrapped = o.wrap_halos(W, dom)
+    nh = dom.nh
+    for sx, sy in ((slice(0, nh), slice(0, nh)), (slice(-nh, None), slice(0, nh)), (slice(0, nh), slice(-nh, None)),
+                   (slice(-nh, None), slice(-nh, None))):
+        assert np.array_equal(W[:, sx, sy], wrapped[:, sx, sy])
+
+
+def test_edge_halos_zero_gradient_mean_of_the_two_face_blocks():
+    """ANY_ANY edges (ZEROGRADIENT on both faces): buffer cell (p, q) of the edge block takes the
+    mean of the face halos nh cells further in along x and along y, unmirrored: at west_south
+    0.5 (W(x=p, y=0) + W(x=0, y=q)), at east_north 0.5 (W(x=N-nh+p, y=M-1) + W(x=N-1, y=M-nh+q))
+    in interior indices (p, q = 0..nh-1 within the block)."""
+    N, M, nh = 7, 6, 3
+    dom, W0 = _indexed_state((N, M, 1), nh)
+    s = o.Setup(bcs={f: "ZEROGRADIENT" for f in o.FACE_LOCATIONS})
+    W, U = o.create_buffers(W0, dom, s)
+    W, U = o.halo_update(W, U, dom, s, fill_edge=True)
+    for p in range(nh):
+        for q in range(nh):
+            ws = W[:, p, q, 0]
+            assert np.allclose(ws, 0.5 * (W0[:, p, 0, 0] + W0[:, 0, q, 0]), rtol=1e-15)
+            en = W[:, nh + N + p, nh + M + q, 0]
+            assert np.allclose(en, 0.5 * (W0[:, N - nh + p, M - 1, 0] + W0[:, N - 1, M - nh + q, 0]), rtol=1e-15)
+    assert np.allclose(U[:, 0, 0, 0], o.cons_from_prim(W[:, 0:1, 0:1, 0:1], 1.4)[:, 0, 0, 0], rtol=1e-15)
+
+
+def test_edge_halos_symmetry_mirror_the_face_halo():
+    """SYMMETRY on the first face (west_south: SYMMETRY_ANY): the south face halo mirrored across
+    x = 0 with u negated; on the second face only (east_south with east ZEROGRADIENT, south
+    SYMMETRY: ANY_SYMMETRY) the east face halo mirrored across y = 0 with v negated."""
+    N, M, nh = 6, 7, 3
+    dom, W0 = _indexed_state((N, M, 1), nh)
+    s = o.Setup(bcs={"west": "SYMMETRY", "east": "ZEROGRADIENT", "south": "SYMMETRY", "north": "ZEROGRADIENT",
+                     "top": "PERIODIC", "bottom": "PERIODIC"})
+    W, U = o.halo_update(*o.create_buffers(W0, dom, s), dom, s, fill_edge=True)
+    sign_u = np.array([1, -1, 1, 1, 1.0])
+    sign_v = np.array([1, 1, -1, 1, 1.0])
+    for p in range(nh):
+        for q in range(nh):
+            # west_south: south halo (mirror of y-row nh-1-q, v negated) at x = nh-1-p, u negated
+            src = W0[:, nh - 1 - p, nh - 1 - q, 0] * sign_v
+            assert np.array_equal(W[:, p, q, 0], src * sign_u)
+            # east_south: east halo (copy of x = N-1) at y = nh-1-q mirrored, v negated
+            src = W0[:, N - 1, nh - 1 - q, 0] * sign_v
+            assert np.array_equal(W[:, nh + N + p, q, 0], src)
+    assert o.edge_type(s, "west_south") == "SYMMETRY_ANY" and o.edge_type(s, "east_south") == "ANY_SYMMETRY"
+
+
+def test_edge_halos_3d_only_on_active_edges():
+    """3-D: an edge along z (west_south) spans the interior z cells; a 2-D x-z domain has only
+    x-z edges; vertex halos stay as they were."""
+    dom, W0 = _indexed_state((5, 4, 6))
+    s = o.Setup(bcs={f: "ZEROGRADIENT" for f in o.FACE_LOCATIONS})
+    W, U = o.create_buffers(W0, dom, s)
+    W2, _ = o.halo_update(W, U, dom, s, fill_edge=True)
+    nh = dom.nh
+    assert np.allclose(W2[:, 1, 2, nh + 3], 0.5 * (W0[:, 1, 0, 3] + W0[:, 0, 2, 3]), rtol=1e-15)
+    assert np.array_equal(W2[:, :nh, :nh, :nh], W[:, :nh, :nh, :nh])   # a vertex block: untouched
+
+
+def _viscous_setup(bcs):
+    tr = o.Transport(viscous=True, R=1.0, mu_ref=0.01, T_ref=1.0, mu_exp=0.7, prandtl=0.72)
+    return o.Setup(stencil="WENO5-JS", variable="PRIMITIVE", bcs=bcs, transport=tr)
+
+
+def test_viscous_flux_with_zero_gradient_walls_is_translation_invariant():
+    """A state varying along x only, periodic in x, ZEROGRADIENT in y: the x-face dissipative flux
+    equals that of the fully periodic domain, bit for bit, and is the same on every row, wall rows
+    included (their transverse neighbours and edge halos are the zero-gradient copies)."""
+    n, m = 16, 5
+    x = (np.arange(n) + 0.5) / n
+    prof = np.stack([1 + 0.2 * np.sin(2 * np.pi * x), 0.3 * np.cos(2 * np.pi * x), 0.1 * np.sin(4 * np.pi * x),
+                     0 * x, 1 + 0.1 * np.cos(2 * np.pi * x)])
+    bcs2 = {"east": "PERIODIC", "west": "PERIODIC", "north": "ZEROGRADIENT", "south": "ZEROGRADIENT",
+            "top": "PERIODIC", "bottom": "PERIODIC"}
+    s2 = _viscous_setup(bcs2)
+    dom2 = o.Domain((n, m, 1), ((0, 1), (0, 0.3), (0, 1)), nh=3)
+    W2, _ = o.create_buffers(np.repeat(prof[:, :, None, None], m, axis=2), dom2, s2)
+    s1 = _viscous_setup({f: "PERIODIC" for f in o.FACE_LOCATIONS})
+    W1, _ = o.create_buffers(np.repeat(prof[:, :, None, None], m, axis=2), dom2, s1)
+    F2 = o.dissipative_flux_xi(W2, dom2, 0, s2)
+    F1 = o.dissipative_flux_xi(W1, dom2, 0, s1)              # the same rows on a periodic domain
+    assert np.array_equal(F2, F1)
+    assert np.allclose(F2, F2[:, :, :1], rtol=0, atol=0)     # and every row is the same
+    Fy = o.dissipative_flux_xi(W2, dom2, 1, s2)
+    assert np.all(Fy[[0, 3]] == 0.0) and np.all(np.isfinite(Fy))
+
+
+def test_viscous_flux_of_a_linear_shear_at_zero_gradient_walls():
+    """u = a y (rho, p uniform; constant viscosity), periodic in x, ZEROGRADIENT in y: at the
+    y faces tau_xy = mu a inside and 0 at the two wall faces (the halo copies the wall cell); at
+    the x faces du/dy is the centred derivative of the cells, a inside and a/2 on the first and
+    last rows (one neighbour is a copy)."""
+    n, m, a = 6, 8, 0.7
+    dom = o.Domain((n, m, 1), ((0, 1), (0, 1), (0, 1)), nh=3)
+    y = (np.arange(m) + 0.5) / m
+    W0 = np.zeros((5, n, m, 1))
+    W0[0], W0[4] = 1.0, 1.0
+    W0[1] = a * y[None, :, None]
+    tr = o.Transport(viscous=True, R=1.0, mu_ref=0.02, T_ref=1.0, mu_exp=0.0)
+    s = o.Setup(stencil="WENO5-JS", variable="PRIMITIVE", transport=tr,
+                bcs={"east": "PERIODIC", "west": "PERIODIC", "north": "ZEROGRADIENT", "south": "ZEROGRADIENT",
+                     "top": "PERIODIC", "bottom": "PERIODIC"})
+    W, _ = o.create_buffers(W0, dom, s)
+    Fy = o.dissipative_flux_xi(W, dom, 1, s)               # faces along y: -tau_yx in momentum x
+    tau = -Fy[1, 0, :, 0]
+    assert np.allclose(tau[1:-1], 0.02 * a, rtol=1e-13) and tau[0] == 0.0 and tau[-1] == 0.0
+    Fx = o.dissipative_flux_xi(W, dom, 0, s)               # faces along x: -tau_xy in momentum y
+    tau_x = -Fx[2, 0, :, 0]
+    assert np.allclose(tau_x[1:-1], 0.02 * a, rtol=1e-13)
+    assert np.allclose(tau_x[[0, -1]], 0.02 * a / 2, rtol=1e-13)

@@ -99,9 +99,27 @@ def test_input_manager_builds_native_transport():
     assert InputManager(_case(), num).material_manager.transport is None
 
 
-def test_input_manager_rejects_non_periodic_dissipative_runs():
-    with pytest.raises(NotImplementedError):
-        InputManager(_case(bc="ZEROGRADIENT"), NUM)
+def test_input_manager_dissipative_halos_flag():
+    """Periodic domains wrap the dissipative stencils; any other active face makes the unfused RHS
+    read the halos (the transport's halos flag), and the halo manager fills the edge halos for
+    every dissipative run (halo_manager.py:125-129)."""
+    from jaxfluids_amd.halos import HaloManager
+
+    assert InputManager(_case(), NUM).material_manager.transport.halos == 0
+    for bc in ("ZEROGRADIENT", "SYMMETRY"):
+        im = InputManager(_case(bc=bc), NUM)
+        assert im.material_manager.transport.halos == 1
+        hm = HaloManager(im.case_setup.boundary_conditions, im.domain_information, im.equation_manager,
+                         im.material_manager)
+        assert hm.fill_edge_halos_material
+    num = copy.deepcopy(NUM)
+    num["active_physics"] = {"is_convective_flux": True}
+    im = InputManager(_case(bc="ZEROGRADIENT"), num)
+    assert not HaloManager(im.case_setup.boundary_conditions, im.domain_information, im.equation_manager,
+                           im.material_manager).fill_edge_halos_material
+
+
+def test_input_manager_requires_transport_for_dissipative_runs():
     case = _case()
     del case["material_properties"]["transport"]
     with pytest.raises(AssertionError):
