@@ -18,8 +18,9 @@ ensemble: the batched-ensemble workload (64 uneven 2-D Riemann problems, one lau
 stage for all cases) against plain per-case launches of the same stages.
 adjoint: the Kelvin-Helmholtz 256^2 adjoint (500 fixed-dt steps): backward / forward wall time
 and a finite-difference check of the gradient.
-adjoint_viscous: the viscous Taylor-Green 128^3 adjoint (Re 1600, 1000 fixed-dt steps, binomial
-checkpointing in 20 slots, enstrophy loss): the same measurements.
+adjoint_viscous: the viscous Taylor-Green 128^3 adjoint (Re 1600, 1000 fixed-dt steps, a checkpoint
+every 50 steps, enstrophy loss): the same measurements, with binomial checkpointing in 20 slots
+beside it.
 """
 from __future__ import annotations
 
@@ -33,6 +34,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+from jaxfluids_amd.workloads import tgv_primitives  # noqa: E402
 
 METRIC = "Mcell-updates/s per RK3 stage at 256^3 FP64; %HBM peak; adjoint/fwd ratio"
 HBM_PEAK_GBS = 8000.0
@@ -62,23 +65,6 @@ def tgv_setups(n: int, nh: int, dtype: str, variable: str):
         "precision": {"is_double_precision_compute": dtype == "f64"},
     }
     return case, num
-
-
-def tgv_primitives(n: int, gamma=1.4, mach=0.1):
-    """Inviscid TGV on [0,2pi)^3: rho=1, u=sin x cos y cos z, v=-cos x sin y cos z, w=0,
-    p = 1/(gamma Ma^2) + (1/16)(cos2x+cos2y)(cos2z+2)."""
-    L = 2 * np.pi
-    d = L / n
-    x = np.linspace(d / 2, L - d / 2, n)
-    sx, cx = np.sin(x), np.cos(x)
-    c2 = np.cos(2 * x)
-    W = np.empty((5, n, n, n))
-    W[0] = 1.0
-    W[1] = sx[:, None, None] * cx[None, :, None] * cx[None, None, :]
-    W[2] = -cx[:, None, None] * sx[None, :, None] * cx[None, None, :]
-    W[3] = 0.0
-    W[4] = 1.0 / gamma / mach ** 2 + 1.0 / 16.0 * ((c2[:, None, None] + c2[None, :, None]) * (c2[None, None, :] + 2))
-    return W
 
 
 def measured_stage_figure(n: int, dtype: str, variable: str, nh: int, key: str = "stage_hbm_bytes"):
@@ -322,7 +308,8 @@ def bench_adjoint(nsteps: int = 500, n: int = 256, slots: int = 20, fd_check: bo
         torch.cuda.synchronize()
         return time.perf_counter() - t0, out
 
-    # warm up kernels and caches
+    # warm up kernels and caches; the binomial schedule table is computed once per (steps, slots)
+    binomial_table(nsteps, slots)
     with torch.no_grad():
         rollout(W0d, st, 5, 0)
     x = W0d.clone().requires_grad_(True)
@@ -333,22 +320,31 @@ def bench_adjoint(nsteps: int = 500, n: int = 256, slots: int = 20, fd_check: bo
     L0 = float(kinetic_energy(WT, dV))
     res = {}
     grads = {}
-    for name, sl in (("store_every_step", nsteps), ("binomial", slots)):
+    for name, sl, schedule in (("store_every_step", nsteps, "binomial"), ("binomial", slots, "binomial"),
+                               ("segments", slots, "segments")):
         x = W0d.clone().requires_grad_(True)
-        f0 = st.forward_steps
-        t_f, out = timed(lambda: kinetic_energy(rollout(x, st, nsteps, sl), dV))
+        t_f, out = timed(lambda: kinetic_energy(rollout(x, st, nsteps, sl, schedule), dV))
         f1 = st.forward_steps
         t_b, _ = timed(lambda: out.backward())
-        T, _ = binomial_table(nsteps, sl)
         res[name] = {"snapshot_slots": sl, "forward_s": round(t_f, 4), "backward_s": round(t_b, 4),
                      "backward_over_forward": round(t_b / t_fwd, 3),
-                     "recomputed_steps": int(st.forward_steps - f1), "schedule_steps": int(T[nsteps, sl])}
+                     "taped_forward_over_forward": round(t_f / t_fwd, 3),
+                     "gradient_over_forward": round((t_f + t_b) / t_fwd, 3),
+                     "recomputed_steps": int(st.forward_steps - f1)}
+        if name == "binomial":
+            T, _ = binomial_table(nsteps, sl)
+            res[name]["schedule_steps"] = int(T[nsteps, sl])
+        if name == "segments":
+            res[name]["segment_steps"] = -(-nsteps // sl)
         grads[name] = x.grad.detach()
-    same = bool(torch.equal(grads["store_every_step"], grads["binomial"]))
+    same = bool(torch.equal(grads["store_every_step"], grads["binomial"])
+                and torch.equal(grads["store_every_step"], grads["segments"]))
     out = {"workload": f"adjoint of 2-D Kelvin-Helmholtz {n}^2 periodic, WENO5-JS PRIMITIVE + HLLC-Einfeldt + RK3, "
                        f"fixed dt {dt:.6e}, {nsteps} steps, FP64, loss = kinetic energy at t_end",
            "forward_s": round(t_fwd, 4), "ms_per_forward_step": round(t_fwd / nsteps * 1e3, 4),
            "adjoint_over_forward": res["store_every_step"]["backward_over_forward"],
+           "adjoint_schedule": "store every step",
+           "gradient_over_forward": res["store_every_step"]["gradient_over_forward"],
            "modes": res, "checkpointed_gradient_identical": same, "loss": L0}
     if fd_check:
         out["fd_check"] = fd_check_along(lambda x: kinetic_energy(rollout(x, st, nsteps, 0), dV), W0d,
@@ -383,10 +379,11 @@ def bench_adjoint_viscous(nsteps: int = 1000, n: int = 128, slots: int = 20, sto
     """Adjoint of the 3-D viscous Taylor-Green vortex: 128^3 periodic [0, 2pi)^3, Re 1600, Mach 0.1,
     Prandtl 0.71, power-law viscosity mu = mu_ref (T / T0)^0.76, WENO5-Z PRIMITIVE + HLLC-Einfeldt
     + RK3 with second-order central viscous and heat fluxes, fixed dt (CFL 0.5 of the initial
-    state), 1000 steps with binomial checkpointing in 20 snapshot slots (one per 50 steps), FP64;
-    loss = enstrophy at t_end; gradient w.r.t. the initial velocity field.  The adjoint/forward
-    ratio with every step stored is measured over the first `store_steps` steps (a full store of
-    1000 steps would hold 190 GB)."""
+    state), 1000 steps, FP64; loss = enstrophy at t_end; gradient w.r.t. the initial velocity
+    field.  Schedules: a checkpoint every 50 steps (20 checkpoints), each 50-step segment
+    recomputed once with a slot per step (the headline adjoint_over_forward); binomial
+    checkpointing in 20 snapshot slots; and every step stored, measured over the first
+    `store_steps` steps (a full store of 1000 steps would hold 190 GB)."""
     import torch
 
     from jaxfluids_amd import _native
@@ -417,6 +414,7 @@ def bench_adjoint_viscous(nsteps: int = 1000, n: int = 128, slots: int = 20, sto
         torch.cuda.synchronize()
         return time.perf_counter() - t0, r
 
+    binomial_table(nsteps, slots)  # computed once per (steps, slots)
     with torch.no_grad():
         rollout(W0d, st, 3, 0)
     x = W0d.clone().requires_grad_(True)
@@ -429,28 +427,46 @@ def bench_adjoint_viscous(nsteps: int = 1000, n: int = 128, slots: int = 20, sto
     t_f, lo = timed(lambda: enstrophy(rollout(x, st, store_steps, store_steps), dx))
     t_b, _ = timed(lambda: lo.backward())
     res["store_every_step"] = {"steps": store_steps, "forward_s": round(t_fwd_s, 4), "backward_s": round(t_b, 4),
-                               "backward_over_forward": round(t_b / t_fwd_s, 3)}
+                               "backward_over_forward": round(t_b / t_fwd_s, 3),
+                               "taped_forward_over_forward": round(t_f / t_fwd_s, 3),
+                               "gradient_over_forward": round((t_f + t_b) / t_fwd_s, 3)}
     del x, lo
     st.release_buffers()
     torch.cuda.empty_cache()
     with torch.no_grad():
         t_fwd, WT = timed(lambda: rollout(W0d, st, nsteps, 0))
     L0 = float(enstrophy(WT, dx))
-    x = W0d.clone().requires_grad_(True)
-    t_f, lo = timed(lambda: enstrophy(rollout(x, st, nsteps, slots), dx))
-    f1 = st.forward_steps
-    t_b, _ = timed(lambda: lo.backward())
-    T, _ = binomial_table(nsteps, slots)
-    res["binomial"] = {"steps": nsteps, "snapshot_slots": slots, "forward_s": round(t_f, 4),
-                       "backward_s": round(t_b, 4), "backward_over_forward": round(t_b / t_fwd, 3),
-                       "recomputed_steps": int(st.forward_steps - f1), "schedule_steps": int(T[nsteps, slots])}
-    g = x.grad.detach()
+    grads = {}
+    for name, schedule in (("segments", "segments"), ("binomial", "binomial")):
+        x = W0d.clone().requires_grad_(True)
+        t_f, lo = timed(lambda: enstrophy(rollout(x, st, nsteps, slots, schedule), dx))
+        f1 = st.forward_steps
+        t_b, _ = timed(lambda: lo.backward())
+        res[name] = {"steps": nsteps, "snapshot_slots": slots, "forward_s": round(t_f, 4),
+                     "backward_s": round(t_b, 4), "backward_over_forward": round(t_b / t_fwd, 3),
+                     "taped_forward_over_forward": round(t_f / t_fwd, 3),
+                     "gradient_over_forward": round((t_f + t_b) / t_fwd, 3),
+                     "recomputed_steps": int(st.forward_steps - f1)}
+        if name == "segments":
+            res[name]["segment_steps"] = -(-nsteps // slots)
+        else:
+            T, _ = binomial_table(nsteps, slots)
+            res[name]["schedule_steps"] = int(T[nsteps, slots])
+        grads[name] = x.grad.detach()
+        del x, lo
+        st.release_buffers()
+        torch.cuda.empty_cache()
+    g = grads["segments"]
     out = {"workload": f"adjoint of 3-D viscous Taylor-Green {n}^3 periodic, Re 1600, Mach 0.1, Pr 0.71, power-law "
                        f"viscosity (0.76), WENO5-Z PRIMITIVE + HLLC-Einfeldt + RK3 + CENTRAL2 viscous/heat fluxes, "
                        f"fixed dt {dt:.6e}, {nsteps} steps, FP64, loss = enstrophy at t_end",
            "forward_s": round(t_fwd, 4), "ms_per_forward_step": round(t_fwd / nsteps * 1e3, 4),
            "forward_mcell_updates_per_s": round(n ** 3 * 3 * nsteps / t_fwd / 1e6, 1),
-           "adjoint_over_forward": res["store_every_step"]["backward_over_forward"],
+           "adjoint_over_forward": res["segments"]["backward_over_forward"],
+           "adjoint_schedule": f"checkpoint every {res['segments']['segment_steps']} steps ({slots} checkpoints), "
+                               "each segment recomputed once with a slot per step",
+           "gradient_over_forward": res["segments"]["gradient_over_forward"],
+           "checkpointed_gradient_identical": bool(torch.equal(grads["segments"], grads["binomial"])),
            "modes": res, "loss": L0, "grad_finite": bool(torch.isfinite(g).all().item())}
     if fd_check:
         out["fd_check"] = fd_check_along(lambda y: enstrophy(rollout(y, st, nsteps, 0), dx), W0d, g,

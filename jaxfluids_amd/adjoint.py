@@ -56,6 +56,10 @@ def reverse_rollout(snaps: dict, nsteps: int, snapshots: int, lam, advance, step
     state no longer needed.  At most `snapshots` states beyond snaps[0] are alive at once, and
     the forward steps taken are T[nsteps, snapshots] of binomial_table (fewer where snaps already
     holds first-level snapshots)."""
+    if all(k in snaps for k in range(nsteps)):  # every step stored: no schedule to compute
+        for k in range(nsteps - 1, -1, -1):
+            lam = step_vjp(snaps[k], lam)
+        return lam
     T, M = binomial_table(nsteps, snapshots)
     stack = [("seg", 0, nsteps, min(snapshots, nsteps))]
     while stack:
@@ -307,16 +311,40 @@ class FixedDtStepper:
                                                               state[2:] if len(state) == 4 else None),
                                store, None if pool is None else pool.release)
 
+    def reverse_segments(self, snaps: Dict[int, Tuple[torch.Tensor, torch.Tensor]], nsteps: int, segment: int,
+                         lam: torch.Tensor, pool: Optional[StatePool] = None) -> torch.Tensor:
+        """Reverse nsteps steps checkpointed every `segment` steps (snaps[0], snaps[segment], ...):
+        each segment, last first, is recomputed once from its checkpoint with every step's state
+        and stage primitives kept, then reversed with the three stage VJPs per step -- the
+        schedule of a rematerialised segment (jax.checkpoint around `segment` steps)."""
+        S = self.buffers("segment_states", 2 * (segment + 1))
+        K = self.buffers("segment_stages", 2 * segment)
+        for start in range(((nsteps - 1) // segment) * segment, -1, -segment):
+            m = min(segment, nsteps - start)
+            state = snaps.pop(start)
+            S[0].copy_(state[0])
+            S[1].copy_(state[1])
+            if pool is not None:
+                pool.release(state)
+            for n in range(m):
+                self.step_into(S[2 * n], S[2 * n + 1], K[2 * n], K[2 * n + 1], S[2 * n + 2], S[2 * n + 3])
+            for n in range(m - 1, -1, -1):
+                lam = self.step_vjp(S[2 * n], S[2 * n + 1], lam, (K[2 * n], K[2 * n + 1]))
+        return lam
+
 
 class RK3Rollout(torch.autograd.Function):
     """W_T = rollout(W_0): nsteps fixed-dt RK3 steps from interior primitives W_0.
 
-    Forward stores snaps[0] and the first-level binomial snapshots; backward reverses the
-    rollout with ``stepper.reverse`` and maps dL/dU_0 to dL/dW_0."""
+    Forward stores snaps[0] and the first-level binomial snapshots (or, with segments, a
+    checkpoint every ceil(nsteps / snapshots) steps); backward reverses the rollout with
+    ``stepper.reverse`` (``reverse_segments``) and maps dL/dU_0 to dL/dW_0."""
 
     @staticmethod
-    def forward(ctx, W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: int) -> torch.Tensor:
+    def forward(ctx, W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: int,
+                segments: bool = False) -> torch.Tensor:
         W, U = stepper.fill(W0.detach())
+        ctx.segment = 0
         # with a snapshot at every step (snapshots >= nsteps - 1) the stage primitives are kept too,
         # so the reverse sweep evaluates only the three stage VJPs per step; the steps write straight
         # into the stepper's rollout buffers
@@ -334,6 +362,15 @@ class RK3Rollout(torch.autograd.Function):
                 snaps[n] = (Wi, Ui, K[2 * n], K[2 * n + 1])
             W = S[2 * nsteps]
             stepper.tape += 1
+        elif segments:
+            seg = -(-nsteps // max(1, snapshots))
+            pool = StatePool(-(-nsteps // seg), stepper.shape, stepper.device, precision.get_eps())
+            snaps = {}
+            for n in range(nsteps):
+                if n % seg == 0:
+                    snaps[n] = pool.store((W, U))
+                stepper.step(W, U)
+            ctx.segment = seg
         else:
             pool = StatePool(min(snapshots, nsteps) + 1, stepper.shape, stepper.device, precision.get_eps())
             snaps = {0: pool.store((W, U))}
@@ -357,16 +394,29 @@ class RK3Rollout(torch.autograd.Function):
                                "before this rollout's backward; use one stepper per pending rollout")
         lam = torch.zeros(st.shape, dtype=torch.float64, device=st.device)
         lam[st.interior] = prim_vjp(ctx.WT, gWT.to(torch.float64), st.gamma)
-        lam = st.reverse(ctx.snaps, ctx.nsteps, ctx.snapshots, lam, ctx.pool)
+        if ctx.segment:
+            lam = st.reverse_segments(ctx.snaps, ctx.nsteps, ctx.segment, lam, ctx.pool)
+        else:
+            lam = st.reverse(ctx.snaps, ctx.nsteps, ctx.snapshots, lam, ctx.pool)
         ctx.snaps = None
         ctx.pool = None
         gW0 = cons_vjp(ctx.W0.to(torch.float64), lam[st.interior], st.gamma)
-        return gW0.to(ctx.W0.dtype), None, None, None
+        return gW0.to(ctx.W0.dtype), None, None, None, None
 
 
-def rollout(W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: Optional[int] = None) -> torch.Tensor:
-    """Differentiable fixed-dt rollout; snapshots defaults to storing every step's state."""
-    return RK3Rollout.apply(W0, stepper, int(nsteps), int(nsteps if snapshots is None else snapshots))
+def rollout(W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: Optional[int] = None,
+            schedule: str = "binomial") -> torch.Tensor:
+    """Differentiable fixed-dt rollout; snapshots defaults to storing every step's state.
+
+    schedule "binomial": at most `snapshots` stored states beyond the initial one, the fewest
+    recomputed steps for that memory (the backward recomputes the stage primitives per step);
+    "segments": a checkpoint every ceil(nsteps / snapshots) steps, each segment recomputed once
+    with its steps' states and stage primitives held (nsteps recomputed steps, one segment of
+    full storage on top of the checkpoints)."""
+    if schedule not in ("binomial", "segments"):
+        raise ValueError(f"unknown checkpoint schedule {schedule!r}")
+    snapshots = int(nsteps if snapshots is None else snapshots)
+    return RK3Rollout.apply(W0, stepper, int(nsteps), snapshots, schedule == "segments" and snapshots < nsteps - 1)
 
 
 def kinetic_energy(W: torch.Tensor, cell_volume: float) -> torch.Tensor:

@@ -199,6 +199,28 @@ def test_emulated_batch_step_matches_oracle(emu, scheme, nh):
         assert abs(ens.dt[i] - dtr) <= 1e-14 * dtr, (i, ens.dt[i], dtr)
 
 
+@pytest.mark.parametrize("fs", ["ROE", "CLLF"])
+def test_emulated_batch_flux_splitting_fp32_storage(emu, fs):
+    """The FP32-storage batched stage with flux splitting (float pools, FP64 arithmetic) against
+    the FP64 oracle after one step: the float32 rounding of the state, < 1e-6 relative L-inf."""
+    scheme = ("WENO5-Z", "PRIMITIVE", "FLUX-SPLITTING-" + fs, "EINFELDT", "ROE")
+    prims = [case_state(c[0], c[1], seed=800 + i) for i, c in enumerate(CASES)]
+    ens = HostEnsemble(emu, scheme, nh=3, chunk=16, dtype=np.float32)
+    refs = [oracle_case(i, scheme, p) for i, p in enumerate(prims)]
+    for i, (dom, s, W, U) in enumerate(refs):
+        ens.view(0, i)[...] = W
+        ens.view(1, i)[...] = U
+        ens.dt[i] = o.compute_dt(W, dom, s)
+    dt0 = ens.dt.copy()
+    ens.step()
+    assert emu.emu_violations() == 0, hex(emu.emu_first_violation())
+    for i, (dom, s, W, U) in enumerate(refs):
+        Wr, _ = o.rk3_step(W, U, dt0[i], dom, s)
+        err = rel(ens.view(0, i)[dom.interior].astype(np.float64), Wr[dom.interior])
+        assert 1e-9 < err < 1e-6, (i, err)
+        assert abs(ens.dt[i] - o.compute_dt(Wr, dom, s)) <= 1e-6 * dt0[i], i
+
+
 def test_emulated_batch_chunks_agree(emu):
     """Different x-chunk lengths give the same state bit for bit."""
     scheme = SCHEMES[1]

@@ -53,20 +53,22 @@ def emu():
     return lib
 
 
-def fused_step(emu, dom, s, W, U, dt, transport=None):
+def fused_step(emu, dom, s, W, U, dt, transport=None, dtype=np.float64):
     """Three fused stages as SimulationManager.do_runge_kutta_stages runs them (the last one
-    writes into the step's input buffers and the next dt); returns (W, U, dt_next)."""
+    writes into the step's input buffers and the next dt); returns (W, U, dt_next).  dtype:
+    the storage of W and U (float32: the FP32-storage instantiation, FP64 arithmetic)."""
     from jaxfluids_amd import _native
 
     grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
     tr = None if transport is None else _native.make_transport(**dataclasses.asdict(transport))
-    scheme = _native.make_scheme(s.variable, s.stencil, s.riemann, s.signal_speed, s.frozen_state, 0, s.gamma,
+    code = _native.F64 if dtype == np.float64 else _native.F32
+    scheme = _native.make_scheme(s.variable, s.stencil, s.riemann, s.signal_speed, s.frozen_state, code, s.gamma,
                                  s.eps_stencil, tr, s.interpolation_limiter, s.limit_velocity, s.limiter_eps,
                                  s.flux_limiter, s.flux_partition, s.flux_limiter_eps)
     bcs = _native.bc_array({f: (s.bcs[f] if o.FACE_AXIS[f] in dom.active else "PERIODIC") for f in FACES})
     nbytes = emu.jfa_stage_workspace_bytes(grid, scheme)
     assert nbytes > 0
-    Wa, Ua = np.ascontiguousarray(W.copy()), np.ascontiguousarray(U.copy())
+    Wa, Ua = np.ascontiguousarray(W.astype(dtype)), np.ascontiguousarray(U.astype(dtype))
     Wb, Ub, Wc, Uc = (np.full_like(Wa, np.nan) for _ in range(4))
     ws = np.zeros(nbytes, dtype=np.uint8)
     dta = np.array([dt])
@@ -77,7 +79,8 @@ def fused_step(emu, dom, s, W, U, dt, transport=None):
     P = lambda a: None if a is None else a.ctypes.data  # noqa: E731
     for k, (wi, ui, wo, uo) in enumerate([(Wa, Ua, Wb, Ub), (Wb, Ub, Wc, Uc), (Wc, Uc, Wa, Ua)]):
         rc = emu.jfa_rk3_stage_fused(P(wi), P(ui), P(Ua if k else None), P(wo), P(uo), grid, scheme, bcs, k, P(dta),
-                                     0.0, s.cfl, o.EPS64, P(dtn if k == 2 else None), P(ws), nbytes, None)
+                                     0.0, s.cfl, float(np.finfo(dtype).eps), P(dtn if k == 2 else None), P(ws),
+                                     nbytes, None)
         assert rc == 0, rc
     assert emu.emu_violations() == 0, hex(emu.emu_first_violation())
     assert emu.emu_accesses() > 0
@@ -320,6 +323,26 @@ def test_fused_step_flux_splitting_with_flux_limiter(emu, fl):
     Wg, Ug, _ = fused_step(emu, dom, s, W, U, dt)
     I = dom.interior
     assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL
+
+
+@pytest.mark.parametrize("fs", ["ROE", "LLF"])
+@pytest.mark.parametrize("grid", ["3d", "2d-xz"])
+def test_fused_step_flux_splitting_fp32_storage(emu, fs, grid):
+    """The FP32-storage instantiation of the flux-splitting sweeps (float W and U, FP64
+    arithmetic) against the FP64 oracle: the error is the float32 rounding of the state, stated
+    here as < 1e-6 relative L-inf."""
+    dom, W0 = grid_state(grid, seed=17)
+    s = o.Setup(gamma=1.4, stencil="WENO5-Z", variable="PRIMITIVE", riemann="FLUX-SPLITTING-" + fs,
+                signal_speed="EINFELDT", frozen_state="ROE", bcs={f: "ZEROGRADIENT" for f in FACES}, cfl=0.5)
+    W, U = o.create_buffers(W0, dom, s)
+    dt = o.compute_dt(W, dom, s)
+    Wr, Ur = o.rk3_step(W, U, dt, dom, s)
+    Wg, Ug, dtn = fused_step(emu, dom, s, W, U, dt, dtype=np.float32)
+    I = dom.interior
+    err = rel(Wg[I].astype(np.float64), Wr[I])
+    assert 1e-9 < err < 1e-6, err  # float32 rounding, not FP64 agreement
+    assert rel(Ug[I].astype(np.float64), Ur[I]) < 1e-6
+    assert abs(dtn - o.compute_dt(Wr, dom, s)) <= 1e-6 * dt
 
 
 @pytest.mark.parametrize("grid", ["3d", "2d-xy", "1d"])

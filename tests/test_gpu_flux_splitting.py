@@ -152,3 +152,36 @@ def test_ensemble_step_matches_oracle(fs):
         for r in face_regions(dom):
             assert rel(Wg[r], W[r]) < TOL and rel(Ug[r], U[r]) < TOL, (i, rel(Wg[r], W[r]))
         assert abs(got_dt[i] - o.compute_dt(W, dom, s)) <= 1e-14 * dt, i
+
+
+@pytest.mark.parametrize("fs", ["ROE", "LLF"])
+def test_fp32_storage_error_stated(fs):
+    """The FP32-storage flux-splitting instantiations (float W and U, FP64 arithmetic): one fused
+    RK3 step of TGV 24^3 and one batched step of the 5 uneven cases, against the FP64 oracle.
+    The error is the float32 rounding of the state, stated here as < 1e-6 relative L-inf."""
+    from test_gpu_ensemble import CASES, case_state, make_ensemble, oracle_case
+
+    say(f"[fs fp32 storage] {fs}")
+    dom, W0 = o.tgv_initial(24, nh=3)
+    s = o.Setup(gamma=1.4, stencil="WENO5-Z", variable="PRIMITIVE", riemann="FLUX-SPLITTING-" + fs,
+                signal_speed="EINFELDT", frozen_state="ROE")
+    W, U = o.create_buffers(W0, dom, s)
+    dt = o.compute_dt(W, dom, s)
+    Wr, _ = o.rk3_step(W, U, dt, dom, s)
+    sim = _sim(dom, s)
+    Wd, Ud = dev(W, torch.float32), dev(U, torch.float32)
+    dtd = torch.full((1,), dt, dtype=torch.float64, device="cuda")
+    Wd, Ud = sim.do_runge_kutta_stages(Wd, Ud, dtd)
+    err = rel(host(Wd)[dom.interior], Wr[dom.interior])
+    say(f"    fused fp32-storage rel Linf vs FP64 oracle: {err:.3e}")
+    assert 1e-9 < err < 1e-6
+    scheme = ("WENO5-Z", "PRIMITIVE", "FLUX-SPLITTING-" + fs, "EINFELDT", "ROE")
+    prims = [case_state(c[0], c[1], seed=600 + i) for i, c in enumerate(CASES)]
+    ens = make_ensemble(scheme, dtype=torch.float32)
+    ens.initialize(prims)
+    ens.step(1)
+    for i, p in enumerate(prims):
+        dom_i, s_i, W_i, U_i = oracle_case(i, scheme, p)
+        W1, _ = o.rk3_step(W_i, U_i, o.compute_dt(W_i, dom_i, s_i), dom_i, s_i)
+        err = rel(host(ens.state(i)[0])[dom_i.interior], W1[dom_i.interior])
+        assert err < 1e-6, (i, err)
