@@ -1,0 +1,221 @@
+"""PyTorch (CPU, float64) restatement of one RK3 stage with PRIMITIVE reconstruction, for
+reverse-mode checks.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this module; only ``tests/``
+use it, as the checker of the hand-written stage VJP (jfa_rk3_stage_vjp): torch.autograd of
+this restatement stands in for jax.grad of the reference, which cannot be imported here.
+
+The stage map is  f(U_in) = c0 U_in + c1 U_n + dt m rhs(prim(U_in))  on the interior, with the
+face halos filled from the interior as the reference fills them (the same formulas, operands
+and association order as ``jxf_oracle``; each function cites the reference file:line).
+Covered: WENO5-JS / WENO5-Z on the primitives, HLLC with the ARITHMETIC / RUSANOV / DAVIS /
+EINFELDT signal speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, inviscid.
+
+Derivative conventions at non-smooth points follow torch's, which are jax's for the operations
+used: d|x|/dx = sign(x) (0 at 0), d sign(x)/dx = 0, and maximum / minimum give half of the
+cotangent to each operand at a tie (jax's balanced max); ``tests/test_torch_stage.py`` pins
+them.
+"""
+from __future__ import annotations
+
+import torch
+
+from .jxf_oracle import (FACE_AXIS, FACE_LOCATIONS, RK3_CONS_MULTIPLIER, RK3_TIMESTEP_MULTIPLIER,
+                         SPATIAL_STENCIL_EPS, VELOCITY_MINOR_AXES, WENO5_SHIFTS, Domain, _face_fill_slice,
+                         shift_slice)
+
+_DR = (1 / 10, 6 / 10, 3 / 10)   # weno5_base.py:24
+_CR = ((1 / 3, -7 / 6, 11 / 6),  # weno5_base.py:25-27
+       (-1 / 6, 5 / 6, 1 / 3),
+       (1 / 3, 5 / 6, -1 / 6))
+
+
+# ------------------------------------------------------------------ WENO5
+def weno5_smoothness(a, b, c, d, e):
+    """weno5_base.py:34-41."""
+    beta_0 = 13.0 / 12.0 * torch.square(a - 2 * b + c) + 1.0 / 4.0 * torch.square(a - 4 * b + 3 * c)
+    beta_1 = 13.0 / 12.0 * torch.square(b - 2 * c + d) + 1.0 / 4.0 * torch.square(b - d)
+    beta_2 = 13.0 / 12.0 * torch.square(c - 2 * d + e) + 1.0 / 4.0 * torch.square(3 * c - 4 * d + e)
+    return beta_0, beta_1, beta_2
+
+
+def weno5_polynomials(a, b, c, d, e):
+    """weno5_base.py:44-50."""
+    return (_CR[0][0] * a + _CR[0][1] * b + _CR[0][2] * c,
+            _CR[1][0] * b + _CR[1][1] * c + _CR[1][2] * d,
+            _CR[2][0] * c + _CR[2][1] * d + _CR[2][2] * e)
+
+
+def weno5_js(a, b, c, d, e, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno5_js.py:39-58."""
+    b0, b1, b2 = weno5_smoothness(a, b, c, d, e)
+    al0 = _DR[0] * (1.0 / (b0 * b0 + eps))
+    al1 = _DR[1] * (1.0 / (b1 * b1 + eps))
+    al2 = _DR[2] * (1.0 / (b2 * b2 + eps))
+    one_alpha = 1.0 / (al0 + al1 + al2)
+    p0, p1, p2 = weno5_polynomials(a, b, c, d, e)
+    return (al0 * one_alpha) * p0 + (al1 * one_alpha) * p1 + (al2 * one_alpha) * p2
+
+
+def weno5_z(a, b, c, d, e, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno5_z.py:38-59."""
+    b0, b1, b2 = weno5_smoothness(a, b, c, d, e)
+    tau_5 = torch.abs(b0 - b2)
+    al0 = _DR[0] * (1.0 + tau_5 / (b0 + eps))
+    al1 = _DR[1] * (1.0 + tau_5 / (b1 + eps))
+    al2 = _DR[2] * (1.0 + tau_5 / (b2 + eps))
+    one_alpha = 1.0 / (al0 + al1 + al2)
+    p0, p1, p2 = weno5_polynomials(a, b, c, d, e)
+    return (al0 * one_alpha) * p0 + (al1 * one_alpha) * p1 + (al2 * one_alpha) * p2
+
+
+WENO5 = {"WENO5-JS": weno5_js, "WENO5-Z": weno5_z}
+
+
+# ------------------------------------------------------------------ ideal gas
+def cons_from_prim(W, gamma):
+    """equation_manager.py:93-101, ideal_gas.py:85-88."""
+    rho = W[0]
+    e = W[4] / (rho * (gamma - 1.0))
+    E = rho * (0.5 * (torch.square(W[1]) + torch.square(W[2]) + torch.square(W[3])) + e)
+    return torch.stack([rho, rho * W[1], rho * W[2], rho * W[3], E])
+
+
+def prim_from_cons(U, gamma):
+    """equation_manager.py:164-171, ideal_gas.py:73-75."""
+    rho = U[0]
+    one_rho = 1.0 / rho
+    u, v, w = U[1] * one_rho, U[2] * one_rho, U[3] * one_rho
+    e = U[4] * one_rho - 0.5 * (torch.square(u) + torch.square(v) + torch.square(w))
+    return torch.stack([rho, u, v, w, (gamma - 1.0) * e * rho])
+
+
+def physical_flux_xi(W, U, axis):
+    """equation_manager.py:237-252."""
+    f = [U[axis + 1], U[axis + 1] * W[1], U[axis + 1] * W[2], U[axis + 1] * W[3], None]
+    f[axis + 1] = f[axis + 1] + W[4]
+    f[4] = W[axis + 1] * (U[4] + W[4])
+    return torch.stack(f)
+
+
+# ------------------------------------------------------------------ signal speeds, Riemann solvers
+def _speeds(kind, u_L, u_R, a_L, a_R, rho_L, rho_R):
+    """signal_speeds.py:10-133 (ARITHMETIC :10-29, RUSANOV :31-49, DAVIS :51-69, EINFELDT :109-133)."""
+    if kind == "ARITHMETIC":
+        u_mean, a_mean = 0.5 * (u_L + u_R), 0.5 * (a_L + a_R)
+        return torch.minimum(u_mean - a_mean, u_L - a_L), torch.maximum(u_mean + a_mean, u_R + a_R)
+    if kind == "RUSANOV":
+        S_plus = torch.maximum(torch.abs(u_L) + a_L, torch.abs(u_R) + a_R)
+        return -S_plus, S_plus
+    if kind == "DAVIS":
+        return torch.minimum(u_L - a_L, u_R - a_R), torch.maximum(u_L + a_L, u_R + a_R)
+    if kind == "EINFELDT":
+        sL, sR = torch.sqrt(rho_L), torch.sqrt(rho_R)
+        one_dens = 1.0 / (sL + sR)
+        eta2 = 0.5 * sL * sR * one_dens * one_dens
+        u_bar = (sL * u_L + sR * u_R) * one_dens
+        d_bar = torch.sqrt((sL * a_L * a_L + sR * a_R * a_R) * one_dens + eta2 * torch.square(u_R - u_L))
+        return torch.minimum(u_bar - d_bar, u_L - a_L), torch.maximum(u_bar + d_bar, u_R + a_R)
+    raise NotImplementedError(kind)
+
+
+def _flux_star(W, U, S_K, S_star, axis, left):
+    """HLLC._compute_flux_star_K_single_phase (HLLC.py:41-78)."""
+    v = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    pre = (S_K - W[v]) / (S_K - S_star) * W[0]
+    us = [pre, pre, pre, pre, pre * (U[4] / U[0] + (S_star - W[v]) * (S_star + W[4] / W[0] / (S_K - W[v])))]
+    us[v] = us[v] * S_star
+    us[m0] = us[m0] * W[m0]
+    us[m1] = us[m1] * W[m1]
+    S = torch.minimum(S_K, torch.zeros_like(S_K)) if left else torch.maximum(S_K, torch.zeros_like(S_K))
+    return physical_flux_xi(W, U, axis) + S * (torch.stack(us) - U)
+
+
+def hllc(W_L, W_R, U_L, U_R, axis, gamma, signal_speed):
+    """HLLC._solve_riemann_problem_xi_single_phase (HLLC.py:80-126), S* from signal_speeds.py:159-199."""
+    v = axis + 1
+    c_L = torch.sqrt(gamma * W_L[4] / W_L[0])
+    c_R = torch.sqrt(gamma * W_R[4] / W_R[0])
+    S_L, S_R = _speeds(signal_speed, W_L[v], W_R[v], c_L, c_R, W_L[0], W_R[0])
+    rdu_L = W_L[0] * (S_L - W_L[v])
+    rdu_R = W_R[0] * (S_R - W_R[v])
+    S_star = ((W_R[4] - W_L[4]) + (W_L[v] * rdu_L - W_R[v] * rdu_R)) / (rdu_L - rdu_R)
+    F_L = _flux_star(W_L, U_L, S_L, S_star, axis, True)
+    F_R = _flux_star(W_R, U_R, S_R, S_star, axis, False)
+    sgn = torch.sign(S_star)
+    return 0.5 * (1 + sgn) * F_L + 0.5 * (1 - sgn) * F_R
+
+
+def rusanov(W_L, W_R, U_L, U_R, axis, gamma, signal_speed=None):
+    """Rusanov._solve_riemann_problem_xi_single_phase (Rusanov.py:25-47)."""
+    v = axis + 1
+    c_L = torch.sqrt(gamma * W_L[4] / W_L[0])
+    c_R = torch.sqrt(gamma * W_R[4] / W_R[0])
+    alpha = torch.maximum(torch.abs(W_L[v]) + c_L, torch.abs(W_R[v]) + c_R)
+    return 0.5 * (physical_flux_xi(W_L, U_L, axis) + physical_flux_xi(W_R, U_R, axis)) - 0.5 * alpha * (U_R - U_L)
+
+
+RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov}
+
+
+# ------------------------------------------------------------------ RHS and halos
+def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float):
+    """SpaceSolver.compute_rhs (space_solver.py:266,289-314,489,597-599) with PRIMITIVE
+    reconstruction (high_order_godunov.py:267-280): 0.0 + sum over axes of (1/dx)(F[:-1] - F[1:])."""
+    rhs = 0.0
+    for axis in dom.active:
+        W_L = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[0]])
+        W_R = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[1]])
+        F = RIEMANN[riemann](W_L, W_R, cons_from_prim(W_L, gamma), cons_from_prim(W_R, gamma), axis, gamma,
+                             signal_speed)
+        n = F.shape[axis + 1]
+        rhs = rhs + (0.0 + dom.one_dx[axis] * (F.narrow(axis + 1, 0, n - 1) - F.narrow(axis + 1, 1, n - 1)))
+    return rhs
+
+
+def face_halos(W, dom: Domain, bcs: dict):
+    """face_halo_update (halos/outer/material.py:94-287; retrieve slices
+    boundary_condition.py:564-595, fill slices halo_slices.py:66-73) of a buffer whose interior is
+    set: PERIODIC copies the opposite interior layer, ZEROGRADIENT the first/last interior cell,
+    SYMMETRY the mirrored layer with the normal velocity negated (:721-731)."""
+    nh = dom.nh
+    for loc in FACE_LOCATIONS:
+        a = FACE_AXIS[loc]
+        if a not in dom.active:
+            continue
+        hi = loc in ("east", "north", "top")
+        bc = bcs[loc]
+        s = [dom.dom(0), dom.dom(1), dom.dom(2)]
+        if bc == "PERIODIC":
+            s[a] = slice(nh, 2 * nh) if hi else slice(-2 * nh, -nh)
+        elif bc == "ZEROGRADIENT":
+            s[a] = slice(-nh - 1, -nh) if hi else slice(nh, nh + 1)
+        elif bc == "SYMMETRY":  # the mirrored layer: the nh cells next to the face, reversed
+            s[a] = slice(-2 * nh, -nh) if hi else slice(nh, 2 * nh)
+        else:
+            raise NotImplementedError(bc)
+        halo = W[(Ellipsis,) + tuple(s)]
+        if bc == "SYMMETRY":
+            halo = torch.flip(halo, dims=(a + 1,))
+            sign = torch.ones(5, dtype=W.dtype).reshape(5, 1, 1, 1)
+            sign[a + 1] = -1.0
+            halo = halo * sign
+        W = W.clone()
+        W[_face_fill_slice(dom, loc)] = halo  # ZEROGRADIENT's single layer broadcasts over the nh halo cells
+    return W
+
+
+def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, bcs: dict,
+              stage: int, dt: float, eps: float):
+    """f(U_in interior) -> U_out interior: W = prim(U_in) on the interior of an eps-filled buffer,
+    face halos of W, rhs, then c0 U_in + c1 U_n + dt m rhs (RK3.py:27-62, time_integrator.py:57)."""
+    m = RK3_TIMESTEP_MULTIPLIER[stage]
+    c0, c1 = (1.0, 0.0) if stage == 0 else RK3_CONS_MULTIPLIER[stage - 1]
+    W = torch.full((5,) + dom.shape, eps, dtype=torch.float64)
+    W[dom.interior] = prim_from_cons(U_int, gamma)
+    W = face_halos(W, dom, bcs)
+    rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma)
+    if stage == 0:
+        return U_int + (dt * m) * rhs
+    return (c0 * U_int + c1 * Un_int) + (dt * m) * rhs

@@ -1,7 +1,8 @@
 """GPU checks of the reverse mode (jfa_rk3_stage_vjp, FixedDtStepper, RK3Rollout).
 
 * stage VJP on the device: dot-product identity against central differences of the oracle's
-  stage map (as the CPU emulation test does for the same source);
+  stage map (as the CPU emulation test does for the same source), and every entry against
+  torch.autograd of the PyTorch restatement (oracle/torch_stage.py);
 * rollout gradient (torch.autograd through RK3Rollout) against Richardson central differences
   of the oracle rollout (small case) and of the device forward (Kelvin-Helmholtz 64^2, 40 steps,
   the 1e-6 bar of the adjoint workload);
@@ -75,6 +76,49 @@ def test_stage_vjp_device_dot_product(case, stage):
     if stage:
         c1 = o.RK3_CONS_MULTIPLIER[stage - 1][1]
         assert np.allclose(host(ln)[dom.interior], c1 * lam_int, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("stage", [0, 1, 2])
+def test_stage_vjp_device_matches_autograd(case, stage):
+    """Every entry of the device J^T lam against torch.autograd of the PyTorch restatement of the
+    stage (oracle/torch_stage.py), the stand-in for jax.grad; 1e-11 of the largest entry (the
+    rounding of the two reverse evaluations, as in the host-emulated test)."""
+    from jaxfluids_amd import _native
+    from oracle import torch_stage as ts
+
+    say(f"[stage vjp vs autograd] case {case} stage {stage}")
+    cells, bcs, (stencil, riemann, speed) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann=riemann, signal_speed=speed,
+                bcs=dict(zip(FACES, bcs)))
+    rng = np.random.default_rng(5000 + 10 * case + stage)
+    W0 = smooth_state(dom, 70 + case)
+    Uint = o.cons_from_prim(W0, s.gamma)
+    Un_int = Uint * (1.0 + 0.01 * rng.standard_normal(Uint.shape))
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    _, W_in = stage_map(dom, s, stage, dt, Un_int)(Uint)
+    lam_int = rng.standard_normal(Uint.shape)
+    lam_out = np.zeros((5,) + dom.shape)
+    lam_out[dom.interior] = lam_int
+    grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
+    scheme = _native.make_scheme("PRIMITIVE", stencil, riemann, speed, "ARITHMETIC", _native.F64, 1.4,
+                                 s.eps_stencil)
+    lib = _native.load()
+    ws = torch.empty(lib.jfa_stage_vjp_workspace_bytes(grid, scheme), dtype=torch.uint8, device="cuda")
+    Wd, ld = dev(W_in), dev(lam_out)
+    li, ln = torch.zeros_like(ld), torch.zeros_like(ld)
+    P = _native.ptr
+    rc = lib.jfa_rk3_stage_vjp(P(Wd), P(ld), P(li), P(ln if stage else None), grid, scheme,
+                               _native.bc_array(s.bcs), stage, dt, P(ws), ws.numel(), _native.stream_ptr())
+    _native.check(rc, "jfa_rk3_stage_vjp")
+    U = torch.as_tensor(Uint).clone().requires_grad_(True)
+    out = ts.stage_map(U, torch.as_tensor(Un_int), dom, stencil, riemann, speed, s.gamma, s.bcs, stage, dt, o.EPS64)
+    (ref,) = torch.autograd.grad(out, U, torch.as_tensor(lam_int))
+    ref = ref.numpy()
+    got = host(li)[dom.interior]
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err <= 1e-11, err
 
 
 def _kh_stepper(n, dt):

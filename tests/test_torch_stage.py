@@ -1,0 +1,94 @@
+"""The stage VJP (jfa_rk3_stage_vjp, host build of the library sources) against reverse-mode
+automatic differentiation of an independent restatement: torch.autograd of oracle/torch_stage.py,
+the stand-in for jax.grad of the reference.  Unlike the finite-difference checks
+(tests/test_emu_adjoint.py), this compares every entry of J^T lam to rounding level.
+
+Pins of the restatement itself: its forward equals the NumPy oracle's stage map, and the
+derivative conventions at non-smooth points are jax's (d|x|/dx = 0 and d sign/dx = 0 at 0,
+maximum / minimum split the cotangent evenly at a tie: lax.max's balanced JVP).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import jxf_oracle as o
+from oracle import torch_stage as ts
+from test_emu_adjoint import CASES, FACES, emu, run_vjp, smooth_state, stage_map  # noqa: F401
+
+# Both sides compute in FP64 with different association (hand-derived reverse formulas against
+# autograd's chain of elementwise derivatives); the derivatives of the WENO weights amplify
+# rounding (terms in tau / beta^2), and the largest differences measured over these cases are
+# 1.2e-12 of the largest entry.  A wrong or missing term of the reverse sweep is orders larger.
+TOL = 1e-11
+
+
+def _setup(case):
+    cells, bcs, (stencil, riemann, speed) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann=riemann, signal_speed=speed,
+                bcs=dict(zip(FACES, bcs)))
+    return dom, s
+
+
+def _torch_map(dom, s, stage, dt, Un_int):
+    return lambda U: ts.stage_map(U, torch.as_tensor(Un_int), dom, s.stencil, s.riemann, s.signal_speed, s.gamma,
+                                  s.bcs, stage, dt, o.EPS64)
+
+
+def test_tie_conventions_are_jax_conventions():
+    """torch's derivatives at the kinks of abs, sign, maximum and minimum, which the
+    restatement relies on to stand in for jax.grad."""
+    x = torch.tensor([0.0, 2.0], dtype=torch.float64, requires_grad=True)
+    torch.abs(x).sum().backward()
+    assert x.grad.tolist() == [0.0, 1.0]
+    x = torch.tensor([0.0, -3.0], dtype=torch.float64, requires_grad=True)
+    torch.sign(x).sum().backward()
+    assert x.grad.tolist() == [0.0, 0.0]
+    for fn in (torch.maximum, torch.minimum):
+        a = torch.tensor([1.5, 1.0], dtype=torch.float64, requires_grad=True)
+        b = torch.tensor([1.5, 2.0], dtype=torch.float64, requires_grad=True)
+        fn(a, b).sum().backward()
+        assert a.grad[0].item() == 0.5 and b.grad[0].item() == 0.5, fn
+        assert a.grad[1].item() + b.grad[1].item() == 1.0
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_restatement_forward_matches_oracle(case):
+    dom, s = _setup(case)
+    W0 = smooth_state(dom, 50 + case)
+    Uint = o.cons_from_prim(W0, s.gamma)
+    Un_int = 1.01 * Uint
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    for stage in range(3):
+        ref, _ = stage_map(dom, s, stage, dt, Un_int)(Uint)
+        got = _torch_map(dom, s, stage, dt, Un_int)(torch.as_tensor(Uint)).numpy()
+        assert np.max(np.abs(got - ref)) <= 1e-15 * np.max(np.abs(ref)), stage
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("stage", [0, 1, 2])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
+    """lam_in = J^T lam_out over the interior, and lam_n = c1 lam_out, entry by entry."""
+    dom, s = _setup(case)
+    rng = np.random.default_rng(3000 + 10 * case + stage)
+    W0 = smooth_state(dom, 50 + case)
+    Uint = o.cons_from_prim(W0, s.gamma)
+    Un_int = Uint * (1.0 + 0.01 * rng.standard_normal(Uint.shape))
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    _, W_in = stage_map(dom, s, stage, dt, Un_int)(Uint)
+    lam_int = rng.standard_normal(Uint.shape)
+    lam_out = np.zeros((5,) + dom.shape)
+    lam_out[dom.interior] = lam_int
+    lam_in, lam_n = run_vjp(emu, dom, s, W_in, lam_out, stage, dt, sweep)
+
+    U = torch.as_tensor(Uint).clone().requires_grad_(True)
+    out = _torch_map(dom, s, stage, dt, Un_int)(U)
+    (ref,) = torch.autograd.grad(out, U, torch.as_tensor(lam_int))
+    ref = ref.numpy()
+    got = lam_in[dom.interior]
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err <= TOL, err
+    if stage:
+        c1 = o.RK3_CONS_MULTIPLIER[stage - 1][1]
+        assert np.array_equal(lam_n[dom.interior], c1 * lam_int)
