@@ -9,7 +9,8 @@ The stage map is  f(U_in) = c0 U_in + c1 U_n + dt m rhs(prim(U_in))  on the inte
 face halos filled from the interior as the reference fills them (the same formulas, operands
 and association order as ``jxf_oracle``; each function cites the reference file:line).
 Covered: WENO5-JS / WENO5-Z on the primitives, HLLC with the ARITHMETIC / RUSANOV / DAVIS /
-EINFELDT signal speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, inviscid.
+EINFELDT signal speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
+domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
 Derivative conventions at non-smooth points follow torch's, which are jax's for the operations
 used: d|x|/dx = sign(x) (0 at 0), d sign(x)/dx = 0, and maximum / minimum give half of the
@@ -20,9 +21,9 @@ from __future__ import annotations
 
 import torch
 
-from .jxf_oracle import (FACE_AXIS, FACE_LOCATIONS, RK3_CONS_MULTIPLIER, RK3_TIMESTEP_MULTIPLIER,
-                         SPATIAL_STENCIL_EPS, VELOCITY_MINOR_AXES, WENO5_SHIFTS, Domain, _face_fill_slice,
-                         shift_slice)
+from .jxf_oracle import (EPS64, FACE_AXIS, FACE_LOCATIONS, RK3_CONS_MULTIPLIER, RK3_TIMESTEP_MULTIPLIER,
+                         SPATIAL_STENCIL_EPS, VELOCITY_MINOR_AXES, WENO5_SHIFTS, Domain, Transport,
+                         _face_fill_slice, shift_slice)
 
 _DR = (1 / 10, 6 / 10, 3 / 10)   # weno5_base.py:24
 _CR = ((1 / 3, -7 / 6, 11 / 6),  # weno5_base.py:25-27
@@ -159,16 +160,93 @@ def rusanov(W_L, W_R, U_L, U_R, axis, gamma, signal_speed=None):
 RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov}
 
 
+# ------------------------------------------------------------------ dissipative fluxes (periodic)
+def _law(T, ref, T_ref, exp, C):
+    """material.py:82-124: SUTHERLAND ref ((T_ref + C)/(T + C)) (T/T_ref)^1.5 (power3_2 = x sqrt(x),
+    math/power_functions.py:29-30), else the power law ref (T/T_ref)^exp (exp = 0: constant)."""
+    if C is not None:
+        x = T / T_ref
+        return ref * ((T_ref + C) / (T + C)) * (x * torch.sqrt(x))
+    return ref * (T / T_ref) ** exp if exp != 0.0 else ref * torch.ones_like(T)
+
+
+def _wrap(W, dom: Domain):
+    """The buffer with every halo (faces, edges, vertices) a periodic copy of the interior."""
+    Wi = W[dom.interior]
+    nh = dom.nh
+    for i in dom.active:
+        n = Wi.shape[1 + i]
+        Wi = torch.cat([Wi.narrow(1 + i, n - nh, nh), Wi, Wi.narrow(1 + i, 0, nh)], dim=1 + i)
+    return Wi
+
+
+def _face_cells(buf, dom: Domain, axis: int, k: int, other=None, ok: int = 0):
+    """Cells f + k of every face f along axis (k = -1 left, 0 right), shifted by ok along other."""
+    n = dom.nh
+    sl = [dom.dom(0), dom.dom(1), dom.dom(2)]
+    hi = -n + k + 1
+    sl[axis] = slice(n + k, None if hi == 0 else hi)
+    if other is not None and ok != 0:
+        sl[other] = slice(n + ok, -n + ok)
+    return buf[(Ellipsis,) + tuple(sl)]
+
+
+def dissipative_flux_xi(W, dom: Domain, axis: int, tr: Transport, gamma: float):
+    """compute_viscous_flux_xi / compute_tau / compute_heat_flux_xi (source_term_solver.py:188-350,
+    414-448,504-580) on a periodic domain, as the amounts added to the convective face flux
+    (space_solver.py:565-587): momentum -tau_axis,k, energy -sum_k tau_axis,k u_k + q."""
+    Wp = _wrap(W, dom)
+    L, R = _face_cells(Wp, dom, axis, -1), _face_cells(Wp, dom, axis, 0)
+    T_L, T_R = L[4] / (L[0] * tr.R), R[4] / (R[0] * tr.R)                     # ideal_gas.py:81-83
+    T_f = 0.5 * (T_L + T_R)                                                      # central_2
+    T_f = torch.where(T_f <= 0.0, torch.full_like(T_f, EPS64), T_f)              # :301
+    mu1 = _law(T_f, tr.mu_ref, tr.T_ref, tr.mu_exp, tr.mu_sutherland)
+    mu2 = tr.bulk_viscosity - 2.0 / 3.0 * mu1                                    # :527
+    zero = torch.zeros_like(T_f)
+    g = [[zero] * 3 for _ in range(3)]  # g[k][i] = d u_k / d x_i at the faces (:414-448,538-580)
+    for i in dom.active:
+        for k in range(3):
+            if i == axis:
+                g[k][i] = (R[1 + k] - L[1 + k]) * dom.one_dx[i]                  # deriv_face_2
+            else:
+                h = 0.5 * dom.one_dx[i]                                          # deriv_center_2
+                dL = (_face_cells(Wp, dom, axis, -1, i, 1)[1 + k] - _face_cells(Wp, dom, axis, -1, i, -1)[1 + k]) * h
+                dR = (_face_cells(Wp, dom, axis, 0, i, 1)[1 + k] - _face_cells(Wp, dom, axis, 0, i, -1)[1 + k]) * h
+                g[k][i] = 0.5 * (dL + dR)                                        # central_2
+    tau = [zero, zero, zero]
+    if tr.viscous:
+        for k in dom.active:
+            tau[k] = mu1 * (g[axis][k] + g[k][axis])
+        div = g[dom.active[0]][dom.active[0]]
+        for k in dom.active[1:]:
+            div = div + g[k][k]
+        tau[axis] = tau[axis] + mu2 * div
+    vel_tau = 0.0
+    for k in dom.active:
+        vel_tau = vel_tau + tau[k] * (0.5 * (L[1 + k] + R[1 + k]))
+    E = -vel_tau if tr.viscous else zero
+    if tr.heat:
+        if tr.prandtl is not None:
+            kappa = (gamma / (gamma - 1.0) * tr.R) * mu1 / tr.prandtl           # material.py:115-116
+        else:
+            kappa = _law(T_f, tr.kappa_ref, tr.kappa_T_ref, tr.kappa_exp, tr.kappa_sutherland)
+        E = E + (-kappa * ((T_R - T_L) * dom.one_dx[axis]))                      # deriv_face_2
+    return torch.stack([zero, -tau[0], -tau[1], -tau[2], E])
+
+
 # ------------------------------------------------------------------ RHS and halos
-def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float):
+def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, transport=None):
     """SpaceSolver.compute_rhs (space_solver.py:266,289-314,489,597-599) with PRIMITIVE
-    reconstruction (high_order_godunov.py:267-280): 0.0 + sum over axes of (1/dx)(F[:-1] - F[1:])."""
+    reconstruction (high_order_godunov.py:267-280): 0.0 + sum over axes of (1/dx)(F[:-1] - F[1:]),
+    the dissipative face flux added to the convective one (:565-587) with a transport."""
     rhs = 0.0
     for axis in dom.active:
         W_L = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[0]])
         W_R = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[1]])
         F = RIEMANN[riemann](W_L, W_R, cons_from_prim(W_L, gamma), cons_from_prim(W_R, gamma), axis, gamma,
                              signal_speed)
+        if transport is not None:
+            F = F + dissipative_flux_xi(W, dom, axis, transport, gamma)
         n = F.shape[axis + 1]
         rhs = rhs + (0.0 + dom.one_dx[axis] * (F.narrow(axis + 1, 0, n - 1) - F.narrow(axis + 1, 1, n - 1)))
     return rhs
@@ -207,7 +285,7 @@ def face_halos(W, dom: Domain, bcs: dict):
 
 
 def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, bcs: dict,
-              stage: int, dt: float, eps: float):
+              stage: int, dt: float, eps: float, transport=None):
     """f(U_in interior) -> U_out interior: W = prim(U_in) on the interior of an eps-filled buffer,
     face halos of W, rhs, then c0 U_in + c1 U_n + dt m rhs (RK3.py:27-62, time_integrator.py:57)."""
     m = RK3_TIMESTEP_MULTIPLIER[stage]
@@ -215,7 +293,9 @@ def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_spe
     W = torch.full((5,) + dom.shape, eps, dtype=torch.float64)
     W[dom.interior] = prim_from_cons(U_int, gamma)
     W = face_halos(W, dom, bcs)
-    rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma)
+    if transport is not None:
+        assert all(bcs[f] == "PERIODIC" for f in FACE_LOCATIONS if FACE_AXIS[f] in dom.active)
+    rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma, transport)
     if stage == 0:
         return U_int + (dt * m) * rhs
     return (c0 * U_int + c1 * Un_int) + (dt * m) * rhs

@@ -78,22 +78,14 @@ def test_stage_vjp_device_dot_product(case, stage):
         assert np.allclose(host(ln)[dom.interior], c1 * lam_int, rtol=0, atol=0)
 
 
-@pytest.mark.parametrize("case", range(len(CASES)))
-@pytest.mark.parametrize("stage", [0, 1, 2])
-def test_stage_vjp_device_matches_autograd(case, stage):
-    """Every entry of the device J^T lam against torch.autograd of the PyTorch restatement of the
-    stage (oracle/torch_stage.py), the stand-in for jax.grad; 1e-11 of the largest entry (the
-    rounding of the two reverse evaluations, as in the host-emulated test)."""
+def _vjp_vs_autograd(dom, s, stage, seed_state, seed):
+    import dataclasses
+
     from jaxfluids_amd import _native
     from oracle import torch_stage as ts
 
-    say(f"[stage vjp vs autograd] case {case} stage {stage}")
-    cells, bcs, (stencil, riemann, speed) = CASES[case]
-    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
-    s = o.Setup(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann=riemann, signal_speed=speed,
-                bcs=dict(zip(FACES, bcs)))
-    rng = np.random.default_rng(5000 + 10 * case + stage)
-    W0 = smooth_state(dom, 70 + case)
+    rng = np.random.default_rng(seed)
+    W0 = smooth_state(dom, seed_state)
     Uint = o.cons_from_prim(W0, s.gamma)
     Un_int = Uint * (1.0 + 0.01 * rng.standard_normal(Uint.shape))
     dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
@@ -102,8 +94,9 @@ def test_stage_vjp_device_matches_autograd(case, stage):
     lam_out = np.zeros((5,) + dom.shape)
     lam_out[dom.interior] = lam_int
     grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
-    scheme = _native.make_scheme("PRIMITIVE", stencil, riemann, speed, "ARITHMETIC", _native.F64, 1.4,
-                                 s.eps_stencil)
+    tr = None if s.transport is None else _native.make_transport(**dataclasses.asdict(s.transport))
+    scheme = _native.make_scheme("PRIMITIVE", s.stencil, s.riemann, s.signal_speed, "ARITHMETIC", _native.F64, 1.4,
+                                 s.eps_stencil, tr)
     lib = _native.load()
     ws = torch.empty(lib.jfa_stage_vjp_workspace_bytes(grid, scheme), dtype=torch.uint8, device="cuda")
     Wd, ld = dev(W_in), dev(lam_out)
@@ -113,12 +106,41 @@ def test_stage_vjp_device_matches_autograd(case, stage):
                                _native.bc_array(s.bcs), stage, dt, P(ws), ws.numel(), _native.stream_ptr())
     _native.check(rc, "jfa_rk3_stage_vjp")
     U = torch.as_tensor(Uint).clone().requires_grad_(True)
-    out = ts.stage_map(U, torch.as_tensor(Un_int), dom, stencil, riemann, speed, s.gamma, s.bcs, stage, dt, o.EPS64)
+    out = ts.stage_map(U, torch.as_tensor(Un_int), dom, s.stencil, s.riemann, s.signal_speed, s.gamma, s.bcs, stage,
+                       dt, o.EPS64, s.transport)
     (ref,) = torch.autograd.grad(out, U, torch.as_tensor(lam_int))
     ref = ref.numpy()
     got = host(li)[dom.interior]
     err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
     assert err <= 1e-11, err
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("stage", [0, 1, 2])
+def test_stage_vjp_device_matches_autograd(case, stage):
+    """Every entry of the device J^T lam against torch.autograd of the PyTorch restatement of the
+    stage (oracle/torch_stage.py), the stand-in for jax.grad; 1e-11 of the largest entry (the
+    rounding of the two reverse evaluations, as in the host-emulated test)."""
+    say(f"[stage vjp vs autograd] case {case} stage {stage}")
+    cells, bcs, (stencil, riemann, speed) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann=riemann, signal_speed=speed,
+                bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, stage, 70 + case, 5000 + 10 * case + stage)
+
+
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
+@pytest.mark.parametrize("stage", [0, 2])
+def test_viscous_stage_vjp_device_matches_autograd(case, stage):
+    """The same with viscous and / or heat fluxes (periodic)."""
+    from test_emu_adjoint import VISCOUS_CASES
+
+    say(f"[viscous stage vjp vs autograd] case {case} stage {stage}")
+    cells, (stencil, riemann, speed), tr = VISCOUS_CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann=riemann, signal_speed=speed,
+                bcs={f: "PERIODIC" for f in FACES}, transport=tr)
+    _vjp_vs_autograd(dom, s, stage, 85 + case, 6000 + 10 * case + stage)
 
 
 def _kh_stepper(n, dt):

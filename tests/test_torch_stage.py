@@ -13,7 +13,7 @@ import torch
 
 from oracle import jxf_oracle as o
 from oracle import torch_stage as ts
-from test_emu_adjoint import CASES, FACES, emu, run_vjp, smooth_state, stage_map  # noqa: F401
+from test_emu_adjoint import CASES, FACES, VISCOUS_CASES, emu, run_vjp, smooth_state, stage_map  # noqa: F401
 
 # Both sides compute in FP64 with different association (hand-derived reverse formulas against
 # autograd's chain of elementwise derivatives); the derivatives of the WENO weights amplify
@@ -30,9 +30,39 @@ def _setup(case):
     return dom, s
 
 
+def _viscous_setup(case):
+    cells, (stencil, riemann, speed), tr = VISCOUS_CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann=riemann, signal_speed=speed,
+                bcs={f: "PERIODIC" for f in FACES}, transport=tr)
+    return dom, s
+
+
 def _torch_map(dom, s, stage, dt, Un_int):
     return lambda U: ts.stage_map(U, torch.as_tensor(Un_int), dom, s.stencil, s.riemann, s.signal_speed, s.gamma,
-                                  s.bcs, stage, dt, o.EPS64)
+                                  s.bcs, stage, dt, o.EPS64, s.transport)
+
+
+def _compare(emu, dom, s, stage, sweep, seed_state, seed):  # noqa: F811
+    rng = np.random.default_rng(seed)
+    W0 = smooth_state(dom, seed_state)
+    Uint = o.cons_from_prim(W0, s.gamma)
+    Un_int = Uint * (1.0 + 0.01 * rng.standard_normal(Uint.shape))
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    _, W_in = stage_map(dom, s, stage, dt, Un_int)(Uint)
+    lam_int = rng.standard_normal(Uint.shape)
+    lam_out = np.zeros((5,) + dom.shape)
+    lam_out[dom.interior] = lam_int
+    lam_in, lam_n = run_vjp(emu, dom, s, W_in, lam_out, stage, dt, sweep)
+    U = torch.as_tensor(Uint).clone().requires_grad_(True)
+    out = _torch_map(dom, s, stage, dt, Un_int)(U)
+    (ref,) = torch.autograd.grad(out, U, torch.as_tensor(lam_int))
+    ref = ref.numpy()
+    err = np.max(np.abs(lam_in[dom.interior] - ref)) / np.max(np.abs(ref))
+    assert err <= TOL, err
+    if stage:
+        c1 = o.RK3_CONS_MULTIPLIER[stage - 1][1]
+        assert np.array_equal(lam_n[dom.interior], c1 * lam_int)
 
 
 def test_tie_conventions_are_jax_conventions():
@@ -71,24 +101,25 @@ def test_restatement_forward_matches_oracle(case):
 def test_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
     """lam_in = J^T lam_out over the interior, and lam_n = c1 lam_out, entry by entry."""
     dom, s = _setup(case)
-    rng = np.random.default_rng(3000 + 10 * case + stage)
-    W0 = smooth_state(dom, 50 + case)
-    Uint = o.cons_from_prim(W0, s.gamma)
-    Un_int = Uint * (1.0 + 0.01 * rng.standard_normal(Uint.shape))
-    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
-    _, W_in = stage_map(dom, s, stage, dt, Un_int)(Uint)
-    lam_int = rng.standard_normal(Uint.shape)
-    lam_out = np.zeros((5,) + dom.shape)
-    lam_out[dom.interior] = lam_int
-    lam_in, lam_n = run_vjp(emu, dom, s, W_in, lam_out, stage, dt, sweep)
+    _compare(emu, dom, s, stage, sweep, 50 + case, 3000 + 10 * case + stage)
 
-    U = torch.as_tensor(Uint).clone().requires_grad_(True)
-    out = _torch_map(dom, s, stage, dt, Un_int)(U)
-    (ref,) = torch.autograd.grad(out, U, torch.as_tensor(lam_int))
-    ref = ref.numpy()
-    got = lam_in[dom.interior]
-    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
-    assert err <= TOL, err
-    if stage:
-        c1 = o.RK3_CONS_MULTIPLIER[stage - 1][1]
-        assert np.array_equal(lam_n[dom.interior], c1 * lam_int)
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("stage", [0, 2])
+@pytest.mark.parametrize("case", range(len(VISCOUS_CASES)))
+def test_viscous_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
+    """The same with viscous and / or heat fluxes (periodic domains: power law, Sutherland,
+    Prandtl, own conductivity laws, bulk viscosity, heat only)."""
+    dom, s = _viscous_setup(case)
+    _compare(emu, dom, s, stage, sweep, 80 + case, 4000 + 10 * case + stage)
+
+
+def test_viscous_forward_matches_oracle():
+    for case in range(len(VISCOUS_CASES)):
+        dom, s = _viscous_setup(case)
+        W0 = smooth_state(dom, 80 + case)
+        Uint = o.cons_from_prim(W0, s.gamma)
+        for stage in range(3):
+            ref, _ = stage_map(dom, s, stage, 1e-3, 1.01 * Uint)(Uint)
+            got = _torch_map(dom, s, stage, 1e-3, 1.01 * Uint)(torch.as_tensor(Uint)).numpy()
+            assert np.max(np.abs(got - ref)) <= 1e-15 * np.max(np.abs(ref)), (case, stage)
