@@ -190,6 +190,22 @@ def test_fused_step_viscous_boundaries(emu, grid, bcs):
     check_step(emu, dom, s, W0, transport=tr)
 
 
+@pytest.mark.parametrize("grid", ["3d", "2d-xz"])
+def test_fused_step_viscous_boundaries_fp32_storage(emu, grid):
+    """The FP32-storage instantiation with dissipative fluxes on a mixed non-periodic domain (the
+    float edge-halo fill included) against the FP64 oracle: the float32 rounding of the state."""
+    dom, W0 = grid_state(grid, seed=21)
+    tr = o.Transport(R=1.0, mu_ref=2e-2, T_ref=1.0, mu_exp=0.76, bulk_viscosity=1e-2, prandtl=0.71)
+    s = o.Setup(gamma=1.4, bcs=VISCOUS_BCS["mixed"], cfl=0.5, transport=tr, **SCHEMES["z-charprim-hllc"])
+    W, U = o.create_buffers(W0, dom, s)
+    dt = o.compute_dt(W, dom, s)
+    Wr, _ = o.rk3_step(W, U, dt, dom, s)
+    Wg, _, _ = fused_step(emu, dom, s, W, U, dt, transport=tr, dtype=np.float32)
+    for t in [dom.interior] + halo_regions(dom, edges=True):
+        err = rel(Wg[t].astype(np.float64), Wr[t])
+        assert err < 1e-6, (t, err)
+
+
 @pytest.mark.parametrize("bcs", list(VISCOUS_BCS))
 @pytest.mark.parametrize("grid", ["3d", "2d-yz"])
 def test_edge_halo_update(emu, grid, bcs):
