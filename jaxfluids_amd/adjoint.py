@@ -154,14 +154,18 @@ def cons_vjp(W: torch.Tensor, gU: torch.Tensor, gamma: float) -> torch.Tensor:
 
 # ------------------------------------------------------------------ stepper
 class FixedDtStepper:
-    """Fixed-dt RK3 steps through the fused HIP stage and their VJPs, FP64, primitive
-    reconstruction.  ``cells``/``ranges``/``bcs`` as in the case setup; the scheme strings are
-    the reference's registry names; ``transport`` (``_native.make_transport``) adds the viscous
-    and heat fluxes on periodic domains."""
+    """Fixed-dt RK3 steps through the fused HIP stage and their VJPs, FP64, with PRIMITIVE or
+    CHAR-PRIMITIVE reconstruction (the latter with the ARITHMETIC frozen state).
+    ``cells``/``ranges``/``bcs`` as in the case setup; the scheme strings are the reference's
+    registry names; ``transport`` (``_native.make_transport``) adds the viscous and heat fluxes
+    on periodic domains."""
 
     def __init__(self, cells, ranges, bcs: Dict[str, str], dt: float, *, nh: int = 3, stencil: str = "WENO5-JS",
                  riemann: str = "HLLC", signal_speed: str = "EINFELDT", gamma: float = 1.4,
-                 transport: Optional[_native.JfaTransport] = None, device=None) -> None:
+                 transport: Optional[_native.JfaTransport] = None, device=None,
+                 variable: str = "PRIMITIVE") -> None:
+        if variable not in ("PRIMITIVE", "CHAR-PRIMITIVE"):
+            raise NotImplementedError(f"reconstruction_variable {variable} has no reverse mode")
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
         self.cells = tuple(int(c) for c in cells)
         self.nh = int(nh)
@@ -169,7 +173,7 @@ class FixedDtStepper:
         dx = [(r[1] - r[0]) / n for r, n in zip(ranges, self.cells)]
         self.grid = _native.make_grid(self.cells, self.nh, dx)
         self.transport = transport  # viscous / heat fluxes (periodic domains); None: inviscid
-        self.scheme = _native.make_scheme("PRIMITIVE", stencil, riemann, signal_speed if riemann == "HLLC" else None,
+        self.scheme = _native.make_scheme(variable, stencil, riemann, signal_speed if riemann == "HLLC" else None,
                                           "ARITHMETIC", _native.F64, gamma, precision.get_spatial_stencil_eps(),
                                           transport)
         full = {f: bcs.get(f, "PERIODIC") for f in _native.FACE_LOCATIONS}

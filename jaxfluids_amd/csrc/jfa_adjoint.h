@@ -355,20 +355,123 @@ __device__ __forceinline__ void rusanov_vjp(const double WL[5], const double WR[
     cons_vjp(WR, gUR, prm, gWR);
 }
 
-// VJP of one face flux, first part: cotangents of the reconstructed face states WL, WR.
-template <int AXIS, int STENCIL, int RIEMANN>
+// ------------------------------------------------------------------ CHAR-PRIMITIVE projection
+// With an ARITHMETIC frozen state a = rho_ave, q = p_ave of the face's two cells
+// (eigendecomposition.py:139-148,223-231) the projection of a window cell w
+// (get_characteristics_from_primitives, :421-433) and the face states from the reconstructed
+// characteristics x (get_primitives_from_characteristics, :512-523) are
+//   x0 = k2 p - k1 u_n,  x1 = rho - k3 p,  x2 = u_m0,  x3 = u_m1,  x4 = k2 p + k1 u_n,
+//   W_rho = a (x0 + x4) + x1,  W_n = c (x4 - x0),  W_m0 = x2,  W_m1 = x3,  W_p = rcc (x0 + x4),
+// with c = sqrt(gamma q / a), k1 = 1/(2c), k2 = 1/(2 rcc), k3 = 1/c^2 = a / rcc, rcc = gamma q
+// (the forward's face_states evaluates the same values).  As functions of (a, q):
+//   dk1 = k1 (da/a - dq/q)/2,  dk2 = -k2 dq/q,  dk3 = k3 (da/a - dq/q),  dc = c (dq/q - da/a)/2.
+struct CharK {
+    double a, q, c, rcc, k1, k2, k3;
+};
+
+__device__ __forceinline__ CharK char_frozen(const double wl[5], const double wr[5], const Params& prm) {
+    Frozen f;
+    frozen_state<false>(wl, wr, prm, f);
+    CharK K;
+    K.a = f.P[0];
+    K.q = f.P[4];
+    K.c = f.c;
+    K.rcc = f.rcc;
+    K.k1 = 0.5 * f.inv_c;
+    K.k3 = f.inv_c * f.inv_c;
+    K.k2 = 0.5 * rcp1(f.rcc);
+    return K;
+}
+
+// characteristic variable k of window cell w (the expressions of face_states)
+template <int AXIS>
+__device__ __forceinline__ double char_value(int k, const double w[5], const CharK& K) {
+    constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
+    if (k == 0) return K.k2 * w[4] - K.k1 * w[v];
+    if (k == 1) return w[0] - K.k3 * w[4];
+    if (k == 2) return w[m0];
+    if (k == 3) return w[m1];
+    return K.k2 * w[4] + K.k1 * w[v];
+}
+
+// The primitive variables characteristic k reads and their coefficients: x_k = c0 w[i0] + c1 w[i1]
+// (n = 1 or 2 terms).
+template <int AXIS>
+__device__ __forceinline__ int char_terms(int k, const CharK& K, int& i0, double& c0, int& i1, double& c1) {
+    constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
+    if (k == 0) { i0 = 4; c0 = K.k2; i1 = v; c1 = -K.k1; return 2; }
+    if (k == 1) { i0 = 0; c0 = 1.0; i1 = 4; c1 = -K.k3; return 2; }
+    if (k == 2) { i0 = m0; c0 = 1.0; i1 = m0; c1 = 0.0; return 1; }
+    if (k == 3) { i0 = m1; c0 = 1.0; i1 = m1; c1 = 0.0; return 1; }
+    i0 = 4; c0 = K.k2; i1 = v; c1 = K.k1;
+    return 2;
+}
+
+// Adds to gk = (g k1, g k2, g k3) the cotangents of the projection coefficients from the
+// window cotangent g6 of characteristic k (cells w6[j][.]).
+template <int AXIS>
+__device__ __forceinline__ void char_coeff_vjp(int k, const double p6[6], const double u6[6], const double g6[6],
+                                               double gk[3]) {
+    if (k == 0 || k == 4) {
+        const double sg = k == 0 ? -1.0 : 1.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            gk[1] += p6[j] * g6[j];
+            gk[0] += sg * u6[j] * g6[j];
+        }
+    } else if (k == 1) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) gk[2] -= p6[j] * g6[j];
+    }
+}
+
+// Cotangents (g a, g q) of the frozen state from those of the projection coefficients gk and of
+// the back projection fz = (g a, g q, g c) (face_state_vjp).
+__device__ __forceinline__ void char_frozen_vjp(const CharK& K, const double gk[3], const double fz[3], double& ga,
+                                                double& gq) {
+    ga = fz[0] + gk[0] * K.k1 / (2.0 * K.a) + gk[2] * K.k3 / K.a - fz[2] * K.c / (2.0 * K.a);
+    gq = fz[1] - gk[0] * K.k1 / (2.0 * K.q) - gk[1] * K.k2 / K.q - gk[2] * K.k3 / K.q + fz[2] * K.c / (2.0 * K.q);
+}
+
+// VJP of one face flux, first part: cotangents of the reconstructed face states.  PRIMITIVE:
+// gl, gr are those of WL, WR.  CHAR-PRIMITIVE (ARITHMETIC frozen state): gl, gr are those of
+// the reconstructed characteristics and fz = (g a, g q, g c) the frozen state's share through
+// the back projection (W_rho depends on a, W_n on c, W_p on rcc = gamma q).
+template <int AXIS, int STENCIL, int RIEMANN, int VAR>
 __device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const double mu[5], const Params& prm,
-                                               double gWL[5], double gWR[5]) {
+                                               double gl[5], double gr[5], double fz[3]) {
     double wl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, wr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double WL[5], WR[5], UL[5], UR[5];
-    face_states<AXIS, JFA_PRIMITIVE, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        gWL[i] = 0.0;
-        gWR[i] = 0.0;
-    }
+    face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
+    double gWL[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gWR[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if constexpr (RIEMANN == JFA_HLLC) hllc_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     else rusanov_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
+    if constexpr (VAR == JFA_PRIMITIVE) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            gl[i] = gWL[i];
+            gr[i] = gWR[i];
+        }
+        fz[0] = fz[1] = fz[2] = 0.0;
+    } else {
+        constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
+        const CharK K = char_frozen(win[2], win[3], prm);
+        // x0 + x4 = W_p / rcc, x4 - x0 = W_n / c
+        gl[0] = K.a * gWL[0] - K.c * gWL[v] + K.rcc * gWL[4];
+        gl[4] = K.a * gWL[0] + K.c * gWL[v] + K.rcc * gWL[4];
+        gl[1] = gWL[0];
+        gl[2] = gWL[m0];
+        gl[3] = gWL[m1];
+        gr[0] = K.a * gWR[0] - K.c * gWR[v] + K.rcc * gWR[4];
+        gr[4] = K.a * gWR[0] + K.c * gWR[v] + K.rcc * gWR[4];
+        gr[1] = gWR[0];
+        gr[2] = gWR[m0];
+        gr[3] = gWR[m1];
+        const double sL = WL[4] / K.rcc, sR = WR[4] / K.rcc;
+        fz[0] = sL * gWL[0] + sR * gWR[0];
+        fz[1] = prm.gamma * (sL * gWL[4] + sR * gWR[4]);
+        fz[2] = (WL[v] / K.c) * gWL[v] + (WR[v] / K.c) * gWR[v];
+    }
 }
 
 // Second part, one variable at a time: cotangents of the six window cells x[0..5] of that
@@ -382,17 +485,40 @@ __device__ __forceinline__ void window_vjp_var(const double x[6], double gl, dou
     weno5_vjp<STENCIL>(x[5], x[4], x[3], x[2], x[1], eps, gr, out[5], out[4], out[3], out[2], out[1]);
 }
 
-// Face f (right cell at buffer index br, stride sa along the face normal): cotangents of WL, WR
-// from a window loaded here, so the window is not held across the Riemann VJP.
-template <int AXIS, int STENCIL, int RIEMANN>
+// Face f (right cell at buffer index br, stride sa along the face normal): cotangents of the
+// face states from a window loaded here, so the window is not held across the Riemann VJP.
+template <int AXIS, int STENCIL, int RIEMANN, int VAR>
 __device__ __forceinline__ void face_state_vjp_at(const double* __restrict__ W, int64_t br, int64_t sa, int64_t fs,
-                                                  const double mu[5], const Params& prm, double gl[5], double gr[5]) {
+                                                  const double mu[5], const Params& prm, double gl[5], double gr[5],
+                                                  double fz[3]) {
     double win[6][5];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
 #pragma unroll
         for (int i = 0; i < 5; ++i) win[k][i] = ld(W, i * fs + br + (int64_t)(k - 3) * sa);
-    face_state_vjp<AXIS, STENCIL, RIEMANN>(win, mu, prm, gl, gr);
+    face_state_vjp<AXIS, STENCIL, RIEMANN, VAR>(win, mu, prm, gl, gr, fz);
+}
+
+// CHAR-PRIMITIVE, second part for characteristic k of the face: its window (projected from the
+// cells, reloaded; they are in cache), the WENO VJP of its two reconstructions into g6, and its
+// share of the coefficient cotangents gk.
+template <int AXIS, int STENCIL>
+__device__ __forceinline__ void char_window_vjp(const double* __restrict__ W, int64_t br, int64_t sa, int64_t fs,
+                                                int k, const CharK& K, double gl, double gr, double eps, double g6[6],
+                                                double gk[3]) {
+    constexpr int v = Ax<AXIS>::v;
+    double x6[6], p6[6], u6[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double w[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) w[i] = ld(W, i * fs + br + (int64_t)(j - 3) * sa);
+        x6[j] = char_value<AXIS>(k, w, K);
+        p6[j] = w[4];
+        u6[j] = w[v];
+    }
+    window_vjp_var<STENCIL>(x6, gl, gr, eps, g6);
+    char_coeff_vjp<AXIS>(k, p6, u6, g6, gk);
 }
 
 // The six window cells of variable i of that face (reloaded; they are in cache).
@@ -524,7 +650,7 @@ __device__ __forceinline__ void add_cell(double* __restrict__ gW, int64_t fs, in
 // the faces f in [o0-2, o1+2] (clipped to 0..N) whose windows touch them; each face adds its
 // window cotangents to the owned cells in place (the six contributions a cell receives come
 // from consecutive faces of the same thread, so they hit cache and never race).
-template <int A, int IN, int STENCIL, int RIEMANN>
+template <int A, int IN, int STENCIL, int RIEMANN, int VAR>
 __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __restrict__ W,
                                                               const double* __restrict__ lam, Geo G, Params prm,
                                                               double ts, int L, double* __restrict__ gW) {
@@ -543,21 +669,67 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
     for (int f = fa; f <= fb; ++f) {
         int64_t br = bcol + (int64_t)f * sa;
         JFA_OPAQUE(br);  // keeps per-tap addresses from becoming loop-carried registers
-        double mu[5], gl[5], gr[5];
+        double mu[5], gl[5], gr[5], fz[3];
         face_mu(lam, br, sa, fs, f, N, scale, mu);
-        face_state_vjp_at<A, STENCIL, RIEMANN>(W, br, sa, fs, mu, prm, gl, gr);
+        face_state_vjp_at<A, STENCIL, RIEMANN, VAR>(W, br, sa, fs, mu, prm, gl, gr, fz);
+        if constexpr (VAR == JFA_PRIMITIVE) {
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            double x6[6], o6[6];
-            load_var6(W, br, sa, fs, i, x6);
-            window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
+            for (int i = 0; i < 5; ++i) {
+                double x6[6], o6[6];
+                load_var6(W, br, sa, fs, i, x6);
+                window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) {
+                for (int k = 0; k < 6; ++k) {
+                    const int cell = f - 3 + k;
+                    if (cell >= o0 && cell < o1) {
+                        double* p = gW + i * fs + br + (int64_t)(k - 3) * sa;
+                        JFA_ACCESS(p, sizeof(double));
+                        *p += o6[k];
+                    }
+                }
+            }
+        } else {
+            double wl[5], wr[5];
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                wl[i] = ld(W, i * fs + br - sa);
+                wr[i] = ld(W, i * fs + br);
+            }
+            const CharK K = char_frozen(wl, wr, prm);
+            double gk[3] = {0.0, 0.0, 0.0};
+            for (int c = 0; c < 5; ++c) {
+                double g6[6];
+                char_window_vjp<A, STENCIL>(W, br, sa, fs, c, K, gl[c], gr[c], prm.eps, g6, gk);
+                int i0, i1;
+                double c0, c1;
+                const int nt = char_terms<A>(c, K, i0, c0, i1, c1);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const int cell = f - 3 + k;
+                    if (cell >= o0 && cell < o1) {
+                        double* p = gW + i0 * fs + br + (int64_t)(k - 3) * sa;
+                        JFA_ACCESS(p, sizeof(double));
+                        *p += c0 * g6[k];
+                        if (nt == 2) {
+                            double* q = gW + i1 * fs + br + (int64_t)(k - 3) * sa;
+                            JFA_ACCESS(q, sizeof(double));
+                            *q += c1 * g6[k];
+                        }
+                    }
+                }
+            }
+            double ga, gq;
+            char_frozen_vjp(K, gk, fz, ga, gq);
+#pragma unroll
+            for (int k = 2; k < 4; ++k) {  // a and q are the means of the face's two cells
                 const int cell = f - 3 + k;
                 if (cell >= o0 && cell < o1) {
-                    double* p = gW + i * fs + br + (int64_t)(k - 3) * sa;
+                    double* p = gW + br + (int64_t)(k - 3) * sa;
                     JFA_ACCESS(p, sizeof(double));
-                    *p += o6[k];
+                    *p += 0.5 * ga;
+                    double* q = gW + 4 * fs + br + (int64_t)(k - 3) * sa;
+                    JFA_ACCESS(q, sizeof(double));
+                    *q += 0.5 * gq;
                 }
             }
         }
@@ -568,7 +740,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
 // Wave w of a row evaluates the faces F0 .. F0+63, F0 = -5 + 59 w (lane l: face F0 + l, zero
 // outside 0..N) and lanes 5..63 own the cells F0 + l - 3, i.e. [-3 + 59 w, -3 + 59 w + 59)
 // clipped to the row's N + 6 cells.  Rows run over the two other axes P, Q.
-template <int A, int STENCIL, int RIEMANN>
+template <int A, int STENCIL, int RIEMANN, int VAR>
 __global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __restrict__ W,
                                                              const double* __restrict__ lam, Geo G, Params prm,
                                                              double ts, int waves_per_row, int64_t nwaves,
@@ -589,22 +761,61 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __res
     const int f = -5 + 59 * w + lane;
     const bool vface = vwave && f >= 0 && f <= N;
     const int64_t br = brow + (vface ? f : 0) * sa;
-    double gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gr[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fz[3] = {0.0, 0.0, 0.0};
     if (vface) {
         double mu[5];
         face_mu(lam, br, sa, fs, f, N, ts * G.one_dx[A], mu);
-        face_state_vjp_at<A, STENCIL, RIEMANN>(W, br, sa, fs, mu, prm, gl, gr);
+        face_state_vjp_at<A, STENCIL, RIEMANN, VAR>(W, br, sa, fs, mu, prm, gl, gr, fz);
     }
     // cell c = f - 3 gathers window slot k of face c + 3 - k, held by lane - k
-    double s[5];
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if constexpr (VAR == JFA_PRIMITIVE) {
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        double x6[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0}, o6[6];  // lanes without a face: zero cotangents
-        if (vface) load_var6(W, br, sa, fs, i, x6);
-        window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
-        s[i] = 0.0;
+        for (int i = 0; i < 5; ++i) {
+            double x6[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0}, o6[6];  // lanes without a face: zero cotangents
+            if (vface) load_var6(W, br, sa, fs, i, x6);
+            window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) s[i] += __shfl_up(o6[k], (unsigned)k, 64);
+            for (int k = 0; k < 6; ++k) s[i] += __shfl_up(o6[k], (unsigned)k, 64);
+        }
+    } else {
+        // lanes without a face evaluate face 0 of the row (its window lies in the interior and
+        // the face halos) with zero cotangents, so they add nothing
+        const int64_t bq = vface ? br : brow;
+        double wl[5], wr[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            wl[i] = ld(W, i * fs + bq - sa);
+            wr[i] = ld(W, i * fs + bq);
+        }
+        const CharK K = char_frozen(wl, wr, prm);
+        double gk[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            double g6[6];
+            char_window_vjp<A, STENCIL>(W, bq, sa, fs, c, K, gl[c], gr[c], prm.eps, g6, gk);
+            int i0, i1;
+            double c0, c1;
+            const int nt = char_terms<A>(c, K, i0, c0, i1, c1);
+            double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                t0 += __shfl_up(c0 * g6[k], (unsigned)k, 64);
+                t1 += __shfl_up(c1 * g6[k], (unsigned)k, 64);
+            }
+            // i0, i1 are uniform across the wave (they depend on c and A only)
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                if (i == i0) s[i] += t0;
+                if (nt == 2 && i == i1) s[i] += t1;
+            }
+        }
+        double ga, gq;
+        char_frozen_vjp(K, gk, fz, ga, gq);
+        if (!vface) ga = gq = 0.0;
+        // a and q are the means of the face's two cells (window slots 2 and 3)
+        s[0] += __shfl_up(0.5 * ga, 2u, 64) + __shfl_up(0.5 * ga, 3u, 64);
+        s[4] += __shfl_up(0.5 * gq, 2u, 64) + __shfl_up(0.5 * gq, 3u, 64);
     }
     const int cell = f - 3;
     if (vwave && lane >= 5 && cell <= N + 2) add_cell(gW, fs, brow + (int64_t)cell * sa, s);
@@ -777,7 +988,7 @@ __global__ __launch_bounds__(kBlock) void adj_zero_kernel(double* __restrict__ p
 }
 
 // ------------------------------------------------------------------ host side
-template <int A, int IN, int STENCIL, int RIEMANN>
+template <int A, int IN, int STENCIL, int RIEMANN, int VAR>
 static void launch_adj_march(const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
                              double* gW, hipStream_t st) {
     constexpr int O = 3 - A - IN;
@@ -786,11 +997,11 @@ static void launch_adj_march(const double* W, const double* lam, const Geo& G, c
     const int L = 16;
     const int ncell = G.n[A] + 6;
     dim3 grid((unsigned)((G.n[IN] + bx - 1) / bx), (unsigned)G.n[O], (unsigned)((ncell + L - 1) / L));
-    hipLaunchKernelGGL((adj_march_kernel<A, IN, STENCIL, RIEMANN>), grid, dim3(bx), 0, st, W, lam, G, prm, ts, L,
-                       gW);
+    hipLaunchKernelGGL((adj_march_kernel<A, IN, STENCIL, RIEMANN, VAR>), grid, dim3(bx), 0, st, W, lam, G, prm, ts,
+                       L, gW);
 }
 
-template <int A, int STENCIL, int RIEMANN>
+template <int A, int STENCIL, int RIEMANN, int VAR>
 static void launch_adj_lanes(const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
                              double* gW, hipStream_t st) {
     constexpr int P = A == 0 ? 1 : 0;
@@ -798,8 +1009,8 @@ static void launch_adj_lanes(const double* W, const double* lam, const Geo& G, c
     const int waves_per_row = (G.n[A] + 6 + 58) / 59;
     const int64_t nwaves = (int64_t)waves_per_row * G.n[P] * G.n[Q];
     const int64_t blocks = (nwaves + kAdjBlock / 64 - 1) / (kAdjBlock / 64);
-    hipLaunchKernelGGL((adj_lane_kernel<A, STENCIL, RIEMANN>), dim3((unsigned)blocks), dim3(kAdjBlock), 0, st, W,
-                       lam, G, prm, ts, waves_per_row, nwaves, gW);
+    hipLaunchKernelGGL((adj_lane_kernel<A, STENCIL, RIEMANN, VAR>), dim3((unsigned)blocks), dim3(kAdjBlock), 0, st,
+                       W, lam, G, prm, ts, waves_per_row, nwaves, gW);
 }
 
 // Waves of the marching sweep along A (columns along IN, chunks of 16 cells along A).
@@ -813,7 +1024,7 @@ static inline int64_t march_waves(const Geo& G, int A, int IN) {
 int64_t adjoint_march_min_waves();
 
 // Transpose sweeps of the convective flux along every active axis into gW.
-template <int STENCIL, int RIEMANN>
+template <int STENCIL, int RIEMANN, int VAR>
 static void adj_sweeps(const double* W, const double* lam, const Geo& G, const Params& prm, double ts, double* gW,
                        hipStream_t st) {
     const int IN = innermost_axis(G);
@@ -821,19 +1032,19 @@ static void adj_sweeps(const double* W, const double* lam, const Geo& G, const P
     for (int A = 0; A < IN; ++A) {
         if (!G.active[A]) continue;
         if (march_waves(G, A, IN) < min_waves) {
-            if (A == 0) launch_adj_lanes<0, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
-            else launch_adj_lanes<1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+            if (A == 0) launch_adj_lanes<0, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
+            else launch_adj_lanes<1, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
         } else if (A == 0 && IN == 1) {
-            launch_adj_march<0, 1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+            launch_adj_march<0, 1, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
         } else if (A == 0) {
-            launch_adj_march<0, 2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+            launch_adj_march<0, 2, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
         } else {
-            launch_adj_march<1, 2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+            launch_adj_march<1, 2, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
         }
     }
-    if (IN == 0) launch_adj_lanes<0, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
-    else if (IN == 1) launch_adj_lanes<1, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
-    else launch_adj_lanes<2, STENCIL, RIEMANN>(W, lam, G, prm, ts, gW, st);
+    if (IN == 0) launch_adj_lanes<0, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
+    else if (IN == 1) launch_adj_lanes<1, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
+    else launch_adj_lanes<2, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
 }
 
 }  // namespace jfa

@@ -9,11 +9,11 @@ static std::atomic<int64_t> g_march_min_waves{2048};
 
 int64_t adjoint_march_min_waves() { return g_march_min_waves.load(std::memory_order_relaxed); }
 
-template <int STENCIL>
+template <int STENCIL, int VAR>
 static void adj_sweeps_r(int riemann, const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
                          double* gW, hipStream_t st) {
-    if (riemann == JFA_HLLC) adj_sweeps<STENCIL, JFA_HLLC>(W, lam, G, prm, ts, gW, st);
-    else adj_sweeps<STENCIL, JFA_RUSANOV>(W, lam, G, prm, ts, gW, st);
+    if (riemann == JFA_HLLC) adj_sweeps<STENCIL, JFA_HLLC, VAR>(W, lam, G, prm, ts, gW, st);
+    else adj_sweeps<STENCIL, JFA_RUSANOV, VAR>(W, lam, G, prm, ts, gW, st);
 }
 
 }  // namespace jfa
@@ -49,7 +49,11 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     if (rc) return rc;
     rc = check_scheme(scheme);
     if (rc) return rc;
-    if (scheme->dtype != JFA_F64 || scheme->recon_var != JFA_PRIMITIVE) return JFA_EUNSUPPORTED;
+    if (scheme->dtype != JFA_F64) return JFA_EUNSUPPORTED;
+    // PRIMITIVE, or CHAR-PRIMITIVE with the ARITHMETIC frozen state
+    if (scheme->recon_var != JFA_PRIMITIVE &&
+        !(scheme->recon_var == JFA_CHAR_PRIMITIVE && scheme->frozen_state == JFA_FROZEN_ARITHMETIC))
+        return JFA_EUNSUPPORTED;
     if (scheme->riemann == JFA_HLLC && scheme->signal_speed == JFA_SS_TORO) return JFA_EUNSUPPORTED;
     if (scheme->interpolation_limiter || scheme->flux_limiter) return JFA_EUNSUPPORTED;
     if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z) return JFA_EUNSUPPORTED;
@@ -80,8 +84,14 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     hipLaunchKernelGGL(adj_zero_kernel<0>, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kBlock), 0, st, gW, nz);
     const double* W = static_cast<const double*>(W_in);
     const double* lam = static_cast<const double*>(lam_out);
-    if (scheme->stencil == JFA_WENO5_JS) adj_sweeps_r<JFA_WENO5_JS>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
-    else adj_sweeps_r<JFA_WENO5_Z>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+    const bool ch = scheme->recon_var == JFA_CHAR_PRIMITIVE;
+    if (scheme->stencil == JFA_WENO5_JS) {
+        if (ch) adj_sweeps_r<JFA_WENO5_JS, JFA_CHAR_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+        else adj_sweeps_r<JFA_WENO5_JS, JFA_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+    } else {
+        if (ch) adj_sweeps_r<JFA_WENO5_Z, JFA_CHAR_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+        else adj_sweeps_r<JFA_WENO5_Z, JFA_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+    }
     double* X = nullptr;
     if (prm.dissip) {
         X = gW + nz;  // written whole by the cell kernel: no clearing

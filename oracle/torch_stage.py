@@ -8,7 +8,8 @@ this restatement stands in for jax.grad of the reference, which cannot be import
 The stage map is  f(U_in) = c0 U_in + c1 U_n + dt m rhs(prim(U_in))  on the interior, with the
 face halos filled from the interior as the reference fills them (the same formulas, operands
 and association order as ``jxf_oracle``; each function cites the reference file:line).
-Covered: WENO5-JS / WENO5-Z on the primitives, HLLC with the ARITHMETIC / RUSANOV / DAVIS /
+Covered: WENO5-JS / WENO5-Z on the primitives or, with the ARITHMETIC frozen state, on their
+characteristic projection (CHAR-PRIMITIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
 EINFELDT signal speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
 domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
@@ -22,8 +23,8 @@ from __future__ import annotations
 import torch
 
 from .jxf_oracle import (EPS64, FACE_AXIS, FACE_LOCATIONS, RK3_CONS_MULTIPLIER, RK3_TIMESTEP_MULTIPLIER,
-                         SPATIAL_STENCIL_EPS, VELOCITY_MINOR_AXES, WENO5_SHIFTS, Domain, Transport,
-                         _face_fill_slice, shift_slice)
+                         SPATIAL_STENCIL_EPS, VEL_ID_EIGS, VELOCITY_MINOR_AXES, WENO5_SHIFTS, WENO5_WINDOW_IDX,
+                         Domain, Transport, _face_fill_slice, shift_slice)
 
 _DR = (1 / 10, 6 / 10, 3 / 10)   # weno5_base.py:24
 _CR = ((1 / 3, -7 / 6, 11 / 6),  # weno5_base.py:25-27
@@ -160,6 +161,50 @@ def rusanov(W_L, W_R, U_L, U_R, axis, gamma, signal_speed=None):
 RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov}
 
 
+# ------------------------------------------------------------------ characteristic projection
+def _char_from_prim(x, P, c, cc, axis):
+    """get_characteristics_from_primitives, single phase (eigendecomposition.py:421-433)."""
+    v = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    e0, e1 = VEL_ID_EIGS[axis]
+    out = [None] * 5
+    out[0] = -0.5 / c * x[v] + 0.5 / (cc * P[0]) * x[4]
+    out[1] = x[0] - 1.0 / cc * x[4]
+    out[e0] = x[m0]
+    out[e1] = x[m1]
+    out[4] = 0.5 / c * x[v] + 0.5 / (cc * P[0]) * x[4]
+    return torch.stack(out)
+
+
+def _prim_from_char(x, P, c, cc, axis):
+    """get_primitives_from_characteristics, single phase (eigendecomposition.py:512-523)."""
+    v = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    e0, e1 = VEL_ID_EIGS[axis]
+    out = [None] * 5
+    out[0] = P[0] * (x[0] + x[4]) + x[1]
+    out[v] = c * (-x[0] + x[4])
+    out[m0] = x[e0]
+    out[m1] = x[e1]
+    out[4] = cc * P[0] * (x[0] + x[4])
+    return torch.stack(out)
+
+
+def char_primitive_states(W, dom: Domain, axis: int, stencil: str, gamma: float):
+    """HighOrderGodunov.reconstruct_xi, CHAR-PRIMITIVE (high_order_godunov.py:298-316,401-402) with
+    the ARITHMETIC frozen state (eigendecomposition.py:139-148,223-231): the 6-cell window of every
+    face (:75-118), projected with the frozen state of the face's two cells, WENO5 on the
+    characteristics (window slices weno5_base.py:31), projected back."""
+    win = [W[shift_slice(dom, axis, k)] for k in range(-3, 3)]
+    P = 0.5 * (win[2] + win[3])
+    c = torch.sqrt(gamma * P[4] / P[0])
+    cc = c * c
+    ch = [_char_from_prim(w, P, c, cc, axis) for w in win]
+    XL = WENO5[stencil](*[ch[i] for i in WENO5_WINDOW_IDX[0]])
+    XR = WENO5[stencil](*[ch[i] for i in WENO5_WINDOW_IDX[1]])
+    return _prim_from_char(XL, P, c, cc, axis), _prim_from_char(XR, P, c, cc, axis)
+
+
 # ------------------------------------------------------------------ dissipative fluxes (periodic)
 def _law(T, ref, T_ref, exp, C):
     """material.py:82-124: SUTHERLAND ref ((T_ref + C)/(T + C)) (T/T_ref)^1.5 (power3_2 = x sqrt(x),
@@ -235,14 +280,18 @@ def dissipative_flux_xi(W, dom: Domain, axis: int, tr: Transport, gamma: float):
 
 
 # ------------------------------------------------------------------ RHS and halos
-def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, transport=None):
+def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, transport=None,
+                variable: str = "PRIMITIVE"):
     """SpaceSolver.compute_rhs (space_solver.py:266,289-314,489,597-599) with PRIMITIVE
     reconstruction (high_order_godunov.py:267-280): 0.0 + sum over axes of (1/dx)(F[:-1] - F[1:]),
     the dissipative face flux added to the convective one (:565-587) with a transport."""
     rhs = 0.0
     for axis in dom.active:
-        W_L = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[0]])
-        W_R = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[1]])
+        if variable == "CHAR-PRIMITIVE":
+            W_L, W_R = char_primitive_states(W, dom, axis, stencil, gamma)
+        else:
+            W_L = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[0]])
+            W_R = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[1]])
         F = RIEMANN[riemann](W_L, W_R, cons_from_prim(W_L, gamma), cons_from_prim(W_R, gamma), axis, gamma,
                              signal_speed)
         if transport is not None:
@@ -285,7 +334,7 @@ def face_halos(W, dom: Domain, bcs: dict):
 
 
 def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, bcs: dict,
-              stage: int, dt: float, eps: float, transport=None):
+              stage: int, dt: float, eps: float, transport=None, variable: str = "PRIMITIVE"):
     """f(U_in interior) -> U_out interior: W = prim(U_in) on the interior of an eps-filled buffer,
     face halos of W, rhs, then c0 U_in + c1 U_n + dt m rhs (RK3.py:27-62, time_integrator.py:57)."""
     m = RK3_TIMESTEP_MULTIPLIER[stage]
@@ -295,7 +344,7 @@ def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_spe
     W = face_halos(W, dom, bcs)
     if transport is not None:
         assert all(bcs[f] == "PERIODIC" for f in FACE_LOCATIONS if FACE_AXIS[f] in dom.active)
-    rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma, transport)
+    rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma, transport, variable)
     if stage == 0:
         return U_int + (dt * m) * rhs
     return (c0 * U_int + c1 * Un_int) + (dt * m) * rhs

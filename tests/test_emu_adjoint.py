@@ -111,7 +111,7 @@ def run_vjp(emu, dom, s, W_in, lam_out, stage, dt, sweep="march"):
 
     grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
     transport = None if s.transport is None else _native.make_transport(**dataclasses.asdict(s.transport))
-    scheme = _native.make_scheme("PRIMITIVE", s.stencil, s.riemann, s.signal_speed, "ARITHMETIC", _native.F64,
+    scheme = _native.make_scheme(s.variable, s.stencil, s.riemann, s.signal_speed, s.frozen_state, _native.F64,
                                  s.gamma, s.eps_stencil, transport)
     bcs = _native.bc_array(s.bcs)
     nbytes = emu.jfa_stage_vjp_workspace_bytes(grid, scheme)

@@ -95,7 +95,7 @@ def _vjp_vs_autograd(dom, s, stage, seed_state, seed):
     lam_out[dom.interior] = lam_int
     grid = _native.make_grid(dom.cells, dom.nh, dom.dx)
     tr = None if s.transport is None else _native.make_transport(**dataclasses.asdict(s.transport))
-    scheme = _native.make_scheme("PRIMITIVE", s.stencil, s.riemann, s.signal_speed, "ARITHMETIC", _native.F64, 1.4,
+    scheme = _native.make_scheme(s.variable, s.stencil, s.riemann, s.signal_speed, s.frozen_state, _native.F64, 1.4,
                                  s.eps_stencil, tr)
     lib = _native.load()
     ws = torch.empty(lib.jfa_stage_vjp_workspace_bytes(grid, scheme), dtype=torch.uint8, device="cuda")
@@ -107,7 +107,7 @@ def _vjp_vs_autograd(dom, s, stage, seed_state, seed):
     _native.check(rc, "jfa_rk3_stage_vjp")
     U = torch.as_tensor(Uint).clone().requires_grad_(True)
     out = ts.stage_map(U, torch.as_tensor(Un_int), dom, s.stencil, s.riemann, s.signal_speed, s.gamma, s.bcs, stage,
-                       dt, o.EPS64, s.transport)
+                       dt, o.EPS64, s.transport, s.variable)
     (ref,) = torch.autograd.grad(out, U, torch.as_tensor(lam_int))
     ref = ref.numpy()
     got = host(li)[dom.interior]
@@ -143,11 +143,57 @@ def test_viscous_stage_vjp_device_matches_autograd(case, stage):
     _vjp_vs_autograd(dom, s, stage, 85 + case, 6000 + 10 * case + stage)
 
 
-def _kh_stepper(n, dt):
+def _kh_stepper(n, dt, variable="PRIMITIVE"):
     from jaxfluids_amd.adjoint import FixedDtStepper
 
     return FixedDtStepper((n, n, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), {f: "PERIODIC" for f in FACES}, dt,
-                          nh=3, stencil="WENO5-JS", riemann="HLLC", signal_speed="EINFELDT")
+                          nh=3, stencil="WENO5-JS", riemann="HLLC", signal_speed="EINFELDT", variable=variable)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("stage", [0, 2])
+def test_char_primitive_stage_vjp_device_matches_autograd(case, stage):
+    """CHAR-PRIMITIVE reconstruction (ARITHMETIC frozen state): every entry of the device J^T lam
+    against torch.autograd of the restatement."""
+    say(f"[char stage vjp vs autograd] case {case} stage {stage}")
+    cells, bcs, (stencil, riemann, speed) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="CHAR-PRIMITIVE", riemann=riemann, signal_speed=speed,
+                frozen_state="ARITHMETIC", bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, stage, 90 + case, 8000 + 10 * case + stage)
+
+
+def test_char_primitive_rollout_gradient_against_device_fd():
+    """Kelvin-Helmholtz 48^2, 24 fixed-dt steps with CHAR-PRIMITIVE reconstruction through
+    RK3Rollout (4 binomial slots): the gradient along a Fourier-mode direction against Richardson
+    central differences of the device forward, to 1e-6."""
+    from jaxfluids_amd.adjoint import kinetic_energy, rollout
+    from jaxfluids_amd.workloads import fourier_velocity_direction, kelvin_helmholtz_primitives
+
+    say("[char kh gradient vs device fd] 48^2 x 24 steps")
+    n, nsteps = 48, 24
+    W0 = kelvin_helmholtz_primitives(n, seed=4, thickness=0.06)
+    dom = o.Domain((n, n, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), nh=3)
+    s = o.Setup(gamma=1.4, stencil="WENO5-JS", variable="CHAR-PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT",
+                bcs={f: "PERIODIC" for f in FACES})
+    dt = 0.5 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    dV = 1.0 / n ** 2
+    st = _kh_stepper(n, dt, "CHAR-PRIMITIVE")
+    W0d = dev(W0).requires_grad_(True)
+    L = kinetic_energy(rollout(W0d, st, nsteps, snapshots=4), dV)
+    L.backward()
+    g = W0d.grad.detach()
+    dd = dev(fourier_velocity_direction((n, n, 1), seed=11))
+
+    def loss(h):
+        with torch.no_grad():
+            return float(kinetic_energy(rollout(dev(W0) + h * dd, st, nsteps, snapshots=0), dV))
+
+    D = lambda h: (loss(h) - loss(-h)) / (2 * h)
+    fd = (4.0 * D(0.5e-4) - D(1e-4)) / 3.0
+    an = float(torch.sum(g * dd))
+    say(f"  fd {fd:.15e} adjoint {an:.15e} rel {abs(fd - an) / abs(an):.2e}")
+    assert abs(fd - an) <= 1e-6 * abs(an)
 
 
 def test_rollout_gradient_against_oracle_fd():

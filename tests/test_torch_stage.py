@@ -22,11 +22,11 @@ from test_emu_adjoint import CASES, FACES, VISCOUS_CASES, emu, run_vjp, smooth_s
 TOL = 1e-11
 
 
-def _setup(case):
+def _setup(case, variable="PRIMITIVE"):
     cells, bcs, (stencil, riemann, speed) = CASES[case]
     dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
-    s = o.Setup(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann=riemann, signal_speed=speed,
-                bcs=dict(zip(FACES, bcs)))
+    s = o.Setup(gamma=1.4, stencil=stencil, variable=variable, riemann=riemann, signal_speed=speed,
+                frozen_state="ARITHMETIC", bcs=dict(zip(FACES, bcs)))
     return dom, s
 
 
@@ -40,7 +40,7 @@ def _viscous_setup(case):
 
 def _torch_map(dom, s, stage, dt, Un_int):
     return lambda U: ts.stage_map(U, torch.as_tensor(Un_int), dom, s.stencil, s.riemann, s.signal_speed, s.gamma,
-                                  s.bcs, stage, dt, o.EPS64, s.transport)
+                                  s.bcs, stage, dt, o.EPS64, s.transport, s.variable)
 
 
 def _compare(emu, dom, s, stage, sweep, seed_state, seed):  # noqa: F811
@@ -82,9 +82,10 @@ def test_tie_conventions_are_jax_conventions():
         assert a.grad[1].item() + b.grad[1].item() == 1.0
 
 
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_restatement_forward_matches_oracle(case):
-    dom, s = _setup(case)
+def test_restatement_forward_matches_oracle(case, variable):
+    dom, s = _setup(case, variable)
     W0 = smooth_state(dom, 50 + case)
     Uint = o.cons_from_prim(W0, s.gamma)
     Un_int = 1.01 * Uint
@@ -102,6 +103,16 @@ def test_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
     """lam_in = J^T lam_out over the interior, and lam_n = c1 lam_out, entry by entry."""
     dom, s = _setup(case)
     _compare(emu, dom, s, stage, sweep, 50 + case, 3000 + 10 * case + stage)
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("stage", [0, 1, 2])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_char_primitive_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
+    """CHAR-PRIMITIVE reconstruction (ARITHMETIC frozen state): the reverse of the projection, of
+    the frozen state's dependence on the face's two cells and of the back projection."""
+    dom, s = _setup(case, "CHAR-PRIMITIVE")
+    _compare(emu, dom, s, stage, sweep, 50 + case, 7000 + 10 * case + stage)
 
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
