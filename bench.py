@@ -17,7 +17,8 @@ timed on a bounded sample of the same workload.
 ensemble: the batched-ensemble workload (64 uneven 2-D Riemann problems, one launch per RK3
 stage for all cases) against plain per-case launches of the same stages.
 adjoint: the Kelvin-Helmholtz 256^2 adjoint (500 fixed-dt steps): backward / forward wall time
-and a finite-difference check of the gradient.
+and a finite-difference check of the gradient, with primitive and (char_primitive) characteristic
+reconstruction.
 adjoint_viscous: the viscous Taylor-Green 128^3 adjoint (Re 1600, 1000 fixed-dt steps, a checkpoint
 every 50 steps, enstrophy loss): the same measurements, with binomial checkpointing in 20 slots
 beside it.
@@ -349,6 +350,22 @@ def bench_adjoint(nsteps: int = 500, n: int = 256, slots: int = 20, fd_check: bo
     if fd_check:
         out["fd_check"] = fd_check_along(lambda x: kinetic_energy(rollout(x, st, nsteps, 0), dV), W0d,
                                          grads["store_every_step"], fourier_velocity_direction((n, n, 1), seed=1))
+    # the same rollout with CHAR-PRIMITIVE reconstruction (the TGV line's variable mode), every step stored
+    sc = FixedDtStepper((n, n, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), faces, dt, nh=3, stencil="WENO5-JS",
+                        riemann="HLLC", signal_speed="EINFELDT", variable="CHAR-PRIMITIVE")
+    with torch.no_grad():
+        rollout(W0d, sc, 5, 0)
+        t_fc, _ = timed(lambda: rollout(W0d, sc, nsteps, 0))
+    x = W0d.clone().requires_grad_(True)
+    t_f, lo = timed(lambda: kinetic_energy(rollout(x, sc, nsteps), dV))
+    t_b, _ = timed(lambda: lo.backward())
+    out["char_primitive"] = {"forward_s": round(t_fc, 4), "backward_s": round(t_b, 4),
+                             "backward_over_forward": round(t_b / t_fc, 3),
+                             "taped_forward_over_forward": round(t_f / t_fc, 3)}
+    if fd_check:
+        out["char_primitive"]["fd_check"] = fd_check_along(
+            lambda y: kinetic_energy(rollout(y, sc, nsteps, 0), dV), W0d, x.grad.detach(),
+            fourier_velocity_direction((n, n, 1), seed=1))
     return out
 
 
