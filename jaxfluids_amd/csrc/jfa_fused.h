@@ -540,14 +540,15 @@ int fused_stage_impl(const void* W_in, const void* U_in, const void* U_n, void* 
     int64_t nparts = 0;
     // the WAVESPEED slots follow the CFL partials (inside the workspace's 256-byte tail)
     unsigned long long* slots = reinterpret_cast<unsigned long long*>(part + fused_partials(G));
-    // Non-periodic dissipative fluxes read the halos (periodic ones wrap onto the interior).  The
-    // inner kernel's scatter fills the face halos of the new state; with dissipative fluxes the
-    // edge halos follow (fill_edge_halos_material, halo_manager.py:125-129,183-186).
+    // Non-periodic dissipative fluxes read the halos (periodic ones wrap onto the interior, so no
+    // stencil reads an edge halo there).  The inner kernel's scatter fills the face halos of the
+    // new state; on non-periodic domains the edge halos follow (fill_edge_halos_material,
+    // halo_manager.py:125-129,183-186).
     const Params base = make_params(s);
     const int diss_halos = dissipative_halos(base, G, bc) ? 1 : 0;
     int rc = run_sweeps<T>(s, W_in, U_in, G, R, ia, dt_dev, dt_host, st, &nparts, slots, diss_halos);
     if (rc) return rc;
-    if (base.dissip) {
+    if (diss_halos) {
         launch_edge_halos<T>(static_cast<T*>(W_out), static_cast<T*>(U_out), G, bc, base, st);
         rc = check_launch();
         if (rc) return rc;

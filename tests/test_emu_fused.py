@@ -95,8 +95,10 @@ def check_step(emu, dom, s, W0, transport=None, dt_scale=1.0):
     assert np.all(np.isfinite(Wr[I])) and np.all(Wr[4][I] > 0)
     Wg, Ug, dtn = fused_step(emu, dom, s, W, U, dt, transport)
     assert rel(Wg[I], Wr[I]) < TOL and rel(Ug[I], Ur[I]) < TOL, (rel(Wg[I], Wr[I]), rel(Ug[I], Ur[I]))
-    # face halos written by the last stage; with dissipative fluxes the edge halos too
-    for t in halo_regions(dom, edges=transport is not None):
+    # face halos written by the last stage; with dissipative fluxes on a non-periodic domain the
+    # edge halos too (periodic ones wrap onto the interior and leave the edge halos unwritten)
+    periodic = all(s.bcs[f] == "PERIODIC" for f in FACES if o.FACE_AXIS[f] in dom.active)
+    for t in halo_regions(dom, edges=transport is not None and not periodic):
         assert rel(Wg[t], Wr[t]) < TOL and rel(Ug[t], Ur[t]) < TOL, t
     assert abs(dtn - o.compute_dt(Wr, dom, s)) <= 1e-14 * dt
 
