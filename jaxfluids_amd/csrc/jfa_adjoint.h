@@ -141,36 +141,67 @@ __device__ __forceinline__ void sound_speed_vjp(double c, double rho, double gam
     grho -= gc * c / (2.0 * rho);
 }
 
+// Shares of a min / max cotangent g for its operands x, y: all of it to the one the forward
+// takes, half each on a tie (jax's balanced rule for lax.min / lax.max; a NaN operand gets the
+// halves too).
+__device__ __forceinline__ void minmax_vjp(bool is_min, double x, double y, double g, double& gx, double& gy) {
+    const bool x_wins = is_min ? x < y : x > y, y_wins = is_min ? y < x : y > x;
+    if (x_wins) {
+        gx += g;
+    } else if (y_wins) {
+        gy += g;
+    } else {
+        gx += 0.5 * g;
+        gy += 0.5 * g;
+    }
+}
+
 // Wave-speed estimates (signal_speeds.py): adds the VJP of (SL, SR) to u, a (sound speed), rho.
-// The branch of each min/max is the one the forward takes (first operand on ties).
 __device__ __forceinline__ void signal_speeds_vjp(int kind, double uL, double uR, double aL, double aR, double rL,
                                                   double rR, double gSL, double gSR, double& guL, double& guR,
                                                   double& gaL, double& gaR, double& grL, double& grR) {
     switch (kind) {
         case JFA_SS_ARITHMETIC: {
             const double um = 0.5 * (uL + uR), am = 0.5 * (aL + aR);
-            double gum = 0.0, gam = 0.0;
-            if (um - am <= uL - aL) { gum += gSL; gam -= gSL; } else { guL += gSL; gaL -= gSL; }
-            if (um + am >= uR + aR) { gum += gSR; gam += gSR; } else { guR += gSR; gaR += gSR; }
+            double gum = 0.0, gam = 0.0, g1 = 0.0, g2 = 0.0;
+            minmax_vjp(true, um - am, uL - aL, gSL, g1, g2);  // S_L = min(um - am, uL - aL)
+            gum += g1;
+            gam -= g1;
+            guL += g2;
+            gaL -= g2;
+            g1 = g2 = 0.0;
+            minmax_vjp(false, um + am, uR + aR, gSR, g1, g2);  // S_R = max(um + am, uR + aR)
+            gum += g1;
+            gam += g1;
+            guR += g2;
+            gaR += g2;
             guL += 0.5 * gum;
             guR += 0.5 * gum;
             gaL += 0.5 * gam;
             gaR += 0.5 * gam;
         } break;
         case JFA_SS_RUSANOV: {
-            const double gsp = gSR - gSL;
-            if (fabs(uL) + aL >= fabs(uR) + aR) {
-                guL += gsp * (uL > 0.0 ? 1.0 : (uL < 0.0 ? -1.0 : 0.0));
-                gaL += gsp;
-            } else {
-                guR += gsp * (uR > 0.0 ? 1.0 : (uR < 0.0 ? -1.0 : 0.0));
-                gaR += gsp;
-            }
+            double g1 = 0.0, g2 = 0.0;  // S_+ = max(|uL| + aL, |uR| + aR), (S_L, S_R) = (-S_+, S_+)
+            minmax_vjp(false, fabs(uL) + aL, fabs(uR) + aR, gSR - gSL, g1, g2);
+            guL += g1 * (uL > 0.0 ? 1.0 : (uL < 0.0 ? -1.0 : 0.0));
+            gaL += g1;
+            guR += g2 * (uR > 0.0 ? 1.0 : (uR < 0.0 ? -1.0 : 0.0));
+            gaR += g2;
         } break;
-        case JFA_SS_DAVIS:
-            if (uL - aL <= uR - aR) { guL += gSL; gaL -= gSL; } else { guR += gSL; gaR -= gSL; }
-            if (uL + aL >= uR + aR) { guL += gSR; gaL += gSR; } else { guR += gSR; gaR += gSR; }
-            break;
+        case JFA_SS_DAVIS: {
+            double g1 = 0.0, g2 = 0.0;
+            minmax_vjp(true, uL - aL, uR - aR, gSL, g1, g2);
+            guL += g1;
+            gaL -= g1;
+            guR += g2;
+            gaR -= g2;
+            g1 = g2 = 0.0;
+            minmax_vjp(false, uL + aL, uR + aR, gSR, g1, g2);
+            guL += g1;
+            gaL += g1;
+            guR += g2;
+            gaR += g2;
+        } break;
         default: {  // EINFELDT
             const double sL = sqrt(rL), sR = sqrt(rR);
             const double od = 1.0 / (sL + sR);
@@ -180,9 +211,18 @@ __device__ __forceinline__ void signal_speeds_vjp(int kind, double uL, double uR
             const double du = uR - uL;
             const double t1 = sL * aL * aL + sR * aR * aR;
             const double db = sqrt(t1 * od + eta2 * du * du);
-            double gub = 0.0, gdb = 0.0;
-            if (ub - db <= uL - aL) { gub += gSL; gdb -= gSL; } else { guL += gSL; gaL -= gSL; }
-            if (ub + db >= uR + aR) { gub += gSR; gdb += gSR; } else { guR += gSR; gaR += gSR; }
+            double gub = 0.0, gdb = 0.0, g1 = 0.0, g2 = 0.0;
+            minmax_vjp(true, ub - db, uL - aL, gSL, g1, g2);
+            gub += g1;
+            gdb -= g1;
+            guL += g2;
+            gaL -= g2;
+            g1 = g2 = 0.0;
+            minmax_vjp(false, ub + db, uR + aR, gSR, g1, g2);
+            gub += g1;
+            gdb += g1;
+            guR += g2;
+            gaR += g2;
             const double gQ = db > 0.0 ? gdb / (2.0 * db) : 0.0;
             const double gt1 = gQ * od;
             double god = gQ * t1;
@@ -242,7 +282,8 @@ __device__ __forceinline__ void hllc_star_vjp(const double W[5], const double U[
         gus[i] = Sc * mu[i];
         gU[i] -= Sc * mu[i];
     }
-    if (LEFT ? SK < 0.0 : SK > 0.0) gSK += gSc;
+    double gzero = 0.0;  // Sc = min(SK, 0) (left) / max(SK, 0) (right)
+    minmax_vjp(LEFT, SK, 0.0, gSc, gSK, gzero);
     const double gpre = gus[0] + gus[v] * Ss + gus[m0] * W[m0] + gus[m1] * W[m1];
     gSs += gus[v] * pre;
     gW[m0] += gus[m0] * pre;
@@ -328,8 +369,7 @@ __device__ __forceinline__ void rusanov_vjp(const double WL[5], const double WR[
     constexpr int v = Ax<AXIS>::v;
     const double g = prm.gamma;
     const double cL = sqrt(g * WL[4] / WL[0]), cR = sqrt(g * WR[4] / WR[0]);
-    const bool left = fabs(WL[v]) + cL >= fabs(WR[v]) + cR;
-    const double alpha = left ? fabs(WL[v]) + cL : fabs(WR[v]) + cR;
+    const double alpha = fmax(fabs(WL[v]) + cL, fabs(WR[v]) + cR);
     double gUL[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gUR[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, half[5];
     double galpha = 0.0;
 #pragma unroll
@@ -341,14 +381,12 @@ __device__ __forceinline__ void rusanov_vjp(const double WL[5], const double WR[
     }
     physical_flux_vjp<AXIS>(WL, UL, half, gWL, gUL);
     physical_flux_vjp<AXIS>(WR, UR, half, gWR, gUR);
-    double gcL = 0.0, gcR = 0.0;
-    if (left) {
-        gWL[v] += galpha * (WL[v] > 0.0 ? 1.0 : (WL[v] < 0.0 ? -1.0 : 0.0));
-        gcL += galpha;
-    } else {
-        gWR[v] += galpha * (WR[v] > 0.0 ? 1.0 : (WR[v] < 0.0 ? -1.0 : 0.0));
-        gcR += galpha;
-    }
+    double gcL = 0.0, gcR = 0.0, g1 = 0.0, g2 = 0.0;
+    minmax_vjp(false, fabs(WL[v]) + cL, fabs(WR[v]) + cR, galpha, g1, g2);  // alpha = max(., .)
+    gWL[v] += g1 * (WL[v] > 0.0 ? 1.0 : (WL[v] < 0.0 ? -1.0 : 0.0));
+    gcL += g1;
+    gWR[v] += g2 * (WR[v] > 0.0 ? 1.0 : (WR[v] < 0.0 ? -1.0 : 0.0));
+    gcR += g2;
     sound_speed_vjp(cL, WL[0], g, gcL, gWL[0], gWL[4]);
     sound_speed_vjp(cR, WR[0], g, gcR, gWR[0], gWR[4]);
     cons_vjp(WL, gUL, prm, gWL);

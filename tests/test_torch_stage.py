@@ -134,3 +134,4 @@ def test_viscous_forward_matches_oracle():
             ref, _ = stage_map(dom, s, stage, 1e-3, 1.01 * Uint)(Uint)
             got = _torch_map(dom, s, stage, 1e-3, 1.01 * Uint)(torch.as_tensor(Uint)).numpy()
             assert np.max(np.abs(got - ref)) <= 1e-15 * np.max(np.abs(ref)), (case, stage)
+
