@@ -156,11 +156,53 @@ __device__ __forceinline__ void minmax_vjp(bool is_min, double x, double y, doub
     }
 }
 
-// Wave-speed estimates (signal_speeds.py): adds the VJP of (SL, SR) to u, a (sound speed), rho.
+// Wave-speed estimates (signal_speeds.py): adds the VJP of (SL, SR) to u, a (sound speed), rho
+// and (TORO) p.
 __device__ __forceinline__ void signal_speeds_vjp(int kind, double uL, double uR, double aL, double aR, double rL,
-                                                  double rR, double gSL, double gSR, double& guL, double& guR,
-                                                  double& gaL, double& gaR, double& grL, double& grR) {
+                                                  double rR, double pL, double pR, double gamma, double gSL,
+                                                  double gSR, double& guL, double& guR, double& gaL, double& gaR,
+                                                  double& grL, double& grR, double& gpL, double& gpR) {
     switch (kind) {
+        case JFA_SS_TORO: {  // signal_speeds.py:135-157, estimate_pressure :201-231
+            const double rho_bar = 0.5 * (rL + rR), a_bar = 0.5 * (aL + aR);
+            const double p_pvrs = 0.5 * (pL + pR) - 0.5 * (uR - uL) * rho_bar * a_bar;
+            const double ps = fmax(0.0, p_pvrs);
+            const double g_ = (gamma + 1.0) * 0.5 / gamma;
+            // q_K = 1 (p* <= p_K) or sqrt(1 + g_ (p*/p_K - 1)); S_L = uL - aL qL, S_R = uR + aR qR
+            double gps = 0.0;
+            guL += gSL;
+            guR += gSR;
+            if (ps > pL) {
+                const double qL = sqrt(1.0 + g_ * (ps / pL - 1.0));
+                gaL -= gSL * qL;
+                const double gq = -gSL * aL * g_ / (2.0 * qL * pL);
+                gps += gq;
+                gpL -= gq * ps / pL;
+            } else {
+                gaL -= gSL;
+            }
+            if (ps > pR) {
+                const double qR = sqrt(1.0 + g_ * (ps / pR - 1.0));
+                gaR += gSR * qR;
+                const double gq = gSR * aR * g_ / (2.0 * qR * pR);
+                gps += gq;
+                gpR -= gq * ps / pR;
+            } else {
+                gaR += gSR;
+            }
+            double gzero = 0.0, gpv = 0.0;
+            minmax_vjp(false, 0.0, p_pvrs, gps, gzero, gpv);  // p* = max(0, p_pvrs)
+            gpL += 0.5 * gpv;
+            gpR += 0.5 * gpv;
+            const double du = uR - uL;
+            guR -= 0.5 * rho_bar * a_bar * gpv;
+            guL += 0.5 * rho_bar * a_bar * gpv;
+            const double grb = -0.5 * du * a_bar * gpv, gab = -0.5 * du * rho_bar * gpv;
+            grL += 0.5 * grb;
+            grR += 0.5 * grb;
+            gaL += 0.5 * gab;
+            gaR += 0.5 * gab;
+        } break;
         case JFA_SS_ARITHMETIC: {
             const double um = 0.5 * (uL + uR), am = 0.5 * (aL + aR);
             double gum = 0.0, gam = 0.0, g1 = 0.0, g2 = 0.0;
@@ -353,8 +395,8 @@ __device__ __forceinline__ void hllc_vjp(const double WL[5], const double WR[5],
     gSR += gaR * WR[0];
     gWR[v] -= gaR * WR[0];
     double gcL = 0.0, gcR = 0.0;
-    signal_speeds_vjp(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], gSL, gSR, gWL[v], gWR[v], gcL, gcR,
-                      gWL[0], gWR[0]);
+    signal_speeds_vjp(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], WL[4], WR[4], g, gSL, gSR, gWL[v],
+                      gWR[v], gcL, gcR, gWL[0], gWR[0], gWL[4], gWR[4]);
     sound_speed_vjp(cL, WL[0], g, gcL, gWL[0], gWL[4]);
     sound_speed_vjp(cR, WR[0], g, gcR, gWR[0], gWR[4]);
     cons_vjp(WL, gUL, prm, gWL);

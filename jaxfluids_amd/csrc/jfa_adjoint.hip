@@ -54,7 +54,6 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     if (scheme->recon_var != JFA_PRIMITIVE &&
         !(scheme->recon_var == JFA_CHAR_PRIMITIVE && scheme->frozen_state == JFA_FROZEN_ARITHMETIC))
         return JFA_EUNSUPPORTED;
-    if (scheme->riemann == JFA_HLLC && scheme->signal_speed == JFA_SS_TORO) return JFA_EUNSUPPORTED;
     if (scheme->interpolation_limiter || scheme->flux_limiter) return JFA_EUNSUPPORTED;
     if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z) return JFA_EUNSUPPORTED;
     if (scheme->riemann != JFA_HLLC && scheme->riemann != JFA_RUSANOV) return JFA_EUNSUPPORTED;

@@ -10,7 +10,7 @@ face halos filled from the interior as the reference fills them (the same formul
 and association order as ``jxf_oracle``; each function cites the reference file:line).
 Covered: WENO5-JS / WENO5-Z on the primitives or, with the ARITHMETIC frozen state, on their
 characteristic projection (CHAR-PRIMITIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
-EINFELDT signal speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
+EINFELDT / TORO signal speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
 domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
 Derivative conventions at non-smooth points follow torch's, which are jax's for the operations
@@ -101,8 +101,22 @@ def physical_flux_xi(W, U, axis):
 
 
 # ------------------------------------------------------------------ signal speeds, Riemann solvers
-def _speeds(kind, u_L, u_R, a_L, a_R, rho_L, rho_R):
-    """signal_speeds.py:10-133 (ARITHMETIC :10-29, RUSANOV :31-49, DAVIS :51-69, EINFELDT :109-133)."""
+def _speeds(kind, u_L, u_R, a_L, a_R, rho_L, rho_R, p_L=None, p_R=None, gamma=None):
+    """signal_speeds.py:10-157 (ARITHMETIC :10-29, RUSANOV :31-49, DAVIS :51-69, EINFELDT :109-133,
+    TORO :135-157 with estimate_pressure :201-231)."""
+    if kind == "TORO":
+        rho_bar, a_bar = 0.5 * (rho_L + rho_R), 0.5 * (a_L + a_R)
+        p_pvrs = 0.5 * (p_L + p_R) - 0.5 * (u_R - u_L) * rho_bar * a_bar
+        p_star = torch.maximum(torch.zeros_like(p_pvrs), p_pvrs)
+        g_ = (gamma + 1) * 0.5 / gamma
+
+        def q(p_K):  # 1.0 * (p* <= p_K) + sqrt(1 + g_ (p*/p_K - 1)) * (p* > p_K); the shock branch
+            # is evaluated where it is taken only, so its NaN elsewhere does not reach the gradient
+            shock = p_star > p_K
+            arg = torch.where(shock, 1 + g_ * (p_star / p_K - 1), torch.ones_like(p_K))
+            return torch.where(shock, torch.sqrt(arg), torch.ones_like(p_K))
+
+        return u_L - a_L * q(p_L), u_R + a_R * q(p_R)
     if kind == "ARITHMETIC":
         u_mean, a_mean = 0.5 * (u_L + u_R), 0.5 * (a_L + a_R)
         return torch.minimum(u_mean - a_mean, u_L - a_L), torch.maximum(u_mean + a_mean, u_R + a_R)
@@ -139,7 +153,7 @@ def hllc(W_L, W_R, U_L, U_R, axis, gamma, signal_speed):
     v = axis + 1
     c_L = torch.sqrt(gamma * W_L[4] / W_L[0])
     c_R = torch.sqrt(gamma * W_R[4] / W_R[0])
-    S_L, S_R = _speeds(signal_speed, W_L[v], W_R[v], c_L, c_R, W_L[0], W_R[0])
+    S_L, S_R = _speeds(signal_speed, W_L[v], W_R[v], c_L, c_R, W_L[0], W_R[0], W_L[4], W_R[4], gamma)
     rdu_L = W_L[0] * (S_L - W_L[v])
     rdu_R = W_R[0] * (S_R - W_R[v])
     S_star = ((W_R[4] - W_L[4]) + (W_L[v] * rdu_L - W_R[v] * rdu_R)) / (rdu_L - rdu_R)

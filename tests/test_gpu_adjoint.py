@@ -163,6 +163,18 @@ def test_char_primitive_stage_vjp_device_matches_autograd(case, stage):
     _vjp_vs_autograd(dom, s, stage, 90 + case, 8000 + 10 * case + stage)
 
 
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("case", [0, 1, 6])
+def test_toro_speeds_stage_vjp_device_matches_autograd(case, variable):
+    """HLLC with Toro's wave speeds: every entry of the device J^T lam against autograd."""
+    say(f"[toro stage vjp vs autograd] case {case} {variable}")
+    cells, bcs, (stencil, _, _) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable=variable, riemann="HLLC", signal_speed="TORO",
+                frozen_state="ARITHMETIC", bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, 2, 95 + case, 9500 + 10 * case)
+
+
 def test_char_primitive_rollout_gradient_against_device_fd():
     """Kelvin-Helmholtz 48^2, 24 fixed-dt steps with CHAR-PRIMITIVE reconstruction through
     RK3Rollout (4 binomial slots): the gradient along a Fourier-mode direction against Richardson

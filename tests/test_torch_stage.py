@@ -116,6 +116,17 @@ def test_char_primitive_stage_vjp_matches_autograd(emu, case, stage, sweep):  # 
 
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("case", [0, 1, 3, 6])
+def test_toro_speeds_stage_vjp_matches_autograd(emu, case, variable, sweep):  # noqa: F811
+    """HLLC with Toro's pressure-based wave speeds (the PVRS estimate, its max with 0 and the
+    shock / rarefaction factor q_K) on the boundary cases' grids."""
+    dom, s = _setup(case, variable)
+    s.riemann, s.signal_speed = "HLLC", "TORO"
+    _compare(emu, dom, s, 2, sweep, 50 + case, 9000 + 10 * case)
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
 @pytest.mark.parametrize("stage", [0, 2])
 @pytest.mark.parametrize("case", range(len(VISCOUS_CASES)))
 def test_viscous_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
