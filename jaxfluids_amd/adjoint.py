@@ -154,8 +154,8 @@ def cons_vjp(W: torch.Tensor, gU: torch.Tensor, gamma: float) -> torch.Tensor:
 
 # ------------------------------------------------------------------ stepper
 class FixedDtStepper:
-    """Fixed-dt RK3 steps through the fused HIP stage and their VJPs, FP64, with PRIMITIVE or
-    CHAR-PRIMITIVE reconstruction (the latter with the ARITHMETIC frozen state).
+    """Fixed-dt RK3 steps through the fused HIP stage and their VJPs, FP64, with PRIMITIVE,
+    CONSERVATIVE or CHAR-PRIMITIVE reconstruction (the latter with the ARITHMETIC frozen state).
     ``cells``/``ranges``/``bcs`` as in the case setup; the scheme strings are the reference's
     registry names; ``transport`` (``_native.make_transport``) adds the viscous and heat fluxes
     on periodic domains."""
@@ -164,7 +164,7 @@ class FixedDtStepper:
                  riemann: str = "HLLC", signal_speed: str = "EINFELDT", gamma: float = 1.4,
                  transport: Optional[_native.JfaTransport] = None, device=None,
                  variable: str = "PRIMITIVE") -> None:
-        if variable not in ("PRIMITIVE", "CHAR-PRIMITIVE"):
+        if variable not in ("PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE"):
             raise NotImplementedError(f"reconstruction_variable {variable} has no reverse mode")
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
         self.cells = tuple(int(c) for c in cells)

@@ -533,6 +533,12 @@ __device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const 
             gr[i] = gWR[i];
         }
         fz[0] = fz[1] = fz[2] = 0.0;
+    } else if constexpr (VAR == JFA_CONSERVATIVE) {
+        // W_L = prim(U_L) of the reconstructed conservatives (high_order_godunov.py:282-296);
+        // gWL already holds the flux's whole dependence on the face state (its U_L = cons(W_L))
+        prim_vjp(WL, gWL, prm, gl);
+        prim_vjp(WR, gWR, prm, gr);
+        fz[0] = fz[1] = fz[2] = 0.0;
     } else {
         constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
         const CharK K = char_frozen(win[2], win[3], prm);
@@ -576,6 +582,15 @@ __device__ __forceinline__ void face_state_vjp_at(const double* __restrict__ W, 
     for (int k = 0; k < 6; ++k)
 #pragma unroll
         for (int i = 0; i < 5; ++i) win[k][i] = ld(W, i * fs + br + (int64_t)(k - 3) * sa);
+    if constexpr (VAR == JFA_CONSERVATIVE) {  // the forward reconstructs the conservatives
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double u[5];
+            cons_from_prim(win[k], prm, u);
+#pragma unroll
+            for (int i = 0; i < 5; ++i) win[k][i] = u[i];
+        }
+    }
     face_state_vjp<AXIS, STENCIL, RIEMANN, VAR>(win, mu, prm, gl, gr, fz);
 }
 
@@ -606,6 +621,44 @@ __device__ __forceinline__ void load_var6(const double* __restrict__ W, int64_t 
                                           double x[6]) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) x[k] = ld(W, i * fs + br + (int64_t)(k - 3) * sa);
+}
+
+// CONSERVATIVE, second part for conserved variable i of the face: its window U_i of the six
+// cells (cons of the primitive window, as the forward's U buffer holds it up to rounding), the
+// WENO VJP of its two reconstructions into g6, and d U_i / d W of each cell in dU[j][.], so the
+// caller adds g6[j] dU[j][t] to the cotangent of W_j[t] (U = cons(W), equation_manager.py:93-101).
+// d U_i / d W_t is identically zero unless this holds
+__host__ __device__ constexpr bool cons_dep(int i, int t) { return i == 4 || t == 0 || t == i; }
+
+template <int STENCIL>
+__device__ __forceinline__ void cons_window_vjp(const double* __restrict__ W, int64_t br, int64_t sa, int64_t fs,
+                                                int i, const Params& prm, double gl, double gr, double g6[6],
+                                                double dU[6][5]) {
+    double x6[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double w[5], u[5];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) w[t] = ld(W, t * fs + br + (int64_t)(j - 3) * sa);
+        cons_from_prim(w, prm, u);
+        x6[j] = u[i];
+        // d U_i / d W: rho -> (1, 0, 0, 0, 0); rho u_k -> (u_k, rho e_k); E -> (|u|^2/2, rho u, 1/(gamma-1))
+#pragma unroll
+        for (int t = 0; t < 5; ++t) dU[j][t] = 0.0;
+        if (i == 0) {
+            dU[j][0] = 1.0;
+        } else if (i < 4) {
+            dU[j][0] = w[i];
+            dU[j][i] = w[0];
+        } else {
+            dU[j][0] = 0.5 * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+            dU[j][1] = w[0] * w[1];
+            dU[j][2] = w[0] * w[2];
+            dU[j][3] = w[0] * w[3];
+            dU[j][4] = prm.inv_gm1;
+        }
+    }
+    window_vjp_var<STENCIL>(x6, gl, gr, prm.eps, g6);
 }
 
 // ------------------------------------------------------------------ dissipative fluxes
@@ -768,6 +821,25 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
                     }
                 }
             }
+        } else if constexpr (VAR == JFA_CONSERVATIVE) {
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                double g6[6], dU[6][5];
+                cons_window_vjp<STENCIL>(W, br, sa, fs, i, prm, gl[i], gr[i], g6, dU);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const int cell = f - 3 + k;
+                    if (cell >= o0 && cell < o1) {
+#pragma unroll
+                        for (int t = 0; t < 5; ++t) {
+                            if (!cons_dep(i, t)) continue;
+                            double* p = gW + t * fs + br + (int64_t)(k - 3) * sa;
+                            JFA_ACCESS(p, sizeof(double));
+                            *p += g6[k] * dU[k][t];
+                        }
+                    }
+                }
+            }
         } else {
             double wl[5], wr[5];
 #pragma unroll
@@ -857,6 +929,19 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __res
             window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
 #pragma unroll
             for (int k = 0; k < 6; ++k) s[i] += __shfl_up(o6[k], (unsigned)k, 64);
+        }
+    } else if constexpr (VAR == JFA_CONSERVATIVE) {
+        const int64_t bq = vface ? br : brow;  // lanes without a face: face 0, zero cotangents
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            double g6[6], dU[6][5];
+            cons_window_vjp<STENCIL>(W, bq, sa, fs, i, prm, gl[i], gr[i], g6, dU);
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                if (!cons_dep(i, t)) continue;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) s[t] += __shfl_up(g6[k] * dU[k][t], (unsigned)k, 64);
+            }
         }
     } else {
         // lanes without a face evaluate face 0 of the row (its window lies in the interior and

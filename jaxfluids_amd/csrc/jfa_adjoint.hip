@@ -50,8 +50,8 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     rc = check_scheme(scheme);
     if (rc) return rc;
     if (scheme->dtype != JFA_F64) return JFA_EUNSUPPORTED;
-    // PRIMITIVE, or CHAR-PRIMITIVE with the ARITHMETIC frozen state
-    if (scheme->recon_var != JFA_PRIMITIVE &&
+    // PRIMITIVE, CONSERVATIVE, or CHAR-PRIMITIVE with the ARITHMETIC frozen state
+    if (scheme->recon_var != JFA_PRIMITIVE && scheme->recon_var != JFA_CONSERVATIVE &&
         !(scheme->recon_var == JFA_CHAR_PRIMITIVE && scheme->frozen_state == JFA_FROZEN_ARITHMETIC))
         return JFA_EUNSUPPORTED;
     if (scheme->interpolation_limiter || scheme->flux_limiter) return JFA_EUNSUPPORTED;
@@ -83,12 +83,18 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     hipLaunchKernelGGL(adj_zero_kernel<0>, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kBlock), 0, st, gW, nz);
     const double* W = static_cast<const double*>(W_in);
     const double* lam = static_cast<const double*>(lam_out);
-    const bool ch = scheme->recon_var == JFA_CHAR_PRIMITIVE;
+    const int var = scheme->recon_var;
     if (scheme->stencil == JFA_WENO5_JS) {
-        if (ch) adj_sweeps_r<JFA_WENO5_JS, JFA_CHAR_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+        if (var == JFA_CHAR_PRIMITIVE)
+            adj_sweeps_r<JFA_WENO5_JS, JFA_CHAR_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+        else if (var == JFA_CONSERVATIVE)
+            adj_sweeps_r<JFA_WENO5_JS, JFA_CONSERVATIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
         else adj_sweeps_r<JFA_WENO5_JS, JFA_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
     } else {
-        if (ch) adj_sweeps_r<JFA_WENO5_Z, JFA_CHAR_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+        if (var == JFA_CHAR_PRIMITIVE)
+            adj_sweeps_r<JFA_WENO5_Z, JFA_CHAR_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+        else if (var == JFA_CONSERVATIVE)
+            adj_sweeps_r<JFA_WENO5_Z, JFA_CONSERVATIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
         else adj_sweeps_r<JFA_WENO5_Z, JFA_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
     }
     double* X = nullptr;

@@ -1,5 +1,4 @@
-"""PyTorch (CPU, float64) restatement of one RK3 stage with PRIMITIVE reconstruction, for
-reverse-mode checks.
+"""PyTorch (CPU, float64) restatement of one RK3 stage, for reverse-mode checks.
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this module; only ``tests/``
 use it, as the checker of the hand-written stage VJP (jfa_rk3_stage_vjp): torch.autograd of
@@ -8,8 +7,8 @@ this restatement stands in for jax.grad of the reference, which cannot be import
 The stage map is  f(U_in) = c0 U_in + c1 U_n + dt m rhs(prim(U_in))  on the interior, with the
 face halos filled from the interior as the reference fills them (the same formulas, operands
 and association order as ``jxf_oracle``; each function cites the reference file:line).
-Covered: WENO5-JS / WENO5-Z on the primitives or, with the ARITHMETIC frozen state, on their
-characteristic projection (CHAR-PRIMITIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
+Covered: WENO5-JS / WENO5-Z on the primitives, on the conservatives (CONSERVATIVE) or, with
+the ARITHMETIC frozen state, on the primitives' characteristic projection (CHAR-PRIMITIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
 EINFELDT / TORO signal speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
 domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
@@ -295,19 +294,26 @@ def dissipative_flux_xi(W, dom: Domain, axis: int, tr: Transport, gamma: float):
 
 # ------------------------------------------------------------------ RHS and halos
 def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, transport=None,
-                variable: str = "PRIMITIVE"):
+                variable: str = "PRIMITIVE", U=None):
     """SpaceSolver.compute_rhs (space_solver.py:266,289-314,489,597-599) with PRIMITIVE
-    reconstruction (high_order_godunov.py:267-280): 0.0 + sum over axes of (1/dx)(F[:-1] - F[1:]),
+    (high_order_godunov.py:267-280) or CONSERVATIVE (:282-296: WENO on the conservative buffer U,
+    then W_L = prim(U_L)) reconstruction: 0.0 + sum over axes of (1/dx)(F[:-1] - F[1:]),
     the dissipative face flux added to the convective one (:565-587) with a transport."""
     rhs = 0.0
     for axis in dom.active:
         if variable == "CHAR-PRIMITIVE":
             W_L, W_R = char_primitive_states(W, dom, axis, stencil, gamma)
+        elif variable == "CONSERVATIVE":
+            U_L = WENO5[stencil](*[U[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[0]])
+            U_R = WENO5[stencil](*[U[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[1]])
         else:
             W_L = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[0]])
             W_R = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[1]])
-        F = RIEMANN[riemann](W_L, W_R, cons_from_prim(W_L, gamma), cons_from_prim(W_R, gamma), axis, gamma,
-                             signal_speed)
+        if variable == "CONSERVATIVE":
+            W_L, W_R = prim_from_cons(U_L, gamma), prim_from_cons(U_R, gamma)
+        else:
+            U_L, U_R = cons_from_prim(W_L, gamma), cons_from_prim(W_R, gamma)
+        F = RIEMANN[riemann](W_L, W_R, U_L, U_R, axis, gamma, signal_speed)
         if transport is not None:
             F = F + dissipative_flux_xi(W, dom, axis, transport, gamma)
         n = F.shape[axis + 1]
@@ -350,7 +356,8 @@ def face_halos(W, dom: Domain, bcs: dict):
 def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, bcs: dict,
               stage: int, dt: float, eps: float, transport=None, variable: str = "PRIMITIVE"):
     """f(U_in interior) -> U_out interior: W = prim(U_in) on the interior of an eps-filled buffer,
-    face halos of W, rhs, then c0 U_in + c1 U_n + dt m rhs (RK3.py:27-62, time_integrator.py:57)."""
+    face halos of W, rhs, then c0 U_in + c1 U_n + dt m rhs (RK3.py:27-62, time_integrator.py:57).
+    The conservative buffer is U_in on the interior and cons(W) in the halos (outer/material.py:248-250)."""
     m = RK3_TIMESTEP_MULTIPLIER[stage]
     c0, c1 = (1.0, 0.0) if stage == 0 else RK3_CONS_MULTIPLIER[stage - 1]
     W = torch.full((5,) + dom.shape, eps, dtype=torch.float64)
@@ -358,7 +365,11 @@ def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_spe
     W = face_halos(W, dom, bcs)
     if transport is not None:
         assert all(bcs[f] == "PERIODIC" for f in FACE_LOCATIONS if FACE_AXIS[f] in dom.active)
-    rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma, transport, variable)
+    U = None
+    if variable == "CONSERVATIVE":
+        U = cons_from_prim(W, gamma).clone()
+        U[dom.interior] = U_int
+    rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma, transport, variable, U)
     if stage == 0:
         return U_int + (dt * m) * rhs
     return (c0 * U_int + c1 * Un_int) + (dt * m) * rhs

@@ -163,7 +163,20 @@ def test_char_primitive_stage_vjp_device_matches_autograd(case, stage):
     _vjp_vs_autograd(dom, s, stage, 90 + case, 8000 + 10 * case + stage)
 
 
-@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("stage", [0, 2])
+def test_conservative_stage_vjp_device_matches_autograd(case, stage):
+    """CONSERVATIVE reconstruction: every entry of the device J^T lam against torch.autograd of
+    the restatement."""
+    say(f"[conservative stage vjp vs autograd] case {case} stage {stage}")
+    cells, bcs, (stencil, riemann, speed) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="CONSERVATIVE", riemann=riemann, signal_speed=speed,
+                frozen_state="ARITHMETIC", bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, stage, 90 + case, 8500 + 10 * case + stage)
+
+
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("case", [0, 1, 6])
 def test_toro_speeds_stage_vjp_device_matches_autograd(case, variable):
     """HLLC with Toro's wave speeds: every entry of the device J^T lam against autograd."""
@@ -175,22 +188,23 @@ def test_toro_speeds_stage_vjp_device_matches_autograd(case, variable):
     _vjp_vs_autograd(dom, s, 2, 95 + case, 9500 + 10 * case)
 
 
-def test_char_primitive_rollout_gradient_against_device_fd():
-    """Kelvin-Helmholtz 48^2, 24 fixed-dt steps with CHAR-PRIMITIVE reconstruction through
-    RK3Rollout (4 binomial slots): the gradient along a Fourier-mode direction against Richardson
-    central differences of the device forward, to 1e-6."""
+@pytest.mark.parametrize("variable", ["CHAR-PRIMITIVE", "CONSERVATIVE"])
+def test_reconstruction_variable_rollout_gradient_against_device_fd(variable):
+    """Kelvin-Helmholtz 48^2, 24 fixed-dt steps with CHAR-PRIMITIVE or CONSERVATIVE
+    reconstruction through RK3Rollout (4 binomial slots): the gradient along a Fourier-mode
+    direction against Richardson central differences of the device forward, to 1e-6."""
     from jaxfluids_amd.adjoint import kinetic_energy, rollout
     from jaxfluids_amd.workloads import fourier_velocity_direction, kelvin_helmholtz_primitives
 
-    say("[char kh gradient vs device fd] 48^2 x 24 steps")
+    say(f"[{variable} kh gradient vs device fd] 48^2 x 24 steps")
     n, nsteps = 48, 24
     W0 = kelvin_helmholtz_primitives(n, seed=4, thickness=0.06)
     dom = o.Domain((n, n, 1), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), nh=3)
-    s = o.Setup(gamma=1.4, stencil="WENO5-JS", variable="CHAR-PRIMITIVE", riemann="HLLC", signal_speed="EINFELDT",
+    s = o.Setup(gamma=1.4, stencil="WENO5-JS", variable=variable, riemann="HLLC", signal_speed="EINFELDT",
                 bcs={f: "PERIODIC" for f in FACES})
     dt = 0.5 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
     dV = 1.0 / n ** 2
-    st = _kh_stepper(n, dt, "CHAR-PRIMITIVE")
+    st = _kh_stepper(n, dt, variable)
     W0d = dev(W0).requires_grad_(True)
     L = kinetic_energy(rollout(W0d, st, nsteps, snapshots=4), dV)
     L.backward()

@@ -82,7 +82,7 @@ def test_tie_conventions_are_jax_conventions():
         assert a.grad[1].item() + b.grad[1].item() == 1.0
 
 
-@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_restatement_forward_matches_oracle(case, variable):
     dom, s = _setup(case, variable)
@@ -116,7 +116,17 @@ def test_char_primitive_stage_vjp_matches_autograd(emu, case, stage, sweep):  # 
 
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
-@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("stage", [0, 1, 2])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_conservative_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
+    """CONSERVATIVE reconstruction: WENO on the conservatives of the window cells (their
+    dependence on the primitives, U = cons(W)), then the face's primitives prim(U_L), prim(U_R)."""
+    dom, s = _setup(case, "CONSERVATIVE")
+    _compare(emu, dom, s, stage, sweep, 50 + case, 8000 + 10 * case + stage)
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("case", [0, 1, 3, 6])
 def test_toro_speeds_stage_vjp_matches_autograd(emu, case, variable, sweep):  # noqa: F811
     """HLLC with Toro's pressure-based wave speeds (the PVRS estimate, its max with 0 and the
