@@ -146,14 +146,11 @@ __device__ __forceinline__ void sound_speed_vjp(double c, double rho, double gam
 // halves too).
 __device__ __forceinline__ void minmax_vjp(bool is_min, double x, double y, double g, double& gx, double& gy) {
     const bool x_wins = is_min ? x < y : x > y, y_wins = is_min ? y < x : y > x;
-    if (x_wins) {
-        gx += g;
-    } else if (y_wins) {
-        gy += g;
-    } else {
-        gx += 0.5 * g;
-        gy += 0.5 * g;
-    }
+    // share of x as a value, not a branch per output: a branch that adds g to one of two
+    // references made the compiler keep the pair in private (scratch) memory
+    const double wx = x_wins ? 1.0 : (y_wins ? 0.0 : 0.5);
+    gx += wx * g;
+    gy += (1.0 - wx) * g;
 }
 
 // Wave-speed estimates (signal_speeds.py): adds the VJP of (SL, SR) to u, a (sound speed), rho
