@@ -316,7 +316,7 @@ int jfa_rk3_stage_fused_noalloc(const void* W_in, const void* U_in, const void* 
 
 /* ---- Reverse mode of the RK3 stage at fixed dt (FP64; PRIMITIVE or CONSERVATIVE
  * reconstruction, or CHAR-PRIMITIVE with the ARITHMETIC frozen state, WENO5-JS/-Z,
- * HLLC with ARITHMETIC/RUSANOV/DAVIS/EINFELDT/TORO speeds or Rusanov; with scheme->transport, the
+ * HLLC or HLL with ARITHMETIC/RUSANOV/DAVIS/EINFELDT/TORO speeds, or Rusanov; with scheme->transport, the
  * viscous and heat fluxes too, on periodic domains).  The reference obtains this
  * derivative by jax.grad through SimulationManager.do_runge_kutta_stages
  * (simulation_manager.py:770-1047, RK3.py:27-62); here it is a hand-written VJP.

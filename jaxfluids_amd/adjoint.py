@@ -173,7 +173,8 @@ class FixedDtStepper:
         dx = [(r[1] - r[0]) / n for r, n in zip(ranges, self.cells)]
         self.grid = _native.make_grid(self.cells, self.nh, dx)
         self.transport = transport  # viscous / heat fluxes (periodic domains); None: inviscid
-        self.scheme = _native.make_scheme(variable, stencil, riemann, signal_speed if riemann == "HLLC" else None,
+        self.scheme = _native.make_scheme(variable, stencil, riemann,
+                                          signal_speed if riemann in ("HLLC", "HLL") else None,
                                           "ARITHMETIC", _native.F64, gamma, precision.get_spatial_stencil_eps(),
                                           transport)
         full = {f: bcs.get(f, "PERIODIC") for f in _native.FACE_LOCATIONS}

@@ -400,6 +400,52 @@ __device__ __forceinline__ void hllc_vjp(const double WL[5], const double WR[5],
     cons_vjp(WR, gUR, prm, gWR);
 }
 
+// HLL (HLL.py:29-61): F = N / D with N = s_R F_L - s_L F_R + s_L s_R (U_R - U_L),
+// D = s_R - s_L + eps, s_L = min(S_L, 0), s_R = max(S_R, 0); adds the VJP for cotangent mu.
+template <int AXIS>
+__device__ __forceinline__ void hll_vjp(const double WL[5], const double WR[5], const double UL[5],
+                                        const double UR[5], const double mu[5], const Params& prm, double gWL[5],
+                                        double gWR[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    const double g = prm.gamma;
+    const double cL = sqrt(g * WL[4] / WL[0]), cR = sqrt(g * WR[4] / WR[0]);
+    double SL, SR;
+    signal_speeds(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], WL[4], WR[4], g, SL, SR);
+    const double sl = fmin(SL, 0.0), sr = fmax(SR, 0.0);
+    const double inv = 1.0 / (sr - sl + prm.eps_precision);
+    double FL[5], FR[5];
+    physical_flux<AXIS>(WL, UL, FL);
+    physical_flux<AXIS>(WR, UR, FR);
+    double gUL[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gUR[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gFL[5], gFR[5];
+    double gsl = 0.0, gsr = 0.0, gD = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const double gN = mu[i] * inv;
+        const double dU = UR[i] - UL[i];
+        gD -= gN * ((sr * FL[i] - sl * FR[i] + sl * sr * dU) * inv);
+        gFL[i] = sr * gN;
+        gFR[i] = -sl * gN;
+        gUR[i] += sl * sr * gN;
+        gUL[i] -= sl * sr * gN;
+        gsr += gN * (FL[i] + sl * dU);
+        gsl += gN * (sr * dU - FR[i]);
+    }
+    gsr += gD;
+    gsl -= gD;
+    physical_flux_vjp<AXIS>(WL, UL, gFL, gWL, gUL);
+    physical_flux_vjp<AXIS>(WR, UR, gFR, gWR, gUR);
+    double gSL = 0.0, gSR = 0.0, gz = 0.0;
+    minmax_vjp(true, SL, 0.0, gsl, gSL, gz);   // s_L = min(S_L, 0)
+    minmax_vjp(false, SR, 0.0, gsr, gSR, gz);  // s_R = max(S_R, 0)
+    double gcL = 0.0, gcR = 0.0;
+    signal_speeds_vjp(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], WL[4], WR[4], g, gSL, gSR, gWL[v],
+                      gWR[v], gcL, gcR, gWL[0], gWR[0], gWL[4], gWR[4]);
+    sound_speed_vjp(cL, WL[0], g, gcL, gWL[0], gWL[4]);
+    sound_speed_vjp(cR, WR[0], g, gcR, gWR[0], gWR[4]);
+    cons_vjp(WL, gUL, prm, gWL);
+    cons_vjp(WR, gUR, prm, gWR);
+}
+
 // Rusanov (Rusanov.py:35-45)
 template <int AXIS>
 __device__ __forceinline__ void rusanov_vjp(const double WL[5], const double WR[5], const double UL[5],
@@ -522,6 +568,7 @@ __device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const 
     face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
     double gWL[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gWR[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if constexpr (RIEMANN == JFA_HLLC) hllc_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
+    else if constexpr (RIEMANN == JFA_HLL) hll_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     else rusanov_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     if constexpr (VAR == JFA_PRIMITIVE) {
 #pragma unroll

@@ -13,6 +13,7 @@ template <int STENCIL, int VAR>
 static void adj_sweeps_r(int riemann, const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
                          double* gW, hipStream_t st) {
     if (riemann == JFA_HLLC) adj_sweeps<STENCIL, JFA_HLLC, VAR>(W, lam, G, prm, ts, gW, st);
+    else if (riemann == JFA_HLL) adj_sweeps<STENCIL, JFA_HLL, VAR>(W, lam, G, prm, ts, gW, st);
     else adj_sweeps<STENCIL, JFA_RUSANOV, VAR>(W, lam, G, prm, ts, gW, st);
 }
 
@@ -56,7 +57,8 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
         return JFA_EUNSUPPORTED;
     if (scheme->interpolation_limiter || scheme->flux_limiter) return JFA_EUNSUPPORTED;
     if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z) return JFA_EUNSUPPORTED;
-    if (scheme->riemann != JFA_HLLC && scheme->riemann != JFA_RUSANOV) return JFA_EUNSUPPORTED;
+    if (scheme->riemann != JFA_HLLC && scheme->riemann != JFA_HLL && scheme->riemann != JFA_RUSANOV)
+        return JFA_EUNSUPPORTED;
     if (!W_in || !lam_out || !lam_in || !bc || !workspace || stage < 0 || stage > 2 || (stage > 0 && !lam_n))
         return JFA_EINVAL;
     if (lam_in == lam_out || lam_n == lam_out || (stage > 0 && lam_n == lam_in)) return JFA_EINVAL;

@@ -9,7 +9,7 @@ face halos filled from the interior as the reference fills them (the same formul
 and association order as ``jxf_oracle``; each function cites the reference file:line).
 Covered: WENO5-JS / WENO5-Z on the primitives, on the conservatives (CONSERVATIVE) or, with
 the ARITHMETIC frozen state, on the primitives' characteristic projection (CHAR-PRIMITIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
-EINFELDT / TORO signal speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
+EINFELDT / TORO signal speeds, HLL with the same speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
 domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
 Derivative conventions at non-smooth points follow torch's, which are jax's for the operations
@@ -171,7 +171,21 @@ def rusanov(W_L, W_R, U_L, U_R, axis, gamma, signal_speed=None):
     return 0.5 * (physical_flux_xi(W_L, U_L, axis) + physical_flux_xi(W_R, U_R, axis)) - 0.5 * alpha * (U_R - U_L)
 
 
-RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov}
+def hll(W_L, W_R, U_L, U_R, axis, gamma, signal_speed, eps=EPS64):
+    """HLL._solve_riemann_problem_xi_single_phase (HLL.py:29-61); eps the working precision's
+    machine epsilon (riemann_solver.py:28)."""
+    v = axis + 1
+    c_L = torch.sqrt(gamma * W_L[4] / W_L[0])
+    c_R = torch.sqrt(gamma * W_R[4] / W_R[0])
+    S_L, S_R = _speeds(signal_speed, W_L[v], W_R[v], c_L, c_R, W_L[0], W_R[0], W_L[4], W_R[4], gamma)
+    s_L = torch.minimum(S_L, torch.zeros_like(S_L))
+    s_R = torch.maximum(S_R, torch.zeros_like(S_R))
+    F_L = physical_flux_xi(W_L, U_L, axis)
+    F_R = physical_flux_xi(W_R, U_R, axis)
+    return (s_R * F_L - s_L * F_R + s_L * s_R * (U_R - U_L)) / (s_R - s_L + eps)
+
+
+RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll}
 
 
 # ------------------------------------------------------------------ characteristic projection

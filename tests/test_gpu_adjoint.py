@@ -188,6 +188,19 @@ def test_toro_speeds_stage_vjp_device_matches_autograd(case, variable):
     _vjp_vs_autograd(dom, s, 2, 95 + case, 9500 + 10 * case)
 
 
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "TORO"])
+@pytest.mark.parametrize("case", [0, 1, 6])
+def test_hll_stage_vjp_device_matches_autograd(case, speed, variable):
+    """HLL: every entry of the device J^T lam against autograd."""
+    say(f"[hll stage vjp vs autograd] case {case} {speed} {variable}")
+    cells, bcs, (stencil, _, _) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable=variable, riemann="HLL", signal_speed=speed,
+                frozen_state="ARITHMETIC", bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, 2, 97 + case, 9700 + 10 * case)
+
+
 @pytest.mark.parametrize("variable", ["CHAR-PRIMITIVE", "CONSERVATIVE"])
 def test_reconstruction_variable_rollout_gradient_against_device_fd(variable):
     """Kelvin-Helmholtz 48^2, 24 fixed-dt steps with CHAR-PRIMITIVE or CONSERVATIVE

@@ -136,6 +136,32 @@ def test_toro_speeds_stage_vjp_matches_autograd(emu, case, variable, sweep):  # 
     _compare(emu, dom, s, 2, sweep, 50 + case, 9000 + 10 * case)
 
 
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "TORO"])
+@pytest.mark.parametrize("case", [0, 1, 6])
+def test_hll_restatement_forward_matches_oracle(case, speed, variable):
+    dom, s = _setup(case, variable)
+    s.riemann, s.signal_speed = "HLL", speed
+    W0 = smooth_state(dom, 50 + case)
+    Uint = o.cons_from_prim(W0, s.gamma)
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    ref, _ = stage_map(dom, s, 1, dt, 1.01 * Uint)(Uint)
+    got = _torch_map(dom, s, 1, dt, 1.01 * Uint)(torch.as_tensor(Uint)).numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-15 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("speed", ["EINFELDT", "ARITHMETIC", "RUSANOV", "DAVIS", "TORO"])
+@pytest.mark.parametrize("case", [0, 1, 6])
+def test_hll_stage_vjp_matches_autograd(emu, case, speed, variable, sweep):  # noqa: F811
+    """HLL with every signal speed: the two-wave flux, its clipped speeds min(S_L, 0),
+    max(S_R, 0) and the precision epsilon in the denominator."""
+    dom, s = _setup(case, variable)
+    s.riemann, s.signal_speed = "HLL", speed
+    _compare(emu, dom, s, 2, sweep, 50 + case, 9300 + 10 * case)
+
+
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
 @pytest.mark.parametrize("stage", [0, 2])
 @pytest.mark.parametrize("case", range(len(VISCOUS_CASES)))
