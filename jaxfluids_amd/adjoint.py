@@ -155,7 +155,8 @@ def cons_vjp(W: torch.Tensor, gU: torch.Tensor, gamma: float) -> torch.Tensor:
 # ------------------------------------------------------------------ stepper
 class FixedDtStepper:
     """Fixed-dt RK3 steps through the fused HIP stage and their VJPs, FP64, with PRIMITIVE,
-    CONSERVATIVE or CHAR-PRIMITIVE reconstruction (the latter with the ARITHMETIC frozen state).
+    CONSERVATIVE or CHAR-PRIMITIVE reconstruction (the latter with the ARITHMETIC or ROE
+    frozen state, ``frozen_state``).
     ``cells``/``ranges``/``bcs`` as in the case setup; the scheme strings are the reference's
     registry names; ``transport`` (``_native.make_transport``) adds the viscous and heat fluxes
     on periodic domains."""
@@ -163,7 +164,7 @@ class FixedDtStepper:
     def __init__(self, cells, ranges, bcs: Dict[str, str], dt: float, *, nh: int = 3, stencil: str = "WENO5-JS",
                  riemann: str = "HLLC", signal_speed: str = "EINFELDT", gamma: float = 1.4,
                  transport: Optional[_native.JfaTransport] = None, device=None,
-                 variable: str = "PRIMITIVE") -> None:
+                 variable: str = "PRIMITIVE", frozen_state: str = "ARITHMETIC") -> None:
         if variable not in ("PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE"):
             raise NotImplementedError(f"reconstruction_variable {variable} has no reverse mode")
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
@@ -175,7 +176,7 @@ class FixedDtStepper:
         self.transport = transport  # viscous / heat fluxes (periodic domains); None: inviscid
         self.scheme = _native.make_scheme(variable, stencil, riemann,
                                           signal_speed if riemann in ("HLLC", "HLL") else None,
-                                          "ARITHMETIC", _native.F64, gamma, precision.get_spatial_stencil_eps(),
+                                          frozen_state, _native.F64, gamma, precision.get_spatial_stencil_eps(),
                                           transport)
         full = {f: bcs.get(f, "PERIODIC") for f in _native.FACE_LOCATIONS}
         self.bc = _native.bc_array(full)

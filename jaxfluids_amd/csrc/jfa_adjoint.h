@@ -497,7 +497,10 @@ __device__ __forceinline__ CharK char_frozen(const double wl[5], const double wr
     frozen_state<false>(wl, wr, prm, f);
     CharK K;
     K.a = f.P[0];
-    K.q = f.P[4];
+    // ARITHMETIC: q is the mean pressure.  ROE: the projection depends on the Roe density a and
+    // c^2 only, written as q = a c^2 / gamma so that c^2 = gamma q / a and rcc = gamma q hold
+    // for both states
+    K.q = prm.frozen == JFA_FROZEN_ARITHMETIC ? f.P[4] : f.rcc / prm.gamma;
     K.c = f.c;
     K.rcc = f.rcc;
     K.k1 = 0.5 * f.inv_c;
@@ -554,6 +557,51 @@ __device__ __forceinline__ void char_frozen_vjp(const CharK& K, const double gk[
                                                 double& gq) {
     ga = fz[0] + gk[0] * K.k1 / (2.0 * K.a) + gk[2] * K.k3 / K.a - fz[2] * K.c / (2.0 * K.a);
     gq = fz[1] - gk[0] * K.k1 / (2.0 * K.q) - gk[1] * K.k2 / K.q - gk[2] * K.k3 / K.q + fz[2] * K.c / (2.0 * K.q);
+}
+
+// Cotangents of the face's left / right cell primitives (gwl, gwr, overwritten) from those of
+// the frozen state (g a, g q).  ARITHMETIC: a, q are the means of rho and p.  ROE
+// (eigendecomposition.py:233-275): a = sqrt(rho_L rho_R) and q = a s / gamma with
+// s = c^2 = psi + g (psi + a r^2 |u_R - u_L|^2 / 2), psi = (p_L / s_L + p_R / s_R) r,
+// s_K = sqrt(rho_K), r = 1 / (s_L + s_R); the Grueneisen average g = (s_L g + s_R g) r equals
+// gamma - 1 for a single ideal gas and is taken as that constant.
+__device__ __forceinline__ void frozen_cells_vjp(const double wl[5], const double wr[5], const CharK& K, double ga,
+                                                 double gq, const Params& prm, double gwl[5], double gwr[5]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) gwl[i] = gwr[i] = 0.0;
+    if (prm.frozen == JFA_FROZEN_ARITHMETIC) {
+        gwl[0] = gwr[0] = 0.5 * ga;
+        gwl[4] = gwr[4] = 0.5 * gq;
+        return;
+    }
+    const double g = prm.gm1, ig = 1.0 / prm.gamma;
+    const double a = K.a, sc = K.c * K.c;
+    const double sL = sqrt(wl[0]), sR = sqrt(wr[0]);
+    const double r = 1.0 / (sL + sR);
+    double dv[3], dv2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        dv[k] = wr[1 + k] - wl[1 + k];
+        dv2 += dv[k] * dv[k];
+    }
+    const double gs = gq * a * ig;
+    double gat = ga + gq * sc * ig;                // a directly and through q
+    const double gpsi = gs * (1.0 + g);
+    const double gd = gs * 0.5 * g * a * r * r;    // of |du|^2
+    gat += gs * 0.5 * g * r * r * dv2;
+    double gr = gs * g * a * r * dv2;
+    const double tL = wl[4] / sL, tR = wr[4] / sR;  // psi = (tL + tR) r
+    gr += gpsi * (tL + tR);
+    gwl[4] += gpsi * r / sL;
+    gwr[4] += gpsi * r / sR;
+    double gsL = -gpsi * r * tL / sL - gr * r * r, gsR = -gpsi * r * tR / sR - gr * r * r;
+    gwl[0] += gsL / (2.0 * sL) + gat * a / (2.0 * wl[0]);
+    gwr[0] += gsR / (2.0 * sR) + gat * a / (2.0 * wr[0]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gwl[1 + k] -= 2.0 * gd * dv[k];
+        gwr[1 + k] += 2.0 * gd * dv[k];
+    }
 }
 
 // VJP of one face flux, first part: cotangents of the reconstructed face states.  PRIMITIVE:
@@ -914,18 +962,19 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
                     }
                 }
             }
-            double ga, gq;
+            double ga, gq, gwf[2][5];
             char_frozen_vjp(K, gk, fz, ga, gq);
+            frozen_cells_vjp(wl, wr, K, ga, gq, prm, gwf[0], gwf[1]);
 #pragma unroll
-            for (int k = 2; k < 4; ++k) {  // a and q are the means of the face's two cells
+            for (int k = 2; k < 4; ++k) {  // the frozen state reads the face's two cells
                 const int cell = f - 3 + k;
                 if (cell >= o0 && cell < o1) {
-                    double* p = gW + br + (int64_t)(k - 3) * sa;
-                    JFA_ACCESS(p, sizeof(double));
-                    *p += 0.5 * ga;
-                    double* q = gW + 4 * fs + br + (int64_t)(k - 3) * sa;
-                    JFA_ACCESS(q, sizeof(double));
-                    *q += 0.5 * gq;
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) {
+                        double* p = gW + i * fs + br + (int64_t)(k - 3) * sa;
+                        JFA_ACCESS(p, sizeof(double));
+                        *p += gwf[k - 2][i];
+                    }
                 }
             }
         }
@@ -1019,12 +1068,13 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __res
                 if (nt == 2 && i == i1) s[i] += t1;
             }
         }
-        double ga, gq;
+        double ga, gq, gwl[5], gwr[5];
         char_frozen_vjp(K, gk, fz, ga, gq);
         if (!vface) ga = gq = 0.0;
-        // a and q are the means of the face's two cells (window slots 2 and 3)
-        s[0] += __shfl_up(0.5 * ga, 2u, 64) + __shfl_up(0.5 * ga, 3u, 64);
-        s[4] += __shfl_up(0.5 * gq, 2u, 64) + __shfl_up(0.5 * gq, 3u, 64);
+        frozen_cells_vjp(wl, wr, K, ga, gq, prm, gwl, gwr);
+        // the frozen state reads the face's two cells (window slots 2 and 3)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) s[i] += __shfl_up(gwl[i], 2u, 64) + __shfl_up(gwr[i], 3u, 64);
     }
     const int cell = f - 3;
     if (vwave && lane >= 5 && cell <= N + 2) add_cell(gW, fs, brow + (int64_t)cell * sa, s);

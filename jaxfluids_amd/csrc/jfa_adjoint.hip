@@ -51,9 +51,9 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     rc = check_scheme(scheme);
     if (rc) return rc;
     if (scheme->dtype != JFA_F64) return JFA_EUNSUPPORTED;
-    // PRIMITIVE, CONSERVATIVE, or CHAR-PRIMITIVE with the ARITHMETIC frozen state
+    // PRIMITIVE, CONSERVATIVE, or CHAR-PRIMITIVE with the ARITHMETIC or ROE frozen state
     if (scheme->recon_var != JFA_PRIMITIVE && scheme->recon_var != JFA_CONSERVATIVE &&
-        !(scheme->recon_var == JFA_CHAR_PRIMITIVE && scheme->frozen_state == JFA_FROZEN_ARITHMETIC))
+        scheme->recon_var != JFA_CHAR_PRIMITIVE)
         return JFA_EUNSUPPORTED;
     if (scheme->interpolation_limiter || scheme->flux_limiter) return JFA_EUNSUPPORTED;
     if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z) return JFA_EUNSUPPORTED;

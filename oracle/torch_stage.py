@@ -8,7 +8,7 @@ The stage map is  f(U_in) = c0 U_in + c1 U_n + dt m rhs(prim(U_in))  on the inte
 face halos filled from the interior as the reference fills them (the same formulas, operands
 and association order as ``jxf_oracle``; each function cites the reference file:line).
 Covered: WENO5-JS / WENO5-Z on the primitives, on the conservatives (CONSERVATIVE) or, with
-the ARITHMETIC frozen state, on the primitives' characteristic projection (CHAR-PRIMITIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
+the ARITHMETIC or ROE frozen state, on the primitives' characteristic projection (CHAR-PRIMITIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
 EINFELDT / TORO signal speeds, HLL with the same speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
 domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
@@ -217,15 +217,35 @@ def _prim_from_char(x, P, c, cc, axis):
     return torch.stack(out)
 
 
-def char_primitive_states(W, dom: Domain, axis: int, stencil: str, gamma: float):
+def _frozen(W_L, W_R, gamma, frozen):
+    """compute_frozen_state, single phase (eigendecomposition.py:139-281): the density average
+    P[0] and the sound speed c, c^2 the projection uses; ARITHMETIC means (:148,223-230) or ROE
+    averages (:233-275)."""
+    if frozen == "ARITHMETIC":
+        P = 0.5 * (W_L + W_R)
+        c = torch.sqrt(gamma * P[4] / P[0])
+        return P, c, c * c
+    sL, sR = torch.sqrt(W_L[0]), torch.sqrt(W_R[0])
+    rho_div = 1.0 / (sL + sR)                                              # :240
+    P0 = torch.sqrt(W_L[0] * W_R[0])                                       # :237
+    psi_ave = (sL * (W_L[4] / W_L[0]) + sR * (W_R[4] / W_R[0])) * rho_div  # :258
+    g = gamma - 1
+    grueneisen_ave = (sL * g + sR * g) * rho_div                           # :260-262
+    dvel = W_R[1:4] - W_L[1:4]
+    sq_dvel = torch.square(dvel[0]) + torch.square(dvel[1]) + torch.square(dvel[2])  # :265
+    p_over_rho_ave = (sL * W_L[4] / W_L[0] + sR * W_R[4] / W_R[0]) * rho_div \
+        + 0.5 * P0 * rho_div * rho_div * sq_dvel                           # :267-268
+    cc = psi_ave + grueneisen_ave * p_over_rho_ave                         # :274
+    return torch.stack([P0]), torch.sqrt(cc), cc                           # :275
+
+
+def char_primitive_states(W, dom: Domain, axis: int, stencil: str, gamma: float, frozen: str = "ARITHMETIC"):
     """HighOrderGodunov.reconstruct_xi, CHAR-PRIMITIVE (high_order_godunov.py:298-316,401-402) with
-    the ARITHMETIC frozen state (eigendecomposition.py:139-148,223-231): the 6-cell window of every
-    face (:75-118), projected with the frozen state of the face's two cells, WENO5 on the
-    characteristics (window slices weno5_base.py:31), projected back."""
+    the ARITHMETIC or ROE frozen state of the face's two cells: the 6-cell window of every face
+    (eigendecomposition.py:75-118) projected with it, WENO5 on the characteristics (window slices
+    weno5_base.py:31), projected back."""
     win = [W[shift_slice(dom, axis, k)] for k in range(-3, 3)]
-    P = 0.5 * (win[2] + win[3])
-    c = torch.sqrt(gamma * P[4] / P[0])
-    cc = c * c
+    P, c, cc = _frozen(win[2], win[3], gamma, frozen)
     ch = [_char_from_prim(w, P, c, cc, axis) for w in win]
     XL = WENO5[stencil](*[ch[i] for i in WENO5_WINDOW_IDX[0]])
     XR = WENO5[stencil](*[ch[i] for i in WENO5_WINDOW_IDX[1]])
@@ -308,7 +328,7 @@ def dissipative_flux_xi(W, dom: Domain, axis: int, tr: Transport, gamma: float):
 
 # ------------------------------------------------------------------ RHS and halos
 def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, transport=None,
-                variable: str = "PRIMITIVE", U=None):
+                variable: str = "PRIMITIVE", U=None, frozen: str = "ARITHMETIC"):
     """SpaceSolver.compute_rhs (space_solver.py:266,289-314,489,597-599) with PRIMITIVE
     (high_order_godunov.py:267-280) or CONSERVATIVE (:282-296: WENO on the conservative buffer U,
     then W_L = prim(U_L)) reconstruction: 0.0 + sum over axes of (1/dx)(F[:-1] - F[1:]),
@@ -316,7 +336,7 @@ def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma:
     rhs = 0.0
     for axis in dom.active:
         if variable == "CHAR-PRIMITIVE":
-            W_L, W_R = char_primitive_states(W, dom, axis, stencil, gamma)
+            W_L, W_R = char_primitive_states(W, dom, axis, stencil, gamma, frozen)
         elif variable == "CONSERVATIVE":
             U_L = WENO5[stencil](*[U[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[0]])
             U_R = WENO5[stencil](*[U[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[1]])
@@ -368,7 +388,8 @@ def face_halos(W, dom: Domain, bcs: dict):
 
 
 def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_speed, gamma: float, bcs: dict,
-              stage: int, dt: float, eps: float, transport=None, variable: str = "PRIMITIVE"):
+              stage: int, dt: float, eps: float, transport=None, variable: str = "PRIMITIVE",
+              frozen: str = "ARITHMETIC"):
     """f(U_in interior) -> U_out interior: W = prim(U_in) on the interior of an eps-filled buffer,
     face halos of W, rhs, then c0 U_in + c1 U_n + dt m rhs (RK3.py:27-62, time_integrator.py:57).
     The conservative buffer is U_in on the interior and cons(W) in the halos (outer/material.py:248-250)."""
@@ -383,7 +404,7 @@ def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_spe
     if variable == "CONSERVATIVE":
         U = cons_from_prim(W, gamma).clone()
         U[dom.interior] = U_int
-    rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma, transport, variable, U)
+    rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma, transport, variable, U, frozen)
     if stage == 0:
         return U_int + (dt * m) * rhs
     return (c0 * U_int + c1 * Un_int) + (dt * m) * rhs

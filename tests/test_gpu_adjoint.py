@@ -107,7 +107,7 @@ def _vjp_vs_autograd(dom, s, stage, seed_state, seed):
     _native.check(rc, "jfa_rk3_stage_vjp")
     U = torch.as_tensor(Uint).clone().requires_grad_(True)
     out = ts.stage_map(U, torch.as_tensor(Un_int), dom, s.stencil, s.riemann, s.signal_speed, s.gamma, s.bcs, stage,
-                       dt, o.EPS64, s.transport, s.variable)
+                       dt, o.EPS64, s.transport, s.variable, s.frozen_state)
     (ref,) = torch.autograd.grad(out, U, torch.as_tensor(lam_int))
     ref = ref.numpy()
     got = host(li)[dom.interior]
@@ -186,6 +186,18 @@ def test_toro_speeds_stage_vjp_device_matches_autograd(case, variable):
     s = o.Setup(gamma=1.4, stencil=stencil, variable=variable, riemann="HLLC", signal_speed="TORO",
                 frozen_state="ARITHMETIC", bcs=dict(zip(FACES, bcs)))
     _vjp_vs_autograd(dom, s, 2, 95 + case, 9500 + 10 * case)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_roe_char_primitive_stage_vjp_device_matches_autograd(case):
+    """CHAR-PRIMITIVE with the ROE frozen state: every entry of the device J^T lam against
+    torch.autograd of the restatement."""
+    say(f"[roe char stage vjp vs autograd] case {case}")
+    cells, bcs, (stencil, riemann, speed) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="CHAR-PRIMITIVE", riemann=riemann, signal_speed=speed,
+                frozen_state="ROE", bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, 2, 99 + case, 9900 + 10 * case)
 
 
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])

@@ -137,15 +137,15 @@ def test_stage_vjp_refuses_stencils_without_a_reverse_mode():
 
 @pytest.mark.skipif(not os.path.exists(_native.LIB_PATH), reason="library not built")
 def test_stage_vjp_reconstruction_variables():
-    """Reverse modes exist for PRIMITIVE, CONSERVATIVE and CHAR-PRIMITIVE with the ARITHMETIC
-    frozen state: the other combinations return JFA_EUNSUPPORTED before any device call (the supported ones get
+    """Reverse modes exist for PRIMITIVE, CONSERVATIVE and CHAR-PRIMITIVE (ARITHMETIC or ROE
+    frozen state); CHAR-CONSERVATIVE has none: the other combinations return JFA_EUNSUPPORTED before any device call (the supported ones get
     as far as the argument check, JFA_EINVAL for the NULL buffers)."""
     lib = _native.load()
     grid = _native.make_grid((8, 8, 8), 3, (0.1, 0.1, 0.1))
     bc = _native.bc_array({f: "PERIODIC" for f in ("east", "west", "north", "south", "top", "bottom")})
     for var, frozen, want in (("PRIMITIVE", "ROE", _native.JFA_EINVAL),
                               ("CHAR-PRIMITIVE", "ARITHMETIC", _native.JFA_EINVAL),
-                              ("CHAR-PRIMITIVE", "ROE", _native.JFA_EUNSUPPORTED),
+                              ("CHAR-PRIMITIVE", "ROE", _native.JFA_EINVAL),
                               ("CONSERVATIVE", "ARITHMETIC", _native.JFA_EINVAL),
                               ("CHAR-CONSERVATIVE", "ARITHMETIC", _native.JFA_EUNSUPPORTED)):
         scheme = _native.make_scheme(var, "WENO5-Z", "HLLC", "EINFELDT", frozen, _native.F64, 1.4, 1e-30)

@@ -40,7 +40,7 @@ def _viscous_setup(case):
 
 def _torch_map(dom, s, stage, dt, Un_int):
     return lambda U: ts.stage_map(U, torch.as_tensor(Un_int), dom, s.stencil, s.riemann, s.signal_speed, s.gamma,
-                                  s.bcs, stage, dt, o.EPS64, s.transport, s.variable)
+                                  s.bcs, stage, dt, o.EPS64, s.transport, s.variable, s.frozen_state)
 
 
 def _compare(emu, dom, s, stage, sweep, seed_state, seed):  # noqa: F811
@@ -134,6 +134,31 @@ def test_toro_speeds_stage_vjp_matches_autograd(emu, case, variable, sweep):  # 
     dom, s = _setup(case, variable)
     s.riemann, s.signal_speed = "HLLC", "TORO"
     _compare(emu, dom, s, 2, sweep, 50 + case, 9000 + 10 * case)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_roe_char_primitive_forward_matches_oracle(case):
+    dom, s = _setup(case, "CHAR-PRIMITIVE")
+    s.frozen_state = "ROE"
+    W0 = smooth_state(dom, 50 + case)
+    Uint = o.cons_from_prim(W0, s.gamma)
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    for stage in range(3):
+        ref, _ = stage_map(dom, s, stage, dt, 1.01 * Uint)(Uint)
+        got = _torch_map(dom, s, stage, dt, 1.01 * Uint)(torch.as_tensor(Uint)).numpy()
+        assert np.max(np.abs(got - ref)) <= 1e-15 * np.max(np.abs(ref)), stage
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("stage", [0, 2])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_roe_char_primitive_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
+    """CHAR-PRIMITIVE with the ROE frozen state: the projection's dependence on the Roe density
+    sqrt(rho_L rho_R) and on c^2 = psi + g (psi + rho~ (sqrt rho_L + sqrt rho_R)^-2 |du|^2 / 2),
+    which reaches every primitive of the face's two cells."""
+    dom, s = _setup(case, "CHAR-PRIMITIVE")
+    s.frozen_state = "ROE"
+    _compare(emu, dom, s, stage, sweep, 50 + case, 9500 + 10 * case + stage)
 
 
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
