@@ -155,8 +155,8 @@ def cons_vjp(W: torch.Tensor, gU: torch.Tensor, gamma: float) -> torch.Tensor:
 # ------------------------------------------------------------------ stepper
 class FixedDtStepper:
     """Fixed-dt RK3 steps through the fused HIP stage and their VJPs, FP64, with PRIMITIVE,
-    CONSERVATIVE or CHAR-PRIMITIVE reconstruction (the latter with the ARITHMETIC or ROE
-    frozen state, ``frozen_state``).
+    CONSERVATIVE, CHAR-PRIMITIVE or CHAR-CONSERVATIVE reconstruction (the characteristic ones
+    with the ARITHMETIC or ROE frozen state, ``frozen_state``).
     ``cells``/``ranges``/``bcs`` as in the case setup; the scheme strings are the reference's
     registry names; ``transport`` (``_native.make_transport``) adds the viscous and heat fluxes
     on periodic domains."""
@@ -165,7 +165,7 @@ class FixedDtStepper:
                  riemann: str = "HLLC", signal_speed: str = "EINFELDT", gamma: float = 1.4,
                  transport: Optional[_native.JfaTransport] = None, device=None,
                  variable: str = "PRIMITIVE", frozen_state: str = "ARITHMETIC") -> None:
-        if variable not in ("PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE"):
+        if variable not in ("PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE", "CHAR-CONSERVATIVE"):
             raise NotImplementedError(f"reconstruction_variable {variable} has no reverse mode")
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
         self.cells = tuple(int(c) for c in cells)

@@ -559,12 +559,73 @@ __device__ __forceinline__ void char_frozen_vjp(const CharK& K, const double gk[
     gq = fz[1] - gk[0] * K.k1 / (2.0 * K.q) - gk[1] * K.k2 / K.q - gk[2] * K.k3 / K.q + fz[2] * K.c / (2.0 * K.q);
 }
 
-// Cotangents of the face's left / right cell primitives (gwl, gwr, overwritten) from those of
-// the frozen state (g a, g q).  ARITHMETIC: a, q are the means of rho and p.  ROE
-// (eigendecomposition.py:233-275): a = sqrt(rho_L rho_R) and q = a s / gamma with
-// s = c^2 = psi + g (psi + a r^2 |u_R - u_L|^2 / 2), psi = (p_L / s_L + p_R / s_R) r,
+// ROE averages of the face's two cells (eigendecomposition.py:233-275): a = sqrt(rho_L rho_R),
+// u~ = (s_L u_L + s_R u_R) r, H~ = (s_L H_L + s_R H_R) r and
+// c^2 = psi + g (psi + a r^2 |u_R - u_L|^2 / 2), psi = (p_L / s_L + p_R / s_R) r, with
 // s_K = sqrt(rho_K), r = 1 / (s_L + s_R); the Grueneisen average g = (s_L g + s_R g) r equals
-// gamma - 1 for a single ideal gas and is taken as that constant.
+// gamma - 1 for a single ideal gas and is taken as that constant.  Adds to gwl, gwr the VJP for
+// the cotangents ga (of a, also where c^2 reads it), gs (of c^2), gu (of u~, may be null) and gH
+// (of H~), given u~ (ua) and H~ (Ha).
+__device__ __forceinline__ void roe_frozen_vjp(const double wl[5], const double wr[5], double a, double ga, double gs,
+                                               const double* gu, const double* ua, double gH, double Ha,
+                                               const Params& prm, double gwl[5], double gwr[5]) {
+    const double g = prm.gm1;
+    const double sL = sqrt(wl[0]), sR = sqrt(wr[0]);
+    const double r = 1.0 / (sL + sR);
+    double dv[3], dv2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        dv[k] = wr[1 + k] - wl[1 + k];
+        dv2 += dv[k] * dv[k];
+    }
+    const double gpsi = gs * (1.0 + g);
+    const double gd = gs * 0.5 * g * a * r * r;  // of |du|^2
+    const double gat = ga + gs * 0.5 * g * r * r * dv2;
+    double gr = gs * g * a * r * dv2;
+    const double tL = wl[4] / sL, tR = wr[4] / sR;  // psi = (tL + tR) r
+    gr += gpsi * (tL + tR);
+    gwl[4] += gpsi * r / sL;
+    gwr[4] += gpsi * r / sR;
+    double gsL = -gpsi * r * tL / sL, gsR = -gpsi * r * tR / sR;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gwl[1 + k] -= 2.0 * gd * dv[k];
+        gwr[1 + k] += 2.0 * gd * dv[k];
+    }
+    if (gu != nullptr) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            gwl[1 + k] += gu[k] * sL * r;
+            gwr[1 + k] += gu[k] * sR * r;
+            gsL += gu[k] * (wl[1 + k] - ua[k]) * r;
+            gsR += gu[k] * (wr[1 + k] - ua[k]) * r;
+        }
+        // H_K = gamma p_K / ((gamma - 1) rho_K) + |u_K|^2 / 2 (ideal_gas.py:90-110)
+        const double hf = prm.gamma * prm.inv_gm1;
+        const double* w2[2] = {wl, wr};
+        double* g2[2] = {gwl, gwr};
+#pragma unroll
+        for (int K = 0; K < 2; ++K) {
+            const double* w = w2[K];
+            const double sK = K == 0 ? sL : sR;
+            const double HK = hf * w[4] / w[0] + 0.5 * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+            const double gHK = gH * sK * r;
+            (K == 0 ? gsL : gsR) += gH * (HK - Ha) * r;
+            g2[K][4] += gHK * hf / w[0];
+            g2[K][0] -= gHK * hf * w[4] / (w[0] * w[0]);
+#pragma unroll
+            for (int k = 1; k < 4; ++k) g2[K][k] += gHK * w[k];
+        }
+    }
+    gsL -= gr * r * r;
+    gsR -= gr * r * r;
+    gwl[0] += gsL / (2.0 * sL) + gat * a / (2.0 * wl[0]);
+    gwr[0] += gsR / (2.0 * sR) + gat * a / (2.0 * wr[0]);
+}
+
+// Cotangents of the face's left / right cell primitives (gwl, gwr, overwritten) from those of
+// the CHAR-PRIMITIVE frozen state (g a, g q).  ARITHMETIC: a, q are the means of rho and p.
+// ROE: a is the Roe density and q = a c^2 / gamma (roe_frozen_vjp).
 __device__ __forceinline__ void frozen_cells_vjp(const double wl[5], const double wr[5], const CharK& K, double ga,
                                                  double gq, const Params& prm, double gwl[5], double gwr[5]) {
 #pragma unroll
@@ -574,34 +635,121 @@ __device__ __forceinline__ void frozen_cells_vjp(const double wl[5], const doubl
         gwl[4] = gwr[4] = 0.5 * gq;
         return;
     }
-    const double g = prm.gm1, ig = 1.0 / prm.gamma;
-    const double a = K.a, sc = K.c * K.c;
-    const double sL = sqrt(wl[0]), sR = sqrt(wr[0]);
-    const double r = 1.0 / (sL + sR);
-    double dv[3], dv2 = 0.0;
+    const double ig = 1.0 / prm.gamma;
+    roe_frozen_vjp(wl, wr, K.a, ga + gq * K.c * K.c * ig, gq * K.a * ig, nullptr, nullptr, 0.0, 0.0, prm, gwl, gwr);
+}
+
+// ------------------------------------------------------------------ CHAR-CONSERVATIVE
+// Cotangents of the frozen state read by the conservative eigenvectors
+// (eigendecomposition.py:595-666), fz[0..9] = g of (rho~, u~, v~, w~, H~, c, c^2, 1/c^2,
+// 1/rho~, |u~|^2); the Grueneisen average is a constant of the single ideal gas.
+enum { FZ_P0 = 0, FZ_H = 4, FZ_C = 5, FZ_CC = 6, FZ_OC = 7, FZ_OR = 8, FZ_Q2 = 9, FZ_N = 10 };
+
+// right eigenvectors: adds the VJP of R[i][a] for cotangents gR[i][a] = gA[i] xA[a] + gB[i] xB[a]
+template <int AXIS>
+__device__ __forceinline__ void eigen_r_vjp(const Frozen& f, const double gA[5], const double xA[5],
+                                            const double gB[5], const double xB[5], double fz[FZ_N]) {
+    constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
+    auto G = [&](int i, int a) { return gA[i] * xA[a] + gB[i] * xB[a]; };
+    const double* P = f.P;
+    const double g = f.g, c = f.c;
+    double t;
+    t = G(v, 0); fz[v] += t; fz[FZ_C] -= t;                                      // P_v - c
+    fz[m0] += G(m0, 0);                                                          // P_m0
+    fz[m1] += G(m1, 0);                                                          // P_m1
+    t = G(4, 0); fz[FZ_H] += t; fz[v] -= t * c; fz[FZ_C] -= t * P[v];            // H - P_v c
+    fz[1] += G(1, v) * g;                                                        // g P_1
+    fz[2] += G(2, v) * g;
+    fz[3] += G(3, v) * g;
+    t = G(4, v); fz[FZ_H] += t * g; fz[FZ_CC] -= t;                              // g H - c^2
+    fz[FZ_P0] -= G(m0, m0);                                                      // -rho
+    t = G(4, m0); fz[FZ_P0] -= t * P[m0]; fz[m0] -= t * P[0];                    // -rho P_m0
+    fz[FZ_P0] += G(m1, m1);                                                      // rho
+    t = G(4, m1); fz[FZ_P0] += t * P[m1]; fz[m1] += t * P[0];                    // rho P_m1
+    t = G(v, 4); fz[v] += t; fz[FZ_C] += t;                                      // P_v + c
+    fz[m0] += G(m0, 4);
+    fz[m1] += G(m1, 4);
+    t = G(4, 4); fz[FZ_H] += t; fz[v] += t * c; fz[FZ_C] += t * P[v];            // H + P_v c
+}
+
+// left eigenvectors: adds the VJP of L[a][b] for cotangents gL[a][b]
+template <int AXIS>
+__device__ __forceinline__ void eigen_l_vjp(const Frozen& f, const double (&gL)[5][5], double fz[FZ_N]) {
+    constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
+    const double* P = f.P;
+    const double g = f.g, c = f.c, H = f.H, q2 = f.q2;
+    const double oc = f.inv_c * f.inv_c, orr = 1.0 / P[0];
+    double t;
+    // rows 0 and 4: 0.5 oc (g q2 - g H +- (P_v +- c) c), 0.5 oc (-P_v g -+ c), 0.5 oc (-P_m g), 0.5 oc g
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        dv[k] = wr[1 + k] - wl[1 + k];
-        dv2 += dv[k] * dv[k];
+    for (int a = 0; a < 5; a += 4) {
+        const double sg = a == 0 ? 1.0 : -1.0;
+        t = gL[a][0];
+        fz[FZ_OC] += 0.5 * t * (g * q2 - g * H + sg * (P[v] + sg * c) * c);
+        t *= 0.5 * oc;
+        fz[FZ_Q2] += t * g;
+        fz[FZ_H] -= t * g;
+        fz[v] += sg * t * c;
+        fz[FZ_C] += t * (sg * P[v] + 2.0 * c);
+        t = gL[a][v];
+        fz[FZ_OC] += 0.5 * t * (-P[v] * g - sg * c);
+        fz[v] -= 0.5 * oc * g * t;
+        fz[FZ_C] -= sg * 0.5 * oc * t;
+        t = gL[a][m0];
+        fz[FZ_OC] -= 0.5 * t * P[m0] * g;
+        fz[m0] -= 0.5 * oc * g * t;
+        t = gL[a][m1];
+        fz[FZ_OC] -= 0.5 * t * P[m1] * g;
+        fz[m1] -= 0.5 * oc * g * t;
+        fz[FZ_OC] += 0.5 * g * gL[a][4];
     }
-    const double gs = gq * a * ig;
-    double gat = ga + gq * sc * ig;                // a directly and through q
-    const double gpsi = gs * (1.0 + g);
-    const double gd = gs * 0.5 * g * a * r * r;    // of |du|^2
-    gat += gs * 0.5 * g * r * r * dv2;
-    double gr = gs * g * a * r * dv2;
-    const double tL = wl[4] / sL, tR = wr[4] / sR;  // psi = (tL + tR) r
-    gr += gpsi * (tL + tR);
-    gwl[4] += gpsi * r / sL;
-    gwr[4] += gpsi * r / sR;
-    double gsL = -gpsi * r * tL / sL - gr * r * r, gsR = -gpsi * r * tR / sR - gr * r * r;
-    gwl[0] += gsL / (2.0 * sL) + gat * a / (2.0 * wl[0]);
-    gwr[0] += gsR / (2.0 * sR) + gat * a / (2.0 * wr[0]);
+    // row v: oc (H - q2), P_k oc, -oc
+    t = gL[v][0];
+    fz[FZ_OC] += t * (H - q2);
+    fz[FZ_H] += t * oc;
+    fz[FZ_Q2] -= t * oc;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        gwl[1 + k] -= 2.0 * gd * dv[k];
-        gwr[1 + k] += 2.0 * gd * dv[k];
+    for (int k = 1; k < 4; ++k) {
+        fz[k] += gL[v][k] * oc;
+        fz[FZ_OC] += gL[v][k] * P[k];
     }
+    fz[FZ_OC] -= gL[v][4];
+    // rows m0, m1: +-P_m / rho, -+1 / rho
+    fz[m0] += gL[m0][0] * orr;
+    fz[FZ_OR] += gL[m0][0] * P[m0] - gL[m0][m0];
+    fz[m1] -= gL[m1][0] * orr;
+    fz[FZ_OR] += -gL[m1][0] * P[m1] + gL[m1][m1];
+}
+
+// Cotangents of the face's two cells (gwl, gwr, overwritten) from the frozen-state cotangents
+// fz: 1/c^2, 1/rho~, c and |u~|^2 first folded into c^2, rho~ and u~, then ARITHMETIC means
+// (c^2 = gamma p~ / rho~, H~ = gamma p~ / ((gamma - 1) rho~) + |u~|^2 / 2) or ROE averages.
+__device__ __forceinline__ void char_cons_frozen_vjp(const double wl[5], const double wr[5], const Frozen& f,
+                                                     double fz[FZ_N], const Params& prm, double gwl[5],
+                                                     double gwr[5]) {
+    const double oc = f.inv_c * f.inv_c, orr = 1.0 / f.P[0];
+    double gcc = fz[FZ_CC] - fz[FZ_OC] * oc * oc + fz[FZ_C] / (2.0 * f.c);
+    double gP0 = fz[FZ_P0] - fz[FZ_OR] * orr * orr;
+    double gu[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gu[k] = fz[1 + k] + 2.0 * f.P[1 + k] * fz[FZ_Q2];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) gwl[i] = gwr[i] = 0.0;
+    if (prm.frozen == JFA_FROZEN_ARITHMETIC) {
+        const double gH = fz[FZ_H];
+        const double hf = prm.gamma * prm.inv_gm1;
+        double gP4 = gcc * prm.gamma * orr + gH * hf * orr;
+        gP0 -= gcc * f.cc * orr + gH * (f.H - 0.5 * f.q2) * orr;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gu[k] += gH * f.P[1 + k];
+        gwl[0] = gwr[0] = 0.5 * gP0;
+        gwl[4] = gwr[4] = 0.5 * gP4;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) gwl[1 + k] = gwr[1 + k] = 0.5 * gu[k];
+        return;
+    }
+    const double ua[3] = {f.P[1], f.P[2], f.P[3]};
+    roe_frozen_vjp(wl, wr, f.P[0], gP0, gcc, gu, ua, fz[FZ_H], f.H, prm, gwl, gwr);
 }
 
 // VJP of one face flux, first part: cotangents of the reconstructed face states.  PRIMITIVE:
@@ -609,9 +757,9 @@ __device__ __forceinline__ void frozen_cells_vjp(const double wl[5], const doubl
 // the reconstructed characteristics and fz = (g a, g q, g c) the frozen state's share through
 // the back projection (W_rho depends on a, W_n on c, W_p on rcc = gamma q).
 template <int AXIS, int STENCIL, int RIEMANN, int VAR>
-__device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const double mu[5], const Params& prm,
-                                               double gl[5], double gr[5], double fz[3]) {
-    double wl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, wr[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+__device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const double wl[5], const double wr[5],
+                                               const double mu[5], const Params& prm, double gl[5], double gr[5],
+                                               double fz[FZ_N]) {
     double WL[5], WR[5], UL[5], UR[5];
     face_states<AXIS, VAR, STENCIL>(win, wl, wr, prm, WL, WR, UL, UR);
     double gWL[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gWR[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -631,6 +779,32 @@ __device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const 
         prim_vjp(WL, gWL, prm, gl);
         prim_vjp(WR, gWR, prm, gr);
         fz[0] = fz[1] = fz[2] = 0.0;
+    } else if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
+        // U_L = R x_L of the reconstructed characteristics x_L (high_order_godunov.py:404-417),
+        // W_L = prim(U_L): gl = R^T gU_L, and the right eigenvectors' share of the frozen state.
+        // x_L = L U_L (L = R^-1 to rounding) stands in for the forward's reconstructed values.
+        double gUL[5], gUR[5];
+        prim_vjp(WL, gWL, prm, gUL);
+        prim_vjp(WR, gWR, prm, gUR);
+        Frozen f;
+        frozen_state<true>(wl, wr, prm, f);
+        double Lm[5][5], Rm[5][5];
+        conservative_eigenvectors<AXIS>(f, Lm, Rm);
+        double xL[5], xR[5];
+#pragma unroll
+        for (int a = 0; a < 5; ++a) {
+            xL[a] = xR[a] = gl[a] = gr[a] = 0.0;
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                xL[a] += Lm[a][b] * UL[b];
+                xR[a] += Lm[a][b] * UR[b];
+                gl[a] += Rm[b][a] * gUL[b];
+                gr[a] += Rm[b][a] * gUR[b];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < FZ_N; ++i) fz[i] = 0.0;
+        eigen_r_vjp<AXIS>(f, gUL, xL, gUR, xR, fz);
     } else {
         constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
         const CharK K = char_frozen(win[2], win[3], prm);
@@ -668,13 +842,18 @@ __device__ __forceinline__ void window_vjp_var(const double x[6], double gl, dou
 template <int AXIS, int STENCIL, int RIEMANN, int VAR>
 __device__ __forceinline__ void face_state_vjp_at(const double* __restrict__ W, int64_t br, int64_t sa, int64_t fs,
                                                   const double mu[5], const Params& prm, double gl[5], double gr[5],
-                                                  double fz[3]) {
-    double win[6][5];
+                                                  double fz[FZ_N]) {
+    double win[6][5], wl[5], wr[5];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
 #pragma unroll
         for (int i = 0; i < 5; ++i) win[k][i] = ld(W, i * fs + br + (int64_t)(k - 3) * sa);
-    if constexpr (VAR == JFA_CONSERVATIVE) {  // the forward reconstructs the conservatives
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {  // the face's two cells (read by the CHAR-CONSERVATIVE frozen state)
+        wl[i] = win[2][i];
+        wr[i] = win[3][i];
+    }
+    if constexpr (VAR == JFA_CONSERVATIVE || VAR == JFA_CHAR_CONSERVATIVE) {  // conservative windows
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             double u[5];
@@ -683,7 +862,7 @@ __device__ __forceinline__ void face_state_vjp_at(const double* __restrict__ W, 
             for (int i = 0; i < 5; ++i) win[k][i] = u[i];
         }
     }
-    face_state_vjp<AXIS, STENCIL, RIEMANN, VAR>(win, mu, prm, gl, gr, fz);
+    face_state_vjp<AXIS, STENCIL, RIEMANN, VAR>(win, wl, wr, mu, prm, gl, gr, fz);
 }
 
 // CHAR-PRIMITIVE, second part for characteristic k of the face: its window (projected from the
@@ -751,6 +930,51 @@ __device__ __forceinline__ void cons_window_vjp(const double* __restrict__ W, in
         }
     }
     window_vjp_var<STENCIL>(x6, gl, gr, prm.eps, g6);
+}
+
+// CHAR-CONSERVATIVE, second part: the window cotangents of all five characteristics
+// x_a = sum_b L[a][b] U_j[b] of the six cells, taken back to their conservatives in gU6 (then
+// to the primitives by the caller, U = cons(W)), and the left eigenvectors' share of the frozen
+// state, added to fz.  f, Lm: the face's frozen state and left eigenvectors.
+template <int AXIS, int STENCIL>
+__device__ __forceinline__ void char_cons_window_vjp(const double* __restrict__ W, int64_t br, int64_t sa,
+                                                     int64_t fs, const Frozen& f, const double (&Lm)[5][5],
+                                                     const double gl[5], const double gr[5], const Params& prm,
+                                                     double (&gU6)[6][5], double fz[FZ_N]) {
+    double u6[6][5];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double w[5];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) w[t] = ld(W, t * fs + br + (int64_t)(j - 3) * sa);
+        cons_from_prim(w, prm, u6[j]);
+#pragma unroll
+        for (int t = 0; t < 5; ++t) gU6[j][t] = 0.0;
+    }
+    double gL[5][5];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+        double x6[6], g6[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            double acc = Lm[a][0] * u6[j][0];
+#pragma unroll
+            for (int b = 1; b < 5; ++b) acc += Lm[a][b] * u6[j][b];
+            x6[j] = acc;
+        }
+        window_vjp_var<STENCIL>(x6, gl[a], gr[a], prm.eps, g6);
+#pragma unroll
+        for (int b = 0; b < 5; ++b) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                acc += g6[j] * u6[j][b];
+                gU6[j][b] += Lm[a][b] * g6[j];
+            }
+            gL[a][b] = acc;
+        }
+    }
+    eigen_l_vjp<AXIS>(f, gL, fz);
 }
 
 // ------------------------------------------------------------------ dissipative fluxes
@@ -894,7 +1118,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
     for (int f = fa; f <= fb; ++f) {
         int64_t br = bcol + (int64_t)f * sa;
         JFA_OPAQUE(br);  // keeps per-tap addresses from becoming loop-carried registers
-        double mu[5], gl[5], gr[5], fz[3];
+        double mu[5], gl[5], gr[5], fz[FZ_N];
         face_mu(lam, br, sa, fs, f, N, scale, mu);
         face_state_vjp_at<A, STENCIL, RIEMANN, VAR>(W, br, sa, fs, mu, prm, gl, gr, fz);
         if constexpr (VAR == JFA_PRIMITIVE) {
@@ -910,6 +1134,40 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
                         double* p = gW + i * fs + br + (int64_t)(k - 3) * sa;
                         JFA_ACCESS(p, sizeof(double));
                         *p += o6[k];
+                    }
+                }
+            }
+        } else if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
+            double wl[5], wr[5];
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                wl[i] = ld(W, i * fs + br - sa);
+                wr[i] = ld(W, i * fs + br);
+            }
+            Frozen fr;
+            frozen_state<true>(wl, wr, prm, fr);
+            double Lm[5][5], Rm[5][5];
+            conservative_eigenvectors<A>(fr, Lm, Rm);
+            double gU6[6][5];
+            char_cons_window_vjp<A, STENCIL>(W, br, sa, fs, fr, Lm, gl, gr, prm, gU6, fz);
+            double gwf[2][5];
+            char_cons_frozen_vjp(wl, wr, fr, fz, prm, gwf[0], gwf[1]);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int cell = f - 3 + k;
+                if (cell >= o0 && cell < o1) {
+                    double w[5], g[5];
+#pragma unroll
+                    for (int t = 0; t < 5; ++t) {
+                        w[t] = ld(W, t * fs + br + (int64_t)(k - 3) * sa);
+                        g[t] = (k == 2 || k == 3) ? gwf[k - 2][t] : 0.0;
+                    }
+                    cons_vjp(w, gU6[k], prm, g);
+#pragma unroll
+                    for (int t = 0; t < 5; ++t) {
+                        double* p = gW + t * fs + br + (int64_t)(k - 3) * sa;
+                        JFA_ACCESS(p, sizeof(double));
+                        *p += g[t];
                     }
                 }
             }
@@ -1006,7 +1264,9 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __res
     const int f = -5 + 59 * w + lane;
     const bool vface = vwave && f >= 0 && f <= N;
     const int64_t br = brow + (vface ? f : 0) * sa;
-    double gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gr[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fz[3] = {0.0, 0.0, 0.0};
+    double gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gr[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fz[FZ_N];
+#pragma unroll
+    for (int i = 0; i < FZ_N; ++i) fz[i] = 0.0;
     if (vface) {
         double mu[5];
         face_mu(lam, br, sa, fs, f, N, ts * G.one_dx[A], mu);
@@ -1022,6 +1282,37 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __res
             window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
 #pragma unroll
             for (int k = 0; k < 6; ++k) s[i] += __shfl_up(o6[k], (unsigned)k, 64);
+        }
+    } else if constexpr (VAR == JFA_CHAR_CONSERVATIVE) {
+        const int64_t bq = vface ? br : brow;  // lanes without a face: face 0, zero cotangents
+        double wl[5], wr[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            wl[i] = ld(W, i * fs + bq - sa);
+            wr[i] = ld(W, i * fs + bq);
+        }
+        Frozen fr;
+        frozen_state<true>(wl, wr, prm, fr);
+        double Lm[5][5], Rm[5][5];
+        conservative_eigenvectors<A>(fr, Lm, Rm);
+        double gU6[6][5];
+        char_cons_window_vjp<A, STENCIL>(W, bq, sa, fs, fr, Lm, gl, gr, prm, gU6, fz);
+        double gwf[2][5];
+        char_cons_frozen_vjp(wl, wr, fr, fz, prm, gwf[0], gwf[1]);
+        if (!vface)
+#pragma unroll
+            for (int t = 0; t < 5; ++t) gwf[0][t] = gwf[1][t] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double w[5], g[5];
+#pragma unroll
+            for (int t = 0; t < 5; ++t) {
+                w[t] = ld(W, t * fs + bq + (int64_t)(k - 3) * sa);
+                g[t] = (k == 2 || k == 3) ? gwf[k - 2][t] : 0.0;
+            }
+            cons_vjp(w, gU6[k], prm, g);
+#pragma unroll
+            for (int t = 0; t < 5; ++t) s[t] += __shfl_up(g[t], (unsigned)k, 64);
         }
     } else if constexpr (VAR == JFA_CONSERVATIVE) {
         const int64_t bq = vface ? br : brow;  // lanes without a face: face 0, zero cotangents

@@ -51,9 +51,9 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     rc = check_scheme(scheme);
     if (rc) return rc;
     if (scheme->dtype != JFA_F64) return JFA_EUNSUPPORTED;
-    // PRIMITIVE, CONSERVATIVE, or CHAR-PRIMITIVE with the ARITHMETIC or ROE frozen state
+    // PRIMITIVE, CONSERVATIVE, CHAR-PRIMITIVE or CHAR-CONSERVATIVE (ARITHMETIC or ROE frozen state)
     if (scheme->recon_var != JFA_PRIMITIVE && scheme->recon_var != JFA_CONSERVATIVE &&
-        scheme->recon_var != JFA_CHAR_PRIMITIVE)
+        scheme->recon_var != JFA_CHAR_PRIMITIVE && scheme->recon_var != JFA_CHAR_CONSERVATIVE)
         return JFA_EUNSUPPORTED;
     if (scheme->interpolation_limiter || scheme->flux_limiter) return JFA_EUNSUPPORTED;
     if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z) return JFA_EUNSUPPORTED;
@@ -91,12 +91,16 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
             adj_sweeps_r<JFA_WENO5_JS, JFA_CHAR_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
         else if (var == JFA_CONSERVATIVE)
             adj_sweeps_r<JFA_WENO5_JS, JFA_CONSERVATIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+        else if (var == JFA_CHAR_CONSERVATIVE)
+            adj_sweeps_r<JFA_WENO5_JS, JFA_CHAR_CONSERVATIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
         else adj_sweeps_r<JFA_WENO5_JS, JFA_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
     } else {
         if (var == JFA_CHAR_PRIMITIVE)
             adj_sweeps_r<JFA_WENO5_Z, JFA_CHAR_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
         else if (var == JFA_CONSERVATIVE)
             adj_sweeps_r<JFA_WENO5_Z, JFA_CONSERVATIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+        else if (var == JFA_CHAR_CONSERVATIVE)
+            adj_sweeps_r<JFA_WENO5_Z, JFA_CHAR_CONSERVATIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
         else adj_sweeps_r<JFA_WENO5_Z, JFA_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
     }
     double* X = nullptr;

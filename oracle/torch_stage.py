@@ -8,7 +8,8 @@ The stage map is  f(U_in) = c0 U_in + c1 U_n + dt m rhs(prim(U_in))  on the inte
 face halos filled from the interior as the reference fills them (the same formulas, operands
 and association order as ``jxf_oracle``; each function cites the reference file:line).
 Covered: WENO5-JS / WENO5-Z on the primitives, on the conservatives (CONSERVATIVE) or, with
-the ARITHMETIC or ROE frozen state, on the primitives' characteristic projection (CHAR-PRIMITIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
+the ARITHMETIC or ROE frozen state, on the characteristic projection of the primitives
+(CHAR-PRIMITIVE) or of the conservatives (CHAR-CONSERVATIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
 EINFELDT / TORO signal speeds, HLL with the same speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
 domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
@@ -252,6 +253,114 @@ def char_primitive_states(W, dom: Domain, axis: int, stencil: str, gamma: float,
     return _prim_from_char(XL, P, c, cc, axis), _prim_from_char(XR, P, c, cc, axis)
 
 
+def _frozen_full(W_L, W_R, gamma, frozen):
+    """compute_frozen_state, single phase (eigendecomposition.py:139-281), everything the
+    conservative eigenvectors read: primes_ave, enthalpy_ave, grueneisen_ave, c, c^2, |u_ave|^2."""
+    def enthalpy(p, rho, u):  # ideal_gas.py:90-110
+        return (p / (gamma - 1) + 0.5 * rho * (torch.square(u[0]) + torch.square(u[1]) + torch.square(u[2])) + p) / rho
+
+    if frozen == "ARITHMETIC":
+        P = 0.5 * (W_L + W_R)                                                   # :148
+        g = (gamma - 1) * torch.ones_like(P[0])                                 # :216-218
+        H = enthalpy(P[4], P[0], P[1:4])                                        # :219-222
+        c = torch.sqrt(gamma * P[4] / P[0])                                     # :223-225
+        cc = c * c                                                              # :230
+    else:
+        sL, sR = torch.sqrt(W_L[0]), torch.sqrt(W_R[0])
+        P = (sL * W_L + sR * W_R) / (sL + sR)                                   # :293
+        P = torch.cat([torch.sqrt(W_L[0] * W_R[0])[None], P[1:]])               # :237
+        rho_div = 1.0 / (sL + sR)                                               # :240
+        H = (sL * enthalpy(W_L[4], W_L[0], W_L[1:4]) + sR * enthalpy(W_R[4], W_R[0], W_R[1:4])) * rho_div  # :250
+        psi_ave = (sL * (W_L[4] / W_L[0]) + sR * (W_R[4] / W_R[0])) * rho_div  # :258
+        g = (sL * (gamma - 1) + sR * (gamma - 1)) * rho_div                     # :260-262
+        dvel = W_R[1:4] - W_L[1:4]
+        sq_dvel = torch.square(dvel[0]) + torch.square(dvel[1]) + torch.square(dvel[2])  # :265
+        p_over_rho_ave = (sL * W_L[4] / W_L[0] + sR * W_R[4] / W_R[0]) * rho_div \
+            + 0.5 * P[0] * rho_div * rho_div * sq_dvel                          # :267-268
+        cc = psi_ave + g * p_over_rho_ave                                       # :274
+        c = torch.sqrt(cc)                                                      # :275
+    q2 = torch.square(P[1]) + torch.square(P[2]) + torch.square(P[3])          # :231, :271
+    return P, H, g, c, cc, q2
+
+
+def _eigen_conservatives(P, H, g, c, cc, q2, axis):
+    """eigendecomposition_conservatives, single phase (eigendecomposition.py:595-666): R, L as
+    5 x 5 nested lists indexed [row][col]."""
+    one_cc = 1.0 / cc
+    one_rho = 1.0 / P[0]
+    z = torch.zeros_like(P[0])
+    o = torch.ones_like(P[0])
+    R = [[z] * 5 for _ in range(5)]
+    L = [[z] * 5 for _ in range(5)]
+    v = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    R[0][0] = o                                                   # :608-631
+    R[v][0] = P[v] - c
+    R[m0][0] = P[m0]
+    R[m1][0] = P[m1]
+    R[4][0] = H - P[v] * c
+    R[0][v] = g
+    R[1][v] = g * P[1]
+    R[2][v] = g * P[2]
+    R[3][v] = g * P[3]
+    R[4][v] = g * H - cc
+    R[m0][m0] = -P[0]
+    R[4][m0] = -P[0] * P[m0]
+    R[m1][m1] = P[0]
+    R[4][m1] = P[0] * P[m1]
+    R[0][4] = o
+    R[v][4] = P[v] + c
+    R[m0][4] = P[m0]
+    R[m1][4] = P[m1]
+    R[4][4] = H + P[v] * c
+    L[0][0] = 0.5 * one_cc * (g * q2 - g * H + (P[v] + c) * c)  # :635-658
+    L[0][v] = 0.5 * one_cc * (-P[v] * g - c)
+    L[0][m0] = 0.5 * one_cc * (-P[m0] * g)
+    L[0][m1] = 0.5 * one_cc * (-P[m1] * g)
+    L[0][4] = 0.5 * one_cc * g
+    L[v][0] = one_cc * (H - q2)
+    L[v][1] = P[1] * one_cc
+    L[v][2] = P[2] * one_cc
+    L[v][3] = P[3] * one_cc
+    L[v][4] = -one_cc
+    L[m0][0] = P[m0] * one_rho
+    L[m0][m0] = -one_rho
+    L[m1][0] = -P[m1] * one_rho
+    L[m1][m1] = one_rho
+    L[4][0] = 0.5 * one_cc * (g * q2 - g * H - (P[v] - c) * c)
+    L[4][v] = 0.5 * one_cc * (-P[v] * g + c)
+    L[4][m0] = 0.5 * one_cc * (-P[m0] * g)
+    L[4][m1] = 0.5 * one_cc * (-P[m1] * g)
+    L[4][4] = 0.5 * one_cc * g
+    return R, L
+
+
+def _transform(M, x):
+    """einsum 'ij...,j...->i...' (eigendecomposition.py:717-742), summed in j order."""
+    out = []
+    for i in range(5):
+        acc = M[i][0] * x[0]
+        for j in range(1, 5):
+            acc = acc + M[i][j] * x[j]
+        out.append(acc)
+    return torch.stack(out)
+
+
+def char_conservative_states(W, U, dom: Domain, axis: int, stencil: str, gamma: float, frozen: str = "ARITHMETIC"):
+    """HighOrderGodunov.reconstruct_xi, CHAR-CONSERVATIVE (high_order_godunov.py:404-417): the
+    conservative 6-cell window projected with the left eigenvectors at the frozen state of the
+    face's two cells, WENO5 on the characteristics, back with the right eigenvectors; the face
+    primitives are prim(U_L), prim(U_R)."""
+    win = [U[shift_slice(dom, axis, k)] for k in range(-3, 3)]
+    cl, cr = W[shift_slice(dom, axis, -1)], W[shift_slice(dom, axis, 0)]
+    R, L = _eigen_conservatives(*_frozen_full(cl, cr, gamma, frozen), axis)
+    ch = [_transform(L, w) for w in win]
+    XL = WENO5[stencil](*[ch[i] for i in WENO5_WINDOW_IDX[0]])
+    XR = WENO5[stencil](*[ch[i] for i in WENO5_WINDOW_IDX[1]])
+    U_L, U_R = _transform(R, XL), _transform(R, XR)
+    return U_L, U_R
+
+
 # ------------------------------------------------------------------ dissipative fluxes (periodic)
 def _law(T, ref, T_ref, exp, C):
     """material.py:82-124: SUTHERLAND ref ((T_ref + C)/(T + C)) (T/T_ref)^1.5 (power3_2 = x sqrt(x),
@@ -340,10 +449,12 @@ def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma:
         elif variable == "CONSERVATIVE":
             U_L = WENO5[stencil](*[U[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[0]])
             U_R = WENO5[stencil](*[U[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[1]])
+        elif variable == "CHAR-CONSERVATIVE":
+            U_L, U_R = char_conservative_states(W, U, dom, axis, stencil, gamma, frozen)
         else:
             W_L = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[0]])
             W_R = WENO5[stencil](*[W[shift_slice(dom, axis, k)] for k in WENO5_SHIFTS[1]])
-        if variable == "CONSERVATIVE":
+        if variable in ("CONSERVATIVE", "CHAR-CONSERVATIVE"):
             W_L, W_R = prim_from_cons(U_L, gamma), prim_from_cons(U_R, gamma)
         else:
             U_L, U_R = cons_from_prim(W_L, gamma), cons_from_prim(W_R, gamma)
@@ -401,7 +512,7 @@ def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_spe
     if transport is not None:
         assert all(bcs[f] == "PERIODIC" for f in FACE_LOCATIONS if FACE_AXIS[f] in dom.active)
     U = None
-    if variable == "CONSERVATIVE":
+    if variable in ("CONSERVATIVE", "CHAR-CONSERVATIVE"):
         U = cons_from_prim(W, gamma).clone()
         U[dom.interior] = U_int
     rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma, transport, variable, U, frozen)

@@ -200,6 +200,19 @@ def test_roe_char_primitive_stage_vjp_device_matches_autograd(case):
     _vjp_vs_autograd(dom, s, 2, 99 + case, 9900 + 10 * case)
 
 
+@pytest.mark.parametrize("frozen", ["ARITHMETIC", "ROE"])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_char_conservative_stage_vjp_device_matches_autograd(case, frozen):
+    """CHAR-CONSERVATIVE (ARITHMETIC and ROE frozen states): every entry of the device J^T lam
+    against torch.autograd of the restatement."""
+    say(f"[char-conservative stage vjp vs autograd] case {case} {frozen}")
+    cells, bcs, (stencil, riemann, speed) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="CHAR-CONSERVATIVE", riemann=riemann, signal_speed=speed,
+                frozen_state=frozen, bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, 2, 93 + case, 9800 + 10 * case)
+
+
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "TORO"])
 @pytest.mark.parametrize("case", [0, 1, 6])

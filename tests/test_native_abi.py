@@ -137,9 +137,9 @@ def test_stage_vjp_refuses_stencils_without_a_reverse_mode():
 
 @pytest.mark.skipif(not os.path.exists(_native.LIB_PATH), reason="library not built")
 def test_stage_vjp_reconstruction_variables():
-    """Reverse modes exist for PRIMITIVE, CONSERVATIVE and CHAR-PRIMITIVE (ARITHMETIC or ROE
-    frozen state); CHAR-CONSERVATIVE has none: the other combinations return JFA_EUNSUPPORTED before any device call (the supported ones get
-    as far as the argument check, JFA_EINVAL for the NULL buffers)."""
+    """Reverse modes exist for every reconstruction variable (the characteristic ones with the
+    ARITHMETIC or ROE frozen state): each gets as far as the argument check, JFA_EINVAL for the
+    NULL buffers."""
     lib = _native.load()
     grid = _native.make_grid((8, 8, 8), 3, (0.1, 0.1, 0.1))
     bc = _native.bc_array({f: "PERIODIC" for f in ("east", "west", "north", "south", "top", "bottom")})
@@ -147,7 +147,8 @@ def test_stage_vjp_reconstruction_variables():
                               ("CHAR-PRIMITIVE", "ARITHMETIC", _native.JFA_EINVAL),
                               ("CHAR-PRIMITIVE", "ROE", _native.JFA_EINVAL),
                               ("CONSERVATIVE", "ARITHMETIC", _native.JFA_EINVAL),
-                              ("CHAR-CONSERVATIVE", "ARITHMETIC", _native.JFA_EUNSUPPORTED)):
+                              ("CHAR-CONSERVATIVE", "ARITHMETIC", _native.JFA_EINVAL),
+                              ("CHAR-CONSERVATIVE", "ROE", _native.JFA_EINVAL)):
         scheme = _native.make_scheme(var, "WENO5-Z", "HLLC", "EINFELDT", frozen, _native.F64, 1.4, 1e-30)
         rc = lib.jfa_rk3_stage_vjp(None, None, None, None, grid, scheme, bc, 0, 1e-3, None, 0, None)
         assert rc == want, (var, frozen, rc)

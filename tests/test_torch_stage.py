@@ -136,10 +136,12 @@ def test_toro_speeds_stage_vjp_matches_autograd(emu, case, variable, sweep):  # 
     _compare(emu, dom, s, 2, sweep, 50 + case, 9000 + 10 * case)
 
 
+@pytest.mark.parametrize("variable, frozen", [("CHAR-PRIMITIVE", "ROE"), ("CHAR-CONSERVATIVE", "ARITHMETIC"),
+                                              ("CHAR-CONSERVATIVE", "ROE")])
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_roe_char_primitive_forward_matches_oracle(case):
-    dom, s = _setup(case, "CHAR-PRIMITIVE")
-    s.frozen_state = "ROE"
+def test_frozen_state_forward_matches_oracle(case, variable, frozen):
+    dom, s = _setup(case, variable)
+    s.frozen_state = frozen
     W0 = smooth_state(dom, 50 + case)
     Uint = o.cons_from_prim(W0, s.gamma)
     dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
@@ -159,6 +161,19 @@ def test_roe_char_primitive_stage_vjp_matches_autograd(emu, case, stage, sweep):
     dom, s = _setup(case, "CHAR-PRIMITIVE")
     s.frozen_state = "ROE"
     _compare(emu, dom, s, stage, sweep, 50 + case, 9500 + 10 * case + stage)
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("frozen", ["ARITHMETIC", "ROE"])
+@pytest.mark.parametrize("stage", [0, 2])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_char_conservative_stage_vjp_matches_autograd(emu, case, stage, frozen, sweep):  # noqa: F811
+    """CHAR-CONSERVATIVE: the conservative window projected with the left eigenvectors, WENO5
+    per characteristic, back with the right eigenvectors; the eigenvectors' dependence on the
+    frozen state (rho~, u~, H~, c) of the face's two cells."""
+    dom, s = _setup(case, "CHAR-CONSERVATIVE")
+    s.frozen_state = frozen
+    _compare(emu, dom, s, stage, sweep, 50 + case, 9600 + 10 * case + stage)
 
 
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
