@@ -97,7 +97,7 @@ def test_restatement_forward_matches_oracle(case, variable):
 
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
-@pytest.mark.parametrize("stage", [0, 1, 2])
+@pytest.mark.parametrize("stage", [0, 2])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
     """lam_in = J^T lam_out over the interior, and lam_n = c1 lam_out, entry by entry."""
@@ -106,7 +106,7 @@ def test_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
 
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
-@pytest.mark.parametrize("stage", [0, 1, 2])
+@pytest.mark.parametrize("stage", [0, 2])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_char_primitive_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
     """CHAR-PRIMITIVE reconstruction (ARITHMETIC frozen state): the reverse of the projection, of
@@ -116,7 +116,7 @@ def test_char_primitive_stage_vjp_matches_autograd(emu, case, stage, sweep):  # 
 
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
-@pytest.mark.parametrize("stage", [0, 1, 2])
+@pytest.mark.parametrize("stage", [0, 2])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_conservative_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
     """CONSERVATIVE reconstruction: WENO on the conservatives of the window cells (their
@@ -153,7 +153,7 @@ def test_frozen_state_forward_matches_oracle(case, variable, frozen):
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
 @pytest.mark.parametrize("stage", [0, 2])
-@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("case", [0, 1, 3, 6])
 def test_roe_char_primitive_stage_vjp_matches_autograd(emu, case, stage, sweep):  # noqa: F811
     """CHAR-PRIMITIVE with the ROE frozen state: the projection's dependence on the Roe density
     sqrt(rho_L rho_R) and on c^2 = psi + g (psi + rho~ (sqrt rho_L + sqrt rho_R)^-2 |du|^2 / 2),
@@ -166,7 +166,7 @@ def test_roe_char_primitive_stage_vjp_matches_autograd(emu, case, stage, sweep):
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
 @pytest.mark.parametrize("frozen", ["ARITHMETIC", "ROE"])
 @pytest.mark.parametrize("stage", [0, 2])
-@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("case", [0, 1, 3, 6])
 def test_char_conservative_stage_vjp_matches_autograd(emu, case, stage, frozen, sweep):  # noqa: F811
     """CHAR-CONSERVATIVE: the conservative window projected with the left eigenvectors, WENO5
     per characteristic, back with the right eigenvectors; the eigenvectors' dependence on the
@@ -191,9 +191,10 @@ def test_hll_restatement_forward_matches_oracle(case, speed, variable):
 
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
-@pytest.mark.parametrize("variable", ["PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE"])
-@pytest.mark.parametrize("speed", ["EINFELDT", "ARITHMETIC", "RUSANOV", "DAVIS", "TORO"])
-@pytest.mark.parametrize("case", [0, 1, 6])
+@pytest.mark.parametrize("case, speed, variable",
+                         [(1, sp, var) for sp in ("EINFELDT", "ARITHMETIC", "RUSANOV", "DAVIS", "TORO")
+                          for var in ("PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE")]
+                         + [(0, "EINFELDT", "PRIMITIVE"), (6, "EINFELDT", "CHAR-PRIMITIVE"), (6, "TORO", "CONSERVATIVE")])
 def test_hll_stage_vjp_matches_autograd(emu, case, speed, variable, sweep):  # noqa: F811
     """HLL with every signal speed: the two-wave flux, its clipped speeds min(S_L, 0),
     max(S_R, 0) and the precision epsilon in the denominator."""
