@@ -11,6 +11,7 @@ in PyTorch-ROCm device tensors.
 from . import _native
 from .config import precision
 from .convective_fluxes import DICT_CONVECTIVE_SOLVER, FluxSplittingScheme, HighOrderGodunov
+from .data_types import FeedForwardSetup
 from .domain import DomainInformation
 from .equation_manager import EquationInformation, EquationManager, IdealGas, MaterialManager
 from .halos import HaloManager

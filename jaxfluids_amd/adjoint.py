@@ -167,19 +167,43 @@ class FixedDtStepper:
                  variable: str = "PRIMITIVE", frozen_state: str = "ARITHMETIC") -> None:
         if variable not in ("PRIMITIVE", "CONSERVATIVE", "CHAR-PRIMITIVE", "CHAR-CONSERVATIVE"):
             raise NotImplementedError(f"reconstruction_variable {variable} has no reverse mode")
+        cells = tuple(int(c) for c in cells)
+        dx = [(r[1] - r[0]) / n for r, n in zip(ranges, cells)]
+        scheme = _native.make_scheme(variable, stencil, riemann, signal_speed if riemann in ("HLLC", "HLL") else None,
+                                     frozen_state, _native.F64, gamma, precision.get_spatial_stencil_eps(), transport)
+        full = {f: bcs.get(f, "PERIODIC") for f in _native.FACE_LOCATIONS}
+        self._setup(cells, nh, gamma, _native.make_grid(cells, int(nh), dx), scheme, _native.bc_array(full), transport,
+                    dt, device)
+
+    @classmethod
+    def from_simulation_manager(cls, sim, dt: float, device=None) -> "FixedDtStepper":
+        """The stepper of a SimulationManager's own numerical setup (its scheme in FP64, grid and
+        boundary conditions).  Raises NotImplementedError for a setup the stage VJP has no
+        reverse mode for, before any step runs."""
+        if precision.get_eps() > 1e-10:
+            raise NotImplementedError("the reverse mode runs in double precision only")
+        dom = sim.domain_information
+        transport = sim.space_solver.transport
+        scheme = sim.convective_flux_solver.scheme(_native.F64, transport)
+        bc = sim.halo_manager.bc_array
+        # the entry point checks the scheme before its buffers: NULL buffers tell the two apart
+        rc = _native.load().jfa_rk3_stage_vjp(None, None, None, None, dom.grid, scheme, bc, 0, 1.0, None, 0, None)
+        if rc == _native.JFA_EUNSUPPORTED:
+            raise NotImplementedError("this numerical setup has no reverse mode (jfa_rk3_stage_vjp)")
+        self = cls.__new__(cls)
+        self._setup(dom.global_number_of_cells, dom.nh_conservatives, sim.material_manager.get_gamma(), dom.grid,
+                    scheme, bc, transport, dt, device)
+        return self
+
+    def _setup(self, cells, nh, gamma, grid, scheme, bc, transport, dt, device) -> None:
         self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
         self.cells = tuple(int(c) for c in cells)
         self.nh = int(nh)
         self.gamma = float(gamma)
-        dx = [(r[1] - r[0]) / n for r, n in zip(ranges, self.cells)]
-        self.grid = _native.make_grid(self.cells, self.nh, dx)
+        self.grid = grid
         self.transport = transport  # viscous / heat fluxes (periodic domains); None: inviscid
-        self.scheme = _native.make_scheme(variable, stencil, riemann,
-                                          signal_speed if riemann in ("HLLC", "HLL") else None,
-                                          frozen_state, _native.F64, gamma, precision.get_spatial_stencil_eps(),
-                                          transport)
-        full = {f: bcs.get(f, "PERIODIC") for f in _native.FACE_LOCATIONS}
-        self.bc = _native.bc_array(full)
+        self.scheme = scheme
+        self.bc = bc
         self.shape = (5,) + tuple(n + 2 * self.nh if n > 1 else 1 for n in self.cells)
         self.interior = (slice(None),) + tuple(slice(self.nh, -self.nh) if n > 1 else slice(None) for n in self.cells)
         lib = _native.load()
@@ -201,6 +225,7 @@ class FixedDtStepper:
         self.dt_value = float(dt)
         self.dt = torch.full((1,), float(dt), dtype=torch.float64, device=self.device)
         self.forward_steps = 0  # forward steps executed (rollouts and recomputation)
+        self.last_state = None  # (W, U) buffers the last rollout ended with
 
     # -- state
     def fill(self, W_int: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
@@ -348,13 +373,19 @@ class RK3Rollout(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: int,
-                segments: bool = False) -> torch.Tensor:
-        W, U = stepper.fill(W0.detach())
+                segments: bool = False, shared: bool = True, start=None) -> torch.Tensor:
+        # start: the (W, U) buffers a previous rollout ended with (stepper.last_state), whose
+        # interior primitives W0 are; the state continues from them instead of U = cons(W0)
+        if start is not None:
+            W, U = start[0].clone(), start[1].clone()
+        else:
+            W, U = stepper.fill(W0.detach())
         ctx.segment = 0
         # with a snapshot at every step (snapshots >= nsteps - 1) the stage primitives are kept too,
         # so the reverse sweep evaluates only the three stage VJPs per step; the steps write straight
-        # into the stepper's rollout buffers
-        keep = snapshots >= nsteps - 1
+        # into the stepper's rollout buffers (one pending rollout per stepper: shared=False keeps the
+        # snapshots in a pool of this rollout's own instead)
+        keep = shared and snapshots >= nsteps - 1
         pool = None
         if keep:
             S = stepper.buffers("states", 2 * (nsteps + 1))   # (W_n, U_n) of every step
@@ -366,7 +397,7 @@ class RK3Rollout(torch.autograd.Function):
                 Wi, Ui, Wo, Uo = S[2 * n], S[2 * n + 1], S[2 * n + 2], S[2 * n + 3]
                 stepper.step_into(Wi, Ui, K[2 * n], K[2 * n + 1], Wo, Uo)
                 snaps[n] = (Wi, Ui, K[2 * n], K[2 * n + 1])
-            W = S[2 * nsteps]
+            W, U = S[2 * nsteps], S[2 * nsteps + 1]
             stepper.tape += 1
         elif segments:
             seg = -(-nsteps // max(1, snapshots))
@@ -388,6 +419,7 @@ class RK3Rollout(torch.autograd.Function):
         ctx.stepper, ctx.nsteps, ctx.snapshots, ctx.snaps, ctx.pool = stepper, nsteps, snapshots, snaps, pool
         ctx.tape = stepper.tape if keep else None
         ctx.W0 = W0.detach()
+        stepper.last_state = (W, U)  # the buffers the rollout ended with (shared ones are reused)
         WT = W[stepper.interior].clone()
         ctx.WT = WT
         return WT
@@ -407,22 +439,29 @@ class RK3Rollout(torch.autograd.Function):
         ctx.snaps = None
         ctx.pool = None
         gW0 = cons_vjp(ctx.W0.to(torch.float64), lam[st.interior], st.gamma)
-        return gW0.to(ctx.W0.dtype), None, None, None, None
+        return gW0.to(ctx.W0.dtype), None, None, None, None, None, None
 
 
 def rollout(W0: torch.Tensor, stepper: FixedDtStepper, nsteps: int, snapshots: Optional[int] = None,
-            schedule: str = "binomial") -> torch.Tensor:
+            schedule: str = "binomial", shared: bool = True, start=None) -> torch.Tensor:
     """Differentiable fixed-dt rollout; snapshots defaults to storing every step's state.
 
     schedule "binomial": at most `snapshots` stored states beyond the initial one, the fewest
     recomputed steps for that memory (the backward recomputes the stage primitives per step);
     "segments": a checkpoint every ceil(nsteps / snapshots) steps, each segment recomputed once
     with its steps' states and stage primitives held (nsteps recomputed steps, one segment of
-    full storage on top of the checkpoints)."""
+    full storage on top of the checkpoints).  A rollout that stores every step writes into the
+    stepper's own buffers, which hold one pending rollout; ``shared=False`` gives it a snapshot
+    pool of its own, so that several rollouts of one stepper can await their backward (chained
+    rollouts).  ``start`` continues from the buffers (W, U) of a previous rollout
+    (``stepper.last_state``, of which W0 is the interior) so that the conservative state is
+    carried exactly, as the reference's scan carries its buffers; the derivative is the same
+    (U = cons(W))."""
     if schedule not in ("binomial", "segments"):
         raise ValueError(f"unknown checkpoint schedule {schedule!r}")
     snapshots = int(nsteps if snapshots is None else snapshots)
-    return RK3Rollout.apply(W0, stepper, int(nsteps), snapshots, schedule == "segments" and snapshots < nsteps - 1)
+    return RK3Rollout.apply(W0, stepper, int(nsteps), snapshots, schedule == "segments" and snapshots < nsteps - 1,
+                            bool(shared), start)
 
 
 def kinetic_energy(W: torch.Tensor, cell_volume: float) -> torch.Tensor:

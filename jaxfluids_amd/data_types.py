@@ -183,3 +183,18 @@ class DiscretizationCounter(NamedTuple):
 class JaxFluidsBuffers(NamedTuple):
     material_fields: MaterialFieldBuffers
     time_control_variables: TimeControlVariables
+
+
+class FeedForwardSetup(NamedTuple):
+    """feed_forward/data_types.py:5-12.  The scan / fori_loop choice and the per-step
+    checkpoint flag select how the reference traces the loop; here they change nothing in the
+    results.  is_checkpoint_inner_step keeps one state per block of inner_steps and recomputes
+    the block once in the backward pass (jax.checkpoint around the block), otherwise every
+    step's state is kept."""
+    outer_steps: int
+    inner_steps: int = 1
+    is_scan: bool = True
+    is_checkpoint_inner_step: bool = True
+    is_checkpoint_integration_step: bool = False
+    is_include_t0: bool = True
+    is_include_halos: bool = True

@@ -207,6 +207,69 @@ class SimulationManager:
         return jxf_buffers
 
 
+    # ------------------------------------------------------------------ differentiable rollouts
+    def _feed_forward(self, primes_init: torch.Tensor, physical_timestep_size: float, t_start: float,
+                      feed_forward_setup) -> tuple:
+        """SimulationManager._feed_forward (simulation_manager.py:1296-1412; feed_forward.py:28,231-523):
+        outer_steps blocks of inner_steps fixed-dt RK3 steps from the interior primitives
+        primes_init, differentiable with torch.autograd (stage VJPs of jfa_rk3_stage_vjp).
+        Returns ({"primitives": [n_out, 5, ...]}, times [n_out]): the state at t_start when
+        is_include_t0, then after every block; with is_include_halos the whole buffers (face
+        halos, and the edge halos of viscous / heat fluxes, as the step fills them), else the
+        interiors.  The time advances by physical_timestep_size per step, no CFL step is taken.
+        Double precision only."""
+        from .adjoint import rollout
+
+        ffs = feed_forward_setup
+        dt, t = float(physical_timestep_size), float(t_start)
+        key = ("feed_forward", dt)
+        if getattr(self, "_ff_key", None) != key:
+            from .adjoint import FixedDtStepper
+
+            self._ff_stepper = FixedDtStepper.from_simulation_manager(self, dt)
+            self._ff_key = key
+        st = self._ff_stepper
+        W = primes_init.to(device=st.device, dtype=torch.float64)
+        eps = precision.get_eps()
+
+        def post(W):
+            return self.halo_manager.halo_buffer(W, eps) if ffs.is_include_halos else W
+
+        outs, times = ([post(W)], [t]) if ffs.is_include_t0 else ([], [])
+        inner = int(ffs.inner_steps)
+        state = None  # the (W, U) buffers carried from block to block
+        for _ in range(int(ffs.outer_steps)):
+            # each block keeps its snapshots in a pool of its own (the blocks' backward passes
+            # run after all of them): one state per block, the block recomputed once, or every step
+            if ffs.is_checkpoint_inner_step:
+                W = rollout(W, st, inner, snapshots=1, schedule="segments", shared=False, start=state)
+            else:
+                W = rollout(W, st, inner, snapshots=inner, shared=False, start=state)
+            state = st.last_state
+            for _ in range(inner):
+                t = t + dt
+            outs.append(post(W))
+            times.append(t)
+        return {"primitives": torch.stack(outs)}, torch.tensor(times, dtype=torch.float64)
+
+    def feed_forward(self, batch_primes_init: torch.Tensor, physical_timestep_size, t_start,
+                     feed_forward_setup) -> tuple:
+        """SimulationManager.feed_forward (simulation_manager.py:1248-1294): _feed_forward over a
+        batch (leading axis) of initial states, time step sizes and start times; outputs stacked
+        along a leading batch axis."""
+        n = batch_primes_init.shape[0]
+        dts = [float(x) for x in torch.as_tensor(physical_timestep_size).reshape(-1).tolist()]
+        t0s = [float(x) for x in torch.as_tensor(t_start).reshape(-1).tolist()]
+        if len(dts) != n or len(t0s) != n:
+            raise ValueError("physical_timestep_size and t_start need one entry per initial state")
+        sols, times = [], []
+        for i in range(n):
+            sol, tt = self._feed_forward(batch_primes_init[i], dts[i], t0s[i], feed_forward_setup)
+            sols.append(sol["primitives"])
+            times.append(tt)
+        return {"primitives": torch.stack(sols)}, torch.stack(times)
+
+
 class _Output:
     """The save_dt schedule of io_utils/output_writer.py:55-72,261-266 and the HDF5/XDMF writers."""
 
