@@ -150,3 +150,39 @@ def test_feed_forward_gradient_against_device_fd(checkpoint):
         fd = (4.0 * D(0.5e-4) - D(1e-4)) / 3.0
     print(f"  fd {fd:.15e} adjoint {an:.15e}", flush=True)
     assert abs(fd - an) <= 1e-6 * abs(an)
+
+
+def test_feed_forward_refuses_setups_without_reverse_mode():
+    """A stencil without a reverse mode is refused before any step (no GPU needed: the entry
+    point checks the scheme before its buffers)."""
+    import os
+
+    from jaxfluids_amd import _native
+    from jaxfluids_amd.adjoint import FixedDtStepper
+
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("library not built")
+    sim_teno = _kh_sim(16)
+    sim_teno.convective_flux_solver.scheme = (lambda dt, tr: _native.make_scheme(
+        "PRIMITIVE", "TENO5", "HLLC", "EINFELDT", "ARITHMETIC", dt, 1.4, 1e-30, tr))
+    with pytest.raises(NotImplementedError):
+        FixedDtStepper.from_simulation_manager(sim_teno, 1e-3)
+
+
+@pytest.mark.gpu
+def test_feed_forward_batch_matches_single():
+    """feed_forward over a batch equals _feed_forward of each member."""
+    from jaxfluids_amd import FeedForwardSetup
+    from jaxfluids_amd.workloads import kelvin_helmholtz_primitives
+
+    print("[feed_forward batch]", flush=True)
+    n = 24
+    sim = _kh_sim(n)
+    ffs = FeedForwardSetup(outer_steps=2, inner_steps=3, is_include_halos=False)
+    W = torch.stack([torch.as_tensor(kelvin_helmholtz_primitives(n, seed=s, thickness=0.06)) for s in (1, 2)])
+    dts, t0s = [1e-3, 2e-3], [0.0, 0.25]
+    sol, times = sim.feed_forward(W.cuda(), dts, t0s, ffs)
+    assert sol["primitives"].shape == (2, 3, 5, n, n, 1) and times.shape == (2, 3)
+    for i in range(2):
+        s1, t1 = sim._feed_forward(W[i].cuda(), dts[i], t0s[i], ffs)
+        assert torch.equal(sol["primitives"][i], s1["primitives"]) and torch.equal(times[i], t1)
