@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjaxfluids_amd.so")
-SOURCES = ["jfa_fused_charprim.hip", "jfa_fused_fsplit.hip", "jfa_fused_charcons.hip", "jfa_fused_prim.hip", "jfa_fused_cons.hip",
+SOURCES = ["jfa_adjoint_ext.hip", "jfa_fused_charprim.hip", "jfa_fused_fsplit.hip", "jfa_fused_charcons.hip", "jfa_fused_prim.hip", "jfa_fused_cons.hip",
            "jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_kernels.hip", "jfa_adjoint.hip", "jfa_fused_f64.hip",
            "jfa_fused_f32.hip"]
 ARCH = os.environ.get("JFA_OFFLOAD_ARCH", "gfx950")

@@ -829,12 +829,104 @@ __device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const 
 // Second part, one variable at a time: cotangents of the six window cells x[0..5] of that
 // variable (the left state reads cells -3..1 = window 0..4, the right state cells 2..-2 =
 // window 5..1).
+// WENO3 one side (weno3 of the forward, weno3_base.py:20-45, weno3_js.py / weno3_z.py):
+// v = u_i + (e0 dm + e1 dp) / (2 (e0 + e1)) with dm = u_i - u_im, dp = u_ip - u_i and the
+// weights multiplied through, e0 = (1/3)(b1^2 + eps), e1 = (2/3)(b0^2 + eps) (JS) or
+// e0 = (1/3)(q0 + tau) q1, e1 = (2/3)(q1 + tau) q0, q_k = b_k + eps, tau = |b0 - b1| (Z);
+// b0 = dm^2, b1 = dp^2.  Adds the VJP for cotangent g.
+template <bool Z>
+__device__ __forceinline__ void weno3_vjp(double um, double ui, double up, double eps, double g, double& gum,
+                                          double& gui, double& gup) {
+    const double dm = ui - um, dp = up - ui;
+    const double b0 = dm * dm, b1 = dp * dp;
+    double e0, e1, q0 = 0.0, q1 = 0.0, tau = 0.0;
+    if constexpr (Z) {
+        tau = fabs(b0 - b1);
+        q0 = b0 + eps;
+        q1 = b1 + eps;
+        e0 = (1.0 / 3.0) * ((q0 + tau) * q1);
+        e1 = (2.0 / 3.0) * ((q1 + tau) * q0);
+    } else {
+        e0 = (1.0 / 3.0) * (b1 * b1 + eps);
+        e1 = (2.0 / 3.0) * (b0 * b0 + eps);
+    }
+    const double S = e0 + e1, N = e0 * dm + e1 * dp;
+    const double gN = 0.5 * g / S, gS = -0.5 * g * N / (S * S);
+    const double ge0 = gN * dm + gS, ge1 = gN * dp + gS;
+    double gdm = gN * e0, gdp = gN * e1, gb0, gb1;
+    if constexpr (Z) {
+        const double gq0 = (1.0 / 3.0) * q1 * ge0 + (2.0 / 3.0) * (q1 + tau) * ge1;
+        const double gq1 = (1.0 / 3.0) * (q0 + tau) * ge0 + (2.0 / 3.0) * q0 * ge1;
+        const double gt = (1.0 / 3.0) * q1 * ge0 + (2.0 / 3.0) * q0 * ge1;
+        const double sg = b0 > b1 ? 1.0 : (b0 < b1 ? -1.0 : 0.0);  // d|x|/dx = 0 at 0
+        gb0 = gq0 + gt * sg;
+        gb1 = gq1 - gt * sg;
+    } else {
+        gb0 = ge1 * (2.0 / 3.0) * 2.0 * b0;
+        gb1 = ge0 * (1.0 / 3.0) * 2.0 * b1;
+    }
+    gdm += 2.0 * dm * gb0;
+    gdp += 2.0 * dp * gb1;
+    gui += g + gdm - gdp;
+    gum -= gdm;
+    gup += gdp;
+}
+
+// TENO5 / TENO5-A one side (teno5 of the forward, teno5.py:32-73, teno5_a.py:42-99): the
+// weights w_k are d_k or 0 by the sharp cutoff, constant where the cutoff does not switch
+// (jax's derivative of the comparison is zero), so v = c + sum_k w_k del_k / (sum w + eps) is
+// linear in the stencil: del = ((5 d1 - 2 d0)/6, (d1 + 2 d2)/6, (4 d2 - d3)/6).
+template <bool A>
+__device__ __forceinline__ void teno5_vjp(double a, double b, double c, double d, double e, const Params& prm,
+                                          double g, double& ga, double& gb, double& gc, double& gd, double& ge) {
+    const double d0 = b - a, d1 = c - b, d2 = d - c, d3 = e - d;
+    double w0, w1, w2;
+    teno5_weights<A>(d0, d1, d2, d3, prm, w0, w1, w2);
+    const double r = g / (w0 + w1 + w2 + prm.eps);
+    const double g0 = w0 * r, g1 = w1 * r, g2 = w2 * r;  // of del_k
+    const double gd0 = -(2.0 / 6.0) * g0, gd1 = (5.0 / 6.0) * g0 + (1.0 / 6.0) * g1;
+    const double gd2 = (2.0 / 6.0) * g1 + (4.0 / 6.0) * g2, gd3 = -(1.0 / 6.0) * g2;
+    ga -= gd0;
+    gb += gd0 - gd1;
+    gc += g + gd1 - gd2;
+    gd += gd2 - gd3;
+    ge += gd3;
+}
+
+// Second part of a face VJP, one variable at a time (see above); STENCIL = kStencilExt selects
+// WENO1, WENO3-JS/-Z or TENO5 / TENO5-A at run time (prm.stencil), as the forward does.
 template <int STENCIL>
-__device__ __forceinline__ void window_vjp_var(const double x[6], double gl, double gr, double eps, double out[6]) {
+__device__ __forceinline__ void window_vjp_var(const double x[6], double gl, double gr, const Params& prm,
+                                               double out[6]) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) out[k] = 0.0;
-    weno5_vjp<STENCIL>(x[0], x[1], x[2], x[3], x[4], eps, gl, out[0], out[1], out[2], out[3], out[4]);
-    weno5_vjp<STENCIL>(x[5], x[4], x[3], x[2], x[1], eps, gr, out[5], out[4], out[3], out[2], out[1]);
+    if constexpr (STENCIL == kStencilExt) {
+        switch (prm.stencil) {
+            case JFA_WENO1:  // the adjacent cells
+                out[2] += gl;
+                out[3] += gr;
+                break;
+            case JFA_WENO3_JS:  // cells -2..0 and 1..-1
+                weno3_vjp<false>(x[1], x[2], x[3], prm.eps, gl, out[1], out[2], out[3]);
+                weno3_vjp<false>(x[4], x[3], x[2], prm.eps, gr, out[4], out[3], out[2]);
+                break;
+            case JFA_WENO3_Z:
+                weno3_vjp<true>(x[1], x[2], x[3], prm.eps, gl, out[1], out[2], out[3]);
+                weno3_vjp<true>(x[4], x[3], x[2], prm.eps, gr, out[4], out[3], out[2]);
+                break;
+            case JFA_TENO5:
+                teno5_vjp<false>(x[0], x[1], x[2], x[3], x[4], prm, gl, out[0], out[1], out[2], out[3], out[4]);
+                teno5_vjp<false>(x[5], x[4], x[3], x[2], x[1], prm, gr, out[5], out[4], out[3], out[2], out[1]);
+                break;
+            default:  // TENO5-A (the entry point admits no other)
+                teno5_vjp<true>(x[0], x[1], x[2], x[3], x[4], prm, gl, out[0], out[1], out[2], out[3], out[4]);
+                teno5_vjp<true>(x[5], x[4], x[3], x[2], x[1], prm, gr, out[5], out[4], out[3], out[2], out[1]);
+                break;
+        }
+    } else {
+        weno5_vjp<STENCIL>(x[0], x[1], x[2], x[3], x[4], prm.eps, gl, out[0], out[1], out[2], out[3], out[4]);
+        weno5_vjp<STENCIL>(x[5], x[4], x[3], x[2], x[1], prm.eps, gr, out[5], out[4], out[3], out[2], out[1]);
+    }
 }
 
 // Face f (right cell at buffer index br, stride sa along the face normal): cotangents of the
@@ -870,8 +962,8 @@ __device__ __forceinline__ void face_state_vjp_at(const double* __restrict__ W, 
 // share of the coefficient cotangents gk.
 template <int AXIS, int STENCIL>
 __device__ __forceinline__ void char_window_vjp(const double* __restrict__ W, int64_t br, int64_t sa, int64_t fs,
-                                                int k, const CharK& K, double gl, double gr, double eps, double g6[6],
-                                                double gk[3]) {
+                                                int k, const CharK& K, double gl, double gr, const Params& prm,
+                                                double g6[6], double gk[3]) {
     constexpr int v = Ax<AXIS>::v;
     double x6[6], p6[6], u6[6];
 #pragma unroll
@@ -883,7 +975,7 @@ __device__ __forceinline__ void char_window_vjp(const double* __restrict__ W, in
         p6[j] = w[4];
         u6[j] = w[v];
     }
-    window_vjp_var<STENCIL>(x6, gl, gr, eps, g6);
+    window_vjp_var<STENCIL>(x6, gl, gr, prm, g6);
     char_coeff_vjp<AXIS>(k, p6, u6, g6, gk);
 }
 
@@ -929,7 +1021,7 @@ __device__ __forceinline__ void cons_window_vjp(const double* __restrict__ W, in
             dU[j][4] = prm.inv_gm1;
         }
     }
-    window_vjp_var<STENCIL>(x6, gl, gr, prm.eps, g6);
+    window_vjp_var<STENCIL>(x6, gl, gr, prm, g6);
 }
 
 // CHAR-CONSERVATIVE, second part: the window cotangents of all five characteristics
@@ -962,7 +1054,7 @@ __device__ __forceinline__ void char_cons_window_vjp(const double* __restrict__ 
             for (int b = 1; b < 5; ++b) acc += Lm[a][b] * u6[j][b];
             x6[j] = acc;
         }
-        window_vjp_var<STENCIL>(x6, gl[a], gr[a], prm.eps, g6);
+        window_vjp_var<STENCIL>(x6, gl[a], gr[a], prm, g6);
 #pragma unroll
         for (int b = 0; b < 5; ++b) {
             double acc = 0.0;
@@ -1126,7 +1218,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
             for (int i = 0; i < 5; ++i) {
                 double x6[6], o6[6];
                 load_var6(W, br, sa, fs, i, x6);
-                window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
+                window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm, o6);
 #pragma unroll
                 for (int k = 0; k < 6; ++k) {
                     const int cell = f - 3 + k;
@@ -1201,7 +1293,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
             double gk[3] = {0.0, 0.0, 0.0};
             for (int c = 0; c < 5; ++c) {
                 double g6[6];
-                char_window_vjp<A, STENCIL>(W, br, sa, fs, c, K, gl[c], gr[c], prm.eps, g6, gk);
+                char_window_vjp<A, STENCIL>(W, br, sa, fs, c, K, gl[c], gr[c], prm, g6, gk);
                 int i0, i1;
                 double c0, c1;
                 const int nt = char_terms<A>(c, K, i0, c0, i1, c1);
@@ -1279,7 +1371,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __res
         for (int i = 0; i < 5; ++i) {
             double x6[6] = {1.0, 1.0, 1.0, 1.0, 1.0, 1.0}, o6[6];  // lanes without a face: zero cotangents
             if (vface) load_var6(W, br, sa, fs, i, x6);
-            window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm.eps, o6);
+            window_vjp_var<STENCIL>(x6, gl[i], gr[i], prm, o6);
 #pragma unroll
             for (int k = 0; k < 6; ++k) s[i] += __shfl_up(o6[k], (unsigned)k, 64);
         }
@@ -1342,7 +1434,7 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __res
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             double g6[6];
-            char_window_vjp<A, STENCIL>(W, bq, sa, fs, c, K, gl[c], gr[c], prm.eps, g6, gk);
+            char_window_vjp<A, STENCIL>(W, bq, sa, fs, c, K, gl[c], gr[c], prm, g6, gk);
             int i0, i1;
             double c0, c1;
             const int nt = char_terms<A>(c, K, i0, c0, i1, c1);
@@ -1497,6 +1589,7 @@ __device__ __forceinline__ void halo_gather(const double* __restrict__ gW, const
     }
 }
 
+template <int UNUSED = 0>  // a template, so the translation units that include this header share it
 __global__ __launch_bounds__(kBlock) void adj_update_kernel(const double* __restrict__ W,
                                                             const double* __restrict__ lam,
                                                             const double* __restrict__ gW, Geo G, Params prm,
@@ -1596,5 +1689,19 @@ static void adj_sweeps(const double* W, const double* lam, const Geo& G, const P
     else if (IN == 1) launch_adj_lanes<1, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
     else launch_adj_lanes<2, STENCIL, RIEMANN, VAR>(W, lam, G, prm, ts, gW, st);
 }
+
+// adj_sweeps for a Riemann solver chosen at run time.
+template <int STENCIL, int VAR>
+inline void adj_sweeps_r(int riemann, const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
+                         double* gW, hipStream_t st) {
+    if (riemann == JFA_HLLC) adj_sweeps<STENCIL, JFA_HLLC, VAR>(W, lam, G, prm, ts, gW, st);
+    else if (riemann == JFA_HLL) adj_sweeps<STENCIL, JFA_HLL, VAR>(W, lam, G, prm, ts, gW, st);
+    else adj_sweeps<STENCIL, JFA_RUSANOV, VAR>(W, lam, G, prm, ts, gW, st);
+}
+
+// The sweeps of the run-time-selected stencils (WENO1, WENO3-JS/-Z, TENO5, TENO5-A;
+// jfa_adjoint_ext.hip, a translation unit of its own).
+void adj_sweeps_ext(int var, int riemann, const double* W, const double* lam, const Geo& G, const Params& prm,
+                    double ts, double* gW, hipStream_t st);
 
 }  // namespace jfa

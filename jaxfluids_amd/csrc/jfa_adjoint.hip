@@ -9,14 +9,6 @@ static std::atomic<int64_t> g_march_min_waves{2048};
 
 int64_t adjoint_march_min_waves() { return g_march_min_waves.load(std::memory_order_relaxed); }
 
-template <int STENCIL, int VAR>
-static void adj_sweeps_r(int riemann, const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
-                         double* gW, hipStream_t st) {
-    if (riemann == JFA_HLLC) adj_sweeps<STENCIL, JFA_HLLC, VAR>(W, lam, G, prm, ts, gW, st);
-    else if (riemann == JFA_HLL) adj_sweeps<STENCIL, JFA_HLL, VAR>(W, lam, G, prm, ts, gW, st);
-    else adj_sweeps<STENCIL, JFA_RUSANOV, VAR>(W, lam, G, prm, ts, gW, st);
-}
-
 }  // namespace jfa
 
 using namespace jfa;
@@ -56,7 +48,9 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
         scheme->recon_var != JFA_CHAR_PRIMITIVE && scheme->recon_var != JFA_CHAR_CONSERVATIVE)
         return JFA_EUNSUPPORTED;
     if (scheme->interpolation_limiter || scheme->flux_limiter) return JFA_EUNSUPPORTED;
-    if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z) return JFA_EUNSUPPORTED;
+    const bool ext = scheme->stencil == JFA_WENO1 || scheme->stencil == JFA_WENO3_JS ||
+                     scheme->stencil == JFA_WENO3_Z || scheme->stencil == JFA_TENO5 || scheme->stencil == JFA_TENO5_A;
+    if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z && !ext) return JFA_EUNSUPPORTED;
     if (scheme->riemann != JFA_HLLC && scheme->riemann != JFA_HLL && scheme->riemann != JFA_RUSANOV)
         return JFA_EUNSUPPORTED;
     if (!W_in || !lam_out || !lam_in || !bc || !workspace || stage < 0 || stage > 2 || (stage > 0 && !lam_n))
@@ -86,7 +80,9 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     const double* W = static_cast<const double*>(W_in);
     const double* lam = static_cast<const double*>(lam_out);
     const int var = scheme->recon_var;
-    if (scheme->stencil == JFA_WENO5_JS) {
+    if (ext) {
+        adj_sweeps_ext(var, scheme->riemann, W, lam, G, prm, m * dt, gW, st);
+    } else if (scheme->stencil == JFA_WENO5_JS) {
         if (var == JFA_CHAR_PRIMITIVE)
             adj_sweeps_r<JFA_WENO5_JS, JFA_CHAR_PRIMITIVE>(scheme->riemann, W, lam, G, prm, m * dt, gW, st);
         else if (var == JFA_CONSERVATIVE)
@@ -110,7 +106,7 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
                            st, W, lam, G, prm, m * dt, gW, X);
     }
     const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
-    hipLaunchKernelGGL(adj_update_kernel, flat_grid(ncell), dim3(kBlock), 0, st, W, lam, gW, G, prm, b6, c0, c1,
+    hipLaunchKernelGGL(adj_update_kernel<0>, flat_grid(ncell), dim3(kBlock), 0, st, W, lam, gW, G, prm, b6, c0, c1,
                        static_cast<double*>(lam_in), stage > 0 ? static_cast<double*>(lam_n) : nullptr, X);
     return check_launch();
 }

@@ -231,9 +231,10 @@ __device__ __forceinline__ double weno3(double um, double ui, double up, double 
 }
 
 // One TENO5 side: stencil (u_imm, u_im, u_i, u_ip, u_ipp); A selects TENO5-A.
+// The TENO5 weights w_k = d_k or 0 (the sharp cutoff) from the first differences of the stencil.
 template <bool A>
-__device__ __forceinline__ double teno5(double a, double b, double c, double d, double e, const Params& prm) {
-    const double d0 = b - a, d1 = c - b, d2 = d - c, d3 = e - d;
+__device__ __forceinline__ void teno5_weights(double d0, double d1, double d2, double d3, const Params& prm,
+                                              double& w0, double& w1, double& w2) {
     const double D0 = d1 - d0, D1 = d2 - d1, D2 = d3 - d2;
     const double S0 = fma(3.0, d1, -d0), S1 = d1 + d2, S2 = fma(-3.0, d2, d3);
     // 4 beta_k and 4 eps: tau5 / (beta_k + eps) is unchanged
@@ -261,7 +262,16 @@ __device__ __forceinline__ double teno5(double a, double b, double c, double d, 
     }
     const double thr = CT * (g0 + g1 + g2);
     constexpr double w_[3] = {A ? 0.1 : 0.05, A ? 0.6 : 0.55, A ? 0.3 : 0.40};
-    const double w0 = g0 < thr ? 0.0 : w_[0], w1 = g1 < thr ? 0.0 : w_[1], w2 = g2 < thr ? 0.0 : w_[2];
+    w0 = g0 < thr ? 0.0 : w_[0];
+    w1 = g1 < thr ? 0.0 : w_[1];
+    w2 = g2 < thr ? 0.0 : w_[2];
+}
+
+template <bool A>
+__device__ __forceinline__ double teno5(double a, double b, double c, double d, double e, const Params& prm) {
+    const double d0 = b - a, d1 = c - b, d2 = d - c, d3 = e - d;
+    double w0, w1, w2;
+    teno5_weights<A>(d0, d1, d2, d3, prm, w0, w1, w2);
     const double del0 = (5.0 * d1 - 2.0 * d0) * (1.0 / 6.0), del1 = fma(2.0, d2, d1) * (1.0 / 6.0),
                  del2 = fma(4.0, d2, -d3) * (1.0 / 6.0);
     const double num = fma(w2, del2, fma(w1, del1, w0 * del0));

@@ -7,7 +7,7 @@ this restatement stands in for jax.grad of the reference, which cannot be import
 The stage map is  f(U_in) = c0 U_in + c1 U_n + dt m rhs(prim(U_in))  on the interior, with the
 face halos filled from the interior as the reference fills them (the same formulas, operands
 and association order as ``jxf_oracle``; each function cites the reference file:line).
-Covered: WENO5-JS / WENO5-Z on the primitives, on the conservatives (CONSERVATIVE) or, with
+Covered: WENO5-JS / WENO5-Z (and WENO1, WENO3-JS / -Z, TENO5, TENO5-A) on the primitives, on the conservatives (CONSERVATIVE) or, with
 the ARITHMETIC or ROE frozen state, on the characteristic projection of the primitives
 (CHAR-PRIMITIVE) or of the conservatives (CHAR-CONSERVATIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
 EINFELDT / TORO signal speeds, HLL with the same speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
@@ -71,7 +71,74 @@ def weno5_z(a, b, c, d, e, eps=SPATIAL_STENCIL_EPS):
     return (al0 * one_alpha) * p0 + (al1 * one_alpha) * p1 + (al2 * one_alpha) * p2
 
 
-WENO5 = {"WENO5-JS": weno5_js, "WENO5-Z": weno5_z}
+_DR3 = (1 / 3, 2 / 3)              # weno3_base.py:24
+_CR3 = ((-0.5, 1.5), (0.5, 0.5))   # weno3_base.py:25
+
+
+def weno1(a, b, c, d, e, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno1_js.py:25-30: the adjacent cell's value."""
+    return c
+
+
+def _weno3_parts(b, c, d):
+    """weno3_base.py:32-45: smoothness and candidates of the stencil (u_im, u_i, u_ip)."""
+    return (torch.square(c - b), torch.square(d - c)), (_CR3[0][0] * b + _CR3[0][1] * c, _CR3[1][0] * c + _CR3[1][1] * d)
+
+
+def weno3_js(a, b, c, d, e, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno3_js.py:27-40."""
+    (b0, b1), (p0, p1) = _weno3_parts(b, c, d)
+    al0 = _DR3[0] * (1.0 / (b0 * b0 + eps))
+    al1 = _DR3[1] * (1.0 / (b1 * b1 + eps))
+    one_alpha = 1.0 / (al0 + al1)
+    return al0 * one_alpha * p0 + al1 * one_alpha * p1
+
+
+def weno3_z(a, b, c, d, e, eps=SPATIAL_STENCIL_EPS):
+    """weno/weno3_z.py:27-39."""
+    (b0, b1), (p0, p1) = _weno3_parts(b, c, d)
+    tau_3 = torch.abs(b0 - b1)
+    al0 = _DR3[0] * (1.0 + tau_3 / (b0 + eps))
+    al1 = _DR3[1] * (1.0 + tau_3 / (b1 + eps))
+    one_alpha = 1.0 / (al0 + al1)
+    return al0 * one_alpha * p0 + al1 * one_alpha * p1
+
+
+def _teno5(a, b, c, d, e, CT, dr, eps):
+    """teno/teno5.py:44-73: gamma_k = (1 + tau5/(beta_k + eps))^6, sharp cutoff gamma_k / sum < CT,
+    omega_k = d_k delta_k / (sum d delta + eps), value sum omega_k p_k."""
+    beta = weno5_smoothness(a, b, c, d, e)
+    tau_5 = torch.abs(beta[0] - beta[2])
+    gamma = [torch.pow(1.0 + tau_5 / (bk + eps), 6.0) for bk in beta]
+    one_gamma_sum = 1.0 / (gamma[0] + gamma[1] + gamma[2])
+    w = [dr[k] * torch.where(gamma[k] * one_gamma_sum < CT, torch.zeros_like(tau_5), torch.ones_like(tau_5))
+         for k in range(3)]
+    one_dk = 1.0 / (w[0] + w[1] + w[2] + eps)
+    p = weno5_polynomials(a, b, c, d, e)
+    return (w[0] * one_dk) * p[0] + (w[1] * one_dk) * p[1] + (w[2] * one_dk) * p[2]
+
+
+def teno5(a, b, c, d, e, eps=SPATIAL_STENCIL_EPS):
+    """teno/teno5.py:32-73: d = (0.05, 0.55, 0.40), CT = 1e-5."""
+    return _teno5(a, b, c, d, e, 1e-5, (0.05, 0.55, 0.40), eps)
+
+
+def teno5_a(a, b, c, d, e, eps=SPATIAL_STENCIL_EPS):
+    """teno/teno5_a.py:42-99: the adaptive cutoff from eta of neighbouring differences (xi = 1e-3,
+    Cr = 0.24), WENO5's d."""
+    ed = 0.9 * 0.24 / (1 - 0.24) * 1e-3 ** 2
+    eta0 = (torch.abs(2 * (c - b) * (b - a)) + ed) / ((c - b) ** 2 + (b - a) ** 2 + ed)
+    eta1 = (torch.abs(2 * (d - c) * (c - b)) + ed) / ((d - c) ** 2 + (c - b) ** 2 + ed)
+    eta2 = (torch.abs(2 * (e - d) * (d - c)) + ed) / ((e - d) ** 2 + (d - c) ** 2 + ed)
+    eta = torch.minimum(eta0, torch.minimum(eta1, eta2))
+    m = 1 - torch.minimum(torch.ones_like(eta), eta / 0.24)
+    g = torch.pow(1 - m, 4) * (1 + 4 * m)
+    CT = torch.pow(10.0, -(torch.ceil(10.0 - 5.0 * (1 - g)) - 1.0))
+    return _teno5(a, b, c, d, e, CT, _DR, eps)
+
+
+WENO5 = {"WENO5-JS": weno5_js, "WENO5-Z": weno5_z, "WENO1": weno1, "WENO3-JS": weno3_js, "WENO3-Z": weno3_z,
+         "TENO5": teno5, "TENO5-A": teno5_a}
 
 
 # ------------------------------------------------------------------ ideal gas

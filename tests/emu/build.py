@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "jaxfluids_amd", "csrc")
 OUT = os.path.join(HERE, "_build", "libjfa_emu.so")
-SOURCES = [os.path.join(CSRC, f) for f in ("jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip", "jfa_kernels.hip",
+SOURCES = [os.path.join(CSRC, f) for f in ("jfa_batch_f64.hip", "jfa_batch_f32.hip", "jfa_adjoint.hip", "jfa_adjoint_ext.hip", "jfa_kernels.hip",
                                             "jfa_fused_f64.hip", "jfa_fused_f32.hip", "jfa_fused_prim.hip",
                                             "jfa_fused_cons.hip", "jfa_fused_charprim.hip", "jfa_fused_charcons.hip",
                                             "jfa_fused_fsplit.hip")]

@@ -200,6 +200,20 @@ def test_roe_char_primitive_stage_vjp_device_matches_autograd(case):
     _vjp_vs_autograd(dom, s, 2, 99 + case, 9900 + 10 * case)
 
 
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("stencil", ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A"])
+@pytest.mark.parametrize("case", [0, 1, 6])
+def test_stencil_stage_vjp_device_matches_autograd(case, stencil, variable):
+    """WENO1, WENO3-JS/-Z, TENO5 and TENO5-A: every entry of the device J^T lam against
+    torch.autograd of the restatement."""
+    say(f"[stencil stage vjp vs autograd] case {case} {stencil} {variable}")
+    cells, bcs, (_, riemann, speed) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable=variable, riemann=riemann, signal_speed=speed,
+                frozen_state="ARITHMETIC", bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, 2, 91 + case, 9150 + 10 * case)
+
+
 @pytest.mark.parametrize("frozen", ["ARITHMETIC", "ROE"])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_char_conservative_stage_vjp_device_matches_autograd(case, frozen):

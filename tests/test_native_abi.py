@@ -124,15 +124,19 @@ def test_setup_reader_accepts_further_stencils():
 
 @pytest.mark.skipif(not os.path.exists(_native.LIB_PATH), reason="library not built")
 def test_stage_vjp_refuses_stencils_without_a_reverse_mode():
-    """jfa_rk3_stage_vjp validates the scheme before any device call: the stencils other than
-    WENO5-JS/-Z have no reverse mode and return JFA_EUNSUPPORTED."""
+    """jfa_rk3_stage_vjp validates the scheme before any device call: WENO5-JS/-Z, WENO1,
+    WENO3-JS/-Z, TENO5 and TENO5-A have reverse modes (JFA_EINVAL for the NULL buffers); the
+    six- and seven-point stencils return JFA_EUNSUPPORTED."""
     lib = _native.load()
     grid = _native.make_grid((8, 8, 8), 3, (0.1, 0.1, 0.1))
     bc = _native.bc_array({f: "PERIODIC" for f in ("east", "west", "north", "south", "top", "bottom")})
-    for name in ("WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A"):
+    for name, want in (("WENO1", _native.JFA_EINVAL), ("WENO3-JS", _native.JFA_EINVAL),
+                       ("WENO3-Z", _native.JFA_EINVAL), ("TENO5", _native.JFA_EINVAL),
+                       ("TENO5-A", _native.JFA_EINVAL), ("WENO6-CU", _native.JFA_EUNSUPPORTED),
+                       ("TENO6", _native.JFA_EUNSUPPORTED), ("WENO7-JS", _native.JFA_EUNSUPPORTED)):
         scheme = _native.make_scheme("PRIMITIVE", name, "HLLC", "EINFELDT", "ARITHMETIC", _native.F64, 1.4, 1e-30)
         rc = lib.jfa_rk3_stage_vjp(None, None, None, None, grid, scheme, bc, 0, 1e-3, None, 0, None)
-        assert rc == _native.JFA_EUNSUPPORTED, (name, rc)
+        assert rc == want, (name, rc)
 
 
 @pytest.mark.skipif(not os.path.exists(_native.LIB_PATH), reason="library not built")

@@ -176,6 +176,35 @@ def test_char_conservative_stage_vjp_matches_autograd(emu, case, stage, frozen, 
     _compare(emu, dom, s, stage, sweep, 50 + case, 9600 + 10 * case + stage)
 
 
+EXT_STENCILS = ["WENO1", "WENO3-JS", "WENO3-Z", "TENO5", "TENO5-A"]
+
+
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE", "CHAR-CONSERVATIVE"])
+@pytest.mark.parametrize("stencil", EXT_STENCILS)
+@pytest.mark.parametrize("case", [0, 1, 6])
+def test_stencil_restatement_forward_matches_oracle(case, stencil, variable):
+    dom, s = _setup(case, variable)
+    s.stencil = stencil
+    W0 = smooth_state(dom, 50 + case)
+    Uint = o.cons_from_prim(W0, s.gamma)
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    ref, _ = stage_map(dom, s, 2, dt, 1.01 * Uint)(Uint)
+    got = _torch_map(dom, s, 2, dt, 1.01 * Uint)(torch.as_tensor(Uint)).numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-15 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("case, stencil, variable",
+                         [(1, st, var) for st in EXT_STENCILS for var in ("PRIMITIVE", "CHAR-PRIMITIVE")]
+                         + [(6, "WENO3-Z", "CONSERVATIVE"), (6, "TENO5", "PRIMITIVE"), (0, "TENO5-A", "CHAR-CONSERVATIVE")])
+def test_stencil_stage_vjp_matches_autograd(emu, case, stencil, variable, sweep):  # noqa: F811
+    """WENO1, WENO3-JS / -Z (their weights' derivatives) and TENO5 / TENO5-A (weights constant
+    between cutoff switches, as jax differentiates the comparison)."""
+    dom, s = _setup(case, variable)
+    s.stencil = stencil
+    _compare(emu, dom, s, 2, sweep, 50 + case, 9100 + 10 * case)
+
+
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "TORO"])
 @pytest.mark.parametrize("case", [0, 1, 6])
