@@ -258,8 +258,9 @@ class SimulationManager:
         batch (leading axis) of initial states, time step sizes and start times; outputs stacked
         along a leading batch axis."""
         n = batch_primes_init.shape[0]
-        dts = [float(x) for x in torch.as_tensor(physical_timestep_size).reshape(-1).tolist()]
-        t0s = [float(x) for x in torch.as_tensor(t_start).reshape(-1).tolist()]
+        # float64: a list of Python floats would otherwise become a float32 tensor
+        dts = [float(x) for x in torch.as_tensor(physical_timestep_size, dtype=torch.float64).reshape(-1).tolist()]
+        t0s = [float(x) for x in torch.as_tensor(t_start, dtype=torch.float64).reshape(-1).tolist()]
         if len(dts) != n or len(t0s) != n:
             raise ValueError("physical_timestep_size and t_start need one entry per initial state")
         sols, times = [], []
