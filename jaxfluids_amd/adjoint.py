@@ -169,7 +169,8 @@ class FixedDtStepper:
             raise NotImplementedError(f"reconstruction_variable {variable} has no reverse mode")
         cells = tuple(int(c) for c in cells)
         dx = [(r[1] - r[0]) / n for r, n in zip(ranges, cells)]
-        scheme = _native.make_scheme(variable, stencil, riemann, signal_speed if riemann in ("HLLC", "HLL") else None,
+        scheme = _native.make_scheme(variable, stencil, riemann,
+                                     signal_speed if riemann in ("HLLC", "HLL", "HLLC-LM") else None,
                                      frozen_state, _native.F64, gamma, precision.get_spatial_stencil_eps(), transport)
         full = {f: bcs.get(f, "PERIODIC") for f in _native.FACE_LOCATIONS}
         self._setup(cells, nh, gamma, _native.make_grid(cells, int(nh), dx), scheme, _native.bc_array(full), transport,

@@ -290,14 +290,13 @@ __device__ __forceinline__ void signal_speeds_vjp(int kind, double uL, double uR
     }
 }
 
-// HLLC star flux of side K (HLLC.py:41-78, the form of hllc_star_flux):
-//   F = phys(W, U) + Sc (u* - U),  d = SK - u, inv = 1/(SK - Ss), pre = rho d inv,
-//   u* = (pre, pre Ss | pre w_minor, inv (d E + (Ss - u)(rho d Ss + p)))
-// Adds its VJP for cotangent mu to gW, gU, gSK, gSs.
-template <int AXIS, bool LEFT>
-__device__ __forceinline__ void hllc_star_vjp(const double W[5], const double U[5], double SK, double Ss,
-                                              const double mu[5], double gW[5], double gU[5], double& gSK,
-                                              double& gSs) {
+// HLLC star state of side K (HLLC.py:52-63, hllc_star_state): d = SK - u, inv = 1/(SK - Ss),
+// pre = rho d inv, u* = (pre, pre Ss | pre w_minor, inv (d E + (Ss - u)(rho d Ss + p))).
+// Adds its VJP for cotangent gus to gW, gU, gSK, gSs.
+template <int AXIS>
+__device__ __forceinline__ void hllc_state_vjp(const double W[5], const double U[5], double SK, double Ss,
+                                               const double gus[5], double gW[5], double gU[5], double& gSK,
+                                               double& gSs) {
     constexpr int v = Ax<AXIS>::v, m0 = Ax<AXIS>::m0, m1 = Ax<AXIS>::m1;
     const double u = W[v];
     const double d = SK - u;
@@ -307,22 +306,6 @@ __device__ __forceinline__ void hllc_star_vjp(const double W[5], const double U[
     const double A = Ss - u;
     const double B = rd * Ss + W[4];
     const double Q = d * U[4] + A * B;
-    double us[5];
-    us[0] = pre;
-    us[v] = pre * Ss;
-    us[m0] = pre * W[m0];
-    us[m1] = pre * W[m1];
-    us[4] = inv * Q;
-    const double Sc = LEFT ? fmin(SK, 0.0) : fmax(SK, 0.0);
-    double gSc = 0.0, gus[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        gSc += mu[i] * (us[i] - U[i]);
-        gus[i] = Sc * mu[i];
-        gU[i] -= Sc * mu[i];
-    }
-    double gzero = 0.0;  // Sc = min(SK, 0) (left) / max(SK, 0) (right)
-    minmax_vjp(LEFT, SK, 0.0, gSc, gSK, gzero);
     const double gpre = gus[0] + gus[v] * Ss + gus[m0] * W[m0] + gus[m1] * W[m1];
     gSs += gus[v] * pre;
     gW[m0] += gus[m0] * pre;
@@ -347,6 +330,28 @@ __device__ __forceinline__ void hllc_star_vjp(const double W[5], const double U[
     gSK += gd;
     gu -= gd;
     gW[v] += gu;
+}
+
+// HLLC star flux of side K (HLLC.py:41-78, the form of hllc_star_flux):
+//   F = phys(W, U) + Sc (u* - U) with the star state u* above.
+// Adds its VJP for cotangent mu to gW, gU, gSK, gSs.
+template <int AXIS, bool LEFT>
+__device__ __forceinline__ void hllc_star_vjp(const double W[5], const double U[5], double SK, double Ss,
+                                              const double mu[5], double gW[5], double gU[5], double& gSK,
+                                              double& gSs) {
+    double us[5];
+    hllc_star_state<AXIS>(W, U, SK, Ss, us);
+    const double Sc = LEFT ? fmin(SK, 0.0) : fmax(SK, 0.0);
+    double gSc = 0.0, gus[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        gSc += mu[i] * (us[i] - U[i]);
+        gus[i] = Sc * mu[i];
+        gU[i] -= Sc * mu[i];
+    }
+    double gzero = 0.0;  // Sc = min(SK, 0) (left) / max(SK, 0) (right)
+    minmax_vjp(LEFT, SK, 0.0, gSc, gSK, gzero);
+    hllc_state_vjp<AXIS>(W, U, SK, Ss, gus, gW, gU, gSK, gSs);
     physical_flux_vjp<AXIS>(W, U, mu, gW, gU);
 }
 
@@ -438,6 +443,85 @@ __device__ __forceinline__ void hll_vjp(const double WL[5], const double WR[5], 
     minmax_vjp(true, SL, 0.0, gsl, gSL, gz);   // s_L = min(S_L, 0)
     minmax_vjp(false, SR, 0.0, gsr, gSR, gz);  // s_R = max(S_R, 0)
     double gcL = 0.0, gcR = 0.0;
+    signal_speeds_vjp(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], WL[4], WR[4], g, gSL, gSR, gWL[v],
+                      gWR[v], gcL, gcR, gWL[0], gWR[0], gWL[4], gWR[4]);
+    sound_speed_vjp(cL, WL[0], g, gcL, gWL[0], gWL[4]);
+    sound_speed_vjp(cR, WR[0], g, gcR, gWR[0], gWR[4]);
+    cons_vjp(WL, gUL, prm, gWL);
+    cons_vjp(WR, gUR, prm, gWR);
+}
+
+// HLLC-LM (HLLCLM.py:59-136; hllc_lm of the forward): F = bL F_L + bR F_R + bS F* with the
+// sign blends bL, bR, bS (piecewise constant: jax's derivative of sign is zero) and
+//   F* = (F_L + F_R)/2 + (phi S_L (u*_L - U_L) + |S*| (u*_L - u*_R) + phi S_R (u*_R - U_R))/2,
+// phi = sin(min(1, Ma / 0.1) pi / 2), Ma = max(|u_L / c_L|, |u_R / c_R|).  Adds the VJP for
+// cotangent mu.
+template <int AXIS>
+__device__ __forceinline__ void hllc_lm_vjp(const double WL[5], const double WR[5], const double UL[5],
+                                            const double UR[5], const double mu[5], const Params& prm,
+                                            double gWL[5], double gWR[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    const double g = prm.gamma;
+    const double cL = sqrt(g * WL[4] / WL[0]), cR = sqrt(g * WR[4] / WR[0]);
+    double SL, SR;
+    signal_speeds(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], WL[4], WR[4], g, SL, SR);
+    const double aL = WL[0] * (SL - WL[v]), aR = WR[0] * (SR - WR[v]);
+    const double num = (WR[4] - WL[4]) + (WL[v] * aL - WR[v] * aR);
+    const double den = aL - aR;
+    const double Ss = num / den;
+    double usL[5], usR[5];
+    hllc_star_state<AXIS>(WL, UL, SL, Ss, usL);
+    hllc_star_state<AXIS>(WR, UR, SR, Ss, usR);
+    const double xL = WL[v] / cL, xR = WR[v] / cR;
+    const double Ma = fmax(fabs(xL), fabs(xR));
+    const double mq = fmin(1.0, Ma / 0.1);
+    const double hp = 0.5 * 3.141592653589793;
+    const double phi = sin(mq * hp);
+    const double wL = phi * SL, wR = phi * SR, aS = fabs(Ss);
+    const double bL = 0.5 * (1.0 + sgn(SL)), bR = 0.5 * (1.0 - sgn(SR));
+    const double bS = 0.25 * (1.0 - sgn(SL)) * (1.0 + sgn(SR));
+    double gUL[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gUR[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double gFL[5], gFR[5], gusL[5], gusR[5];
+    double gwL = 0.0, gwR = 0.0, gaS = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const double gFs = bS * mu[i];
+        gFL[i] = bL * mu[i] + 0.5 * gFs;
+        gFR[i] = bR * mu[i] + 0.5 * gFs;
+        gusL[i] = 0.5 * gFs * (wL + aS);
+        gusR[i] = 0.5 * gFs * (wR - aS);
+        gUL[i] -= 0.5 * gFs * wL;
+        gUR[i] -= 0.5 * gFs * wR;
+        gwL += 0.5 * gFs * (usL[i] - UL[i]);
+        gwR += 0.5 * gFs * (usR[i] - UR[i]);
+        gaS += 0.5 * gFs * (usL[i] - usR[i]);
+    }
+    physical_flux_vjp<AXIS>(WL, UL, gFL, gWL, gUL);
+    physical_flux_vjp<AXIS>(WR, UR, gFR, gWR, gUR);
+    double gSL = gwL * phi, gSR = gwR * phi, gSs = gaS * sgn(Ss);
+    const double gphi = gwL * SL + gwR * SR;
+    double gone = 0.0, gq = 0.0, g1 = 0.0, g2 = 0.0;
+    minmax_vjp(true, 1.0, Ma / 0.1, gphi * cos(mq * hp) * hp, gone, gq);  // min(1, Ma / 0.1)
+    minmax_vjp(false, fabs(xL), fabs(xR), gq / 0.1, g1, g2);             // Ma = max(|xL|, |xR|)
+    const double gxL = g1 * sgn(xL), gxR = g2 * sgn(xR);
+    double gcL = -gxL * xL / cL, gcR = -gxR * xR / cR;
+    gWL[v] += gxL / cL;
+    gWR[v] += gxR / cR;
+    hllc_state_vjp<AXIS>(WL, UL, SL, Ss, gusL, gWL, gUL, gSL, gSs);
+    hllc_state_vjp<AXIS>(WR, UR, SR, Ss, gusR, gWR, gUR, gSR, gSs);
+    // S* = num / den (signal_speeds.py:194-197)
+    const double gnum = gSs / den, gden = -gSs * Ss / den;
+    gWR[4] += gnum;
+    gWL[4] -= gnum;
+    gWL[v] += gnum * aL;
+    gWR[v] -= gnum * aR;
+    const double gaL = gnum * WL[v] + gden, gaR = -gnum * WR[v] - gden;
+    gWL[0] += gaL * (SL - WL[v]);
+    gSL += gaL * WL[0];
+    gWL[v] -= gaL * WL[0];
+    gWR[0] += gaR * (SR - WR[v]);
+    gSR += gaR * WR[0];
+    gWR[v] -= gaR * WR[0];
     signal_speeds_vjp(prm.signal_speed, WL[v], WR[v], cL, cR, WL[0], WR[0], WL[4], WR[4], g, gSL, gSR, gWL[v],
                       gWR[v], gcL, gcR, gWL[0], gWR[0], gWL[4], gWR[4]);
     sound_speed_vjp(cL, WL[0], g, gcL, gWL[0], gWL[4]);
@@ -765,6 +849,7 @@ __device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const 
     double gWL[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gWR[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if constexpr (RIEMANN == JFA_HLLC) hllc_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     else if constexpr (RIEMANN == JFA_HLL) hll_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
+    else if constexpr (RIEMANN == JFA_HLLC_LM) hllc_lm_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     else rusanov_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     if constexpr (VAR == JFA_PRIMITIVE) {
 #pragma unroll
@@ -1696,6 +1781,7 @@ inline void adj_sweeps_r(int riemann, const double* W, const double* lam, const 
                          double* gW, hipStream_t st) {
     if (riemann == JFA_HLLC) adj_sweeps<STENCIL, JFA_HLLC, VAR>(W, lam, G, prm, ts, gW, st);
     else if (riemann == JFA_HLL) adj_sweeps<STENCIL, JFA_HLL, VAR>(W, lam, G, prm, ts, gW, st);
+    else if (riemann == JFA_HLLC_LM) adj_sweeps<STENCIL, JFA_HLLC_LM, VAR>(W, lam, G, prm, ts, gW, st);
     else adj_sweeps<STENCIL, JFA_RUSANOV, VAR>(W, lam, G, prm, ts, gW, st);
 }
 

@@ -10,7 +10,7 @@ and association order as ``jxf_oracle``; each function cites the reference file:
 Covered: WENO5-JS / WENO5-Z (and WENO1, WENO3-JS / -Z, TENO5, TENO5-A) on the primitives, on the conservatives (CONSERVATIVE) or, with
 the ARITHMETIC or ROE frozen state, on the characteristic projection of the primitives
 (CHAR-PRIMITIVE) or of the conservatives (CHAR-CONSERVATIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
-EINFELDT / TORO signal speeds, HLL with the same speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
+EINFELDT / TORO signal speeds, HLL and HLLC-LM with the same speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
 domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
 Derivative conventions at non-smooth points follow torch's, which are jax's for the operations
@@ -253,7 +253,39 @@ def hll(W_L, W_R, U_L, U_R, axis, gamma, signal_speed, eps=EPS64):
     return (s_R * F_L - s_L * F_R + s_L * s_R * (U_R - U_L)) / (s_R - s_L + eps)
 
 
-RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll}
+def hllc_lm(W_L, W_R, U_L, U_R, axis, gamma, signal_speed):
+    """HLLCLM._solve_riemann_problem_xi_single_phase (HLLCLM.py:59-136): HLLC star states, wave
+    speeds scaled by phi = sin(min(1, Ma / 0.1) pi / 2), Ma = max(|u_L / c_L|, |u_R / c_R|)
+    (:37-57), the central star flux and the sign blend of the outer waves."""
+    v = axis + 1
+    m0, m1 = VELOCITY_MINOR_AXES[axis]
+    c_L = torch.sqrt(gamma * W_L[4] / W_L[0])
+    c_R = torch.sqrt(gamma * W_R[4] / W_R[0])
+    S_L, S_R = _speeds(signal_speed, W_L[v], W_R[v], c_L, c_R, W_L[0], W_R[0], W_L[4], W_R[4], gamma)
+    rdu_L = W_L[0] * (S_L - W_L[v])
+    rdu_R = W_R[0] * (S_R - W_R[v])
+    S_star = ((W_R[4] - W_L[4]) + (W_L[v] * rdu_L - W_R[v] * rdu_R)) / (rdu_L - rdu_R)
+
+    def star(W, U, S_K):  # :95-108
+        pre = (S_K - W[v]) / (S_K - S_star) * W[0]
+        u = [pre, pre, pre, pre, pre * (U[4] / U[0] + (S_star - W[v]) * (S_star + W[4] / W[0] / (S_K - W[v])))]
+        u[v] = u[v] * S_star
+        u[m0] = u[m0] * W[m0]
+        u[m1] = u[m1] * W[m1]
+        return torch.stack(u)
+
+    u_star_L, u_star_R = star(W_L, U_L, S_L), star(W_R, U_R, S_R)
+    Ma = torch.maximum(torch.abs(W_L[v] / c_L), torch.abs(W_R[v] / c_R))
+    phi = torch.sin(torch.minimum(torch.ones_like(Ma), Ma / 0.1) * torch.pi * 0.5)
+    F_L = physical_flux_xi(W_L, U_L, axis)
+    F_R = physical_flux_xi(W_R, U_R, axis)
+    flux_star = 0.5 * (F_L + F_R) + 0.5 * ((phi * S_L) * (u_star_L - U_L) + torch.abs(S_star) * (u_star_L - u_star_R)
+                                           + (phi * S_R) * (u_star_R - U_R))
+    return (0.5 * (1 + torch.sign(S_L)) * F_L + 0.5 * (1 - torch.sign(S_R)) * F_R
+            + 0.25 * (1 - torch.sign(S_L)) * (1 + torch.sign(S_R)) * flux_star)
+
+
+RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll, "HLLC-LM": hllc_lm}
 
 
 # ------------------------------------------------------------------ characteristic projection

@@ -162,11 +162,11 @@ def test_feed_forward_refuses_setups_without_reverse_mode():
 
     if not os.path.exists(_native.LIB_PATH):
         pytest.skip("library not built")
-    sim_teno = _kh_sim(16)
-    sim_teno.convective_flux_solver.scheme = (lambda dt, tr: _native.make_scheme(
-        "PRIMITIVE", "TENO5", "HLLC", "EINFELDT", "ARITHMETIC", dt, 1.4, 1e-30, tr))
+    sim_teno6 = _kh_sim(16)
+    sim_teno6.convective_flux_solver.scheme = (lambda dt, tr: _native.make_scheme(
+        "PRIMITIVE", "TENO6", "HLLC", "EINFELDT", "ARITHMETIC", dt, 1.4, 1e-30, tr))
     with pytest.raises(NotImplementedError):
-        FixedDtStepper.from_simulation_manager(sim_teno, 1e-3)
+        FixedDtStepper.from_simulation_manager(sim_teno6, 1e-3)
 
 
 @pytest.mark.gpu

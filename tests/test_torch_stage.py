@@ -205,12 +205,13 @@ def test_stencil_stage_vjp_matches_autograd(emu, case, stencil, variable, sweep)
     _compare(emu, dom, s, 2, sweep, 50 + case, 9100 + 10 * case)
 
 
+@pytest.mark.parametrize("riemann", ["HLL", "HLLC-LM"])
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "TORO"])
 @pytest.mark.parametrize("case", [0, 1, 6])
-def test_hll_restatement_forward_matches_oracle(case, speed, variable):
+def test_hll_restatement_forward_matches_oracle(case, speed, variable, riemann):
     dom, s = _setup(case, variable)
-    s.riemann, s.signal_speed = "HLL", speed
+    s.riemann, s.signal_speed = riemann, speed
     W0 = smooth_state(dom, 50 + case)
     Uint = o.cons_from_prim(W0, s.gamma)
     dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
@@ -230,6 +231,19 @@ def test_hll_stage_vjp_matches_autograd(emu, case, speed, variable, sweep):  # n
     dom, s = _setup(case, variable)
     s.riemann, s.signal_speed = "HLL", speed
     _compare(emu, dom, s, 2, sweep, 50 + case, 9300 + 10 * case)
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("case, speed, variable",
+                         [(1, sp, "PRIMITIVE") for sp in ("EINFELDT", "ARITHMETIC", "RUSANOV", "DAVIS", "TORO")]
+                         + [(1, "EINFELDT", "CHAR-PRIMITIVE"), (6, "EINFELDT", "CONSERVATIVE"),
+                            (0, "DAVIS", "CHAR-CONSERVATIVE")])
+def test_hllc_lm_stage_vjp_matches_autograd(emu, case, speed, variable, sweep):  # noqa: F811
+    """HLLC-LM: the HLLC star states, the low-Mach factor phi = sin(min(1, Ma / 0.1) pi / 2),
+    |S*| and the (piecewise constant) sign blends."""
+    dom, s = _setup(case, variable)
+    s.riemann, s.signal_speed = "HLLC-LM", speed
+    _compare(emu, dom, s, 2, sweep, 50 + case, 9400 + 10 * case)
 
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
