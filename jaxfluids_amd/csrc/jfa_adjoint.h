@@ -530,6 +530,64 @@ __device__ __forceinline__ void hllc_lm_vjp(const double WL[5], const double WR[
     cons_vjp(WR, gUR, prm, gWR);
 }
 
+// AUSM+ (AUSMP.py:30-103; ausmp of the forward): F = a (M+ phi_L + M- phi_R) + (P+ p_L + P- p_R) e_n
+// with a = (c_L + c_R)/2, M = M+(M_L) + M-(M_R), M+- = (M +- |M|)/2, phi = (rho, rho u, E + p) and
+// the split functions of M_L = u_L / a, M_R = u_R / a, each piecewise (|M| >= 1: supersonic
+// branch, jnp.where gives its derivative only).  Adds the VJP for cotangent mu.
+template <int AXIS>
+__device__ __forceinline__ void ausmp_vjp(const double WL[5], const double WR[5], const double UL[5],
+                                          const double UR[5], const double mu[5], const Params& prm,
+                                          double gWL[5], double gWR[5]) {
+    constexpr int v = Ax<AXIS>::v;
+    constexpr double alpha = 3.0 / 16.0, beta = 1.0 / 8.0;
+    const double g = prm.gamma;
+    const double cL = sqrt(g * WL[4] / WL[0]), cR = sqrt(g * WR[4] / WR[0]);
+    const double a = 0.5 * (cL + cR);
+    const double Ml = WL[v] / a, Mr = WR[v] / a;
+    const double ql = Ml * Ml - 1.0, qr = Mr * Mr - 1.0;
+    const double lp = Ml + 1.0, rm = Mr - 1.0;
+    const bool sl = fabs(Ml) >= 1.0, sr = fabs(Mr) >= 1.0;
+    const double Mp = sl ? 0.5 * (Ml + fabs(Ml)) : 0.25 * (lp * lp) + beta * (ql * ql);
+    const double Mm = sr ? 0.5 * (Mr - fabs(Mr)) : -0.25 * (rm * rm) - beta * (qr * qr);
+    const double M = Mp + Mm;
+    const double Map = 0.5 * (M + fabs(M)), Mam = 0.5 * (M - fabs(M));
+    const double Pp = sl ? 0.5 * (1.0 + sgn(Ml)) : 0.25 * (lp * lp) * (2.0 - Ml) + alpha * Ml * (ql * ql);
+    const double Pm = sr ? 0.5 * (1.0 - sgn(Mr)) : 0.25 * (rm * rm) * (2.0 + Mr) - alpha * Mr * (qr * qr);
+    double gUL[5], gUR[5];
+    double ga = 0.0, gMap = 0.0, gMam = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        const double phl = i == 4 ? UL[4] + WL[4] : UL[i];
+        const double phr = i == 4 ? UR[4] + WR[4] : UR[i];
+        ga += mu[i] * (Map * phl + Mam * phr);
+        gMap += a * mu[i] * phl;
+        gMam += a * mu[i] * phr;
+        gUL[i] = a * Map * mu[i];
+        gUR[i] = a * Mam * mu[i];
+    }
+    gWL[4] += gUL[4];  // phi_4 = E + p
+    gWR[4] += gUR[4];
+    const double gpa = mu[v];
+    const double gPp = gpa * WL[4], gPm = gpa * WR[4];
+    gWL[4] += gpa * Pp;
+    gWR[4] += gpa * Pm;
+    const double sM = sgn(M);
+    const double gM = gMap * 0.5 * (1.0 + sM) + gMam * 0.5 * (1.0 - sM);
+    const double dMp = sl ? 0.5 * (1.0 + sgn(Ml)) : 0.5 * lp + 4.0 * beta * Ml * ql;
+    const double dMm = sr ? 0.5 * (1.0 - sgn(Mr)) : -0.5 * rm - 4.0 * beta * Mr * qr;
+    const double dPp = sl ? 0.0 : 0.5 * lp * (2.0 - Ml) - 0.25 * (lp * lp) + alpha * (ql * ql) + 4.0 * alpha * Ml * Ml * ql;
+    const double dPm = sr ? 0.0 : 0.5 * rm * (2.0 + Mr) + 0.25 * (rm * rm) - alpha * (qr * qr) - 4.0 * alpha * Mr * Mr * qr;
+    const double gMl = gM * dMp + gPp * dPp, gMr = gM * dMm + gPm * dPm;
+    gWL[v] += gMl / a;
+    gWR[v] += gMr / a;
+    ga -= gMl * Ml / a + gMr * Mr / a;
+    double gcL = 0.5 * ga, gcR = 0.5 * ga;
+    sound_speed_vjp(cL, WL[0], g, gcL, gWL[0], gWL[4]);
+    sound_speed_vjp(cR, WR[0], g, gcR, gWR[0], gWR[4]);
+    cons_vjp(WL, gUL, prm, gWL);
+    cons_vjp(WR, gUR, prm, gWR);
+}
+
 // Rusanov (Rusanov.py:35-45)
 template <int AXIS>
 __device__ __forceinline__ void rusanov_vjp(const double WL[5], const double WR[5], const double UL[5],
@@ -850,6 +908,7 @@ __device__ __forceinline__ void face_state_vjp(const double (&win)[6][5], const 
     if constexpr (RIEMANN == JFA_HLLC) hllc_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     else if constexpr (RIEMANN == JFA_HLL) hll_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     else if constexpr (RIEMANN == JFA_HLLC_LM) hllc_lm_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
+    else if constexpr (RIEMANN == JFA_AUSMP) ausmp_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     else rusanov_vjp<AXIS>(WL, WR, UL, UR, mu, prm, gWL, gWR);
     if constexpr (VAR == JFA_PRIMITIVE) {
 #pragma unroll
@@ -1782,6 +1841,7 @@ inline void adj_sweeps_r(int riemann, const double* W, const double* lam, const 
     if (riemann == JFA_HLLC) adj_sweeps<STENCIL, JFA_HLLC, VAR>(W, lam, G, prm, ts, gW, st);
     else if (riemann == JFA_HLL) adj_sweeps<STENCIL, JFA_HLL, VAR>(W, lam, G, prm, ts, gW, st);
     else if (riemann == JFA_HLLC_LM) adj_sweeps<STENCIL, JFA_HLLC_LM, VAR>(W, lam, G, prm, ts, gW, st);
+    else if (riemann == JFA_AUSMP) adj_sweeps<STENCIL, JFA_AUSMP, VAR>(W, lam, G, prm, ts, gW, st);
     else adj_sweeps<STENCIL, JFA_RUSANOV, VAR>(W, lam, G, prm, ts, gW, st);
 }
 

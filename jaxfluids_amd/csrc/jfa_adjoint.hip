@@ -52,7 +52,7 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
                      scheme->stencil == JFA_WENO3_Z || scheme->stencil == JFA_TENO5 || scheme->stencil == JFA_TENO5_A;
     if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z && !ext) return JFA_EUNSUPPORTED;
     if (scheme->riemann != JFA_HLLC && scheme->riemann != JFA_HLL && scheme->riemann != JFA_HLLC_LM &&
-        scheme->riemann != JFA_RUSANOV)
+        scheme->riemann != JFA_AUSMP && scheme->riemann != JFA_RUSANOV)
         return JFA_EUNSUPPORTED;
     if (!W_in || !lam_out || !lam_in || !bc || !workspace || stage < 0 || stage > 2 || (stage > 0 && !lam_n))
         return JFA_EINVAL;

@@ -10,7 +10,7 @@ and association order as ``jxf_oracle``; each function cites the reference file:
 Covered: WENO5-JS / WENO5-Z (and WENO1, WENO3-JS / -Z, TENO5, TENO5-A) on the primitives, on the conservatives (CONSERVATIVE) or, with
 the ARITHMETIC or ROE frozen state, on the characteristic projection of the primitives
 (CHAR-PRIMITIVE) or of the conservatives (CHAR-CONSERVATIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
-EINFELDT / TORO signal speeds, HLL and HLLC-LM with the same speeds, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
+EINFELDT / TORO signal speeds, HLL and HLLC-LM with the same speeds, AUSM+, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
 domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
 Derivative conventions at non-smooth points follow torch's, which are jax's for the operations
@@ -285,7 +285,32 @@ def hllc_lm(W_L, W_R, U_L, U_R, axis, gamma, signal_speed):
             + 0.25 * (1 - torch.sign(S_L)) * (1 + torch.sign(S_R)) * flux_star)
 
 
-RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll, "HLLC-LM": hllc_lm}
+def ausmp(W_L, W_R, U_L, U_R, axis, gamma, signal_speed=None):
+    """AUSMP._solve_riemann_problem_xi_single_phase (AUSMP.py:30-103): the arithmetic interface
+    speed of sound, alpha = 3/16, beta = 1/8, phi = (rho, rho u, rho v, rho w, E + p)."""
+    v = axis + 1
+    alpha, beta = 3.0 / 16.0, 1.0 / 8.0
+    phi_L = torch.stack([U_L[0], U_L[1], U_L[2], U_L[3], U_L[4] + W_L[4]])
+    phi_R = torch.stack([U_R[0], U_R[1], U_R[2], U_R[3], U_R[4] + W_R[4]])
+    a = 0.5 * (torch.sqrt(gamma * W_L[4] / W_L[0]) + torch.sqrt(gamma * W_R[4] / W_R[0]))
+    M_l = W_L[v] / a
+    M_r = W_R[v] / a
+    M_plus = torch.where(torch.abs(M_l) >= 1, 0.5 * (M_l + torch.abs(M_l)),
+                         0.25 * torch.square(M_l + 1.0) + beta * torch.square(M_l * M_l - 1.0))
+    M_minus = torch.where(torch.abs(M_r) >= 1, 0.5 * (M_r - torch.abs(M_r)),
+                          -0.25 * torch.square(M_r - 1.0) - beta * torch.square(M_r * M_r - 1.0))
+    M = M_plus + M_minus
+    P_plus = torch.where(torch.abs(M_l) >= 1.0, 0.5 * (1 + torch.sign(M_l)),
+                         0.25 * torch.square(M_l + 1.0) * (2.0 - M_l) + alpha * M_l * torch.square(M_l * M_l - 1.0))
+    P_minus = torch.where(torch.abs(M_r) >= 1.0, 0.5 * (1 - torch.sign(M_r)),
+                          0.25 * torch.square(M_r - 1.0) * (2.0 + M_r) - alpha * M_r * torch.square(M_r * M_r - 1.0))
+    F = a * (0.5 * (M + torch.abs(M)) * phi_L + 0.5 * (M - torch.abs(M)) * phi_R)
+    e = torch.zeros_like(F)
+    e[v] = 1.0
+    return F + e * (P_plus * W_L[4] + P_minus * W_R[4])
+
+
+RIEMANN = {"HLLC": hllc, "RUSANOV": rusanov, "HLL": hll, "HLLC-LM": hllc_lm, "AUSMP": ausmp}
 
 
 # ------------------------------------------------------------------ characteristic projection

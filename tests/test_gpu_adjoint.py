@@ -228,6 +228,18 @@ def test_char_conservative_stage_vjp_device_matches_autograd(case, frozen):
 
 
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
+@pytest.mark.parametrize("case", [0, 1, 6])
+def test_ausmp_stage_vjp_device_matches_autograd(case, variable):
+    """AUSM+: every entry of the device J^T lam against autograd."""
+    say(f"[ausmp stage vjp vs autograd] case {case} {variable}")
+    cells, bcs, (stencil, _, _) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable=variable, riemann="AUSMP", signal_speed=None,
+                frozen_state="ARITHMETIC", bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, 2, 94 + case, 9450 + 10 * case)
+
+
+@pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("speed", ["EINFELDT", "ARITHMETIC", "TORO"])
 @pytest.mark.parametrize("case", [0, 1, 6])
 def test_hllc_lm_stage_vjp_device_matches_autograd(case, speed, variable):

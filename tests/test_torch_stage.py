@@ -205,7 +205,7 @@ def test_stencil_stage_vjp_matches_autograd(emu, case, stencil, variable, sweep)
     _compare(emu, dom, s, 2, sweep, 50 + case, 9100 + 10 * case)
 
 
-@pytest.mark.parametrize("riemann", ["HLL", "HLLC-LM"])
+@pytest.mark.parametrize("riemann", ["HLL", "HLLC-LM", "AUSMP"])
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("speed", ["EINFELDT", "DAVIS", "TORO"])
 @pytest.mark.parametrize("case", [0, 1, 6])
@@ -244,6 +244,17 @@ def test_hllc_lm_stage_vjp_matches_autograd(emu, case, speed, variable, sweep): 
     dom, s = _setup(case, variable)
     s.riemann, s.signal_speed = "HLLC-LM", speed
     _compare(emu, dom, s, 2, sweep, 50 + case, 9400 + 10 * case)
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("case, variable", [(0, "PRIMITIVE"), (1, "PRIMITIVE"), (1, "CHAR-PRIMITIVE"),
+                                            (6, "CONSERVATIVE"), (3, "PRIMITIVE")])
+def test_ausmp_stage_vjp_matches_autograd(emu, case, variable, sweep):  # noqa: F811
+    """AUSM+: the split Mach numbers and pressures (subsonic polynomials at these states), the
+    interface speed of sound and M+- = (M +- |M|)/2."""
+    dom, s = _setup(case, variable)
+    s.riemann, s.signal_speed = "AUSMP", None
+    _compare(emu, dom, s, 2, sweep, 50 + case, 9450 + 10 * case)
 
 
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
