@@ -317,7 +317,8 @@ int jfa_rk3_stage_fused_noalloc(const void* W_in, const void* U_in, const void* 
 /* ---- Reverse mode of the RK3 stage at fixed dt (FP64; every reconstruction variable, the
  * characteristic ones with the ARITHMETIC or ROE frozen state; WENO5-JS/-Z, WENO1, WENO3-JS/-Z,
  * TENO5, TENO5-A;
- * HLLC, HLL or HLLC-LM with ARITHMETIC/RUSANOV/DAVIS/EINFELDT/TORO speeds, AUSM+ or Rusanov; with
+ * HLLC, HLL or HLLC-LM with ARITHMETIC/RUSANOV/DAVIS/EINFELDT/TORO speeds, AUSM+, Rusanov or
+ * the flux-splitting scheme (ROE / CLLF / LLF); with
  * scheme->transport, the
  * viscous and heat fluxes too, on periodic domains).  The reference obtains this
  * derivative by jax.grad through SimulationManager.do_runge_kutta_stages

@@ -1322,6 +1322,140 @@ __device__ __forceinline__ void face_mu(const double* __restrict__ lam, int64_t 
     }
 }
 
+// Flux-splitting scheme (FluxSplittingScheme.compute_flux_xi, flux_splitting_scheme.py:65-111;
+// flux_split_face of the forward): F = R (rec_L(f+) + rec_R(f-)), f+- = (L F_k +- g L U_k)/2 over
+// the window cells k, L, R the conservative eigenvectors at the frozen state of the face's two
+// cells, g the eigenvalue magnitudes (ROE at the frozen state, CLLF / LLF from the two cells).
+// Writes to gW6 the cotangents of the six window cells' primitives for face cotangent mu.
+template <int AXIS, int STENCIL>
+__device__ __forceinline__ void flux_split_vjp(const double* __restrict__ W, int64_t br, int64_t sa, int64_t fs,
+                                               const double mu[5], const Params& prm, double (&gW6)[6][5]) {
+    constexpr int v = Ax<AXIS>::v;
+    double win[6][5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            win[k][i] = ld(W, i * fs + br + (int64_t)(k - 3) * sa);
+            gW6[k][i] = 0.0;
+        }
+    Frozen f;
+    frozen_state<true>(win[2], win[3], prm, f);
+    double Lm[5][5], Rm[5][5];
+    conservative_eigenvectors<AXIS>(f, Lm, Rm);
+    const double uL = win[2][v], uR = win[3][v];
+    const double cL = sqrt(prm.gamma * win[2][4] / win[2][0]), cR = sqrt(prm.gamma * win[3][4] / win[3][0]);
+    double g[5];
+    if (prm.riemann == JFA_FS_ROE) {
+        g[0] = fabs(f.P[v] - f.c);
+        g[1] = g[2] = g[3] = fabs(f.P[v]);
+        g[4] = fabs(f.P[v] + f.c);
+    } else if (prm.riemann == JFA_FS_CLLF) {
+        g[0] = fmax(fabs(uL - cL), fabs(uR - cR));
+        g[1] = g[2] = g[3] = fmax(fabs(uL), fabs(uR));
+        g[4] = fmax(fabs(uL + cL), fabs(uR + cR));
+    } else {
+        g[0] = g[1] = g[2] = g[3] = g[4] = fmax(fabs(uL) + cL, fabs(uR) + cR);
+    }
+    double Uk[6][5], Fk[6][5], gU[6][5], gF[6][5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        cons_from_prim(win[k], prm, Uk[k]);
+        physical_flux<AXIS>(win[k], Uk[k], Fk[k]);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) gU[k][i] = gF[k][i] = 0.0;
+    }
+    double fz[FZ_N], gL[5][5], gg[5], fcv[5];
+#pragma unroll
+    for (int i = 0; i < FZ_N; ++i) fz[i] = 0.0;
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+        double gfc = 0.0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) gfc += Rm[i][a] * mu[i];
+        double cu[6], cf[6], pos[6], neg[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            cu[k] = cf[k] = 0.0;
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                cu[k] += Lm[a][b] * Uk[k][b];
+                cf[k] += Lm[a][b] * Fk[k][b];
+            }
+            pos[k] = 0.5 * (cf[k] + g[a] * cu[k]);
+            neg[k] = 0.5 * (cf[k] - g[a] * cu[k]);
+        }
+        fcv[a] = recon_side<STENCIL>(pos[0], pos[1], pos[2], pos[3], pos[4], pos[5], prm, prm.dx2[AXIS]) +
+                 recon_side<STENCIL>(neg[5], neg[4], neg[3], neg[2], neg[1], neg[0], prm, prm.dx2[AXIS]);
+        double gp[6], gn[6];
+        window_vjp_var<STENCIL>(pos, gfc, 0.0, prm, gp);  // f+ from the left
+        window_vjp_var<STENCIL>(neg, 0.0, gfc, prm, gn);  // f- from the right
+        gg[a] = 0.0;
+#pragma unroll
+        for (int b = 0; b < 5; ++b) gL[a][b] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const double gcf = 0.5 * (gp[k] + gn[k]), gcu = 0.5 * g[a] * (gp[k] - gn[k]);
+            gg[a] += 0.5 * cu[k] * (gp[k] - gn[k]);
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                gU[k][b] += Lm[a][b] * gcu;
+                gF[k][b] += Lm[a][b] * gcf;
+                gL[a][b] += gcu * Uk[k][b] + gcf * Fk[k][b];
+            }
+        }
+    }
+    eigen_l_vjp<AXIS>(f, gL, fz);
+    const double z5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    eigen_r_vjp<AXIS>(f, mu, fcv, z5, z5, fz);  // gR[i][a] = mu_i fc_a
+    if (prm.riemann == JFA_FS_ROE) {
+        const double s0 = sgn(f.P[v] - f.c), s4 = sgn(f.P[v] + f.c), sv = sgn(f.P[v]);
+        fz[v] += gg[0] * s0 + (gg[1] + gg[2] + gg[3]) * sv + gg[4] * s4;
+        fz[FZ_C] += gg[4] * s4 - gg[0] * s0;
+    } else {
+        double guL = 0.0, guR = 0.0, gcL = 0.0, gcR = 0.0, x1 = 0.0, x2 = 0.0;
+        if (prm.riemann == JFA_FS_CLLF) {
+            minmax_vjp(false, fabs(uL - cL), fabs(uR - cR), gg[0], x1, x2);
+            guL += x1 * sgn(uL - cL);
+            gcL -= x1 * sgn(uL - cL);
+            guR += x2 * sgn(uR - cR);
+            gcR -= x2 * sgn(uR - cR);
+            x1 = x2 = 0.0;
+            minmax_vjp(false, fabs(uL), fabs(uR), gg[1] + gg[2] + gg[3], x1, x2);
+            guL += x1 * sgn(uL);
+            guR += x2 * sgn(uR);
+            x1 = x2 = 0.0;
+            minmax_vjp(false, fabs(uL + cL), fabs(uR + cR), gg[4], x1, x2);
+            guL += x1 * sgn(uL + cL);
+            gcL += x1 * sgn(uL + cL);
+            guR += x2 * sgn(uR + cR);
+            gcR += x2 * sgn(uR + cR);
+        } else {
+            minmax_vjp(false, fabs(uL) + cL, fabs(uR) + cR, gg[0] + gg[1] + gg[2] + gg[3] + gg[4], x1, x2);
+            guL += x1 * sgn(uL);
+            gcL += x1;
+            guR += x2 * sgn(uR);
+            gcR += x2;
+        }
+        gW6[2][v] += guL;
+        gW6[3][v] += guR;
+        sound_speed_vjp(cL, win[2][0], prm.gamma, gcL, gW6[2][0], gW6[2][4]);
+        sound_speed_vjp(cR, win[3][0], prm.gamma, gcR, gW6[3][0], gW6[3][4]);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        physical_flux_vjp<AXIS>(win[k], Uk[k], gF[k], gW6[k], gU[k]);
+        cons_vjp(win[k], gU[k], prm, gW6[k]);
+    }
+    double gwl[5], gwr[5];
+    char_cons_frozen_vjp(win[2], win[3], f, fz, prm, gwl, gwr);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        gW6[2][i] += gwl[i];
+        gW6[3][i] += gwr[i];
+    }
+}
+
 __device__ __forceinline__ void add_cell(double* __restrict__ gW, int64_t fs, int64_t b, const double g[5]) {
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
@@ -1356,6 +1490,23 @@ __global__ __launch_bounds__(kAdjBlock) void adj_march_kernel(const double* __re
         JFA_OPAQUE(br);  // keeps per-tap addresses from becoming loop-carried registers
         double mu[5], gl[5], gr[5], fz[FZ_N];
         face_mu(lam, br, sa, fs, f, N, scale, mu);
+        if constexpr (RIEMANN == kFluxSplit) {
+            double gW6[6][5];
+            flux_split_vjp<A, STENCIL>(W, br, sa, fs, mu, prm, gW6);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int cell = f - 3 + k;
+                if (cell >= o0 && cell < o1) {
+#pragma unroll
+                    for (int t = 0; t < 5; ++t) {
+                        double* p = gW + t * fs + br + (int64_t)(k - 3) * sa;
+                        JFA_ACCESS(p, sizeof(double));
+                        *p += gW6[k][t];
+                    }
+                }
+            }
+            continue;
+        }
         face_state_vjp_at<A, STENCIL, RIEMANN, VAR>(W, br, sa, fs, mu, prm, gl, gr, fz);
         if constexpr (VAR == JFA_PRIMITIVE) {
 #pragma unroll
@@ -1500,6 +1651,19 @@ __global__ __launch_bounds__(kAdjBlock) void adj_lane_kernel(const double* __res
     const int f = -5 + 59 * w + lane;
     const bool vface = vwave && f >= 0 && f <= N;
     const int64_t br = brow + (vface ? f : 0) * sa;
+    if constexpr (RIEMANN == kFluxSplit) {
+        // lanes without a face evaluate face 0 of the row with a zero cotangent
+        double mu[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gW6[6][5], s6[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (vface) face_mu(lam, br, sa, fs, f, N, ts * G.one_dx[A], mu);
+        flux_split_vjp<A, STENCIL>(W, br, sa, fs, mu, prm, gW6);
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int t = 0; t < 5; ++t) s6[t] += __shfl_up(gW6[k][t], (unsigned)k, 64);
+        const int cell6 = f - 3;
+        if (vwave && lane >= 5 && cell6 <= N + 2) add_cell(gW, fs, brow + (int64_t)cell6 * sa, s6);
+        return;
+    }
     double gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gr[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, fz[FZ_N];
 #pragma unroll
     for (int i = 0; i < FZ_N; ++i) fz[i] = 0.0;
@@ -1838,6 +2002,11 @@ static void adj_sweeps(const double* W, const double* lam, const Geo& G, const P
 template <int STENCIL, int VAR>
 inline void adj_sweeps_r(int riemann, const double* W, const double* lam, const Geo& G, const Params& prm, double ts,
                          double* gW, hipStream_t st) {
+    if (riemann == JFA_FS_ROE || riemann == JFA_FS_CLLF || riemann == JFA_FS_LLF) {
+        // the flux-splitting face flux reads the primitive window (the scheme's variable is unused)
+        if constexpr (VAR == JFA_PRIMITIVE) adj_sweeps<STENCIL, kFluxSplit, JFA_PRIMITIVE>(W, lam, G, prm, ts, gW, st);
+        return;
+    }
     if (riemann == JFA_HLLC) adj_sweeps<STENCIL, JFA_HLLC, VAR>(W, lam, G, prm, ts, gW, st);
     else if (riemann == JFA_HLL) adj_sweeps<STENCIL, JFA_HLL, VAR>(W, lam, G, prm, ts, gW, st);
     else if (riemann == JFA_HLLC_LM) adj_sweeps<STENCIL, JFA_HLLC_LM, VAR>(W, lam, G, prm, ts, gW, st);

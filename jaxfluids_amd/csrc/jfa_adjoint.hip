@@ -51,8 +51,10 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     const bool ext = scheme->stencil == JFA_WENO1 || scheme->stencil == JFA_WENO3_JS ||
                      scheme->stencil == JFA_WENO3_Z || scheme->stencil == JFA_TENO5 || scheme->stencil == JFA_TENO5_A;
     if (scheme->stencil != JFA_WENO5_JS && scheme->stencil != JFA_WENO5_Z && !ext) return JFA_EUNSUPPORTED;
+    const bool fsplit = scheme->riemann == JFA_FS_ROE || scheme->riemann == JFA_FS_CLLF ||
+                        scheme->riemann == JFA_FS_LLF;
     if (scheme->riemann != JFA_HLLC && scheme->riemann != JFA_HLL && scheme->riemann != JFA_HLLC_LM &&
-        scheme->riemann != JFA_AUSMP && scheme->riemann != JFA_RUSANOV)
+        scheme->riemann != JFA_AUSMP && scheme->riemann != JFA_RUSANOV && !fsplit)
         return JFA_EUNSUPPORTED;
     if (!W_in || !lam_out || !lam_in || !bc || !workspace || stage < 0 || stage > 2 || (stage > 0 && !lam_n))
         return JFA_EINVAL;
@@ -80,7 +82,7 @@ int jfa_rk3_stage_vjp(const void* W_in, const void* lam_out, void* lam_in, void*
     hipLaunchKernelGGL(adj_zero_kernel<0>, dim3((unsigned)(zb < 4096 ? zb : 4096)), dim3(kBlock), 0, st, gW, nz);
     const double* W = static_cast<const double*>(W_in);
     const double* lam = static_cast<const double*>(lam_out);
-    const int var = scheme->recon_var;
+    const int var = fsplit ? JFA_PRIMITIVE : scheme->recon_var;  // flux splitting: the primitive window
     if (ext) {
         adj_sweeps_ext(var, scheme->riemann, W, lam, G, prm, m * dt, gW, st);
     } else if (scheme->stencil == JFA_WENO5_JS) {

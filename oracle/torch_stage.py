@@ -10,7 +10,7 @@ and association order as ``jxf_oracle``; each function cites the reference file:
 Covered: WENO5-JS / WENO5-Z (and WENO1, WENO3-JS / -Z, TENO5, TENO5-A) on the primitives, on the conservatives (CONSERVATIVE) or, with
 the ARITHMETIC or ROE frozen state, on the characteristic projection of the primitives
 (CHAR-PRIMITIVE) or of the conservatives (CHAR-CONSERVATIVE), HLLC with the ARITHMETIC / RUSANOV / DAVIS /
-EINFELDT / TORO signal speeds, HLL and HLLC-LM with the same speeds, AUSM+, Rusanov, PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
+EINFELDT / TORO signal speeds, HLL and HLLC-LM with the same speeds, AUSM+, Rusanov, the flux-splitting scheme (ROE / CLLF / LLF), PERIODIC / ZEROGRADIENT / SYMMETRY face halos, and on periodic
 domains the second-order central viscous and heat fluxes with ``jxf_oracle.Transport``'s laws.
 
 Derivative conventions at non-smooth points follow torch's, which are jax's for the operations
@@ -485,6 +485,39 @@ def char_conservative_states(W, U, dom: Domain, axis: int, stencil: str, gamma: 
     return U_L, U_R
 
 
+def flux_splitting_flux_xi(W, U, dom: Domain, axis: int, stencil: str, gamma: float, frozen: str, kind: str):
+    """FluxSplittingScheme.compute_flux_xi (flux_splitting_scheme.py:65-111), single phase: the
+    window of conservatives and physical fluxes projected on the left eigenvectors of the frozen
+    state of the face's two cells, f+- = (L F +- Lambda L U)/2 with the eigenvalue magnitudes
+    of eigendecomposition.py:672-696 (ROE / CLLF / LLF), f+ reconstructed with j = 0, f- with
+    j = 1, the sum projected back with the right eigenvectors."""
+    v = axis + 1
+    win_W = [W[shift_slice(dom, axis, k)] for k in range(-3, 3)]
+    win_U = [U[shift_slice(dom, axis, k)] for k in range(-3, 3)]
+    cl, cr = win_W[2], win_W[3]
+    P, H, g, c, cc, q2 = _frozen_full(cl, cr, gamma, frozen)
+    R, L = _eigen_conservatives(P, H, g, c, cc, q2, axis)
+    if kind == "ROE":                                                        # :674-677
+        g1, g234, g5 = torch.abs(P[v] - c), torch.abs(P[v]), torch.abs(P[v] + c)
+    else:
+        c_L = torch.sqrt(gamma * cl[4] / cl[0])
+        c_R = torch.sqrt(gamma * cr[4] / cr[0])
+        if kind == "CLLF":                                                   # :680-688
+            g1 = torch.maximum(torch.abs(cl[v] - c_L), torch.abs(cr[v] - c_R))
+            g234 = torch.maximum(torch.abs(cl[v]), torch.abs(cr[v]))
+            g5 = torch.maximum(torch.abs(cl[v] + c_L), torch.abs(cr[v] + c_R))
+        else:                                                                # :690-695
+            g1 = g234 = g5 = torch.maximum(torch.abs(cl[v]) + c_L, torch.abs(cr[v]) + c_R)
+    lam = torch.stack([g1, g234, g234, g234, g5])
+    ch = [_transform(L, u) for u in win_U]
+    chf = [_transform(L, physical_flux_xi(w, u, axis)) for w, u in zip(win_W, win_U)]
+    pos = [0.5 * (cf + lam * cu) for cu, cf in zip(ch, chf)]
+    neg = [0.5 * (cf - lam * cu) for cu, cf in zip(ch, chf)]
+    f_L = WENO5[stencil](*[pos[i] for i in WENO5_WINDOW_IDX[0]])
+    f_R = WENO5[stencil](*[neg[i] for i in WENO5_WINDOW_IDX[1]])
+    return _transform(R, f_L + f_R)
+
+
 # ------------------------------------------------------------------ dissipative fluxes (periodic)
 def _law(T, ref, T_ref, exp, C):
     """material.py:82-124: SUTHERLAND ref ((T_ref + C)/(T + C)) (T/T_ref)^1.5 (power3_2 = x sqrt(x),
@@ -568,6 +601,13 @@ def compute_rhs(W, dom: Domain, stencil: str, riemann: str, signal_speed, gamma:
     the dissipative face flux added to the convective one (:565-587) with a transport."""
     rhs = 0.0
     for axis in dom.active:
+        if riemann.startswith("FLUX-SPLITTING-"):
+            F = flux_splitting_flux_xi(W, U, dom, axis, stencil, gamma, frozen, riemann[len("FLUX-SPLITTING-"):])
+            if transport is not None:
+                F = F + dissipative_flux_xi(W, dom, axis, transport, gamma)
+            n = F.shape[axis + 1]
+            rhs = rhs + (0.0 + dom.one_dx[axis] * (F.narrow(axis + 1, 0, n - 1) - F.narrow(axis + 1, 1, n - 1)))
+            continue
         if variable == "CHAR-PRIMITIVE":
             W_L, W_R = char_primitive_states(W, dom, axis, stencil, gamma, frozen)
         elif variable == "CONSERVATIVE":
@@ -636,7 +676,7 @@ def stage_map(U_int, Un_int, dom: Domain, stencil: str, riemann: str, signal_spe
     if transport is not None:
         assert all(bcs[f] == "PERIODIC" for f in FACE_LOCATIONS if FACE_AXIS[f] in dom.active)
     U = None
-    if variable in ("CONSERVATIVE", "CHAR-CONSERVATIVE"):
+    if variable in ("CONSERVATIVE", "CHAR-CONSERVATIVE") or riemann.startswith("FLUX-SPLITTING-"):
         U = cons_from_prim(W, gamma).clone()
         U[dom.interior] = U_int
     rhs = compute_rhs(W, dom, stencil, riemann, signal_speed, gamma, transport, variable, U, frozen)

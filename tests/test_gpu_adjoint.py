@@ -227,6 +227,19 @@ def test_char_conservative_stage_vjp_device_matches_autograd(case, frozen):
     _vjp_vs_autograd(dom, s, 2, 93 + case, 9800 + 10 * case)
 
 
+@pytest.mark.parametrize("frozen", ["ARITHMETIC", "ROE"])
+@pytest.mark.parametrize("kind", ["ROE", "CLLF", "LLF"])
+@pytest.mark.parametrize("case", [0, 1, 6])
+def test_flux_splitting_stage_vjp_device_matches_autograd(case, kind, frozen):
+    """The flux-splitting scheme: every entry of the device J^T lam against autograd."""
+    say(f"[flux splitting stage vjp vs autograd] case {case} {kind} {frozen}")
+    cells, bcs, (stencil, _, _) = CASES[case]
+    dom = o.Domain(cells, ((0.0, 1.0), (0.0, 1.1), (0.0, 0.9)), nh=3)
+    s = o.Setup(gamma=1.4, stencil=stencil, variable="PRIMITIVE", riemann="FLUX-SPLITTING-" + kind,
+                signal_speed=None, frozen_state=frozen, bcs=dict(zip(FACES, bcs)))
+    _vjp_vs_autograd(dom, s, 2, 92 + case, 9750 + 10 * case)
+
+
 @pytest.mark.parametrize("variable", ["PRIMITIVE", "CHAR-PRIMITIVE"])
 @pytest.mark.parametrize("case", [0, 1, 6])
 def test_ausmp_stage_vjp_device_matches_autograd(case, variable):

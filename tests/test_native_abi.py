@@ -160,15 +160,15 @@ def test_stage_vjp_reconstruction_variables():
 
 @pytest.mark.skipif(not os.path.exists(_native.LIB_PATH), reason="library not built")
 def test_stage_vjp_refuses_further_riemann_solvers():
-    """The stage VJP has reverse modes for HLLC, HLL, HLLC-LM, AUSM+ and Rusanov, not for the
-    flux-splitting scheme."""
+    """The stage VJP has reverse modes for every face flux: HLLC, HLL, HLLC-LM, AUSM+, Rusanov
+    and the flux-splitting scheme (JFA_EINVAL for the NULL buffers)."""
     lib = _native.load()
     grid = _native.make_grid((8, 8, 8), 3, (0.1, 0.1, 0.1))
     bc = _native.bc_array({f: "PERIODIC" for f in ("east", "west", "north", "south", "top", "bottom")})
-    for rs in ("FLUX-SPLITTING-ROE", "FLUX-SPLITTING-LLF"):
+    for rs in ("HLL", "HLLC-LM", "AUSMP", "FLUX-SPLITTING-ROE", "FLUX-SPLITTING-CLLF", "FLUX-SPLITTING-LLF"):
         scheme = _native.make_scheme("PRIMITIVE", "WENO5-JS", rs, "EINFELDT", "ARITHMETIC", _native.F64, 1.4, 1e-30)
         assert lib.jfa_rk3_stage_vjp(None, None, None, None, grid, scheme, bc, 0, 1e-3, None, 0, None) == \
-            _native.JFA_EUNSUPPORTED, rs
+            _native.JFA_EINVAL, rs
 
 
 @pytest.mark.skipif(not os.path.exists(_native.LIB_PATH), reason="library not built")

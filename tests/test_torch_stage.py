@@ -246,6 +246,35 @@ def test_hllc_lm_stage_vjp_matches_autograd(emu, case, speed, variable, sweep): 
     _compare(emu, dom, s, 2, sweep, 50 + case, 9400 + 10 * case)
 
 
+@pytest.mark.parametrize("kind", ["ROE", "CLLF", "LLF"])
+@pytest.mark.parametrize("frozen", ["ARITHMETIC", "ROE"])
+@pytest.mark.parametrize("case", [0, 1, 6])
+def test_flux_splitting_restatement_forward_matches_oracle(case, frozen, kind):
+    dom, s = _setup(case)
+    s.riemann, s.signal_speed, s.frozen_state = "FLUX-SPLITTING-" + kind, None, frozen
+    s.convective_solver, s.flux_splitting = "FLUX-SPLITTING", kind
+    W0 = smooth_state(dom, 50 + case)
+    Uint = o.cons_from_prim(W0, s.gamma)
+    dt = 0.2 * o.compute_dt(o.create_buffers(W0, dom, s)[0], dom, s)
+    ref, _ = stage_map(dom, s, 2, dt, 1.01 * Uint)(Uint)
+    got = _torch_map(dom, s, 2, dt, 1.01 * Uint)(torch.as_tensor(Uint)).numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("sweep", ["march", "lanes"])
+@pytest.mark.parametrize("case, kind, frozen, stencil",
+                         [(1, k, "ARITHMETIC", "WENO5-JS") for k in ("ROE", "CLLF", "LLF")]
+                         + [(6, "ROE", "ROE", "WENO5-Z"), (0, "CLLF", "ROE", "WENO3-Z")])
+def test_flux_splitting_stage_vjp_matches_autograd(emu, case, kind, frozen, stencil, sweep):  # noqa: F811
+    """The flux-splitting scheme: the split characteristic fluxes of the window, their WENO
+    reconstructions, the eigenvalue magnitudes and both eigenvector matrices' dependence on the
+    frozen state."""
+    dom, s = _setup(case)
+    s.riemann, s.signal_speed, s.frozen_state, s.stencil = "FLUX-SPLITTING-" + kind, None, frozen, stencil
+    s.convective_solver, s.flux_splitting = "FLUX-SPLITTING", kind
+    _compare(emu, dom, s, 2, sweep, 50 + case, 9700 + 10 * case)
+
+
 @pytest.mark.parametrize("sweep", ["march", "lanes"])
 @pytest.mark.parametrize("case, variable", [(0, "PRIMITIVE"), (1, "PRIMITIVE"), (1, "CHAR-PRIMITIVE"),
                                             (6, "CONSERVATIVE"), (3, "PRIMITIVE")])
